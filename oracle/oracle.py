@@ -1,0 +1,158 @@
+"""ctypes front end of the CPU oracle (oracle/semo_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the semo_mesh_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_P = np.ctypeslib.ndpointer
+_D = _P(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = _P(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liborc.so is not built (make -f oracle/Makefile)")
+        L = C.CDLL(path)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int] * 4 + [_D, _I, _I, _I, _I, _I, _I, C.c_int, _I, _I]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_bc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _D, C.c_int, C.c_double]
+        L.orc_set_case.argtypes = [C.c_void_p, _D, _D, C.c_double, _D, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double]
+        L.orc_set_solver.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.orc_size.restype = C.c_long
+        L.orc_size.argtypes = [C.c_void_p, C.c_char_p]
+        L.orc_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+        L.orc_set.argtypes = [C.c_void_p, C.c_char_p, _D]
+        for nm in ("orc_eos", "orc_recon", "orc_gravity", "orc_assemble_momentum",
+                   "orc_assemble_pressure"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+        L.orc_lsq.argtypes = [C.c_void_p, _D, _D, _D]
+        L.orc_minmax.argtypes = [C.c_void_p, _D, _D, _D]
+        L.orc_solve.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        L.orc_momentum_update.restype = C.c_double
+        L.orc_momentum_update.argtypes = [C.c_void_p]
+        L.orc_pressure_correct.restype = C.c_double
+        L.orc_pressure_correct.argtypes = [C.c_void_p]
+        L.orc_step.argtypes = [C.c_void_p, C.c_int, _D]
+        L.orc_genrcm.argtypes = [C.c_int, _I, _I, _I, _I]
+        L.orc_cell_graph.argtypes = [C.c_void_p, _I, _I]
+        _LIB = L
+    return _LIB
+
+
+_INT_ARRAYS = {"stencil_ptr", "stencil_idx", "rows", "cols"}
+
+
+class Oracle:
+    """One mesh + case, mirroring SEMO's mesh/controls objects."""
+
+    def __init__(self, mesh, case=None):
+        L = lib()
+        self.mesh = mesh
+        self._h = L.orc_create(mesh.n_points, mesh.n_cells, mesh.n_faces, mesh.n_internal_faces,
+                               np.ascontiguousarray(mesh.points, dtype=np.float64),
+                               np.ascontiguousarray(mesh.owner, dtype=np.int32),
+                               np.ascontiguousarray(mesh.neighbour, dtype=np.int32),
+                               np.ascontiguousarray(mesh.face_ptr, dtype=np.int32),
+                               np.ascontiguousarray(mesh.face_pts, dtype=np.int32),
+                               np.ascontiguousarray(mesh.cell_ptr, dtype=np.int32),
+                               np.ascontiguousarray(mesh.cell_pts, dtype=np.int32),
+                               len(mesh.patch_names),
+                               np.ascontiguousarray(mesh.patch_start, dtype=np.int32),
+                               np.ascontiguousarray(mesh.patch_nfaces, dtype=np.int32))
+        if case is not None:
+            self.set_case(case)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def set_case(self, case):
+        L = lib()
+        L.orc_set_case(self._h, np.array(case.rho, dtype=np.float64), np.array(case.mu, dtype=np.float64),
+                       case.sigma, np.array(case.g, dtype=np.float64), case.dt, case.urf_mom,
+                       case.urf_p, case.urf_pu, case.limit)
+        for i, nm in enumerate(self.mesh.patch_names):
+            b = case.bcs[nm]
+            L.orc_set_bc(self._h, i, b.p_type, b.p_value, b.u_type,
+                         np.array(b.u_value, dtype=np.float64), b.vf_type, b.vf_value)
+
+    def set_solver(self, tol=1e-13, maxit=5000):
+        lib().orc_set_solver(self._h, tol, maxit)
+
+    def get(self, name):
+        L = lib()
+        n = L.orc_size(self._h, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=np.int32 if name in _INT_ARRAYS else np.float64)
+        L.orc_get(self._h, name.encode(), out.ctypes.data)
+        return out
+
+    def set(self, name, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        if lib().orc_set(self._h, name.encode(), arr) != 0:
+            raise KeyError(name)
+
+    # stages
+    def eos(self): lib().orc_eos(self._h)
+    def recon(self): lib().orc_recon(self._h)
+    def gravity(self): lib().orc_gravity(self._h)
+    def assemble_momentum(self): lib().orc_assemble_momentum(self._h)
+    def assemble_pressure(self): lib().orc_assemble_pressure(self._h)
+
+    def lsq(self, phi, face_r):
+        g = np.empty(3 * self.mesh.n_cells)
+        lib().orc_lsq(self._h, np.ascontiguousarray(phi, dtype=np.float64),
+                      np.ascontiguousarray(face_r, dtype=np.float64), g)
+        return g.reshape(-1, 3)
+
+    def minmax(self, phi):
+        mx = np.empty(self.mesh.n_cells); mn = np.empty(self.mesh.n_cells)
+        lib().orc_minmax(self._h, np.ascontiguousarray(phi, dtype=np.float64), mx, mn)
+        return mx, mn
+
+    def solve(self, nrhs):
+        r = C.c_double(0.0)
+        it = lib().orc_solve(self._h, nrhs, C.byref(r))
+        return it, r.value
+
+    def momentum_update(self): return lib().orc_momentum_update(self._h)
+    def pressure_correct(self): return lib().orc_pressure_correct(self._h)
+
+    def step(self, correctors=1):
+        norms = np.zeros(2 * correctors)
+        lib().orc_step(self._h, correctors, norms)
+        return norms
+
+    def cell_graph(self):
+        m = self.mesh
+        ptr = np.zeros(m.n_cells + 1, dtype=np.int32)
+        adj = np.zeros(2 * m.n_internal_faces, dtype=np.int32)
+        lib().orc_cell_graph(self._h, ptr, adj)
+        return ptr, adj
+
+    def coo(self):
+        return self.get("rows"), self.get("cols"), self.get("vals")
+
+
+def genrcm(ptr, adj):
+    n = len(ptr) - 1
+    perm = np.zeros(n, dtype=np.int32); inv = np.zeros(n, dtype=np.int32)
+    lib().orc_genrcm(n, np.ascontiguousarray(ptr, dtype=np.int32),
+                     np.ascontiguousarray(adj, dtype=np.int32), perm, inv)
+    return perm, inv

@@ -1,0 +1,67 @@
+# Builds the reference SEMO hot path in place from the reference checkout
+# (never copied) plus our own driver (ref_driver.cpp) into oracle/_ref/.
+#
+#   make -f oracle/ref.mk REF=/root/reference -j8
+#
+# Toolchain notes (recorded so the build is reproducible):
+#  * MPI: MPICH 3.3.2 headers and shared libraries that ship in this image under
+#    /opt/conda (the reference uses the MPI C++ bindings).  The libraries are
+#    linked by path, without -L/opt/conda/lib, whose old libstdc++ breaks the link;
+#    run-time symlinks to the image's libmpi/libmpicxx/libgfortran/libquadmath go
+#    into oracle/_ref/lib, which the driver's rpath names.
+#  * zlib: the system libz of this image (the reference's prebuilt lib/zlib
+#    binaries are never used).
+#  * math.cpp uses the Intel compiler's built-in __int64 type and a redeclared
+#    default argument; g++ accepts both with -D__int64="long long" -fpermissive.
+#  * Every solveAMGCL call is intercepted with -Wl,--wrap (see ref_driver.cpp).
+REF ?= /root/reference
+HERE := $(dir $(lastword $(MAKEFILE_LIST)))
+OUT := $(HERE)_ref
+OBJ := $(OUT)/obj
+MPIDIR := /opt/conda
+CXX := g++
+CXXFLAGS := -std=c++17 -O2 -fpermissive -D__int64="long long" -w \
+            -I$(MPIDIR)/include -I$(REF)/lib/amgcl -I$(REF)/lib -I$(REF)/lib/zlib -I$(REF)/src
+
+SRCS := mesh/build mesh/geometric math/math math/gradient math/RCM mpi/build \
+        controls/build load/load load/loadFiles load/loadOpenFoam load/loadBinary \
+        load/loadCompress load/loadBase64 \
+        solvers/reconIncom solvers/eqMomentum solvers/eqPressure solvers/eqVolfrac \
+        solvers/solveAMGCL solvers/solvePETSc solvers/timestep solvers/eos solvers/transport \
+        solvers/NVD solvers/cellVarMinMax solvers/surfaceNormalVelocity solvers/pressureBased \
+        solvers/source/gravity solvers/source/surfaceTension solvers/source/curvature \
+        turbulenceModels/LES
+OBJS := $(foreach s,$(SRCS),$(OBJ)/$(subst /,_,$(s)).o)
+
+SOLVE1 := _ZN20SEMO_Solvers_Builder10solveAMGCLENSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEER17SEMO_Mesh_BuilderiRSt6vectorIiSaIiEESB_RS8_IdSaIdEESE_SE_
+SOLVE3 := $(SOLVE1)SE_SE_SE_SE_
+
+all: $(OUT)/semo_ref $(OUT)/lib/.stamp
+
+$(OBJ)/.dir:
+	mkdir -p $(OBJ) && touch $@
+
+define objrule
+$(OBJ)/$(subst /,_,$(1)).o: $(REF)/src/$(1).cpp $(OBJ)/.dir
+	$(CXX) $(CXXFLAGS) -c $$< -o $$@
+endef
+$(foreach s,$(SRCS),$(eval $(call objrule,$(s))))
+
+$(OBJ)/ref_driver.o: $(HERE)ref_driver.cpp $(OBJ)/.dir
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/semo_ref: $(OBJS) $(OBJ)/ref_driver.o
+	$(CXX) -o $@ $^ -Wl,--wrap=$(SOLVE1) -Wl,--wrap=$(SOLVE3) \
+	  $(MPIDIR)/lib/libmpicxx.so.12 $(MPIDIR)/lib/libmpi.so.12 -lz \
+	  -Wl,-rpath,'$$ORIGIN/lib' -Wl,--allow-shlib-undefined
+
+$(OUT)/lib/.stamp:
+	mkdir -p $(OUT)/lib
+	for l in libmpi.so.12 libmpicxx.so.12 libgfortran.so.4 libquadmath.so.0; do \
+	  ln -sf $(MPIDIR)/lib/$$l $(OUT)/lib/$$l; done
+	touch $@
+
+clean:
+	rm -rf $(OUT)
+
+.PHONY: all clean

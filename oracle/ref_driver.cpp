@@ -1,0 +1,361 @@
+// Driver that runs the reference SEMO hot path (compiled in place from the
+// reference tree by oracle/ref.mk) on a mesh written by
+// semo_mesh_amd.mesh.Mesh.write_raw, and dumps every stage as raw arrays.
+//
+// It is test infrastructure: it produces golden vectors for the oracle and
+// the GPU path, and is the "reference" CPU baseline.  Nothing in the product
+// links it.
+//
+// The mesh is filled the way SEMO_Mesh_Load::vtu does (src/load/load.cpp:
+// 352-438, 766-788), the case the way IncomPressure's main does
+// (src/main/incompressible/pressureBased.cpp:34-208).  Every call of
+// SEMO_Solvers_Builder::solveAMGCL is intercepted with the linker's --wrap so
+// that the assembled COO system, its right-hand sides and AMGCL's solution are
+// written out before/after the unmodified reference solve runs.
+#include <mpi.h>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+#include <chrono>
+#include <sys/stat.h>
+
+#include "mesh/build.h"
+#include "mesh/geometric.h"
+#include "controls/build.h"
+#include "solvers/build.h"
+#include "math/math.h"
+
+using std::string;
+using std::vector;
+
+static string g_out;
+static int g_call = 0;
+static SEMO_Controls_Builder* g_controls = nullptr;
+static bool g_dump = true;
+
+template <class T>
+static void dump(const string& name, const vector<T>& v) {
+    if (!g_dump) return;
+    string ext = sizeof(T) == 8 ? ".f64" : ".i32";
+    FILE* f = fopen((g_out + "/" + name + ext).c_str(), "wb");
+    if (!f) { perror(name.c_str()); exit(2); }
+    fwrite(v.data(), sizeof(T), v.size(), f);
+    fclose(f);
+}
+
+template <class T>
+static void rd(FILE* f, T* p, size_t n) {
+    if (fread(p, sizeof(T), n, f) != n) { fprintf(stderr, "short mesh file\n"); exit(2); }
+}
+
+static void dump_cellvar(SEMO_Mesh_Builder& mesh, int idx, const string& name) {
+    vector<double> v(mesh.cells.size());
+    for (size_t i = 0; i < mesh.cells.size(); ++i) v[i] = mesh.cells[i].var[idx];
+    dump(name, v);
+}
+
+static void dump_grad(SEMO_Mesh_Builder& mesh, int idx, const string& name) {
+    auto& g = mesh.cellsGradientVar[idx];
+    vector<double> v(3 * mesh.cells.size(), 0.0);
+    for (size_t i = 0; i < g.size() && i < mesh.cells.size(); ++i)
+        for (int k = 0; k < 3; ++k) v[3 * i + k] = g[i][k];
+    dump(name, v);
+}
+
+// ---- solveAMGCL interception -------------------------------------------
+// _ZN20SEMO_Solvers_Builder10solveAMGCL... mangled names of the two overloads
+// (src/solvers/build.h:225-239).
+#define SOLVE1 _ZN20SEMO_Solvers_Builder10solveAMGCLENSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEER17SEMO_Mesh_BuilderiRSt6vectorIiSaIiEESB_RS8_IdSaIdEESE_SE_
+#define SOLVE3 _ZN20SEMO_Solvers_Builder10solveAMGCLENSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEER17SEMO_Mesh_BuilderiRSt6vectorIiSaIiEESB_RS8_IdSaIdEESE_SE_SE_SE_SE_SE_
+#define CAT(a, b) CAT_(a, b)
+#define CAT_(a, b) a##b
+
+extern "C" void CAT(__real_, SOLVE1)(SEMO_Solvers_Builder*, string, SEMO_Mesh_Builder&, int,
+                                     vector<int>&, vector<int>&, vector<double>&,
+                                     vector<double>&, vector<double>&);
+extern "C" void CAT(__real_, SOLVE3)(SEMO_Solvers_Builder*, string, SEMO_Mesh_Builder&, int,
+                                     vector<int>&, vector<int>&, vector<double>&,
+                                     vector<double>&, vector<double>&, vector<double>&,
+                                     vector<double>&, vector<double>&, vector<double>&);
+
+static void dump_state(SEMO_Mesh_Builder& mesh, const string& p) {
+    SEMO_Controls_Builder& c = *g_controls;
+    dump_grad(mesh, c.P, p + "gradP");
+    dump_grad(mesh, c.U, p + "gradU");
+    dump_grad(mesh, c.V, p + "gradV");
+    dump_grad(mesh, c.W, p + "gradW");
+    dump_grad(mesh, c.VF[0], p + "gradVF");
+    dump_cellvar(mesh, c.maximumU, p + "maxU");
+    dump_cellvar(mesh, c.minimumU, p + "minU");
+    dump_cellvar(mesh, c.maximumP, p + "maxP");
+    dump_cellvar(mesh, c.minimumP, p + "minP");
+    for (int k = 0; k < 3; ++k) {
+        dump_cellvar(mesh, c.sourceGravity[k], p + "srcG" + std::to_string(k));
+        dump_cellvar(mesh, c.sourceSurfaceTension[k], p + "srcS" + std::to_string(k));
+    }
+    dump_cellvar(mesh, c.kappa, p + "kappa");
+    dump_cellvar(mesh, c.oldU, p + "oldU"); dump_cellvar(mesh, c.oldV, p + "oldV");
+    dump_cellvar(mesh, c.oldW, p + "oldW");
+    dump_cellvar(mesh, c.Qm[0], p + "Qm0"); dump_cellvar(mesh, c.Qm[1], p + "Qm1");
+    dump_cellvar(mesh, c.Qm[2], p + "Qm2");
+    for (const char* nm : {"P", "U", "V", "W", "VF", "Rho", "mu"}) {
+        int idx = !strcmp(nm, "P") ? c.P : !strcmp(nm, "U") ? c.U : !strcmp(nm, "V") ? c.V
+                : !strcmp(nm, "W") ? c.W : !strcmp(nm, "VF") ? c.VF[0]
+                : !strcmp(nm, "Rho") ? c.Rho : c.mu;
+        dump_cellvar(mesh, idx, p + nm);
+    }
+}
+
+extern "C" void CAT(__wrap_, SOLVE1)(SEMO_Solvers_Builder* self, string eq, SEMO_Mesh_Builder& mesh,
+                                     int B_n, vector<int>& r, vector<int>& c, vector<double>& v,
+                                     vector<double>& B, vector<double>& x) {
+    string p = "call" + std::to_string(g_call++) + "_" + eq + "_";
+    dump(p + "rows", r); dump(p + "cols", c); dump(p + "vals", v); dump(p + "b0", B);
+    dump_state(mesh, p);
+    CAT(__real_, SOLVE1)(self, eq, mesh, B_n, r, c, v, B, x);
+    dump(p + "x0", x);
+}
+
+extern "C" void CAT(__wrap_, SOLVE3)(SEMO_Solvers_Builder* self, string eq, SEMO_Mesh_Builder& mesh,
+                                     int B_n, vector<int>& r, vector<int>& c, vector<double>& v,
+                                     vector<double>& B0, vector<double>& B1, vector<double>& B2,
+                                     vector<double>& x0, vector<double>& x1, vector<double>& x2) {
+    string p = "call" + std::to_string(g_call++) + "_" + eq + "_";
+    dump(p + "rows", r); dump(p + "cols", c); dump(p + "vals", v);
+    dump(p + "b0", B0); dump(p + "b1", B1); dump(p + "b2", B2);
+    dump_state(mesh, p);
+    CAT(__real_, SOLVE3)(self, eq, mesh, B_n, r, c, v, B0, B1, B2, x0, x1, x2);
+    dump(p + "x0", x0); dump(p + "x1", x1); dump(p + "x2", x2);
+}
+
+// ---- case description ----------------------------------------------------
+struct PatchBC {
+    string name, ptype, utype, vftype;
+    double pval = 0, u[3] = {0, 0, 0}, vfval = 0;
+};
+
+int main(int argc, char** argv) {
+    if (argc < 4) {
+        fprintf(stderr, "usage: %s mesh.raw case.txt outdir [nodump]\n", argv[0]);
+        return 2;
+    }
+    MPI::Init();
+    g_out = argv[3];
+    mkdir(g_out.c_str(), 0755);
+    if (argc > 4 && !strcmp(argv[4], "nodump")) g_dump = false;
+
+    // case file: "key value..." lines
+    double dt = 0.01, rho[2] = {1000, 1000}, mu[2] = {1e-3, 1e-3}, sigma = 0, g[3] = {0, 0, 0};
+    double urf_u = 0.3, urf_p = 0.3, urf_pu = 0.3, lim = 1e10;
+    int nsteps = 1, ncorr = 1, nvof = 0;
+    vector<PatchBC> bcs;
+    string initfile;
+    {
+        std::ifstream cf(argv[2]);
+        string key;
+        while (cf >> key) {
+            if (key == "dt") cf >> dt;
+            else if (key == "rho") cf >> rho[0] >> rho[1];
+            else if (key == "mu") cf >> mu[0] >> mu[1];
+            else if (key == "sigma") cf >> sigma;
+            else if (key == "g") cf >> g[0] >> g[1] >> g[2];
+            else if (key == "urf") cf >> urf_u >> urf_p >> urf_pu;
+            else if (key == "limit") cf >> lim;
+            else if (key == "steps") cf >> nsteps;
+            else if (key == "correctors") cf >> ncorr;
+            else if (key == "vof") cf >> nvof;
+            else if (key == "init") cf >> initfile;
+            else if (key == "bc") {
+                PatchBC b;
+                cf >> b.name >> b.ptype >> b.pval >> b.utype >> b.u[0] >> b.u[1] >> b.u[2] >> b.vftype >> b.vfval;
+                bcs.push_back(b);
+            } else { fprintf(stderr, "bad key %s\n", key.c_str()); return 2; }
+        }
+    }
+
+    vector<SEMO_Species> species(2);
+    species[0].name = "water"; species[1].name = "air";
+    for (int s = 0; s < 2; ++s) {
+        species[s].rhoType = "const"; species[s].rho = rho[s]; species[s].mu = mu[s];
+        species[s].sigma = sigma; species[s].phase = "liquid";
+    }
+    SEMO_Controls_Builder controls;
+    g_controls = &controls;
+    controls.setValues(species);
+    controls.timeStep = dt; controls.orgTimeStep = dt;
+    controls.oldTimeStep = dt; controls.old2TimeStep = dt;
+    controls.adjustTimeStep = "no"; controls.maxCo = 1.0; controls.maxTimeStep = dt;
+    controls.gravityAcceleration = {g[0], g[1], g[2]};
+    controls.momVelURF = urf_u; controls.prePreURF = urf_p; controls.preVelURF = urf_pu;
+    controls.vofVofURF = 1.0;
+    controls.momVelAdjustRF = "no"; controls.prePreAdjustRF = "no";
+    controls.preVelAdjustRF = "no"; controls.vofVofAdjustRF = "no";
+    controls.maxP = lim; controls.minP = -lim; controls.maxU = lim; controls.minU = -lim;
+    controls.maxV = lim; controls.minV = -lim; controls.maxW = lim; controls.minW = -lim;
+    controls.maxT = 1e10; controls.minT = -1e10;
+    controls.iterReal = 0; controls.iterTotal = 0; controls.time = 0.0;
+    controls.iterPBsMax = ncorr; controls.iterPreMax = 1; controls.iterMomMax = 1;
+    controls.iterVofMax = nvof;
+    controls.turbType = "laminar";
+    controls.startClock = 0.0;
+
+    // ---- mesh (load.cpp:352-438 semantics) ----
+    SEMO_Mesh_Builder mesh;
+    {
+        FILE* f = fopen(argv[1], "rb");
+        if (!f) { perror(argv[1]); return 2; }
+        int hdr[7];
+        rd(f, hdr, 7);
+        int np = hdr[0], nc = hdr[1], nf = hdr[2], nif = hdr[3], nfp = hdr[4], ncp = hdr[5], npatch = hdr[6];
+        vector<double> pts(3 * (size_t)np);
+        vector<int> owner(nf), nbr(nif), fptr(nf + 1), fpts(nfp), cptr(nc + 1), cpts(ncp);
+        rd(f, pts.data(), pts.size()); rd(f, owner.data(), nf); rd(f, nbr.data(), nif);
+        rd(f, fptr.data(), nf + 1); rd(f, fpts.data(), nfp); rd(f, cptr.data(), nc + 1);
+        rd(f, cpts.data(), ncp);
+        for (int i = 0; i < np; ++i) {
+            mesh.addPoint();
+            mesh.points.back().x = pts[3 * i]; mesh.points.back().y = pts[3 * i + 1];
+            mesh.points.back().z = pts[3 * i + 2];
+        }
+        for (int i = 0; i < nf; ++i) {
+            mesh.addFace();
+            auto& fc = mesh.faces.back();
+            fc.owner = owner[i];
+            fc.neighbour = i < nif ? nbr[i] : -1;
+            for (int k = fptr[i]; k < fptr[i + 1]; ++k) fc.points.push_back(fpts[k]);
+            fc.var.resize(controls.nTotalFaceVar, 0.0);
+            fc.varL.resize(controls.nTotalFaceLRVar, 0.0);
+            fc.varR.resize(controls.nTotalFaceLRVar, 0.0);
+        }
+        for (int i = 0; i < nc; ++i) {
+            mesh.addCell();
+            auto& cl = mesh.cells.back();
+            for (int k = cptr[i]; k < cptr[i + 1]; ++k) cl.points.push_back(cpts[k]);
+            cl.var.resize(controls.nTotalCellVar, 0.0);
+        }
+        for (int i = 0; i < nf; ++i) {   // load.cpp:373-385: ascending face order
+            mesh.cells[owner[i]].faces.push_back(i);
+            if (i < nif) mesh.cells[nbr[i]].faces.push_back(i);
+        }
+        for (int b = 0; b < npatch; ++b) {
+            int sn[3];
+            rd(f, sn, 3);
+            string nm(sn[2], ' ');
+            rd(f, &nm[0], sn[2]);
+            mesh.addBoundary();
+            auto& bd = mesh.boundary.back();
+            bd.name = nm; bd.startFace = sn[0]; bd.nFaces = sn[1];
+            bd.neighbProcNo = -1; bd.myProcNo = -1;
+            bd.type.assign(controls.nTotalCellVar, "zeroGradient");
+            bd.var.assign(controls.nTotalCellVar, 0.0);
+            const PatchBC* pb = nullptr;
+            for (auto& b2 : bcs) if (b2.name == nm) pb = &b2;
+            if (!pb) { fprintf(stderr, "no bc for patch %s\n", nm.c_str()); return 2; }
+            bd.type[controls.P] = pb->ptype; bd.var[controls.P] = pb->pval;
+            bd.type[controls.U] = pb->utype; bd.type[controls.V] = pb->utype; bd.type[controls.W] = pb->utype;
+            bd.var[controls.U] = pb->u[0]; bd.var[controls.V] = pb->u[1]; bd.var[controls.W] = pb->u[2];
+            bd.type[controls.T] = "zeroGradient"; bd.var[controls.T] = 300.0;
+            bd.type[controls.VF[0]] = pb->vftype; bd.var[controls.VF[0]] = pb->vfval;
+        }
+        fclose(f);
+    }
+    mesh.cellsProcVar.resize(controls.nTotalCellVar, vector<double>());
+    mesh.cellsProcGradientVar.resize(controls.nTotalCellVar, vector<vector<double>>());
+    mesh.cellsGradientVar.resize(controls.nTotalCellVar, vector<vector<double>>());
+    mesh.setFaceTypes();
+    mesh.buildLists();
+    mesh.setCountsProcFaces();
+    mesh.setDisplsProcFaces();
+    mesh.cellsGlobal();
+
+    // ---- initial fields ----
+    size_t nc = mesh.cells.size();
+    vector<double> init(5 * nc, 0.0);
+    for (size_t i = 0; i < nc; ++i) init[4 * nc + i] = 1.0;
+    if (!initfile.empty()) {
+        FILE* f = fopen(initfile.c_str(), "rb");
+        if (!f) { perror(initfile.c_str()); return 2; }
+        rd(f, init.data(), init.size());
+        fclose(f);
+    }
+    for (size_t i = 0; i < nc; ++i) {
+        auto& cl = mesh.cells[i];
+        cl.var[controls.P] = init[i]; cl.var[controls.U] = init[nc + i];
+        cl.var[controls.V] = init[2 * nc + i]; cl.var[controls.W] = init[3 * nc + i];
+        cl.var[controls.T] = 300.0;
+        cl.var[controls.VF[0]] = init[4 * nc + i]; cl.var[controls.VF[1]] = 1.0 - init[4 * nc + i];
+        cl.var[controls.MF[0]] = init[4 * nc + i]; cl.var[controls.MF[1]] = 1.0 - init[4 * nc + i];
+    }
+
+    SEMO_Solvers_Builder solvers;
+    solvers.calcCellEOSVF(mesh, controls, species);
+    for (auto& cl : mesh.cells) {       // main/incompressible/pressureBased.cpp:83-98
+        cl.var[controls.Qm[0]] = cl.var[controls.U]; cl.var[controls.Qm[1]] = cl.var[controls.V];
+        cl.var[controls.Qm[2]] = cl.var[controls.W]; cl.var[controls.Qm[3]] = cl.var[controls.VF[0]];
+        cl.var[controls.Qn[0]] = cl.var[controls.U]; cl.var[controls.Qn[1]] = cl.var[controls.V];
+        cl.var[controls.Qn[2]] = cl.var[controls.W]; cl.var[controls.Qn[3]] = cl.var[controls.VF[0]];
+    }
+
+    auto t0 = std::chrono::steady_clock::now();
+    SEMO_Mesh_Geometric geometric;
+    geometric.init(mesh);
+    SEMO_Utility_Math math;
+    math.initLeastSquare(mesh);
+    math.lib_RCM(mesh);
+    auto t1 = std::chrono::steady_clock::now();
+
+    {   // geometry, stencil, LSQ, RCM
+        size_t nf = mesh.faces.size();
+        vector<double> area(nf), nrm(3 * nf), fxyz(3 * nf), wC(nf), magPN(nf), alphaF(nf),
+            ePN(3 * nf), skew(3 * nf), dist(3 * nf);
+        for (size_t i = 0; i < nf; ++i) {
+            auto& fc = mesh.faces[i];
+            area[i] = fc.area; wC[i] = fc.wC; magPN[i] = fc.magPN; alphaF[i] = fc.alphaF;
+            fxyz[3 * i] = fc.x; fxyz[3 * i + 1] = fc.y; fxyz[3 * i + 2] = fc.z;
+            for (int k = 0; k < 3; ++k) {
+                nrm[3 * i + k] = fc.unitNormals[k]; ePN[3 * i + k] = fc.unitNomalsPN[k];
+                skew[3 * i + k] = fc.vecSkewness[k]; dist[3 * i + k] = fc.distCells[k];
+            }
+        }
+        dump("face_area", area); dump("face_normal", nrm); dump("face_xyz", fxyz);
+        dump("face_wC", wC); dump("face_magPN", magPN); dump("face_alphaF", alphaF);
+        dump("face_ePN", ePN); dump("face_skew", skew); dump("face_dist", dist);
+        vector<double> vol(nc), cxyz(3 * nc), lsq(6 * nc);
+        vector<int> sptr(nc + 1, 0), sidx, perm(nc), inv(nc);
+        for (size_t i = 0; i < nc; ++i) {
+            auto& cl = mesh.cells[i];
+            vol[i] = cl.volume; cxyz[3 * i] = cl.x; cxyz[3 * i + 1] = cl.y; cxyz[3 * i + 2] = cl.z;
+            for (int k = 0; k < 6; ++k) lsq[6 * i + k] = cl.coeffLeastSquare1stCellVetexStencil[k];
+            for (int j : cl.stencil) sidx.push_back(j);
+            sptr[i + 1] = (int)sidx.size();
+            perm[i] = cl.RCM; inv[i] = cl.invRCM;
+        }
+        dump("cell_volume", vol); dump("cell_xyz", cxyz); dump("cell_lsq", lsq);
+        dump("stencil_ptr", sptr); dump("stencil_idx", sidx);
+        dump("rcm_perm", perm); dump("rcm_inv", inv);
+    }
+
+    solvers.calcCellTransport(mesh, controls, species);
+    auto t2 = std::chrono::steady_clock::now();
+    for (int s = 0; s < nsteps; ++s) {
+        solvers.incompressiblePressureBased(mesh, controls, species);
+        controls.time += controls.timeStep;
+        ++controls.iterReal;
+    }
+    auto t3 = std::chrono::steady_clock::now();
+    dump_state(mesh, "final_");
+    double ts = std::chrono::duration<double>(t1 - t0).count();
+    double tr = std::chrono::duration<double>(t3 - t2).count();
+    FILE* f = fopen((g_out + "/timing.txt").c_str(), "w");
+    if (f) {
+        fprintf(f, "setup_s %.6f\nrun_s %.6f\ncells %zu\nsteps %d\ncorrectors %d\n", ts, tr, nc, nsteps, ncorr);
+        fclose(f);
+    }
+    printf("SEMO_REF_DONE setup_s=%.6f run_s=%.6f cells=%zu iters=%d\n", ts, tr, nc, nsteps * ncorr);
+    fflush(stdout);
+    MPI_Finalize();
+    return 0;
+}

@@ -1,0 +1,1148 @@
+// SEMO hot-path oracle: a CPU restatement, in FP64, of the reference's
+// pressure-based incompressible assemble-and-solve path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP path is compared
+// against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// may load it.  The product (semo_mesh_amd, libsemo_gpu.so) never links it.
+//
+// Every function states the reference file:line it restates.  The loops keep
+// the reference's per-cell accumulation order and the left-to-right grouping
+// of its arithmetic, so that on the same mesh the results are bit-identical to
+// the compiled reference (pinned in tests/test_oracle_ref.py against fixtures
+// the compiled reference wrote).  The linear solves are this file's own
+// (ILU0-preconditioned BiCGStab to a tight tolerance); SEMO uses AMGCL, so the
+// solved fields agree to the solver tolerance, not bit for bit.
+//
+// Build: make -C oracle   (g++ -O2 -ffp-contract=off; no fast-math)
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+using std::vector;
+
+namespace {
+
+enum BcType { ZERO_GRADIENT = 0, FIXED_VALUE = 1, NO_SLIP = 2, SLIP = 3,
+              SURFACE_NORMAL_FIXED_VALUE = 4, INLET_OUTLET = 5 };
+
+struct Patch {
+    int start, nfaces;
+    int p_type; double p_val;
+    int u_type; double u_val[3];
+    int vf_type; double vf_val;
+};
+
+struct Oracle {
+    // ---- mesh ------------------------------------------------------------
+    int np = 0, nc = 0, nf = 0, nif = 0;
+    vector<double> pts;
+    vector<int> owner, nbr, fptr, fpts, cptr, cpts;
+    vector<Patch> patches;
+    vector<int> cfptr, cfidx;          // cell -> faces, ascending face id
+    vector<int> face_patch;            // boundary face -> patch index
+    // geometry (src/mesh/geometric.cpp:792-1497)
+    vector<double> farea, fnrm, fxyz, fwC, fmagPN, falpha, fePN, fskew, fdist, fvecPF, fvecNF;
+    vector<double> cvol, cxyz;
+    vector<int> sptr, sidx;            // vertex stencil (geometric.cpp:1781-1841)
+    vector<double> lsq;                // 6 coeffs per cell (gradient.cpp:246-430)
+
+    // ---- case --------------------------------------------------------------
+    double rho[2] = {1000, 1000}, mu[2] = {1e-3, 1e-3}, sigma = 0, g[3] = {0, 0, 0};
+    double dt = 0.01, oldDt = 0.01, old2Dt = 0.01;
+    double urf_u = 0.3, urf_p = 0.3, urf_pu = 0.3, lim = 1e10;
+    int iterReal = 0;
+
+    // ---- fields (cell) -------------------------------------------------------
+    vector<double> P, U, V, W, VF, MF, Rho, Mu;
+    vector<double> oldP, oldU, oldV, oldW, oldVF, oldRho, Qm0, Qm1, Qm2, Qm3;
+    vector<double> srcG[4], srcS[4], kappa;
+    vector<double> gP, gU, gV, gW, gVF;    // [3n]
+    vector<double> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
+    // face left/right values (src/solvers/reconIncom.cpp:501-1035)
+    vector<double> LP, RP, LU, RU, LV, RV, LW, RW, LVF, RVF, Lmu, Rmu, LRho, RRho;
+
+    // ---- last assembled system -------------------------------------------------
+    vector<int> Arows, Acols;
+    vector<double> Avals, B0, B1, B2, X0, X1, X2;
+    int last_iters = 0; double last_res = 0;
+    double tol = 1e-13; int maxit = 5000;
+};
+
+// ------------------------------------------------------------------------
+// geometry: calcUnitNormals_Area3dPolygon (geometric.cpp:16-204)
+void face_polygon(const Oracle& o, int f, double nrm[3], double& area, double c[3],
+                  double& VSn, double cc[3]) {
+    int n = o.fptr[f + 1] - o.fptr[f];
+    const int* p = &o.fpts[o.fptr[f]];
+    auto X = [&](int k, int a) { return o.pts[3 * (size_t)p[k] + a]; };
+    double avg[3];
+    for (int a = 0; a < 3; ++a) {             // std::accumulate order, then / size
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += X(k, a);
+        avg[a] = s / (double)n;
+    }
+    double x_tmp[3] = {0, 0, 0}, V0old[3] = {0, 0, 0};
+    double un[3] = {0, 0, 0}, ct[3] = {0, 0, 0}, area_tmp = 0, vsn = 0;
+    for (int two = 0; two < 5; ++two) {
+        un[0] = un[1] = un[2] = 0; ct[0] = ct[1] = ct[2] = 0; area_tmp = 0; vsn = 0;
+        double V0[3];
+        for (int a = 0; a < 3; ++a)
+            V0[a] = two == 0 ? avg[a] : V0old[a] + 1.0 * (x_tmp[a] - V0old[a]);
+        for (int a = 0; a < 3; ++a) { V0old[a] = V0[a]; x_tmp[a] = 0.0; }
+        double totArea = 0.0;
+        for (int i = 0; i < n; ++i) {
+            int b = i, cidx = (i == n - 1) ? 0 : i + 1;
+            double A[3], Bv[3];
+            for (int a = 0; a < 3; ++a) { A[a] = X(b, a) - V0[a]; Bv[a] = X(cidx, a) - V0[a]; }
+            double Nx = 0.5 * (A[1] * Bv[2] - A[2] * Bv[1]);
+            double Ny = 0.5 * (A[2] * Bv[0] - A[0] * Bv[2]);
+            double Nz = 0.5 * (A[0] * Bv[1] - A[1] * Bv[0]);
+            un[0] += Nx; un[1] += Ny; un[2] += Nz;
+            double ta = sqrtl(Nx * Nx + Ny * Ny + Nz * Nz);
+            double r[3];
+            for (int a = 0; a < 3; ++a) r[a] = (V0[a] + X(b, a) + X(cidx, a)) / 3.0;
+            for (int a = 0; a < 3; ++a) x_tmp[a] += r[a] * ta;
+            totArea += ta;
+            vsn += V0[0] * Nx; vsn += V0[1] * Ny; vsn += V0[2] * Nz;
+            double Nv[3] = {Nx, Ny, Nz};
+            for (int a = 0; a < 3; ++a) {
+                double s1 = V0[a] + X(b, a), s2 = X(b, a) + X(cidx, a), s3 = X(cidx, a) + V0[a];
+                ct[a] += 2.0 * Nv[a] * (s1 * s1 + s2 * s2 + s3 * s3);
+            }
+        }
+        double mag = sqrtl(un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
+        un[0] /= mag; un[1] /= mag; un[2] /= mag;
+        area_tmp = mag;
+        for (int a = 0; a < 3; ++a) x_tmp[a] /= totArea;
+    }
+    for (int a = 0; a < 3; ++a) { nrm[a] = un[a]; cc[a] = ct[a]; c[a] = x_tmp[a]; }
+    area = area_tmp; VSn = vsn;
+}
+
+void build_geometry(Oracle& o) {
+    // cell -> faces, ascending (src/load/load.cpp:373-385)
+    vector<int> cnt(o.nc + 1, 0);
+    for (int f = 0; f < o.nf; ++f) { cnt[o.owner[f] + 1]++; if (f < o.nif) cnt[o.nbr[f] + 1]++; }
+    for (int i = 0; i < o.nc; ++i) cnt[i + 1] += cnt[i];
+    o.cfptr = cnt; o.cfidx.assign(cnt[o.nc], 0);
+    vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int f = 0; f < o.nf; ++f) {
+        o.cfidx[pos[o.owner[f]]++] = f;
+        if (f < o.nif) o.cfidx[pos[o.nbr[f]]++] = f;
+    }
+    o.face_patch.assign(o.nf, -1);
+    for (size_t b = 0; b < o.patches.size(); ++b)
+        for (int f = o.patches[b].start; f < o.patches[b].start + o.patches[b].nfaces; ++f)
+            o.face_patch[f] = (int)b;
+
+    int nf = o.nf, nc = o.nc;
+    o.farea.assign(nf, 0); o.fnrm.assign(3 * nf, 0); o.fxyz.assign(3 * nf, 0);
+    o.cvol.assign(nc, 0); o.cxyz.assign(3 * nc, 0);
+    // geometric.cpp:804-877: volume and centroid by face accumulation
+    for (int f = 0; f < nf; ++f) {
+        double nrm[3], area, c[3], VSn, cc[3];
+        face_polygon(o, f, nrm, area, c, VSn, cc);
+        if (!(area >= std::numeric_limits<double>::min())) { fprintf(stderr, "zero face area\n"); }
+        o.farea[f] = area;
+        for (int a = 0; a < 3; ++a) { o.fnrm[3 * f + a] = nrm[a]; o.fxyz[3 * f + a] = c[a]; }
+        int ow = o.owner[f];
+        o.cvol[ow] += VSn / 3.0;
+        for (int a = 0; a < 3; ++a) o.cxyz[3 * ow + a] += cc[a];
+        if (f < o.nif) {
+            int nb = o.nbr[f];
+            o.cvol[nb] -= VSn / 3.0;
+            for (int a = 0; a < 3; ++a) o.cxyz[3 * nb + a] -= cc[a];
+        }
+    }
+    for (int i = 0; i < nc; ++i)
+        for (int a = 0; a < 3; ++a) o.cxyz[3 * i + a] *= 1.0 / (24.0 * 2.0 * o.cvol[i]);
+
+    // geometric.cpp:1206-1227 distCells; 1297-1497 face metrics
+    o.fdist.assign(3 * nf, 0); o.fwC.assign(nf, 0); o.fmagPN.assign(nf, 0); o.falpha.assign(nf, 0);
+    o.fePN.assign(3 * nf, 0); o.fskew.assign(3 * nf, 0); o.fvecPF.assign(3 * nf, 0); o.fvecNF.assign(3 * nf, 0);
+    for (int f = 0; f < nf; ++f) {
+        const double* fc = &o.fxyz[3 * f];
+        const double* n = &o.fnrm[3 * f];
+        const double* cP = &o.cxyz[3 * o.owner[f]];
+        if (f < o.nif) {
+            const double* cN = &o.cxyz[3 * o.nbr[f]];
+            for (int a = 0; a < 3; ++a) o.fdist[3 * f + a] = cN[a] - cP[a];
+            double dFP[3], dFN[3], dPN3[3];
+            for (int a = 0; a < 3; ++a) { dFP[a] = fc[a] - cP[a]; dFN[a] = fc[a] - cN[a]; dPN3[a] = cN[a] - cP[a]; }
+            double dPN = sqrt(dPN3[0] * dPN3[0] + dPN3[1] * dPN3[1] + dPN3[2] * dPN3[2]);
+            for (int a = 0; a < 3; ++a) { o.fvecPF[3 * f + a] = dFP[a]; o.fvecNF[3 * f + a] = dFN[a]; }
+            double dFP_of = std::abs(dFP[0] * n[0] + dFP[1] * n[1] + dFP[2] * n[2]);
+            double dFN_of = std::abs(dFN[0] * n[0] + dFN[1] * n[1] + dFN[2] * n[2]);
+            double wC = dFN_of / (dFP_of + dFN_of);
+            wC = (float)(round(wC * 10000000) / 10000000);          // geometric.cpp:1351
+            o.fwC[f] = wC;
+            o.fmagPN[f] = dPN;
+            for (int a = 0; a < 3; ++a) o.fePN[3 * f + a] = dPN3[a] / dPN;
+            double al = 0.0;
+            al += n[0] * o.fePN[3 * f]; al += n[1] * o.fePN[3 * f + 1]; al += n[2] * o.fePN[3 * f + 2];
+            o.falpha[f] = 1.0 / std::abs(al);
+            double D = -(n[0] * fc[0] + n[1] * fc[1] + n[2] * fc[2]);
+            double u = (n[0] * cP[0] + n[1] * cP[1] + n[2] * cP[2] + D) /
+                       (n[0] * (cP[0] - cN[0]) + n[1] * (cP[1] - cN[1]) + n[2] * (cP[2] - cN[2]));
+            for (int a = 0; a < 3; ++a) {
+                double s = cP[a] - u * (cP[a] - cN[a]);
+                o.fskew[3 * f + a] = fc[a] - s;
+            }
+        } else {
+            for (int a = 0; a < 3; ++a) o.fdist[3 * f + a] = fc[a] - cP[a];
+            o.fwC[f] = (float)(round(0.5 * 10000000) / 10000000);   // geometric.cpp:1434-1436
+            double dFP[3];
+            for (int a = 0; a < 3; ++a) dFP[a] = fc[a] - cP[a];
+            double dFP_len = sqrt(dFP[0] * dFP[0] + dFP[1] * dFP[1] + dFP[2] * dFP[2]);
+            for (int a = 0; a < 3; ++a) { o.fvecPF[3 * f + a] = dFP[a]; o.fvecNF[3 * f + a] = dFP[a]; }
+            double dFP_of = std::abs(dFP[0] * n[0] + dFP[1] * n[1] + dFP[2] * n[2]);
+            o.fmagPN[f] = 2.0 * dFP_of;
+            for (int a = 0; a < 3; ++a) o.fePN[3 * f + a] = dFP[a] / dFP_len;
+            double al = 0.0;
+            al += n[0] * o.fePN[3 * f]; al += n[1] * o.fePN[3 * f + 1]; al += n[2] * o.fePN[3 * f + 2];
+            o.falpha[f] = 1.0 / std::abs(al);
+        }
+    }
+
+    // vertex stencil (geometric.cpp:1786-1841)
+    vector<vector<int>> pst(o.np);
+    for (int i = 0; i < nc; ++i)
+        for (int k = o.cptr[i]; k < o.cptr[i + 1]; ++k) pst[o.cpts[k]].push_back(i);
+    o.sptr.assign(nc + 1, 0); o.sidx.clear();
+    vector<int> tmp;
+    for (int i = 0; i < nc; ++i) {
+        tmp.clear();
+        for (int k = o.cptr[i]; k < o.cptr[i + 1]; ++k)
+            for (int c : pst[o.cpts[k]]) {
+                if (c == i) continue;
+                if (std::find(tmp.begin(), tmp.end(), c) == tmp.end()) tmp.push_back(c);
+            }
+        o.sidx.insert(o.sidx.end(), tmp.begin(), tmp.end());
+        o.sptr[i + 1] = (int)o.sidx.size();
+    }
+
+    // initLeastSquare, cell-vertex stencil, 3x3 block (gradient.cpp:246-430)
+    vector<double> vs(6 * (size_t)nc, 0.0);
+    auto acc = [&](int i, double dx, double dy, double dz) {
+        double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+        double v[3] = {dx, dy, dz};
+        double* s = &vs[6 * (size_t)i];
+        s[0] += wk * v[0] * v[0]; s[1] += wk * v[0] * v[1]; s[2] += wk * v[0] * v[2];
+        s[3] += wk * v[1] * v[1]; s[4] += wk * v[1] * v[2]; s[5] += wk * v[2] * v[2];
+    };
+    for (int i = 0; i < nc; ++i)
+        for (int k = o.sptr[i]; k < o.sptr[i + 1]; ++k) {
+            int j = o.sidx[k];
+            acc(i, o.cxyz[3 * j] - o.cxyz[3 * i], o.cxyz[3 * j + 1] - o.cxyz[3 * i + 1],
+                o.cxyz[3 * j + 2] - o.cxyz[3 * i + 2]);
+        }
+    for (int f = o.nif; f < nf; ++f) {
+        int i = o.owner[f];
+        acc(i, o.fxyz[3 * f] - o.cxyz[3 * i], o.fxyz[3 * f + 1] - o.cxyz[3 * i + 1],
+            o.fxyz[3 * f + 2] - o.cxyz[3 * i + 2]);
+    }
+    o.lsq.assign(6 * (size_t)nc, 0.0);
+    for (int i = 0; i < nc; ++i) {
+        const double* s = &vs[6 * (size_t)i];
+        // packed 9x9 indices 0,1,2,9,10,17 -> s0..s5
+        double v0 = s[0], v1 = s[1], v2 = s[2], v9 = s[3], v10 = s[4], v17 = s[5];
+        double det = v0 * v9 * v17 + v1 * v10 * v2 + v2 * v1 * v10 - v0 * v10 * v10
+                   - v2 * v9 * v2 - v1 * v1 * v17;
+        if (det == 0.0) fprintf(stderr, "detA=0 in LSQ\n");
+        double* c = &o.lsq[6 * (size_t)i];
+        c[0] = (v9 * v17 - v10 * v10) / det;
+        c[1] = (v2 * v10 - v1 * v17) / det;
+        c[2] = (v1 * v10 - v2 * v9) / det;
+        c[3] = (v0 * v17 - v2 * v2) / det;
+        c[4] = (v2 * v1 - v0 * v10) / det;
+        c[5] = (v0 * v9 - v1 * v1) / det;
+    }
+}
+
+// ------------------------------------------------------------------------
+// calcLeastSquare(mesh,"cellVertex","1st","cell",cn,fn,...) (gradient.cpp:593-1159)
+// phi: cell values; bR: face right value at boundary faces (varR[fn]).
+void lsq_grad(const Oracle& o, const vector<double>& phi, const vector<double>& faceR,
+              vector<double>& grad) {
+    int nc = o.nc;
+    vector<double> gt(3 * (size_t)nc, 0.0);
+    for (int i = 0; i < nc; ++i)
+        for (int k = o.sptr[i]; k < o.sptr[i + 1]; ++k) {
+            int j = o.sidx[k];
+            double dx = o.cxyz[3 * j] - o.cxyz[3 * i], dy = o.cxyz[3 * j + 1] - o.cxyz[3 * i + 1],
+                   dz = o.cxyz[3 * j + 2] - o.cxyz[3 * i + 2];
+            double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+            double d = phi[j] - phi[i];
+            gt[3 * i] += wk * dx * d; gt[3 * i + 1] += wk * dy * d; gt[3 * i + 2] += wk * dz * d;
+        }
+    for (int f = o.nif; f < o.nf; ++f) {
+        int i = o.owner[f];
+        double d = faceR[f] - phi[i];
+        double dx = o.fxyz[3 * f] - o.cxyz[3 * i], dy = o.fxyz[3 * f + 1] - o.cxyz[3 * i + 1],
+               dz = o.fxyz[3 * f + 2] - o.cxyz[3 * i + 2];
+        double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+        gt[3 * i] += wk * dx * d; gt[3 * i + 1] += wk * dy * d; gt[3 * i + 2] += wk * dz * d;
+    }
+    grad.assign(3 * (size_t)nc, 0.0);
+    for (int i = 0; i < nc; ++i) {
+        const double* c = &o.lsq[6 * (size_t)i];
+        const double* t = &gt[3 * (size_t)i];
+        double r0 = 0.0; r0 += c[0] * t[0]; r0 += c[1] * t[1]; r0 += c[2] * t[2];
+        double r1 = 0.0; r1 += c[1] * t[0]; r1 += c[3] * t[1]; r1 += c[4] * t[2];
+        double r2 = 0.0; r2 += c[2] * t[0]; r2 += c[4] * t[1]; r2 += c[5] * t[2];
+        grad[3 * i] = r0; grad[3 * i + 1] = r1; grad[3 * i + 2] = r2;
+    }
+}
+
+// setCellVarMinMax (src/solvers/cellVarMinMax.cpp:8-24)
+void minmax(const Oracle& o, const vector<double>& phi, vector<double>& mx, vector<double>& mn) {
+    mx.assign(o.nc, 0); mn.assign(o.nc, 0);
+    for (int i = 0; i < o.nc; ++i) {
+        double a = phi[i], b = phi[i];
+        for (int k = o.sptr[i]; k < o.sptr[i + 1]; ++k) {
+            a = std::max(a, phi[o.sidx[k]]); b = std::min(b, phi[o.sidx[k]]);
+        }
+        mx[i] = a; mn[i] = b;
+    }
+}
+
+// calcIncomCellEOSVF (eos.cpp:10-151) + calcCellTransport (transport.cpp:9-98)
+void cell_eos_transport(Oracle& o) {
+    for (int i = 0; i < o.nc; ++i) {
+        double a0 = std::max(0.0, std::min(1.0, o.VF[i]));
+        double v0 = std::max(0.0, std::min(1.0, a0));
+        double v1 = std::max(0.0, std::min(1.0, 1.0 - a0));
+        double r = 0.0; r += v0 * o.rho[0]; r += v1 * o.rho[1];
+        o.Rho[i] = r;
+        double m0 = std::max(0.0, std::min(1.0, v0 * o.rho[0] / r));
+        o.MF[i] = m0;
+        o.VF[i] = v0;
+    }
+    for (int i = 0; i < o.nc; ++i) {
+        double v0 = o.VF[i], v1 = 1.0 - o.VF[i];
+        double m = 0.0; m += v0 * o.mu[0]; m += v1 * o.mu[1];
+        o.Mu[i] = m + 0.0;
+    }
+}
+
+// setIncomValuesLeftRightFace -> reconIncomZeroOrder + calcOtherDataIncomFromEOS
+// (reconIncom.cpp:9-67, 440-496, 501-1035)
+void recon_faces(Oracle& o) {
+    int nf = o.nf;
+    auto sz = [&](vector<double>& v) { v.assign(nf, 0.0); };
+    sz(o.LP); sz(o.RP); sz(o.LU); sz(o.RU); sz(o.LV); sz(o.RV); sz(o.LW); sz(o.RW);
+    sz(o.LVF); sz(o.RVF); sz(o.Lmu); sz(o.Rmu); sz(o.LRho); sz(o.RRho);
+    for (int f = 0; f < o.nif; ++f) {
+        int l = o.owner[f], r = o.nbr[f];
+        o.LP[f] = o.P[l]; o.LU[f] = o.U[l]; o.LV[f] = o.V[l]; o.LW[f] = o.W[l];
+        o.LVF[f] = o.VF[l]; o.Lmu[f] = o.Mu[l];
+        o.RP[f] = o.P[r]; o.RU[f] = o.U[r]; o.RV[f] = o.V[r]; o.RW[f] = o.W[r];
+        o.RVF[f] = o.VF[r]; o.Rmu[f] = o.Mu[r];
+    }
+    for (const Patch& pb : o.patches) {
+        for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
+            int c = o.owner[f];
+            const double* n = &o.fnrm[3 * f];
+            // pressure (reconIncom.cpp:600-617)
+            if (pb.p_type == FIXED_VALUE) o.LP[f] = pb.p_val; else o.LP[f] = o.P[c];
+            o.RP[f] = o.LP[f];
+            // velocity (reconIncom.cpp:642-852)
+            if (pb.u_type == FIXED_VALUE) {
+                o.LU[f] = pb.u_val[0]; o.LV[f] = pb.u_val[1]; o.LW[f] = pb.u_val[2];
+                // live orifice profile (reconIncom.cpp:707-727)
+                for (int k = 0; k < 24; ++k) {
+                    double cen = 0.0006 * (double)k;
+                    double fx = o.fxyz[3 * f];
+                    if (fx > cen - 0.00016 && fx < cen + 0.00016) {
+                        double xn = fx - cen, zn = o.fxyz[3 * f + 2] - 0.001;
+                        double th = atan(zn / xn);
+                        o.LU[f] = 0.0 * cos(th) - 0.1 * sin(th);
+                        o.LV[f] = -0.1;
+                        o.LW[f] = 0.0 * sin(th) + 0.1 * cos(th);
+                    }
+                }
+                o.RU[f] = pb.u_val[0]; o.RV[f] = pb.u_val[1]; o.RW[f] = pb.u_val[2];
+            } else if (pb.u_type == ZERO_GRADIENT) {
+                o.LU[f] = o.U[c]; o.LV[f] = o.V[c]; o.LW[f] = o.W[c];
+                o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
+            } else if (pb.u_type == SLIP) {
+                double u = o.U[c], v = o.V[c], w = o.W[c];
+                double nv = u * n[0] + v * n[1] + w * n[2];
+                o.LU[f] = u - nv * n[0]; o.LV[f] = v - nv * n[1]; o.LW[f] = w - nv * n[2];
+                o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
+            } else if (pb.u_type == NO_SLIP) {
+                o.LU[f] = o.LV[f] = o.LW[f] = 0.0; o.RU[f] = o.RV[f] = o.RW[f] = 0.0;
+            } else if (pb.u_type == SURFACE_NORMAL_FIXED_VALUE) {
+                double nv = pb.u_val[0];
+                o.LU[f] = nv * n[0]; o.LV[f] = nv * n[1]; o.LW[f] = nv * n[2];
+                o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
+            } else if (pb.u_type == INLET_OUTLET) {
+                double nv = pb.u_val[0];
+                double own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
+                if (own > 0.0) { o.LU[f] = o.U[c]; o.LV[f] = o.V[c]; o.LW[f] = o.W[c]; }
+                else { o.LU[f] = nv * n[0]; o.LV[f] = nv * n[1]; o.LW[f] = nv * n[2]; }
+                o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
+            }
+            // volume fraction (reconIncom.cpp:882-928)
+            if (pb.vf_type == FIXED_VALUE) o.LVF[f] = pb.vf_val;
+            else if (pb.vf_type == INLET_OUTLET) {
+                double own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
+                o.LVF[f] = own > 0.0 ? o.VF[c] : pb.vf_val;
+            } else o.LVF[f] = o.VF[c];
+            o.RVF[f] = o.LVF[f];
+            o.Lmu[f] = o.Mu[c]; o.Rmu[f] = o.Lmu[f];             // reconIncom.cpp:1024-1028
+        }
+    }
+    // face density from the mixture EOS (reconIncom.cpp:445-493, eos.cpp:439-489)
+    for (int f = 0; f < nf; ++f) {
+        for (int side = 0; side < 2; ++side) {
+            double a = side == 0 ? o.LVF[f] : o.RVF[f];
+            double v0 = std::max(0.0, std::min(1.0, a));
+            double v1 = std::max(0.0, std::min(1.0, 1.0 - a));
+            double r = 0.0; r += v0 * o.rho[0]; r += v1 * o.rho[1];
+            (side == 0 ? o.LRho : o.RRho)[f] = r;
+        }
+    }
+}
+
+// calcSourceGravity (src/solvers/source/gravity.cpp:7-75)
+void source_gravity(Oracle& o) {
+    for (int k = 0; k < 4; ++k) o.srcG[k].assign(o.nc, 0.0);
+    for (int f = 0; f < o.nf; ++f) {
+        double wCL = o.fwC[f], wCR = 1.0 - wCL;
+        const double* n = &o.fnrm[3 * f];
+        double UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
+        double UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
+        double UnF = wCL * UnL + wCR * UnR;
+        double RhoF = 1.0 / (wCL / o.LRho[f] + wCR / o.RRho[f]);
+        double gL = 0.0, gR = 0.0;
+        for (int a = 0; a < 3; ++a) { gL += o.g[a] * o.fvecPF[3 * f + a]; gR += o.g[a] * o.fvecNF[3 * f + a]; }
+        int l = o.owner[f];
+        for (int a = 0; a < 3; ++a) o.srcG[a][l] += ((RhoF * gL * n[a]) * o.farea[f] / o.cvol[l]);
+        o.srcG[3][l] += ((RhoF * gL * UnF) * o.farea[f] / o.cvol[l]);
+        if (f < o.nif) {
+            int r = o.nbr[f];
+            for (int a = 0; a < 3; ++a) o.srcG[a][r] -= ((RhoF * gR * n[a]) * o.farea[f] / o.cvol[r]);
+            o.srcG[3][r] -= ((RhoF * gR * UnF) * o.farea[f] / o.cvol[r]);
+        }
+    }
+}
+
+// Rhie-Chow face velocity of the assembly face loops:
+// eqMomentum.cpp:519-602, eqPressure.cpp:416-501,
+// surfaceNormalVelocity.cpp:125-223 (Pressure, Gravity, SurfTens).
+double face_UnF(const Oracle& o, int f) {
+    int l = o.owner[f], r = o.nbr[f];
+    const double* n = &o.fnrm[3 * f];
+    const double* e = &o.fePN[3 * f];
+    const double* sk = &o.fskew[3 * f];
+    double wCL = o.fwC[f], wCR = 1.0 - wCL;
+    double alpha = o.falpha[f], dPN = o.fmagPN[f];
+    double ULs = o.U[l], VLs = o.V[l], WLs = o.W[l], URs = o.U[r], VRs = o.V[r], WRs = o.W[r];
+    for (int a = 0; a < 3; ++a) {
+        ULs += o.gU[3 * l + a] * sk[a]; URs += o.gU[3 * r + a] * sk[a];
+        VLs += o.gV[3 * l + a] * sk[a]; VRs += o.gV[3 * r + a] * sk[a];
+        WLs += o.gW[3 * l + a] * sk[a]; WRs += o.gW[3 * r + a] * sk[a];
+    }
+    ULs = std::max(o.minU[l], std::min(o.maxU[l], ULs)); URs = std::max(o.minU[r], std::min(o.maxU[r], URs));
+    VLs = std::max(o.minV[l], std::min(o.maxV[l], VLs)); VRs = std::max(o.minV[r], std::min(o.maxV[r], VRs));
+    WLs = std::max(o.minW[l], std::min(o.maxW[l], WLs)); WRs = std::max(o.minW[r], std::min(o.maxW[r], WRs));
+    double UnL = ULs * n[0] + VLs * n[1] + WLs * n[2];
+    double UnR = URs * n[0] + VRs * n[1] + WRs * n[2];
+    double UnF = wCL * UnL + wCR * UnR;
+    double RhoL = o.Rho[l], RhoR = o.Rho[r];
+    double Rs = 1.0 / (wCL / RhoL + wCR / RhoR);
+    double t1 = o.dt / Rs;
+    // pressure
+    UnF -= alpha * t1 * (o.P[r] - o.P[l]) / dPN;
+    for (int a = 0; a < 3; ++a) {
+        UnF -= t1 * Rs * wCL / RhoL * o.gP[3 * l + a] * (n[a] - alpha * e[a]);
+        UnF -= t1 * Rs * wCR / RhoR * o.gP[3 * r + a] * (n[a] - alpha * e[a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        UnF += t1 * Rs * wCL / RhoL * o.gP[3 * l + a] * n[a];
+        UnF += t1 * Rs * wCR / RhoR * o.gP[3 * r + a] * n[a];
+    }
+    // gravity
+    for (int a = 0; a < 3; ++a) UnF += alpha * t1 * Rs * o.g[a] * alpha * e[a];
+    for (int a = 0; a < 3; ++a) {
+        UnF += t1 * Rs * wCL / RhoL * o.srcG[a][l] * (n[a] - alpha * e[a]);
+        UnF += t1 * Rs * wCR / RhoR * o.srcG[a][r] * (n[a] - alpha * e[a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        UnF -= alpha * t1 * Rs * wCL / RhoL * o.srcG[a][l] * n[a];
+        UnF -= alpha * t1 * Rs * wCR / RhoR * o.srcG[a][r] * n[a];
+    }
+    // surface tension
+    UnF += alpha * t1 * o.sigma * (wCL * o.kappa[l] + wCR * o.kappa[r]) * (o.VF[r] - o.VF[l]) / dPN;
+    for (int a = 0; a < 3; ++a) {
+        UnF += t1 * Rs * wCL / RhoL * o.srcS[a][l] * (n[a] - alpha * e[a]);
+        UnF += t1 * Rs * wCR / RhoR * o.srcS[a][r] * (n[a] - alpha * e[a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        UnF -= alpha * t1 * Rs * wCL / RhoL * o.srcS[a][l] * n[a];
+        UnF -= alpha * t1 * Rs * wCR / RhoR * o.srcS[a][r] * n[a];
+    }
+    return UnF;
+}
+
+void gradients5(Oracle& o) {
+    // eqMomentum.cpp:95-104 / eqPressure.cpp:96-105 (the zeroing at :57-72 is
+    // overwritten by the least-squares call)
+    lsq_grad(o, o.P, o.RP, o.gP);
+    lsq_grad(o, o.U, o.RU, o.gU);
+    lsq_grad(o, o.V, o.RV, o.gV);
+    lsq_grad(o, o.W, o.RW, o.gW);
+    lsq_grad(o, o.VF, o.RVF, o.gVF);
+}
+
+// calcMomentumEqs assembly (eqMomentum.cpp:9-1008)
+void assemble_momentum(Oracle& o) {
+    int nc = o.nc;
+    minmax(o, o.P, o.maxP, o.minP);
+    minmax(o, o.U, o.maxU, o.minU);
+    minmax(o, o.V, o.maxV, o.minV);
+    minmax(o, o.W, o.maxW, o.minW);
+    gradients5(o);
+    o.Arows.assign(nc, 0); o.Acols.assign(nc, 0); o.Avals.assign(nc, 0.0);
+    o.B0.assign(nc, 0.0); o.B1.assign(nc, 0.0); o.B2.assign(nc, 0.0);
+    double dt = o.dt;
+    for (int i = 0; i < nc; ++i) {               // eqMomentum.cpp:223-326
+        double R = o.Rho[i], vol = o.cvol[i];
+        o.Arows[i] = i; o.Acols[i] = i;
+        o.Avals[i] = R * 1.5 * vol / dt;
+        o.B0[i] = -R * (1.5 * o.U[i] - 2.0 * o.oldU[i] + 0.5 * o.Qm0[i]) * vol / dt;
+        o.B1[i] = -R * (1.5 * o.V[i] - 2.0 * o.oldV[i] + 0.5 * o.Qm1[i]) * vol / dt;
+        o.B2[i] = -R * (1.5 * o.W[i] - 2.0 * o.oldW[i] + 0.5 * o.Qm2[i]) * vol / dt;
+        o.B0[i] += vol * o.srcG[0][i]; o.B1[i] += vol * o.srcG[1][i]; o.B2[i] += vol * o.srcG[2][i];
+        o.B0[i] += vol * o.srcS[0][i]; o.B1[i] += vol * o.srcS[1][i]; o.B2[i] += vol * o.srcS[2][i];
+    }
+    double* B[3] = {o.B0.data(), o.B1.data(), o.B2.data()};
+    for (int f = 0; f < o.nif; ++f) {            // eqMomentum.cpp:331-787
+        int l = o.owner[f], r = o.nbr[f];
+        double area = o.farea[f];
+        const double* n = &o.fnrm[3 * f];
+        const double* e = &o.fePN[3 * f];
+        const double* sk = &o.fskew[3 * f];
+        double wCL = o.fwC[f], wCR = 1.0 - wCL;
+        double muF = 1.0 / (wCL / o.Lmu[f] + wCR / o.Rmu[f]);
+        double RhoL = o.Rho[l], RhoR = o.Rho[r];
+        double dPN = o.fmagPN[f], alpha = o.falpha[f];
+        double UnF = face_UnF(o, f);
+        double wL = (UnF > 0.0) ? 1.0 : 0.0, wR = 1.0 - wL;
+        double PL = o.P[l], PR = o.P[r];
+        double PF = wCL * PL + wCR * PR;
+        double UF = wL * o.LU[f] + wR * o.RU[f];
+        double VFv = wL * o.LV[f] + wR * o.RV[f];
+        double WF = wL * o.LW[f] + wR * o.RW[f];
+        PF = wCL * PL + wCR * PR;
+        for (int a = 0; a < 3; ++a) {
+            PF += wCL * o.gP[3 * l + a] * sk[a];
+            PF += wCR * o.gP[3 * r + a] * sk[a];
+        }
+        PF = std::max(o.minP[l], std::min(o.maxP[l], PF));
+        PF = std::max(o.minP[r], std::min(o.maxP[r], PF));
+        o.Avals[l] += (wL * RhoL * UnF * area + alpha * muF / dPN * area);
+        double v = (wR * RhoL * UnF * area - alpha * muF / dPN * area);
+        if (!(std::abs(v) < std::numeric_limits<double>::min())) {
+            o.Arows.push_back(l); o.Acols.push_back(r); o.Avals.push_back(v);
+        }
+        o.Avals[r] -= (wR * RhoR * UnF * area - alpha * muF / dPN * area);
+        v = -(wL * RhoR * UnF * area + alpha * muF / dPN * area);
+        if (!(std::abs(v) < std::numeric_limits<double>::min())) {
+            o.Arows.push_back(r); o.Acols.push_back(l); o.Avals.push_back(v);
+        }
+        double Fv[3] = {UF, VFv, WF};
+        double uL[3] = {o.U[l], o.V[l], o.W[l]}, uR[3] = {o.U[r], o.V[r], o.W[r]};
+        const double* gL[3] = {&o.gU[3 * l], &o.gV[3 * l], &o.gW[3 * l]};
+        const double* gR[3] = {&o.gU[3 * r], &o.gV[3 * r], &o.gW[3 * r]};
+        for (int k = 0; k < 3; ++k) {
+            double cL = (RhoL * Fv[k] * UnF + PF * n[k]) * area;
+            double cR = (RhoR * Fv[k] * UnF + PF * n[k]) * area;
+            double df = alpha * muF * (uR[k] - uL[k]) / dPN * area;
+            for (int a = 0; a < 3; ++a)
+                df += (muF * (wCL * gL[k][a] + wCR * gR[k][a]) * (n[a] - alpha * e[a]) * area);
+            double dL = ((gL[k][0] * n[0] + gL[k][1] * n[1] + gL[k][2] * n[2]) * muF * area);
+            double dR = ((gR[k][0] * n[0] + gR[k][1] * n[1] + gR[k][2] * n[2]) * muF * area);
+            B[k][l] -= (cL - df - dL);
+            B[k][r] += (cR - df - dR);
+        }
+    }
+    for (const Patch& pb : o.patches) {          // eqMomentum.cpp:795-1008
+        for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
+            int c = o.owner[f];
+            double area = o.farea[f];
+            const double* n = &o.fnrm[3 * f];
+            const double* e = &o.fePN[3 * f];
+            double UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
+            double UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
+            double UnF = 0.5 * UnL + 0.5 * UnR;
+            double RhoL = o.LRho[f];
+            double t1 = 1.0 / RhoL * o.dt;
+            double Fv[3] = {0.5 * o.LU[f] + 0.5 * o.RU[f], 0.5 * o.LV[f] + 0.5 * o.RV[f],
+                            0.5 * o.LW[f] + 0.5 * o.RW[f]};
+            double PF = 0.5 * o.LP[f] + 0.5 * o.RP[f];
+            double cU = 0, cUd = 0;
+            if (pb.u_type == FIXED_VALUE || pb.u_type == NO_SLIP || pb.u_type == SURFACE_NORMAL_FIXED_VALUE) { cU = 0; cUd = 1; }
+            else if (pb.u_type == ZERO_GRADIENT) { cU = 1; cUd = 0; }
+            else if (pb.u_type == SLIP) { cU = 0; cUd = 0; }
+            else if (pb.u_type == INLET_OUTLET) {
+                double own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
+                cU = own > 0.0 ? 1.0 : 0.0; cUd = own > 0.0 ? 0.0 : 1.0;
+            }
+            double muF = 0.5 * o.Lmu[f] + 0.5 * o.Rmu[f];
+            double org[3][2] = {{o.U[c], o.RU[f]}, {o.V[c], o.RV[f]}, {o.W[c], o.RW[f]}};
+            double dPN = 0.5 * o.fmagPN[f], alpha = o.falpha[f];
+            if (pb.p_type == FIXED_VALUE) {
+                UnF -= alpha * t1 * (pb.p_val - o.P[c]) / dPN;
+                for (int a = 0; a < 3; ++a) UnF += alpha * t1 * o.gP[3 * c + a] * e[a];
+            }
+            const double* gc[3] = {&o.gU[3 * c], &o.gV[3 * c], &o.gW[3 * c]};
+            double dn[3];
+            for (int k = 0; k < 3; ++k) {
+                dn[k] = 0.0;
+                dn[k] += alpha * (org[k][1] - org[k][0]) / dPN;
+            }
+            for (int a = 0; a < 3; ++a) {
+                double tv = n[a] - alpha * e[a];
+                for (int k = 0; k < 3; ++k) dn[k] += gc[k][a] * tv;
+            }
+            for (int a = 0; a < 3; ++a)
+                for (int k = 0; k < 3; ++k) dn[k] += (gc[k][a] * n[a]);
+            o.Avals[c] += cU * (RhoL * UnF * area);
+            o.Avals[c] += cUd * (muF * alpha / dPN * area);
+            for (int k = 0; k < 3; ++k)
+                B[k][c] -= (RhoL * Fv[k] * UnF * area + PF * n[k] * area - muF * dn[k] * area);
+        }
+    }
+}
+
+// calcPressureEq assembly (eqPressure.cpp:8-673)
+void assemble_pressure(Oracle& o) {
+    int nc = o.nc;
+    minmax(o, o.U, o.maxU, o.minU);
+    minmax(o, o.V, o.maxV, o.minV);
+    minmax(o, o.W, o.maxW, o.minW);
+    gradients5(o);
+    o.Arows.assign(nc, 0); o.Acols.assign(nc, 0); o.Avals.assign(nc, 0.0);
+    o.B0.assign(nc, 0.0);
+    for (int i = 0; i < nc; ++i) { o.Arows[i] = i; o.Acols[i] = i; }
+    for (int f = 0; f < o.nif; ++f) {
+        int l = o.owner[f], r = o.nbr[f];
+        double area = o.farea[f];
+        double wCL = o.fwC[f], wCR = 1.0 - wCL;
+        double RhoL = o.Rho[l], RhoR = o.Rho[r];
+        double Rs = 1.0 / (wCR / RhoL + wCR / RhoR);       // eqPressure.cpp:396
+        double t1 = o.dt / Rs;
+        double dPN = o.fmagPN[f], alpha = o.falpha[f];
+        double UnF = face_UnF(o, f);
+        o.Avals[l] += (+alpha * t1 / dPN * area);
+        o.Arows.push_back(l); o.Acols.push_back(r); o.Avals.push_back((-alpha * t1 / dPN * area));
+        o.Avals[r] -= (-alpha * t1 / dPN * area);
+        o.Arows.push_back(r); o.Acols.push_back(l); o.Avals.push_back(-(+alpha * t1 / dPN * area));
+        o.B0[l] -= (UnF * area);
+        o.B0[r] += (UnF * area);
+    }
+    for (const Patch& pb : o.patches) {          // eqPressure.cpp:570-673
+        for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
+            int c = o.owner[f];
+            double area = o.farea[f];
+            const double* n = &o.fnrm[3 * f];
+            const double* e = &o.fePN[3 * f];
+            double UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
+            double UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
+            double UnF = 0.5 * UnL + 0.5 * UnR;
+            double RhoL = o.LRho[f];
+            double t1 = 1.0 / RhoL * o.dt;
+            double cPd = pb.p_type == FIXED_VALUE ? 1.0 : 0.0;
+            double dPN = 0.5 * o.fmagPN[f], alpha = o.falpha[f];
+            if (pb.p_type == FIXED_VALUE) {
+                UnF -= alpha * t1 * (o.RP[f] - o.P[c]) / dPN;
+                for (int a = 0; a < 3; ++a) UnF += alpha * t1 * o.gP[3 * c + a] * e[a];
+            }
+            o.Avals[c] += cPd * (alpha * t1 / dPN * area);
+            o.B0[c] -= (UnF * area);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// linear solve: COO -> CSR (solveAMGCL.cpp:206-251), ILU0 + BiCGStab
+struct Csr { int n; vector<int> ptr, col; vector<double> val; };
+
+Csr coo_to_csr(int n, const vector<int>& r, const vector<int>& c, const vector<double>& v) {
+    Csr A; A.n = n; A.ptr.assign(n + 1, 0);
+    size_t nnz = r.size();
+    for (size_t k = 0; k < nnz; ++k) A.ptr[r[k]]++;
+    for (int i = 0, cs = 0; i < n; ++i) { int t = A.ptr[i]; A.ptr[i] = cs; cs += t; }
+    A.ptr[n] = (int)nnz;
+    A.col.assign(nnz, 0); A.val.assign(nnz, 0.0);
+    for (size_t k = 0; k < nnz; ++k) { int d = A.ptr[r[k]]++; A.col[d] = c[k]; A.val[d] = v[k]; }
+    for (int i = 0, last = 0; i <= n; ++i) { int t = A.ptr[i]; A.ptr[i] = last; last = t; }
+    // sort each row by column (AMGCL sorts rows too, mpi/amg.hpp:225) and merge duplicates
+    Csr S; S.n = n; S.ptr.assign(n + 1, 0);
+    vector<std::pair<int, double>> row;
+    for (int i = 0; i < n; ++i) {
+        row.clear();
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) row.push_back({A.col[k], A.val[k]});
+        std::stable_sort(row.begin(), row.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (size_t k = 0; k < row.size(); ++k) {
+            if (k > 0 && row[k].first == row[k - 1].first) { S.val.back() += row[k].second; continue; }
+            S.col.push_back(row[k].first); S.val.push_back(row[k].second);
+        }
+        S.ptr[i + 1] = (int)S.col.size();
+    }
+    return S;
+}
+
+void spmv(const Csr& A, const double* x, double* y) {
+    for (int i = 0; i < A.n; ++i) {
+        double s = 0.0;
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) s += A.val[k] * x[A.col[k]];
+        y[i] = s;
+    }
+}
+
+struct Ilu0 {
+    const Csr* A; vector<double> lu; vector<int> diag;
+    void factor(const Csr& M) {
+        A = &M; lu = M.val; diag.assign(M.n, -1);
+        for (int i = 0; i < M.n; ++i)
+            for (int k = M.ptr[i]; k < M.ptr[i + 1]; ++k) if (M.col[k] == i) diag[i] = k;
+        vector<int> pos(M.n, -1);
+        for (int i = 0; i < M.n; ++i) {
+            for (int k = M.ptr[i]; k < M.ptr[i + 1]; ++k) pos[M.col[k]] = k;
+            for (int k = M.ptr[i]; k < M.ptr[i + 1]; ++k) {
+                int j = M.col[k];
+                if (j >= i) break;
+                lu[k] /= lu[diag[j]];
+                for (int kk = diag[j] + 1; kk < M.ptr[j + 1]; ++kk) {
+                    int p = pos[M.col[kk]];
+                    if (p >= 0) lu[p] -= lu[k] * lu[kk];
+                }
+            }
+            for (int k = M.ptr[i]; k < M.ptr[i + 1]; ++k) pos[M.col[k]] = -1;
+        }
+    }
+    void apply(const double* r, double* z) const {
+        const Csr& M = *A;
+        for (int i = 0; i < M.n; ++i) {
+            double s = r[i];
+            for (int k = M.ptr[i]; k < diag[i]; ++k) s -= lu[k] * z[M.col[k]];
+            z[i] = s;
+        }
+        for (int i = M.n - 1; i >= 0; --i) {
+            double s = z[i];
+            for (int k = diag[i] + 1; k < M.ptr[i + 1]; ++k) s -= lu[k] * z[M.col[k]];
+            z[i] = s / lu[diag[i]];
+        }
+    }
+};
+
+double dot(const vector<double>& a, const vector<double>& b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+// right-preconditioned BiCGStab, x0 = 0, stop at ||b-Ax|| <= tol ||b||
+int bicgstab(const Csr& A, const Ilu0& M, const vector<double>& b, vector<double>& x,
+             double tol, int maxit, double& relres) {
+    int n = A.n;
+    x.assign(n, 0.0);
+    double nb = sqrt(dot(b, b));
+    if (nb == 0.0) { relres = 0.0; return 0; }
+    vector<double> r = b, rh = b, p(n, 0), v(n, 0), s(n), t(n), ph(n), sh(n);
+    double rho_old = 1, al = 1, om = 1;
+    int it;
+    for (it = 1; it <= maxit; ++it) {
+        double rho_n = dot(rh, r);
+        if (rho_n == 0.0) break;
+        double be = (rho_n / rho_old) * (al / om);
+        for (int i = 0; i < n; ++i) p[i] = r[i] + be * (p[i] - om * v[i]);
+        M.apply(p.data(), ph.data());
+        spmv(A, ph.data(), v.data());
+        al = rho_n / dot(rh, v);
+        for (int i = 0; i < n; ++i) s[i] = r[i] - al * v[i];
+        if (sqrt(dot(s, s)) <= tol * nb) {
+            for (int i = 0; i < n; ++i) x[i] += al * ph[i];
+            r = s; break;
+        }
+        M.apply(s.data(), sh.data());
+        spmv(A, sh.data(), t.data());
+        om = dot(t, s) / dot(t, t);
+        for (int i = 0; i < n; ++i) { x[i] += al * ph[i] + om * sh[i]; r[i] = s[i] - om * t[i]; }
+        rho_old = rho_n;
+        if (sqrt(dot(r, r)) <= tol * nb) break;
+    }
+    // true residual
+    vector<double> ax(n);
+    spmv(A, x.data(), ax.data());
+    for (int i = 0; i < n; ++i) ax[i] = b[i] - ax[i];
+    relres = sqrt(dot(ax, ax)) / nb;
+    return it;
+}
+
+void solve_last(Oracle& o, int nrhs) {
+    Csr A = coo_to_csr(o.nc, o.Arows, o.Acols, o.Avals);
+    Ilu0 M; M.factor(A);
+    vector<double>* B[3] = {&o.B0, &o.B1, &o.B2};
+    vector<double>* X[3] = {&o.X0, &o.X1, &o.X2};
+    for (int k = 0; k < nrhs; ++k) {
+        double rr; o.last_iters = bicgstab(A, M, *B[k], *X[k], o.tol, o.maxit, rr);
+        o.last_res = std::max(k == 0 ? 0.0 : o.last_res, rr);
+    }
+}
+
+// momentum update (eqMomentum.cpp:1029-1119)
+double momentum_update(Oracle& o) {
+    double nd = 0, np = 0;
+    for (int i = 0; i < o.nc; ++i) {
+        o.U[i] += o.urf_u * o.X0[i]; o.V[i] += o.urf_u * o.X1[i]; o.W[i] += o.urf_u * o.X2[i];
+        double* q[3] = {&o.U[i], &o.V[i], &o.W[i]};
+        for (int k = 0; k < 3; ++k) {
+            if (*q[k] <= -o.lim) *q[k] = -o.lim;
+            if (*q[k] >= o.lim) *q[k] = o.lim;
+        }
+        nd += pow(o.X0[i], 2.0); nd += pow(o.X1[i], 2.0); nd += pow(o.X2[i], 2.0);
+        np += pow(o.U[i], 2.0); np += pow(o.V[i], 2.0); np += pow(o.W[i], 2.0);
+    }
+    return sqrt(nd) / sqrt(np);
+}
+
+// p' Green-Gauss gradient x2 and the correction (eqPressure.cpp:686-883)
+double pressure_correct(Oracle& o) {
+    int nc = o.nc;
+    const vector<double>& pp = o.X0;
+    vector<double> gp(3 * (size_t)nc, 0.0), tmp(3 * (size_t)nc);
+    for (int pass = 0; pass < 2; ++pass) {
+        std::fill(tmp.begin(), tmp.end(), 0.0);
+        for (int f = 0; f < o.nif; ++f) {
+            int l = o.owner[f], r = o.nbr[f];
+            double wCL = o.fwC[f], wCR = 1.0 - wCL;
+            double vF = wCL * pp[l] + wCR * pp[r];
+            const double* sk = &o.fskew[3 * f];
+            for (int a = 0; a < 3; ++a) {
+                vF += wCL * gp[3 * l + a] * sk[a];
+                vF += wCR * gp[3 * r + a] * sk[a];
+            }
+            const double* n = &o.fnrm[3 * f];
+            for (int a = 0; a < 3; ++a) tmp[3 * l + a] += vF * n[a] * o.farea[f] / o.cvol[l];
+            for (int a = 0; a < 3; ++a) tmp[3 * r + a] -= vF * n[a] * o.farea[f] / o.cvol[r];
+        }
+        for (const Patch& pb : o.patches)
+            for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
+                int c = o.owner[f];
+                double vF = pp[c];
+                if (pb.p_type == FIXED_VALUE) vF = 0.0;
+                const double* n = &o.fnrm[3 * f];
+                for (int a = 0; a < 3; ++a) tmp[3 * c + a] += vF * n[a] * o.farea[f] / o.cvol[c];
+            }
+        gp = tmp;
+    }
+    double nd = 0, np = 0;
+    for (int i = 0; i < nc; ++i) {
+        o.P[i] += o.urf_p * pp[i];
+        double t1 = o.dt / o.Rho[i];
+        double rU = o.urf_pu * t1 * gp[3 * i], rV = o.urf_pu * t1 * gp[3 * i + 1],
+               rW = o.urf_pu * t1 * gp[3 * i + 2];
+        o.U[i] -= rU; o.V[i] -= rV; o.W[i] -= rW;
+        double* q[4] = {&o.P[i], &o.U[i], &o.V[i], &o.W[i]};
+        for (int k = 0; k < 4; ++k) {
+            if (*q[k] <= -o.lim) *q[k] = -o.lim;
+            if (*q[k] >= o.lim) *q[k] = o.lim;
+        }
+        nd += pow(pp[i], 2.0); nd += pow(rU, 2.0); nd += pow(rV, 2.0); nd += pow(rW, 2.0);
+        np += pow(o.P[i], 2.0); np += pow(o.U[i], 2.0); np += pow(o.V[i], 2.0); np += pow(o.W[i], 2.0);
+    }
+    return sqrt(nd) / sqrt(np);
+}
+
+// one call of incompressiblePressureBased with nSpeciesEq = 0
+// (pressureBased.cpp:6-413); sigma must be 0 (no curvature in this oracle).
+void pressure_based_step(Oracle& o, int ncorr, double* norms) {
+    if (o.iterReal == 0) cell_eos_transport(o);
+    if (o.iterReal == 0) { o.Qm0 = o.U; o.Qm1 = o.V; o.Qm2 = o.W; o.Qm3 = o.VF; }
+    else { o.Qm0 = o.oldU; o.Qm1 = o.oldV; o.Qm2 = o.oldW; o.Qm3 = o.oldVF; }
+    o.oldP = o.P; o.oldU = o.U; o.oldV = o.V; o.oldW = o.W; o.oldVF = o.VF; o.oldRho = o.Rho;
+    recon_faces(o);
+    minmax(o, o.VF, o.maxVF, o.minVF);
+    source_gravity(o);
+    for (int k = 0; k < 4; ++k) o.srcS[k].assign(o.nc, 0.0);
+    o.kappa.assign(o.nc, 0.0);
+    for (int it = 0; it < ncorr; ++it) {
+        recon_faces(o);
+        assemble_momentum(o);
+        solve_last(o, 3);
+        double n0 = momentum_update(o);
+        recon_faces(o);
+        assemble_pressure(o);
+        solve_last(o, 1);
+        double n1 = pressure_correct(o);
+        if (norms) { norms[2 * it] = n0; norms[2 * it + 1] = n1; }
+    }
+    o.iterReal++;
+}
+
+// ------------------------------------------------------------------------
+// GENRCM (George & Liu, SPARSPAK) as called by lib_RCM (src/math/RCM.cpp:34-210,
+// 1075-1327, 2541-2676, 2839-2876, 3353-3777).  0-based arrays internally;
+// the traversal order (component roots, pseudo-peripheral search, degree-
+// ascending stable insertion of each new level segment, final reversal) is the
+// published algorithm's.
+struct Graph { int n; const int* ptr; const int* adj; };
+
+void level_set(const Graph& G, int root, vector<int>& mask, int& nlev, vector<int>& lptr,
+               vector<int>& lev) {
+    lev.clear(); lptr.clear();
+    mask[root] = 0; lev.push_back(root);
+    size_t lbeg = 0, lend = 1;
+    while (true) {
+        lptr.push_back((int)lbeg);
+        for (size_t i = lbeg; i < lend; ++i) {
+            int node = lev[i];
+            for (int k = G.ptr[node]; k < G.ptr[node + 1]; ++k) {
+                int nb = G.adj[k];
+                if (mask[nb] != 0) { lev.push_back(nb); mask[nb] = 0; }
+            }
+        }
+        lbeg = lend; lend = lev.size();
+        if (lend <= lbeg) break;
+    }
+    nlev = (int)lptr.size();
+    lptr.push_back((int)lev.size());
+    for (int v : lev) mask[v] = 1;
+}
+
+int root_find(const Graph& G, int root, vector<int>& mask) {
+    int nlev; vector<int> lptr, lev;
+    level_set(G, root, mask, nlev, lptr, lev);
+    int cc = lptr[nlev];
+    if (nlev == 1 || nlev == cc) return root;
+    while (true) {
+        int mindeg = cc;
+        int js = lptr[nlev - 1];
+        root = lev[js];
+        if (js < cc - 1) {
+            for (int j = js; j < cc; ++j) {
+                int node = lev[j], nd = 0;
+                for (int k = G.ptr[node]; k < G.ptr[node + 1]; ++k) if (mask[G.adj[k]] > 0) ++nd;
+                if (nd < mindeg) { root = node; mindeg = nd; }
+            }
+        }
+        int nlev2; vector<int> lptr2, lev2;
+        level_set(G, root, mask, nlev2, lptr2, lev2);
+        if (nlev2 <= nlev) break;
+        nlev = nlev2; lptr = lptr2; lev = lev2;
+        if (cc <= nlev) break;
+    }
+    return root;
+}
+
+int rcm_component(const Graph& G, int root, vector<int>& mask, int* perm, vector<int>& deg) {
+    // degree(): BFS over the component, deg = number of unmasked neighbours
+    vector<int> comp; comp.push_back(root);
+    vector<char> seen(0);
+    {
+        std::vector<char>& vis = seen; vis.assign(G.n, 0); vis[root] = 1;
+        for (size_t i = 0; i < comp.size(); ++i) {
+            int node = comp[i], d = 0;
+            for (int k = G.ptr[node]; k < G.ptr[node + 1]; ++k) {
+                int nb = G.adj[k];
+                if (mask[nb] != 0) { ++d; if (!vis[nb]) { vis[nb] = 1; comp.push_back(nb); } }
+            }
+            deg[node] = d;
+        }
+    }
+    int cc = (int)comp.size();
+    mask[root] = 0;
+    perm[0] = root;
+    if (cc == 1) return 1;
+    int lvlend = 0, lnbr = 1;
+    while (lvlend < lnbr) {
+        int lbeg = lvlend; lvlend = lnbr;
+        for (int i = lbeg; i < lvlend; ++i) {
+            int node = perm[i];
+            int fnbr = lnbr;
+            for (int k = G.ptr[node]; k < G.ptr[node + 1]; ++k) {
+                int nb = G.adj[k];
+                if (mask[nb] != 0) { mask[nb] = 0; perm[lnbr++] = nb; }
+            }
+            if (lnbr - fnbr <= 1) continue;
+            for (int k = fnbr + 1; k < lnbr; ++k) {      // stable insertion sort by degree
+                int nb = perm[k], l = k - 1;
+                while (l >= fnbr && deg[perm[l]] > deg[nb]) { perm[l + 1] = perm[l]; --l; }
+                perm[l + 1] = nb;
+            }
+        }
+    }
+    std::reverse(perm, perm + cc);
+    return cc;
+}
+
+}  // namespace
+
+// ==========================================================================
+// C interface (ctypes)
+extern "C" {
+
+void* orc_create(int np, int nc, int nf, int nif, const double* pts, const int* owner,
+                 const int* nbr, const int* fptr, const int* fpts, const int* cptr,
+                 const int* cpts, int npatch, const int* pstart, const int* pnf) {
+    Oracle* o = new Oracle;
+    o->np = np; o->nc = nc; o->nf = nf; o->nif = nif;
+    o->pts.assign(pts, pts + 3 * (size_t)np);
+    o->owner.assign(owner, owner + nf); o->nbr.assign(nbr, nbr + nif);
+    o->fptr.assign(fptr, fptr + nf + 1); o->fpts.assign(fpts, fpts + fptr[nf]);
+    o->cptr.assign(cptr, cptr + nc + 1); o->cpts.assign(cpts, cpts + cptr[nc]);
+    for (int b = 0; b < npatch; ++b) {
+        Patch p{}; p.start = pstart[b]; p.nfaces = pnf[b];
+        p.p_type = ZERO_GRADIENT; p.u_type = NO_SLIP; p.vf_type = ZERO_GRADIENT;
+        o->patches.push_back(p);
+    }
+    build_geometry(*o);
+    auto z = [&](vector<double>& v, double x = 0.0) { v.assign(nc, x); };
+    z(o->P); z(o->U); z(o->V); z(o->W); z(o->VF, 1.0); z(o->MF, 1.0); z(o->Rho); z(o->Mu);
+    z(o->Qm0); z(o->Qm1); z(o->Qm2); z(o->Qm3, 1.0);
+    z(o->oldP); z(o->oldU); z(o->oldV); z(o->oldW); z(o->oldVF, 1.0); z(o->oldRho);
+    for (int k = 0; k < 4; ++k) { z(o->srcG[k]); z(o->srcS[k]); }
+    z(o->kappa);
+    return o;
+}
+
+void orc_destroy(void* h) { delete (Oracle*)h; }
+
+void orc_set_bc(void* h, int b, int p_type, double p_val, int u_type, const double* u_val,
+                int vf_type, double vf_val) {
+    Oracle& o = *(Oracle*)h;
+    Patch& p = o.patches[b];
+    p.p_type = p_type; p.p_val = p_val; p.u_type = u_type;
+    for (int a = 0; a < 3; ++a) p.u_val[a] = u_val[a];
+    p.vf_type = vf_type; p.vf_val = vf_val;
+}
+
+void orc_set_case(void* h, const double* rho, const double* mu, double sigma, const double* g,
+                  double dt, double urf_u, double urf_p, double urf_pu, double lim) {
+    Oracle& o = *(Oracle*)h;
+    o.rho[0] = rho[0]; o.rho[1] = rho[1]; o.mu[0] = mu[0]; o.mu[1] = mu[1];
+    o.sigma = sigma; for (int a = 0; a < 3; ++a) o.g[a] = g[a];
+    o.dt = o.oldDt = o.old2Dt = dt;
+    o.urf_u = urf_u; o.urf_p = urf_p; o.urf_pu = urf_pu; o.lim = lim;
+}
+
+void orc_set_solver(void* h, double tol, int maxit) {
+    Oracle& o = *(Oracle*)h; o.tol = tol; o.maxit = maxit;
+}
+
+static vector<double>* field(Oracle& o, const char* nm) {
+    std::string s(nm);
+#define F(name, var) if (s == name) return &o.var;
+    F("P", P) F("U", U) F("V", V) F("W", W) F("VF", VF) F("Rho", Rho) F("mu", Mu)
+    F("oldU", oldU) F("oldV", oldV) F("oldW", oldW) F("Qm0", Qm0) F("Qm1", Qm1) F("Qm2", Qm2)
+    F("gradP", gP) F("gradU", gU) F("gradV", gV) F("gradW", gW) F("gradVF", gVF)
+    F("maxP", maxP) F("minP", minP) F("maxU", maxU) F("minU", minU) F("maxV", maxV) F("minV", minV)
+    F("maxW", maxW) F("minW", minW)
+    F("srcG0", srcG[0]) F("srcG1", srcG[1]) F("srcG2", srcG[2]) F("srcG3", srcG[3])
+    F("face_area", farea) F("face_normal", fnrm) F("face_xyz", fxyz) F("face_wC", fwC)
+    F("face_magPN", fmagPN) F("face_alphaF", falpha) F("face_ePN", fePN) F("face_skew", fskew)
+    F("face_dist", fdist) F("cell_volume", cvol) F("cell_xyz", cxyz) F("cell_lsq", lsq)
+    F("b0", B0) F("b1", B1) F("b2", B2) F("x0", X0) F("x1", X1) F("x2", X2) F("vals", Avals)
+    F("faceRP", RP) F("faceRU", RU) F("faceRV", RV) F("faceRW", RW) F("faceRVF", RVF)
+    F("faceLU", LU) F("faceLV", LV) F("faceLW", LW) F("faceLRho", LRho)
+#undef F
+    return nullptr;
+}
+
+// size of a double array, -1 if unknown
+long orc_size(void* h, const char* nm) {
+    Oracle& o = *(Oracle*)h;
+    std::string s(nm);
+    if (s == "stencil_ptr") return o.sptr.size();
+    if (s == "stencil_idx") return o.sidx.size();
+    if (s == "rows" || s == "cols") return o.Arows.size();
+    vector<double>* v = field(o, nm);
+    return v ? (long)v->size() : -1;
+}
+
+int orc_get(void* h, const char* nm, void* out) {
+    Oracle& o = *(Oracle*)h;
+    std::string s(nm);
+    if (s == "stencil_ptr") { memcpy(out, o.sptr.data(), o.sptr.size() * 4); return 0; }
+    if (s == "stencil_idx") { memcpy(out, o.sidx.data(), o.sidx.size() * 4); return 0; }
+    if (s == "rows") { memcpy(out, o.Arows.data(), o.Arows.size() * 4); return 0; }
+    if (s == "cols") { memcpy(out, o.Acols.data(), o.Acols.size() * 4); return 0; }
+    vector<double>* v = field(o, nm);
+    if (!v) return -1;
+    memcpy(out, v->data(), v->size() * 8);
+    return 0;
+}
+
+int orc_set(void* h, const char* nm, const double* in) {
+    Oracle& o = *(Oracle*)h;
+    vector<double>* v = field(o, nm);
+    if (!v) return -1;
+    memcpy(v->data(), in, v->size() * 8);
+    return 0;
+}
+
+// stage entry points
+void orc_eos(void* h) { cell_eos_transport(*(Oracle*)h); }
+void orc_recon(void* h) { recon_faces(*(Oracle*)h); }
+void orc_gravity(void* h) { source_gravity(*(Oracle*)h); }
+void orc_lsq(void* h, const double* phi, const double* faceR, double* grad) {
+    Oracle& o = *(Oracle*)h;
+    vector<double> p(phi, phi + o.nc), fr(faceR, faceR + o.nf), gr;
+    lsq_grad(o, p, fr, gr);
+    memcpy(grad, gr.data(), gr.size() * 8);
+}
+void orc_minmax(void* h, const double* phi, double* mx, double* mn) {
+    Oracle& o = *(Oracle*)h;
+    vector<double> p(phi, phi + o.nc), a, b;
+    minmax(o, p, a, b);
+    memcpy(mx, a.data(), a.size() * 8); memcpy(mn, b.data(), b.size() * 8);
+}
+void orc_assemble_momentum(void* h) { assemble_momentum(*(Oracle*)h); }
+void orc_assemble_pressure(void* h) { assemble_pressure(*(Oracle*)h); }
+int orc_solve(void* h, int nrhs, double* relres) {
+    Oracle& o = *(Oracle*)h; solve_last(o, nrhs); if (relres) *relres = o.last_res; return o.last_iters;
+}
+double orc_momentum_update(void* h) { return momentum_update(*(Oracle*)h); }
+double orc_pressure_correct(void* h) { return pressure_correct(*(Oracle*)h); }
+void orc_step(void* h, int ncorr, double* norms) { pressure_based_step(*(Oracle*)h, ncorr, norms); }
+
+// RCM of an arbitrary graph: adjacency CSR (0-based), perm[new] = old,
+// inv[old] = new (lib_RCM's cell.RCM / cell.invRCM).
+void orc_genrcm(int n, const int* ptr, const int* adj, int* perm, int* inv) {
+    Graph G{n, ptr, adj};
+    vector<int> mask(n, 1), deg(n, 0);
+    int num = 0;
+    for (int i = 0; i < n; ++i) {
+        if (mask[i] == 0) continue;
+        int root = root_find(G, i, mask);
+        num += rcm_component(G, root, mask, perm + num, deg);
+        if (num >= n) break;
+    }
+    for (int i = 0; i < n; ++i) inv[perm[i]] = i;
+}
+
+// lib_RCM's adjacency (RCM.cpp:79-126): for each cell, its internal faces in
+// ascending face order, the other cell.
+void orc_cell_graph(void* h, int* ptr, int* adj) {
+    Oracle& o = *(Oracle*)h;
+    ptr[0] = 0; int m = 0;
+    for (int i = 0; i < o.nc; ++i) {
+        for (int k = o.cfptr[i]; k < o.cfptr[i + 1]; ++k) {
+            int f = o.cfidx[k];
+            if (f >= o.nif) continue;
+            adj[m++] = o.owner[f] == i ? o.nbr[f] : o.owner[f];
+        }
+        ptr[i + 1] = m;
+    }
+}
+
+}  // extern "C"

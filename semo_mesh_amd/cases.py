@@ -1,0 +1,76 @@
+"""Case descriptions for the pressure-based incompressible path.
+
+A case is the physics and boundary conditions that SEMO reads from
+``constant/*`` and ``system/*`` dictionaries (src/controls/build.cpp:7-82,
+87-623; src/load/loadFiles.cpp:12-190): two constant-density phases
+(index 0 = the phase tracked by the volume fraction), gravity, surface
+tension, a fixed time step, under-relaxation factors and per-patch boundary
+conditions.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+# boundary-condition type codes shared with the C ABI (include/semo_gpu.h)
+BC_ZERO_GRADIENT = 0
+BC_FIXED_VALUE = 1
+BC_NO_SLIP = 2
+BC_SLIP = 3
+BC_SURFACE_NORMAL_FIXED_VALUE = 4
+BC_INLET_OUTLET = 5
+BC_NAMES = {BC_ZERO_GRADIENT: "zeroGradient", BC_FIXED_VALUE: "fixedValue",
+            BC_NO_SLIP: "noSlip", BC_SLIP: "slip",
+            BC_SURFACE_NORMAL_FIXED_VALUE: "surfaceNormalFixedValue",
+            BC_INLET_OUTLET: "inletOutlet"}
+
+
+@dataclass
+class PatchBC:
+    p_type: int = BC_ZERO_GRADIENT
+    p_value: float = 0.0
+    u_type: int = BC_NO_SLIP
+    u_value: tuple = (0.0, 0.0, 0.0)
+    vf_type: int = BC_ZERO_GRADIENT
+    vf_value: float = 0.0
+
+
+@dataclass
+class Case:
+    rho: tuple = (1000.0, 1000.0)
+    mu: tuple = (1e-3, 1e-3)
+    sigma: float = 0.0
+    g: tuple = (0.0, 0.0, 0.0)
+    dt: float = 0.01
+    urf_mom: float = 0.3      # momVelURF  (eqMomentum.cpp:1040)
+    urf_p: float = 0.3        # prePreURF  (eqPressure.cpp:827)
+    urf_pu: float = 0.3       # preVelURF  (eqPressure.cpp:832)
+    limit: float = 1e10       # |P|,|U|,|V|,|W| clip bounds (fvSolution limiter)
+    bcs: dict = field(default_factory=dict)   # patch name -> PatchBC
+
+    def write_ref_case(self, path: str, steps: int = 1, correctors: int = 1,
+                       init: str | None = None) -> None:
+        with open(path, "w") as fh:
+            fh.write(f"dt {self.dt!r}\nrho {self.rho[0]!r} {self.rho[1]!r}\n")
+            fh.write(f"mu {self.mu[0]!r} {self.mu[1]!r}\nsigma {self.sigma!r}\n")
+            fh.write("g %r %r %r\n" % tuple(self.g))
+            fh.write(f"urf {self.urf_mom!r} {self.urf_p!r} {self.urf_pu!r}\n")
+            fh.write(f"limit {self.limit!r}\nsteps {steps}\ncorrectors {correctors}\n")
+            if init:
+                fh.write(f"init {init}\n")
+            for name, b in self.bcs.items():
+                fh.write("bc %s %s %r %s %r %r %r %s %r\n" % (
+                    name, BC_NAMES[b.p_type], b.p_value, BC_NAMES[b.u_type],
+                    b.u_value[0], b.u_value[1], b.u_value[2],
+                    BC_NAMES[b.vf_type], b.vf_value))
+
+
+def cavity_case(dt: float = 0.01, lid=(1.0, 0.0, 0.0)) -> Case:
+    """Lid-driven cavity: lid (zmax) fixedValue U, other walls noSlip,
+    pressure zeroGradient everywhere except the one-face ``pref`` patch
+    (fixedValue p = 0).  Single phase run through the two-phase path:
+    alpha = 1, both phases rho = 1000, mu = 1e-3, sigma = 0, g = 0."""
+    wall = PatchBC()
+    bcs = {n: wall for n in ("xmin", "xmax", "ymin", "ymax", "zmin")}
+    bcs["pref"] = PatchBC(p_type=BC_FIXED_VALUE, p_value=0.0)
+    bcs["zmax"] = PatchBC(u_type=BC_FIXED_VALUE, u_value=tuple(lid))
+    return Case(dt=dt, bcs=bcs)

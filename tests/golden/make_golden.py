@@ -1,0 +1,99 @@
+"""Writes the golden vectors under tests/golden/ by running the compiled
+reference (oracle/_ref/semo_ref, built from the reference checkout by
+oracle/ref.mk) on small seeded meshes.
+
+    make -f oracle/ref.mk REF=/root/reference -j8
+    python tests/golden/make_golden.py
+
+Each case stores the mesh arrays it ran on, the reference's stage outputs and
+the fields after the run.  The reference itself never travels: only these
+arrays are committed.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from semo_mesh_amd.cases import cavity_case  # noqa: E402
+from semo_mesh_amd.mesh import cavity_mesh, hex_mesh  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
+
+
+def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
+    with tempfile.TemporaryDirectory() as d:
+        mesh.write_raw(os.path.join(d, "mesh.raw"))
+        ip = None
+        if init is not None:
+            ip = os.path.join(d, "init.f64")
+            np.ascontiguousarray(init, dtype=np.float64).tofile(ip)
+        case.write_ref_case(os.path.join(d, "case.txt"), steps=steps,
+                            correctors=correctors, init=ip)
+        out = os.path.join(d, "out")
+        r = subprocess.run([REF, os.path.join(d, "mesh.raw"), os.path.join(d, "case.txt"), out],
+                           capture_output=True, text=True, timeout=600)
+        if "SEMO_REF_DONE" not in r.stdout:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        arrs = {}
+        for fn in sorted(os.listdir(out)):
+            base, ext = os.path.splitext(fn)
+            if ext not in (".f64", ".i32"):
+                continue
+            if base.startswith("call") and not keep_calls:
+                continue
+            if base.endswith(("_Rho", "_mu", "_VF", "_gradVF", "_kappa")) or "_srcS" in base:
+                continue
+            arrs[base] = np.fromfile(os.path.join(out, fn),
+                                     dtype=np.float64 if ext == ".f64" else np.int32)
+    arrs.update(dict(
+        mesh_points=mesh.points, mesh_owner=mesh.owner, mesh_neighbour=mesh.neighbour,
+        mesh_face_ptr=mesh.face_ptr, mesh_face_pts=mesh.face_pts, mesh_cell_ptr=mesh.cell_ptr,
+        mesh_cell_pts=mesh.cell_pts, mesh_patch_start=mesh.patch_start,
+        mesh_patch_nfaces=mesh.patch_nfaces, mesh_patch_names=np.array(mesh.patch_names),
+        case_mu=np.array(case.mu), case_rho=np.array(case.rho), case_dt=np.array(case.dt),
+        case_lid=np.array(case.bcs["zmax"].u_value), steps=np.array(steps),
+        correctors=np.array(correctors)))
+    if init is not None:
+        arrs["init"] = np.asarray(init)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrs)
+    print(f"{name}: {len(arrs)} arrays, {os.path.getsize(path) / 1024:.0f} KiB")
+
+
+def random_init(nc, seed):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.normal(size=nc) * 10.0, rng.normal(size=nc),
+                           rng.normal(size=nc), rng.normal(size=nc), np.ones(nc)])
+
+
+def main():
+    if not os.path.exists(REF):
+        raise SystemExit("build the reference first: make -f oracle/ref.mk REF=/root/reference")
+    # 1. jittered 4^3 cavity, random fields, one SIMPLE iteration: every stage
+    m = cavity_mesh(4, jitter=0.1)
+    c = cavity_case(); c.mu = (0.5, 0.5)
+    run_case("cavity4_jitter_random", m, c, 1, 1, random_init(m.n_cells, 1))
+    # 2. non-cubic jittered box, 2 time steps x 2 correctors
+    m = hex_mesh(6, 5, 4, jitter=0.1, seed=7)
+    c = cavity_case(); c.mu = (0.05, 0.05); c.dt = 0.02
+    run_case("box654_jitter_2x2", m, c, 2, 2, random_init(m.n_cells, 2))
+    # 3. plain cavity physics (rho 1000, mu 1e-3, dt 0.01) on 8^3, 1 iteration
+    m = cavity_mesh(8)
+    run_case("cavity8_plain", m, cavity_case(), 1, 1, None)
+    # 4. lid columns inside the live orifice profile window (reconIncom.cpp:707-727):
+    #    x-face centres (i+0.5)/64 hit 0.0006*13 +- 1.6e-4 at i = 0
+    m = hex_mesh(8, 4, 4, hi=(0.125, 0.0625, 0.0625), jitter=0.1, seed=11)
+    c = cavity_case(); c.mu = (0.01, 0.01); c.dt = 0.001
+    run_case("orifice_lid", m, c, 1, 1, random_init(m.n_cells, 3))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+"""The CPU oracle against the golden vectors written by the compiled
+reference (tests/golden/make_golden.py).  Geometry, stencil, least-squares
+coefficients, RCM permutation, gradients, limiter bounds, assembled COO
+systems and the corrected fields are compared bit for bit; linear-solve
+results to the solver tolerances (AMGCL stops at 1e-9 relative)."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, golden_case, load_golden
+
+GEOM = ["face_area", "face_normal", "face_xyz", "face_wC", "face_magPN", "face_alphaF",
+        "face_ePN", "face_skew", "face_dist", "cell_volume", "cell_xyz", "cell_lsq"]
+
+
+def calls(d):
+    out = []
+    k = 0
+    while True:
+        m = [key for key in d if key.startswith(f"call{k}_") and key.endswith("_rows")]
+        if not m:
+            return out
+        out.append(m[0][: -len("_rows")])
+        k += 1
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_mesh_generator_reproduces_fixture_mesh(name):
+    """The committed mesh arrays are what the seeded generator makes now."""
+    from semo_mesh_amd.mesh import cavity_mesh, hex_mesh
+    d, mesh = load_golden(name)
+    gen = {"cavity4_jitter_random": lambda: cavity_mesh(4, jitter=0.1),
+           "box654_jitter_2x2": lambda: hex_mesh(6, 5, 4, jitter=0.1, seed=7),
+           "cavity8_plain": lambda: cavity_mesh(8),
+           "orifice_lid": lambda: hex_mesh(8, 4, 4, hi=(0.125, 0.0625, 0.0625), jitter=0.1, seed=11)}[name]()
+    for a in ("points", "owner", "neighbour", "face_ptr", "face_pts", "cell_ptr", "cell_pts",
+              "patch_start", "patch_nfaces"):
+        assert np.array_equal(getattr(gen, a), d["mesh_" + a]), a
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_geometry_stencil_lsq_rcm_bit_exact(name, oracle_lib):
+    d, mesh = load_golden(name)
+    o = oracle_lib.Oracle(mesh, golden_case(d))
+    for k in GEOM:
+        assert np.array_equal(o.get(k), d[k]), k
+    assert np.array_equal(o.get("stencil_ptr"), d["stencil_ptr"])
+    assert np.array_equal(o.get("stencil_idx"), d["stencil_idx"])
+    ptr, adj = o.cell_graph()
+    perm, inv = oracle_lib.genrcm(ptr, adj)
+    assert np.array_equal(perm, d["rcm_perm"])
+    assert np.array_equal(inv, d["rcm_inv"])
+
+
+def _set_state(o, d, p):
+    for f in ("P", "U", "V", "W", "oldU", "oldV", "oldW", "Qm0", "Qm1", "Qm2"):
+        o.set(f, d[p + f])
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_assembly_replay_bit_exact(name, oracle_lib):
+    """Replays every solveAMGCL call: with the reference's fields at call time
+    the oracle assembles the identical COO system and right-hand sides, and
+    with the reference's solution it produces the identical updated fields."""
+    d, mesh = load_golden(name)
+    o = oracle_lib.Oracle(mesh, golden_case(d))
+    cl = calls(d)
+    assert len(cl) == 2 * int(d["steps"]) * int(d["correctors"])
+    o.eos()
+    for i, p in enumerate(cl):
+        p = p + "_"
+        _set_state(o, d, p)
+        o.recon()
+        o.gravity()
+        if "momentum" in p:
+            o.assemble_momentum()
+            for f in ("gradP", "gradU", "gradV", "gradW", "maxU", "minU", "maxP", "minP"):
+                assert np.array_equal(o.get(f), d[p + f]), p + f
+            nb = 3
+        else:
+            o.assemble_pressure()
+            for f in ("gradP", "gradU", "gradV", "gradW"):
+                assert np.array_equal(o.get(f), d[p + f]), p + f
+            nb = 1
+        r, c, v = o.coo()
+        assert np.array_equal(r, d[p + "rows"]) and np.array_equal(c, d[p + "cols"])
+        assert np.array_equal(v, d[p + "vals"]), p + "vals"
+        for k in range(nb):
+            assert np.array_equal(o.get(f"b{k}"), d[p + f"b{k}"]), p + f"b{k}"
+        # the oracle's own solve meets the reference's solution to AMGCL's tolerance
+        it, res = o.solve(nb)
+        assert res < 1e-12
+        for k in range(nb):
+            x = o.get(f"x{k}"); xr = d[p + f"x{k}"]
+            assert np.linalg.norm(x - xr) <= 1e-7 * max(np.linalg.norm(xr), 1e-300), p
+            o.set(f"x{k}", xr)
+        # update with the reference's solution; compare with the next call's fields
+        if nb == 3:
+            o.momentum_update()
+        else:
+            o.pressure_correct()
+        nxt = cl[i + 1] + "_" if i + 1 < len(cl) else "final_"
+        if nb == 3 or nxt == "final_":
+            for f in ("P", "U", "V", "W"):
+                assert np.array_equal(o.get(f), d[nxt + f]), nxt + f
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_full_simple_run_matches_reference(name, oracle_lib):
+    """The oracle's own run (its own linear solves) ends at the reference's
+    fields to within the reference solver's relative tolerance."""
+    d, mesh = load_golden(name)
+    o = oracle_lib.Oracle(mesh, golden_case(d))
+    nc = mesh.n_cells
+    if "init" in d:
+        ini = d["init"]
+        for k, f in enumerate(("P", "U", "V", "W", "VF")):
+            o.set(f, ini[k * nc:(k + 1) * nc])
+    for _ in range(int(d["steps"])):
+        o.step(int(d["correctors"]))
+    # velocity components relative to the velocity field's norm (a component
+    # can be pure round-off by symmetry), pressure relative to its own norm
+    vel = np.sqrt(sum(np.linalg.norm(d["final_" + f]) ** 2 for f in "UVW"))
+    for f in ("P", "U", "V", "W"):
+        a, b = o.get(f), d["final_" + f]
+        scale = np.linalg.norm(b) if f == "P" else vel
+        assert np.linalg.norm(a - b) <= 1e-7 * scale, f

@@ -1,0 +1,126 @@
+"""Closed-form and structural checks on the CPU oracle, independent of the
+reference fixtures: a dropped term, a wrong sign or a transposed index in the
+restatement breaks one of these."""
+import numpy as np
+import pytest
+
+from semo_mesh_amd.cases import cavity_case
+from semo_mesh_amd.mesh import cavity_mesh, hex_mesh
+
+
+def test_uniform_hex_geometry_closed_form(oracle_lib):
+    n = 5
+    h = 1.0 / n
+    m = cavity_mesh(n)
+    o = oracle_lib.Oracle(m, cavity_case())
+    assert np.allclose(o.get("face_area"), h * h, rtol=1e-13)
+    assert np.allclose(o.get("cell_volume"), h ** 3, rtol=1e-12)
+    cells = m.cell_pts.reshape(-1, 8)
+    centre = m.points[cells].mean(axis=1)
+    assert np.allclose(o.get("cell_xyz").reshape(-1, 3), centre, atol=1e-14)
+    assert np.all(o.get("face_wC") == 0.5)
+    assert np.allclose(o.get("face_alphaF"), 1.0, rtol=1e-13)
+    assert np.allclose(o.get("face_skew"), 0.0, atol=1e-15)
+    assert np.allclose(o.get("face_magPN"), h, rtol=1e-12)      # boundary: 2 * h/2
+    nrm = o.get("face_normal").reshape(-1, 3)
+    nif = m.n_internal_faces
+    d = o.get("cell_xyz").reshape(-1, 3)
+    e = d[m.neighbour] - d[m.owner[:nif]]
+    assert np.all(np.einsum("ij,ij->i", nrm[:nif], e) > 0)         # owner -> neighbour
+
+
+def test_jittered_volumes_sum_to_domain(oracle_lib):
+    m = hex_mesh(6, 5, 4, jitter=0.1, seed=3)
+    o = oracle_lib.Oracle(m, cavity_case())
+    assert abs(o.get("cell_volume").sum() - 1.0) < 1e-13
+    # divergence theorem: sum over a closed cell of n*A is zero
+    nrm = o.get("face_normal").reshape(-1, 3) * o.get("face_area")[:, None]
+    acc = np.zeros((m.n_cells, 3))
+    np.add.at(acc, m.owner, nrm)
+    np.add.at(acc, m.neighbour, -nrm[: m.n_internal_faces])
+    assert np.abs(acc).max() < 1e-14
+
+
+def test_lsq_gradient_exact_on_linear_field(oracle_lib):
+    m = hex_mesh(7, 6, 5, jitter=0.1, seed=5)
+    o = oracle_lib.Oracle(m, cavity_case())
+    a = np.array([0.3, -1.7, 2.2])
+    c = o.get("cell_xyz").reshape(-1, 3)
+    f = o.get("face_xyz").reshape(-1, 3)
+    phi = 4.0 + c @ a
+    face_r = 4.0 + f @ a
+    g = o.lsq(phi, face_r)
+    assert np.allclose(g, a[None, :], atol=1e-12)
+
+
+def test_minmax_brute_force(oracle_lib):
+    m = hex_mesh(4, 3, 5, jitter=0.1, seed=9)
+    o = oracle_lib.Oracle(m, cavity_case())
+    phi = np.random.default_rng(0).normal(size=m.n_cells)
+    mx, mn = o.minmax(phi)
+    # brute force: cells sharing at least one point
+    cp = m.cell_pts.reshape(-1, 8)
+    for i in range(m.n_cells):
+        nb = [j for j in range(m.n_cells) if np.intersect1d(cp[i], cp[j]).size]
+        assert mx[i] == phi[nb].max() and mn[i] == phi[nb].min()
+
+
+def _bandwidth(ptr, adj, inv):
+    bw = 0
+    for i in range(len(ptr) - 1):
+        for k in range(ptr[i], ptr[i + 1]):
+            bw = max(bw, abs(int(inv[i]) - int(inv[adj[k]])))
+    return bw
+
+
+def test_rcm_path_graph_and_bandwidth(oracle_lib):
+    # a path 0-1-...-n-1 numbered at random: RCM must give bandwidth 1
+    rng = np.random.default_rng(1)
+    n = 40
+    lab = rng.permutation(n)
+    nbrs = [[] for _ in range(n)]
+    for k in range(n - 1):
+        a, b = lab[k], lab[k + 1]
+        nbrs[a].append(b); nbrs[b].append(a)
+    ptr = np.zeros(n + 1, dtype=np.int32)
+    ptr[1:] = np.cumsum([len(x) for x in nbrs])
+    adj = np.array([y for x in nbrs for y in sorted(x)], dtype=np.int32)
+    perm, inv = oracle_lib.genrcm(ptr, adj)
+    assert sorted(perm) == list(range(n))
+    assert np.array_equal(inv[perm], np.arange(n))
+    assert _bandwidth(ptr, adj, inv) == 1
+    # the reversal: the pseudo-peripheral start ends last
+    assert perm[-1] in (lab[0], lab[-1])
+
+
+def test_rcm_reduces_bandwidth_of_shuffled_mesh(oracle_lib):
+    m = hex_mesh(6, 6, 6)
+    o = oracle_lib.Oracle(m, cavity_case())
+    ptr, adj = o.cell_graph()
+    n = m.n_cells
+    sh = np.random.default_rng(2).permutation(n).astype(np.int32)   # new label of old cell
+    inv_sh = np.argsort(sh).astype(np.int32)
+    # relabelled graph
+    ptr2 = np.zeros(n + 1, dtype=np.int32); adj2 = []
+    for new in range(n):
+        old = inv_sh[new]
+        nb = sorted(sh[adj[ptr[old]:ptr[old + 1]]])
+        adj2 += nb; ptr2[new + 1] = len(adj2)
+    adj2 = np.array(adj2, dtype=np.int32)
+    perm, inv = oracle_lib.genrcm(ptr2, adj2)
+    assert _bandwidth(ptr2, adj2, inv) <= 2 * 36 and _bandwidth(ptr2, adj2, np.arange(n)) > 100
+
+
+def test_pressure_matrix_symmetric_laplacian(oracle_lib):
+    m = hex_mesh(5, 4, 6, jitter=0.1, seed=4)
+    o = oracle_lib.Oracle(m, cavity_case())
+    o.eos(); o.recon(); o.assemble_pressure()
+    r, c, v = o.coo()
+    import scipy.sparse as sp
+    A = sp.coo_matrix((v, (r, c)), shape=(m.n_cells,) * 2).tocsr()
+    assert abs(A - A.T).max() == 0.0
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    # zero row sums except at the pref cell (fixedValue p adds to its diagonal)
+    pref = m.owner[m.patch_start[m.patch("pref")]]
+    assert np.all(np.abs(np.delete(rs, pref)) < 1e-12 * np.abs(v).max())
+    assert rs[pref] > 0
