@@ -1,0 +1,201 @@
+/*
+ * semo_gpu.h -- C ABI of the MI355X (gfx950) assemble-and-solve path of the
+ * SEMO unstructured finite-volume solver.
+ *
+ * The library owns one device context per handle.  All arrays crossing the
+ * boundary are host arrays in the caller's (original, un-permuted) cell and
+ * face numbering; the library renumbers cells by reverse Cuthill-McKee on
+ * upload and keeps every field resident in HBM.  Functions return a
+ * semo_status; semo_ctx_last_error() gives the message of the last failure.
+ *
+ * Each entry point names the reference interface it replaces.  Reference
+ * paths are relative to the SEMO_Mesh source tree.
+ *
+ * Threading: no internal threads; calls on one context are not re-entrant,
+ * calls on distinct contexts are independent.  All device work of a context
+ * is issued on the stream given to semo_ctx_create (0 = the library's own).
+ */
+#ifndef SEMO_GPU_H
+#define SEMO_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SEMO_OK = 0,
+    SEMO_EINVAL = 1,          /* bad argument / inconsistent sizes              */
+    SEMO_EINVAL_BC = 2,       /* unknown boundary type (eqMomentum.cpp:861-864) */
+    SEMO_ESINGULAR_LSQ = 3,   /* detA == 0 in the LSQ setup (gradient.cpp:411)  */
+    SEMO_ENEG_VOLUME = 4,     /* non-positive cell volume (geometric.cpp:1152)  */
+    SEMO_EZERO_AREA = 5,      /* zero face area (geometric.cpp:197-202)         */
+    SEMO_EZERO_PIVOT = 6,     /* singular coarse matrix / zero diagonal         */
+    SEMO_ENOCONV = 7,         /* Krylov did not reach the tolerance             */
+    SEMO_EHIP = 8,            /* HIP runtime error                              */
+    SEMO_ESTATE = 9           /* call order (e.g. no mesh uploaded)             */
+} semo_status;
+
+typedef struct semo_ctx semo_ctx;
+
+/* ---- context --------------------------------------------------------------- */
+int semo_ctx_create(int device, void* hip_stream, semo_ctx** out);
+int semo_ctx_destroy(semo_ctx* ctx);
+const char* semo_ctx_last_error(const semo_ctx* ctx);
+/* bytes of device memory the context holds */
+int64_t semo_ctx_device_bytes(const semo_ctx* ctx);
+
+/* ---- mesh ---------------------------------------------------------------------
+ * Face-based polyhedral mesh as SEMO_Mesh_Load::vtu builds it
+ * (src/load/load.cpp:352-438, src/mesh/build.h:42-175): internal faces first,
+ * then boundary faces grouped in patches; face points ordered so the
+ * right-hand normal points owner -> neighbour.                                */
+typedef struct {
+    int32_t n_points, n_cells, n_faces, n_internal_faces, n_patches;
+    const double* points;         /* [n_points*3]                        */
+    const int32_t* owner;         /* [n_faces]                           */
+    const int32_t* neighbour;     /* [n_internal_faces]                  */
+    const int32_t* face_ptr;      /* [n_faces+1]  face -> points CSR     */
+    const int32_t* face_pts;
+    const int32_t* cell_ptr;      /* [n_cells+1]  cell -> points CSR     */
+    const int32_t* cell_pts;      /*   (SEMO cell.points order)          */
+    const int32_t* patch_start;   /* [n_patches]  first face            */
+    const int32_t* patch_nfaces;  /* [n_patches]                         */
+} semo_mesh_desc;
+
+/* Replaces SEMO_Mesh_Geometric::init + setStencil (src/mesh/geometric.cpp:
+ * 792-1768, 1781-1856), SEMO_Utility_Math::initLeastSquare (src/math/
+ * gradient.cpp:11-589) and, when reorder != 0, lib_RCM (src/math/RCM.cpp:
+ * 34-210) applied to storage.                                                */
+int semo_mesh_upload(semo_ctx* ctx, const semo_mesh_desc* mesh, int reorder);
+/* cell permutation: perm[new] = old, inv[old] = new (lib_RCM's cell.RCM /
+ * cell.invRCM, RCM.cpp:199-207); identity without reordering               */
+int semo_mesh_get_perm(const semo_ctx* ctx, int32_t* perm, int32_t* inv);
+/* device face numbering: face_new_to_old[new] = old; flipped[new] = 1 when the
+ * device face's owner is the original neighbour                              */
+int semo_mesh_get_face_map(const semo_ctx* ctx, int32_t* face_new_to_old, int8_t* flipped);
+/* device owner/neighbour arrays (device numbering)                           */
+int semo_mesh_get_connectivity(const semo_ctx* ctx, int32_t* owner, int32_t* neighbour);
+
+/* Standalone GENRCM on a CSR graph (0-based): replaces lib_RCM's genrcm call
+ * (src/math/RCM.cpp:175-183).  perm[new] = old, inv[old] = new.              */
+int semo_rcm(int32_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t* perm, int32_t* inv);
+
+/* ---- case ------------------------------------------------------------------------
+ * Boundary types (SEMO_Boundary::type strings, src/solvers/reconIncom.cpp:
+ * 591-1032): */
+enum { SEMO_BC_ZERO_GRADIENT = 0, SEMO_BC_FIXED_VALUE = 1, SEMO_BC_NO_SLIP = 2,
+       SEMO_BC_SLIP = 3, SEMO_BC_SURFACE_NORMAL_FIXED_VALUE = 4, SEMO_BC_INLET_OUTLET = 5 };
+
+typedef struct {
+    int32_t p_type;  double p_value;
+    int32_t u_type;  double u_value[3];
+    int32_t vf_type; double vf_value;
+} semo_patch_bc;
+
+typedef struct {
+    double rho[2], mu[2];   /* constant-density phases (species[0..1])       */
+    double sigma;           /* surface tension of phase 0                    */
+    double g[3];            /* gravity                                       */
+    double dt;              /* fixed time step                               */
+    double urf_mom;         /* momVelURF  (eqMomentum.cpp:1040)              */
+    double urf_p;           /* prePreURF  (eqPressure.cpp:827)               */
+    double urf_pu;          /* preVelURF  (eqPressure.cpp:832)               */
+    double limit;           /* |P|,|U|,|V|,|W| clip (controls.maxP ... minW) */
+} semo_case;
+
+/* Replaces the controls/species read by SEMO_Controls_Builder
+ * (src/controls/build.cpp:7-623) for this path; bcs has n_patches entries.   */
+int semo_set_case(semo_ctx* ctx, const semo_case* c, const semo_patch_bc* bcs);
+
+/* ---- linear solver configuration --------------------------------------------------
+ * Replaces the hard-coded AMGCL set-ups of solveAMGCL (src/solvers/
+ * solveAMGCL.cpp:938-980 pressure, 1198-1258 momentum).                       */
+enum { SEMO_PREC_FP64 = 0, SEMO_PREC_MIXED = 1 };
+typedef struct {
+    int32_t precision;      /* SEMO_PREC_FP64 or SEMO_PREC_MIXED (FP32 AMG)  */
+    double tol_p;           /* relative residual, pressure                   */
+    double tol_u;           /* relative residual, momentum                   */
+    int32_t maxit_p, maxit_u;
+    int32_t cheb_degree;    /* Chebyshev smoother degree (pre = post)        */
+    int32_t coarse_size;    /* coarsest AMG level size (dense direct solve)  */
+} semo_solver_params;
+int semo_set_solver(semo_ctx* ctx, const semo_solver_params* p);
+
+typedef struct {
+    int32_t iters[4];       /* per right-hand side                           */
+    double rel_res[4];      /* final ||b-Ax||/||b|| in FP64                  */
+    int32_t n_levels;       /* AMG levels (pressure)                         */
+    double setup_ms;        /* hierarchy set-up time of the last build       */
+} semo_solve_stats;
+
+/* ---- fields ----------------------------------------------------------------------- */
+typedef enum {
+    SEMO_F_P = 0, SEMO_F_U, SEMO_F_V, SEMO_F_W, SEMO_F_VF, SEMO_F_RHO, SEMO_F_MU,
+    SEMO_F_OLDU, SEMO_F_OLDV, SEMO_F_OLDW, SEMO_F_QM0, SEMO_F_QM1, SEMO_F_QM2,
+    SEMO_F_SRCG0, SEMO_F_SRCG1, SEMO_F_SRCG2,
+    SEMO_F_MAXP, SEMO_F_MINP, SEMO_F_MAXU, SEMO_F_MINU, SEMO_F_MAXV, SEMO_F_MINV,
+    SEMO_F_MAXW, SEMO_F_MINW,
+    SEMO_F_ALPHA_P,                                         /* Darcy porosity   */
+    SEMO_F_GRADP = 32, SEMO_F_GRADU, SEMO_F_GRADV, SEMO_F_GRADW, SEMO_F_GRADVF, /* [n*3] */
+    SEMO_F_X0 = 48, SEMO_F_X1, SEMO_F_X2,                  /* last solution     */
+    SEMO_F_B0 = 56, SEMO_F_B1, SEMO_F_B2,                  /* last right sides  */
+    /* geometry (read-only): cell arrays, then face arrays (original face order) */
+    SEMO_G_CELL_VOLUME = 64, SEMO_G_CELL_XYZ, SEMO_G_CELL_LSQ,            /* n, 3n, 6n */
+    SEMO_G_FACE_AREA = 80, SEMO_G_FACE_NORMAL, SEMO_G_FACE_XYZ, SEMO_G_FACE_WC,
+    SEMO_G_FACE_MAGPN, SEMO_G_FACE_ALPHAF, SEMO_G_FACE_EPN, SEMO_G_FACE_SKEW
+} semo_field;
+/* host <-> device copies in the caller's numbering; n = element count       */
+int semo_field_set(semo_ctx* ctx, int field, const double* host, int64_t n);
+int semo_field_get(semo_ctx* ctx, int field, double* host, int64_t n);
+
+/* ---- per-equation entry points (seam 1: src/solvers/build.h) ------------------ */
+/* calcMomentumEqs (src/solvers/eqMomentum.cpp:9-1122): assemble, solve the
+ * three right-hand sides, under-relax and clip U,V,W; *norm = |dU|/|U|.      */
+int semo_calc_momentum_eqs(semo_ctx* ctx, double* norm, semo_solve_stats* st);
+/* calcPressureEq (src/solvers/eqPressure.cpp:8-942): assemble, solve p',
+ * two Green-Gauss passes, correct P and U,V,W; *norm as the reference.       */
+int semo_calc_pressure_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
+/* incompressiblePressureBased (src/solvers/pressureBased.cpp:6-483) for
+ * nSpeciesEq = 0, nPressureEq = 1: one time step of n_correctors
+ * momentum + pressure iterations; norms[2*k], norms[2*k+1] per iteration.    */
+int semo_incompressible_pressure_based(semo_ctx* ctx, int n_correctors, double* norms);
+
+/* Assemble without solving (eq 0 = momentum, 1 = pressure), for inspection.  */
+int semo_assemble(semo_ctx* ctx, int eq);
+/* The last assembled system in the caller's numbering as COO.  Call with
+ * rows == NULL to get *nnz.  b gets n*nrhs values (nrhs = 3 momentum, 1 p).   */
+int semo_matrix_get(semo_ctx* ctx, int64_t* nnz, int32_t* rows, int32_t* cols, double* vals,
+                    double* b);
+
+/* ---- math (seam 3: src/math/math.h) ----------------------------------------------- */
+/* calcLeastSquare(mesh,"cellVertex","1st",...) (src/math/gradient.cpp:593-1159):
+ * phi[n] cell values, face_r[n_faces] right-face values (used at boundary
+ * faces), grad[3n].                                                            */
+int semo_least_square(semo_ctx* ctx, const double* phi, const double* face_r, double* grad);
+/* setCellVarMinMax (src/solvers/cellVarMinMax.cpp:8-72)                        */
+int semo_cell_min_max(semo_ctx* ctx, const double* phi, double* vmax, double* vmin);
+
+/* ---- linear solve (seam 2: solveAMGCL, src/solvers/solveAMGCL.cpp:179-1271) ----
+ * COO system in, solution out; equation = "pressure" (AMG-PCG family) or
+ * "momentum" / "volume_fraction" (BiCGStab family).  Duplicates are summed.   */
+int semo_solve_coo(semo_ctx* ctx, const char* equation, int32_t n, int64_t nnz,
+                   const int32_t* rows, const int32_t* cols, const double* vals,
+                   int32_t nrhs, const double* b, double* x, semo_solve_stats* st);
+
+/* ---- profiling ------------------------------------------------------------------------
+ * When enabled, the library brackets every launch of the named kernels with
+ * HIP events on the context stream and accumulates count, device time and the
+ * algorithmic HBM bytes of each launch.                                        */
+int semo_profile_enable(semo_ctx* ctx, int on);
+int semo_profile_reset(semo_ctx* ctx);
+/* i-th profiled kernel: name (<= 63 chars), launches, total ms, total bytes     */
+int semo_profile_entry(semo_ctx* ctx, int i, char* name, int64_t* count, double* ms,
+                       double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEMO_GPU_H */

@@ -1,0 +1,124 @@
+"""ctypes binding of libsemo_gpu.so (include/semo_gpu.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C semo_mesh_amd/csrc``).  There is no fallback: if it is missing,
+loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsemo_gpu.so")
+_LIB = None
+
+# status codes (semo_status)
+SEMO_OK = 0
+STATUS = {0: "SEMO_OK", 1: "SEMO_EINVAL", 2: "SEMO_EINVAL_BC", 3: "SEMO_ESINGULAR_LSQ",
+          4: "SEMO_ENEG_VOLUME", 5: "SEMO_EZERO_AREA", 6: "SEMO_EZERO_PIVOT", 7: "SEMO_ENOCONV",
+          8: "SEMO_EHIP", 9: "SEMO_ESTATE"}
+
+# field ids (semo_field)
+F = dict(P=0, U=1, V=2, W=3, VF=4, Rho=5, mu=6, oldU=7, oldV=8, oldW=9, Qm0=10, Qm1=11, Qm2=12,
+         srcG0=13, srcG1=14, srcG2=15, maxP=16, minP=17, maxU=18, minU=19, maxV=20, minV=21,
+         maxW=22, minW=23, alphaP=24, gradP=32, gradU=33, gradV=34, gradW=35, gradVF=36,
+         x0=48, x1=49, x2=50, b0=56, b1=57, b2=58,
+         cell_volume=64, cell_xyz=65, cell_lsq=66,
+         face_area=80, face_normal=81, face_xyz=82, face_wC=83, face_magPN=84, face_alphaF=85,
+         face_ePN=86, face_skew=87)
+COMPONENTS = dict(gradP=3, gradU=3, gradV=3, gradW=3, gradVF=3, cell_xyz=3, cell_lsq=6,
+                  face_normal=3, face_xyz=3, face_ePN=3, face_skew=3)
+FACE_FIELDS = {k for k in F if k.startswith("face_")}
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("n_points", C.c_int32), ("n_cells", C.c_int32), ("n_faces", C.c_int32),
+                ("n_internal_faces", C.c_int32), ("n_patches", C.c_int32),
+                ("points", C.c_void_p), ("owner", C.c_void_p), ("neighbour", C.c_void_p),
+                ("face_ptr", C.c_void_p), ("face_pts", C.c_void_p), ("cell_ptr", C.c_void_p),
+                ("cell_pts", C.c_void_p), ("patch_start", C.c_void_p), ("patch_nfaces", C.c_void_p)]
+
+
+class PatchBC(C.Structure):
+    _fields_ = [("p_type", C.c_int32), ("p_value", C.c_double), ("u_type", C.c_int32),
+                ("u_value", C.c_double * 3), ("vf_type", C.c_int32), ("vf_value", C.c_double)]
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [("rho", C.c_double * 2), ("mu", C.c_double * 2), ("sigma", C.c_double),
+                ("g", C.c_double * 3), ("dt", C.c_double), ("urf_mom", C.c_double),
+                ("urf_p", C.c_double), ("urf_pu", C.c_double), ("limit", C.c_double)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("tol_p", C.c_double), ("tol_u", C.c_double),
+                ("maxit_p", C.c_int32), ("maxit_u", C.c_int32), ("cheb_degree", C.c_int32),
+                ("coarse_size", C.c_int32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int32 * 4), ("rel_res", C.c_double * 4), ("n_levels", C.c_int32),
+                ("setup_ms", C.c_double)]
+
+
+EXPORTED = [
+    "semo_ctx_create", "semo_ctx_destroy", "semo_ctx_last_error", "semo_ctx_device_bytes",
+    "semo_mesh_upload", "semo_mesh_get_perm", "semo_mesh_get_face_map", "semo_mesh_get_connectivity",
+    "semo_rcm", "semo_set_case", "semo_set_solver", "semo_field_set", "semo_field_get",
+    "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_incompressible_pressure_based",
+    "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_solve_coo",
+    "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
+]
+
+
+def lib():
+    """Load libsemo_gpu.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                           "or make -C semo_mesh_amd/csrc")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.c_void_p
+    sig = {
+        "semo_ctx_create": (C.c_int, [C.c_int, vp, C.POINTER(vp)]),
+        "semo_ctx_destroy": (C.c_int, [vp]),
+        "semo_ctx_last_error": (C.c_char_p, [vp]),
+        "semo_ctx_device_bytes": (i64, [vp]),
+        "semo_mesh_upload": (C.c_int, [vp, C.POINTER(MeshDesc), C.c_int]),
+        "semo_mesh_get_perm": (C.c_int, [vp, vp, vp]),
+        "semo_mesh_get_face_map": (C.c_int, [vp, vp, vp]),
+        "semo_mesh_get_connectivity": (C.c_int, [vp, vp, vp]),
+        "semo_rcm": (C.c_int, [i32, vp, vp, vp, vp]),
+        "semo_set_case": (C.c_int, [vp, C.POINTER(CaseDesc), vp]),
+        "semo_set_solver": (C.c_int, [vp, C.POINTER(SolverParams)]),
+        "semo_field_set": (C.c_int, [vp, C.c_int, dp, i64]),
+        "semo_field_get": (C.c_int, [vp, C.c_int, dp, i64]),
+        "semo_calc_momentum_eqs": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
+        "semo_calc_pressure_eq": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
+        "semo_incompressible_pressure_based": (C.c_int, [vp, C.c_int, dp]),
+        "semo_assemble": (C.c_int, [vp, C.c_int]),
+        "semo_matrix_get": (C.c_int, [vp, C.POINTER(i64), vp, vp, vp, vp]),
+        "semo_least_square": (C.c_int, [vp, dp, dp, dp]),
+        "semo_cell_min_max": (C.c_int, [vp, dp, dp, dp]),
+        "semo_solve_coo": (C.c_int, [vp, C.c_char_p, i32, i64, vp, vp, vp, i32, dp, dp,
+                                     C.POINTER(SolveStats)]),
+        "semo_profile_enable": (C.c_int, [vp, C.c_int]),
+        "semo_profile_reset": (C.c_int, [vp]),
+        "semo_profile_entry": (C.c_int, [vp, C.c_int, C.c_char_p, C.POINTER(i64),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+class SemoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code

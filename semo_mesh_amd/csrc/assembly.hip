@@ -1,0 +1,841 @@
+// Per-iteration assembly of the pressure-based incompressible path on the
+// device.
+//
+// Layout: every cell field is a dense FP64 array in RCM order; boundary-face
+// values are dense arrays over the boundary faces only (internal faces take
+// zero-order owner/neighbour values, reconIncom.cpp:512-533, so they are never
+// stored).  Matrix coefficients go straight into the static sliced-ELL
+// pattern through precomputed face -> slot maps.
+//
+// Determinism: no floating-point atomics.  A face kernel computes each
+// internal face's contributions once and stores them; a cell kernel then
+// accumulates, for each cell, its faces in ascending face order -- the same
+// per-cell order in which SEMO's face loops scatter (eqMomentum.cpp:331-1008,
+// eqPressure.cpp:246-673) -- and adds the boundary faces inline.
+#include "common.hpp"
+
+namespace semo {
+namespace {
+
+constexpr int TPB = 256;
+inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
+
+struct CaseK {
+    double rho0, rho1, mu0, mu1, sigma, g0, g1, g2, dt, urf_u, urf_p, urf_pu, lim;
+    int body;   // gravity or surface tension present
+};
+
+CaseK case_k(const semo_ctx* c) {
+    CaseK k;
+    k.rho0 = c->cs.rho[0]; k.rho1 = c->cs.rho[1]; k.mu0 = c->cs.mu[0]; k.mu1 = c->cs.mu[1];
+    k.sigma = c->cs.sigma; k.g0 = c->cs.g[0]; k.g1 = c->cs.g[1]; k.g2 = c->cs.g[2];
+    k.dt = c->cs.dt; k.urf_u = c->cs.urf_mom; k.urf_p = c->cs.urf_p; k.urf_pu = c->cs.urf_pu;
+    k.lim = c->cs.limit;
+    k.body = (k.g0 != 0.0 || k.g1 != 0.0 || k.g2 != 0.0 || k.sigma != 0.0) ? 1 : 0;
+    return k;
+}
+
+// std::max / std::min semantics (first argument wins ties), as in the reference
+__device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ inline double clip01(double a) { return smax(0.0, smin(1.0, a)); }
+
+// --------------------------------------------------------------------- EOS --
+// calcIncomCellEOSVF (eos.cpp:10-151) + calcCellTransport (transport.cpp:9-98)
+__global__ void __launch_bounds__(TPB) k_eos(int nc, CaseK k, double* __restrict__ VF,
+                                             double* __restrict__ Rho, double* __restrict__ Mu) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double a = clip01(VF[i]);
+    double v0 = clip01(a), v1 = clip01(1.0 - a);
+    double r = 0.0; r += v0 * k.rho0; r += v1 * k.rho1;
+    Rho[i] = r;
+    VF[i] = v0;
+    double m = 0.0; m += v0 * k.mu0; m += (1.0 - v0) * k.mu1;
+    Mu[i] = m + 0.0;
+}
+
+// ------------------------------------------------------ boundary face values --
+// reconIncomZeroOrder boundary dispatch (reconIncom.cpp:591-1032) and the face
+// density of calcOtherDataIncomFromEOS (reconIncom.cpp:445-493)
+__global__ void __launch_bounds__(TPB) k_recon_boundary(
+    int nbf, int nif, CaseK k, const int* __restrict__ bpatch, const semo_patch_bc* __restrict__ bcs,
+    const int* __restrict__ owner, const double* __restrict__ nrm, const double* __restrict__ fxyz,
+    const double* __restrict__ P, const double* __restrict__ U, const double* __restrict__ V,
+    const double* __restrict__ W, const double* __restrict__ VF, const double* __restrict__ Mu,
+    double* __restrict__ bP, double* __restrict__ bUL, double* __restrict__ bUR,
+    double* __restrict__ bVF, double* __restrict__ bMu, double* __restrict__ bRho) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nbf) return;
+    int f = nif + j;
+    int c = owner[f];
+    semo_patch_bc bc = bcs[bpatch[j]];
+    const double* n = nrm + 3 * (size_t)f;
+    bP[j] = bc.p_type == SEMO_BC_FIXED_VALUE ? bc.p_value : P[c];
+    double uL[3], uR[3];
+    double uc[3] = {U[c], V[c], W[c]};
+    switch (bc.u_type) {
+        case SEMO_BC_FIXED_VALUE: {
+            for (int a = 0; a < 3; ++a) { uL[a] = bc.u_value[a]; uR[a] = bc.u_value[a]; }
+            // live orifice profile on fixedValue-U faces (reconIncom.cpp:707-727)
+            double fx = fxyz[3 * (size_t)f];
+            for (int q = 0; q < 24; ++q) {
+                double cen = 0.0006 * (double)q;
+                if (fx > cen - 0.00016 && fx < cen + 0.00016) {
+                    double th = atan((fxyz[3 * (size_t)f + 2] - 0.001) / (fx - cen));
+                    uL[0] = 0.0 * cos(th) - 0.1 * sin(th);
+                    uL[1] = -0.1;
+                    uL[2] = 0.0 * sin(th) + 0.1 * cos(th);
+                }
+            }
+            break;
+        }
+        case SEMO_BC_ZERO_GRADIENT:
+            for (int a = 0; a < 3; ++a) uL[a] = uR[a] = uc[a];
+            break;
+        case SEMO_BC_SLIP: {
+            double nv = uc[0] * n[0] + uc[1] * n[1] + uc[2] * n[2];
+            for (int a = 0; a < 3; ++a) uL[a] = uR[a] = uc[a] - nv * n[a];
+            break;
+        }
+        case SEMO_BC_SURFACE_NORMAL_FIXED_VALUE:
+            for (int a = 0; a < 3; ++a) uL[a] = uR[a] = bc.u_value[0] * n[a];
+            break;
+        case SEMO_BC_INLET_OUTLET: {
+            double own = uc[0] * n[0] + uc[1] * n[1] + uc[2] * n[2];
+            for (int a = 0; a < 3; ++a) uL[a] = uR[a] = own > 0.0 ? uc[a] : bc.u_value[0] * n[a];
+            break;
+        }
+        default:   // noSlip
+            for (int a = 0; a < 3; ++a) uL[a] = uR[a] = 0.0;
+    }
+    for (int a = 0; a < 3; ++a) { bUL[a * (size_t)nbf + j] = uL[a]; bUR[a * (size_t)nbf + j] = uR[a]; }
+    double vf;
+    if (bc.vf_type == SEMO_BC_FIXED_VALUE) vf = bc.vf_value;
+    else if (bc.vf_type == SEMO_BC_INLET_OUTLET) {
+        double own = uc[0] * n[0] + uc[1] * n[1] + uc[2] * n[2];
+        vf = own > 0.0 ? VF[c] : bc.vf_value;
+    } else vf = VF[c];
+    bVF[j] = vf;
+    bMu[j] = Mu[c];
+    double v0 = clip01(vf), v1 = clip01(1.0 - vf);
+    double r = 0.0; r += v0 * k.rho0; r += v1 * k.rho1;
+    bRho[j] = r;
+}
+
+__device__ inline double mix_rho(double a, const CaseK& k) {
+    double v0 = clip01(a), v1 = clip01(1.0 - a);
+    double r = 0.0; r += v0 * k.rho0; r += v1 * k.rho1;
+    return r;
+}
+
+// ------------------------------------------------------------- gravity src --
+// calcSourceGravity (source/gravity.cpp:7-75), gathered per cell
+__global__ void __launch_bounds__(TPB) k_gravity(
+    int nc, int nif, CaseK k, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ owner, const int* __restrict__ nbr, const double* __restrict__ nrm,
+    const double* __restrict__ area, const double* __restrict__ wCv, const double* __restrict__ pf,
+    const double* __restrict__ nfv, const double* __restrict__ vol, const double* __restrict__ U,
+    const double* __restrict__ V, const double* __restrict__ W, const double* __restrict__ VF,
+    const double* __restrict__ bUL, const double* __restrict__ bUR, const double* __restrict__ bVF,
+    int nbf, double* __restrict__ srcG) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double s[4] = {0, 0, 0, 0};
+    double vi = vol[i];
+    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
+        int f = cf_idx[q];
+        const double* n = nrm + 3 * (size_t)f;
+        double wCL = wCv[f], wCR = 1.0 - wCL;
+        double uL[3], uR[3], aL, aR;
+        if (f < nif) {
+            int l = owner[f], r = nbr[f];
+            uL[0] = U[l]; uL[1] = V[l]; uL[2] = W[l]; uR[0] = U[r]; uR[1] = V[r]; uR[2] = W[r];
+            aL = VF[l]; aR = VF[r];
+        } else {
+            int j = f - nif;
+            for (int a = 0; a < 3; ++a) { uL[a] = bUL[a * (size_t)nbf + j]; uR[a] = bUR[a * (size_t)nbf + j]; }
+            aL = aR = bVF[j];
+        }
+        double UnL = uL[0] * n[0] + uL[1] * n[1] + uL[2] * n[2];
+        double UnR = uR[0] * n[0] + uR[1] * n[1] + uR[2] * n[2];
+        double UnF = wCL * UnL + wCR * UnR;
+        double RhoF = 1.0 / (wCL / mix_rho(aL, k) + wCR / mix_rho(aR, k));
+        double A = area[f];
+        if (owner[f] == i) {
+            double gL = 0.0;
+            gL += k.g0 * pf[3 * (size_t)f]; gL += k.g1 * pf[3 * (size_t)f + 1]; gL += k.g2 * pf[3 * (size_t)f + 2];
+            for (int a = 0; a < 3; ++a) s[a] += ((RhoF * gL * n[a]) * A / vi);
+            s[3] += ((RhoF * gL * UnF) * A / vi);
+        } else {
+            double gR = 0.0;
+            gR += k.g0 * nfv[3 * (size_t)f]; gR += k.g1 * nfv[3 * (size_t)f + 1]; gR += k.g2 * nfv[3 * (size_t)f + 2];
+            for (int a = 0; a < 3; ++a) s[a] -= ((RhoF * gR * n[a]) * A / vi);
+            s[3] -= ((RhoF * gR * UnF) * A / vi);
+        }
+    }
+    for (int a = 0; a < 4; ++a) srcG[a * (size_t)nc + i] = s[a];
+}
+
+// ---------------------------------------------- LSQ gradients + stencil bounds --
+// calcLeastSquare "cellVertex" "1st" "cell" (gradient.cpp:761-816 stencil,
+// 997-1083 boundary faces, 1136-1159 apply) for P, U, V, W, VF in one pass,
+// fused with setCellVarMinMax (cellVarMinMax.cpp:15-24) of U, V, W (and P).
+__global__ void __launch_bounds__(TPB) k_grad_minmax(
+    int nc, int nif, int nbf, int with_p, const int* __restrict__ st_ptr, const int* __restrict__ st_idx,
+    const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx, const double* __restrict__ cxyz,
+    const double* __restrict__ fxyz, const double* __restrict__ lsq, const double* __restrict__ P,
+    const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
+    const double* __restrict__ VF, const double* __restrict__ bP, const double* __restrict__ bUR,
+    const double* __restrict__ bVF, double* __restrict__ gP, double* __restrict__ gU,
+    double* __restrict__ gV, double* __restrict__ gW, double* __restrict__ gVF,
+    double* __restrict__ mxP, double* __restrict__ mnP, double* __restrict__ mxU, double* __restrict__ mnU,
+    double* __restrict__ mxV, double* __restrict__ mnV, double* __restrict__ mxW, double* __restrict__ mnW) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double xi = cxyz[3 * (size_t)i], yi = cxyz[3 * (size_t)i + 1], zi = cxyz[3 * (size_t)i + 2];
+    double ph[5] = {P[i], U[i], V[i], W[i], VF[i]};
+    double t[5][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double mx[4] = {ph[0], ph[1], ph[2], ph[3]}, mn[4] = {ph[0], ph[1], ph[2], ph[3]};
+    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) {
+        int j = st_idx[q];
+        double dx = cxyz[3 * (size_t)j] - xi, dy = cxyz[3 * (size_t)j + 1] - yi, dz = cxyz[3 * (size_t)j + 2] - zi;
+        double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+        double pj[5] = {P[j], U[j], V[j], W[j], VF[j]};
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            double d = pj[v] - ph[v];
+            t[v][0] += wk * dx * d; t[v][1] += wk * dy * d; t[v][2] += wk * dz * d;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { mx[v] = smax(mx[v], pj[v]); mn[v] = smin(mn[v], pj[v]); }
+    }
+    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
+        int f = cf_idx[q];
+        if (f < nif) continue;
+        int j = f - nif;
+        double dx = fxyz[3 * (size_t)f] - xi, dy = fxyz[3 * (size_t)f + 1] - yi, dz = fxyz[3 * (size_t)f + 2] - zi;
+        double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+        double fr[5] = {bP[j], bUR[j], bUR[(size_t)nbf + j], bUR[2 * (size_t)nbf + j], bVF[j]};
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            double d = fr[v] - ph[v];
+            t[v][0] += wk * dx * d; t[v][1] += wk * dy * d; t[v][2] += wk * dz * d;
+        }
+    }
+    const double* C = lsq + 6 * (size_t)i;
+    double* out[5] = {gP, gU, gV, gW, gVF};
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        double r0 = 0.0; r0 += C[0] * t[v][0]; r0 += C[1] * t[v][1]; r0 += C[2] * t[v][2];
+        double r1 = 0.0; r1 += C[1] * t[v][0]; r1 += C[3] * t[v][1]; r1 += C[4] * t[v][2];
+        double r2 = 0.0; r2 += C[2] * t[v][0]; r2 += C[4] * t[v][1]; r2 += C[5] * t[v][2];
+        out[v][3 * (size_t)i] = r0; out[v][3 * (size_t)i + 1] = r1; out[v][3 * (size_t)i + 2] = r2;
+    }
+    if (with_p) { mxP[i] = mx[0]; mnP[i] = mn[0]; }
+    mxU[i] = mx[1]; mnU[i] = mn[1]; mxV[i] = mx[2]; mnV[i] = mn[2]; mxW[i] = mx[3]; mnW[i] = mn[3];
+}
+
+// standalone calcLeastSquare of one field
+__global__ void __launch_bounds__(TPB) k_lsq1(
+    int nc, int nif, const int* __restrict__ st_ptr, const int* __restrict__ st_idx,
+    const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx, const double* __restrict__ cxyz,
+    const double* __restrict__ fxyz, const double* __restrict__ lsq, const double* __restrict__ phi,
+    const double* __restrict__ faceR, double* __restrict__ g) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double xi = cxyz[3 * (size_t)i], yi = cxyz[3 * (size_t)i + 1], zi = cxyz[3 * (size_t)i + 2];
+    double p = phi[i], t0 = 0, t1 = 0, t2 = 0;
+    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) {
+        int j = st_idx[q];
+        double dx = cxyz[3 * (size_t)j] - xi, dy = cxyz[3 * (size_t)j + 1] - yi, dz = cxyz[3 * (size_t)j + 2] - zi;
+        double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+        double d = phi[j] - p;
+        t0 += wk * dx * d; t1 += wk * dy * d; t2 += wk * dz * d;
+    }
+    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
+        int f = cf_idx[q];
+        if (f < nif) continue;
+        double dx = fxyz[3 * (size_t)f] - xi, dy = fxyz[3 * (size_t)f + 1] - yi, dz = fxyz[3 * (size_t)f + 2] - zi;
+        double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+        double d = faceR[f] - p;
+        t0 += wk * dx * d; t1 += wk * dy * d; t2 += wk * dz * d;
+    }
+    const double* C = lsq + 6 * (size_t)i;
+    double r0 = 0.0; r0 += C[0] * t0; r0 += C[1] * t1; r0 += C[2] * t2;
+    double r1 = 0.0; r1 += C[1] * t0; r1 += C[3] * t1; r1 += C[4] * t2;
+    double r2 = 0.0; r2 += C[2] * t0; r2 += C[4] * t1; r2 += C[5] * t2;
+    g[3 * (size_t)i] = r0; g[3 * (size_t)i + 1] = r1; g[3 * (size_t)i + 2] = r2;
+}
+
+__global__ void __launch_bounds__(TPB) k_minmax1(int nc, const int* __restrict__ st_ptr,
+                                                 const int* __restrict__ st_idx, const double* __restrict__ phi,
+                                                 double* __restrict__ mx, double* __restrict__ mn) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double a = phi[i], b = phi[i];
+    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) { double v = phi[st_idx[q]]; a = smax(a, v); b = smin(b, v); }
+    mx[i] = a; mn[i] = b;
+}
+
+// --------------------------------------------------------- Rhie-Chow face --
+// Face-normal velocity of the assembly face loops: skew-corrected, stencil-
+// clipped linear interpolation (eqMomentum.cpp:519-553) followed by
+// calcInterpolVelPressure / Gravity / SurfTens (surfaceNormalVelocity.cpp:
+// 125-223).
+struct FaceIn {
+    const double *P, *U, *V, *W, *VF, *Rho, *kappa, *srcG, *srcS;
+    const double *gP, *gU, *gV, *gW;
+    const double *mxU, *mnU, *mxV, *mnV, *mxW, *mnW, *mxP, *mnP;
+    const double *nrm, *ePN, *skew, *wC, *alpha, *magPN, *area;
+    const int *owner, *nbr;
+    int nc;
+};
+
+__device__ inline double face_unf(const FaceIn& in, const CaseK& k, int f, int l, int r) {
+    const double* n = in.nrm + 3 * (size_t)f;
+    const double* e = in.ePN + 3 * (size_t)f;
+    const double* sk = in.skew + 3 * (size_t)f;
+    double wCL = in.wC[f], wCR = 1.0 - wCL, alpha = in.alpha[f], dPN = in.magPN[f];
+    double uL[3] = {in.U[l], in.V[l], in.W[l]}, uR[3] = {in.U[r], in.V[r], in.W[r]};
+    const double* gl[3] = {in.gU + 3 * (size_t)l, in.gV + 3 * (size_t)l, in.gW + 3 * (size_t)l};
+    const double* gr[3] = {in.gU + 3 * (size_t)r, in.gV + 3 * (size_t)r, in.gW + 3 * (size_t)r};
+    for (int a = 0; a < 3; ++a)
+        for (int v = 0; v < 3; ++v) { uL[v] += gl[v][a] * sk[a]; uR[v] += gr[v][a] * sk[a]; }
+    uL[0] = smax(in.mnU[l], smin(in.mxU[l], uL[0])); uR[0] = smax(in.mnU[r], smin(in.mxU[r], uR[0]));
+    uL[1] = smax(in.mnV[l], smin(in.mxV[l], uL[1])); uR[1] = smax(in.mnV[r], smin(in.mxV[r], uR[1]));
+    uL[2] = smax(in.mnW[l], smin(in.mxW[l], uL[2])); uR[2] = smax(in.mnW[r], smin(in.mxW[r], uR[2]));
+    double UnL = uL[0] * n[0] + uL[1] * n[1] + uL[2] * n[2];
+    double UnR = uR[0] * n[0] + uR[1] * n[1] + uR[2] * n[2];
+    double UnF = wCL * UnL + wCR * UnR;
+    double RL = in.Rho[l], RR = in.Rho[r];
+    double Rs = 1.0 / (wCL / RL + wCR / RR);
+    double t1 = k.dt / Rs;
+    const double* gPl = in.gP + 3 * (size_t)l;
+    const double* gPr = in.gP + 3 * (size_t)r;
+    UnF -= alpha * t1 * (in.P[r] - in.P[l]) / dPN;
+    for (int a = 0; a < 3; ++a) {
+        UnF -= t1 * Rs * wCL / RL * gPl[a] * (n[a] - alpha * e[a]);
+        UnF -= t1 * Rs * wCR / RR * gPr[a] * (n[a] - alpha * e[a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        UnF += t1 * Rs * wCL / RL * gPl[a] * n[a];
+        UnF += t1 * Rs * wCR / RR * gPr[a] * n[a];
+    }
+    if (k.body) {
+        double g[3] = {k.g0, k.g1, k.g2};
+        size_t nc = (size_t)in.nc;
+        for (int a = 0; a < 3; ++a) UnF += alpha * t1 * Rs * g[a] * alpha * e[a];
+        for (int a = 0; a < 3; ++a) {
+            UnF += t1 * Rs * wCL / RL * in.srcG[a * nc + l] * (n[a] - alpha * e[a]);
+            UnF += t1 * Rs * wCR / RR * in.srcG[a * nc + r] * (n[a] - alpha * e[a]);
+        }
+        for (int a = 0; a < 3; ++a) {
+            UnF -= alpha * t1 * Rs * wCL / RL * in.srcG[a * nc + l] * n[a];
+            UnF -= alpha * t1 * Rs * wCR / RR * in.srcG[a * nc + r] * n[a];
+        }
+        UnF += alpha * t1 * k.sigma * (wCL * in.kappa[l] + wCR * in.kappa[r]) * (in.VF[r] - in.VF[l]) / dPN;
+        for (int a = 0; a < 3; ++a) {
+            UnF += t1 * Rs * wCL / RL * in.srcS[a * nc + l] * (n[a] - alpha * e[a]);
+            UnF += t1 * Rs * wCR / RR * in.srcS[a * nc + r] * (n[a] - alpha * e[a]);
+        }
+        for (int a = 0; a < 3; ++a) {
+            UnF -= alpha * t1 * Rs * wCL / RL * in.srcS[a * nc + l] * n[a];
+            UnF -= alpha * t1 * Rs * wCR / RR * in.srcS[a * nc + r] * n[a];
+        }
+    }
+    return UnF;
+}
+
+// --------------------------------------------------------------- momentum --
+// internal face of calcMomentumEqs (eqMomentum.cpp:331-787): off-diagonals in
+// place; diagonal and right-hand-side contributions to the per-face scratch
+//   s[0] = X  (owner diag  +=), s[1] = Y (neighbour diag -=),
+//   s[2..4] = owner B_k -=,   s[5..7] = neighbour B_k +=
+__global__ void __launch_bounds__(TPB) k_mom_face(
+    int nif, CaseK k, FaceIn in, const double* __restrict__ Mu, const int* __restrict__ slot_own,
+    const int* __restrict__ slot_nbr, double* __restrict__ aval, double* __restrict__ s) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    int l = in.owner[f], r = in.nbr[f];
+    double area = in.area[f];
+    const double* n = in.nrm + 3 * (size_t)f;
+    const double* e = in.ePN + 3 * (size_t)f;
+    const double* sk = in.skew + 3 * (size_t)f;
+    double wCL = in.wC[f], wCR = 1.0 - wCL;
+    double muF = 1.0 / (wCL / Mu[l] + wCR / Mu[r]);
+    double RL = in.Rho[l], RR = in.Rho[r];
+    double dPN = in.magPN[f], alpha = in.alpha[f];
+    double UnF = face_unf(in, k, f, l, r);
+    double wL = (UnF > 0.0) ? 1.0 : 0.0, wR = 1.0 - wL;
+    double PL = in.P[l], PR = in.P[r];
+    double uL[3] = {in.U[l], in.V[l], in.W[l]}, uR[3] = {in.U[r], in.V[r], in.W[r]};
+    double Fv[3] = {wL * uL[0] + wR * uR[0], wL * uL[1] + wR * uR[1], wL * uL[2] + wR * uR[2]};
+    const double* gPl = in.gP + 3 * (size_t)l;
+    const double* gPr = in.gP + 3 * (size_t)r;
+    double PF = wCL * PL + wCR * PR;
+    for (int a = 0; a < 3; ++a) { PF += wCL * gPl[a] * sk[a]; PF += wCR * gPr[a] * sk[a]; }
+    PF = smax(in.mnP[l], smin(in.mxP[l], PF));
+    PF = smax(in.mnP[r], smin(in.mxP[r], PF));
+    double X = (wL * RL * UnF * area + alpha * muF / dPN * area);
+    double ao = (wR * RL * UnF * area - alpha * muF / dPN * area);
+    double Y = (wR * RR * UnF * area - alpha * muF / dPN * area);
+    double an = -(wL * RR * UnF * area + alpha * muF / dPN * area);
+    // entries with |v| < DBL_MIN are dropped (eqMomentum.cpp:699-711)
+    aval[slot_own[f]] = fabs(ao) < 2.2250738585072014e-308 ? 0.0 : ao;
+    aval[slot_nbr[f]] = fabs(an) < 2.2250738585072014e-308 ? 0.0 : an;
+    s[f] = X;
+    s[(size_t)nif + f] = Y;
+    const double* gl[3] = {in.gU + 3 * (size_t)l, in.gV + 3 * (size_t)l, in.gW + 3 * (size_t)l};
+    const double* gr[3] = {in.gU + 3 * (size_t)r, in.gV + 3 * (size_t)r, in.gW + 3 * (size_t)r};
+    for (int v = 0; v < 3; ++v) {
+        double cL = (RL * Fv[v] * UnF + PF * n[v]) * area;
+        double cR = (RR * Fv[v] * UnF + PF * n[v]) * area;
+        double df = alpha * muF * (uR[v] - uL[v]) / dPN * area;
+        for (int a = 0; a < 3; ++a) df += (muF * (wCL * gl[v][a] + wCR * gr[v][a]) * (n[a] - alpha * e[a]) * area);
+        double dL = ((gl[v][0] * n[0] + gl[v][1] * n[1] + gl[v][2] * n[2]) * muF * area);
+        double dR = ((gr[v][0] * n[0] + gr[v][1] * n[1] + gr[v][2] * n[2]) * muF * area);
+        s[(size_t)(2 + v) * nif + f] = (cL - df - dL);
+        s[(size_t)(5 + v) * nif + f] = (cR - df - dR);
+    }
+}
+
+// cell part of calcMomentumEqs: time term and sources (eqMomentum.cpp:223-326),
+// the cell's internal-face contributions in ascending face order, then its
+// boundary faces (eqMomentum.cpp:795-1008); writes diagonal and 3 RHS.
+__global__ void __launch_bounds__(TPB) k_mom_cell(
+    int nc, int nif, int nbf, CaseK k, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ owner, const int* __restrict__ bpatch, const semo_patch_bc* __restrict__ bcs,
+    const double* __restrict__ vol, const double* __restrict__ nrm, const double* __restrict__ ePN,
+    const double* __restrict__ area, const double* __restrict__ magPN, const double* __restrict__ alphaF,
+    const double* __restrict__ Rho, const double* __restrict__ P, const double* __restrict__ U,
+    const double* __restrict__ V, const double* __restrict__ W, const double* __restrict__ oU,
+    const double* __restrict__ oV, const double* __restrict__ oW, const double* __restrict__ q0,
+    const double* __restrict__ q1, const double* __restrict__ q2, const double* __restrict__ srcG,
+    const double* __restrict__ srcS, const double* __restrict__ alphaP, const double* __restrict__ gP,
+    const double* __restrict__ gU, const double* __restrict__ gV, const double* __restrict__ gW,
+    const double* __restrict__ bP, const double* __restrict__ bUL, const double* __restrict__ bUR,
+    const double* __restrict__ bMu, const double* __restrict__ bRho, const double* __restrict__ s,
+    double* __restrict__ diag, double* __restrict__ B) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double R = Rho[i], vl = vol[i], dt = k.dt;
+    double d = R * 1.5 * vl / dt;
+    double b[3];
+    b[0] = -R * (1.5 * U[i] - 2.0 * oU[i] + 0.5 * q0[i]) * vl / dt;
+    b[1] = -R * (1.5 * V[i] - 2.0 * oV[i] + 0.5 * q1[i]) * vl / dt;
+    b[2] = -R * (1.5 * W[i] - 2.0 * oW[i] + 0.5 * q2[i]) * vl / dt;
+    if (k.body) {
+        for (int v = 0; v < 3; ++v) b[v] += vl * srcG[v * (size_t)nc + i];
+        for (int v = 0; v < 3; ++v) b[v] += vl * srcS[v * (size_t)nc + i];
+    }
+    double uc[3] = {U[i], V[i], W[i]};
+    if (alphaP) {   // Darcy porosity source -alpha_p*U*V, implicit (adjoint harness)
+        double ap = alphaP[i] * vl;
+        d += ap;
+        for (int v = 0; v < 3; ++v) b[v] -= ap * uc[v];
+    }
+    const double* gc[3] = {gU + 3 * (size_t)i, gV + 3 * (size_t)i, gW + 3 * (size_t)i};
+    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
+        int f = cf_idx[q];
+        if (f < nif) {
+            if (owner[f] == i) {
+                d += s[f];
+                for (int v = 0; v < 3; ++v) b[v] -= s[(size_t)(2 + v) * nif + f];
+            } else {
+                d -= s[(size_t)nif + f];
+                for (int v = 0; v < 3; ++v) b[v] += s[(size_t)(5 + v) * nif + f];
+            }
+            continue;
+        }
+        int j = f - nif;
+        semo_patch_bc bc = bcs[bpatch[j]];
+        const double* n = nrm + 3 * (size_t)f;
+        const double* e = ePN + 3 * (size_t)f;
+        double A = area[f];
+        double ul[3], ur[3];
+        for (int a = 0; a < 3; ++a) { ul[a] = bUL[a * (size_t)nbf + j]; ur[a] = bUR[a * (size_t)nbf + j]; }
+        double UnL = ul[0] * n[0] + ul[1] * n[1] + ul[2] * n[2];
+        double UnR = ur[0] * n[0] + ur[1] * n[1] + ur[2] * n[2];
+        double UnF = 0.5 * UnL + 0.5 * UnR;
+        double RL = bRho[j];
+        double t1 = 1.0 / RL * dt;
+        double Fv[3] = {0.5 * ul[0] + 0.5 * ur[0], 0.5 * ul[1] + 0.5 * ur[1], 0.5 * ul[2] + 0.5 * ur[2]};
+        double PF = 0.5 * bP[j] + 0.5 * bP[j];
+        double cU = 0.0, cUd = 1.0;
+        if (bc.u_type == SEMO_BC_ZERO_GRADIENT) { cU = 1.0; cUd = 0.0; }
+        else if (bc.u_type == SEMO_BC_SLIP) { cU = 0.0; cUd = 0.0; }
+        else if (bc.u_type == SEMO_BC_INLET_OUTLET) {
+            double own = uc[0] * n[0] + uc[1] * n[1] + uc[2] * n[2];
+            cU = own > 0.0 ? 1.0 : 0.0; cUd = own > 0.0 ? 0.0 : 1.0;
+        }
+        double muF = 0.5 * bMu[j] + 0.5 * bMu[j];
+        double dPN = 0.5 * magPN[f], alpha = alphaF[f];
+        if (bc.p_type == SEMO_BC_FIXED_VALUE) {
+            UnF -= alpha * t1 * (bc.p_value - P[i]) / dPN;
+            for (int a = 0; a < 3; ++a) UnF += alpha * t1 * gP[3 * (size_t)i + a] * e[a];
+        }
+        double dn[3];
+        for (int v = 0; v < 3; ++v) { dn[v] = 0.0; dn[v] += alpha * (ur[v] - uc[v]) / dPN; }
+        for (int a = 0; a < 3; ++a) {
+            double tv = n[a] - alpha * e[a];
+            for (int v = 0; v < 3; ++v) dn[v] += gc[v][a] * tv;
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int v = 0; v < 3; ++v) dn[v] += (gc[v][a] * n[a]);
+        d += cU * (RL * UnF * A);
+        d += cUd * (muF * alpha / dPN * A);
+        for (int v = 0; v < 3; ++v) b[v] -= (RL * Fv[v] * UnF * A + PF * n[v] * A - muF * dn[v] * A);
+    }
+    diag[i] = d;
+    for (int v = 0; v < 3; ++v) B[v * (size_t)nc + i] = b[v];
+}
+
+// --------------------------------------------------------------- pressure --
+// internal face of calcPressureEq (eqPressure.cpp:246-564):
+//   s[0] = alpha*dt/rho* /dPN*A (rho* with wCR twice, eqPressure.cpp:396)
+//   s[1] = UnF*A
+__global__ void __launch_bounds__(TPB) k_p_face(
+    int nif, CaseK k, FaceIn in, const int* __restrict__ slot_own, const int* __restrict__ slot_nbr,
+    double* __restrict__ aval, double* __restrict__ s) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    int l = in.owner[f], r = in.nbr[f];
+    double area = in.area[f];
+    double wCL = in.wC[f], wCR = 1.0 - wCL;
+    double Rs = 1.0 / (wCR / in.Rho[l] + wCR / in.Rho[r]);
+    double t1 = k.dt / Rs;
+    double dPN = in.magPN[f], alpha = in.alpha[f];
+    double UnF = face_unf(in, k, f, l, r);
+    double cf = (+alpha * t1 / dPN * area);
+    aval[slot_own[f]] = (-alpha * t1 / dPN * area);
+    aval[slot_nbr[f]] = -(+alpha * t1 / dPN * area);
+    s[f] = cf;
+    s[(size_t)nif + f] = (UnF * area);
+}
+
+__global__ void __launch_bounds__(TPB) k_p_cell(
+    int nc, int nif, int nbf, CaseK k, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ owner, const int* __restrict__ bpatch, const semo_patch_bc* __restrict__ bcs,
+    const double* __restrict__ nrm, const double* __restrict__ ePN, const double* __restrict__ area,
+    const double* __restrict__ magPN, const double* __restrict__ alphaF, const double* __restrict__ P,
+    const double* __restrict__ gP, const double* __restrict__ bP, const double* __restrict__ bUL,
+    const double* __restrict__ bUR, const double* __restrict__ bRho, const double* __restrict__ s,
+    double* __restrict__ diag, double* __restrict__ B) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double d = 0.0, b = 0.0;
+    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
+        int f = cf_idx[q];
+        if (f < nif) {
+            if (owner[f] == i) { d += s[f]; b -= s[(size_t)nif + f]; }
+            else { d -= -s[f]; b += s[(size_t)nif + f]; }
+            continue;
+        }
+        int j = f - nif;
+        semo_patch_bc bc = bcs[bpatch[j]];
+        const double* n = nrm + 3 * (size_t)f;
+        const double* e = ePN + 3 * (size_t)f;
+        double UnL = bUL[j] * n[0] + bUL[(size_t)nbf + j] * n[1] + bUL[2 * (size_t)nbf + j] * n[2];
+        double UnR = bUR[j] * n[0] + bUR[(size_t)nbf + j] * n[1] + bUR[2 * (size_t)nbf + j] * n[2];
+        double UnF = 0.5 * UnL + 0.5 * UnR;
+        double t1 = 1.0 / bRho[j] * k.dt;
+        double dPN = 0.5 * magPN[f], alpha = alphaF[f], A = area[f];
+        double cPd = 0.0;
+        if (bc.p_type == SEMO_BC_FIXED_VALUE) {
+            cPd = 1.0;
+            UnF -= alpha * t1 * (bP[j] - P[i]) / dPN;
+            for (int a = 0; a < 3; ++a) UnF += alpha * t1 * gP[3 * (size_t)i + a] * e[a];
+        }
+        d += cPd * (alpha * t1 / dPN * A);
+        b -= (UnF * A);
+    }
+    diag[i] = d;
+    B[i] = b;
+}
+
+// ------------------------------------------- p' gradient and correction --
+// eqPressure.cpp:686-809: two Green-Gauss passes of p' with skew correction
+__global__ void __launch_bounds__(TPB) k_gg_face(int nif, const int* __restrict__ owner,
+                                                 const int* __restrict__ nbr, const double* __restrict__ wCv,
+                                                 const double* __restrict__ skew, const double* __restrict__ pp,
+                                                 const double* __restrict__ gp, double* __restrict__ vF) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    int l = owner[f], r = nbr[f];
+    double wCL = wCv[f], wCR = 1.0 - wCL;
+    double v = wCL * pp[l] + wCR * pp[r];
+    const double* sk = skew + 3 * (size_t)f;
+    for (int a = 0; a < 3; ++a) { v += wCL * gp[3 * (size_t)l + a] * sk[a]; v += wCR * gp[3 * (size_t)r + a] * sk[a]; }
+    vF[f] = v;
+}
+
+__global__ void __launch_bounds__(TPB) k_gg_cell(
+    int nc, int nif, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ owner, const int* __restrict__ bpatch, const semo_patch_bc* __restrict__ bcs,
+    const double* __restrict__ nrm, const double* __restrict__ area, const double* __restrict__ vol,
+    const double* __restrict__ pp, const double* __restrict__ vF, double* __restrict__ gout) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double g[3] = {0, 0, 0};
+    double vi = vol[i];
+    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
+        int f = cf_idx[q];
+        const double* n = nrm + 3 * (size_t)f;
+        double A = area[f];
+        if (f < nif) {
+            double v = vF[f];
+            if (owner[f] == i) { for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi; }
+            else { for (int a = 0; a < 3; ++a) g[a] -= v * n[a] * A / vi; }
+        } else {
+            double v = pp[i];
+            if (bcs[bpatch[f - nif]].p_type == SEMO_BC_FIXED_VALUE) v = 0.0;
+            for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi;
+        }
+    }
+    for (int a = 0; a < 3; ++a) gout[3 * (size_t)i + a] = g[a];
+}
+
+// block-partial sums of two quantities (deterministic two-stage reduction)
+__device__ inline void block_sum2(double a, double b, double* __restrict__ part) {
+    __shared__ double sa[TPB / 64], sb[TPB / 64];
+    for (int o = 32; o > 0; o >>= 1) { a += __shfl_down(a, o, 64); b += __shfl_down(b, o, 64); }
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { sa[w] = a; sb[w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0, y = 0;
+        for (int k = 0; k < TPB / 64; ++k) { x += sa[k]; y += sb[k]; }
+        part[2 * blockIdx.x] = x; part[2 * blockIdx.x + 1] = y;
+    }
+}
+
+__device__ inline double clipv(double v, double lim) {
+    if (v <= -lim) v = -lim;
+    if (v >= lim) v = lim;
+    return v;
+}
+
+// eqPressure.cpp:823-883
+__global__ void __launch_bounds__(TPB) k_p_correct(
+    int nc, CaseK k, const double* __restrict__ pp, const double* __restrict__ gp,
+    const double* __restrict__ Rho, double* __restrict__ P, double* __restrict__ U,
+    double* __restrict__ V, double* __restrict__ W, double* __restrict__ part) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double nd = 0.0, np = 0.0;
+    if (i < nc) {
+        double x = pp[i];
+        double p = P[i] + k.urf_p * x;
+        double t1 = k.dt / Rho[i];
+        double rU = k.urf_pu * t1 * gp[3 * (size_t)i], rV = k.urf_pu * t1 * gp[3 * (size_t)i + 1],
+               rW = k.urf_pu * t1 * gp[3 * (size_t)i + 2];
+        double u = U[i] - rU, v = V[i] - rV, w = W[i] - rW;
+        p = clipv(p, k.lim); u = clipv(u, k.lim); v = clipv(v, k.lim); w = clipv(w, k.lim);
+        P[i] = p; U[i] = u; V[i] = v; W[i] = w;
+        nd = x * x + rU * rU + rV * rV + rW * rW;
+        np = p * p + u * u + v * v + w * w;
+    }
+    block_sum2(nd, np, part);
+}
+
+// eqMomentum.cpp:1029-1085
+__global__ void __launch_bounds__(TPB) k_mom_update(int nc, CaseK k, const double* __restrict__ X,
+                                                    double* __restrict__ U, double* __restrict__ V,
+                                                    double* __restrict__ W, double* __restrict__ part) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double nd = 0.0, np = 0.0;
+    if (i < nc) {
+        double x0 = X[i], x1 = X[(size_t)nc + i], x2 = X[2 * (size_t)nc + i];
+        double u = clipv(U[i] + k.urf_u * x0, k.lim);
+        double v = clipv(V[i] + k.urf_u * x1, k.lim);
+        double w = clipv(W[i] + k.urf_u * x2, k.lim);
+        U[i] = u; V[i] = v; W[i] = w;
+        nd = x0 * x0 + x1 * x1 + x2 * x2;
+        np = u * u + v * v + w * w;
+    }
+    block_sum2(nd, np, part);
+}
+
+__global__ void k_copy(size_t n, double* __restrict__ d, const double* __restrict__ s) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+FaceIn face_in(semo_ctx* c) {
+    FaceIn in;
+    in.P = c->P.p; in.U = c->U.p; in.V = c->V.p; in.W = c->W.p; in.VF = c->VF.p; in.Rho = c->Rho.p;
+    in.kappa = c->kappa.p; in.srcG = c->srcG.p; in.srcS = c->srcS.p;
+    in.gP = c->gP.p; in.gU = c->gU.p; in.gV = c->gV.p; in.gW = c->gW.p;
+    in.mxU = c->mxU.p; in.mnU = c->mnU.p; in.mxV = c->mxV.p; in.mnV = c->mnV.p; in.mxW = c->mxW.p;
+    in.mnW = c->mnW.p; in.mxP = c->mxP.p; in.mnP = c->mnP.p;
+    in.nrm = c->f_nrm.p; in.ePN = c->f_ePN.p; in.skew = c->f_skew.p; in.wC = c->f_wC.p;
+    in.alpha = c->f_alpha.p; in.magPN = c->f_magPN.p; in.area = c->f_area.p;
+    in.owner = c->d_owner.p; in.nbr = c->d_nbr.p; in.nc = c->nc;
+    return in;
+}
+
+double finish_ratio(semo_ctx* c, int nb) {
+    std::vector<double> h(2 * (size_t)nb);
+    SEMO_HIP(hipMemcpyAsync(h.data(), c->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    double a = 0, b = 0;
+    for (int k = 0; k < nb; ++k) { a += h[2 * k]; b += h[2 * k + 1]; }
+    return sqrt(a) / sqrt(b);
+}
+
+struct ProfScope {
+    semo_ctx* c; const char* name; double bytes; hipEvent_t a{};
+    ProfScope(semo_ctx* c_, const char* n, double b) : c(c_), name(n), bytes(b) {
+        if (c->prof.on) c->prof.begin(c->stream, a);
+    }
+    ~ProfScope() { if (c->prof.on) c->prof.end(c->stream, name, a, bytes); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- drivers --
+void fields_alloc(semo_ctx* c) {
+    size_t n = c->nc, nb = c->nbf;
+    for (DBuf<double>* b : {&c->P, &c->U, &c->V, &c->W, &c->VF, &c->Rho, &c->Mu, &c->oldP, &c->oldU,
+                            &c->oldV, &c->oldW, &c->oldVF, &c->Qm0, &c->Qm1, &c->Qm2, &c->kappa,
+                            &c->mxP, &c->mnP, &c->mxU, &c->mnU, &c->mxV, &c->mnV, &c->mxW, &c->mnW})
+        { b->alloc(n); b->zero(c->stream); }
+    c->srcG.alloc(4 * n); c->srcG.zero(c->stream);
+    c->srcS.alloc(4 * n); c->srcS.zero(c->stream);
+    for (DBuf<double>* b : {&c->gP, &c->gU, &c->gV, &c->gW, &c->gVF, &c->gpp, &c->gtmp})
+        { b->alloc(3 * n); b->zero(c->stream); }
+    for (DBuf<double>* b : {&c->bP, &c->bVF, &c->bMu, &c->bRho}) { b->alloc(nb); b->zero(c->stream); }
+    c->bUL.alloc(3 * nb); c->bUR.alloc(3 * nb);
+    c->Bvec.alloc(3 * n); c->Bvec.zero(c->stream);
+    c->Xvec.alloc(3 * n); c->Xvec.zero(c->stream);
+    c->fscr.alloc(8 * (size_t)std::max(c->nif, 1));
+    c->red.alloc(2 * (size_t)nblk(std::max<long>(n, 1)) + 64);
+    c->scal.alloc(64);
+    std::vector<double> one(n, 1.0);
+    c->VF.upload(one, c->stream);
+    c->oldVF.upload(one, c->stream);
+}
+
+void eos_transport(semo_ctx* c) {
+    CaseK k = case_k(c);
+    ProfScope ps(c, "eos", 24.0 * c->nc);
+    hipLaunchKernelGGL(k_eos, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, k, c->VF.p, c->Rho.p, c->Mu.p);
+}
+
+void recon_boundary(semo_ctx* c) {
+    if (c->nbf == 0) return;
+    CaseK k = case_k(c);
+    ProfScope ps(c, "recon_boundary", 120.0 * c->nbf);
+    hipLaunchKernelGGL(k_recon_boundary, dim3(nblk(c->nbf)), dim3(TPB), 0, c->stream, c->nbf, c->nif, k,
+                       c->d_bpatch.p, c->d_bcs.p, c->d_owner.p, c->f_nrm.p, c->f_xyz.p, c->P.p, c->U.p,
+                       c->V.p, c->W.p, c->VF.p, c->Mu.p, c->bP.p, c->bUL.p, c->bUR.p, c->bVF.p, c->bMu.p,
+                       c->bRho.p);
+}
+
+void source_gravity(semo_ctx* c) {
+    CaseK k = case_k(c);
+    if (k.g0 == 0.0 && k.g1 == 0.0 && k.g2 == 0.0) { c->srcG.zero(c->stream); return; }
+    hipLaunchKernelGGL(k_gravity, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, k,
+                       c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_nbr.p, c->f_nrm.p, c->f_area.p,
+                       c->f_wC.p, c->f_pf.p, c->f_nf.p, c->c_vol.p, c->U.p, c->V.p, c->W.p, c->VF.p,
+                       c->bUL.p, c->bUR.p, c->bVF.p, c->nbf, c->srcG.p);
+}
+
+void grad_minmax(semo_ctx* c, bool with_p) {
+    // algorithmic bytes: own 5 fields + centroid + 6 coeffs + stencil ptr, 5 grads + 8 bounds out
+    double bytes = (double)c->nc * (5 * 8 + 24 + 48 + 4 + 5 * 24 + (with_p ? 64 : 48)) +
+                   4.0 * c->h_cf_ptr.back() * 0 + 4.0 * (double)c->d_st_idx.n;
+    ProfScope ps(c, "grad_minmax", bytes);
+    hipLaunchKernelGGL(k_grad_minmax, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf,
+                       with_p ? 1 : 0, c->d_st_ptr.p, c->d_st_idx.p, c->d_cf_ptr.p, c->d_cf_idx.p, c->c_xyz.p,
+                       c->f_xyz.p, c->c_lsq.p, c->P.p, c->U.p, c->V.p, c->W.p, c->VF.p, c->bP.p, c->bUR.p,
+                       c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p, c->gVF.p, c->mxP.p, c->mnP.p, c->mxU.p,
+                       c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
+}
+
+void assemble_momentum(semo_ctx* c) {
+    CaseK k = case_k(c);
+    grad_minmax(c, true);
+    if (c->nif) {
+        double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 8 * 8) + (double)c->nc * 8 * (4 + 12 + 8 + 2);
+        ProfScope ps(c, "mom_face", bytes);
+        hipLaunchKernelGGL(k_mom_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, k, face_in(c),
+                           c->Mu.p, c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
+    }
+    {
+        double bytes = (double)c->nc * 8 * (1 + 1 + 3 + 3 + 3 + 4 + 4) + 8.0 * 8 * c->nif + 4.0 * c->h_cf_idx.size();
+        ProfScope ps(c, "mom_cell", bytes);
+        hipLaunchKernelGGL(k_mom_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
+                           c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p,
+                           c->f_nrm.p, c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->Rho.p, c->P.p,
+                           c->U.p, c->V.p, c->W.p, c->oldU.p, c->oldV.p, c->oldW.p, c->Qm0.p, c->Qm1.p, c->Qm2.p,
+                           c->srcG.p, c->srcS.p, c->alphaP.n ? c->alphaP.p : nullptr, c->gP.p, c->gU.p, c->gV.p,
+                           c->gW.p, c->bP.p, c->bUL.p, c->bUR.p, c->bMu.p, c->bRho.p, c->fscr.p, c->Amat.diag.p,
+                           c->Bvec.p);
+    }
+    c->last_eq = 0;
+}
+
+void assemble_pressure(semo_ctx* c) {
+    CaseK k = case_k(c);
+    grad_minmax(c, false);
+    if (c->nif) {
+        double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 2 * 8) + (double)c->nc * 8 * (4 + 12 + 6 + 2);
+        ProfScope ps(c, "p_face", bytes);
+        hipLaunchKernelGGL(k_p_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, k, face_in(c),
+                           c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
+    }
+    {
+        ProfScope ps(c, "p_cell", (double)c->nc * 16 + 16.0 * c->nif + 4.0 * c->h_cf_idx.size());
+        hipLaunchKernelGGL(k_p_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
+                           c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p,
+                           c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->P.p, c->gP.p, c->bP.p,
+                           c->bUL.p, c->bUR.p, c->bRho.p, c->fscr.p, c->Amat.diag.p, c->Bvec.p);
+    }
+    c->last_eq = 1;
+}
+
+double momentum_update(semo_ctx* c) {
+    CaseK k = case_k(c);
+    int nb = nblk(c->nc);
+    hipLaunchKernelGGL(k_mom_update, dim3(nb), dim3(TPB), 0, c->stream, c->nc, k, c->Xvec.p, c->U.p, c->V.p,
+                       c->W.p, c->red.p);
+    return finish_ratio(c, nb);
+}
+
+double pressure_correct(semo_ctx* c) {
+    CaseK k = case_k(c);
+    const double* pp = c->Xvec.p;
+    c->gpp.zero(c->stream);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (c->nif)
+            hipLaunchKernelGGL(k_gg_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, c->d_owner.p,
+                               c->d_nbr.p, c->f_wC.p, c->f_skew.p, pp, c->gpp.p, c->fscr.p);
+        hipLaunchKernelGGL(k_gg_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->d_cf_ptr.p,
+                           c->d_cf_idx.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p,
+                           c->c_vol.p, pp, c->fscr.p, c->gtmp.p);
+        std::swap(c->gpp, c->gtmp);
+    }
+    int nb = nblk(c->nc);
+    hipLaunchKernelGGL(k_p_correct, dim3(nb), dim3(TPB), 0, c->stream, c->nc, k, pp, c->gpp.p, c->Rho.p,
+                       c->P.p, c->U.p, c->V.p, c->W.p, c->red.p);
+    return finish_ratio(c, nb);
+}
+
+void lsq_standalone(semo_ctx* c, const double* phi, const double* faceR, double* grad) {
+    hipLaunchKernelGGL(k_lsq1, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->d_st_ptr.p,
+                       c->d_st_idx.p, c->d_cf_ptr.p, c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p, phi,
+                       faceR, grad);
+}
+
+void minmax_standalone(semo_ctx* c, const double* phi, double* mx, double* mn) {
+    hipLaunchKernelGGL(k_minmax1, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->d_st_ptr.p,
+                       c->d_st_idx.p, phi, mx, mn);
+}
+
+void copy_d(semo_ctx* c, double* dst, const double* src, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, c->stream, n, dst, src);
+}
+
+}  // namespace semo

@@ -1,0 +1,477 @@
+// C ABI of libsemo_gpu: context, mesh, fields, per-equation entry points and
+// the SIMPLE driver.  Every function catches semo::Error and returns its code;
+// the message is kept for semo_ctx_last_error.
+#include <algorithm>
+#include <cstring>
+#include <functional>
+
+#include "common.hpp"
+
+namespace semo {
+void solve_coo(semo_ctx* c, const char* eq, int n, int64_t nnz, const int* rows, const int* cols,
+               const double* vals, int nrhs, const double* b, double* x, semo_solve_stats* st);
+
+// -------------------------------------------------------------- profiler ---
+hipEvent_t Profiler::get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    SEMO_HIP(hipEventCreate(&e));
+    return e;
+}
+void Profiler::begin(hipStream_t s, hipEvent_t& a) {
+    a = get();
+    SEMO_HIP(hipEventRecord(a, s));
+}
+void Profiler::end(hipStream_t s, const char* name, hipEvent_t a, double bytes) {
+    hipEvent_t b = get();
+    SEMO_HIP(hipEventRecord(b, s));
+    pending.push_back({name, a, b, bytes});
+    if (pending.size() > 4096) flush();
+}
+void Profiler::flush() {
+    for (auto& p : pending) {
+        SEMO_HIP(hipEventSynchronize(p.b));
+        float ms = 0;
+        SEMO_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        KernelStat& k = stats[p.name];
+        k.count++; k.ms += ms; k.bytes += p.bytes;
+        pool.push_back(p.a); pool.push_back(p.b);
+    }
+    pending.clear();
+}
+Profiler::~Profiler() {
+    for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+namespace {
+
+void need_mesh(semo_ctx* c) { SEMO_CHECK(c->has_mesh, SEMO_ESTATE, "no mesh uploaded"); }
+void need_case(semo_ctx* c) {
+    need_mesh(c);
+    SEMO_CHECK(c->has_case, SEMO_ESTATE, "no case set (semo_set_case)");
+}
+
+// cell arrays: caller numbering <-> device numbering
+void to_dev(semo_ctx* c, DBuf<double>& d, const double* h, int comps) {
+    std::vector<double> t((size_t)c->nc * comps);
+    for (int i = 0; i < c->nc; ++i)
+        for (int k = 0; k < comps; ++k) t[(size_t)i * comps + k] = h[(size_t)c->perm[i] * comps + k];
+    d.upload(t, c->stream);
+}
+void from_dev(semo_ctx* c, const double* dptr, double* h, int comps, size_t stride_block = 0) {
+    std::vector<double> t((size_t)c->nc * comps);
+    SEMO_HIP(hipMemcpyAsync(t.data(), dptr, t.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->nc; ++i)
+        for (int k = 0; k < comps; ++k) h[(size_t)c->perm[i] * comps + k] = t[(size_t)i * comps + k];
+    (void)stride_block;
+}
+void face_from_dev(semo_ctx* c, const DBuf<double>& d, double* h, int comps, bool odd) {
+    std::vector<double> t(d.n);
+    d.download(t.data(), t.size(), c->stream);
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->nf; ++k) {
+        int o = c->fperm[k];
+        double sg = (odd && c->fflip[k]) ? -1.0 : 1.0;
+        for (int a = 0; a < comps; ++a) h[(size_t)o * comps + a] = sg * t[(size_t)k * comps + a];
+    }
+}
+
+DBuf<double>* cell_field(semo_ctx* c, int f, int& comps) {
+    comps = 1;
+    switch (f) {
+        case SEMO_F_P: return &c->P; case SEMO_F_U: return &c->U; case SEMO_F_V: return &c->V;
+        case SEMO_F_W: return &c->W; case SEMO_F_VF: return &c->VF; case SEMO_F_RHO: return &c->Rho;
+        case SEMO_F_MU: return &c->Mu; case SEMO_F_OLDU: return &c->oldU; case SEMO_F_OLDV: return &c->oldV;
+        case SEMO_F_OLDW: return &c->oldW; case SEMO_F_QM0: return &c->Qm0; case SEMO_F_QM1: return &c->Qm1;
+        case SEMO_F_QM2: return &c->Qm2;
+        case SEMO_F_MAXP: return &c->mxP; case SEMO_F_MINP: return &c->mnP; case SEMO_F_MAXU: return &c->mxU;
+        case SEMO_F_MINU: return &c->mnU; case SEMO_F_MAXV: return &c->mxV; case SEMO_F_MINV: return &c->mnV;
+        case SEMO_F_MAXW: return &c->mxW; case SEMO_F_MINW: return &c->mnW;
+        case SEMO_F_ALPHA_P: return &c->alphaP;
+        case SEMO_F_GRADP: comps = 3; return &c->gP; case SEMO_F_GRADU: comps = 3; return &c->gU;
+        case SEMO_F_GRADV: comps = 3; return &c->gV; case SEMO_F_GRADW: comps = 3; return &c->gW;
+        case SEMO_F_GRADVF: comps = 3; return &c->gVF;
+        case SEMO_G_CELL_VOLUME: return &c->c_vol;
+        case SEMO_G_CELL_XYZ: comps = 3; return &c->c_xyz;
+        case SEMO_G_CELL_LSQ: comps = 6; return &c->c_lsq;
+        default: return nullptr;
+    }
+}
+
+int guard(semo_ctx* c, const std::function<void()>& fn) {
+    try {
+        fn();
+        if (c) c->err.clear();
+        return SEMO_OK;
+    } catch (const Error& e) {
+        if (c) c->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return SEMO_EINVAL;
+    }
+}
+
+void step_setup(semo_ctx* c) {
+    // pressureBased.cpp:17-69: first-step EOS/transport, time levels
+    if (c->iterReal == 0) eos_transport(c);
+    const DBuf<double>& q0 = c->iterReal == 0 ? c->U : c->oldU;
+    const DBuf<double>& q1 = c->iterReal == 0 ? c->V : c->oldV;
+    const DBuf<double>& q2 = c->iterReal == 0 ? c->W : c->oldW;
+    copy_d(c, c->Qm0.p, q0.p, c->nc); copy_d(c, c->Qm1.p, q1.p, c->nc); copy_d(c, c->Qm2.p, q2.p, c->nc);
+    copy_d(c, c->oldP.p, c->P.p, c->nc); copy_d(c, c->oldU.p, c->U.p, c->nc);
+    copy_d(c, c->oldV.p, c->V.p, c->nc); copy_d(c, c->oldW.p, c->W.p, c->nc);
+    copy_d(c, c->oldVF.p, c->VF.p, c->nc);
+    recon_boundary(c);
+    source_gravity(c);        // calcSourceGravity (pressureBased.cpp:141)
+}
+
+double momentum_eq(semo_ctx* c, semo_solve_stats* st) {
+    recon_boundary(c);
+    assemble_momentum(c);
+    solve_momentum(c, 3, st);
+    return momentum_update(c);
+}
+
+double pressure_eq(semo_ctx* c, semo_solve_stats* st) {
+    recon_boundary(c);
+    assemble_pressure(c);
+    solve_pressure(c, c->Bvec.p, c->Xvec.p, st);
+    return pressure_correct(c);
+}
+
+}  // namespace
+}  // namespace semo
+
+using namespace semo;
+
+extern "C" {
+
+int semo_ctx_create(int device, void* stream, semo_ctx** out) {
+    if (!out) return SEMO_EINVAL;
+    *out = nullptr;
+    semo_ctx* c = new semo_ctx;
+    int rc = guard(c, [&] {
+        int nd = 0;
+        SEMO_HIP(hipGetDeviceCount(&nd));
+        SEMO_CHECK(device >= 0 && device < nd, SEMO_EINVAL, "no such device");
+        SEMO_HIP(hipSetDevice(device));
+        c->device = device;
+        if (stream) c->stream = (hipStream_t)stream;
+        else { SEMO_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)); c->own_stream = true; }
+        c->sp.precision = SEMO_PREC_FP64; c->sp.tol_p = 1e-8; c->sp.tol_u = 1e-8;
+        c->sp.maxit_p = 500; c->sp.maxit_u = 500; c->sp.cheb_degree = 2; c->sp.coarse_size = 1000;
+    });
+    if (rc != SEMO_OK) { delete c; return rc; }
+    *out = c;
+    return SEMO_OK;
+}
+
+int semo_ctx_destroy(semo_ctx* c) {
+    if (!c) return SEMO_EINVAL;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    bool own = c->own_stream;
+    hipStream_t s = c->stream;
+    delete c;
+    if (own) (void)hipStreamDestroy(s);
+    return SEMO_OK;
+}
+
+const char* semo_ctx_last_error(const semo_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int64_t semo_ctx_device_bytes(const semo_ctx* c) {
+    if (!c) return 0;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return -1;
+    return (int64_t)(total_b - free_b);
+}
+
+int semo_mesh_upload(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        SEMO_HIP(hipSetDevice(c->device));
+        c->has_mesh = false; c->has_case = false; c->iterReal = 0;
+        c->amg_p = Amg();
+        mesh_setup(c, m, reorder);
+        geometry_compute(c);
+        fields_alloc(c);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int semo_mesh_get_perm(const semo_ctx* c, int32_t* perm, int32_t* inv) {
+    if (!c || !c->has_mesh) return SEMO_ESTATE;
+    if (perm) std::copy(c->perm.begin(), c->perm.end(), perm);
+    if (inv) std::copy(c->inv.begin(), c->inv.end(), inv);
+    return SEMO_OK;
+}
+
+int semo_mesh_get_face_map(const semo_ctx* c, int32_t* fmap, int8_t* flip) {
+    if (!c || !c->has_mesh) return SEMO_ESTATE;
+    if (fmap) std::copy(c->fperm.begin(), c->fperm.end(), fmap);
+    if (flip) std::copy(c->fflip.begin(), c->fflip.end(), flip);
+    return SEMO_OK;
+}
+
+int semo_mesh_get_connectivity(const semo_ctx* c, int32_t* owner, int32_t* nbr) {
+    if (!c || !c->has_mesh) return SEMO_ESTATE;
+    if (owner) std::copy(c->h_owner.begin(), c->h_owner.end(), owner);
+    if (nbr) std::copy(c->h_nbr.begin(), c->h_nbr.end(), nbr);
+    return SEMO_OK;
+}
+
+int semo_rcm(int32_t n, const int32_t* ptr, const int32_t* adj, int32_t* perm, int32_t* inv) {
+    if (n <= 0 || !ptr || !adj || !perm || !inv) return SEMO_EINVAL;
+    for (int i = 0; i < n; ++i) {
+        if (ptr[i + 1] < ptr[i]) return SEMO_EINVAL;
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) if (adj[k] < 0 || adj[k] >= n) return SEMO_EINVAL;
+    }
+    semo::genrcm(n, ptr, adj, perm, inv);
+    return SEMO_OK;
+}
+
+int semo_set_case(semo_ctx* c, const semo_case* cs, const semo_patch_bc* bcs) {
+    if (!c || !cs) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        SEMO_CHECK(cs->dt > 0 && cs->rho[0] > 0 && cs->rho[1] > 0 && cs->mu[0] > 0 && cs->mu[1] > 0,
+                   SEMO_EINVAL, "case: dt, rho and mu must be positive");
+        SEMO_CHECK(cs->sigma == 0.0, SEMO_EINVAL, "surface tension (curvature) is not available on this path");
+        for (int b = 0; b < c->npatch; ++b) {
+            const semo_patch_bc& p = bcs[b];
+            SEMO_CHECK(p.p_type == SEMO_BC_ZERO_GRADIENT || p.p_type == SEMO_BC_FIXED_VALUE, SEMO_EINVAL_BC,
+                       "pressure boundary type not defined");
+            SEMO_CHECK(p.u_type >= SEMO_BC_ZERO_GRADIENT && p.u_type <= SEMO_BC_INLET_OUTLET, SEMO_EINVAL_BC,
+                       "velocity boundary type not defined");
+            SEMO_CHECK(p.vf_type == SEMO_BC_ZERO_GRADIENT || p.vf_type == SEMO_BC_FIXED_VALUE ||
+                           p.vf_type == SEMO_BC_INLET_OUTLET, SEMO_EINVAL_BC,
+                       "volume-fraction boundary type not defined");
+        }
+        c->cs = *cs;
+        c->bcs.assign(bcs, bcs + c->npatch);
+        c->d_bcs.upload(c->bcs, c->stream);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+        c->has_case = true;
+        c->amg_p.built = false;
+    });
+}
+
+int semo_set_solver(semo_ctx* c, const semo_solver_params* p) {
+    if (!c || !p) return SEMO_EINVAL;
+    return guard(c, [&] {
+        SEMO_CHECK(p->tol_p > 0 && p->tol_u > 0 && p->maxit_p > 0 && p->maxit_u > 0, SEMO_EINVAL, "solver params");
+        SEMO_CHECK(p->precision == SEMO_PREC_FP64 || p->precision == SEMO_PREC_MIXED, SEMO_EINVAL, "precision");
+        bool rebuild = p->coarse_size != c->sp.coarse_size;
+        c->sp = *p;
+        if (c->sp.cheb_degree <= 0) c->sp.cheb_degree = 2;
+        if (c->sp.coarse_size <= 0) c->sp.coarse_size = 1000;
+        if (rebuild) c->amg_p.built = false;
+    });
+}
+
+int semo_field_set(semo_ctx* c, int f, const double* h, int64_t n) {
+    if (!c || !h) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        int comps;
+        DBuf<double>* d = cell_field(c, f, comps);
+        SEMO_CHECK(d && f < SEMO_G_CELL_VOLUME, SEMO_EINVAL, "field not settable");
+        SEMO_CHECK(n == (int64_t)c->nc * comps, SEMO_EINVAL, "field size");
+        if (f >= SEMO_F_SRCG0 && f <= SEMO_F_SRCG2) SEMO_CHECK(false, SEMO_EINVAL, "field not settable");
+        to_dev(c, *d, h, comps);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int semo_field_get(semo_ctx* c, int f, double* h, int64_t n) {
+    if (!c || !h) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        if (f >= SEMO_F_SRCG0 && f <= SEMO_F_SRCG2) {
+            SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
+            from_dev(c, c->srcG.p + (size_t)(f - SEMO_F_SRCG0) * c->nc, h, 1);
+            return;
+        }
+        if (f >= SEMO_F_X0 && f <= SEMO_F_X2) {
+            SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
+            from_dev(c, c->Xvec.p + (size_t)(f - SEMO_F_X0) * c->nc, h, 1);
+            return;
+        }
+        if (f >= SEMO_F_B0 && f <= SEMO_F_B2) {
+            SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
+            from_dev(c, c->Bvec.p + (size_t)(f - SEMO_F_B0) * c->nc, h, 1);
+            return;
+        }
+        if (f >= SEMO_G_FACE_AREA) {
+            const DBuf<double>* d = nullptr; int comps = 1; bool odd = false;
+            switch (f) {
+                case SEMO_G_FACE_AREA: d = &c->f_area; break;
+                case SEMO_G_FACE_NORMAL: d = &c->f_nrm; comps = 3; odd = true; break;
+                case SEMO_G_FACE_XYZ: d = &c->f_xyz; comps = 3; break;
+                case SEMO_G_FACE_WC: d = &c->f_wC; break;
+                case SEMO_G_FACE_MAGPN: d = &c->f_magPN; break;
+                case SEMO_G_FACE_ALPHAF: d = &c->f_alpha; break;
+                case SEMO_G_FACE_EPN: d = &c->f_ePN; comps = 3; odd = true; break;
+                case SEMO_G_FACE_SKEW: d = &c->f_skew; comps = 3; break;
+                default: SEMO_CHECK(false, SEMO_EINVAL, "unknown field");
+            }
+            SEMO_CHECK(n == (int64_t)c->nf * comps, SEMO_EINVAL, "field size");
+            face_from_dev(c, *d, h, comps, odd);
+            if (f == SEMO_G_FACE_WC)   // orientation of flipped faces
+                for (int k = 0; k < c->nif; ++k) if (c->fflip[k]) h[c->fperm[k]] = 1.0 - h[c->fperm[k]];
+            return;
+        }
+        int comps;
+        DBuf<double>* d = cell_field(c, f, comps);
+        SEMO_CHECK(d && d->n >= (size_t)c->nc * comps, SEMO_EINVAL, "unknown or empty field");
+        SEMO_CHECK(n == (int64_t)c->nc * comps, SEMO_EINVAL, "field size");
+        from_dev(c, d->p, h, comps);
+    });
+}
+
+int semo_calc_momentum_eqs(semo_ctx* c, double* norm, semo_solve_stats* st) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        double nv = momentum_eq(c, st);
+        if (norm) *norm = nv;
+    });
+}
+
+int semo_calc_pressure_eq(semo_ctx* c, double* norm, semo_solve_stats* st) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        double nv = pressure_eq(c, st);
+        if (norm) *norm = nv;
+    });
+}
+
+int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
+    if (!c || ncorr < 1) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        step_setup(c);
+        for (int it = 0; it < ncorr; ++it) {
+            semo_solve_stats st{};
+            double n0 = momentum_eq(c, &st);
+            double n1 = pressure_eq(c, &st);
+            c->last_stats = st;
+            if (norms) { norms[2 * it] = n0; norms[2 * it + 1] = n1; }
+        }
+        c->iterReal++;
+    });
+}
+
+int semo_assemble(semo_ctx* c, int eq) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        SEMO_CHECK(eq == 0 || eq == 1, SEMO_EINVAL, "eq must be 0 (momentum) or 1 (pressure)");
+        recon_boundary(c);
+        if (eq == 0) assemble_momentum(c); else assemble_pressure(c);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int semo_matrix_get(semo_ctx* c, int64_t* nnz, int32_t* rows, int32_t* cols, double* vals, double* b) {
+    if (!c || !nnz) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        SEMO_CHECK(c->last_eq >= 0, SEMO_ESTATE, "nothing assembled");
+        std::vector<double> val(c->Amat.val.n), diag(c->nc);
+        c->Amat.val.download(val.data(), val.size(), c->stream);
+        c->Amat.diag.download(diag.data(), diag.size(), c->stream);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+        int64_t m = c->nc;
+        const SellPattern& p = c->pat;
+        // real slots only; entries dropped by the reference (exact zeros) skipped
+        std::vector<int32_t> r, cc; std::vector<double> v;
+        for (int i = 0; i < c->nc; ++i) { r.push_back(c->perm[i]); cc.push_back(c->perm[i]); v.push_back(diag[i]); }
+        for (int f = 0; f < c->nif; ++f) {
+            int so = c->h_slot_own[f], sn = c->h_slot_nbr[f];
+            int o = c->h_owner[f], n = c->h_nbr[f];
+            if (!(c->last_eq == 0 && val[so] == 0.0)) { r.push_back(c->perm[o]); cc.push_back(c->perm[n]); v.push_back(val[so]); }
+            if (!(c->last_eq == 0 && val[sn] == 0.0)) { r.push_back(c->perm[n]); cc.push_back(c->perm[o]); v.push_back(val[sn]); }
+        }
+        (void)p;
+        m = (int64_t)r.size();
+        if (rows) {
+            SEMO_CHECK(*nnz >= m, SEMO_EINVAL, "nnz buffer too small");
+            std::copy(r.begin(), r.end(), rows);
+            std::copy(cc.begin(), cc.end(), cols);
+            std::copy(v.begin(), v.end(), vals);
+            if (b) {
+                int nr = c->last_eq == 0 ? 3 : 1;
+                for (int k = 0; k < nr; ++k) from_dev(c, c->Bvec.p + (size_t)k * c->nc, b + (size_t)k * c->nc, 1);
+            }
+        }
+        *nnz = m;
+    });
+}
+
+int semo_least_square(semo_ctx* c, const double* phi, const double* face_r, double* grad) {
+    if (!c || !phi || !face_r || !grad) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        DBuf<double> dphi, dfr, dg;
+        to_dev(c, dphi, phi, 1);
+        std::vector<double> fr(c->nf);
+        for (int k = 0; k < c->nf; ++k) fr[k] = face_r[c->fperm[k]];
+        dfr.upload(fr, c->stream);
+        dg.alloc(3 * (size_t)c->nc);
+        lsq_standalone(c, dphi.p, dfr.p, dg.p);
+        from_dev(c, dg.p, grad, 3);
+    });
+}
+
+int semo_cell_min_max(semo_ctx* c, const double* phi, double* vmax, double* vmin) {
+    if (!c || !phi || !vmax || !vmin) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        DBuf<double> dphi, a, b;
+        to_dev(c, dphi, phi, 1);
+        a.alloc(c->nc); b.alloc(c->nc);
+        minmax_standalone(c, dphi.p, a.p, b.p);
+        from_dev(c, a.p, vmax, 1);
+        from_dev(c, b.p, vmin, 1);
+    });
+}
+
+int semo_solve_coo(semo_ctx* c, const char* eq, int32_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                   const double* vals, int32_t nrhs, const double* b, double* x, semo_solve_stats* st) {
+    if (!c || !rows || !cols || !vals || !b || !x) return SEMO_EINVAL;
+    return guard(c, [&] {
+        SEMO_HIP(hipSetDevice(c->device));
+        solve_coo(c, eq, n, nnz, rows, cols, vals, nrhs, b, x, st);
+    });
+}
+
+int semo_profile_enable(semo_ctx* c, int on) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] { c->prof.flush(); c->prof.on = on != 0; });
+}
+
+int semo_profile_reset(semo_ctx* c) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] { c->prof.flush(); c->prof.stats.clear(); });
+}
+
+int semo_profile_entry(semo_ctx* c, int i, char* name, int64_t* count, double* ms, double* bytes) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        c->prof.flush();
+        SEMO_CHECK(i >= 0 && i < (int)c->prof.stats.size(), SEMO_EINVAL, "no such entry");
+        auto it = c->prof.stats.begin();
+        std::advance(it, i);
+        if (name) { strncpy(name, it->first.c_str(), 63); name[63] = 0; }
+        if (count) *count = it->second.count;
+        if (ms) *ms = it->second.ms;
+        if (bytes) *bytes = it->second.bytes;
+    });
+}
+
+}  // extern "C"

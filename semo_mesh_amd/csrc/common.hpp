@@ -1,0 +1,241 @@
+// Internal declarations of libsemo_gpu (MI355X / gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/semo_gpu.h"
+
+namespace semo {
+
+// ---------------------------------------------------------------- errors ----
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SEMO_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::semo::Error(SEMO_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define SEMO_CHECK(cond, code, msg)                                                      \
+    do {                                                                                 \
+        if (!(cond)) throw ::semo::Error((code), (msg));                                 \
+    } while (0)
+
+// device error flags raised by kernels (bit set in ctx->d_flags)
+enum : int { FLAG_ZERO_AREA = 1, FLAG_NEG_VOLUME = 2, FLAG_SINGULAR_LSQ = 4,
+             FLAG_BAD_FACE = 8, FLAG_ZERO_DIAG = 16 };
+
+// ------------------------------------------------------------ device buffer -
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr; n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count) SEMO_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    // grow-only allocation (contents not preserved when it grows)
+    void ensure(size_t count) {
+        if (count > n || !p) alloc(count);
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) SEMO_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    void download(T* h, size_t count, hipStream_t s) const {
+        if (count) SEMO_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    void zero(hipStream_t s) {
+        if (n) SEMO_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// --------------------------------------------------------------- profiler ---
+struct KernelStat { int64_t count = 0; double ms = 0; double bytes = 0; };
+struct Profiler {
+    bool on = false;
+    std::map<std::string, KernelStat> stats;
+    struct Pending { std::string name; hipEvent_t a, b; double bytes; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get();
+    void begin(hipStream_t s, hipEvent_t& a);
+    void end(hipStream_t s, const char* name, hipEvent_t a, double bytes);
+    void flush();   // resolves pending events (synchronizes)
+    ~Profiler();
+};
+
+// ------------------------------------------------------- sliced-ELL matrix ---
+// Off-diagonal entries in slices of 64 rows (one wavefront per slice, one lane
+// per row), column-major inside a slice, plus a separate diagonal.  Padding
+// entries have col = row and val = 0.
+constexpr int SL = 64;
+struct SellPattern {
+    int n = 0, nslices = 0;
+    std::vector<int> sl_ptr;    // element offset of each slice (nslices+1)
+    std::vector<int> col;       // padded columns
+    int64_t nnz_offdiag = 0;    // real off-diagonal entries
+};
+template <class T>
+struct DevSell {
+    int n = 0, nslices = 0;
+    int64_t nnz = 0;            // real entries incl. diagonal
+    DBuf<int> sl_ptr, col;
+    DBuf<T> val, diag;
+    size_t padded() const { return col.n; }
+};
+
+// ------------------------------------------------------------ AMG level -----
+struct AmgLevel {
+    int n = 0;
+    DevSell<double> A64;        // operator (fp64)
+    DevSell<float> A32;         // operator (fp32 copy, mixed precision)
+    DBuf<int> agg;              // fine row -> aggregate (coarse row), for R/P of next
+    DevSell<double> P64, R64;   // smoothed prolongation / restriction (next level)
+    DevSell<float> P32, R32;
+    DBuf<double> dinv64; DBuf<float> dinv32;
+    double lmax = 2.0;          // spectral bound of D^-1 A
+    // work vectors
+    DBuf<double> x64, b64, r64, t64, u64;
+    DBuf<float> x32, b32, r32, t32, u32;
+};
+
+struct Amg {
+    std::vector<AmgLevel> lv;
+    int nc_dense = 0;
+    DBuf<double> coarse_inv;    // dense inverse of the coarsest operator (fp64)
+    double setup_ms = 0;
+    bool built = false;
+    std::vector<double> key;    // host copy of the fine values the hierarchy was built from
+};
+
+}  // namespace semo
+
+// ------------------------------------------------------------------ context --
+struct semo_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    semo::Profiler prof;
+
+    // ---- mesh (device numbering) ----
+    bool has_mesh = false;
+    int np = 0, nc = 0, nf = 0, nif = 0, nbf = 0, npatch = 0;
+    std::vector<int> perm, inv;            // cells: new->old, old->new
+    std::vector<int> fperm;                // faces: new->old
+    std::vector<int8_t> fflip;
+    std::vector<int> patch_start, patch_nfaces;
+    std::vector<int> h_owner, h_nbr;       // device numbering (host copies)
+    std::vector<int> h_cf_ptr, h_cf_idx;   // cell -> faces (host copy)
+
+    semo::DBuf<double> d_pts;
+    semo::DBuf<int> d_owner, d_nbr, d_fptr, d_fpts;
+    semo::DBuf<int> d_cf_ptr, d_cf_idx;    // cell -> faces, ascending
+    semo::DBuf<int> d_st_ptr, d_st_idx;    // vertex stencil
+    semo::DBuf<int> d_bpatch;              // boundary face -> patch
+    semo::DBuf<int> d_flags;               // kernel error flags
+    // geometry
+    semo::DBuf<double> f_area, f_nrm, f_xyz, f_wC, f_magPN, f_alpha, f_ePN, f_skew, f_pf, f_nf;
+    semo::DBuf<double> f_vsn, f_cc;        // scratch of the geometry pass
+    semo::DBuf<double> c_vol, c_xyz, c_lsq;
+
+    // ---- matrix pattern ----
+    semo::SellPattern pat;
+    std::vector<int> h_slot_own, h_slot_nbr;   // per internal face
+    semo::DBuf<int> d_slot_own, d_slot_nbr, d_tslot;
+    semo::DevSell<double> Amat;               // assembled system (values per equation)
+    semo::DBuf<double> Bvec;                  // [3][n] right-hand sides
+    semo::DBuf<double> Xvec;                  // [3][n] solutions
+    semo::DBuf<double> fscr;                  // per-face scratch [8][nif]
+    int last_eq = -1;
+
+    // ---- case ----
+    semo_case cs{};
+    std::vector<semo_patch_bc> bcs;
+    semo::DBuf<semo_patch_bc> d_bcs;
+    bool has_case = false;
+    int iterReal = 0;
+
+    // ---- fields (device numbering) ----
+    semo::DBuf<double> P, U, V, W, VF, Rho, Mu, oldP, oldU, oldV, oldW, oldVF, Qm0, Qm1, Qm2,
+        srcG, srcS, kappa, alphaP;
+    semo::DBuf<double> gP, gU, gV, gW, gVF;   // [3n] interleaved
+    semo::DBuf<double> mxP, mnP, mxU, mnU, mxV, mnV, mxW, mnW;
+    semo::DBuf<double> bP, bUL, bUR, bVF, bMu, bRho;   // boundary faces (bUL/bUR: [3][nbf])
+    semo::DBuf<double> gpp, gtmp;              // p' Green-Gauss gradient
+    semo::DBuf<double> red;                    // reduction partials
+    semo::DBuf<double> scal;                   // device scalars
+
+    // ---- solver ----
+    semo_solver_params sp{};
+    semo::Amg amg_p;
+    semo::DBuf<double> amg_key_val, amg_key_diag;   // values the pressure hierarchy was built from
+    semo::DBuf<double> dinv_mom;
+    semo::DBuf<int> d_flag1;
+    semo::DBuf<double> kv64;                   // Krylov work (fp64)
+    semo::DBuf<float> kv32;
+    semo_solve_stats last_stats{};
+};
+
+namespace semo {
+// mesh.cpp
+void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder);
+void genrcm(int n, const int* ptr, const int* adj, int* perm, int* inv);
+SellPattern sell_from_csr(int n, const std::vector<int>& ptr, const std::vector<int>& col,
+                          std::vector<int>* pos_of_csr);
+// geometry.hip
+void geometry_compute(semo_ctx* c);
+// assembly.hip
+void fields_alloc(semo_ctx* c);
+void eos_transport(semo_ctx* c);
+void recon_boundary(semo_ctx* c);
+void source_gravity(semo_ctx* c);
+void grad_minmax(semo_ctx* c, bool with_p_minmax);
+void assemble_momentum(semo_ctx* c);
+void assemble_pressure(semo_ctx* c);
+double momentum_update(semo_ctx* c);
+double pressure_correct(semo_ctx* c);
+void lsq_standalone(semo_ctx* c, const double* d_phi, const double* d_faceR, double* d_grad);
+void minmax_standalone(semo_ctx* c, const double* d_phi, double* d_mx, double* d_mn);
+void copy_d(semo_ctx* c, double* dst, const double* src, size_t n);
+// linalg.hip
+template <class T> void sell_upload(DevSell<T>& M, const SellPattern& p, const std::vector<double>& vals,
+                                    const std::vector<double>& diag, hipStream_t s);
+void spmv64(semo_ctx* c, const DevSell<double>& A, const double* x, double* y);
+double dot64(semo_ctx* c, const double* a, const double* b, int n);
+double norm64(semo_ctx* c, const double* a, int n);
+// krylov.cpp / amg.cpp
+void amg_build(semo_ctx* c, Amg& amg, const DevSell<double>& A);
+void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* st);
+void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st);
+}  // namespace semo

@@ -1,0 +1,201 @@
+"""Python front end of the MI355X assemble-and-solve path.
+
+``GpuSolver`` mirrors the entry points of SEMO's ``SEMO_Solvers_Builder`` /
+``SEMO_Utility_Math`` for the pressure-based incompressible path
+(src/solvers/build.h, src/math/math.h): the same operation names, argument
+meaning and return values, over one device context that keeps the mesh and
+every field resident in HBM.  Arrays are numpy arrays in the caller's cell and
+face numbering.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import F, COMPONENTS, FACE_FIELDS, SemoError
+
+PREC = {"fp64": 0, "mixed": 1}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class GpuSolver:
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self._L = _lib.lib()
+        h = C.c_void_p()
+        rc = self._L.semo_ctx_create(device, C.c_void_p(stream) if stream else None, C.byref(h))
+        if rc != 0:
+            raise SemoError(rc, "semo_ctx_create failed")
+        self._h = h
+        self.mesh = None
+        self.last_stats = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.semo_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ok(self, rc):
+        if rc != 0:
+            raise SemoError(rc, self._L.semo_ctx_last_error(self._h).decode())
+
+    # ---- mesh / case / solver ---------------------------------------------------
+    def upload_mesh(self, mesh, reorder: bool = True):
+        """SEMO_Mesh_Geometric::init + setStencil + initLeastSquare, plus the
+        RCM reordering of storage (lib_RCM)."""
+        self._keep = [np.ascontiguousarray(mesh.points, dtype=np.float64)] + [
+            np.ascontiguousarray(a, dtype=np.int32) for a in
+            (mesh.owner, mesh.neighbour, mesh.face_ptr, mesh.face_pts, mesh.cell_ptr,
+             mesh.cell_pts, mesh.patch_start, mesh.patch_nfaces)]
+        k = self._keep
+        d = _lib.MeshDesc(mesh.n_points, mesh.n_cells, mesh.n_faces, mesh.n_internal_faces,
+                          len(mesh.patch_names), *[a.ctypes.data for a in k])
+        self._ok(self._L.semo_mesh_upload(self._h, C.byref(d), 1 if reorder else 0))
+        self.mesh = mesh
+        del self._keep
+
+    def set_case(self, case):
+        m = self.mesh
+        cs = _lib.CaseDesc((C.c_double * 2)(*case.rho), (C.c_double * 2)(*case.mu), case.sigma,
+                           (C.c_double * 3)(*case.g), case.dt, case.urf_mom, case.urf_p,
+                           case.urf_pu, case.limit)
+        bcs = (_lib.PatchBC * len(m.patch_names))()
+        for i, nm in enumerate(m.patch_names):
+            b = case.bcs[nm]
+            bcs[i] = _lib.PatchBC(b.p_type, b.p_value, b.u_type, (C.c_double * 3)(*b.u_value),
+                                  b.vf_type, b.vf_value)
+        self._ok(self._L.semo_set_case(self._h, C.byref(cs), bcs))
+
+    def set_solver(self, precision="fp64", tol_p=1e-8, tol_u=1e-8, maxit_p=500, maxit_u=500,
+                   cheb_degree=2, coarse_size=1000):
+        p = _lib.SolverParams(PREC[precision], tol_p, tol_u, maxit_p, maxit_u, cheb_degree, coarse_size)
+        self._ok(self._L.semo_set_solver(self._h, C.byref(p)))
+
+    # ---- fields ---------------------------------------------------------------------------
+    def _size(self, name):
+        n = self.mesh.n_faces if name in FACE_FIELDS else self.mesh.n_cells
+        return n * COMPONENTS.get(name, 1)
+
+    def set_field(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        self._ok(self._L.semo_field_set(self._h, F[name], _ptr(a), a.size))
+
+    def get_field(self, name):
+        n = self._size(name)
+        a = np.empty(n, dtype=np.float64)
+        self._ok(self._L.semo_field_get(self._h, F[name], _ptr(a), n))
+        c = COMPONENTS.get(name, 1)
+        return a.reshape(-1, c) if c > 1 else a
+
+    def permutation(self):
+        n = self.mesh.n_cells
+        perm = np.empty(n, dtype=np.int32); inv = np.empty(n, dtype=np.int32)
+        self._ok(self._L.semo_mesh_get_perm(self._h, _ptr(perm), _ptr(inv)))
+        return perm, inv
+
+    def face_map(self):
+        m = self.mesh
+        fm = np.empty(m.n_faces, dtype=np.int32); fl = np.empty(m.n_faces, dtype=np.int8)
+        self._ok(self._L.semo_mesh_get_face_map(self._h, _ptr(fm), _ptr(fl)))
+        return fm, fl
+
+    def connectivity(self):
+        m = self.mesh
+        o = np.empty(m.n_faces, dtype=np.int32); nb = np.empty(m.n_internal_faces, dtype=np.int32)
+        self._ok(self._L.semo_mesh_get_connectivity(self._h, _ptr(o), _ptr(nb)))
+        return o, nb
+
+    # ---- SEMO_Solvers_Builder entry points -------------------------------------------------
+    def calcMomentumEqs(self):
+        n = C.c_double(); st = _lib.SolveStats()
+        self._ok(self._L.semo_calc_momentum_eqs(self._h, C.byref(n), C.byref(st)))
+        self.last_stats = st
+        return n.value
+
+    def calcPressureEq(self):
+        n = C.c_double(); st = _lib.SolveStats()
+        self._ok(self._L.semo_calc_pressure_eq(self._h, C.byref(n), C.byref(st)))
+        self.last_stats = st
+        return n.value
+
+    def incompressiblePressureBased(self, n_correctors: int = 1):
+        norms = np.zeros(2 * n_correctors)
+        self._ok(self._L.semo_incompressible_pressure_based(self._h, n_correctors, _ptr(norms)))
+        return norms
+
+    def assemble(self, eq: str):
+        self._ok(self._L.semo_assemble(self._h, {"momentum": 0, "pressure": 1}[eq]))
+
+    def matrix(self, eq: str):
+        """COO (rows, cols, vals) and right-hand sides of the last assembly."""
+        nnz = C.c_int64(0)
+        self._ok(self._L.semo_matrix_get(self._h, C.byref(nnz), None, None, None, None))
+        r = np.empty(nnz.value, dtype=np.int32); c = np.empty(nnz.value, dtype=np.int32)
+        v = np.empty(nnz.value, dtype=np.float64)
+        nr = 3 if eq == "momentum" else 1
+        b = np.empty(nr * self.mesh.n_cells, dtype=np.float64)
+        self._ok(self._L.semo_matrix_get(self._h, C.byref(nnz), _ptr(r), _ptr(c), _ptr(v), _ptr(b)))
+        return r, c, v, b.reshape(nr, -1)
+
+    # ---- SEMO_Utility_Math ------------------------------------------------------------------
+    def calcLeastSquare(self, phi, face_r):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        fr = np.ascontiguousarray(face_r, dtype=np.float64)
+        g = np.empty(3 * self.mesh.n_cells)
+        self._ok(self._L.semo_least_square(self._h, _ptr(phi), _ptr(fr), _ptr(g)))
+        return g.reshape(-1, 3)
+
+    def setCellVarMinMax(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        mx = np.empty(self.mesh.n_cells); mn = np.empty(self.mesh.n_cells)
+        self._ok(self._L.semo_cell_min_max(self._h, _ptr(phi), _ptr(mx), _ptr(mn)))
+        return mx, mn
+
+    def solveAMGCL(self, equation, rows, cols, vals, b):
+        """Seam 2: solve a COO system; b is (nrhs, n) or (n,)."""
+        b = np.atleast_2d(np.ascontiguousarray(b, dtype=np.float64))
+        nrhs, n = b.shape
+        r = np.ascontiguousarray(rows, dtype=np.int32); c = np.ascontiguousarray(cols, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        x = np.empty_like(b)
+        st = _lib.SolveStats()
+        self._ok(self._L.semo_solve_coo(self._h, equation.encode(), n, len(r), _ptr(r), _ptr(c), _ptr(v),
+                                        nrhs, _ptr(b), _ptr(x), C.byref(st)))
+        self.last_stats = st
+        return x if nrhs > 1 else x[0]
+
+    # ---- profiling ---------------------------------------------------------------------------
+    def profile(self, on=True):
+        self._ok(self._L.semo_profile_enable(self._h, 1 if on else 0))
+
+    def profile_reset(self):
+        self._ok(self._L.semo_profile_reset(self._h))
+
+    def profile_stats(self):
+        out = {}
+        i = 0
+        name = C.create_string_buffer(64)
+        cnt = C.c_int64(); ms = C.c_double(); by = C.c_double()
+        while self._L.semo_profile_entry(self._h, i, name, C.byref(cnt), C.byref(ms), C.byref(by)) == 0:
+            out[name.value.decode()] = dict(count=cnt.value, ms=ms.value, bytes=by.value)
+            i += 1
+        return out
+
+
+def rcm(adj_ptr, adj):
+    """Standalone GENRCM (lib_RCM): perm[new] = old, inv[old] = new."""
+    L = _lib.lib()
+    n = len(adj_ptr) - 1
+    p = np.ascontiguousarray(adj_ptr, dtype=np.int32); a = np.ascontiguousarray(adj, dtype=np.int32)
+    perm = np.empty(n, dtype=np.int32); inv = np.empty(n, dtype=np.int32)
+    rc = L.semo_rcm(n, _ptr(p), _ptr(a), _ptr(perm), _ptr(inv))
+    if rc != 0:
+        raise SemoError(rc, "semo_rcm")
+    return perm, inv

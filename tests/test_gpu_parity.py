@@ -1,0 +1,242 @@
+"""HIP path (libsemo_gpu.so, called through its C ABI) against the CPU
+oracle and the reference's golden vectors.
+
+Bars: on a mesh uploaded without renumbering the FP64 assembly reproduces the
+oracle bit for bit (both follow the reference's per-cell summation order;
+the library is built with -ffp-contract=off); with RCM renumbering the
+per-cell order of the sums changes, so results agree to round-off
+(relative 1e-12).  Solved fields agree with the oracle to relative L2 1e-10
+(FP64 path) / 1e-6 (mixed path) at tight solver tolerances.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import GOLDEN_CASES, golden_case, load_golden
+from semo_mesh_amd.cases import cavity_case
+from semo_mesh_amd.mesh import cavity_mesh, hex_mesh
+
+pytestmark = pytest.mark.gpu
+
+GEOM = ["face_area", "face_normal", "face_xyz", "face_wC", "face_magPN", "face_alphaF",
+        "face_ePN", "face_skew", "cell_volume", "cell_xyz", "cell_lsq"]
+
+
+def _gpu(mesh, case, reorder):
+    from semo_mesh_amd.solver import GpuSolver
+    g = GpuSolver(0)
+    g.upload_mesh(mesh, reorder=reorder)
+    g.set_case(case)
+    return g
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel(); b = np.asarray(b, dtype=np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+@pytest.mark.parametrize("reorder", [False, True])
+def test_geometry_vs_reference(name, reorder):
+    d, mesh = load_golden(name)
+    g = _gpu(mesh, golden_case(d), reorder)
+    for k in GEOM:
+        a = g.get_field(k).ravel()
+        if reorder:
+            assert _rel(a, d[k]) < 1e-13, k
+        else:
+            # sqrt is correctly rounded on both sides; the reference's sqrtl
+            # (x87 extended) can differ by one double-rounding ulp
+            assert np.allclose(a, d[k], rtol=4e-16, atol=0), k
+    g.close()
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_rcm_permutation_and_connectivity_bit_exact(name):
+    d, mesh = load_golden(name)
+    g = _gpu(mesh, golden_case(d), True)
+    perm, inv = g.permutation()
+    assert np.array_equal(perm, d["rcm_perm"]) and np.array_equal(inv, d["rcm_inv"])
+    # device connectivity is the original one renumbered
+    own, nbr = g.connectivity()
+    fmap, flip = g.face_map()
+    nif = mesh.n_internal_faces
+    o0 = mesh.owner[fmap]; n0 = np.full(mesh.n_faces, -1); n0[:nif] = mesh.neighbour[fmap[:nif]]
+    a = np.where(flip[:nif] == 1, n0[:nif], o0[:nif]); b = np.where(flip[:nif] == 1, o0[:nif], n0[:nif])
+    assert np.array_equal(perm[own[:nif]], a) and np.array_equal(perm[nbr], b)
+    assert np.array_equal(perm[own[nif:]], o0[nif:])
+    assert np.all(own[:nif] < nbr)
+    assert sorted(fmap) == list(range(mesh.n_faces))
+    g.close()
+
+
+def _state(o, g, seed, nc, vf=None):
+    rng = np.random.default_rng(seed)
+    f = dict(P=rng.normal(size=nc) * 10.0, U=rng.normal(size=nc), V=rng.normal(size=nc),
+             W=rng.normal(size=nc))
+    f["VF"] = np.ones(nc) if vf is None else vf
+    for k, v in f.items():
+        o.set(k, v)
+    o.eos()
+    for k in ("U", "V", "W"):
+        o.set("old" + k, o.get(k))
+        o.set("Qm" + "UVW".index(k).__str__(), o.get(k))
+    for k in ("P", "U", "V", "W", "VF", "Rho", "mu"):
+        g.set_field(k, o.get(k))
+    for k in ("U", "V", "W"):
+        g.set_field("old" + k, o.get(k))
+        g.set_field("Qm" + str("UVW".index(k)), o.get(k))
+
+
+def _csr(r, c, v, n):
+    return sp.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+@pytest.mark.parametrize("eq", ["momentum", "pressure"])
+def test_assembly_vs_oracle(oracle_lib, eq, reorder):
+    mesh = hex_mesh(7, 6, 5, jitter=0.1, seed=21)
+    case = cavity_case(); case.mu = (0.05, 0.05)
+    o = oracle_lib.Oracle(mesh, case)
+    g = _gpu(mesh, case, reorder)
+    _state(o, g, 5, mesh.n_cells)
+    o.recon(); o.gravity()
+    (o.assemble_momentum if eq == "momentum" else o.assemble_pressure)()
+    g.assemble(eq)
+    r, c, v = o.coo()
+    A0 = _csr(r, c, v, mesh.n_cells)
+    gr, gc, gv, gb = g.matrix(eq)
+    A1 = _csr(gr, gc, gv, mesh.n_cells)
+    nb = 3 if eq == "momentum" else 1
+    for f in ("gradP", "gradU", "gradV", "gradW"):
+        a, b = g.get_field(f), o.get(f).reshape(-1, 3)
+        assert (np.array_equal(a, b) if not reorder else _rel(a, b) < 1e-12), f
+    if not reorder:
+        assert (A1 != A0).nnz == 0
+        for k in range(nb):
+            assert np.array_equal(gb[k], o.get(f"b{k}")), k
+    else:
+        assert abs(A1 - A0).max() <= 1e-12 * abs(A0).max()
+        for k in range(nb):
+            assert _rel(gb[k], o.get(f"b{k}")) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["cavity4_jitter_random", "box654_jitter_2x2", "orifice_lid"])
+def test_assembly_replay_vs_reference(name):
+    """The reference's own COO systems, replayed from its call-time fields."""
+    d, mesh = load_golden(name)
+    g = _gpu(mesh, golden_case(d), False)
+    k = 0
+    while f"call{k}_momentum_rows" in d or f"call{k}_pressure_rows" in d:
+        eq = "momentum" if f"call{k}_momentum_rows" in d else "pressure"
+        p = f"call{k}_{eq}_"
+        for f in ("P", "U", "V", "W", "VF", "Rho", "mu", "oldU", "oldV", "oldW", "Qm0", "Qm1", "Qm2"):
+            g.set_field(f, d[p + f])
+        g.assemble(eq)
+        gr, gc, gv, gb = g.matrix(eq)
+        n = mesh.n_cells
+        A1 = _csr(gr, gc, gv, n)
+        A0 = _csr(d[p + "rows"], d[p + "cols"], d[p + "vals"], n)
+        assert (A1 != A0).nnz == 0, p
+        for q in range(3 if eq == "momentum" else 1):
+            assert np.array_equal(gb[q], d[p + f"b{q}"]), p
+        k += 1
+    g.close()
+
+
+def test_lsq_and_minmax_vs_oracle(oracle_lib):
+    mesh = hex_mesh(9, 7, 6, jitter=0.1, seed=3)
+    case = cavity_case()
+    o = oracle_lib.Oracle(mesh, case)
+    for reorder in (False, True):
+        g = _gpu(mesh, case, reorder)
+        rng = np.random.default_rng(1)
+        phi = rng.normal(size=mesh.n_cells); fr = rng.normal(size=mesh.n_faces)
+        a = g.calcLeastSquare(phi, fr); b = o.lsq(phi, fr)
+        assert (np.array_equal(a, b) if not reorder else _rel(a, b) < 1e-12)
+        mx, mn = g.setCellVarMinMax(phi); mx0, mn0 = o.minmax(phi)
+        assert np.array_equal(mx, mx0) and np.array_equal(mn, mn0)
+        g.close()
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_simple_run_vs_reference(name):
+    """Full SIMPLE run on the device against the reference's final fields (the
+    reference solves to AMGCL's 1e-9, so 1e-7 relative)."""
+    d, mesh = load_golden(name)
+    g = _gpu(mesh, golden_case(d), True)
+    g.set_solver("fp64", tol_p=1e-13, tol_u=1e-13, maxit_p=2000, maxit_u=2000)
+    nc = mesh.n_cells
+    if "init" in d:
+        for k, f in enumerate(("P", "U", "V", "W", "VF")):
+            g.set_field(f, d["init"][k * nc:(k + 1) * nc])
+    for _ in range(int(d["steps"])):
+        g.incompressiblePressureBased(int(d["correctors"]))
+    vel = np.sqrt(sum(np.linalg.norm(d["final_" + f]) ** 2 for f in "UVW"))
+    for f in ("P", "U", "V", "W"):
+        a, b = g.get_field(f), d["final_" + f]
+        scale = np.linalg.norm(b) if f == "P" else vel
+        assert np.linalg.norm(a - b) <= 1e-7 * scale, f
+    g.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_simple_iterations_vs_oracle(oracle_lib, precision, tol):
+    """Three SIMPLE iterations on a jittered 12^3 cavity: every field within
+    relative L2 1e-10 (FP64) / 1e-6 (mixed) of the oracle, both solved to a
+    relative residual of 1e-13 (FP64) / 1e-10 (mixed)."""
+    mesh = cavity_mesh(12, jitter=0.1)
+    case = cavity_case(); case.mu = (0.01, 0.01)
+    o = oracle_lib.Oracle(mesh, case)
+    o.set_solver(1e-14, 10000)
+    g = _gpu(mesh, case, True)
+    g.set_solver(precision, tol_p=1e-13 if precision == "fp64" else 1e-10, tol_u=1e-13,
+                 maxit_p=3000, maxit_u=3000)
+    for _ in range(3):
+        o.step(1)
+        g.incompressiblePressureBased(1)
+    vel = np.sqrt(sum(np.linalg.norm(o.get(f)) ** 2 for f in "UVW"))
+    for f in ("P", "U", "V", "W"):
+        a, b = g.get_field(f), o.get(f)
+        scale = np.linalg.norm(b) if f == "P" else vel
+        assert np.linalg.norm(a - b) <= tol * scale, (f, np.linalg.norm(a - b) / scale)
+    g.close()
+
+
+def test_solve_coo_seam():
+    """solveAMGCL seam: a jittered-mesh pressure system and a momentum system
+    solved from COO, checked by their FP64 residuals."""
+    from semo_mesh_amd.solver import GpuSolver
+    d, mesh = load_golden("box654_jitter_2x2")
+    g = GpuSolver(0)
+    g.set_solver("fp64", tol_p=1e-12, tol_u=1e-12)
+    for eq in ("pressure", "momentum"):
+        p = "call1_pressure_" if eq == "pressure" else "call0_momentum_"
+        n = mesh.n_cells
+        A = _csr(d[p + "rows"], d[p + "cols"], d[p + "vals"], n)
+        nb = 1 if eq == "pressure" else 3
+        b = np.stack([d[p + f"b{k}"] for k in range(nb)])
+        x = np.atleast_2d(g.solveAMGCL(eq, d[p + "rows"], d[p + "cols"], d[p + "vals"], b))
+        for k in range(nb):
+            assert np.linalg.norm(b[k] - A @ x[k]) <= 1e-11 * np.linalg.norm(b[k])
+            xr = d[p + f"x{k}"]
+            assert np.linalg.norm(x[k] - xr) <= 1e-7 * np.linalg.norm(xr)
+    g.close()
+
+
+def test_error_paths():
+    from semo_mesh_amd.solver import GpuSolver
+    from semo_mesh_amd._lib import SemoError
+    mesh = cavity_mesh(3)
+    g = GpuSolver(0)
+    with pytest.raises(SemoError):
+        g.calcMomentumEqs()                        # no mesh yet
+    g.upload_mesh(mesh)
+    case = cavity_case()
+    case.bcs["xmin"].u_type = 99                   # unknown boundary type
+    with pytest.raises(SemoError) as e:
+        g.set_case(case)
+    assert e.value.code == 2
+    g.close()
