@@ -1,0 +1,164 @@
+"""Throughput of the SEMO assemble-and-solve hot path on one MI355X.
+
+Workload (the 2.1M-cell configuration): lid-driven cavity on the unit cube,
+128^3 hexahedra with every interior vertex jittered by a seeded U(-0.1h, 0.1h),
+RCM-reordered, rho = 1000, mu = 1e-3, lid U = (1,0,0), dt = 0.01.  A step is
+one SIMPLE iteration of SEMO's pressure-based solver: face reconstruction,
+gradients, momentum assembly + three solves, pressure assembly + solve, p'
+gradient and the correction, each linear solve to a relative residual of
+1e-8 in FP64.  The FP64 path (FP64 AMG-PCG) and the mixed path (FP32 AMG
+preconditioner inside the FP64 flexible CG) run the same steps from the same
+state; ``value`` is the mixed path's Mcell-iterations/s.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-iter/s at 2.1M cells, mixed vs FP64 GPU at 1e-8 residual; adjoint grads/s"
+
+
+def run_path(mesh, case, precision, steps, warmup, profile_kernels=True):
+    from semo_mesh_amd.solver import GpuSolver
+    g = GpuSolver(0)
+    t0 = time.perf_counter()
+    g.upload_mesh(mesh, reorder=True)
+    g.set_case(case)
+    g.set_solver(precision, tol_p=1e-8, tol_u=1e-8, maxit_p=1000, maxit_u=1000)
+    t_upload = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(warmup):                       # the first one builds the AMG hierarchy
+        g.incompressiblePressureBased(1)
+    t_warm = time.perf_counter() - t0
+    if profile_kernels:
+        g.profile(True)
+        g.profile_reset()
+    iters_p, iters_u, res_p = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.incompressiblePressureBased(1)
+    t = time.perf_counter() - t0
+    prof = g.profile_stats() if profile_kernels else {}
+    g.profile(False)
+    fields = {f: g.get_field(f) for f in ("P", "U", "V", "W")}
+    g.close()
+    return dict(seconds=t, upload_s=t_upload, warmup_s=t_warm, prof=prof, fields=fields)
+
+
+def cpu_baseline(n=48):
+    """The compiled reference (SEMO + AMGCL, 1 MPI rank, no OpenMP) on a
+    bounded sample: one SIMPLE iteration of the same cavity at n^3."""
+    from semo_mesh_amd.cases import cavity_case
+    from semo_mesh_amd.mesh import cavity_mesh
+    ref = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
+    m = cavity_mesh(n, jitter=0.1)
+    with tempfile.TemporaryDirectory() as d:
+        m.write_raw(os.path.join(d, "mesh.raw"))
+        cavity_case().write_ref_case(os.path.join(d, "case.txt"), steps=1, correctors=1)
+        if os.path.exists(ref):
+            r = subprocess.run([ref, os.path.join(d, "mesh.raw"), os.path.join(d, "case.txt"),
+                                os.path.join(d, "out"), "nodump"], capture_output=True, text=True,
+                               timeout=600)
+            line = [x for x in r.stdout.splitlines() if x.startswith("SEMO_REF_DONE")]
+            if line:
+                kv = dict(tok.split("=") for tok in line[0].split()[1:])
+                run_s = float(kv["run_s"])
+                return dict(value=m.n_cells / run_s / 1e6, unit="Mcell-iter/s", cores=1, kind="reference",
+                            sample=f"compiled SEMO+AMGCL, 1 rank, one SIMPLE iteration of the jittered "
+                                   f"{n}^3 cavity ({m.n_cells} cells), {run_s:.2f} s; mesh geometry / "
+                                   f"LSQ / RCM set-up excluded")
+        # fallback: the repo's CPU oracle (single thread)
+        from oracle.oracle import Oracle
+        o = Oracle(m, cavity_case())
+        o.set_solver(1e-8, 5000)
+        t0 = time.perf_counter()
+        o.step(1)
+        run_s = time.perf_counter() - t0
+        return dict(value=m.n_cells / run_s / 1e6, unit="Mcell-iter/s", cores=1, kind="port",
+                    sample=f"CPU oracle, one SIMPLE iteration of the jittered {n}^3 cavity, {run_s:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(domain decomposition over RCCL is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from semo_mesh_amd.cases import cavity_case
+    from semo_mesh_amd.mesh import cavity_mesh
+    t0 = time.perf_counter()
+    mesh = cavity_mesh(args.n, jitter=0.1)
+    t_mesh = time.perf_counter() - t0
+    case = cavity_case()
+    print(f"[bench] mesh {args.n}^3: {mesh.n_cells} cells, {mesh.n_faces} faces ({t_mesh:.1f} s)",
+          file=sys.stderr, flush=True)
+
+    res64 = None
+    if not args.no_fp64:
+        res64 = run_path(mesh, case, "fp64", args.steps, args.warmup)
+        print(f"[bench] fp64: {args.steps} steps in {res64['seconds']:.3f} s", file=sys.stderr, flush=True)
+    resmx = run_path(mesh, case, "mixed", args.steps, args.warmup)
+    print(f"[bench] mixed: {args.steps} steps in {resmx['seconds']:.3f} s", file=sys.stderr, flush=True)
+
+    nc = mesh.n_cells
+    value = nc * args.steps / resmx["seconds"] / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-iter/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * resmx["seconds"] / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 Krylov / f32 AMG", "data": "synthetic",
+        "config": {"workload": f"jittered lid-driven cavity {args.n}^3 hex ({nc} cells), RCM, "
+                               "1 SIMPLE iteration per step, solves to 1e-8",
+                   "cells": nc, "faces": mesh.n_faces, "seed": 20261202},
+    }
+    if res64 is not None:
+        v64 = nc * args.steps / res64["seconds"] / 1e6
+        out["fp64"] = {"value": round(v64, 3), "ms_per_step": round(1e3 * res64["seconds"] / args.steps, 3)}
+        out["speedup_mixed_vs_fp64"] = round(value / v64, 3)
+        err = {}
+        for f in ("P", "U", "V", "W"):
+            a, b = resmx["fields"][f], res64["fields"][f]
+            err[f] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        out["mixed_vs_fp64_rel_l2"] = err
+    # dominant kernel by device time inside the timed mixed run
+    prof = resmx["prof"]
+    if prof:
+        name, st = max(prof.items(), key=lambda kv: kv[1]["ms"])
+        avg_ms = st["ms"] / st["count"]
+        per_launch = st["bytes"] / st["count"]
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": None, "avg_us": round(avg_ms * 1e3, 2),
+                           "bytes_per_launch": per_launch, "launches": st["count"]}
+        out["kernels"] = {k: {"count": v["count"], "ms": round(v["ms"], 3),
+                              "GBs": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -12,6 +12,29 @@ namespace {
 constexpr int TPB = 256;
 inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
 
+// sqrt of a double as x86 `sqrtl` then a store to double computes it: rounded
+// first to the 64-bit x87 significand, then to 53 bits (ties to even).  That
+// differs from the correctly rounded sqrt only when the 64-bit result is a
+// midpoint between two doubles; detect that case from the exact FMA remainder
+// r = x - s*s of the correctly rounded s and apply the tie-to-even choice.
+__device__ inline double sqrt_x87(double x) {
+    double s = sqrt(x);
+    if (!(x > 0.0) || isinf(x)) return s;
+    double r = fma(-s, s, x);                 // exact
+    if (r == 0.0) return s;
+    // neighbour of s on the side of the true root, and the midpoint offset h
+    double nb = r > 0.0 ? nextafter(s, INFINITY) : nextafter(s, 0.0);
+    double h = 0.5 * (nb - s);                // exact: half the local spacing, signed
+    // (s+h)^2 - x = -(r - 2sh - h^2); the 64-bit rounding lands on the midpoint
+    // iff |root - (s+h)| <= 2^-12 |nb - s|, i.e. |r - 2 s h - h^2| <= 2^-11 |nb-s| (s+h)
+    double d = r - 2.0 * s * h - h * h;
+    double tol = ldexp(fabs(nb - s), -11) * (s + h);
+    if (fabs(d) > tol) return s;
+    // tie between s and nb: even significand wins
+    long long bits_s = __double_as_longlong(s);
+    return (bits_s & 1LL) ? nb : s;
+}
+
 // calcUnitNormals_Area3dPolygon (geometric.cpp:16-204): 5 passes of a fan
 // triangulation about the running area-weighted centroid.
 __global__ void __launch_bounds__(TPB) k_face_polygon(
@@ -48,7 +71,7 @@ __global__ void __launch_bounds__(TPB) k_face_polygon(
             double B0 = X[c][0] - V0[0], B1 = X[c][1] - V0[1], B2 = X[c][2] - V0[2];
             double Nv[3] = {0.5 * (A1 * B2 - A2 * B1), 0.5 * (A2 * B0 - A0 * B2), 0.5 * (A0 * B1 - A1 * B0)};
             un[0] += Nv[0]; un[1] += Nv[1]; un[2] += Nv[2];
-            double ta = sqrt(Nv[0] * Nv[0] + Nv[1] * Nv[1] + Nv[2] * Nv[2]);
+            double ta = sqrt_x87(Nv[0] * Nv[0] + Nv[1] * Nv[1] + Nv[2] * Nv[2]);
             for (int a = 0; a < 3; ++a) xt[a] += ((V0[a] + X[i][a] + X[c][a]) / 3.0) * ta;
             tot += ta;
             vs += V0[0] * Nv[0]; vs += V0[1] * Nv[1]; vs += V0[2] * Nv[2];
@@ -57,7 +80,7 @@ __global__ void __launch_bounds__(TPB) k_face_polygon(
                 ct[a] += 2.0 * Nv[a] * (s1 * s1 + s2 * s2 + s3 * s3);
             }
         }
-        double mag = sqrt(un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
+        double mag = sqrt_x87(un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
         un[0] /= mag; un[1] /= mag; un[2] /= mag;
         ar = mag;
         for (int a = 0; a < 3; ++a) xt[a] /= tot;

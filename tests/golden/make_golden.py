@@ -49,7 +49,7 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
                 continue
             if base.startswith("call") and not keep_calls:
                 continue
-            if base.endswith(("_Rho", "_mu", "_VF", "_gradVF", "_kappa")) or "_srcS" in base:
+            if base.endswith(("_gradVF", "_kappa")) or "_srcS" in base:
                 continue
             arrs[base] = np.fromfile(os.path.join(out, fn),
                                      dtype=np.float64 if ext == ".f64" else np.int32)
