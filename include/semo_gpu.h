@@ -163,6 +163,9 @@ int semo_calc_pressure_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
  * momentum + pressure iterations; norms[2*k], norms[2*k+1] per iteration.    */
 int semo_incompressible_pressure_based(semo_ctx* ctx, int n_correctors, double* norms);
 
+/* Statistics of the last momentum (eq 0) / pressure (eq 1) solve.             */
+int semo_get_solve_stats(const semo_ctx* ctx, int eq, semo_solve_stats* st);
+
 /* Assemble without solving (eq 0 = momentum, 1 = pressure), for inspection.  */
 int semo_assemble(semo_ctx* ctx, int eq);
 /* The last assembled system in the caller's numbering as COO.  Call with

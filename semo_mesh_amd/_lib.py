@@ -67,7 +67,7 @@ EXPORTED = [
     "semo_mesh_upload", "semo_mesh_get_perm", "semo_mesh_get_face_map", "semo_mesh_get_connectivity",
     "semo_rcm", "semo_set_case", "semo_set_solver", "semo_field_set", "semo_field_get",
     "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_incompressible_pressure_based",
-    "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_solve_coo",
+    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
 ]
 
@@ -99,6 +99,7 @@ def lib():
         "semo_calc_momentum_eqs": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
         "semo_calc_pressure_eq": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
         "semo_incompressible_pressure_based": (C.c_int, [vp, C.c_int, dp]),
+        "semo_get_solve_stats": (C.c_int, [vp, C.c_int, C.POINTER(SolveStats)]),
         "semo_assemble": (C.c_int, [vp, C.c_int]),
         "semo_matrix_get": (C.c_int, [vp, C.POINTER(i64), vp, vp, vp, vp]),
         "semo_least_square": (C.c_int, [vp, dp, dp, dp]),

@@ -169,7 +169,7 @@ void upload_csr(DevSell<T>& M, const HCsr& A, bool square, hipStream_t s) {
     if (square) split_diag(A, ptr, col, val, diag);
     else { ptr = A.ptr; col = A.col; val = A.val; }
     std::vector<int> pos;
-    SellPattern p = sell_from_csr(A.n, ptr, col, &pos);
+    SellPattern p = sell_from_csr(A.n, A.m, ptr, col, &pos);
     std::vector<double> pv(p.col.size(), 0.0);
     for (size_t k = 0; k < col.size(); ++k) pv[pos[k]] = val[k];
     sell_upload<T>(M, p, pv, diag, s);

@@ -336,7 +336,8 @@ int semo_calc_momentum_eqs(semo_ctx* c, double* norm, semo_solve_stats* st) {
     if (!c) return SEMO_EINVAL;
     return guard(c, [&] {
         need_case(c);
-        double nv = momentum_eq(c, st);
+        double nv = momentum_eq(c, &c->last_stats_mom);
+        if (st) *st = c->last_stats_mom;
         if (norm) *norm = nv;
     });
 }
@@ -345,7 +346,8 @@ int semo_calc_pressure_eq(semo_ctx* c, double* norm, semo_solve_stats* st) {
     if (!c) return SEMO_EINVAL;
     return guard(c, [&] {
         need_case(c);
-        double nv = pressure_eq(c, st);
+        double nv = pressure_eq(c, &c->last_stats_p);
+        if (st) *st = c->last_stats_p;
         if (norm) *norm = nv;
     });
 }
@@ -356,14 +358,18 @@ int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
         need_case(c);
         step_setup(c);
         for (int it = 0; it < ncorr; ++it) {
-            semo_solve_stats st{};
-            double n0 = momentum_eq(c, &st);
-            double n1 = pressure_eq(c, &st);
-            c->last_stats = st;
+            double n0 = momentum_eq(c, &c->last_stats_mom);
+            double n1 = pressure_eq(c, &c->last_stats_p);
             if (norms) { norms[2 * it] = n0; norms[2 * it + 1] = n1; }
         }
         c->iterReal++;
     });
+}
+
+int semo_get_solve_stats(const semo_ctx* c, int eq, semo_solve_stats* st) {
+    if (!c || !st || (eq != 0 && eq != 1)) return SEMO_EINVAL;
+    *st = eq == 0 ? c->last_stats_mom : c->last_stats_p;
+    return SEMO_OK;
 }
 
 int semo_assemble(semo_ctx* c, int eq) {

@@ -97,17 +97,18 @@ struct Profiler {
 // ------------------------------------------------------- sliced-ELL matrix ---
 // Off-diagonal entries in slices of 64 rows (one wavefront per slice, one lane
 // per row), column-major inside a slice, plus a separate diagonal.  Padding
-// entries have col = row and val = 0.
+// entries have val = 0 and a column inside [0, ncols) (the row itself for
+// square matrices, 0 for rectangular ones).
 constexpr int SL = 64;
 struct SellPattern {
-    int n = 0, nslices = 0;
+    int n = 0, ncols = 0, nslices = 0;
     std::vector<int> sl_ptr;    // element offset of each slice (nslices+1)
     std::vector<int> col;       // padded columns
     int64_t nnz_offdiag = 0;    // real off-diagonal entries
 };
 template <class T>
 struct DevSell {
-    int n = 0, nslices = 0;
+    int n = 0, ncols = 0, nslices = 0;
     int64_t nnz = 0;            // real entries incl. diagonal
     DBuf<int> sl_ptr, col;
     DBuf<T> val, diag;
@@ -204,14 +205,14 @@ struct semo_ctx {
     semo::DBuf<int> d_flag1;
     semo::DBuf<double> kv64;                   // Krylov work (fp64)
     semo::DBuf<float> kv32;
-    semo_solve_stats last_stats{};
+    semo_solve_stats last_stats_mom{}, last_stats_p{};
 };
 
 namespace semo {
 // mesh.cpp
 void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder);
 void genrcm(int n, const int* ptr, const int* adj, int* perm, int* inv);
-SellPattern sell_from_csr(int n, const std::vector<int>& ptr, const std::vector<int>& col,
+SellPattern sell_from_csr(int n, int ncols, const std::vector<int>& ptr, const std::vector<int>& col,
                           std::vector<int>* pos_of_csr);
 // geometry.hip
 void geometry_compute(semo_ctx* c);

@@ -119,10 +119,15 @@ void genrcm(int n, const int* ptr, const int* adj, int* perm, int* inv) {
 }
 
 // ------------------------------------------------------------ sliced ELL ---
-SellPattern sell_from_csr(int n, const std::vector<int>& ptr, const std::vector<int>& col,
+SellPattern sell_from_csr(int n, int ncols, const std::vector<int>& ptr, const std::vector<int>& col,
                           std::vector<int>* pos) {
+    SEMO_CHECK(n >= 0 && ncols >= 1 && (int)ptr.size() == n + 1 && ptr[0] == 0 &&
+                   ptr[n] == (int)col.size(), SEMO_EINVAL, "sliced ELL: inconsistent CSR");
+    for (int r = 0; r < n; ++r) SEMO_CHECK(ptr[r + 1] >= ptr[r], SEMO_EINVAL, "sliced ELL: row pointers");
+    for (int cidx : col) SEMO_CHECK(cidx >= 0 && cidx < ncols, SEMO_EINVAL, "sliced ELL: column out of range");
     SellPattern p;
     p.n = n;
+    p.ncols = ncols;
     p.nslices = (n + SL - 1) / SL;
     p.sl_ptr.assign(p.nslices + 1, 0);
     for (int s = 0; s < p.nslices; ++s) {
@@ -142,7 +147,9 @@ SellPattern sell_from_csr(int n, const std::vector<int>& ptr, const std::vector<
                     p.col[e] = col[ptr[r] + k];
                     if (pos) (*pos)[ptr[r] + k] = e;
                 } else {
-                    p.col[e] = r < n ? r : 0;
+                    // padding (value 0) must still index inside the operand
+                    // vector: the row itself for square matrices, else column 0
+                    p.col[e] = (r < n && r < ncols) ? r : 0;
                 }
             }
         }
@@ -308,7 +315,7 @@ void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
         }
     }
     std::vector<int> pos_of;
-    c->pat = sell_from_csr(nc, rptr, rcol, &pos_of);
+    c->pat = sell_from_csr(nc, nc, rptr, rcol, &pos_of);
     c->h_slot_own.assign(nif, -1); c->h_slot_nbr.assign(nif, -1);
     for (size_t k = 0; k < rcol.size(); ++k) {
         int f = rface[k];
@@ -338,7 +345,7 @@ void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
     c->d_slot_own.upload(c->h_slot_own, s);
     c->d_slot_nbr.upload(c->h_slot_nbr, s);
     c->d_tslot.upload(tslot, s);
-    c->Amat.n = nc; c->Amat.nslices = c->pat.nslices; c->Amat.nnz = nc + c->pat.nnz_offdiag;
+    c->Amat.n = nc; c->Amat.ncols = nc; c->Amat.nslices = c->pat.nslices; c->Amat.nnz = nc + c->pat.nnz_offdiag;
     c->Amat.sl_ptr.upload(c->pat.sl_ptr, s);
     c->Amat.col.upload(c->pat.col, s);
     c->Amat.val.alloc(c->pat.col.size());

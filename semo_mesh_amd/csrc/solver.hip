@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(TPB) k_dinv(int n, const double* __restrict__ 
 template <class T>
 void sell_upload(DevSell<T>& M, const SellPattern& p, const std::vector<double>& vals,
                  const std::vector<double>& diag, hipStream_t s) {
-    M.n = p.n; M.nslices = p.nslices;
+    M.n = p.n; M.ncols = p.ncols; M.nslices = p.nslices;
     M.nnz = p.nnz_offdiag + (int64_t)diag.size();
     M.sl_ptr.upload(p.sl_ptr, s);
     M.col.upload(p.col, s);
@@ -584,7 +584,7 @@ void solve_coo(semo_ctx* c, const char* eq, int n, int64_t nnz, const int* rows,
     }
     for (int i = 0; i < n; ++i) SEMO_CHECK(diag[i] != 0.0, SEMO_EZERO_PIVOT, "zero diagonal");
     std::vector<int> pos;
-    SellPattern p = sell_from_csr(n, optr, ocol, &pos);
+    SellPattern p = sell_from_csr(n, n, optr, ocol, &pos);
     std::vector<double> pv(p.col.size(), 0.0);
     for (size_t k = 0; k < ocol.size(); ++k) pv[pos[k]] = oval[k];
     DevSell<double> A;

@@ -130,6 +130,14 @@ class GpuSolver:
         self._ok(self._L.semo_incompressible_pressure_based(self._h, n_correctors, _ptr(norms)))
         return norms
 
+    def solve_stats(self, eq: str):
+        """Iterations / final relative residuals / AMG levels of the last
+        momentum or pressure solve."""
+        st = _lib.SolveStats()
+        self._ok(self._L.semo_get_solve_stats(self._h, {"momentum": 0, "pressure": 1}[eq], C.byref(st)))
+        return dict(iters=list(st.iters), rel_res=list(st.rel_res), n_levels=st.n_levels,
+                    setup_ms=st.setup_ms)
+
     def assemble(self, eq: str):
         self._ok(self._L.semo_assemble(self._h, {"momentum": 0, "pressure": 1}[eq]))
 
