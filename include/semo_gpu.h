@@ -104,6 +104,9 @@ typedef struct {
     double urf_p;           /* prePreURF  (eqPressure.cpp:827)               */
     double urf_pu;          /* preVelURF  (eqPressure.cpp:832)               */
     double limit;           /* |P|,|U|,|V|,|W| clip (controls.maxP ... minW) */
+    double urf_vf;          /* vofVofURF  (eqVolfrac.cpp:776)                */
+    int32_t n_vof;          /* nSpeciesEq: VOF iterations per time step
+                               (pressureBased.cpp:104-127); 0 = one phase   */
 } semo_case;
 
 /* Replaces the controls/species read by SEMO_Controls_Builder
@@ -139,13 +142,17 @@ typedef enum {
     SEMO_F_MAXP, SEMO_F_MINP, SEMO_F_MAXU, SEMO_F_MINU, SEMO_F_MAXV, SEMO_F_MINV,
     SEMO_F_MAXW, SEMO_F_MINW,
     SEMO_F_ALPHA_P,                                         /* Darcy porosity   */
+    SEMO_F_QM3, SEMO_F_OLDVF,                               /* alpha time levels */
     SEMO_F_GRADP = 32, SEMO_F_GRADU, SEMO_F_GRADV, SEMO_F_GRADW, SEMO_F_GRADVF, /* [n*3] */
     SEMO_F_X0 = 48, SEMO_F_X1, SEMO_F_X2,                  /* last solution     */
     SEMO_F_B0 = 56, SEMO_F_B1, SEMO_F_B2,                  /* last right sides  */
     /* geometry (read-only): cell arrays, then face arrays (original face order) */
     SEMO_G_CELL_VOLUME = 64, SEMO_G_CELL_XYZ, SEMO_G_CELL_LSQ,            /* n, 3n, 6n */
     SEMO_G_FACE_AREA = 80, SEMO_G_FACE_NORMAL, SEMO_G_FACE_XYZ, SEMO_G_FACE_WC,
-    SEMO_G_FACE_MAGPN, SEMO_G_FACE_ALPHAF, SEMO_G_FACE_EPN, SEMO_G_FACE_SKEW
+    SEMO_G_FACE_MAGPN, SEMO_G_FACE_ALPHAF, SEMO_G_FACE_EPN, SEMO_G_FACE_SKEW,
+    /* face volume fractions varL/varR[fVF] after the last VOF assembly (MSTACS
+       on internal faces, boundary values on boundary faces), read-only      */
+    SEMO_F_FACE_VFL = 96, SEMO_F_FACE_VFR
 } semo_field;
 /* host <-> device copies in the caller's numbering; n = element count       */
 int semo_field_set(semo_ctx* ctx, int field, const double* host, int64_t n);
@@ -158,18 +165,26 @@ int semo_calc_momentum_eqs(semo_ctx* ctx, double* norm, semo_solve_stats* st);
 /* calcPressureEq (src/solvers/eqPressure.cpp:8-942): assemble, solve p',
  * two Green-Gauss passes, correct P and U,V,W; *norm as the reference.       */
 int semo_calc_pressure_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
+/* One VOF iteration of incompressiblePressureBased (pressureBased.cpp:104-127):
+ * setIncomValuesLeftRightFaceWithVfMSTACS (src/solvers/reconIncom.cpp:345-430,
+ * calcMSTACS src/solvers/NVD.cpp:1098-1862), calcVolfracEq (src/solvers/
+ * eqVolfrac.cpp:9-787), calcIncomCellEOSVF + calcCellTransport; *norm =
+ * sqrt(sum dalpha^2) as calcVolfracEq returns.                                 */
+int semo_calc_volfrac_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
 /* incompressiblePressureBased (src/solvers/pressureBased.cpp:6-483) for
- * nSpeciesEq = 0, nPressureEq = 1: one time step of n_correctors
- * momentum + pressure iterations; norms[2*k], norms[2*k+1] per iteration.    */
+ * nPressureEq = 1: one time step of n_vof VOF iterations (semo_case.n_vof),
+ * then n_correctors momentum + pressure iterations; norms[2*k], norms[2*k+1]
+ * per corrector.                                                               */
 int semo_incompressible_pressure_based(semo_ctx* ctx, int n_correctors, double* norms);
 
-/* Statistics of the last momentum (eq 0) / pressure (eq 1) solve.             */
+/* Statistics of the last momentum (eq 0) / pressure (eq 1) / VOF (eq 2) solve. */
 int semo_get_solve_stats(const semo_ctx* ctx, int eq, semo_solve_stats* st);
 
-/* Assemble without solving (eq 0 = momentum, 1 = pressure), for inspection.  */
+/* Assemble without solving (eq 0 = momentum, 1 = pressure, 2 = volume
+ * fraction incl. its MSTACS face values), for inspection.                      */
 int semo_assemble(semo_ctx* ctx, int eq);
 /* The last assembled system in the caller's numbering as COO.  Call with
- * rows == NULL to get *nnz.  b gets n*nrhs values (nrhs = 3 momentum, 1 p).   */
+ * rows == NULL to get *nnz.  b gets n*nrhs values (nrhs = 3 momentum, else 1). */
 int semo_matrix_get(semo_ctx* ctx, int64_t* nnz, int32_t* rows, int32_t* cols, double* vals,
                     double* b);
 

@@ -32,6 +32,12 @@ def lib():
         L.orc_set_case.argtypes = [C.c_void_p, _D, _D, C.c_double, _D, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_double]
         L.orc_set_solver.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.orc_set_vof.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        for nm in ("orc_mstacs", "orc_assemble_vof"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+        for nm in ("orc_vof_update", "orc_vof_norm"):
+            getattr(L, nm).restype = C.c_double
+            getattr(L, nm).argtypes = [C.c_void_p]
         L.orc_size.restype = C.c_long
         L.orc_size.argtypes = [C.c_void_p, C.c_char_p]
         L.orc_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
@@ -86,6 +92,7 @@ class Oracle:
         L.orc_set_case(self._h, np.array(case.rho, dtype=np.float64), np.array(case.mu, dtype=np.float64),
                        case.sigma, np.array(case.g, dtype=np.float64), case.dt, case.urf_mom,
                        case.urf_p, case.urf_pu, case.limit)
+        L.orc_set_vof(self._h, getattr(case, "urf_vf", 1.0), getattr(case, "n_vof", 0))
         for i, nm in enumerate(self.mesh.patch_names):
             b = case.bcs[nm]
             L.orc_set_bc(self._h, i, b.p_type, b.p_value, b.u_type,
@@ -131,6 +138,10 @@ class Oracle:
         it = lib().orc_solve(self._h, nrhs, C.byref(r))
         return it, r.value
 
+    def mstacs(self): lib().orc_mstacs(self._h)
+    def assemble_vof(self): lib().orc_assemble_vof(self._h)
+    def vof_update(self): return lib().orc_vof_update(self._h)
+    def vof_norm(self): return lib().orc_vof_norm(self._h)
     def momentum_update(self): return lib().orc_momentum_update(self._h)
     def pressure_correct(self): return lib().orc_pressure_correct(self._h)
 

@@ -101,6 +101,15 @@ static void dump_state(SEMO_Mesh_Builder& mesh, const string& p) {
     dump_cellvar(mesh, c.oldW, p + "oldW");
     dump_cellvar(mesh, c.Qm[0], p + "Qm0"); dump_cellvar(mesh, c.Qm[1], p + "Qm1");
     dump_cellvar(mesh, c.Qm[2], p + "Qm2");
+    dump_cellvar(mesh, c.Qm[3], p + "Qm3");
+    dump_cellvar(mesh, c.oldVF[0], p + "oldVF");
+    {   // face volume fractions (MSTACS output on internal faces)
+        vector<double> lv(mesh.faces.size()), rv(mesh.faces.size());
+        for (size_t i = 0; i < mesh.faces.size(); ++i) {
+            lv[i] = mesh.faces[i].varL[c.fVF[0]]; rv[i] = mesh.faces[i].varR[c.fVF[0]];
+        }
+        dump(p + "faceLVF", lv); dump(p + "faceRVF", rv);
+    }
     for (const char* nm : {"P", "U", "V", "W", "VF", "Rho", "mu"}) {
         int idx = !strcmp(nm, "P") ? c.P : !strcmp(nm, "U") ? c.U : !strcmp(nm, "V") ? c.V
                 : !strcmp(nm, "W") ? c.W : !strcmp(nm, "VF") ? c.VF[0]
@@ -149,7 +158,7 @@ int main(int argc, char** argv) {
 
     // case file: "key value..." lines
     double dt = 0.01, rho[2] = {1000, 1000}, mu[2] = {1e-3, 1e-3}, sigma = 0, g[3] = {0, 0, 0};
-    double urf_u = 0.3, urf_p = 0.3, urf_pu = 0.3, lim = 1e10;
+    double urf_u = 0.3, urf_p = 0.3, urf_pu = 0.3, lim = 1e10, urf_vf = 1.0;
     int nsteps = 1, ncorr = 1, nvof = 0;
     vector<PatchBC> bcs;
     string initfile;
@@ -167,6 +176,7 @@ int main(int argc, char** argv) {
             else if (key == "steps") cf >> nsteps;
             else if (key == "correctors") cf >> ncorr;
             else if (key == "vof") cf >> nvof;
+            else if (key == "urfvf") cf >> urf_vf;
             else if (key == "init") cf >> initfile;
             else if (key == "bc") {
                 PatchBC b;
@@ -190,7 +200,7 @@ int main(int argc, char** argv) {
     controls.adjustTimeStep = "no"; controls.maxCo = 1.0; controls.maxTimeStep = dt;
     controls.gravityAcceleration = {g[0], g[1], g[2]};
     controls.momVelURF = urf_u; controls.prePreURF = urf_p; controls.preVelURF = urf_pu;
-    controls.vofVofURF = 1.0;
+    controls.vofVofURF = urf_vf;
     controls.momVelAdjustRF = "no"; controls.prePreAdjustRF = "no";
     controls.preVelAdjustRF = "no"; controls.vofVofAdjustRF = "no";
     controls.maxP = lim; controls.minP = -lim; controls.maxU = lim; controls.minU = -lim;

@@ -55,6 +55,8 @@ struct Oracle {
     double rho[2] = {1000, 1000}, mu[2] = {1e-3, 1e-3}, sigma = 0, g[3] = {0, 0, 0};
     double dt = 0.01, oldDt = 0.01, old2Dt = 0.01;
     double urf_u = 0.3, urf_p = 0.3, urf_pu = 0.3, lim = 1e10;
+    double urf_vf = 1.0;               // vofVofURF (controls/build.cpp:475)
+    int n_vof = 0;                     // nSpeciesEq: VOF iterations per step (controls/build.cpp:409)
     int iterReal = 0;
 
     // ---- fields (cell) -------------------------------------------------------
@@ -65,6 +67,8 @@ struct Oracle {
     vector<double> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
     // face left/right values (src/solvers/reconIncom.cpp:501-1035)
     vector<double> LP, RP, LU, RU, LV, RV, LW, RW, LVF, RVF, Lmu, Rmu, LRho, RRho;
+    vector<double> courant;            // per-cell Courant number of the last MSTACS pass
+    double vof_norm = 0.0;             // return value of the last calcVolfracEq
 
     // ---- last assembled system -------------------------------------------------
     vector<int> Arows, Acols;
@@ -672,6 +676,163 @@ void assemble_pressure(Oracle& o) {
 }
 
 // ------------------------------------------------------------------------
+// setIncomValuesLeftRightFaceWithVfMSTACS (reconIncom.cpp:345-430) with
+// calcMSTACS (NVD.cpp:1098-1862), one rank: zero-order face values, the LSQ
+// gradient of VF, per-cell Courant numbers, then the high-order volume
+// fraction on each side of every internal face (LVF/RVF of internal faces).
+void mstacs_faces(Oracle& o) {
+    recon_faces(o);
+    vector<double> g;
+    lsq_grad(o, o.VF, o.RVF, g);                                  // reconIncom.cpp:366-369
+    const double tmpCFL = 1.0, eps = 1.e-12;
+    o.courant.assign(o.nc, 0.0);
+    for (int i = 0; i < o.nc; ++i) {                              // NVD.cpp:1114-1135
+        double magVel = sqrt(o.U[i] * o.U[i] + o.V[i] * o.V[i] + o.W[i] * o.W[i]);
+        double maxA = 0.0;
+        for (int k = o.cfptr[i]; k < o.cfptr[i + 1]; ++k) maxA = std::max(maxA, o.farea[o.cfidx[k]]);
+        o.courant[i] = 1.0 / tmpCFL * magVel * o.dt * maxA / o.cvol[i];
+    }
+    for (int f = 0; f < o.nif; ++f) {                             // NVD.cpp:1166-1858
+        int l = o.owner[f], r = o.nbr[f];
+        double wCL = o.fwC[f], wCR = 1.0 - wCL;
+        double L1 = o.VF[l], R1 = o.VF[r];
+        const double* gl = &g[3 * (size_t)l];
+        const double* gr = &g[3 * (size_t)r];
+        const double* pn = &o.fdist[3 * (size_t)f];              // vecPN of an internal face
+        const double* n = &o.fnrm[3 * (size_t)f];
+        const double* sk = &o.fskew[3 * (size_t)f];
+        double coDDL = o.courant[l], coDDR = o.courant[r];
+        double L2 = 0.0;
+        for (int a = 0; a < 3; ++a) L2 += gl[a] * pn[a];
+        L2 = R1 - 2.0 * L2;
+        double R2 = 0.0;
+        for (int a = 0; a < 3; ++a) R2 += gr[a] * pn[a];
+        R2 = L1 + 2.0 * R2;
+        L2 = std::max(0.0, std::min(1.0, L2));
+        R2 = std::max(0.0, std::min(1.0, R2));
+        double vfL = L1, vfR = R1;
+        for (int LR = 0; LR < 2; ++LR) {
+            if (LR == 0 && (vfL > 1.0 - eps || vfL < eps)) continue;
+            if (LR == 1 && (vfR > 1.0 - eps || vfR < eps)) continue;
+            double alpU, alpD, alpA;
+            double coDD = std::max(coDDL, coDDR);
+            if (LR == 0) { alpU = L2; alpD = L1; alpA = R1; }
+            else { alpU = R2; alpD = R1; alpA = L1; }
+            // blending weight from the interpolated interface normal (NVD.cpp:1437-1477)
+            double mf[3];
+            for (int a = 0; a < 3; ++a) mf[a] = wCL * gl[a] + wCR * gr[a];
+            double mag = mf[0] * mf[0];
+            mag += mf[1] * mf[1];
+            mag += mf[2] * mf[2];
+            mag = sqrt(mag);
+            for (int a = 0; a < 3; ++a) mf[a] = mf[a] / (mag + 1.e-200);
+            double cosT = mf[0] * n[0];
+            cosT += mf[1] * n[1];
+            cosT += mf[2] * n[2];
+            cosT = std::abs(cosT);
+            double gamF = std::min(pow(cosT, 4.0), 1.0);
+            double tCd = (alpD - alpU) / (std::abs(alpA - alpU) + 1.e-200) * (alpA > alpU ? 1.0 : -1.0);
+            double side = LR == 0 ? 2.0 * wCR : 2.0 * wCL;
+            auto skewed = [&](double gam) {                       // NVD.cpp:1628-1671
+                double v = alpD + gam * (alpA - alpD);
+                v += wCL * gl[0] * sk[0]; v += wCL * gl[1] * sk[1]; v += wCL * gl[2] * sk[2];
+                v += wCR * gr[0] * sk[0]; v += wCR * gr[1] * sk[1]; v += wCR * gr[2] * sk[2];
+                return std::max(0.0, std::min(1.0, v));
+            };
+            // compressive (CICSAM-like) branch (NVD.cpp:1602-1626)
+            double gam = 0.0;
+            if (tCd >= 0.0 && tCd < 1.0) {
+                if (coDD > 0.0 && coDD <= 0.33) {
+                    double tCf = std::min(1.0, tCd / coDD);
+                    gam = (tCf - tCd) / (1.0 - tCd);
+                } else if (coDD > 0.33 && coDD <= 1.0) {
+                    double tCf = std::min(1.0, 3.0 * tCd);
+                    gam = (tCf - tCd) / (1.0 - tCd);
+                }
+            }
+            gam *= side;
+            gam = std::max(0.0, std::min(1.0, gam));
+            double vfComp = skewed(gam);
+            // diffusive (STACS-like) branch (NVD.cpp:1741-1767)
+            gam = 0.0;
+            if (tCd >= 0.0 && tCd < 0.2) { double tCf = std::min(1.0, 3.0 * tCd); gam = (tCf - tCd) / (1.0 - tCd); }
+            else if (tCd >= 0.2 && tCd < 0.5) { double tCf = 0.5 + 0.5 * tCd; gam = (tCf - tCd) / (1.0 - tCd); }
+            else if (tCd >= 0.5 && tCd < 0.8333) { double tCf = 0.375 + 0.75 * tCd; gam = (tCf - tCd) / (1.0 - tCd); }
+            else if (tCd >= 0.8333 && tCd < 1.0) { double tCf = 1.0; gam = (tCf - tCd) / (1.0 - tCd); }
+            gam *= side;
+            gam = std::max(0.0, std::min(1.0, gam));
+            double vfDiff = skewed(gam);
+            double vf = gamF * vfComp + (1.0 - gamF) * vfDiff;
+            if (LR == 0) vfL = vf; else vfR = vf;
+        }
+        o.LVF[f] = vfL;
+        o.RVF[f] = vfR;
+    }
+}
+
+// calcVolfracEq assembly (eqVolfrac.cpp:9-765)
+void assemble_vof(Oracle& o) {
+    int nc = o.nc;
+    minmax(o, o.U, o.maxU, o.minU);
+    minmax(o, o.V, o.maxV, o.minV);
+    minmax(o, o.W, o.maxW, o.minW);
+    gradients5(o);                                                // eqVolfrac.cpp:90-99
+    o.Arows.assign(nc, 0); o.Acols.assign(nc, 0); o.Avals.assign(nc, 0.0);
+    o.B0.assign(nc, 0.0);
+    for (int i = 0; i < nc; ++i) {                                // eqVolfrac.cpp:221-262
+        double vol = o.cvol[i];
+        o.Arows[i] = i; o.Acols[i] = i;
+        o.Avals[i] = 1.5 * vol / o.dt;
+        o.B0[i] = -(1.5 * o.VF[i] - 2.0 * o.oldVF[i] + 0.5 * o.Qm3[i]) * vol / o.dt;
+    }
+    for (int f = 0; f < o.nif; ++f) {                             // eqVolfrac.cpp:266-607
+        int l = o.owner[f], r = o.nbr[f];
+        double area = o.farea[f];
+        double UnF = face_UnF(o, f);
+        double wL = (UnF > 0.0) ? 1.0 : 0.0, wR = 1.0 - wL;
+        double VFF = wL * o.LVF[f] + wR * o.RVF[f];
+        o.Avals[l] += (wL * UnF * area);
+        double v = (wR * UnF * area);
+        if (!(std::abs(v) < std::numeric_limits<double>::min())) {
+            o.Arows.push_back(l); o.Acols.push_back(r); o.Avals.push_back(v);
+        }
+        o.Avals[r] -= (wR * UnF * area);
+        v = -(wL * UnF * area);
+        if (!(std::abs(v) < std::numeric_limits<double>::min())) {
+            o.Arows.push_back(r); o.Acols.push_back(l); o.Avals.push_back(v);
+        }
+        o.B0[l] -= (VFF * UnF * area);
+        o.B0[r] += (VFF * UnF * area);
+    }
+    for (const Patch& pb : o.patches) {                          // eqVolfrac.cpp:611-703
+        for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
+            int c = o.owner[f];
+            double area = o.farea[f];
+            const double* n = &o.fnrm[3 * f];
+            const double* e = &o.fePN[3 * f];
+            double UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
+            double UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
+            double UnF = 0.5 * UnL + 0.5 * UnR;
+            double t1 = 1.0 / o.LRho[f] * o.dt;
+            double VFF = 0.5 * o.LVF[f] + 0.5 * o.RVF[f];
+            double cVF = 0.0;
+            if (pb.vf_type == ZERO_GRADIENT) cVF = 1.0;
+            else if (pb.vf_type == INLET_OUTLET) {
+                double own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
+                cVF = (own > 0.0) ? 1.0 : 0.0;
+            }
+            double dPN = 0.5 * o.fmagPN[f], alpha = o.falpha[f];
+            if (pb.p_type == FIXED_VALUE) {
+                UnF -= alpha * t1 * (pb.p_val - o.P[c]) / dPN;
+                for (int a = 0; a < 3; ++a) UnF += alpha * t1 * o.gP[3 * c + a] * e[a];
+            }
+            o.Avals[c] += cVF * (UnF * area);
+            o.B0[c] -= (VFF * UnF * area);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // linear solve: COO -> CSR (solveAMGCL.cpp:206-251), ILU0 + BiCGStab
 struct Csr { int n; vector<int> ptr, col; vector<double> val; };
 
@@ -800,6 +961,18 @@ void solve_last(Oracle& o, int nrhs) {
 }
 
 // momentum update (eqMomentum.cpp:1029-1119)
+// VF += vofVofURF dVF, clipped to [0,1]; returns sqrt(sum dVF^2)
+// (eqVolfrac.cpp:770-783)
+double vof_update(Oracle& o) {
+    double nd = 0.0;
+    for (int i = 0; i < o.nc; ++i) {
+        o.VF[i] += o.urf_vf * o.X0[i];
+        o.VF[i] = std::max(0.0, std::min(1.0, o.VF[i]));
+        nd += pow(o.X0[i], 2.0);
+    }
+    return sqrt(nd);
+}
+
 double momentum_update(Oracle& o) {
     double nd = 0, np = 0;
     for (int i = 0; i < o.nc; ++i) {
@@ -870,6 +1043,13 @@ void pressure_based_step(Oracle& o, int ncorr, double* norms) {
     if (o.iterReal == 0) { o.Qm0 = o.U; o.Qm1 = o.V; o.Qm2 = o.W; o.Qm3 = o.VF; }
     else { o.Qm0 = o.oldU; o.Qm1 = o.oldV; o.Qm2 = o.oldW; o.Qm3 = o.oldVF; }
     o.oldP = o.P; o.oldU = o.U; o.oldV = o.V; o.oldW = o.W; o.oldVF = o.VF; o.oldRho = o.Rho;
+    for (int k = 0; k < o.n_vof; ++k) {               // pressureBased.cpp:104-127
+        mstacs_faces(o);
+        assemble_vof(o);
+        solve_last(o, 1);
+        o.vof_norm = vof_update(o);
+        cell_eos_transport(o);
+    }
     recon_faces(o);
     minmax(o, o.VF, o.maxVF, o.minVF);
     source_gravity(o);
@@ -1035,6 +1215,12 @@ void orc_set_case(void* h, const double* rho, const double* mu, double sigma, co
     o.urf_u = urf_u; o.urf_p = urf_p; o.urf_pu = urf_pu; o.lim = lim;
 }
 
+// two-phase controls: vofVofURF and nSpeciesEq (VOF iterations per step)
+void orc_set_vof(void* h, double urf_vf, int n_vof) {
+    Oracle& o = *(Oracle*)h;
+    o.urf_vf = urf_vf; o.n_vof = n_vof;
+}
+
 void orc_set_solver(void* h, double tol, int maxit) {
     Oracle& o = *(Oracle*)h; o.tol = tol; o.maxit = maxit;
 }
@@ -1053,7 +1239,8 @@ static vector<double>* field(Oracle& o, const char* nm) {
     F("face_dist", fdist) F("cell_volume", cvol) F("cell_xyz", cxyz) F("cell_lsq", lsq)
     F("b0", B0) F("b1", B1) F("b2", B2) F("x0", X0) F("x1", X1) F("x2", X2) F("vals", Avals)
     F("faceRP", RP) F("faceRU", RU) F("faceRV", RV) F("faceRW", RW) F("faceRVF", RVF)
-    F("faceLU", LU) F("faceLV", LV) F("faceLW", LW) F("faceLRho", LRho)
+    F("faceLU", LU) F("faceLV", LV) F("faceLW", LW) F("faceLRho", LRho) F("faceLVF", LVF)
+    F("oldVF", oldVF) F("Qm3", Qm3) F("courant", courant)
 #undef F
     return nullptr;
 }
@@ -1113,6 +1300,10 @@ int orc_solve(void* h, int nrhs, double* relres) {
 }
 double orc_momentum_update(void* h) { return momentum_update(*(Oracle*)h); }
 double orc_pressure_correct(void* h) { return pressure_correct(*(Oracle*)h); }
+void orc_mstacs(void* h) { mstacs_faces(*(Oracle*)h); }
+void orc_assemble_vof(void* h) { assemble_vof(*(Oracle*)h); }
+double orc_vof_update(void* h) { return vof_update(*(Oracle*)h); }
+double orc_vof_norm(void* h) { return ((Oracle*)h)->vof_norm; }
 void orc_step(void* h, int ncorr, double* norms) { pressure_based_step(*(Oracle*)h, ncorr, norms); }
 
 // RCM of an arbitrary graph: adjacency CSR (0-based), perm[new] = old,

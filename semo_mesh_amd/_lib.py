@@ -22,14 +22,14 @@ STATUS = {0: "SEMO_OK", 1: "SEMO_EINVAL", 2: "SEMO_EINVAL_BC", 3: "SEMO_ESINGULA
 # field ids (semo_field)
 F = dict(P=0, U=1, V=2, W=3, VF=4, Rho=5, mu=6, oldU=7, oldV=8, oldW=9, Qm0=10, Qm1=11, Qm2=12,
          srcG0=13, srcG1=14, srcG2=15, maxP=16, minP=17, maxU=18, minU=19, maxV=20, minV=21,
-         maxW=22, minW=23, alphaP=24, gradP=32, gradU=33, gradV=34, gradW=35, gradVF=36,
+         maxW=22, minW=23, alphaP=24, Qm3=25, oldVF=26, gradP=32, gradU=33, gradV=34, gradW=35, gradVF=36,
          x0=48, x1=49, x2=50, b0=56, b1=57, b2=58,
          cell_volume=64, cell_xyz=65, cell_lsq=66,
          face_area=80, face_normal=81, face_xyz=82, face_wC=83, face_magPN=84, face_alphaF=85,
-         face_ePN=86, face_skew=87)
+         face_ePN=86, face_skew=87, faceLVF=96, faceRVF=97)
 COMPONENTS = dict(gradP=3, gradU=3, gradV=3, gradW=3, gradVF=3, cell_xyz=3, cell_lsq=6,
                   face_normal=3, face_xyz=3, face_ePN=3, face_skew=3)
-FACE_FIELDS = {k for k in F if k.startswith("face_")}
+FACE_FIELDS = {k for k in F if k.startswith("face")}
 
 
 class MeshDesc(C.Structure):
@@ -48,7 +48,8 @@ class PatchBC(C.Structure):
 class CaseDesc(C.Structure):
     _fields_ = [("rho", C.c_double * 2), ("mu", C.c_double * 2), ("sigma", C.c_double),
                 ("g", C.c_double * 3), ("dt", C.c_double), ("urf_mom", C.c_double),
-                ("urf_p", C.c_double), ("urf_pu", C.c_double), ("limit", C.c_double)]
+                ("urf_p", C.c_double), ("urf_pu", C.c_double), ("limit", C.c_double),
+                ("urf_vf", C.c_double), ("n_vof", C.c_int32)]
 
 
 class SolverParams(C.Structure):
@@ -66,7 +67,8 @@ EXPORTED = [
     "semo_ctx_create", "semo_ctx_destroy", "semo_ctx_last_error", "semo_ctx_device_bytes",
     "semo_mesh_upload", "semo_mesh_get_perm", "semo_mesh_get_face_map", "semo_mesh_get_connectivity",
     "semo_rcm", "semo_set_case", "semo_set_solver", "semo_field_set", "semo_field_get",
-    "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_incompressible_pressure_based",
+    "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_calc_volfrac_eq",
+    "semo_incompressible_pressure_based",
     "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
 ]
@@ -98,6 +100,7 @@ def lib():
         "semo_field_get": (C.c_int, [vp, C.c_int, dp, i64]),
         "semo_calc_momentum_eqs": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
         "semo_calc_pressure_eq": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
+        "semo_calc_volfrac_eq": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
         "semo_incompressible_pressure_based": (C.c_int, [vp, C.c_int, dp]),
         "semo_get_solve_stats": (C.c_int, [vp, C.c_int, C.POINTER(SolveStats)]),
         "semo_assemble": (C.c_int, [vp, C.c_int]),

@@ -45,6 +45,8 @@ class Case:
     urf_p: float = 0.3        # prePreURF  (eqPressure.cpp:827)
     urf_pu: float = 0.3       # preVelURF  (eqPressure.cpp:832)
     limit: float = 1e10       # |P|,|U|,|V|,|W| clip bounds (fvSolution limiter)
+    urf_vf: float = 1.0       # vofVofURF  (eqVolfrac.cpp:776)
+    n_vof: int = 0            # nSpeciesEq: VOF iterations per time step (pressureBased.cpp:104-127)
     bcs: dict = field(default_factory=dict)   # patch name -> PatchBC
 
     def write_ref_case(self, path: str, steps: int = 1, correctors: int = 1,
@@ -55,6 +57,8 @@ class Case:
             fh.write("g %r %r %r\n" % tuple(self.g))
             fh.write(f"urf {self.urf_mom!r} {self.urf_p!r} {self.urf_pu!r}\n")
             fh.write(f"limit {self.limit!r}\nsteps {steps}\ncorrectors {correctors}\n")
+            if self.n_vof:
+                fh.write(f"vof {self.n_vof}\nurfvf {self.urf_vf!r}\n")
             if init:
                 fh.write(f"init {init}\n")
             for name, b in self.bcs.items():
@@ -74,3 +78,16 @@ def cavity_case(dt: float = 0.01, lid=(1.0, 0.0, 0.0)) -> Case:
     bcs["pref"] = PatchBC(p_type=BC_FIXED_VALUE, p_value=0.0)
     bcs["zmax"] = PatchBC(u_type=BC_FIXED_VALUE, u_value=tuple(lid))
     return Case(dt=dt, bcs=bcs)
+
+
+def dam_break_case(dt: float = 1e-4, n_vof: int = 1, urf_vf: float = 1.0) -> Case:
+    """Two-phase VOF dam break: water (phase 0: rho 1000, mu 1e-3) and air
+    (rho 1.2, mu 1.8e-5), gravity -9.81 in z.  Walls (xmin, xmax, ymin, ymax,
+    zmin): noSlip, zeroGradient p and alpha.  Top (zmax): open, fixedValue
+    p = 0, zeroGradient U, inletOutlet alpha = 0 (air enters)."""
+    wall = PatchBC()
+    bcs = {n: wall for n in ("xmin", "xmax", "ymin", "ymax", "zmin")}
+    bcs["zmax"] = PatchBC(p_type=BC_FIXED_VALUE, p_value=0.0, u_type=BC_ZERO_GRADIENT,
+                          vf_type=BC_INLET_OUTLET, vf_value=0.0)
+    return Case(rho=(1000.0, 1.2), mu=(1e-3, 1.8e-5), g=(0.0, 0.0, -9.81), dt=dt,
+                urf_vf=urf_vf, n_vof=n_vof, bcs=bcs)

@@ -163,6 +163,19 @@ void split_diag(const HCsr& A, std::vector<int>& ptr, std::vector<int>& col, std
     }
 }
 
+// Device format of an operator: sliced ELL (one lane per row) for short rows
+// of similar length; CSR with `lanes` lanes per row (about four entries per
+// lane) for long rows (restriction, Galerkin coarse operators) or rows whose
+// lengths vary so much that ELL padding would waste more than half the reads.
+int choose_lanes(int n, const std::vector<int>& ptr, size_t padded) {
+    size_t nnz = ptr.empty() ? 0 : (size_t)ptr[n];
+    double avg = n ? (double)nnz / n : 0.0;
+    if (avg <= 10.0 && (double)padded <= 1.5 * (double)std::max<size_t>(nnz, 1)) return 0;
+    int lanes = 4;
+    while (lanes < 64 && 4.0 * lanes < avg) lanes *= 2;
+    return lanes;
+}
+
 template <class T>
 void upload_csr(DevSell<T>& M, const HCsr& A, bool square, hipStream_t s) {
     std::vector<int> ptr, col; std::vector<double> val, diag;
@@ -170,9 +183,14 @@ void upload_csr(DevSell<T>& M, const HCsr& A, bool square, hipStream_t s) {
     else { ptr = A.ptr; col = A.col; val = A.val; }
     std::vector<int> pos;
     SellPattern p = sell_from_csr(A.n, A.m, ptr, col, &pos);
-    std::vector<double> pv(p.col.size(), 0.0);
-    for (size_t k = 0; k < col.size(); ++k) pv[pos[k]] = val[k];
-    sell_upload<T>(M, p, pv, diag, s);
+    int lanes = choose_lanes(A.n, ptr, p.col.size());
+    if (lanes) {
+        csr_upload<T>(M, A.n, A.m, ptr, col, val, diag, lanes, s);
+    } else {
+        std::vector<double> pv(p.col.size(), 0.0);
+        for (size_t k = 0; k < col.size(); ++k) pv[pos[k]] = val[k];
+        sell_upload<T>(M, p, pv, diag, s);
+    }
     M.nnz = (int64_t)A.col.size();
 }
 
@@ -200,9 +218,10 @@ void amg_setup_host(semo_ctx* c, Amg& amg, const HCsr& A0) {
         L.dinv64.upload(dinv, s);
         std::vector<float> df(dinv.begin(), dinv.end());
         L.dinv32.upload(df, s);
+        SEMO_HIP(hipStreamSynchronize(s));      // df / dinv are temporaries
         L.lmax = lambda_max(A, dinv);
-        for (auto* b : {&L.x64, &L.b64, &L.r64, &L.t64, &L.u64}) b->alloc(A.n + 64);
-        for (auto* b : {&L.x32, &L.b32, &L.r32, &L.t32, &L.u32}) b->alloc(A.n + 64);
+        for (auto* b : {&L.x64, &L.b64, &L.r64, &L.t64, &L.u64, &L.w64}) b->alloc(A.n + 64);
+        for (auto* b : {&L.x32, &L.b32, &L.r32, &L.t32, &L.u32, &L.w32}) b->alloc(A.n + 64);
         if (A.n <= coarse) break;
 
         std::vector<int> agg;

@@ -12,6 +12,8 @@
 // accumulates, for each cell, its faces in ascending face order -- the same
 // per-cell order in which SEMO's face loops scatter (eqMomentum.cpp:331-1008,
 // eqPressure.cpp:246-673) -- and adds the boundary faces inline.
+#include <climits>
+
 #include "common.hpp"
 
 namespace semo {
@@ -35,16 +37,40 @@ CaseK case_k(const semo_ctx* c) {
     return k;
 }
 
+// Faces of cell i in ascending order from the sliced-ELL cell -> face list
+// (64-cell slices, entry k of a cell at slice base + 64k + lane; ~f marks a
+// face whose neighbour the cell is; INT_MIN pads).  own: the cell owns f.
+#define CELL_FACES_BEGIN(i, SL_, ELL_, f, own)                                            \
+    for (int e_ = SL_[(i) >> 6] + ((i) & 63); e_ < SL_[((i) >> 6) + 1]; e_ += 64) {      \
+        int code_ = ELL_[e_];                                                              \
+        if (code_ == INT_MIN) break;                                                       \
+        bool own = code_ >= 0;                                                             \
+        int f = own ? code_ : ~code_;
+
 // std::max / std::min semantics (first argument wins ties), as in the reference
 __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ inline double clip01(double a) { return smax(0.0, smin(1.0, a)); }
 
+// block-partial sums of two quantities (deterministic two-stage reduction)
+__device__ inline void block_sum2(double a, double b, double* __restrict__ part) {
+    __shared__ double sa[TPB / 64], sb[TPB / 64];
+    for (int o = 32; o > 0; o >>= 1) { a += __shfl_down(a, o, 64); b += __shfl_down(b, o, 64); }
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { sa[w] = a; sb[w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0, y = 0;
+        for (int k = 0; k < TPB / 64; ++k) { x += sa[k]; y += sb[k]; }
+        part[2 * blockIdx.x] = x; part[2 * blockIdx.x + 1] = y;
+    }
+}
+
 // --------------------------------------------------------------------- EOS --
 // calcIncomCellEOSVF (eos.cpp:10-151) + calcCellTransport (transport.cpp:9-98)
 __global__ void __launch_bounds__(TPB) k_eos(int nc, CaseK k, double* __restrict__ VF,
                                              double* __restrict__ Rho, double* __restrict__ Mu) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double a = clip01(VF[i]);
     double v0 = clip01(a), v1 = clip01(1.0 - a);
@@ -65,7 +91,7 @@ __global__ void __launch_bounds__(TPB) k_recon_boundary(
     const double* __restrict__ W, const double* __restrict__ VF, const double* __restrict__ Mu,
     double* __restrict__ bP, double* __restrict__ bUL, double* __restrict__ bUR,
     double* __restrict__ bVF, double* __restrict__ bMu, double* __restrict__ bRho) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = xcd_block() * blockDim.x + threadIdx.x;
     if (j >= nbf) return;
     int f = nif + j;
     int c = owner[f];
@@ -139,7 +165,7 @@ __global__ void __launch_bounds__(TPB) k_gravity(
     const double* __restrict__ V, const double* __restrict__ W, const double* __restrict__ VF,
     const double* __restrict__ bUL, const double* __restrict__ bUR, const double* __restrict__ bVF,
     int nbf, double* __restrict__ srcG) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double s[4] = {0, 0, 0, 0};
     double vi = vol[i];
@@ -181,27 +207,55 @@ __global__ void __launch_bounds__(TPB) k_gravity(
 // calcLeastSquare "cellVertex" "1st" "cell" (gradient.cpp:761-816 stencil,
 // 997-1083 boundary faces, 1136-1159 apply) for P, U, V, W, VF in one pass,
 // fused with setCellVarMinMax (cellVarMinMax.cpp:15-24) of U, V, W (and P).
+//
+// Layout: each cell's centroid and the five fields sit in one 64-byte record
+// cpk[8i..8i+7] = [x,y,z,P,U,V,W,VF], so a stencil neighbour costs one cache
+// line; the stencil is sliced ELL (lane = cell inside a 64-cell slice) so the
+// index loads of a wavefront are contiguous.  Summation order is the stencil
+// order, then the cell's boundary faces in ascending order (as the reference).
+__global__ void __launch_bounds__(TPB) k_pack5(int nc, const double* __restrict__ P, const double* __restrict__ U,
+                                               const double* __restrict__ V, const double* __restrict__ W,
+                                               const double* __restrict__ VF, double* __restrict__ cpk) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double* o = cpk + 8 * (size_t)i;
+    o[3] = P[i]; o[4] = U[i]; o[5] = V[i]; o[6] = W[i]; o[7] = VF[i];
+}
+
+__global__ void __launch_bounds__(TPB) k_pack_xyz(int nc, const double* __restrict__ cxyz, double* __restrict__ cpk) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double* o = cpk + 8 * (size_t)i;
+    o[0] = cxyz[3 * (size_t)i]; o[1] = cxyz[3 * (size_t)i + 1]; o[2] = cxyz[3 * (size_t)i + 2];
+}
+
 __global__ void __launch_bounds__(TPB) k_grad_minmax(
-    int nc, int nif, int nbf, int with_p, const int* __restrict__ st_ptr, const int* __restrict__ st_idx,
-    const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx, const double* __restrict__ cxyz,
-    const double* __restrict__ fxyz, const double* __restrict__ lsq, const double* __restrict__ P,
-    const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
-    const double* __restrict__ VF, const double* __restrict__ bP, const double* __restrict__ bUR,
+    int nc, int nif, int nbf, int with_p, const int* __restrict__ st_sl, const int* __restrict__ st_ell,
+    const int* __restrict__ bf_ptr, const int* __restrict__ bf_idx, const double* __restrict__ cpk,
+    const double* __restrict__ fxyz, const double* __restrict__ lsq,
+    const double* __restrict__ bP, const double* __restrict__ bUR,
     const double* __restrict__ bVF, double* __restrict__ gP, double* __restrict__ gU,
     double* __restrict__ gV, double* __restrict__ gW, double* __restrict__ gVF,
     double* __restrict__ mxP, double* __restrict__ mnP, double* __restrict__ mxU, double* __restrict__ mnU,
     double* __restrict__ mxV, double* __restrict__ mnV, double* __restrict__ mxW, double* __restrict__ mnW) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
-    double xi = cxyz[3 * (size_t)i], yi = cxyz[3 * (size_t)i + 1], zi = cxyz[3 * (size_t)i + 2];
-    double ph[5] = {P[i], U[i], V[i], W[i], VF[i]};
+    const double2* me = reinterpret_cast<const double2*>(cpk + 8 * (size_t)i);
+    double2 m0 = me[0], m1 = me[1], m2 = me[2], m3 = me[3];
+    double xi = m0.x, yi = m0.y, zi = m1.x;
+    double ph[5] = {m1.y, m2.x, m2.y, m3.x, m3.y};
     double t[5][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     double mx[4] = {ph[0], ph[1], ph[2], ph[3]}, mn[4] = {ph[0], ph[1], ph[2], ph[3]};
-    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) {
-        int j = st_idx[q];
-        double dx = cxyz[3 * (size_t)j] - xi, dy = cxyz[3 * (size_t)j + 1] - yi, dz = cxyz[3 * (size_t)j + 2] - zi;
+    int s = i >> 6, lane = i & 63;
+    int e0 = st_sl[s], e1 = st_sl[s + 1];
+    for (int e = e0 + lane; e < e1; e += 64) {
+        int j = st_ell[e];
+        if (j < 0) break;
+        const double2* o = reinterpret_cast<const double2*>(cpk + 8 * (size_t)j);
+        double2 o0 = o[0], o1 = o[1], o2 = o[2], o3 = o[3];
+        double dx = o0.x - xi, dy = o0.y - yi, dz = o1.x - zi;
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
-        double pj[5] = {P[j], U[j], V[j], W[j], VF[j]};
+        double pj[5] = {o1.y, o2.x, o2.y, o3.x, o3.y};
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
             double d = pj[v] - ph[v];
@@ -210,9 +264,8 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
 #pragma unroll
         for (int v = 0; v < 4; ++v) { mx[v] = smax(mx[v], pj[v]); mn[v] = smin(mn[v], pj[v]); }
     }
-    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
-        int f = cf_idx[q];
-        if (f < nif) continue;
+    for (int q = bf_ptr[i]; q < bf_ptr[i + 1]; ++q) {
+        int f = bf_idx[q];
         int j = f - nif;
         double dx = fxyz[3 * (size_t)f] - xi, dy = fxyz[3 * (size_t)f + 1] - yi, dz = fxyz[3 * (size_t)f + 2] - zi;
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
@@ -242,7 +295,7 @@ __global__ void __launch_bounds__(TPB) k_lsq1(
     const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx, const double* __restrict__ cxyz,
     const double* __restrict__ fxyz, const double* __restrict__ lsq, const double* __restrict__ phi,
     const double* __restrict__ faceR, double* __restrict__ g) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double xi = cxyz[3 * (size_t)i], yi = cxyz[3 * (size_t)i + 1], zi = cxyz[3 * (size_t)i + 2];
     double p = phi[i], t0 = 0, t1 = 0, t2 = 0;
@@ -271,7 +324,7 @@ __global__ void __launch_bounds__(TPB) k_lsq1(
 __global__ void __launch_bounds__(TPB) k_minmax1(int nc, const int* __restrict__ st_ptr,
                                                  const int* __restrict__ st_idx, const double* __restrict__ phi,
                                                  double* __restrict__ mx, double* __restrict__ mn) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double a = phi[i], b = phi[i];
     for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) { double v = phi[st_idx[q]]; a = smax(a, v); b = smin(b, v); }
@@ -355,7 +408,7 @@ __device__ inline double face_unf(const FaceIn& in, const CaseK& k, int f, int l
 __global__ void __launch_bounds__(TPB) k_mom_face(
     int nif, CaseK k, FaceIn in, const double* __restrict__ Mu, const int* __restrict__ slot_own,
     const int* __restrict__ slot_nbr, double* __restrict__ aval, double* __restrict__ s) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nif) return;
     int l = in.owner[f], r = in.nbr[f];
     double area = in.area[f];
@@ -417,7 +470,7 @@ __global__ void __launch_bounds__(TPB) k_mom_cell(
     const double* __restrict__ bP, const double* __restrict__ bUL, const double* __restrict__ bUR,
     const double* __restrict__ bMu, const double* __restrict__ bRho, const double* __restrict__ s,
     double* __restrict__ diag, double* __restrict__ B) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double R = Rho[i], vl = vol[i], dt = k.dt;
     double d = R * 1.5 * vl / dt;
@@ -436,10 +489,9 @@ __global__ void __launch_bounds__(TPB) k_mom_cell(
         for (int v = 0; v < 3; ++v) b[v] -= ap * uc[v];
     }
     const double* gc[3] = {gU + 3 * (size_t)i, gV + 3 * (size_t)i, gW + 3 * (size_t)i};
-    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
-        int f = cf_idx[q];
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
         if (f < nif) {
-            if (owner[f] == i) {
+            if (own) {
                 d += s[f];
                 for (int v = 0; v < 3; ++v) b[v] -= s[(size_t)(2 + v) * nif + f];
             } else {
@@ -466,8 +518,8 @@ __global__ void __launch_bounds__(TPB) k_mom_cell(
         if (bc.u_type == SEMO_BC_ZERO_GRADIENT) { cU = 1.0; cUd = 0.0; }
         else if (bc.u_type == SEMO_BC_SLIP) { cU = 0.0; cUd = 0.0; }
         else if (bc.u_type == SEMO_BC_INLET_OUTLET) {
-            double own = uc[0] * n[0] + uc[1] * n[1] + uc[2] * n[2];
-            cU = own > 0.0 ? 1.0 : 0.0; cUd = own > 0.0 ? 0.0 : 1.0;
+            double ownU = uc[0] * n[0] + uc[1] * n[1] + uc[2] * n[2];
+            cU = ownU > 0.0 ? 1.0 : 0.0; cUd = ownU > 0.0 ? 0.0 : 1.0;
         }
         double muF = 0.5 * bMu[j] + 0.5 * bMu[j];
         double dPN = 0.5 * magPN[f], alpha = alphaF[f];
@@ -498,7 +550,7 @@ __global__ void __launch_bounds__(TPB) k_mom_cell(
 __global__ void __launch_bounds__(TPB) k_p_face(
     int nif, CaseK k, FaceIn in, const int* __restrict__ slot_own, const int* __restrict__ slot_nbr,
     double* __restrict__ aval, double* __restrict__ s) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nif) return;
     int l = in.owner[f], r = in.nbr[f];
     double area = in.area[f];
@@ -522,13 +574,12 @@ __global__ void __launch_bounds__(TPB) k_p_cell(
     const double* __restrict__ gP, const double* __restrict__ bP, const double* __restrict__ bUL,
     const double* __restrict__ bUR, const double* __restrict__ bRho, const double* __restrict__ s,
     double* __restrict__ diag, double* __restrict__ B) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double d = 0.0, b = 0.0;
-    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
-        int f = cf_idx[q];
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
         if (f < nif) {
-            if (owner[f] == i) { d += s[f]; b -= s[(size_t)nif + f]; }
+            if (own) { d += s[f]; b -= s[(size_t)nif + f]; }
             else { d -= -s[f]; b += s[(size_t)nif + f]; }
             continue;
         }
@@ -554,13 +605,211 @@ __global__ void __launch_bounds__(TPB) k_p_cell(
     B[i] = b;
 }
 
+// ---------------------------------------------------------- volume fraction --
+// largest face area of each cell (the Courant number of calcMSTACS,
+// NVD.cpp:1119-1127); static per mesh
+__global__ void __launch_bounds__(TPB) k_cell_max_area(int nc, const int* __restrict__ cf_ptr,
+                                                       const int* __restrict__ cf_idx,
+                                                       const double* __restrict__ area, double* __restrict__ maxA) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double m = 0.0;
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
+        (void)own;
+        m = smax(m, area[f]);
+    }
+    maxA[i] = m;
+}
+
+// calcMSTACS (NVD.cpp:1098-1862) for one internal face: the high-order
+// volume fraction on the owner (L) and neighbour (R) side.  Courant numbers
+// |U| dt maxA / V (NVD.cpp:1114-1135) are formed inline for the two cells.
+__global__ void __launch_bounds__(TPB) k_mstacs_face(
+    int nif, CaseK k, const int* __restrict__ owner, const int* __restrict__ nbr,
+    const double* __restrict__ wCv, const double* __restrict__ nrm, const double* __restrict__ skew,
+    const double* __restrict__ cxyz, const double* __restrict__ vol, const double* __restrict__ maxA,
+    const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
+    const double* __restrict__ VF, const double* __restrict__ gVF, double* __restrict__ vfL_out,
+    double* __restrict__ vfR_out) {
+    int f = xcd_block() * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    int l = owner[f], r = nbr[f];
+    double wCL = wCv[f], wCR = 1.0 - wCL;
+    double L1 = VF[l], R1 = VF[r];
+    const double eps = 1.e-12;
+    if ((L1 > 1.0 - eps || L1 < eps) && (R1 > 1.0 - eps || R1 < eps)) {   // both sides skipped
+        vfL_out[f] = L1; vfR_out[f] = R1;
+        return;
+    }
+    double gl[3], gr[3], pn[3];
+    for (int a = 0; a < 3; ++a) {
+        gl[a] = gVF[3 * (size_t)l + a]; gr[a] = gVF[3 * (size_t)r + a];
+        pn[a] = cxyz[3 * (size_t)r + a] - cxyz[3 * (size_t)l + a];      // vecPN (geometric.cpp:1425-1427)
+    }
+    const double* n = nrm + 3 * (size_t)f;
+    const double* sk = skew + 3 * (size_t)f;
+    double ul = sqrt(U[l] * U[l] + V[l] * V[l] + W[l] * W[l]);
+    double ur = sqrt(U[r] * U[r] + V[r] * V[r] + W[r] * W[r]);
+    double coL = ul * k.dt * maxA[l] / vol[l];
+    double coR = ur * k.dt * maxA[r] / vol[r];
+    double coDD = smax(coL, coR);
+    double L2 = 0.0;
+    for (int a = 0; a < 3; ++a) L2 += gl[a] * pn[a];
+    L2 = R1 - 2.0 * L2;
+    double R2 = 0.0;
+    for (int a = 0; a < 3; ++a) R2 += gr[a] * pn[a];
+    R2 = L1 + 2.0 * R2;
+    L2 = clip01(L2);
+    R2 = clip01(R2);
+    // blending weight from the interpolated interface normal (NVD.cpp:1437-1477)
+    double mf[3];
+    for (int a = 0; a < 3; ++a) mf[a] = wCL * gl[a] + wCR * gr[a];
+    double mag = mf[0] * mf[0];
+    mag += mf[1] * mf[1];
+    mag += mf[2] * mf[2];
+    mag = sqrt(mag);
+    for (int a = 0; a < 3; ++a) mf[a] = mf[a] / (mag + 1.e-200);
+    double cosT = mf[0] * n[0];
+    cosT += mf[1] * n[1];
+    cosT += mf[2] * n[2];
+    cosT = fabs(cosT);
+    double gamF = smin(pow(cosT, 4.0), 1.0);
+    double out[2] = {L1, R1};
+    for (int LR = 0; LR < 2; ++LR) {
+        double self = LR == 0 ? L1 : R1;
+        if (self > 1.0 - eps || self < eps) continue;
+        double alpU = LR == 0 ? L2 : R2, alpD = self, alpA = LR == 0 ? R1 : L1;
+        double tCd = (alpD - alpU) / (fabs(alpA - alpU) + 1.e-200) * (alpA > alpU ? 1.0 : -1.0);
+        double side = LR == 0 ? 2.0 * wCR : 2.0 * wCL;
+        // compressive branch (NVD.cpp:1602-1626)
+        double gam = 0.0;
+        if (tCd >= 0.0 && tCd < 1.0) {
+            if (coDD > 0.0 && coDD <= 0.33) {
+                double tCf = smin(1.0, tCd / coDD);
+                gam = (tCf - tCd) / (1.0 - tCd);
+            } else if (coDD > 0.33 && coDD <= 1.0) {
+                double tCf = smin(1.0, 3.0 * tCd);
+                gam = (tCf - tCd) / (1.0 - tCd);
+            }
+        }
+        gam *= side;
+        gam = clip01(gam);
+        double vc = alpD + gam * (alpA - alpD);
+        vc += wCL * gl[0] * sk[0]; vc += wCL * gl[1] * sk[1]; vc += wCL * gl[2] * sk[2];
+        vc += wCR * gr[0] * sk[0]; vc += wCR * gr[1] * sk[1]; vc += wCR * gr[2] * sk[2];
+        vc = clip01(vc);
+        // diffusive branch (NVD.cpp:1741-1767)
+        gam = 0.0;
+        if (tCd >= 0.0 && tCd < 0.2) { double tCf = smin(1.0, 3.0 * tCd); gam = (tCf - tCd) / (1.0 - tCd); }
+        else if (tCd >= 0.2 && tCd < 0.5) { double tCf = 0.5 + 0.5 * tCd; gam = (tCf - tCd) / (1.0 - tCd); }
+        else if (tCd >= 0.5 && tCd < 0.8333) { double tCf = 0.375 + 0.75 * tCd; gam = (tCf - tCd) / (1.0 - tCd); }
+        else if (tCd >= 0.8333 && tCd < 1.0) { double tCf = 1.0; gam = (tCf - tCd) / (1.0 - tCd); }
+        gam *= side;
+        gam = clip01(gam);
+        double vd = alpD + gam * (alpA - alpD);
+        vd += wCL * gl[0] * sk[0]; vd += wCL * gl[1] * sk[1]; vd += wCL * gl[2] * sk[2];
+        vd += wCR * gr[0] * sk[0]; vd += wCR * gr[1] * sk[1]; vd += wCR * gr[2] * sk[2];
+        vd = clip01(vd);
+        out[LR] = gamF * vc + (1.0 - gamF) * vd;
+    }
+    vfL_out[f] = out[0];
+    vfR_out[f] = out[1];
+}
+
+// internal face of calcVolfracEq (eqVolfrac.cpp:266-607): off-diagonals in
+// place; s[0] = owner diag +=, s[1] = neighbour diag -=, s[2] = VF_f UnF A
+// (owner B -=, neighbour B +=)
+__global__ void __launch_bounds__(TPB) k_vof_face(
+    int nif, CaseK k, FaceIn in, const double* __restrict__ vfL, const double* __restrict__ vfR,
+    const int* __restrict__ slot_own, const int* __restrict__ slot_nbr, double* __restrict__ aval,
+    double* __restrict__ s) {
+    int f = xcd_block() * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    int l = in.owner[f], r = in.nbr[f];
+    double A = in.area[f];
+    double UnF = face_unf(in, k, f, l, r);
+    double wL = (UnF > 0.0) ? 1.0 : 0.0, wR = 1.0 - wL;
+    double VFF = wL * vfL[f] + wR * vfR[f];
+    double ao = (wR * UnF * A);
+    double an = -(wL * UnF * A);
+    // entries with |v| < DBL_MIN are dropped (eqVolfrac.cpp:579-591)
+    aval[slot_own[f]] = fabs(ao) < 2.2250738585072014e-308 ? 0.0 : ao;
+    aval[slot_nbr[f]] = fabs(an) < 2.2250738585072014e-308 ? 0.0 : an;
+    s[f] = (wL * UnF * A);
+    s[(size_t)nif + f] = (wR * UnF * A);
+    s[2 * (size_t)nif + f] = (VFF * UnF * A);
+}
+
+// cell part of calcVolfracEq: time term (eqVolfrac.cpp:221-262), internal
+// faces in ascending order, then boundary faces (eqVolfrac.cpp:611-703)
+__global__ void __launch_bounds__(TPB) k_vof_cell(
+    int nc, int nif, int nbf, CaseK k, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ bpatch, const semo_patch_bc* __restrict__ bcs, const double* __restrict__ vol,
+    const double* __restrict__ nrm, const double* __restrict__ ePN, const double* __restrict__ area,
+    const double* __restrict__ magPN, const double* __restrict__ alphaF, const double* __restrict__ P,
+    const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
+    const double* __restrict__ VF, const double* __restrict__ oVF, const double* __restrict__ q3,
+    const double* __restrict__ gP, const double* __restrict__ bVF, const double* __restrict__ bUL,
+    const double* __restrict__ bUR, const double* __restrict__ bRho, const double* __restrict__ s,
+    double* __restrict__ diag, double* __restrict__ B) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double vl = vol[i], dt = k.dt;
+    double d = 1.5 * vl / dt;
+    double b = -(1.5 * VF[i] - 2.0 * oVF[i] + 0.5 * q3[i]) * vl / dt;
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
+        if (f < nif) {
+            if (own) { d += s[f]; b -= s[2 * (size_t)nif + f]; }
+            else { d -= s[(size_t)nif + f]; b += s[2 * (size_t)nif + f]; }
+            continue;
+        }
+        int j = f - nif;
+        semo_patch_bc bc = bcs[bpatch[j]];
+        const double* n = nrm + 3 * (size_t)f;
+        const double* e = ePN + 3 * (size_t)f;
+        double UnL = bUL[j] * n[0] + bUL[(size_t)nbf + j] * n[1] + bUL[2 * (size_t)nbf + j] * n[2];
+        double UnR = bUR[j] * n[0] + bUR[(size_t)nbf + j] * n[1] + bUR[2 * (size_t)nbf + j] * n[2];
+        double UnF = 0.5 * UnL + 0.5 * UnR;
+        double t1 = 1.0 / bRho[j] * dt;
+        double VFF = 0.5 * bVF[j] + 0.5 * bVF[j];
+        double cVF = 0.0;
+        if (bc.vf_type == SEMO_BC_ZERO_GRADIENT) cVF = 1.0;
+        else if (bc.vf_type == SEMO_BC_INLET_OUTLET) {
+            double ownU = U[i] * n[0] + V[i] * n[1] + W[i] * n[2];
+            cVF = (ownU > 0.0) ? 1.0 : 0.0;
+        }
+        double dPN = 0.5 * magPN[f], alpha = alphaF[f], A = area[f];
+        if (bc.p_type == SEMO_BC_FIXED_VALUE) {
+            UnF -= alpha * t1 * (bc.p_value - P[i]) / dPN;
+            for (int a = 0; a < 3; ++a) UnF += alpha * t1 * gP[3 * (size_t)i + a] * e[a];
+        }
+        d += cVF * (UnF * A);
+        b -= (VFF * UnF * A);
+    }
+    diag[i] = d;
+    B[i] = b;
+}
+
+// VF += urf dVF, clipped to [0,1]; block partials of dVF^2 (eqVolfrac.cpp:770-783)
+__global__ void __launch_bounds__(TPB) k_vof_update(int nc, double urf, const double* __restrict__ x,
+                                                    double* __restrict__ VF, double* __restrict__ part) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double s2 = 0.0;
+    if (i < nc) {
+        double xi = x[i];
+        VF[i] = clip01(VF[i] + urf * xi);
+        s2 = xi * xi;
+    }
+    block_sum2(s2, 0.0, part);
+}
+
 // ------------------------------------------- p' gradient and correction --
 // eqPressure.cpp:686-809: two Green-Gauss passes of p' with skew correction
 __global__ void __launch_bounds__(TPB) k_gg_face(int nif, const int* __restrict__ owner,
                                                  const int* __restrict__ nbr, const double* __restrict__ wCv,
                                                  const double* __restrict__ skew, const double* __restrict__ pp,
                                                  const double* __restrict__ gp, double* __restrict__ vF) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nif) return;
     int l = owner[f], r = nbr[f];
     double wCL = wCv[f], wCR = 1.0 - wCL;
@@ -575,17 +824,16 @@ __global__ void __launch_bounds__(TPB) k_gg_cell(
     const int* __restrict__ owner, const int* __restrict__ bpatch, const semo_patch_bc* __restrict__ bcs,
     const double* __restrict__ nrm, const double* __restrict__ area, const double* __restrict__ vol,
     const double* __restrict__ pp, const double* __restrict__ vF, double* __restrict__ gout) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double g[3] = {0, 0, 0};
     double vi = vol[i];
-    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
-        int f = cf_idx[q];
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
         const double* n = nrm + 3 * (size_t)f;
         double A = area[f];
         if (f < nif) {
             double v = vF[f];
-            if (owner[f] == i) { for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi; }
+            if (own) { for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi; }
             else { for (int a = 0; a < 3; ++a) g[a] -= v * n[a] * A / vi; }
         } else {
             double v = pp[i];
@@ -596,19 +844,6 @@ __global__ void __launch_bounds__(TPB) k_gg_cell(
     for (int a = 0; a < 3; ++a) gout[3 * (size_t)i + a] = g[a];
 }
 
-// block-partial sums of two quantities (deterministic two-stage reduction)
-__device__ inline void block_sum2(double a, double b, double* __restrict__ part) {
-    __shared__ double sa[TPB / 64], sb[TPB / 64];
-    for (int o = 32; o > 0; o >>= 1) { a += __shfl_down(a, o, 64); b += __shfl_down(b, o, 64); }
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) { sa[w] = a; sb[w] = b; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = 0, y = 0;
-        for (int k = 0; k < TPB / 64; ++k) { x += sa[k]; y += sb[k]; }
-        part[2 * blockIdx.x] = x; part[2 * blockIdx.x + 1] = y;
-    }
-}
 
 __device__ inline double clipv(double v, double lim) {
     if (v <= -lim) v = -lim;
@@ -621,7 +856,7 @@ __global__ void __launch_bounds__(TPB) k_p_correct(
     int nc, CaseK k, const double* __restrict__ pp, const double* __restrict__ gp,
     const double* __restrict__ Rho, double* __restrict__ P, double* __restrict__ U,
     double* __restrict__ V, double* __restrict__ W, double* __restrict__ part) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     double nd = 0.0, np = 0.0;
     if (i < nc) {
         double x = pp[i];
@@ -642,7 +877,7 @@ __global__ void __launch_bounds__(TPB) k_p_correct(
 __global__ void __launch_bounds__(TPB) k_mom_update(int nc, CaseK k, const double* __restrict__ X,
                                                     double* __restrict__ U, double* __restrict__ V,
                                                     double* __restrict__ W, double* __restrict__ part) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     double nd = 0.0, np = 0.0;
     if (i < nc) {
         double x0 = X[i], x1 = X[(size_t)nc + i], x2 = X[2 * (size_t)nc + i];
@@ -709,11 +944,18 @@ void fields_alloc(semo_ctx* c) {
     c->Bvec.alloc(3 * n); c->Bvec.zero(c->stream);
     c->Xvec.alloc(3 * n); c->Xvec.zero(c->stream);
     c->fscr.alloc(8 * (size_t)std::max(c->nif, 1));
-    c->red.alloc(2 * (size_t)nblk(std::max<long>(n, 1)) + 64);
-    c->scal.alloc(64);
+    reduction_alloc(c, (long)n);
     std::vector<double> one(n, 1.0);
     c->VF.upload(one, c->stream);
     c->oldVF.upload(one, c->stream);
+    c->cpk.alloc(8 * n);
+    hipLaunchKernelGGL(k_pack_xyz, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->c_xyz.p, c->cpk.p);
+    c->Qm3.alloc(n); c->Qm3.zero(c->stream);
+    c->fVFL.alloc(std::max(c->nif, 1)); c->fVFR.alloc(std::max(c->nif, 1));
+    c->c_maxA.alloc(n);
+    hipLaunchKernelGGL(k_cell_max_area, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->d_cf_sl.p,
+                       c->d_cf_ell.p, c->f_area.p, c->c_maxA.p);
+    SEMO_HIP(hipStreamSynchronize(c->stream));   // host sources above are temporaries
 }
 
 void eos_transport(semo_ctx* c) {
@@ -742,13 +984,21 @@ void source_gravity(semo_ctx* c) {
 }
 
 void grad_minmax(semo_ctx* c, bool with_p) {
-    // algorithmic bytes: own 5 fields + centroid + 6 coeffs + stencil ptr, 5 grads + 8 bounds out
-    double bytes = (double)c->nc * (5 * 8 + 24 + 48 + 4 + 5 * 24 + (with_p ? 64 : 48)) +
-                   4.0 * c->h_cf_ptr.back() * 0 + 4.0 * (double)c->d_st_idx.n;
+    {
+        ProfScope ps(c, "pack5", 80.0 * c->nc);
+        hipLaunchKernelGGL(k_pack5, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->P.p, c->U.p, c->V.p,
+                           c->W.p, c->VF.p, c->cpk.p);
+    }
+    // algorithmic bytes of a k-field LSQ gradient: 4 S (stencil indices) +
+    // k (8n in + 24n out) + 48n (coefficients) + 24n (centroids), k = 5, plus
+    // the stencil bounds written (6 or 8 arrays) and the boundary faces
+    double n = c->nc, S = (double)c->d_st_idx.n;
+    double bytes = 4.0 * S + 5.0 * 32.0 * n + 48.0 * n + 24.0 * n + (with_p ? 64.0 : 48.0) * n +
+                   (double)c->nbf * (24.0 + 40.0 + 4.0);
     ProfScope ps(c, "grad_minmax", bytes);
     hipLaunchKernelGGL(k_grad_minmax, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf,
-                       with_p ? 1 : 0, c->d_st_ptr.p, c->d_st_idx.p, c->d_cf_ptr.p, c->d_cf_idx.p, c->c_xyz.p,
-                       c->f_xyz.p, c->c_lsq.p, c->P.p, c->U.p, c->V.p, c->W.p, c->VF.p, c->bP.p, c->bUR.p,
+                       with_p ? 1 : 0, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p,
+                       c->f_xyz.p, c->c_lsq.p, c->bP.p, c->bUR.p,
                        c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p, c->gVF.p, c->mxP.p, c->mnP.p, c->mxU.p,
                        c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
 }
@@ -766,7 +1016,7 @@ void assemble_momentum(semo_ctx* c) {
         double bytes = (double)c->nc * 8 * (1 + 1 + 3 + 3 + 3 + 4 + 4) + 8.0 * 8 * c->nif + 4.0 * c->h_cf_idx.size();
         ProfScope ps(c, "mom_cell", bytes);
         hipLaunchKernelGGL(k_mom_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
-                           c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p,
+                           c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p,
                            c->f_nrm.p, c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->Rho.p, c->P.p,
                            c->U.p, c->V.p, c->W.p, c->oldU.p, c->oldV.p, c->oldW.p, c->Qm0.p, c->Qm1.p, c->Qm2.p,
                            c->srcG.p, c->srcS.p, c->alphaP.n ? c->alphaP.p : nullptr, c->gP.p, c->gU.p, c->gV.p,
@@ -788,11 +1038,55 @@ void assemble_pressure(semo_ctx* c) {
     {
         ProfScope ps(c, "p_cell", (double)c->nc * 16 + 16.0 * c->nif + 4.0 * c->h_cf_idx.size());
         hipLaunchKernelGGL(k_p_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
-                           c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p,
+                           c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p,
                            c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->P.p, c->gP.p, c->bP.p,
                            c->bUL.p, c->bUR.p, c->bRho.p, c->fscr.p, c->Amat.diag.p, c->Bvec.p);
     }
     c->last_eq = 1;
+}
+
+// setIncomValuesLeftRightFaceWithVfMSTACS (reconIncom.cpp:345-430) and the
+// calcVolfracEq assembly (eqVolfrac.cpp:9-765).  The LSQ gradient of alpha
+// that MSTACS needs is the one of the five-field pass the assembly needs
+// anyway (same field, same boundary values).
+void assemble_vof(semo_ctx* c) {
+    CaseK k = case_k(c);
+    grad_minmax(c, false);
+    if (c->nif) {
+        {
+            ProfScope ps(c, "mstacs_face", (double)c->nif * (8 * 8 + 8 * 3 + 16) + (double)c->nc * 8 * 12);
+            hipLaunchKernelGGL(k_mstacs_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, k,
+                               c->d_owner.p, c->d_nbr.p, c->f_wC.p, c->f_nrm.p, c->f_skew.p, c->c_xyz.p,
+                               c->c_vol.p, c->c_maxA.p, c->U.p, c->V.p, c->W.p, c->VF.p, c->gVF.p, c->fVFL.p,
+                               c->fVFR.p);
+        }
+        double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 16 + 3 * 8) + (double)c->nc * 8 * (4 + 12 + 6 + 2);
+        ProfScope ps(c, "vof_face", bytes);
+        hipLaunchKernelGGL(k_vof_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, k, face_in(c),
+                           c->fVFL.p, c->fVFR.p, c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
+    }
+    {
+        ProfScope ps(c, "vof_cell", (double)c->nc * 8 * 6 + 24.0 * c->nif + 4.0 * c->h_cf_idx.size());
+        hipLaunchKernelGGL(k_vof_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
+                           c->d_cf_sl.p, c->d_cf_ell.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p, c->f_nrm.p,
+                           c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->P.p, c->U.p, c->V.p, c->W.p,
+                           c->VF.p, c->oldVF.p, c->Qm3.p, c->gP.p, c->bVF.p, c->bUL.p, c->bUR.p, c->bRho.p,
+                           c->fscr.p, c->Amat.diag.p, c->Bvec.p);
+    }
+    c->last_eq = 2;
+}
+
+// VF += vofVofURF dVF, clipped; returns sqrt(sum dVF^2) (eqVolfrac.cpp:770-783)
+double vof_update(semo_ctx* c) {
+    int nb = nblk(c->nc);
+    hipLaunchKernelGGL(k_vof_update, dim3(nb), dim3(TPB), 0, c->stream, c->nc, c->cs.urf_vf, c->Xvec.p, c->VF.p,
+                       c->red.p);
+    std::vector<double> h(2 * (size_t)nb);
+    SEMO_HIP(hipMemcpyAsync(h.data(), c->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    double a = 0;
+    for (int q = 0; q < nb; ++q) a += h[2 * q];
+    return sqrt(a);
 }
 
 double momentum_update(semo_ctx* c) {
@@ -811,8 +1105,8 @@ double pressure_correct(semo_ctx* c) {
         if (c->nif)
             hipLaunchKernelGGL(k_gg_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, c->d_owner.p,
                                c->d_nbr.p, c->f_wC.p, c->f_skew.p, pp, c->gpp.p, c->fscr.p);
-        hipLaunchKernelGGL(k_gg_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->d_cf_ptr.p,
-                           c->d_cf_idx.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p,
+        hipLaunchKernelGGL(k_gg_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->d_cf_sl.p,
+                           c->d_cf_ell.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p,
                            c->c_vol.p, pp, c->fscr.p, c->gtmp.p);
         std::swap(c->gpp, c->gtmp);
     }

@@ -58,6 +58,7 @@ void to_dev(semo_ctx* c, DBuf<double>& d, const double* h, int comps) {
     for (int i = 0; i < c->nc; ++i)
         for (int k = 0; k < comps; ++k) t[(size_t)i * comps + k] = h[(size_t)c->perm[i] * comps + k];
     d.upload(t, c->stream);
+    SEMO_HIP(hipStreamSynchronize(c->stream));   // t is a temporary
 }
 void from_dev(semo_ctx* c, const double* dptr, double* h, int comps, size_t stride_block = 0) {
     std::vector<double> t((size_t)c->nc * comps);
@@ -90,6 +91,7 @@ DBuf<double>* cell_field(semo_ctx* c, int f, int& comps) {
         case SEMO_F_MINU: return &c->mnU; case SEMO_F_MAXV: return &c->mxV; case SEMO_F_MINV: return &c->mnV;
         case SEMO_F_MAXW: return &c->mxW; case SEMO_F_MINW: return &c->mnW;
         case SEMO_F_ALPHA_P: return &c->alphaP;
+        case SEMO_F_QM3: return &c->Qm3; case SEMO_F_OLDVF: return &c->oldVF;
         case SEMO_F_GRADP: comps = 3; return &c->gP; case SEMO_F_GRADU: comps = 3; return &c->gU;
         case SEMO_F_GRADV: comps = 3; return &c->gV; case SEMO_F_GRADW: comps = 3; return &c->gW;
         case SEMO_F_GRADVF: comps = 3; return &c->gVF;
@@ -120,12 +122,29 @@ void step_setup(semo_ctx* c) {
     const DBuf<double>& q0 = c->iterReal == 0 ? c->U : c->oldU;
     const DBuf<double>& q1 = c->iterReal == 0 ? c->V : c->oldV;
     const DBuf<double>& q2 = c->iterReal == 0 ? c->W : c->oldW;
+    const DBuf<double>& q3 = c->iterReal == 0 ? c->VF : c->oldVF;
     copy_d(c, c->Qm0.p, q0.p, c->nc); copy_d(c, c->Qm1.p, q1.p, c->nc); copy_d(c, c->Qm2.p, q2.p, c->nc);
+    copy_d(c, c->Qm3.p, q3.p, c->nc);
     copy_d(c, c->oldP.p, c->P.p, c->nc); copy_d(c, c->oldU.p, c->U.p, c->nc);
     copy_d(c, c->oldV.p, c->V.p, c->nc); copy_d(c, c->oldW.p, c->W.p, c->nc);
     copy_d(c, c->oldVF.p, c->VF.p, c->nc);
+}
+
+// after the VOF loop: face values and the body-force source of the step
+// (pressureBased.cpp:128-130)
+void step_sources(semo_ctx* c) {
     recon_boundary(c);
-    source_gravity(c);        // calcSourceGravity (pressureBased.cpp:141)
+    source_gravity(c);
+}
+
+// one VOF iteration (pressureBased.cpp:104-127)
+double volfrac_eq(semo_ctx* c, semo_solve_stats* st) {
+    recon_boundary(c);
+    assemble_vof(c);
+    solve_momentum(c, 1, st);
+    double nv = vof_update(c);
+    eos_transport(c);
+    return nv;
 }
 
 double momentum_eq(semo_ctx* c, semo_solve_stats* st) {
@@ -240,6 +259,7 @@ int semo_set_case(semo_ctx* c, const semo_case* cs, const semo_patch_bc* bcs) {
         SEMO_CHECK(cs->dt > 0 && cs->rho[0] > 0 && cs->rho[1] > 0 && cs->mu[0] > 0 && cs->mu[1] > 0,
                    SEMO_EINVAL, "case: dt, rho and mu must be positive");
         SEMO_CHECK(cs->sigma == 0.0, SEMO_EINVAL, "surface tension (curvature) is not available on this path");
+        SEMO_CHECK(cs->n_vof >= 0 && cs->urf_vf > 0.0, SEMO_EINVAL, "case: n_vof >= 0 and urf_vf > 0");
         for (int b = 0; b < c->npatch; ++b) {
             const semo_patch_bc& p = bcs[b];
             SEMO_CHECK(p.p_type == SEMO_BC_ZERO_GRADIENT || p.p_type == SEMO_BC_FIXED_VALUE, SEMO_EINVAL_BC,
@@ -278,9 +298,16 @@ int semo_field_set(semo_ctx* c, int f, const double* h, int64_t n) {
         need_mesh(c);
         int comps;
         DBuf<double>* d = cell_field(c, f, comps);
+        if (f >= SEMO_F_SRCG0 && f <= SEMO_F_SRCG2) {   // one component of the gravity source
+            SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
+            DBuf<double> t;
+            to_dev(c, t, h, 1);
+            copy_d(c, c->srcG.p + (size_t)(f - SEMO_F_SRCG0) * c->nc, t.p, c->nc);
+            SEMO_HIP(hipStreamSynchronize(c->stream));
+            return;
+        }
         SEMO_CHECK(d && f < SEMO_G_CELL_VOLUME, SEMO_EINVAL, "field not settable");
         SEMO_CHECK(n == (int64_t)c->nc * comps, SEMO_EINVAL, "field size");
-        if (f >= SEMO_F_SRCG0 && f <= SEMO_F_SRCG2) SEMO_CHECK(false, SEMO_EINVAL, "field not settable");
         to_dev(c, *d, h, comps);
         SEMO_HIP(hipStreamSynchronize(c->stream));
     });
@@ -303,6 +330,22 @@ int semo_field_get(semo_ctx* c, int f, double* h, int64_t n) {
         if (f >= SEMO_F_B0 && f <= SEMO_F_B2) {
             SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
             from_dev(c, c->Bvec.p + (size_t)(f - SEMO_F_B0) * c->nc, h, 1);
+            return;
+        }
+        if (f == SEMO_F_FACE_VFL || f == SEMO_F_FACE_VFR) {
+            SEMO_CHECK(n == c->nf, SEMO_EINVAL, "field size");
+            std::vector<double> l(std::max(c->nif, 1)), r(std::max(c->nif, 1)), b(std::max(c->nbf, 1));
+            c->fVFL.download(l.data(), c->nif, c->stream);
+            c->fVFR.download(r.data(), c->nif, c->stream);
+            c->bVF.download(b.data(), c->nbf, c->stream);
+            SEMO_HIP(hipStreamSynchronize(c->stream));
+            bool left = f == SEMO_F_FACE_VFL;
+            for (int k = 0; k < c->nf; ++k) {
+                double v;
+                if (k < c->nif) v = (left != (c->fflip[k] != 0)) ? l[k] : r[k];   // a flipped face swaps sides
+                else v = b[k - c->nif];
+                h[c->fperm[k]] = v;
+            }
             return;
         }
         if (f >= SEMO_G_FACE_AREA) {
@@ -352,11 +395,23 @@ int semo_calc_pressure_eq(semo_ctx* c, double* norm, semo_solve_stats* st) {
     });
 }
 
+int semo_calc_volfrac_eq(semo_ctx* c, double* norm, semo_solve_stats* st) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        double nv = volfrac_eq(c, &c->last_stats_vf);
+        if (st) *st = c->last_stats_vf;
+        if (norm) *norm = nv;
+    });
+}
+
 int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
     if (!c || ncorr < 1) return SEMO_EINVAL;
     return guard(c, [&] {
         need_case(c);
         step_setup(c);
+        for (int k = 0; k < c->cs.n_vof; ++k) volfrac_eq(c, &c->last_stats_vf);
+        step_sources(c);
         for (int it = 0; it < ncorr; ++it) {
             double n0 = momentum_eq(c, &c->last_stats_mom);
             double n1 = pressure_eq(c, &c->last_stats_p);
@@ -367,8 +422,8 @@ int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
 }
 
 int semo_get_solve_stats(const semo_ctx* c, int eq, semo_solve_stats* st) {
-    if (!c || !st || (eq != 0 && eq != 1)) return SEMO_EINVAL;
-    *st = eq == 0 ? c->last_stats_mom : c->last_stats_p;
+    if (!c || !st || eq < 0 || eq > 2) return SEMO_EINVAL;
+    *st = eq == 0 ? c->last_stats_mom : eq == 1 ? c->last_stats_p : c->last_stats_vf;
     return SEMO_OK;
 }
 
@@ -376,9 +431,11 @@ int semo_assemble(semo_ctx* c, int eq) {
     if (!c) return SEMO_EINVAL;
     return guard(c, [&] {
         need_case(c);
-        SEMO_CHECK(eq == 0 || eq == 1, SEMO_EINVAL, "eq must be 0 (momentum) or 1 (pressure)");
+        SEMO_CHECK(eq >= 0 && eq <= 2, SEMO_EINVAL, "eq must be 0 (momentum), 1 (pressure) or 2 (volume fraction)");
         recon_boundary(c);
-        if (eq == 0) assemble_momentum(c); else assemble_pressure(c);
+        if (eq == 0) assemble_momentum(c);
+        else if (eq == 1) assemble_pressure(c);
+        else assemble_vof(c);
         SEMO_HIP(hipStreamSynchronize(c->stream));
     });
 }
@@ -400,8 +457,9 @@ int semo_matrix_get(semo_ctx* c, int64_t* nnz, int32_t* rows, int32_t* cols, dou
         for (int f = 0; f < c->nif; ++f) {
             int so = c->h_slot_own[f], sn = c->h_slot_nbr[f];
             int o = c->h_owner[f], n = c->h_nbr[f];
-            if (!(c->last_eq == 0 && val[so] == 0.0)) { r.push_back(c->perm[o]); cc.push_back(c->perm[n]); v.push_back(val[so]); }
-            if (!(c->last_eq == 0 && val[sn] == 0.0)) { r.push_back(c->perm[n]); cc.push_back(c->perm[o]); v.push_back(val[sn]); }
+            bool drop = c->last_eq != 1;   // momentum and VOF drop |v| < DBL_MIN, pressure keeps all
+            if (!(drop && val[so] == 0.0)) { r.push_back(c->perm[o]); cc.push_back(c->perm[n]); v.push_back(val[so]); }
+            if (!(drop && val[sn] == 0.0)) { r.push_back(c->perm[n]); cc.push_back(c->perm[o]); v.push_back(val[sn]); }
         }
         (void)p;
         m = (int64_t)r.size();

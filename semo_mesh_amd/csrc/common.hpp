@@ -37,6 +37,19 @@ struct Error : std::runtime_error {
 enum : int { FLAG_ZERO_AREA = 1, FLAG_NEG_VOLUME = 2, FLAG_SINGULAR_LSQ = 4,
              FLAG_BAD_FACE = 8, FLAG_ZERO_DIAG = 16 };
 
+// ------------------------------------------------------ XCD-aware blocks ---
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8), and each XCD has its own L2.  Remapping the block index so that
+// each XCD walks one contiguous range of the grid, in order, keeps the cells
+// and faces that neighbour each other (RCM order) in the same L2.  This is a
+// bijection on [0, gridDim.x) for any grid size; only locality depends on the
+// dispatch order, never correctness.
+__device__ inline int xcd_block() {
+    int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // ------------------------------------------------------------ device buffer -
 template <class T>
 struct DBuf {
@@ -99,6 +112,9 @@ struct Profiler {
 // per row), column-major inside a slice, plus a separate diagonal.  Padding
 // entries have val = 0 and a column inside [0, ncols) (the row itself for
 // square matrices, 0 for rectangular ones).
+// Operators with long or ragged rows (AMG restriction and coarse levels) use
+// CSR instead (lanes > 0): sl_ptr then holds the n+1 row offsets and `lanes`
+// consecutive lanes of a wavefront share one row.
 constexpr int SL = 64;
 struct SellPattern {
     int n = 0, ncols = 0, nslices = 0;
@@ -109,6 +125,7 @@ struct SellPattern {
 template <class T>
 struct DevSell {
     int n = 0, ncols = 0, nslices = 0;
+    int lanes = 0;              // 0: sliced ELL; 4..64: CSR, lanes per row
     int64_t nnz = 0;            // real entries incl. diagonal
     DBuf<int> sl_ptr, col;
     DBuf<T> val, diag;
@@ -125,9 +142,9 @@ struct AmgLevel {
     DevSell<float> P32, R32;
     DBuf<double> dinv64; DBuf<float> dinv32;
     double lmax = 2.0;          // spectral bound of D^-1 A
-    // work vectors
-    DBuf<double> x64, b64, r64, t64, u64;
-    DBuf<float> x32, b32, r32, t32, u32;
+    // work vectors (w: second buffer of the iterate x for out-of-place updates)
+    DBuf<double> x64, b64, r64, t64, u64, w64;
+    DBuf<float> x32, b32, r32, t32, u32, w32;
 };
 
 struct Amg {
@@ -162,7 +179,14 @@ struct semo_ctx {
     semo::DBuf<double> d_pts;
     semo::DBuf<int> d_owner, d_nbr, d_fptr, d_fpts;
     semo::DBuf<int> d_cf_ptr, d_cf_idx;    // cell -> faces, ascending
-    semo::DBuf<int> d_st_ptr, d_st_idx;    // vertex stencil
+    semo::DBuf<int> d_st_ptr, d_st_idx;    // vertex stencil (CSR)
+    // the same stencil as sliced ELL: 64-cell slices, entry k of cell
+    // 64s+lane at st_sl[s] + 64k + lane, padding -1 after the last entry
+    semo::DBuf<int> d_st_sl, d_st_ell;
+    // cell -> faces as sliced ELL (ascending faces; ~f for a face the cell
+    // is the neighbour of; padding INT_MIN) and cell -> boundary faces (CSR)
+    semo::DBuf<int> d_cf_sl, d_cf_ell, d_bf_ptr, d_bf_idx;
+    semo::DBuf<double> cpk;                // packed [x,y,z,P,U,V,W,VF] per cell
     semo::DBuf<int> d_bpatch;              // boundary face -> patch
     semo::DBuf<int> d_flags;               // kernel error flags
     // geometry
@@ -190,6 +214,9 @@ struct semo_ctx {
     // ---- fields (device numbering) ----
     semo::DBuf<double> P, U, V, W, VF, Rho, Mu, oldP, oldU, oldV, oldW, oldVF, Qm0, Qm1, Qm2,
         srcG, srcS, kappa, alphaP;
+    semo::DBuf<double> Qm3;                    // alpha two time levels back (controls.Qm[3])
+    semo::DBuf<double> c_maxA;                 // largest face area per cell (static)
+    semo::DBuf<double> fVFL, fVFR;             // MSTACS face volume fractions [nif]
     semo::DBuf<double> gP, gU, gV, gW, gVF;   // [3n] interleaved
     semo::DBuf<double> mxP, mnP, mxU, mnU, mxV, mnV, mxW, mnW;
     semo::DBuf<double> bP, bUL, bUR, bVF, bMu, bRho;   // boundary faces (bUL/bUR: [3][nbf])
@@ -205,7 +232,7 @@ struct semo_ctx {
     semo::DBuf<int> d_flag1;
     semo::DBuf<double> kv64;                   // Krylov work (fp64)
     semo::DBuf<float> kv32;
-    semo_solve_stats last_stats_mom{}, last_stats_p{};
+    semo_solve_stats last_stats_mom{}, last_stats_p{}, last_stats_vf{};
 };
 
 namespace semo {
@@ -224,18 +251,21 @@ void source_gravity(semo_ctx* c);
 void grad_minmax(semo_ctx* c, bool with_p_minmax);
 void assemble_momentum(semo_ctx* c);
 void assemble_pressure(semo_ctx* c);
+void assemble_vof(semo_ctx* c);
+double vof_update(semo_ctx* c);
 double momentum_update(semo_ctx* c);
 double pressure_correct(semo_ctx* c);
 void lsq_standalone(semo_ctx* c, const double* d_phi, const double* d_faceR, double* d_grad);
 void minmax_standalone(semo_ctx* c, const double* d_phi, double* d_mx, double* d_mn);
 void copy_d(semo_ctx* c, double* dst, const double* src, size_t n);
-// linalg.hip
+// solver.hip
 template <class T> void sell_upload(DevSell<T>& M, const SellPattern& p, const std::vector<double>& vals,
                                     const std::vector<double>& diag, hipStream_t s);
-void spmv64(semo_ctx* c, const DevSell<double>& A, const double* x, double* y);
-double dot64(semo_ctx* c, const double* a, const double* b, int n);
-double norm64(semo_ctx* c, const double* a, int n);
-// krylov.cpp / amg.cpp
+template <class T> void csr_upload(DevSell<T>& M, int n, int ncols, const std::vector<int>& ptr,
+                                   const std::vector<int>& col, const std::vector<double>& vals,
+                                   const std::vector<double>& diag, int lanes, hipStream_t s);
+void reduction_alloc(semo_ctx* c, long n);
+// amg_setup.cpp / solver.hip
 void amg_build(semo_ctx* c, Amg& amg, const DevSell<double>& A);
 void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* st);
 void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st);

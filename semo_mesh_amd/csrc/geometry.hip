@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(TPB) k_face_polygon(
     const double* __restrict__ pts, double* __restrict__ area, double* __restrict__ nrm,
     double* __restrict__ fxyz, double* __restrict__ vsn_out, double* __restrict__ cc_out,
     int* __restrict__ flags) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     int b0 = fptr[f], n = fptr[f + 1] - b0;
     if (n < 3 || n > 16) { atomicOr(flags, FLAG_BAD_FACE); return; }
@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(TPB) k_cell_volume(
     int nc, int nif, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
     const int* __restrict__ owner, const double* __restrict__ vsn, const double* __restrict__ cc,
     double* __restrict__ vol, double* __restrict__ cxyz, int* __restrict__ flags) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double v = 0.0, x = 0.0, y = 0.0, z = 0.0;
     for (int k = cf_ptr[i]; k < cf_ptr[i + 1]; ++k) {
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(TPB) k_face_metrics(
     double* __restrict__ wC_o, double* __restrict__ magPN_o, double* __restrict__ alpha_o,
     double* __restrict__ ePN_o, double* __restrict__ skew_o, double* __restrict__ pf_o,
     double* __restrict__ nf_o) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     const double* n = nrm + 3 * (size_t)f;
     const double* fc = fxyz + 3 * (size_t)f;
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(TPB) k_lsq_init(
     int nc, int nif, const int* __restrict__ st_ptr, const int* __restrict__ st_idx,
     const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx, const double* __restrict__ cxyz,
     const double* __restrict__ fxyz, double* __restrict__ lsq, int* __restrict__ flags) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     double xi = cxyz[3 * (size_t)i], yi = cxyz[3 * (size_t)i + 1], zi = cxyz[3 * (size_t)i + 2];

@@ -6,7 +6,9 @@
 // but never applies it; here it is applied to storage so that owner /
 // neighbour gathers and SpMV column accesses stay inside a narrow band.
 #include <algorithm>
+#include <climits>
 #include <cstring>
+#include <functional>
 #include <numeric>
 
 #include "common.hpp"
@@ -341,6 +343,41 @@ void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
     c->d_cf_idx.upload(c->h_cf_idx, s);
     c->d_st_ptr.upload(st_ptr, s);
     c->d_st_idx.upload(st_idx, s);
+    // sliced-ELL copies of the stencil and of cell -> faces for coalesced
+    // gathers (same per-cell order as the CSR lists)
+    auto to_sliced = [&](const std::vector<int>& ptr, const std::vector<int>& idx, int pad,
+                         const std::function<int(int, int)>& enc, std::vector<int>& sl,
+                         std::vector<int>& ell) {
+        int ns = (nc + SL - 1) / SL;
+        sl.assign(ns + 1, 0);
+        for (int sl_i = 0; sl_i < ns; ++sl_i) {
+            int w = 0;
+            for (int r = sl_i * SL; r < std::min(nc, (sl_i + 1) * SL); ++r) w = std::max(w, ptr[r + 1] - ptr[r]);
+            sl[sl_i + 1] = sl[sl_i] + w * SL;
+        }
+        ell.assign(sl[ns], pad);
+        for (int r = 0; r < nc; ++r) {
+            int sl_i = r / SL, lane = r % SL;
+            for (int k = ptr[r]; k < ptr[r + 1]; ++k) ell[sl[sl_i] + (k - ptr[r]) * SL + lane] = enc(r, idx[k]);
+        }
+    };
+    {
+        std::vector<int> sl, ell;
+        to_sliced(st_ptr, st_idx, -1, [](int, int j) { return j; }, sl, ell);
+        c->d_st_sl.upload(sl, s); c->d_st_ell.upload(ell, s);
+        SEMO_HIP(hipStreamSynchronize(s));
+        to_sliced(c->h_cf_ptr, c->h_cf_idx, INT32_MIN,
+                  [&](int r, int f) { return c->h_owner[f] == r ? f : ~f; }, sl, ell);
+        c->d_cf_sl.upload(sl, s); c->d_cf_ell.upload(ell, s);
+        std::vector<int> bptr(nc + 1, 0), bidx;
+        for (int r = 0; r < nc; ++r) {
+            for (int k = c->h_cf_ptr[r]; k < c->h_cf_ptr[r + 1]; ++k)
+                if (c->h_cf_idx[k] >= nif) bidx.push_back(c->h_cf_idx[k]);
+            bptr[r + 1] = (int)bidx.size();
+        }
+        c->d_bf_ptr.upload(bptr, s); c->d_bf_idx.upload(bidx, s);
+        SEMO_HIP(hipStreamSynchronize(s));
+    }
     c->d_bpatch.upload(bpatch, s);
     c->d_slot_own.upload(c->h_slot_own, s);
     c->d_slot_nbr.upload(c->h_slot_nbr, s);

@@ -1,10 +1,18 @@
 // Linear algebra and Krylov solvers on the device.
 //
-// * sliced-ELL SpMV (one wavefront per 64-row slice, coalesced val/col loads),
-//   FP64 and FP32 values;
-// * deterministic two-stage dot products (wavefront shuffles -> per-block
-//   partials -> one ordered final sum), results kept in device memory so the
-//   Krylov scalars never leave the GPU except for the convergence test;
+// * row kernels over two operator formats: sliced ELL (one lane per row,
+//   coalesced val/col loads; short regular rows such as the fine-mesh
+//   operator and the prolongation) and CSR with 4..64 lanes per row (long or
+//   ragged rows: AMG restriction and Galerkin coarse operators);
+// * fused Chebyshev steps (SpMV + residual/direction/iterate updates in one
+//   pass) and a fused post-smoother start (A x, residual, first direction);
+// * deterministic reductions fused into the kernels that produce their
+//   operands: per-block partials in a fixed order, then one single-block
+//   kernel sums them in a fixed order and computes the Krylov scalars, so
+//   they never leave the GPU except for the convergence test.  (A
+//   last-block-done scheme would need a device-scope release fence in every
+//   block, i.e. an L2 write-back per block on this chip: measured ~0.3 ms per
+//   launch at 2M rows, against ~5 us for the extra launch.)
 // * AMG V-cycle with Chebyshev smoothing, smoothed-aggregation transfer
 //   operators and a dense FP64 coarsest solve (explicit inverse, one GEMV);
 // * flexible PCG for the pressure (FP64 AMG, or FP32 AMG inside the FP64
@@ -27,49 +35,9 @@ void amg_build_csr(semo_ctx* c, Amg& amg, int n, const std::vector<int>& ptr,
 namespace {
 
 constexpr int TPB = 256;
-constexpr int RB = 1024;   // max reduction blocks
+constexpr int RB = 1024;   // grid of the grid-stride reductions
 inline int nblk(long n) { return (int)((n + TPB - 1) / TPB); }
 inline int rblk(long n) { return (int)std::min<long>(RB, std::max<long>(1, (n + TPB - 1) / TPB)); }
-
-// ------------------------------------------------------------------- SpMV --
-// y = beta_y*y + alpha*(D x + O x) with D the diagonal (optional) and O the
-// sliced-ELL off-diagonal part; MODE 0: y = A x, 1: y = b - A x, 2: y += A x
-template <class T, class TX, class TY, int MODE>
-__global__ void __launch_bounds__(TPB) k_spmv(int n, const int* __restrict__ sl_ptr,
-                                              const int* __restrict__ col, const T* __restrict__ val,
-                                              const T* __restrict__ diag, const TX* __restrict__ x,
-                                              const TY* __restrict__ b, TY* __restrict__ y) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    int s = row >> 6, lane = row & 63;
-    int nsl = (n + 63) >> 6;
-    if (s >= nsl) return;
-    int e0 = sl_ptr[s], e1 = sl_ptr[s + 1];
-    TY acc = 0;
-    for (int e = e0 + lane; e < e1; e += 64) acc += (TY)val[e] * (TY)x[col[e]];
-    if (row >= n) return;
-    if (diag) acc += (TY)diag[row] * (TY)x[row];
-    if (MODE == 0) y[row] = acc;
-    else if (MODE == 1) y[row] = b[row] - acc;
-    else y[row] += acc;
-}
-
-template <class T, class TX, class TY, int MODE>
-void spmv(semo_ctx* c, const DevSell<T>& A, const TX* x, const TY* b, TY* y, bool with_diag = true,
-          const char* tag = nullptr) {
-    if (A.n == 0) return;
-    hipEvent_t ev{};
-    bool prof = c->prof.on && tag;
-    if (prof) c->prof.begin(c->stream, ev);
-    int nrows = A.nslices * 64;
-    hipLaunchKernelGGL((k_spmv<T, TX, TY, MODE>), dim3(nblk(nrows)), dim3(TPB), 0, c->stream, A.n,
-                       A.sl_ptr.p, A.col.p, A.val.p, with_diag ? A.diag.p : (const T*)nullptr, x, b, y);
-    if (prof) {
-        // algorithmic bytes: values + columns of the real entries, x and y once
-        double bytes = (double)A.nnz * (sizeof(T) + 4) + (double)A.n * (sizeof(TX) + sizeof(TY) * (MODE == 0 ? 1 : 2)) +
-                       4.0 * (A.nslices + 1);
-        c->prof.end(c->stream, tag, ev, bytes);
-    }
-}
 
 // ------------------------------------------------------------- reductions --
 template <class T>
@@ -78,179 +46,324 @@ __device__ inline T wave_sum(T v) {
     return v;
 }
 
-// partial sums of a.b and c.d (c,d may be null) over a grid-stride range
-template <class TA>
-__global__ void __launch_bounds__(TPB) k_dot2(int n, const TA* __restrict__ a, const TA* __restrict__ b,
-                                              const TA* __restrict__ cc, const TA* __restrict__ d,
-                                              double* __restrict__ part) {
-    double s1 = 0.0, s2 = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        s1 += (double)a[i] * (double)b[i];
-        if (cc) s2 += (double)cc[i] * (double)d[i];
-    }
-    __shared__ double sh1[TPB / 64], sh2[TPB / 64];
-    s1 = wave_sum(s1); s2 = wave_sum(s2);
-    if ((threadIdx.x & 63) == 0) { sh1[threadIdx.x >> 6] = s1; sh2[threadIdx.x >> 6] = s2; }
+// Sums the two per-thread values over the block in a fixed order and stores
+// them as this block's partials part[2b], part[2b+1].  Every thread of the
+// block must call it.
+__device__ inline void block_partials(double (&v)[2], double* __restrict__ part) {
+    __shared__ double sh[2][TPB / 64];
+    v[0] = wave_sum(v[0]); v[1] = wave_sum(v[1]);
+    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = v[0]; sh[1][threadIdx.x >> 6] = v[1]; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t1 = 0, t2 = 0;
-        for (int w = 0; w < TPB / 64; ++w) { t1 += sh1[w]; t2 += sh2[w]; }
-        part[2 * blockIdx.x] = t1; part[2 * blockIdx.x + 1] = t2;
+        double t0 = 0.0, t1 = 0.0;
+        for (int w = 0; w < TPB / 64; ++w) { t0 += sh[0][w]; t1 += sh[1][w]; }
+        part[2 * blockIdx.x] = t0; part[2 * blockIdx.x + 1] = t1;
     }
 }
 
-// ordered final sum of nb partial pairs into out[0], out[1]
-__global__ void __launch_bounds__(TPB) k_dot_final(int nb, const double* __restrict__ part,
-                                                   double* __restrict__ out) {
-    double s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < nb; i += TPB) { s1 += part[2 * i]; s2 += part[2 * i + 1]; }
-    __shared__ double sh1[TPB], sh2[TPB];
-    sh1[threadIdx.x] = s1; sh2[threadIdx.x] = s2;
+// Krylov scalars in device memory (slots of c->scal)
+enum { S_RZ = 0, S_ZRO = 1, S_RR = 2, S_PQ = 4, S_X = 6, S_ALPHA = 10, S_BETA = 11, S_OMEGA = 12,
+       S_TT = 14, S_TS = 15, S_RHO = 16, S_AV = 20 };
+
+// what the last block does with the totals d[0], d[1]
+enum { W_STORE = 0,      // s[slot] = d0, s[slot+1] = d1
+       W_FCG_ALPHA,      // d0 = p.Ap:  alpha = rz / p.Ap
+       W_FCG_BETA,       // d0 = r.z, d1 = z.r_old: Polak-Ribiere beta, rz = d0
+       W_BICG_INIT,      // d0 = b.b: rho = rr = b.b, beta = 0, alpha = omega = 1
+       W_BICG_ALPHA,     // d0 = rh.v: alpha = rho / rh.v
+       W_BICG_OMEGA,     // d0 = t.t, d1 = t.s: omega = t.s / t.t
+       W_BICG_RHO };     // d0 = r.r, d1 = rh.r: beta = (rho'/rho)(alpha/omega), rho = rho'
+
+__device__ void finish_scalars(int what, int slot, const double (&d)[2], double* s) {
+    switch (what) {
+    case W_STORE: s[slot] = d[0]; s[slot + 1] = d[1]; break;
+    case W_FCG_ALPHA: s[S_PQ] = d[0]; s[S_ALPHA] = s[S_RZ] / d[0]; break;
+    case W_FCG_BETA: s[S_BETA] = (d[0] - d[1]) / s[S_RZ]; s[S_RZ] = d[0]; s[S_ZRO] = d[1]; break;
+    case W_BICG_INIT:
+        s[S_X] = d[0]; s[S_RR] = d[0]; s[S_RHO] = d[0];
+        s[S_BETA] = 0.0; s[S_ALPHA] = 1.0; s[S_OMEGA] = 1.0;
+        break;
+    case W_BICG_ALPHA: s[S_AV] = d[0]; s[S_ALPHA] = s[S_RHO] / d[0]; break;
+    case W_BICG_OMEGA: s[S_TT] = d[0]; s[S_TS] = d[1]; s[S_OMEGA] = d[0] != 0.0 ? d[1] / d[0] : 0.0; break;
+    case W_BICG_RHO:
+        s[S_RR] = d[0];
+        s[S_BETA] = (d[1] / s[S_RHO]) * (s[S_ALPHA] / s[S_OMEGA]);
+        s[S_RHO] = d[1];
+        break;
+    default: break;
+    }
+}
+
+struct Red {                 // where a launch leaves its block partials
+    double* part;
+};
+
+__device__ inline void reduce2(double (&v)[2], const Red& rd) { block_partials(v, rd.part); }
+
+// ordered sum of nb partial pairs, then the scalar update `what`
+__global__ void __launch_bounds__(TPB) k_finish(int nb, const double* __restrict__ part, double* __restrict__ s,
+                                                int what, int slot) {
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < nb; i += TPB) { a0 += part[2 * i]; a1 += part[2 * i + 1]; }
+    __shared__ double sh[2][TPB];
+    sh[0][threadIdx.x] = a0; sh[1][threadIdx.x] = a1;
     __syncthreads();
+    for (int w = TPB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        double t1 = 0, t2 = 0;
-        for (int w = 0; w < TPB; ++w) { t1 += sh1[w]; t2 += sh2[w]; }
-        out[0] = t1; out[1] = t2;
+        double d[2] = {sh[0][0], sh[1][0]};
+        finish_scalars(what, slot, d, s);
     }
 }
 
-template <class TA>
-void dot2(semo_ctx* c, int n, const TA* a, const TA* b, const TA* cc, const TA* d, double* out) {
-    int nb = rblk(n);
-    hipLaunchKernelGGL(k_dot2<TA>, dim3(nb), dim3(TPB), 0, c->stream, n, a, b, cc, d, c->red.p);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(TPB), 0, c->stream, nb, c->red.p, out);
+// a.b (+ c.d) over a grid-stride range
+__global__ void __launch_bounds__(TPB) k_dot(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                             const double* __restrict__ cc, const double* __restrict__ d, Red rd) {
+    double v[2] = {0.0, 0.0};
+    for (int i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[0] += a[i] * b[i];
+        if (cc) v[1] += cc[i] * d[i];
+    }
+    reduce2(v, rd);
 }
 
-// ------------------------------------------------------------ vector ops --
-// scalar slots in c->scal
-// (dot2 writes a pair: slot and slot+1)
-enum { S_RZ = 0, S_ZRO = 1, S_RR = 2, S_PQ = 4, S_X = 6, S_RZOLD = 8, S_ALPHA = 10, S_BETA = 11,
-       S_OMEGA = 12, S_TT = 14, S_TS = 15, S_RHO = 16, S_RHOOLD = 18, S_AV = 20 };
+// ------------------------------------------------------------ row kernels --
+enum { OP_Y = 0,       // y = A x
+       OP_RES = 1,     // y = b - A x
+       OP_ADD = 2,     // y += A x
+       OP_CHEB = 3,    // Chebyshev step: x = d_k, y = d_k+1, r -= D^-1 A d_k, xo += d_k+1
+       OP_CHEBX = 4 }; // Chebyshev start from x != 0: r = D^-1 (b - A x), y = r c1, xo = x + y
 
-__global__ void k_cg_scalars(double* s, int what) {
-    if (threadIdx.x != 0) return;
-    if (what == 0) s[S_ALPHA] = s[S_RZ] / s[S_PQ];                   // alpha = rz / pAp
-    else if (what == 1) {                                            // Polak-Ribiere beta
-        double rz_old = s[S_RZOLD];
-        s[S_BETA] = (s[S_RZ] - s[S_ZRO]) / rz_old;
+template <class T>
+struct RowArgs {
+    int n;
+    const int* __restrict__ ptr;    // slice offsets (sliced ELL) or row offsets (CSR)
+    const int* __restrict__ col;
+    const T* __restrict__ val;
+    const T* __restrict__ diag;     // null: no diagonal part
+    const T* __restrict__ x;        // gathered operand
+    const T* __restrict__ b;
+    T* __restrict__ y;
+    const T* __restrict__ dinv;
+    T* __restrict__ r;
+    T* __restrict__ xo;
+    double* __restrict__ zo;        // optional FP64 copy of the new iterate
+    T c1, c2;
+};
+
+// G == 0: sliced ELL, one lane per row.  G > 0: CSR, G consecutive lanes per
+// row, strided partial sums combined by a fixed shuffle tree (the result is
+// deterministic for a given G).
+template <class T, int G, int OP>
+__global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
+    int gid = xcd_block() * TPB + threadIdx.x;
+    int row;
+    T acc = 0;
+    if constexpr (G == 0) {
+        row = gid;
+        int s = row >> 6, lane = row & 63;
+        if (s >= ((a.n + 63) >> 6)) return;
+        for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64) acc += a.val[e] * a.x[a.col[e]];
+        if (row >= a.n) return;
+    } else {
+        row = gid / G;
+        int sub = gid & (G - 1);
+        if (row >= a.n) return;
+        for (int e = a.ptr[row] + sub; e < a.ptr[row + 1]; e += G) acc += a.val[e] * a.x[a.col[e]];
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, G);
+        if (sub) return;
+    }
+    if (a.diag) acc += a.diag[row] * a.x[row];
+    if constexpr (OP == OP_Y) {
+        a.y[row] = acc;
+    } else if constexpr (OP == OP_RES) {
+        a.y[row] = a.b[row] - acc;
+    } else if constexpr (OP == OP_ADD) {
+        a.y[row] += acc;
+    } else if constexpr (OP == OP_CHEB) {
+        T ri = a.r[row] - a.dinv[row] * acc;
+        T di = a.c1 * a.x[row] + a.c2 * ri;
+        a.r[row] = ri; a.y[row] = di;
+        T xn = a.xo[row] + di;
+        a.xo[row] = xn;
+        if (a.zo) a.zo[row] = (double)xn;
+    } else {
+        T ri = a.dinv[row] * (a.b[row] - acc);
+        T di = ri * a.c1;
+        a.r[row] = ri; a.y[row] = di;
+        T xn = a.x[row] + di;
+        a.xo[row] = xn;
+        if (a.zo) a.zo[row] = (double)xn;
     }
 }
 
-// x += alpha p ; r -= alpha q ; rold = r (before update)
-__global__ void __launch_bounds__(TPB) k_cg_update(int n, const double* __restrict__ s, double* __restrict__ x,
-                                                   double* __restrict__ r, double* __restrict__ rold,
-                                                   const double* __restrict__ p, const double* __restrict__ q) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double a = s[S_ALPHA];
-    x[i] += a * p[i];
-    double ri = r[i];
-    rold[i] = ri;
-    r[i] = ri - a * q[i];
+template <class T, int OP>
+void row_launch(semo_ctx* c, const DevSell<T>& A, RowArgs<T> a, const char* tag, double bytes) {
+    if (A.n == 0) return;
+    a.n = A.n; a.ptr = A.sl_ptr.p; a.col = A.col.p; a.val = A.val.p; a.diag = A.diag.p;
+    hipEvent_t ev{};
+    bool prof = c->prof.on && tag;
+    if (prof) c->prof.begin(c->stream, ev);
+    long rows = A.n;
+    switch (A.lanes) {
+    case 0: hipLaunchKernelGGL((k_row<T, 0, OP>), dim3(nblk((long)A.nslices * 64)), dim3(TPB), 0, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_row<T, 4, OP>), dim3(nblk(rows * 4)), dim3(TPB), 0, c->stream, a); break;
+    case 8: hipLaunchKernelGGL((k_row<T, 8, OP>), dim3(nblk(rows * 8)), dim3(TPB), 0, c->stream, a); break;
+    case 16: hipLaunchKernelGGL((k_row<T, 16, OP>), dim3(nblk(rows * 16)), dim3(TPB), 0, c->stream, a); break;
+    case 32: hipLaunchKernelGGL((k_row<T, 32, OP>), dim3(nblk(rows * 32)), dim3(TPB), 0, c->stream, a); break;
+    case 64: hipLaunchKernelGGL((k_row<T, 64, OP>), dim3(nblk(rows * 64)), dim3(TPB), 0, c->stream, a); break;
+    default: throw Error(SEMO_EINVAL, "operator with an unsupported lane count");
+    }
+    if (prof) c->prof.end(c->stream, tag, ev, bytes);
 }
 
+// algorithmic bytes of one pass over A: values + columns of the real
+// entries, row/slice offsets, plus `vec` vector accesses of n entries
+template <class T>
+double op_bytes(const DevSell<T>& A, int vec) {
+    double ptrs = A.lanes ? 4.0 * (A.n + 1) : 4.0 * (A.nslices + 1);
+    return (double)A.nnz * (sizeof(T) + 4) + ptrs + (double)A.n * sizeof(T) * vec;
+}
+
+template <class T, int OP>
+void spmv(semo_ctx* c, const DevSell<T>& A, const T* x, const T* b, T* y, const char* tag = nullptr) {
+    RowArgs<T> a{};
+    a.x = x; a.b = b; a.y = y;
+    row_launch<T, OP>(c, A, a, tag, tag ? op_bytes(A, OP == OP_Y ? 2 : 3) : 0.0);
+}
+
+// y = A x (sliced ELL, FP64) with d0 = (a0 ? a0 : y).y and d1 = a1.y
+__global__ void __launch_bounds__(TPB) k_spmv_dot(int n, const int* __restrict__ sl_ptr, const int* __restrict__ col,
+                                                  const double* __restrict__ val, const double* __restrict__ diag,
+                                                  const double* __restrict__ x, double* __restrict__ y,
+                                                  const double* __restrict__ a0, const double* __restrict__ a1, Red rd) {
+    int row = xcd_block() * blockDim.x + threadIdx.x;
+    int s = row >> 6, lane = row & 63;
+    double v[2] = {0.0, 0.0};
+    if (s < ((n + 63) >> 6)) {
+        double acc = 0.0;
+        for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) acc += val[e] * x[col[e]];
+        if (row < n) {
+            acc += diag[row] * x[row];
+            y[row] = acc;
+            v[0] = (a0 ? a0[row] : acc) * acc;
+            if (a1) v[1] = a1[row] * acc;
+        }
+    }
+    reduce2(v, rd);
+}
+
+// ------------------------------------------------------- Krylov vectors --
+// x = 0, r = b, d0 = b.b
+__global__ void __launch_bounds__(TPB) k_fcg_init(int n, const double* __restrict__ b, double* __restrict__ x,
+                                                  double* __restrict__ r, Red rd) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < n) { double bi = b[i]; x[i] = 0.0; r[i] = bi; v[0] = bi * bi; }
+    reduce2(v, rd);
+}
+
+// x += alpha p ; r_old = r ; r -= alpha q ; d0 = r.r
+__global__ void __launch_bounds__(TPB) k_fcg_update(int n, const double* __restrict__ s, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ rold,
+                                                    const double* __restrict__ p, const double* __restrict__ q,
+                                                    Red rd) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        double a = s[S_ALPHA];
+        x[i] += a * p[i];
+        double ri = r[i];
+        rold[i] = ri;
+        ri -= a * q[i];
+        r[i] = ri;
+        v[0] = ri * ri;
+    }
+    reduce2(v, rd);
+}
+
+// p = z + s[sidx] p
 __global__ void __launch_bounds__(TPB) k_xpby(int n, const double* __restrict__ s, int sidx,
                                               const double* __restrict__ z, double* __restrict__ p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     p[i] = z[i] + s[sidx] * p[i];
 }
 
 template <class TA, class TB>
 __global__ void __launch_bounds__(TPB) k_cast(int n, const TA* __restrict__ a, TB* __restrict__ b) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (TB)a[i];
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) k_scale(int n, const T* __restrict__ d, const T* __restrict__ a,
-                                               T* __restrict__ b) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = d[i] * a[i];
-}
-
-__global__ void __launch_bounds__(TPB) k_zero64(int n, double* __restrict__ a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = 0.0;
-}
-
-// BiCGStab scalar updates on the device
-__global__ void k_bicg_scalars(double* s, int what) {
-    if (threadIdx.x != 0) return;
-    if (what == 0) {            // beta = (rho/rho_old)*(alpha/omega); first iteration: beta = 0
-        double rho = s[S_RHO];
-        s[S_BETA] = s[S_RHOOLD] == 0.0 ? 0.0 : (rho / s[S_RHOOLD]) * (s[S_ALPHA] / s[S_OMEGA]);
-    } else if (what == 1) {     // alpha = rho / (rhat . v)
-        s[S_ALPHA] = s[S_RHO] / s[S_AV];
-    } else if (what == 2) {     // omega = (t.s)/(t.t)
-        s[S_OMEGA] = s[S_TT] != 0.0 ? s[S_TS] / s[S_TT] : 0.0;
-        s[S_RHOOLD] = s[S_RHO];
+// x = 0, r = rh = b, p = v = 0, d0 = b.b
+__global__ void __launch_bounds__(TPB) k_bicg_init(int n, const double* __restrict__ b, double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ rh,
+                                                   double* __restrict__ p, double* __restrict__ v, Red rd) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double w[2] = {0.0, 0.0};
+    if (i < n) {
+        double bi = b[i];
+        x[i] = 0.0; r[i] = bi; rh[i] = bi; p[i] = 0.0; v[i] = 0.0;
+        w[0] = bi * bi;
     }
+    reduce2(w, rd);
 }
-
-// p = r + beta (p - omega v)
+// p = r + beta (p - omega v) ; ph = D^-1 p
 __global__ void __launch_bounds__(TPB) k_bicg_p(int n, const double* __restrict__ s, const double* __restrict__ r,
-                                                const double* __restrict__ v, double* __restrict__ p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = r[i] + s[S_BETA] * (p[i] - s[S_OMEGA] * v[i]);
+                                                const double* __restrict__ v, const double* __restrict__ dinv,
+                                                double* __restrict__ p, double* __restrict__ ph) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double pi = r[i] + s[S_BETA] * (p[i] - s[S_OMEGA] * v[i]);
+    p[i] = pi; ph[i] = dinv[i] * pi;
 }
-// sv = r - alpha v
+// sv = r - alpha v ; sh = D^-1 sv
 __global__ void __launch_bounds__(TPB) k_bicg_s(int n, const double* __restrict__ s, const double* __restrict__ r,
-                                                const double* __restrict__ v, double* __restrict__ sv) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) sv[i] = r[i] - s[S_ALPHA] * v[i];
+                                                const double* __restrict__ v, const double* __restrict__ dinv,
+                                                double* __restrict__ sv, double* __restrict__ sh) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double si = r[i] - s[S_ALPHA] * v[i];
+    sv[i] = si; sh[i] = dinv[i] * si;
 }
-// x += alpha ph + omega sh ; r = sv - omega t
+// x += alpha ph + omega sh ; r = sv - omega t ; d0 = r.r, d1 = rh.r
 __global__ void __launch_bounds__(TPB) k_bicg_x(int n, const double* __restrict__ s, const double* __restrict__ ph,
                                                 const double* __restrict__ sh, const double* __restrict__ sv,
-                                                const double* __restrict__ t, double* __restrict__ x,
-                                                double* __restrict__ r) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    x[i] += s[S_ALPHA] * ph[i] + s[S_OMEGA] * sh[i];
-    r[i] = sv[i] - s[S_OMEGA] * t[i];
+                                                const double* __restrict__ t, const double* __restrict__ rh,
+                                                double* __restrict__ x, double* __restrict__ r, Red rd) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double w[2] = {0.0, 0.0};
+    if (i < n) {
+        x[i] += s[S_ALPHA] * ph[i] + s[S_OMEGA] * sh[i];
+        double ri = sv[i] - s[S_OMEGA] * t[i];
+        r[i] = ri;
+        w[0] = ri * ri; w[1] = rh[i] * ri;
+    }
+    reduce2(w, rd);
 }
 
 // ---------------------------------------------------------- Chebyshev ----
-// first step from x = 0: r = D^-1 b, d = r/theta, x = d
+// start from x = 0: r = D^-1 b, d = r/theta, x = d.  bsrc (optional): FP64
+// right-hand side, converted into b on the way.  zo (optional): FP64 copy of x.
 template <class T>
-__global__ void __launch_bounds__(TPB) k_cheb_init(int n, const T* __restrict__ dinv, const T* __restrict__ b,
-                                                   T theta_inv, T* __restrict__ r, T* __restrict__ d,
-                                                   T* __restrict__ x) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(TPB) k_cheb_init(int n, const T* __restrict__ dinv, T* __restrict__ b,
+                                                   const double* __restrict__ bsrc, T theta_inv, T* __restrict__ r,
+                                                   T* __restrict__ d, T* __restrict__ x, double* __restrict__ zo) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    T ri = dinv[i] * b[i];
+    T bi;
+    if (bsrc) { bi = (T)bsrc[i]; b[i] = bi; } else bi = b[i];
+    T ri = dinv[i] * bi;
     T di = ri * theta_inv;
     r[i] = ri; d[i] = di; x[i] = di;
-}
-// first step with a nonzero x: r = D^-1 (b - A x) (t holds A x), d = r/theta, x += d
-template <class T>
-__global__ void __launch_bounds__(TPB) k_cheb_init_x(int n, const T* __restrict__ dinv, const T* __restrict__ b,
-                                                     const T* __restrict__ t, T theta_inv, T* __restrict__ r,
-                                                     T* __restrict__ d, T* __restrict__ x) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    T ri = dinv[i] * (b[i] - t[i]);
-    T di = ri * theta_inv;
-    r[i] = ri; d[i] = di; x[i] += di;
-}
-// fused: t = A d (sliced ELL) ; r -= D^-1 t ; d' = c1 d + c2 r ; x += d'
-template <class T>
-__global__ void __launch_bounds__(TPB) k_cheb_step(int n, const int* __restrict__ sl_ptr, const int* __restrict__ col,
-                                                   const T* __restrict__ val, const T* __restrict__ diag,
-                                                   const T* __restrict__ dinv, const T* __restrict__ din,
-                                                   T* __restrict__ dout, T* __restrict__ r, T* __restrict__ x,
-                                                   T c1, T c2) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    int s = row >> 6, lane = row & 63;
-    if (s >= ((n + 63) >> 6)) return;
-    T acc = 0;
-    for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) acc += val[e] * din[col[e]];
-    if (row >= n) return;
-    acc += diag[row] * din[row];
-    T ri = r[row] - dinv[row] * acc;
-    T di = c1 * din[row] + c2 * ri;
-    r[row] = ri; dout[row] = di; x[row] += di;
+    if (zo) zo[i] = (double)di;
 }
 
 // coarse dense solve: y = Ainv x (FP64), x/y of type T
@@ -282,9 +395,12 @@ template <> const DevSell<float>& opP<float>(const AmgLevel& L) { return L.P32; 
 template <class T> const DevSell<T>& opR(const AmgLevel& L);
 template <> const DevSell<double>& opR<double>(const AmgLevel& L) { return L.R64; }
 template <> const DevSell<float>& opR<float>(const AmgLevel& L) { return L.R32; }
-template <class T> T* vx(AmgLevel& L);
-template <> double* vx<double>(AmgLevel& L) { return L.x64.p; }
-template <> float* vx<float>(AmgLevel& L) { return L.x32.p; }
+template <class T> DBuf<T>& bx(AmgLevel& L);
+template <> DBuf<double>& bx<double>(AmgLevel& L) { return L.x64; }
+template <> DBuf<float>& bx<float>(AmgLevel& L) { return L.x32; }
+template <class T> DBuf<T>& bw(AmgLevel& L);
+template <> DBuf<double>& bw<double>(AmgLevel& L) { return L.w64; }
+template <> DBuf<float>& bw<float>(AmgLevel& L) { return L.w32; }
 template <class T> T* vb(AmgLevel& L);
 template <> double* vb<double>(AmgLevel& L) { return L.b64.p; }
 template <> float* vb<float>(AmgLevel& L) { return L.b32.p; }
@@ -301,73 +417,84 @@ template <class T> const T* vdinv(const AmgLevel& L);
 template <> const double* vdinv<double>(const AmgLevel& L) { return L.dinv64.p; }
 template <> const float* vdinv<float>(const AmgLevel& L) { return L.dinv32.p; }
 
-// Chebyshev polynomial smoother in D^-1 A on [lmax/30, 1.1 lmax].
-// x_zero: start from x = 0 (saves one SpMV).  dA/dB: double-buffered search
-// direction, r: preconditioned residual.
+struct Tags { const char *cheb, *chebx, *resid, *restrict_, *prolong; };
+Tags level_tags(int l, bool fp64) {
+    if (l == 0)
+        return fp64 ? Tags{"amg_cheb_fp64_L0", "amg_chebx_fp64_L0", "amg_resid_fp64_L0", "amg_restrict_fp64_L0",
+                           "amg_prolong_fp64_L0"}
+                    : Tags{"amg_cheb_fp32_L0", "amg_chebx_fp32_L0", "amg_resid_fp32_L0", "amg_restrict_fp32_L0",
+                           "amg_prolong_fp32_L0"};
+    const char* t = fp64 ? "amg_coarse_fp64" : "amg_coarse_fp32";
+    return Tags{t, t, t, t, t};
+}
+
+// Chebyshev polynomial smoother in D^-1 A on [lmax/30, 1.1 lmax] for the
+// iterate held in the level's x buffer.  x_zero: start from x = 0 (no SpMV
+// for the start).  bsrc: FP64 right-hand side to convert into b first.  zo:
+// FP64 copy of the final iterate.
 template <class T>
-void cheb_smooth(semo_ctx* c, AmgLevel& L, T* x, const T* b, T* dA, T* dB, T* r, int degree, bool x_zero,
-                 const char* tag) {
+void cheb_smooth(semo_ctx* c, AmgLevel& L, T* b, int degree, bool x_zero, const double* bsrc, double* zo,
+                 const Tags& tg) {
     const DevSell<T>& A = opA<T>(L);
     int n = L.n;
+    T* dA = vt<T>(L); T* dB = vu<T>(L); T* r = vr<T>(L);
     double hi = 1.1 * L.lmax, lo = hi / 30.0;
     double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
     double sigma = theta / delta, rho = 1.0 / sigma;
     if (x_zero) {
-        hipLaunchKernelGGL(k_cheb_init<T>, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, vdinv<T>(L), b,
-                           (T)(1.0 / theta), r, dA, x);
+        hipLaunchKernelGGL(k_cheb_init<T>, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, vdinv<T>(L), b, bsrc,
+                           (T)(1.0 / theta), r, dA, bx<T>(L).p, degree == 1 ? zo : nullptr);
     } else {
-        spmv<T, T, T, 0>(c, A, x, (const T*)nullptr, dB);
-        hipLaunchKernelGGL(k_cheb_init_x<T>, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, vdinv<T>(L), b, dB,
-                           (T)(1.0 / theta), r, dA, x);
+        RowArgs<T> a{};
+        a.x = bx<T>(L).p; a.b = b; a.y = dA; a.dinv = vdinv<T>(L); a.r = r; a.xo = bw<T>(L).p;
+        a.zo = degree == 1 ? zo : nullptr; a.c1 = (T)(1.0 / theta);
+        row_launch<T, OP_CHEBX>(c, A, a, tg.chebx, op_bytes(A, 7));
+        std::swap(bx<T>(L), bw<T>(L));       // the new iterate is the level's x from here on
     }
+    T* x = bx<T>(L).p;
     for (int k = 1; k < degree; ++k) {
         double rho_new = 1.0 / (2.0 * sigma - rho);
         double c1 = rho_new * rho, c2 = 2.0 * rho_new / delta;
         rho = rho_new;
-        hipEvent_t ev{};
-        if (c->prof.on && tag) c->prof.begin(c->stream, ev);
-        hipLaunchKernelGGL(k_cheb_step<T>, dim3(nblk(A.nslices * 64)), dim3(TPB), 0, c->stream, n, A.sl_ptr.p,
-                           A.col.p, A.val.p, A.diag.p, vdinv<T>(L), dA, dB, r, x, (T)c1, (T)c2);
-        if (c->prof.on && tag)
-            c->prof.end(c->stream, tag, ev,
-                        (double)A.nnz * (sizeof(T) + 4) + (double)n * sizeof(T) * 8);
+        RowArgs<T> a{};
+        a.x = dA; a.y = dB; a.dinv = vdinv<T>(L); a.r = r; a.xo = x; a.c1 = (T)c1; a.c2 = (T)c2;
+        a.zo = k == degree - 1 ? zo : nullptr;
+        row_launch<T, OP_CHEB>(c, A, a, tg.cheb, op_bytes(A, 8));
         std::swap(dA, dB);
     }
 }
 
+// one V-cycle for level l on b; the result is left in the level's x buffer
+// (and, at the finest level, converted into zo)
 template <class T>
-void vcycle(semo_ctx* c, Amg& amg, int l, int degree) {
+void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double* zo) {
     AmgLevel& L = amg.lv[l];
     int n = L.n;
-    T* x = vx<T>(L); T* b = vb<T>(L); T* r = vr<T>(L); T* t = vt<T>(L); T* u = vu<T>(L);
+    T* b = vb<T>(L);
     if (l + 1 == (int)amg.lv.size()) {     // coarsest: dense direct solve in FP64
-        hipLaunchKernelGGL(k_dense_gemv<T>, dim3((n + 3) / 4), dim3(TPB), 0, c->stream, n, amg.coarse_inv.p, b, x);
+        if (bsrc) hipLaunchKernelGGL((k_cast<double, T>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, bsrc, b);
+        hipLaunchKernelGGL(k_dense_gemv<T>, dim3((n + 3) / 4), dim3(TPB), 0, c->stream, n, amg.coarse_inv.p, b,
+                           bx<T>(L).p);
+        if (zo) hipLaunchKernelGGL((k_cast<T, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, bx<T>(L).p, zo);
         return;
     }
     AmgLevel& C = amg.lv[l + 1];
-    const char* tag = l == 0 ? (sizeof(T) == 8 ? "amg_cheb_fp64_L0" : "amg_cheb_fp32_L0") : nullptr;
-    cheb_smooth<T>(c, L, x, b, t, u, r, degree, true, tag);
-    spmv<T, T, T, 1>(c, opA<T>(L), x, b, t, true, l == 0 ? (sizeof(T) == 8 ? "amg_resid_fp64_L0" : "amg_resid_fp32_L0") : nullptr);
-    spmv<T, T, T, 0>(c, opR<T>(L), t, (const T*)nullptr, vb<T>(C), false);
-    vcycle<T>(c, amg, l + 1, degree);
-    spmv<T, T, T, 2>(c, opP<T>(L), vx<T>(C), (const T*)nullptr, x, false);
-    cheb_smooth<T>(c, L, x, b, t, u, r, degree, false, tag);
+    Tags tg = level_tags(l, sizeof(T) == 8);
+    T* t = vt<T>(L);
+    cheb_smooth<T>(c, L, b, degree, true, bsrc, nullptr, tg);
+    const DevSell<T>& A = opA<T>(L);
+    spmv<T, OP_RES>(c, A, bx<T>(L).p, b, t, tg.resid);
+    spmv<T, OP_Y>(c, opR<T>(L), t, (const T*)nullptr, vb<T>(C), tg.restrict_);
+    vcycle<T>(c, amg, l + 1, degree, nullptr, nullptr);
+    spmv<T, OP_ADD>(c, opP<T>(L), bx<T>(C).p, (const T*)nullptr, bx<T>(L).p, tg.prolong);
+    cheb_smooth<T>(c, L, b, degree, false, nullptr, zo, tg);
 }
 
+// z = M^-1 r with one V-cycle in precision T
 template <class T>
 void precondition(semo_ctx* c, Amg& amg, const double* r64, double* z64) {
-    AmgLevel& L0 = amg.lv[0];
-    int n = L0.n;
     int deg = c->sp.cheb_degree > 0 ? c->sp.cheb_degree : 2;
-    if (sizeof(T) == 8) {
-        hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, r64, L0.b64.p);
-        vcycle<double>(c, amg, 0, deg);
-        hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, L0.x64.p, z64);
-    } else {
-        hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, r64, L0.b32.p);
-        vcycle<float>(c, amg, 0, deg);
-        hipLaunchKernelGGL((k_cast<float, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, L0.x32.p, z64);
-    }
+    vcycle<T>(c, amg, 0, deg, r64, z64);
 }
 
 double host_scalar(semo_ctx* c, int idx) {
@@ -377,44 +504,70 @@ double host_scalar(semo_ctx* c, int idx) {
     return v;
 }
 
+Red red_of(semo_ctx* c) { return Red{c->red.p}; }
+
+// sums the partials of the previous launch (nb blocks) and applies `what`
+void finish(semo_ctx* c, int nb, int what, int slot = 0) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, c->stream, nb, c->red.p, c->scal.p, what, slot);
+}
+
+// grid-stride dot products a.b (+ c.d), finished as `what`
+void dot_what(semo_ctx* c, int n, const double* a, const double* b, const double* cc, const double* d, int what,
+              int slot = 0) {
+    int nb = rblk(n);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(TPB), 0, c->stream, n, a, b, cc, d, red_of(c));
+    finish(c, nb, what, slot);
+}
+// ... stored into s[slot], s[slot+1]
+void dot_store(semo_ctx* c, int n, const double* a, const double* b, const double* cc, const double* d, int slot) {
+    dot_what(c, n, a, b, cc, d, W_STORE, slot);
+}
+
+// y = A x with dots of y (see k_spmv_dot), for the fine-level sliced-ELL operator
+void spmv_dot(semo_ctx* c, const DevSell<double>& A, const double* x, double* y, const double* a0,
+              const double* a1, int what) {
+    hipEvent_t ev{};
+    bool prof = c->prof.on;
+    if (prof) c->prof.begin(c->stream, ev);
+    int nb = nblk((long)A.nslices * 64);
+    hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(TPB), 0, c->stream, A.n, A.sl_ptr.p, A.col.p, A.val.p,
+                       A.diag.p, x, y, a0, a1, red_of(c));
+    if (prof) c->prof.end(c->stream, "spmv_fp64", ev, op_bytes(A, 3 + (a0 ? 1 : 0) + (a1 ? 1 : 0)));
+    finish(c, nb, what);
+}
+
 // flexible (Polak-Ribiere) PCG with the AMG preconditioner.  x0 = 0,
 // stop at ||r|| <= tol ||b||.
 int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, double* x, double tol, int maxit,
          bool mixed, double* relres) {
+    SEMO_CHECK(A.lanes == 0, SEMO_EINVAL, "Krylov operator must be sliced ELL");
     int n = A.n;
     size_t ln = (size_t)n + 64;
     c->kv64.ensure(8 * ln);
     double* r = c->kv64.p; double* z = r + ln; double* p = z + ln; double* q = p + ln; double* rold = q + ln;
-    double* s = c->scal.p;
-    hipLaunchKernelGGL(k_zero64, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, x);
-    hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, r);
-    dot2<double>(c, n, b, b, (const double*)nullptr, (const double*)nullptr, s + S_X);
+    hipLaunchKernelGGL(k_fcg_init, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, x, r, red_of(c));
+    finish(c, nblk(n), W_STORE, S_X);
     double bb = host_scalar(c, S_X);
     if (bb == 0.0) { *relres = 0.0; return 0; }
     double nb = std::sqrt(bb);
     if (mixed) precondition<float>(c, amg, r, z); else precondition<double>(c, amg, r, z);
     hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, z, p);
-    dot2<double>(c, n, r, z, (const double*)nullptr, (const double*)nullptr, s + S_RZ);
+    dot_store(c, n, r, z, nullptr, nullptr, S_RZ);
     int it = 0;
-    double rr = bb;
     for (it = 1; it <= maxit; ++it) {
-        spmv<double, double, double, 0>(c, A, p, (const double*)nullptr, q, true, "spmv_fp64");
-        dot2<double>(c, n, p, q, (const double*)nullptr, (const double*)nullptr, s + S_PQ);
-        hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(64), 0, c->stream, s, 0);
-        hipLaunchKernelGGL(k_cg_update, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, x, r, rold, p, q);
-        dot2<double>(c, n, r, r, (const double*)nullptr, (const double*)nullptr, s + S_RR);
-        rr = host_scalar(c, S_RR);
+        spmv_dot(c, A, p, q, p, nullptr, W_FCG_ALPHA);                    // q = A p, alpha
+        hipLaunchKernelGGL(k_fcg_update, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->scal.p, x, r, rold, p, q,
+                           red_of(c));
+        finish(c, nblk(n), W_STORE, S_RR);
+        double rr = host_scalar(c, S_RR);
         if (std::sqrt(rr) <= tol * nb) break;
         if (mixed) precondition<float>(c, amg, r, z); else precondition<double>(c, amg, r, z);
-        // rz_old <- rz ; rz = r.z ; zro = z.rold
-        SEMO_HIP(hipMemcpyAsync(s + S_RZOLD, s + S_RZ, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        dot2<double>(c, n, r, z, z, rold, s + S_RZ);
-        hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(64), 0, c->stream, s, 1);
-        hipLaunchKernelGGL(k_xpby, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, (int)S_BETA, z, p);
+        dot_what(c, n, r, z, z, rold, W_FCG_BETA);                        // r.z, z.r_old -> beta
+        hipLaunchKernelGGL(k_xpby, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->scal.p, (int)S_BETA, z, p);
     }
     // true residual in FP64
-    spmv<double, double, double, 1>(c, A, x, b, r);
-    dot2<double>(c, n, r, r, (const double*)nullptr, (const double*)nullptr, s + S_X);
+    spmv<double, OP_RES>(c, A, x, b, r);
+    dot_store(c, n, r, r, nullptr, nullptr, S_X);
     *relres = std::sqrt(host_scalar(c, S_X)) / nb;
     return std::min(it, maxit);
 }
@@ -422,53 +575,38 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
 // right-preconditioned (Jacobi) BiCGStab, x0 = 0
 int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const double* b, double* x, double tol,
              int maxit, double* relres) {
+    SEMO_CHECK(A.lanes == 0, SEMO_EINVAL, "Krylov operator must be sliced ELL");
     int n = A.n;
     size_t ln = (size_t)n + 64;
     c->kv64.ensure(8 * ln);
     double* r = c->kv64.p; double* rh = r + ln; double* p = rh + ln; double* v = p + ln; double* sv = v + ln;
     double* t = sv + ln; double* ph = t + ln; double* sh = ph + ln;
     double* s = c->scal.p;
-    hipLaunchKernelGGL(k_zero64, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, x);
-    hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, r);
-    hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, rh);
-    hipLaunchKernelGGL(k_zero64, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, p);
-    hipLaunchKernelGGL(k_zero64, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, v);
-    dot2<double>(c, n, b, b, (const double*)nullptr, (const double*)nullptr, s + S_X);
+    hipLaunchKernelGGL(k_bicg_init, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, x, r, rh, p, v, red_of(c));
+    finish(c, nblk(n), W_BICG_INIT);
     double bb = host_scalar(c, S_X);
     if (bb == 0.0) { *relres = 0.0; return 0; }
     double nb = std::sqrt(bb);
-    double zero4[4] = {0, 0, 1, 1};
-    // S_RHOOLD = 0 marks the first iteration; alpha = omega = 1
-    SEMO_HIP(hipMemcpyAsync(s + S_RHOOLD, &zero4[0], sizeof(double), hipMemcpyHostToDevice, c->stream));
-    SEMO_HIP(hipMemcpyAsync(s + S_ALPHA, &zero4[2], sizeof(double), hipMemcpyHostToDevice, c->stream));
-    SEMO_HIP(hipMemcpyAsync(s + S_OMEGA, &zero4[3], sizeof(double), hipMemcpyHostToDevice, c->stream));
     int it;
     for (it = 1; it <= maxit; ++it) {
-        dot2<double>(c, n, rh, r, (const double*)nullptr, (const double*)nullptr, s + S_RHO);
-        hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(64), 0, c->stream, s, 0);
-        hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, r, v, p);
-        hipLaunchKernelGGL(k_scale<double>, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, dinv, p, ph);
-        spmv<double, double, double, 0>(c, A, ph, (const double*)nullptr, v, true, "spmv_fp64");
-        dot2<double>(c, n, rh, v, (const double*)nullptr, (const double*)nullptr, s + S_AV);
-        hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(64), 0, c->stream, s, 1);
-        hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, r, v, sv);
-        hipLaunchKernelGGL(k_scale<double>, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, dinv, sv, sh);
-        spmv<double, double, double, 0>(c, A, sh, (const double*)nullptr, t, true, "spmv_fp64");
-        dot2<double>(c, n, t, t, t, sv, s + S_TT);      // S_TT = t.t, S_TS = t.s
-        hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(64), 0, c->stream, s, 2);
-        hipLaunchKernelGGL(k_bicg_x, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, ph, sh, sv, t, x, r);
-        dot2<double>(c, n, r, r, (const double*)nullptr, (const double*)nullptr, s + S_RR);
+        hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, r, v, dinv, p, ph);
+        spmv_dot(c, A, ph, v, rh, nullptr, W_BICG_ALPHA);                 // v = A ph, alpha
+        hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, r, v, dinv, sv, sh);
+        spmv_dot(c, A, sh, t, nullptr, sv, W_BICG_OMEGA);                 // t = A sh, omega
+        hipLaunchKernelGGL(k_bicg_x, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, ph, sh, sv, t, rh, x, r,
+                           red_of(c));
+        finish(c, nblk(n), W_BICG_RHO);
         double rr = host_scalar(c, S_RR);
         if (!(std::sqrt(rr) > tol * nb)) break;
     }
-    spmv<double, double, double, 1>(c, A, x, b, r);
-    dot2<double>(c, n, r, r, (const double*)nullptr, (const double*)nullptr, s + S_X);
+    spmv<double, OP_RES>(c, A, x, b, r);
+    dot_store(c, n, r, r, nullptr, nullptr, S_X);
     *relres = std::sqrt(host_scalar(c, S_X)) / nb;
     return std::min(it, maxit);
 }
 
 __global__ void __launch_bounds__(TPB) k_dinv(int n, const double* __restrict__ d, double* __restrict__ o) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i < n) o[i] = 1.0 / d[i];
 }
 
@@ -478,7 +616,7 @@ __global__ void __launch_bounds__(TPB) k_dinv(int n, const double* __restrict__ 
 template <class T>
 void sell_upload(DevSell<T>& M, const SellPattern& p, const std::vector<double>& vals,
                  const std::vector<double>& diag, hipStream_t s) {
-    M.n = p.n; M.ncols = p.ncols; M.nslices = p.nslices;
+    M.n = p.n; M.ncols = p.ncols; M.nslices = p.nslices; M.lanes = 0;
     M.nnz = p.nnz_offdiag + (int64_t)diag.size();
     M.sl_ptr.upload(p.sl_ptr, s);
     M.col.upload(p.col, s);
@@ -495,8 +633,38 @@ template void sell_upload<double>(DevSell<double>&, const SellPattern&, const st
 template void sell_upload<float>(DevSell<float>&, const SellPattern&, const std::vector<double>&,
                                  const std::vector<double>&, hipStream_t);
 
-void spmv64(semo_ctx* c, const DevSell<double>& A, const double* x, double* y) {
-    spmv<double, double, double, 0>(c, A, x, (const double*)nullptr, y);
+template <class T>
+void csr_upload(DevSell<T>& M, int n, int ncols, const std::vector<int>& ptr, const std::vector<int>& col,
+                const std::vector<double>& vals, const std::vector<double>& diag, int lanes, hipStream_t s) {
+    SEMO_CHECK(n >= 0 && ncols >= 1 && (int)ptr.size() == n + 1 && ptr[0] == 0 && ptr[n] == (int)col.size() &&
+                   vals.size() == col.size() && (diag.empty() || (int)diag.size() == n),
+               SEMO_EINVAL, "CSR operator: inconsistent arrays");
+    SEMO_CHECK(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, SEMO_EINVAL,
+               "CSR operator: lanes per row");
+    for (int r = 0; r < n; ++r) SEMO_CHECK(ptr[r + 1] >= ptr[r], SEMO_EINVAL, "CSR operator: row pointers");
+    for (int cidx : col) SEMO_CHECK(cidx >= 0 && cidx < ncols, SEMO_EINVAL, "CSR operator: column out of range");
+    M.n = n; M.ncols = ncols; M.nslices = (n + SL - 1) / SL; M.lanes = lanes;
+    M.nnz = (int64_t)col.size() + (int64_t)diag.size();
+    M.sl_ptr.upload(ptr, s);
+    M.col.upload(col, s);
+    std::vector<T> v(vals.begin(), vals.end());
+    M.val.upload(v, s);
+    if (!diag.empty()) {
+        std::vector<T> d(diag.begin(), diag.end());
+        M.diag.upload(d, s);
+    } else M.diag.release();
+    SEMO_HIP(hipStreamSynchronize(s));
+}
+template void csr_upload<double>(DevSell<double>&, int, int, const std::vector<int>&, const std::vector<int>&,
+                                 const std::vector<double>&, const std::vector<double>&, int, hipStream_t);
+template void csr_upload<float>(DevSell<float>&, int, int, const std::vector<int>&, const std::vector<int>&,
+                                const std::vector<double>&, const std::vector<double>&, int, hipStream_t);
+
+// partials for reductions over up to n rows (two values per block)
+void reduction_alloc(semo_ctx* c, long n) {
+    size_t need = 2 * (size_t)std::max(nblk(std::max<long>(n, 1) + 64), RB) + 64;
+    if (c->red.n < need) c->red.alloc(need);
+    if (c->scal.n < 64) c->scal.alloc(64);
 }
 
 // ----------------------------------------------------- equation solves ----
@@ -526,6 +694,7 @@ void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* s
         SEMO_HIP(hipMemcpyAsync(c->amg_key_val.p, c->Amat.val.p, c->Amat.val.bytes(), hipMemcpyDeviceToDevice, c->stream));
         SEMO_HIP(hipMemcpyAsync(c->amg_key_diag.p, c->Amat.diag.p, c->Amat.diag.bytes(), hipMemcpyDeviceToDevice, c->stream));
     }
+    reduction_alloc(c, c->nc);
     double rr = 0;
     int it = fpcg(c, amg, c->Amat, b, x, c->sp.tol_p, c->sp.maxit_p, c->sp.precision == SEMO_PREC_MIXED, &rr);
     if (st) {
@@ -537,6 +706,7 @@ void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* s
 
 void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st) {
     int n = c->nc;
+    reduction_alloc(c, n);
     DBuf<double>& dinv = c->dinv_mom;
     dinv.alloc(n + 64);
     hipLaunchKernelGGL(k_dinv, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->Amat.diag.p, dinv.p);
@@ -592,8 +762,7 @@ void solve_coo(semo_ctx* c, const char* eq, int n, int64_t nnz, const int* rows,
     DBuf<double> db, dx;
     db.upload(b, (size_t)n * nrhs, c->stream);
     dx.alloc((size_t)n * nrhs);
-    if (c->red.n < 2 * RB + 64) c->red.alloc(2 * RB + 64);
-    if (c->scal.n < 64) c->scal.alloc(64);
+    reduction_alloc(c, n);
     std::string e(eq ? eq : "");
     if (e == "pressure") {
         Amg amg;

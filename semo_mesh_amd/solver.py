@@ -65,7 +65,8 @@ class GpuSolver:
         m = self.mesh
         cs = _lib.CaseDesc((C.c_double * 2)(*case.rho), (C.c_double * 2)(*case.mu), case.sigma,
                            (C.c_double * 3)(*case.g), case.dt, case.urf_mom, case.urf_p,
-                           case.urf_pu, case.limit)
+                           case.urf_pu, case.limit, getattr(case, "urf_vf", 1.0),
+                           getattr(case, "n_vof", 0))
         bcs = (_lib.PatchBC * len(m.patch_names))()
         for i, nm in enumerate(m.patch_names):
             b = case.bcs[nm]
@@ -125,6 +126,14 @@ class GpuSolver:
         self.last_stats = st
         return n.value
 
+    def calcVolfracEq(self):
+        """One VOF iteration: MSTACS face values, calcVolfracEq, EOS and
+        transport; returns sqrt(sum dalpha^2)."""
+        n = C.c_double(); st = _lib.SolveStats()
+        self._ok(self._L.semo_calc_volfrac_eq(self._h, C.byref(n), C.byref(st)))
+        self.last_stats = st
+        return n.value
+
     def incompressiblePressureBased(self, n_correctors: int = 1):
         norms = np.zeros(2 * n_correctors)
         self._ok(self._L.semo_incompressible_pressure_based(self._h, n_correctors, _ptr(norms)))
@@ -132,14 +141,14 @@ class GpuSolver:
 
     def solve_stats(self, eq: str):
         """Iterations / final relative residuals / AMG levels of the last
-        momentum or pressure solve."""
+        momentum, pressure or volume-fraction solve."""
         st = _lib.SolveStats()
-        self._ok(self._L.semo_get_solve_stats(self._h, {"momentum": 0, "pressure": 1}[eq], C.byref(st)))
+        self._ok(self._L.semo_get_solve_stats(self._h, {"momentum": 0, "pressure": 1, "volume_fraction": 2}[eq], C.byref(st)))
         return dict(iters=list(st.iters), rel_res=list(st.rel_res), n_levels=st.n_levels,
                     setup_ms=st.setup_ms)
 
     def assemble(self, eq: str):
-        self._ok(self._L.semo_assemble(self._h, {"momentum": 0, "pressure": 1}[eq]))
+        self._ok(self._L.semo_assemble(self._h, {"momentum": 0, "pressure": 1, "volume_fraction": 2}[eq]))
 
     def matrix(self, eq: str):
         """COO (rows, cols, vals) and right-hand sides of the last assembly."""

@@ -3,7 +3,7 @@ reference (oracle/_ref/semo_ref, built from the reference checkout by
 oracle/ref.mk) on small seeded meshes.
 
     make -f oracle/ref.mk REF=/root/reference -j8
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]     (default: every case)
 
 Each case stores the mesh arrays it ran on, the reference's stage outputs and
 the fields after the run.  The reference itself never travels: only these
@@ -22,10 +22,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from semo_mesh_amd.cases import cavity_case  # noqa: E402
+from semo_mesh_amd.cases import cavity_case, dam_break_case  # noqa: E402
 from semo_mesh_amd.mesh import cavity_mesh, hex_mesh  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
+
+
+VOF_ONLY = ("_faceLVF", "_faceRVF", "_Qm3", "_oldVF")
 
 
 def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
@@ -51,6 +54,8 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
                 continue
             if base.endswith(("_gradVF", "_kappa")) or "_srcS" in base:
                 continue
+            if base.endswith(VOF_ONLY) and not case.n_vof:
+                continue
             arrs[base] = np.fromfile(os.path.join(out, fn),
                                      dtype=np.float64 if ext == ".f64" else np.int32)
     arrs.update(dict(
@@ -60,7 +65,8 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
         mesh_patch_nfaces=mesh.patch_nfaces, mesh_patch_names=np.array(mesh.patch_names),
         case_mu=np.array(case.mu), case_rho=np.array(case.rho), case_dt=np.array(case.dt),
         case_lid=np.array(case.bcs["zmax"].u_value), steps=np.array(steps),
-        correctors=np.array(correctors)))
+        correctors=np.array(correctors), case_g=np.array(case.g),
+        case_n_vof=np.array(case.n_vof), case_urf_vf=np.array(case.urf_vf)))
     if init is not None:
         arrs["init"] = np.asarray(init)
     path = os.path.join(HERE, name + ".npz")
@@ -74,26 +80,65 @@ def random_init(nc, seed):
                            rng.normal(size=nc), rng.normal(size=nc), np.ones(nc)])
 
 
-def main():
-    if not os.path.exists(REF):
-        raise SystemExit("build the reference first: make -f oracle/ref.mk REF=/root/reference")
-    # 1. jittered 4^3 cavity, random fields, one SIMPLE iteration: every stage
+def vof_init(mesh, seed):
+    """Small random velocity and pressure, a sharp water block x < 0.5, z < 0.5."""
+    nc = mesh.n_cells
+    rng = np.random.default_rng(seed)
+    cen = np.zeros((nc, 3))
+    for i in range(nc):
+        pts = mesh.points[mesh.cell_pts[mesh.cell_ptr[i]:mesh.cell_ptr[i + 1]]]
+        cen[i] = pts.mean(axis=0)
+    vf = ((cen[:, 0] < 0.5) & (cen[:, 2] < 0.5)).astype(np.float64)
+    return np.concatenate([rng.normal(size=nc) * 10.0, rng.normal(size=nc) * 0.1,
+                           rng.normal(size=nc) * 0.1, rng.normal(size=nc) * 0.1, vf])
+
+
+def _cavity4():
+    # jittered 4^3 cavity, random fields, one SIMPLE iteration: every stage
     m = cavity_mesh(4, jitter=0.1)
     c = cavity_case(); c.mu = (0.5, 0.5)
     run_case("cavity4_jitter_random", m, c, 1, 1, random_init(m.n_cells, 1))
-    # 2. non-cubic jittered box, 2 time steps x 2 correctors
+
+
+def _box654():
+    # non-cubic jittered box, 2 time steps x 2 correctors
     m = hex_mesh(6, 5, 4, jitter=0.1, seed=7)
     c = cavity_case(); c.mu = (0.05, 0.05); c.dt = 0.02
     run_case("box654_jitter_2x2", m, c, 2, 2, random_init(m.n_cells, 2))
-    # 3. plain cavity physics (rho 1000, mu 1e-3, dt 0.01) on 8^3, 1 iteration
+
+
+def _cavity8():
+    # plain cavity physics (rho 1000, mu 1e-3, dt 0.01) on 8^3, 1 iteration
     m = cavity_mesh(8)
     run_case("cavity8_plain", m, cavity_case(), 1, 1, None)
-    # 4. lid columns inside the live orifice profile window (reconIncom.cpp:707-727):
-    #    x-face centres (i+0.5)/64 hit 0.0006*13 +- 1.6e-4 at i = 0
+
+
+def _orifice():
+    # lid columns inside the live orifice profile window (reconIncom.cpp:707-727):
+    # x-face centres (i+0.5)/64 hit 0.0006*13 +- 1.6e-4 at i = 0
     m = hex_mesh(8, 4, 4, hi=(0.125, 0.0625, 0.0625), jitter=0.1, seed=11)
     c = cavity_case(); c.mu = (0.01, 0.01); c.dt = 0.001
     run_case("orifice_lid", m, c, 1, 1, random_init(m.n_cells, 3))
 
 
+def _vof_box654():
+    # two-phase VOF (MSTACS face values + volume-fraction equation): water
+    # block in a jittered box, gravity, open top; 2 time steps x 2 correctors
+    m = hex_mesh(6, 5, 4, jitter=0.1, seed=21, ref_face=False)
+    c = dam_break_case(dt=2e-3, n_vof=1, urf_vf=0.8)
+    run_case("vof_box654_2x2", m, c, 2, 2, vof_init(m, 4))
+
+
+CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
+         "cavity8_plain": _cavity8, "orifice_lid": _orifice, "vof_box654_2x2": _vof_box654}
+
+
+def main(names):
+    if not os.path.exists(REF):
+        raise SystemExit("build the reference first: make -f oracle/ref.mk REF=/root/reference")
+    for nm in names or CASES:
+        CASES[nm]()
+
+
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

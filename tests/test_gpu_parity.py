@@ -126,26 +126,53 @@ def test_assembly_vs_oracle(oracle_lib, eq, reorder):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["cavity4_jitter_random", "box654_jitter_2x2", "orifice_lid"])
+def _eq_of_call(d, k):
+    for eq in ("momentum", "pressure", "volume_fraction"):
+        if f"call{k}_{eq}_rows" in d:
+            return eq
+    return None
+
+
+@pytest.mark.parametrize("name", ["cavity4_jitter_random", "box654_jitter_2x2", "orifice_lid",
+                                  "vof_box654_2x2"])
 def test_assembly_replay_vs_reference(name):
-    """The reference's own COO systems, replayed from its call-time fields."""
+    """The reference's own COO systems, replayed from its call-time fields.
+    Momentum and pressure systems are bit-exact.  Volume-fraction systems
+    depend on MSTACS's blend weight pow(cos(theta), 4): the reference's libm
+    pow is not correctly rounded in ~0.1% of cases and the device pow
+    differs from it there in the last bit, so MSTACS face values and the VOF
+    system are compared at 1e-14 relative (all other entries exactly)."""
     d, mesh = load_golden(name)
-    g = _gpu(mesh, golden_case(d), False)
+    case = golden_case(d)
+    g = _gpu(mesh, case, False)
     k = 0
-    while f"call{k}_momentum_rows" in d or f"call{k}_pressure_rows" in d:
-        eq = "momentum" if f"call{k}_momentum_rows" in d else "pressure"
+    while _eq_of_call(d, k):
+        eq = _eq_of_call(d, k)
         p = f"call{k}_{eq}_"
-        for f in ("P", "U", "V", "W", "VF", "Rho", "mu", "oldU", "oldV", "oldW", "Qm0", "Qm1", "Qm2"):
-            g.set_field(f, d[p + f])
+        for f in ("P", "U", "V", "W", "VF", "Rho", "mu", "oldU", "oldV", "oldW", "Qm0", "Qm1", "Qm2",
+                  "Qm3", "oldVF"):
+            if p + f in d:
+                g.set_field(f, d[p + f])
+        if any(case.g):
+            for q in range(3):
+                g.set_field(f"srcG{q}", d[p + f"srcG{q}"])
         g.assemble(eq)
         gr, gc, gv, gb = g.matrix(eq)
         n = mesh.n_cells
         A1 = _csr(gr, gc, gv, n)
         A0 = _csr(d[p + "rows"], d[p + "cols"], d[p + "vals"], n)
-        assert (A1 != A0).nnz == 0, p
-        for q in range(3 if eq == "momentum" else 1):
-            assert np.array_equal(gb[q], d[p + f"b{q}"]), p
+        if eq == "volume_fraction":
+            for f in ("faceLVF", "faceRVF"):
+                a, b = g.get_field(f), d[p + f]
+                assert np.abs(a - b).max() <= 1e-14, (p, f)
+            assert (A1 != A0).nnz == 0 or abs(A1 - A0).max() <= 1e-14 * abs(A0).max(), p
+            assert _rel(gb[0], d[p + "b0"]) <= 1e-14, p
+        else:
+            assert (A1 != A0).nnz == 0, p
+            for q in range(3 if eq == "momentum" else 1):
+                assert np.array_equal(gb[q], d[p + f"b{q}"]), p
         k += 1
+    assert k == len([x for x in d if x.endswith("_rows")])
     g.close()
 
 
@@ -178,9 +205,9 @@ def test_simple_run_vs_reference(name):
     for _ in range(int(d["steps"])):
         g.incompressiblePressureBased(int(d["correctors"]))
     vel = np.sqrt(sum(np.linalg.norm(d["final_" + f]) ** 2 for f in "UVW"))
-    for f in ("P", "U", "V", "W"):
+    for f in ("P", "U", "V", "W", "VF"):
         a, b = g.get_field(f), d["final_" + f]
-        scale = np.linalg.norm(b) if f == "P" else vel
+        scale = np.linalg.norm(b) if f in ("P", "VF") else vel
         assert np.linalg.norm(a - b) <= 1e-7 * scale, f
     g.close()
 
@@ -205,6 +232,40 @@ def test_simple_iterations_vs_oracle(oracle_lib, precision, tol):
         a, b = g.get_field(f), o.get(f)
         scale = np.linalg.norm(b) if f == "P" else vel
         assert np.linalg.norm(a - b) <= tol * scale, (f, np.linalg.norm(a - b) / scale)
+    g.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_vof_steps_vs_oracle(oracle_lib, precision, tol):
+    """Two two-phase time steps (MSTACS + VOF equation, gravity, open top) on a
+    jittered 10x8x8 box: every field, alpha included, within relative L2
+    1e-10 (FP64) / 1e-6 (mixed) of the oracle."""
+    from semo_mesh_amd.cases import dam_break_case
+    mesh = hex_mesh(10, 8, 8, jitter=0.1, seed=5, ref_face=False)
+    case = dam_break_case(dt=2e-3, n_vof=2, urf_vf=0.9)
+    nc = mesh.n_cells
+    cen = np.stack([mesh.points[mesh.cell_pts[mesh.cell_ptr[i]:mesh.cell_ptr[i + 1]]].mean(0)
+                    for i in range(nc)])
+    vf = ((cen[:, 0] < 0.45) & (cen[:, 2] < 0.55)).astype(float)
+    rng = np.random.default_rng(9)
+    init = {"P": rng.normal(size=nc), "U": 0.2 * rng.normal(size=nc), "V": 0.2 * rng.normal(size=nc),
+            "W": 0.2 * rng.normal(size=nc), "VF": vf}
+    o = oracle_lib.Oracle(mesh, case)
+    o.set_solver(1e-14, 10000)
+    g = _gpu(mesh, case, True)
+    g.set_solver(precision, tol_p=1e-13 if precision == "fp64" else 1e-10, tol_u=1e-13,
+                 maxit_p=3000, maxit_u=3000)
+    for f, v in init.items():
+        o.set(f, v); g.set_field(f, v)
+    for _ in range(2):
+        o.step(2)
+        g.incompressiblePressureBased(2)
+    vel = np.sqrt(sum(np.linalg.norm(o.get(f)) ** 2 for f in "UVW"))
+    for f in ("P", "U", "V", "W", "VF"):
+        a, b = g.get_field(f), o.get(f)
+        scale = np.linalg.norm(b) if f in ("P", "VF") else vel
+        assert np.linalg.norm(a - b) <= tol * scale, (f, np.linalg.norm(a - b) / scale)
+    assert 0.0 < o.get("VF").mean() < 1.0
     g.close()
 
 
