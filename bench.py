@@ -58,6 +58,46 @@ def run_path(mesh, case, precision, steps, warmup, profile_kernels=True):
     return dict(seconds=t, upload_s=t_upload, warmup_s=t_warm, prof=prof, fields=fields)
 
 
+def dam_break_leg(steps, warmup, nx=96):
+    """Two-phase VOF dam break on the tetrahedral mesh of the 1.3M-cell
+    configuration: 4 x 2 x 2 m box, nx x nx/2 x nx/2 jittered cubes split into
+    6 tets each, water block x < 1.2, z < 1, gravity, dt = 1e-4, one VOF
+    iteration (MSTACS + volume-fraction equation) and one momentum/pressure
+    corrector per step, mixed-precision pressure solve.  Surface tension is
+    omitted: the curvature pipeline is not built, so sigma = 0."""
+    from semo_mesh_amd.cases import dam_break_case
+    from semo_mesh_amd.mesh import tet_mesh
+    from semo_mesh_amd.solver import GpuSolver
+    t0 = time.perf_counter()
+    m = tet_mesh(nx, nx // 2, nx // 2, hi=(4.0, 2.0, 2.0), jitter=0.1)
+    t_mesh = time.perf_counter() - t0
+    nc = m.n_cells
+    pts = m.points[m.cell_pts.reshape(nc, 4)].mean(axis=1)
+    vf = ((pts[:, 0] < 1.2) & (pts[:, 2] < 1.0)).astype(np.float64)
+    case = dam_break_case(dt=1e-4, n_vof=1, urf_vf=1.0)
+    g = GpuSolver(0)
+    g.upload_mesh(m, reorder=True)
+    g.set_case(case)
+    g.set_solver("mixed", tol_p=1e-8, tol_u=1e-8, maxit_p=1000, maxit_u=1000)
+    g.set_field("VF", vf)
+    for _ in range(warmup):
+        g.incompressiblePressureBased(1)
+    vf0 = float(g.get_field("VF").sum())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.incompressiblePressureBased(1)
+    t = time.perf_counter() - t0
+    vf1 = g.get_field("VF")
+    g.close()
+    return {"workload": f"VOF dam break, {nx}x{nx // 2}x{nx // 2} cubes x 6 tets ({nc} cells), "
+                        "rho 1000/1.2, mu 1e-3/1.8e-5, g = -9.81 z, dt 1e-4, sigma omitted",
+            "cells": nc, "steps": steps, "warmup": warmup,
+            "value": round(nc * steps / t / 1e6, 3), "unit": "Mcell-step/s",
+            "ms_per_step": round(1e3 * t / steps, 3), "mesh_s": round(t_mesh, 1),
+            "alpha_sum_change_rel": float(abs(vf1.sum() - vf0) / vf0),
+            "alpha_min": float(vf1.min()), "alpha_max": float(vf1.max())}
+
+
 def cpu_baseline(n=48):
     """The compiled reference (SEMO + AMGCL, 1 MPI rank, no OpenMP) on a
     bounded sample: one SIMPLE iteration of the same cavity at n^3."""
@@ -99,6 +139,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-vof", action="store_true", help="skip the two-phase dam-break leg")
+    ap.add_argument("--vof-steps", type=int, default=50)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB): the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -169,9 +211,32 @@ def main():
                            "bytes_per_launch": per_launch, "launches": st["count"]}
         out["kernels"] = {k: {"count": v["count"], "ms": round(v["ms"], 3),
                               "GBs": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()}
+        traffic = _measured_traffic(name)
+        if traffic:
+            out["roofline"]["traffic"] = traffic["bytes"]
+            out["roofline"]["traffic_source"] = traffic["source"]
+    if not args.no_vof:
+        out["vof_dam_break"] = dam_break_leg(args.vof_steps, 2)
+        print(f"[bench] dam break: {out['vof_dam_break']['ms_per_step']} ms/step", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
+
+
+def _measured_traffic(tag):
+    """HBM bytes per launch of a profiled kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/hbm_traffic.json
+    (counters scaled by the plain-copy calibration stated there)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            tab = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    e = tab.get("kernels", {}).get(tag)
+    if not e:
+        return None
+    return {"bytes": e["fetch_bytes"] + e["write_bytes"], "source": tab.get("source", path)}
 
 
 if __name__ == "__main__":

@@ -107,6 +107,8 @@ typedef struct {
     double urf_vf;          /* vofVofURF  (eqVolfrac.cpp:776)                */
     int32_t n_vof;          /* nSpeciesEq: VOF iterations per time step
                                (pressureBased.cpp:104-127); 0 = one phase   */
+    int32_t adjust_dt;      /* adjustTimeStep: dt from the Courant number    */
+    double max_co, max_dt;  /* maxCo, maxTimeStep (timestep.cpp:260-330)     */
 } semo_case;
 
 /* Replaces the controls/species read by SEMO_Controls_Builder
@@ -171,6 +173,12 @@ int semo_calc_pressure_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
  * eqVolfrac.cpp:9-787), calcIncomCellEOSVF + calcCellTransport; *norm =
  * sqrt(sum dalpha^2) as calcVolfracEq returns.                                 */
 int semo_calc_volfrac_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
+/* calcTimeStepFromCorantNumber (adjust != 0, src/solvers/timestep.cpp:260-330)
+ * or calcCorantNumberForPrint (adjust == 0, timestep.cpp:202-253): *dt gets
+ * the time step in use afterwards (adjusted: maxCo times the smallest
+ * convective / viscous / gravity / capillary time scale, capped at maxTimeStep)
+ * and *courant the Courant number.                                              */
+int semo_calc_time_step(semo_ctx* ctx, int adjust, double* dt, double* courant);
 /* incompressiblePressureBased (src/solvers/pressureBased.cpp:6-483) for
  * nPressureEq = 1: one time step of n_vof VOF iterations (semo_case.n_vof),
  * then n_correctors momentum + pressure iterations; norms[2*k], norms[2*k+1]

@@ -33,6 +33,13 @@ def lib():
                                    C.c_double, C.c_double, C.c_double]
         L.orc_set_solver.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.orc_set_vof.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.orc_set_time_control.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        L.orc_time_step.restype = C.c_double
+        L.orc_time_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_dt.argtypes = [C.c_void_p, C.c_double]
+        for nm in ("orc_get_dt", "orc_courant"):
+            getattr(L, nm).restype = C.c_double
+            getattr(L, nm).argtypes = [C.c_void_p]
         for nm in ("orc_mstacs", "orc_assemble_vof"):
             getattr(L, nm).argtypes = [C.c_void_p]
         for nm in ("orc_vof_update", "orc_vof_norm"):
@@ -93,6 +100,8 @@ class Oracle:
                        case.sigma, np.array(case.g, dtype=np.float64), case.dt, case.urf_mom,
                        case.urf_p, case.urf_pu, case.limit)
         L.orc_set_vof(self._h, getattr(case, "urf_vf", 1.0), getattr(case, "n_vof", 0))
+        L.orc_set_time_control(self._h, 1 if getattr(case, "adjust_dt", False) else 0,
+                               getattr(case, "max_co", 1.0), getattr(case, "max_dt", 1.0))
         for i, nm in enumerate(self.mesh.patch_names):
             b = case.bcs[nm]
             L.orc_set_bc(self._h, i, b.p_type, b.p_value, b.u_type,
@@ -138,6 +147,10 @@ class Oracle:
         it = lib().orc_solve(self._h, nrhs, C.byref(r))
         return it, r.value
 
+    def time_step(self, adjust): return lib().orc_time_step(self._h, 1 if adjust else 0)
+    def dt(self): return lib().orc_get_dt(self._h)
+    def set_dt(self, dt): lib().orc_set_dt(self._h, float(dt))
+    def courant(self): return lib().orc_courant(self._h)
     def mstacs(self): lib().orc_mstacs(self._h)
     def assemble_vof(self): lib().orc_assemble_vof(self._h)
     def vof_update(self): return lib().orc_vof_update(self._h)

@@ -36,7 +36,7 @@ OBJS := $(foreach s,$(SRCS),$(OBJ)/$(subst /,_,$(s)).o)
 SOLVE1 := _ZN20SEMO_Solvers_Builder10solveAMGCLENSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEER17SEMO_Mesh_BuilderiRSt6vectorIiSaIiEESB_RS8_IdSaIdEESE_SE_
 SOLVE3 := $(SOLVE1)SE_SE_SE_SE_
 
-all: $(OUT)/semo_ref $(OUT)/lib/.stamp
+all: $(OUT)/semo_ref $(OUT)/rcm_ref $(OUT)/lib/.stamp
 
 $(OBJ)/.dir:
 	mkdir -p $(OBJ) && touch $@
@@ -53,6 +53,13 @@ $(OBJ)/ref_driver.o: $(HERE)ref_driver.cpp $(OBJ)/.dir
 $(OUT)/semo_ref: $(OBJS) $(OBJ)/ref_driver.o
 	$(CXX) -o $@ $^ -Wl,--wrap=$(SOLVE1) -Wl,--wrap=$(SOLVE3) \
 	  $(MPIDIR)/lib/libmpicxx.so.12 $(MPIDIR)/lib/libmpi.so.12 -lz \
+	  -Wl,-rpath,'$$ORIGIN/lib' -Wl,--allow-shlib-undefined
+
+$(OBJ)/rcm_ref_main.o: $(HERE)rcm_ref_main.cpp $(OBJ)/.dir
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/rcm_ref: $(OBJ)/math_RCM.o $(OBJ)/rcm_ref_main.o
+	$(CXX) -o $@ $^ $(MPIDIR)/lib/libmpicxx.so.12 $(MPIDIR)/lib/libmpi.so.12 \
 	  -Wl,-rpath,'$$ORIGIN/lib' -Wl,--allow-shlib-undefined
 
 $(OUT)/lib/.stamp:

@@ -102,6 +102,7 @@ static void dump_state(SEMO_Mesh_Builder& mesh, const string& p) {
     dump_cellvar(mesh, c.Qm[0], p + "Qm0"); dump_cellvar(mesh, c.Qm[1], p + "Qm1");
     dump_cellvar(mesh, c.Qm[2], p + "Qm2");
     dump_cellvar(mesh, c.Qm[3], p + "Qm3");
+    dump(p + "dt", vector<double>{c.timeStep});
     dump_cellvar(mesh, c.oldVF[0], p + "oldVF");
     {   // face volume fractions (MSTACS output on internal faces)
         vector<double> lv(mesh.faces.size()), rv(mesh.faces.size());
@@ -159,6 +160,8 @@ int main(int argc, char** argv) {
     // case file: "key value..." lines
     double dt = 0.01, rho[2] = {1000, 1000}, mu[2] = {1e-3, 1e-3}, sigma = 0, g[3] = {0, 0, 0};
     double urf_u = 0.3, urf_p = 0.3, urf_pu = 0.3, lim = 1e10, urf_vf = 1.0;
+    int adjust_dt = 0;
+    double max_co = 1.0, max_dt = -1.0;
     int nsteps = 1, ncorr = 1, nvof = 0;
     vector<PatchBC> bcs;
     string initfile;
@@ -177,6 +180,7 @@ int main(int argc, char** argv) {
             else if (key == "correctors") cf >> ncorr;
             else if (key == "vof") cf >> nvof;
             else if (key == "urfvf") cf >> urf_vf;
+            else if (key == "adjustdt") cf >> adjust_dt >> max_co >> max_dt;
             else if (key == "init") cf >> initfile;
             else if (key == "bc") {
                 PatchBC b;
@@ -197,7 +201,8 @@ int main(int argc, char** argv) {
     controls.setValues(species);
     controls.timeStep = dt; controls.orgTimeStep = dt;
     controls.oldTimeStep = dt; controls.old2TimeStep = dt;
-    controls.adjustTimeStep = "no"; controls.maxCo = 1.0; controls.maxTimeStep = dt;
+    controls.adjustTimeStep = adjust_dt ? "yes" : "no";
+    controls.maxCo = max_co; controls.maxTimeStep = max_dt > 0 ? max_dt : dt;
     controls.gravityAcceleration = {g[0], g[1], g[2]};
     controls.momVelURF = urf_u; controls.prePreURF = urf_p; controls.preVelURF = urf_pu;
     controls.vofVofURF = urf_vf;

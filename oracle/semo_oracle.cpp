@@ -56,6 +56,9 @@ struct Oracle {
     double dt = 0.01, oldDt = 0.01, old2Dt = 0.01;
     double urf_u = 0.3, urf_p = 0.3, urf_pu = 0.3, lim = 1e10;
     double urf_vf = 1.0;               // vofVofURF (controls/build.cpp:475)
+    int adjust_dt = 0;                 // adjustTimeStep == "yes"
+    double max_co = 1.0, max_dt = 1.0; // maxCo, maxTimeStep
+    double courant = 0.0;              // Courant number of the last time-step pass
     int n_vof = 0;                     // nSpeciesEq: VOF iterations per step (controls/build.cpp:409)
     int iterReal = 0;
 
@@ -67,7 +70,7 @@ struct Oracle {
     vector<double> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
     // face left/right values (src/solvers/reconIncom.cpp:501-1035)
     vector<double> LP, RP, LU, RU, LV, RV, LW, RW, LVF, RVF, Lmu, Rmu, LRho, RRho;
-    vector<double> courant;            // per-cell Courant number of the last MSTACS pass
+    vector<double> co_cell;            // per-cell Courant number of the last MSTACS pass
     double vof_norm = 0.0;             // return value of the last calcVolfracEq
 
     // ---- last assembled system -------------------------------------------------
@@ -685,12 +688,12 @@ void mstacs_faces(Oracle& o) {
     vector<double> g;
     lsq_grad(o, o.VF, o.RVF, g);                                  // reconIncom.cpp:366-369
     const double tmpCFL = 1.0, eps = 1.e-12;
-    o.courant.assign(o.nc, 0.0);
+    o.co_cell.assign(o.nc, 0.0);
     for (int i = 0; i < o.nc; ++i) {                              // NVD.cpp:1114-1135
         double magVel = sqrt(o.U[i] * o.U[i] + o.V[i] * o.V[i] + o.W[i] * o.W[i]);
         double maxA = 0.0;
         for (int k = o.cfptr[i]; k < o.cfptr[i + 1]; ++k) maxA = std::max(maxA, o.farea[o.cfidx[k]]);
-        o.courant[i] = 1.0 / tmpCFL * magVel * o.dt * maxA / o.cvol[i];
+        o.co_cell[i] = 1.0 / tmpCFL * magVel * o.dt * maxA / o.cvol[i];
     }
     for (int f = 0; f < o.nif; ++f) {                             // NVD.cpp:1166-1858
         int l = o.owner[f], r = o.nbr[f];
@@ -701,7 +704,7 @@ void mstacs_faces(Oracle& o) {
         const double* pn = &o.fdist[3 * (size_t)f];              // vecPN of an internal face
         const double* n = &o.fnrm[3 * (size_t)f];
         const double* sk = &o.fskew[3 * (size_t)f];
-        double coDDL = o.courant[l], coDDR = o.courant[r];
+        double coDDL = o.co_cell[l], coDDR = o.co_cell[r];
         double L2 = 0.0;
         for (int a = 0; a < 3; ++a) L2 += gl[a] * pn[a];
         L2 = R1 - 2.0 * L2;
@@ -1036,6 +1039,47 @@ double pressure_correct(Oracle& o) {
     return sqrt(nd) / sqrt(np);
 }
 
+// calcTimeStepFromCorantNumber (adjust != 0, timestep.cpp:260-330) or
+// calcCorantNumberForPrint (timestep.cpp:202-253): the smallest of the
+// convective, viscous, gravity and capillary time scales over all cells.
+// Returns the Courant number; with adjust the time step becomes
+// maxCo * that minimum, capped at maxTimeStep.
+double time_step(Oracle& o, int adjust) {
+    double minDt = 1000000.0;
+    for (int i = 0; i < o.nc; ++i) {
+        double maxA = 0.0, minA = 1000000.0;
+        for (int k = o.cfptr[i]; k < o.cfptr[i + 1]; ++k) {
+            maxA = std::max(maxA, o.farea[o.cfidx[k]]);
+            minA = std::min(minA, o.farea[o.cfidx[k]]);
+        }
+        double magVel = sqrt(o.U[i] * o.U[i] + o.V[i] * o.V[i] + o.W[i] * o.W[i]);
+        double convTime = o.cvol[i] / maxA / (magVel + 1.e-200);
+        minDt = std::min(minDt, convTime);
+        double viscTime = 0.5 * minA / (o.Mu[i] + 1.e-200) * o.Rho[i];
+        minDt = std::min(minDt, viscTime);
+        double gravMag = sqrt(pow(o.g[0], 2.0) + pow(o.g[1], 2.0) + pow(o.g[2], 2.0));
+        double gravTime = sqrt(sqrt(minA) / (gravMag + 1.e-200));
+        minDt = std::min(minDt, gravTime);
+        double stTime;
+        if (adjust) {
+            double minDX = o.cvol[i] / maxA;
+            double C_sigma = sqrt(6.28 * o.sigma / (o.rho[0] + o.rho[1]) / minDX);
+            stTime = minDX / (C_sigma + magVel);
+        } else {
+            stTime = sqrt(pow(minA, 1.5) * (1000.0 + 1.2) / (4.0 * 3.141592 * 0.0728));
+        }
+        minDt = std::min(minDt, stTime);
+    }
+    if (!adjust) return o.dt / minDt;
+    o.dt = o.max_co * minDt;
+    double co = o.max_co;
+    if (o.dt > o.max_dt) {
+        co = o.max_co * o.max_dt / o.dt;
+        o.dt = o.max_dt;
+    }
+    return co;
+}
+
 // one call of incompressiblePressureBased with nSpeciesEq = 0
 // (pressureBased.cpp:6-413); sigma must be 0 (no curvature in this oracle).
 void pressure_based_step(Oracle& o, int ncorr, double* norms) {
@@ -1043,6 +1087,7 @@ void pressure_based_step(Oracle& o, int ncorr, double* norms) {
     if (o.iterReal == 0) { o.Qm0 = o.U; o.Qm1 = o.V; o.Qm2 = o.W; o.Qm3 = o.VF; }
     else { o.Qm0 = o.oldU; o.Qm1 = o.oldV; o.Qm2 = o.oldW; o.Qm3 = o.oldVF; }
     o.oldP = o.P; o.oldU = o.U; o.oldV = o.V; o.oldW = o.W; o.oldVF = o.VF; o.oldRho = o.Rho;
+    o.courant = time_step(o, o.adjust_dt);            // pressureBased.cpp:74-90
     for (int k = 0; k < o.n_vof; ++k) {               // pressureBased.cpp:104-127
         mstacs_faces(o);
         assemble_vof(o);
@@ -1154,9 +1199,12 @@ int rcm_component(const Graph& G, int root, vector<int>& mask, int* perm, vector
                 if (mask[nb] != 0) { mask[nb] = 0; perm[lnbr++] = nb; }
             }
             if (lnbr - fnbr <= 1) continue;
-            for (int k = fnbr + 1; k < lnbr; ++k) {      // stable insertion sort by degree
+            // stable insertion sort by degree (RCM.cpp:3549-3570); the loop
+            // condition `while (fnbr < l)` of the reference never compares the
+            // segment's first node, which therefore stays first
+            for (int k = fnbr + 1; k < lnbr; ++k) {
                 int nb = perm[k], l = k - 1;
-                while (l >= fnbr && deg[perm[l]] > deg[nb]) { perm[l + 1] = perm[l]; --l; }
+                while (l > fnbr && deg[perm[l]] > deg[nb]) { perm[l + 1] = perm[l]; --l; }
                 perm[l + 1] = nb;
             }
         }
@@ -1221,6 +1269,16 @@ void orc_set_vof(void* h, double urf_vf, int n_vof) {
     o.urf_vf = urf_vf; o.n_vof = n_vof;
 }
 
+// adjustTimeStep, maxCo, maxTimeStep (controls/build.cpp)
+void orc_set_time_control(void* h, int adjust, double max_co, double max_dt) {
+    Oracle& o = *(Oracle*)h;
+    o.adjust_dt = adjust; o.max_co = max_co; o.max_dt = max_dt;
+}
+double orc_time_step(void* h, int adjust) { return time_step(*(Oracle*)h, adjust); }
+double orc_get_dt(void* h) { return ((Oracle*)h)->dt; }
+void orc_set_dt(void* h, double dt) { ((Oracle*)h)->dt = dt; }
+double orc_courant(void* h) { return ((Oracle*)h)->courant; }
+
 void orc_set_solver(void* h, double tol, int maxit) {
     Oracle& o = *(Oracle*)h; o.tol = tol; o.maxit = maxit;
 }
@@ -1240,7 +1298,7 @@ static vector<double>* field(Oracle& o, const char* nm) {
     F("b0", B0) F("b1", B1) F("b2", B2) F("x0", X0) F("x1", X1) F("x2", X2) F("vals", Avals)
     F("faceRP", RP) F("faceRU", RU) F("faceRV", RV) F("faceRW", RW) F("faceRVF", RVF)
     F("faceLU", LU) F("faceLV", LV) F("faceLW", LW) F("faceLRho", LRho) F("faceLVF", LVF)
-    F("oldVF", oldVF) F("Qm3", Qm3) F("courant", courant)
+    F("oldVF", oldVF) F("Qm3", Qm3) F("courant", co_cell)
 #undef F
     return nullptr;
 }

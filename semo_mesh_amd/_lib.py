@@ -49,7 +49,8 @@ class CaseDesc(C.Structure):
     _fields_ = [("rho", C.c_double * 2), ("mu", C.c_double * 2), ("sigma", C.c_double),
                 ("g", C.c_double * 3), ("dt", C.c_double), ("urf_mom", C.c_double),
                 ("urf_p", C.c_double), ("urf_pu", C.c_double), ("limit", C.c_double),
-                ("urf_vf", C.c_double), ("n_vof", C.c_int32)]
+                ("urf_vf", C.c_double), ("n_vof", C.c_int32), ("adjust_dt", C.c_int32),
+                ("max_co", C.c_double), ("max_dt", C.c_double)]
 
 
 class SolverParams(C.Structure):
@@ -67,7 +68,7 @@ EXPORTED = [
     "semo_ctx_create", "semo_ctx_destroy", "semo_ctx_last_error", "semo_ctx_device_bytes",
     "semo_mesh_upload", "semo_mesh_get_perm", "semo_mesh_get_face_map", "semo_mesh_get_connectivity",
     "semo_rcm", "semo_set_case", "semo_set_solver", "semo_field_set", "semo_field_get",
-    "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_calc_volfrac_eq",
+    "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_calc_volfrac_eq", "semo_calc_time_step",
     "semo_incompressible_pressure_based",
     "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
@@ -101,6 +102,7 @@ def lib():
         "semo_calc_momentum_eqs": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
         "semo_calc_pressure_eq": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
         "semo_calc_volfrac_eq": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(SolveStats)]),
+        "semo_calc_time_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "semo_incompressible_pressure_based": (C.c_int, [vp, C.c_int, dp]),
         "semo_get_solve_stats": (C.c_int, [vp, C.c_int, C.POINTER(SolveStats)]),
         "semo_assemble": (C.c_int, [vp, C.c_int]),

@@ -47,6 +47,9 @@ class Case:
     limit: float = 1e10       # |P|,|U|,|V|,|W| clip bounds (fvSolution limiter)
     urf_vf: float = 1.0       # vofVofURF  (eqVolfrac.cpp:776)
     n_vof: int = 0            # nSpeciesEq: VOF iterations per time step (pressureBased.cpp:104-127)
+    adjust_dt: bool = False   # adjustTimeStep: dt = maxCo * min time scale (timestep.cpp:260-330)
+    max_co: float = 1.0       # maxCo
+    max_dt: float = 1.0       # maxTimeStep
     bcs: dict = field(default_factory=dict)   # patch name -> PatchBC
 
     def write_ref_case(self, path: str, steps: int = 1, correctors: int = 1,
@@ -59,6 +62,8 @@ class Case:
             fh.write(f"limit {self.limit!r}\nsteps {steps}\ncorrectors {correctors}\n")
             if self.n_vof:
                 fh.write(f"vof {self.n_vof}\nurfvf {self.urf_vf!r}\n")
+            if self.adjust_dt:
+                fh.write(f"adjustdt 1 {self.max_co!r} {self.max_dt!r}\n")
             if init:
                 fh.write(f"init {init}\n")
             for name, b in self.bcs.items():

@@ -342,6 +342,7 @@ struct FaceIn {
     const double *mxU, *mnU, *mxV, *mnV, *mxW, *mnW, *mxP, *mnP;
     const double *nrm, *ePN, *skew, *wC, *alpha, *magPN, *area;
     const int *owner, *nbr;
+    const int8_t* flip;
     int nc;
 };
 
@@ -554,7 +555,10 @@ __global__ void __launch_bounds__(TPB) k_p_face(
     if (f >= nif) return;
     int l = in.owner[f], r = in.nbr[f];
     double area = in.area[f];
-    double wCL = in.wC[f], wCR = 1.0 - wCL;
+    // rho* with the neighbour weight twice (eqPressure.cpp:396-397): the
+    // weight of the face's neighbour in the mesh's own orientation, i.e. this
+    // side's owner weight when the renumbering flipped the face
+    double wCL = in.wC[f], wCR = in.flip[f] ? wCL : 1.0 - wCL;
     double Rs = 1.0 / (wCR / in.Rho[l] + wCR / in.Rho[r]);
     double t1 = k.dt / Rs;
     double dPN = in.magPN[f], alpha = in.alpha[f];
@@ -790,6 +794,52 @@ __global__ void __launch_bounds__(TPB) k_vof_cell(
     B[i] = b;
 }
 
+// smallest of the convective, viscous, gravity and capillary time scales of
+// each cell, block minima into part[2b] (timestep.cpp:202-253 with adjust = 0,
+// timestep.cpp:260-330 with adjust = 1)
+__global__ void __launch_bounds__(TPB) k_time_scale(
+    int nc, CaseK k, int adjust, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const double* __restrict__ area, const double* __restrict__ vol, const double* __restrict__ U,
+    const double* __restrict__ V, const double* __restrict__ W, const double* __restrict__ Mu,
+    const double* __restrict__ Rho, double* __restrict__ part) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double minDt = 1000000.0;
+    if (i < nc) {
+        double maxA = 0.0, minA = 1000000.0;
+        CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
+            (void)own;
+            maxA = smax(maxA, area[f]);
+            minA = smin(minA, area[f]);
+        }
+        double magVel = sqrt(U[i] * U[i] + V[i] * V[i] + W[i] * W[i]);
+        double convTime = vol[i] / maxA / (magVel + 1.e-200);
+        minDt = smin(minDt, convTime);
+        double viscTime = 0.5 * minA / (Mu[i] + 1.e-200) * Rho[i];
+        minDt = smin(minDt, viscTime);
+        double gravMag = sqrt(k.g0 * k.g0 + k.g1 * k.g1 + k.g2 * k.g2);
+        double gravTime = sqrt(sqrt(minA) / (gravMag + 1.e-200));
+        minDt = smin(minDt, gravTime);
+        double stTime;
+        if (adjust) {
+            double minDX = vol[i] / maxA;
+            double C_sigma = sqrt(6.28 * k.sigma / (k.rho0 + k.rho1) / minDX);
+            stTime = minDX / (C_sigma + magVel);
+        } else {
+            stTime = sqrt(pow(minA, 1.5) * (1000.0 + 1.2) / (4.0 * 3.141592 * 0.0728));
+        }
+        minDt = smin(minDt, stTime);
+    }
+    __shared__ double sm[TPB / 64];
+    for (int o = 32; o > 0; o >>= 1) minDt = smin(minDt, __shfl_down(minDt, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = minDt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sm[0];
+        for (int w = 1; w < TPB / 64; ++w) m = smin(m, sm[w]);
+        part[2 * blockIdx.x] = m;
+    }
+}
+
 // VF += urf dVF, clipped to [0,1]; block partials of dVF^2 (eqVolfrac.cpp:770-783)
 __global__ void __launch_bounds__(TPB) k_vof_update(int nc, double urf, const double* __restrict__ x,
                                                     double* __restrict__ VF, double* __restrict__ part) {
@@ -905,7 +955,7 @@ FaceIn face_in(semo_ctx* c) {
     in.mnW = c->mnW.p; in.mxP = c->mxP.p; in.mnP = c->mnP.p;
     in.nrm = c->f_nrm.p; in.ePN = c->f_ePN.p; in.skew = c->f_skew.p; in.wC = c->f_wC.p;
     in.alpha = c->f_alpha.p; in.magPN = c->f_magPN.p; in.area = c->f_area.p;
-    in.owner = c->d_owner.p; in.nbr = c->d_nbr.p; in.nc = c->nc;
+    in.owner = c->d_owner.p; in.nbr = c->d_nbr.p; in.flip = c->d_fflip.p; in.nc = c->nc;
     return in;
 }
 
@@ -1074,6 +1124,30 @@ void assemble_vof(semo_ctx* c) {
                            c->fscr.p, c->Amat.diag.p, c->Bvec.p);
     }
     c->last_eq = 2;
+}
+
+// calcTimeStepFromCorantNumber / calcCorantNumberForPrint: returns the
+// Courant number; with adjust the case's time step becomes maxCo times the
+// smallest time scale, capped at maxTimeStep
+double time_step(semo_ctx* c, int adjust) {
+    CaseK k = case_k(c);
+    int nb = nblk(c->nc);
+    hipLaunchKernelGGL(k_time_scale, dim3(nb), dim3(TPB), 0, c->stream, c->nc, k, adjust, c->d_cf_sl.p,
+                       c->d_cf_ell.p, c->f_area.p, c->c_vol.p, c->U.p, c->V.p, c->W.p, c->Mu.p, c->Rho.p,
+                       c->red.p);
+    std::vector<double> h(2 * (size_t)nb);
+    SEMO_HIP(hipMemcpyAsync(h.data(), c->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    double minDt = 1000000.0;
+    for (int q = 0; q < nb; ++q) minDt = std::min(minDt, h[2 * q]);
+    if (!adjust) return c->cs.dt / minDt;
+    c->cs.dt = c->cs.max_co * minDt;
+    double co = c->cs.max_co;
+    if (c->cs.dt > c->cs.max_dt) {
+        co = c->cs.max_co * c->cs.max_dt / c->cs.dt;
+        c->cs.dt = c->cs.max_dt;
+    }
+    return co;
 }
 
 // VF += vofVofURF dVF, clipped; returns sqrt(sum dVF^2) (eqVolfrac.cpp:770-783)

@@ -260,6 +260,8 @@ int semo_set_case(semo_ctx* c, const semo_case* cs, const semo_patch_bc* bcs) {
                    SEMO_EINVAL, "case: dt, rho and mu must be positive");
         SEMO_CHECK(cs->sigma == 0.0, SEMO_EINVAL, "surface tension (curvature) is not available on this path");
         SEMO_CHECK(cs->n_vof >= 0 && cs->urf_vf > 0.0, SEMO_EINVAL, "case: n_vof >= 0 and urf_vf > 0");
+        SEMO_CHECK(!cs->adjust_dt || (cs->max_co > 0.0 && cs->max_dt > 0.0), SEMO_EINVAL,
+                   "case: maxCo and maxTimeStep must be positive");
         for (int b = 0; b < c->npatch; ++b) {
             const semo_patch_bc& p = bcs[b];
             SEMO_CHECK(p.p_type == SEMO_BC_ZERO_GRADIENT || p.p_type == SEMO_BC_FIXED_VALUE, SEMO_EINVAL_BC,
@@ -363,7 +365,7 @@ int semo_field_get(semo_ctx* c, int f, double* h, int64_t n) {
             }
             SEMO_CHECK(n == (int64_t)c->nf * comps, SEMO_EINVAL, "field size");
             face_from_dev(c, *d, h, comps, odd);
-            if (f == SEMO_G_FACE_WC)   // orientation of flipped faces
+            if (f == SEMO_G_FACE_WC)   // orientation of flipped faces (exact complement)
                 for (int k = 0; k < c->nif; ++k) if (c->fflip[k]) h[c->fperm[k]] = 1.0 - h[c->fperm[k]];
             return;
         }
@@ -405,11 +407,23 @@ int semo_calc_volfrac_eq(semo_ctx* c, double* norm, semo_solve_stats* st) {
     });
 }
 
+int semo_calc_time_step(semo_ctx* c, int adjust, double* dt, double* courant) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        double co = time_step(c, adjust);
+        c->courant = co;
+        if (dt) *dt = c->cs.dt;
+        if (courant) *courant = co;
+    });
+}
+
 int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
     if (!c || ncorr < 1) return SEMO_EINVAL;
     return guard(c, [&] {
         need_case(c);
         step_setup(c);
+        c->courant = time_step(c, c->cs.adjust_dt);     // pressureBased.cpp:77-90
         for (int k = 0; k < c->cs.n_vof; ++k) volfrac_eq(c, &c->last_stats_vf);
         step_sources(c);
         for (int it = 0; it < ncorr; ++it) {

@@ -188,6 +188,7 @@ struct semo_ctx {
     semo::DBuf<int> d_cf_sl, d_cf_ell, d_bf_ptr, d_bf_idx;
     semo::DBuf<double> cpk;                // packed [x,y,z,P,U,V,W,VF] per cell
     semo::DBuf<int> d_bpatch;              // boundary face -> patch
+    semo::DBuf<int8_t> d_fflip;            // face flipped by the renumbering (owner = original neighbour)
     semo::DBuf<int> d_flags;               // kernel error flags
     // geometry
     semo::DBuf<double> f_area, f_nrm, f_xyz, f_wC, f_magPN, f_alpha, f_ePN, f_skew, f_pf, f_nf;
@@ -210,6 +211,7 @@ struct semo_ctx {
     semo::DBuf<semo_patch_bc> d_bcs;
     bool has_case = false;
     int iterReal = 0;
+    double courant = 0.0;                  // Courant number of the last time-step pass
 
     // ---- fields (device numbering) ----
     semo::DBuf<double> P, U, V, W, VF, Rho, Mu, oldP, oldU, oldV, oldW, oldVF, Qm0, Qm1, Qm2,
@@ -252,6 +254,7 @@ void grad_minmax(semo_ctx* c, bool with_p_minmax);
 void assemble_momentum(semo_ctx* c);
 void assemble_pressure(semo_ctx* c);
 void assemble_vof(semo_ctx* c);
+double time_step(semo_ctx* c, int adjust);
 double vof_update(semo_ctx* c);
 double momentum_update(semo_ctx* c);
 double pressure_correct(semo_ctx* c);

@@ -121,8 +121,8 @@ __global__ void __launch_bounds__(TPB) k_cell_volume(
 // per-face metrics (geometric.cpp:1206-1227, 1297-1497)
 __global__ void __launch_bounds__(TPB) k_face_metrics(
     int nf, int nif, const int* __restrict__ owner, const int* __restrict__ nbr,
-    const double* __restrict__ nrm, const double* __restrict__ fxyz, const double* __restrict__ cxyz,
-    double* __restrict__ wC_o, double* __restrict__ magPN_o, double* __restrict__ alpha_o,
+    const int8_t* __restrict__ flip, const double* __restrict__ nrm, const double* __restrict__ fxyz,
+    const double* __restrict__ cxyz, double* __restrict__ wC_o, double* __restrict__ magPN_o, double* __restrict__ alpha_o,
     double* __restrict__ ePN_o, double* __restrict__ skew_o, double* __restrict__ pf_o,
     double* __restrict__ nf_o) {
     int f = xcd_block() * blockDim.x + threadIdx.x;
@@ -139,8 +139,17 @@ __global__ void __launch_bounds__(TPB) k_face_metrics(
         double dPN = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         double dFPo = fabs(dFP[0] * n[0] + dFP[1] * n[1] + dFP[2] * n[2]);
         double dFNo = fabs(dFN[0] * n[0] + dFN[1] * n[1] + dFN[2] * n[2]);
-        wC = dFNo / (dFPo + dFNo);
-        wC = (double)(float)(round(wC * 10000000) / 10000000);     // geometric.cpp:1351
+        // geometric.cpp:1349-1351, rounded through float in the mesh's own
+        // orientation: for a face flipped by the renumbering the reference's
+        // owner is this face's neighbour, so its weight is rounded and this
+        // side gets the exact complement
+        if (flip[f]) {
+            wC = dFPo / (dFNo + dFPo);
+            wC = 1.0 - (double)(float)(round(wC * 10000000) / 10000000);
+        } else {
+            wC = dFNo / (dFPo + dFNo);
+            wC = (double)(float)(round(wC * 10000000) / 10000000);
+        }
         magPN = dPN;
         for (int a = 0; a < 3; ++a) { e[a] = d[a] / dPN; pf[a] = dFP[a]; nfv[a] = dFN[a]; }
         double D = -(n[0] * fc[0] + n[1] * fc[1] + n[2] * fc[2]);
@@ -214,8 +223,9 @@ void geometry_compute(semo_ctx* c) {
                        c->d_pts.p, c->f_area.p, c->f_nrm.p, c->f_xyz.p, c->f_vsn.p, c->f_cc.p, c->d_flags.p);
     hipLaunchKernelGGL(k_cell_volume, dim3(nblk(nc)), dim3(TPB), 0, s, nc, c->nif, c->d_cf_ptr.p,
                        c->d_cf_idx.p, c->d_owner.p, c->f_vsn.p, c->f_cc.p, c->c_vol.p, c->c_xyz.p, c->d_flags.p);
+    c->d_fflip.upload(c->fflip, s);
     hipLaunchKernelGGL(k_face_metrics, dim3(nblk(nf)), dim3(TPB), 0, s, nf, c->nif, c->d_owner.p,
-                       c->d_nbr.p, c->f_nrm.p, c->f_xyz.p, c->c_xyz.p, c->f_wC.p, c->f_magPN.p,
+                       c->d_nbr.p, c->d_fflip.p, c->f_nrm.p, c->f_xyz.p, c->c_xyz.p, c->f_wC.p, c->f_magPN.p,
                        c->f_alpha.p, c->f_ePN.p, c->f_skew.p, c->f_pf.p, c->f_nf.p);
     hipLaunchKernelGGL(k_lsq_init, dim3(nblk(nc)), dim3(TPB), 0, s, nc, c->nif, c->d_st_ptr.p,
                        c->d_st_idx.p, c->d_cf_ptr.p, c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p,

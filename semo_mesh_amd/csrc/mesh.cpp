@@ -107,9 +107,13 @@ void genrcm(int n, const int* ptr, const int* adj, int* perm, int* inv) {
                     int w = adj[k];
                     if (live[w]) { live[w] = 0; out[tail++] = w; }
                 }
-                for (int a = first + 1; a < tail; ++a) {     // stable, ascending degree
+                // stable ascending-degree insertion of the new segment; as in the
+                // reference's translation of SPARSPAK (RCM.cpp:3549-3570, loop
+                // `while (fnbr < l)`), the segment's first node is never compared
+                // and so stays first
+                for (int a = first + 1; a < tail; ++a) {
                     int w = out[a], b = a - 1;
-                    while (b >= first && deg[out[b]] > deg[w]) { out[b + 1] = out[b]; --b; }
+                    while (b > first && deg[out[b]] > deg[w]) { out[b + 1] = out[b]; --b; }
                     out[b + 1] = w;
                 }
             }

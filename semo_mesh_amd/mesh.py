@@ -220,3 +220,115 @@ def hex_mesh(nx: int, ny: int, nz: int,
 def cavity_mesh(n: int, jitter: float = 0.0, seed: int = DEFAULT_SEED) -> Mesh:
     """Unit-cube lid-driven cavity mesh (n^3 hexes) with the pref patch."""
     return hex_mesh(n, n, n, jitter=jitter, seed=seed, ref_face=True)
+
+
+def _box_points(nx, ny, nz, lo, hi, jitter, seed):
+    """Lattice points of a box (x fastest) with the seeded interior jitter of
+    ``hex_mesh``; returns the points and their integer lattice coordinates."""
+    px, py, pz = nx + 1, ny + 1, nz + 1
+    h = [(hi[a] - lo[a]) / n for a, n in enumerate((nx, ny, nz))]
+    kk, jj, ii = np.meshgrid(np.arange(pz), np.arange(py), np.arange(px), indexing="ij")
+    ii = ii.ravel(); jj = jj.ravel(); kk = kk.ravel()
+    pts = np.empty((px * py * pz, 3), dtype=np.float64)
+    pts[:, 0] = lo[0] + ii * h[0]
+    pts[:, 1] = lo[1] + jj * h[1]
+    pts[:, 2] = lo[2] + kk * h[2]
+    pts[ii == nx, 0] = hi[0]; pts[jj == ny, 1] = hi[1]; pts[kk == nz, 2] = hi[2]
+    if jitter > 0.0:
+        interior = ((ii > 0) & (ii < nx) & (jj > 0) & (jj < ny) & (kk > 0) & (kk < nz))
+        vid = np.nonzero(interior)[0].astype(np.uint64)
+        for a in range(3):
+            u = unit_uniform(seed, np.uint64(3) * vid + np.uint64(a))
+            pts[vid.astype(np.int64), a] += (2.0 * u - 1.0) * jitter * h[a]
+    return pts, np.stack([ii, jj, kk], axis=1)
+
+
+# Kuhn split of a cube into 6 tetrahedra along its (0,0,0)-(1,1,1) diagonal:
+# for every axis order (a, b, c) the path 0 -> e_a -> e_a + e_b -> (1,1,1).
+# Corner ids are bit masks x + 2y + 4z; all cubes use the same diagonal, so
+# the split is conforming across cube faces.
+_KUHN = [(0, 1 << a, (1 << a) | (1 << b), 7)
+         for a, b in ((0, 1), (0, 2), (1, 0), (1, 2), (2, 0), (2, 1))]
+
+
+def tet_mesh(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+             jitter: float = 0.0, seed: int = DEFAULT_SEED) -> Mesh:
+    """Box of nx*ny*nz cubes, each split into 6 tetrahedra (Kuhn split), in
+    SEMO's face-based layout: internal faces sorted by (owner, neighbour)
+    with owner < neighbour, then the boundary patches xmin, xmax, ymin, ymax,
+    zmin, zmax, each sorted by owner.  Face points are ordered so the
+    right-hand normal points out of the owner.  Interior lattice points get
+    the seeded jitter of ``hex_mesh``."""
+    nx, ny, nz = int(nx), int(ny), int(nz)
+    px, py = nx + 1, ny + 1
+    pts, ijk = _box_points(nx, ny, nz, lo, hi, jitter, seed)
+    ck, cj, ci = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ci = ci.ravel(); cj = cj.ravel(); ck = ck.ravel()
+    ncube = nx * ny * nz
+
+    def corner(b):
+        return (ci + (b & 1)) + px * ((cj + ((b >> 1) & 1)) + py * (ck + ((b >> 2) & 1)))
+
+    corners = np.stack([corner(b) for b in range(8)], axis=1).astype(np.int64)
+    tets = np.stack([corners[:, list(t)] for t in _KUHN], axis=1).reshape(-1, 4)   # cube-major
+    ncell = len(tets)
+    # right-handed (VTK) point order: positive signed volume
+    x = pts[tets]
+    det = np.einsum("ij,ij->i", np.cross(x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]), x[:, 3] - x[:, 0])
+    neg = det < 0
+    tets[neg] = tets[neg][:, [0, 2, 1, 3]]
+    # faces of every tet, matched by their sorted point triples
+    tri = np.concatenate([tets[:, [0, 1, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 3]],
+                          tets[:, [1, 2, 3]]], axis=0)
+    cell_of = np.tile(np.arange(ncell, dtype=np.int64), 4)
+    key = np.sort(tri, axis=1)
+    order = np.lexsort((key[:, 2], key[:, 1], key[:, 0]))
+    ks = key[order]
+    same = np.all(ks[1:] == ks[:-1], axis=1)
+    first = np.nonzero(same)[0]                       # pairs (first, first+1) in sorted order
+    paired = np.zeros(len(ks), dtype=bool); paired[first] = True; paired[first + 1] = True
+    a_idx = order[first]; b_idx = order[first + 1]
+    ca, cb = cell_of[a_idx], cell_of[b_idx]
+    own = np.minimum(ca, cb); nbr = np.maximum(ca, cb)
+    itri = np.where((ca <= cb)[:, None], tri[a_idx], tri[b_idx])
+    bsel = order[~paired]
+    bown = cell_of[bsel]; btri = tri[bsel]
+    cen = pts[tets].mean(axis=1)
+
+    def orient(t, owner_cells):
+        q = pts[t]
+        nrm = np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0])
+        out = np.einsum("ij,ij->i", nrm, q.mean(axis=1) - cen[owner_cells]) > 0
+        t = t.copy()
+        t[~out] = t[~out][:, [0, 2, 1]]
+        return t
+
+    itri = orient(itri, own)
+    o = np.lexsort((nbr, own))
+    own, nbr, itri = own[o], nbr[o], itri[o]
+    btri = orient(btri, bown)
+    bijk = ijk[btri]                                   # (nb, 3 points, 3 axes)
+    names = ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"]
+    lim = [0, nx, 0, ny, 0, nz]
+    bowner = []; bpts = []; starts = []; counts = []
+    nf = len(own)
+    taken = np.zeros(len(bown), dtype=bool)
+    for pi, nm in enumerate(names):
+        ax = pi // 2
+        m = np.all(bijk[:, :, ax] == lim[pi], axis=1) & ~taken
+        taken |= m
+        sel = np.nonzero(m)[0]
+        sel = sel[np.argsort(bown[sel], kind="stable")]
+        starts.append(nf); counts.append(len(sel)); nf += len(sel)
+        bowner.append(bown[sel]); bpts.append(btri[sel])
+    assert taken.all(), "boundary face off the box walls"
+    owner = np.concatenate([own] + bowner).astype(np.int32)
+    face_pts = np.concatenate([itri] + bpts, axis=0).astype(np.int32)
+    nfaces = len(owner)
+    return Mesh(points=pts, owner=owner, neighbour=nbr.astype(np.int32),
+                face_ptr=(3 * np.arange(nfaces + 1)).astype(np.int32),
+                face_pts=face_pts.ravel(),
+                cell_ptr=(4 * np.arange(ncell + 1)).astype(np.int32),
+                cell_pts=tets.astype(np.int32).ravel(),
+                patch_names=names, patch_start=np.array(starts, dtype=np.int32),
+                patch_nfaces=np.array(counts, dtype=np.int32), shape=(nx, ny, nz))

@@ -66,7 +66,8 @@ class GpuSolver:
         cs = _lib.CaseDesc((C.c_double * 2)(*case.rho), (C.c_double * 2)(*case.mu), case.sigma,
                            (C.c_double * 3)(*case.g), case.dt, case.urf_mom, case.urf_p,
                            case.urf_pu, case.limit, getattr(case, "urf_vf", 1.0),
-                           getattr(case, "n_vof", 0))
+                           getattr(case, "n_vof", 0), 1 if getattr(case, "adjust_dt", False) else 0,
+                           getattr(case, "max_co", 1.0), getattr(case, "max_dt", 1.0))
         bcs = (_lib.PatchBC * len(m.patch_names))()
         for i, nm in enumerate(m.patch_names):
             b = case.bcs[nm]
@@ -133,6 +134,13 @@ class GpuSolver:
         self._ok(self._L.semo_calc_volfrac_eq(self._h, C.byref(n), C.byref(st)))
         self.last_stats = st
         return n.value
+
+    def calcTimeStep(self, adjust: bool = True):
+        """calcTimeStepFromCorantNumber (adjust) / calcCorantNumberForPrint:
+        returns (time step in use, Courant number)."""
+        dt = C.c_double(); co = C.c_double()
+        self._ok(self._L.semo_calc_time_step(self._h, 1 if adjust else 0, C.byref(dt), C.byref(co)))
+        return dt.value, co.value
 
     def incompressiblePressureBased(self, n_correctors: int = 1):
         norms = np.zeros(2 * n_correctors)

@@ -23,12 +23,12 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from semo_mesh_amd.cases import cavity_case, dam_break_case  # noqa: E402
-from semo_mesh_amd.mesh import cavity_mesh, hex_mesh  # noqa: E402
+from semo_mesh_amd.mesh import cavity_mesh, hex_mesh, tet_mesh  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
 
 
-VOF_ONLY = ("_faceLVF", "_faceRVF", "_Qm3", "_oldVF")
+VOF_ONLY = ("_faceLVF", "_faceRVF", "_Qm3", "_oldVF", "_dt")
 
 
 def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
@@ -66,7 +66,9 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
         case_mu=np.array(case.mu), case_rho=np.array(case.rho), case_dt=np.array(case.dt),
         case_lid=np.array(case.bcs["zmax"].u_value), steps=np.array(steps),
         correctors=np.array(correctors), case_g=np.array(case.g),
-        case_n_vof=np.array(case.n_vof), case_urf_vf=np.array(case.urf_vf)))
+        case_n_vof=np.array(case.n_vof), case_urf_vf=np.array(case.urf_vf),
+        case_adjust_dt=np.array(int(case.adjust_dt)), case_max_co=np.array(case.max_co),
+        case_max_dt=np.array(case.max_dt)))
     if init is not None:
         arrs["init"] = np.asarray(init)
     path = os.path.join(HERE, name + ".npz")
@@ -129,8 +131,18 @@ def _vof_box654():
     run_case("vof_box654_2x2", m, c, 2, 2, vof_init(m, 4))
 
 
+def _vof_tet():
+    # the same two-phase physics on a jittered Kuhn-split tetrahedral box
+    # with the Courant-number time step (timestep.cpp:260-330)
+    m = tet_mesh(4, 3, 3, hi=(1.0, 0.75, 0.75), jitter=0.1, seed=8)
+    c = dam_break_case(dt=1e-3, n_vof=2, urf_vf=1.0)
+    c.adjust_dt, c.max_co, c.max_dt = True, 0.3, 1.0
+    run_case("vof_tet433_2x1", m, c, 2, 1, vof_init(m, 6))
+
+
 CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
-         "cavity8_plain": _cavity8, "orifice_lid": _orifice, "vof_box654_2x2": _vof_box654}
+         "cavity8_plain": _cavity8, "orifice_lid": _orifice, "vof_box654_2x2": _vof_box654,
+         "vof_tet433_2x1": _vof_tet}
 
 
 def main(names):

@@ -58,6 +58,44 @@ def test_library_rcm_matches_oracle_on_random_graphs(oracle_lib):
         assert np.array_equal(p1, p0) and np.array_equal(i1, i0), trial
 
 
+def _random_graph(rng, n, m):
+    e = rng.integers(0, n, size=(m, 2))
+    e = e[e[:, 0] != e[:, 1]]
+    nb = [set() for _ in range(n)]
+    for a, b in e:
+        nb[a].add(int(b)); nb[b].add(int(a))
+    order = [list(rng.permutation(sorted(x))) for x in nb]          # arbitrary adjacency order
+    ptr = np.zeros(n + 1, dtype=np.int32); ptr[1:] = np.cumsum([len(x) for x in order])
+    adj = np.array([y for x in order for y in x], dtype=np.int32)
+    return ptr, adj
+
+
+REF_RCM = os.path.join(ROOT, "oracle", "_ref", "rcm_ref")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_RCM), reason="reference genrcm harness not built (oracle/ref.mk)")
+def test_rcm_matches_reference_genrcm_on_random_graphs(oracle_lib, tmp_path):
+    """The reference's own genrcm (RCM.cpp, compiled in place) on graphs with
+    several components, isolated nodes and degree ties: the oracle and the
+    library give its permutation exactly."""
+    _lib_or_skip()
+    import subprocess
+    from semo_mesh_amd.solver import rcm
+    rng = np.random.default_rng(11)
+    for trial in range(60):
+        n = int(rng.integers(1, 400))
+        ptr, adj = _random_graph(rng, n, int(rng.integers(0, 4 * n)))
+        g = tmp_path / "g.bin"; q = tmp_path / "p.bin"
+        with open(g, "wb") as fh:
+            np.array([n, len(adj)], dtype=np.int32).tofile(fh); ptr.tofile(fh); adj.tofile(fh)
+        subprocess.run([REF_RCM, str(g), str(q)], check=True, timeout=60)
+        pref = np.fromfile(q, dtype=np.int32)
+        p0, _ = oracle_lib.genrcm(ptr, adj)
+        p1, _ = rcm(ptr, adj)
+        assert np.array_equal(p0, pref), trial
+        assert np.array_equal(p1, pref), trial
+
+
 def test_rcm_rejects_bad_graph():
     _lib = _lib_or_skip()
     from semo_mesh_amd.solver import rcm

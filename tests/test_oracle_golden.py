@@ -68,6 +68,13 @@ def test_assembly_replay_bit_exact(name, oracle_lib):
     for i, p in enumerate(cl):
         p = p + "_"
         _set_state(o, d, p)
+        if p + "dt" in d:
+            if case.adjust_dt and (i == 0 or "pressure" in cl[i - 1]):
+                # first call of a time step: the step's time step comes from
+                # these fields (timestep.cpp:260-330)
+                o.time_step(True)
+                assert o.dt() == d[p + "dt"][0], p + "dt"
+            o.set_dt(d[p + "dt"][0])
         o.recon()
         o.gravity()
         if any(case.g):
