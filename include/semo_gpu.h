@@ -145,6 +145,8 @@ typedef enum {
     SEMO_F_MAXW, SEMO_F_MINW,
     SEMO_F_ALPHA_P,                                         /* Darcy porosity   */
     SEMO_F_QM3, SEMO_F_OLDVF,                               /* alpha time levels */
+    SEMO_F_KAPPA, SEMO_F_SRCS0, SEMO_F_SRCS1, SEMO_F_SRCS2,   /* curvature, sigma kappa grad(alpha) */
+    SEMO_F_LS,                                              /* reinitialised level set (read) */
     SEMO_F_GRADP = 32, SEMO_F_GRADU, SEMO_F_GRADV, SEMO_F_GRADW, SEMO_F_GRADVF, /* [n*3] */
     SEMO_F_X0 = 48, SEMO_F_X1, SEMO_F_X2,                  /* last solution     */
     SEMO_F_B0 = 56, SEMO_F_B1, SEMO_F_B2,                  /* last right sides  */
@@ -173,6 +175,13 @@ int semo_calc_pressure_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
  * eqVolfrac.cpp:9-787), calcIncomCellEOSVF + calcCellTransport; *norm =
  * sqrt(sum dalpha^2) as calcVolfracEq returns.                                 */
 int semo_calc_volfrac_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
+/* calcSourceGravity (src/solvers/source/gravity.cpp:7-75) from the current
+ * fields, after the boundary face values are reconstructed.                   */
+int semo_calc_source_gravity(semo_ctx* ctx);
+/* calcSourceSurfaceTension (src/solvers/source/surfaceTension.cpp:7-229) with
+ * calcCurvature (src/solvers/source/curvature.cpp:7-3537): kappa and the
+ * source sigma kappa grad(alpha) from the current fields (zero for sigma = 0). */
+int semo_calc_source_surface_tension(semo_ctx* ctx);
 /* calcTimeStepFromCorantNumber (adjust != 0, src/solvers/timestep.cpp:260-330)
  * or calcCorantNumberForPrint (adjust == 0, timestep.cpp:202-253): *dt gets
  * the time step in use afterwards (adjusted: maxCo times the smallest

@@ -50,7 +50,7 @@ def lib():
         L.orc_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
         L.orc_set.argtypes = [C.c_void_p, C.c_char_p, _D]
         for nm in ("orc_eos", "orc_recon", "orc_gravity", "orc_assemble_momentum",
-                   "orc_assemble_pressure"):
+                   "orc_assemble_pressure", "orc_surface_tension", "orc_curvature"):
             getattr(L, nm).argtypes = [C.c_void_p]
         L.orc_lsq.argtypes = [C.c_void_p, _D, _D, _D]
         L.orc_minmax.argtypes = [C.c_void_p, _D, _D, _D]
@@ -128,6 +128,8 @@ class Oracle:
     def eos(self): lib().orc_eos(self._h)
     def recon(self): lib().orc_recon(self._h)
     def gravity(self): lib().orc_gravity(self._h)
+    def surface_tension(self): lib().orc_surface_tension(self._h)
+    def curvature(self): lib().orc_curvature(self._h)
     def assemble_momentum(self): lib().orc_assemble_momentum(self._h)
     def assemble_pressure(self): lib().orc_assemble_pressure(self._h)
 

@@ -65,7 +65,7 @@ struct Oracle {
     // ---- fields (cell) -------------------------------------------------------
     vector<double> P, U, V, W, VF, MF, Rho, Mu;
     vector<double> oldP, oldU, oldV, oldW, oldVF, oldRho, Qm0, Qm1, Qm2, Qm3;
-    vector<double> srcG[4], srcS[4], kappa;
+    vector<double> srcG[4], srcS[4], kappa, LS;
     vector<double> gP, gU, gV, gW, gVF;    // [3n]
     vector<double> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
     // face left/right values (src/solvers/reconIncom.cpp:501-1035)
@@ -773,6 +773,216 @@ void mstacs_faces(Oracle& o) {
     }
 }
 
+// calcLeastSquare(...,"input", cn = -1, ...): boundary faces contribute a zero
+// difference (gradient.cpp:1018-1022)
+void lsq_grad_input(const Oracle& o, const vector<double>& phi, vector<double>& grad) {
+    vector<double> fr(o.nf, 0.0);
+    for (int f = o.nif; f < o.nf; ++f) fr[f] = phi[o.owner[f]];
+    lsq_grad(o, phi, fr, grad);
+}
+
+// Inverse of the symmetric 3x3 matrix in packed form (a00,a01,a02,a11,a12,a22)
+// by cofactors, with the reference's determinant expression (curvature.cpp:
+// 2964-2979).  The level-set stage of the reference forms the same (positive
+// definite) matrices' pseudo-inverse with an SVD (math.svdcmp); for a
+// positive definite matrix that is the inverse, equal to round-off.
+void inv_sym3(const double* v, double* out) {
+    double detA = v[0] * v[3] * v[5] + v[1] * v[4] * v[2] + v[2] * v[1] * v[4] - v[0] * v[4] * v[4] -
+                  v[2] * v[3] * v[2] - v[1] * v[1] * v[5];
+    out[0] = (v[3] * v[5] - v[4] * v[4]) / detA;
+    out[1] = (v[2] * v[4] - v[1] * v[5]) / detA;
+    out[2] = (v[1] * v[4] - v[2] * v[3]) / detA;
+    out[3] = (v[0] * v[5] - v[2] * v[2]) / detA;
+    out[4] = (v[2] * v[1] - v[0] * v[4]) / detA;
+    out[5] = (v[0] * v[3] - v[1] * v[1]) / detA;
+}
+
+// calcCurvature (source/curvature.cpp:7-3537) as live: alpha - 1/2 is
+// reinitialised towards a signed distance (4 pseudo-time steps, CFL 0.1) with
+// a sign-weighted face least squares, then kappa = -div(grad LS / |grad LS|)
+// with an unweighted stencil least squares; both smoothing passes have zero
+// iterations in the reference (curvature.cpp:18-19).
+void curvature(Oracle& o) {
+    int nc = o.nc, nf = o.nf, nif = o.nif;
+    const int LSReinitIterMax = 4;
+    const double CFL_LS = 0.1, epsilon2 = 1.e-5;
+    vector<double> a = o.VF;
+    // interface cells: alpha - 1/2 changes sign between a cell and one of its
+    // face values (curvature.cpp:40-117)
+    vector<char> iface(nc, 0);
+    for (int f = 0; f < nf; ++f) {
+        int l = o.owner[f];
+        double wCL = o.fwC[f], wCR = 1.0 - wCL;
+        double fa = f < nif ? wCL * a[l] + wCR * a[o.nbr[f]] : a[l];
+        if ((a[l] - 0.5) * (fa - 0.5) <= 0.0) iface[l] = 1;
+        if (f < nif && (a[o.nbr[f]] - 0.5) * (fa - 0.5) <= 0.0) iface[o.nbr[f]] = 1;
+    }
+    for (int i = 0; i < nc; ++i) a[i] = a[i] - 0.5;
+    vector<double> g;
+    lsq_grad_input(o, a, g);
+    for (int i = 0; i < nc; ++i) {
+        if (!iface[i]) continue;
+        double m = 0.0;
+        m += pow(g[3 * i], 2.0); m += pow(g[3 * i + 1], 2.0); m += pow(g[3 * i + 2], 2.0);
+        if (m > 1.0) a[i] = a[i] / m;
+    }
+    vector<double> tau(nc), vel0(nc);
+    for (int i = 0; i < nc; ++i) {                     // curvature.cpp:140-177
+        double maxA = 0.0;
+        for (int k = o.cfptr[i]; k < o.cfptr[i + 1]; ++k) maxA = std::max(maxA, o.farea[o.cfidx[k]]);
+        tau[i] = o.cvol[i] / maxA / 1.0;
+        tau[i] = std::min(tau[i], pow(o.cvol[i], 0.3));
+        double v = a[i];
+        vel0[i] = v / sqrt(pow(v, 2.0) + pow(std::abs(v) * pow(o.cvol[i], 0.333), 2.0));
+    }
+    vector<double> nrm(3 * (size_t)nc), M(6 * (size_t)nc), rhs(3 * (size_t)nc);
+    for (int iter = 0; iter < LSReinitIterMax; ++iter) {   // curvature.cpp:203-722
+        lsq_grad_input(o, a, g);
+        for (int i = 0; i < nc; ++i) {
+            double mg = sqrt(pow(g[3 * i], 2.0) + pow(g[3 * i + 1], 2.0) + pow(g[3 * i + 2], 2.0));
+            for (int k = 0; k < 3; ++k) nrm[3 * i + k] = mg != 0.0 ? g[3 * i + k] / mg : 0.0;
+        }
+        std::fill(M.begin(), M.end(), 0.0);
+        std::fill(rhs.begin(), rhs.end(), 0.0);
+        for (int f = 0; f < nf; ++f) {
+            const double* d = &o.fdist[3 * (size_t)f];
+            int l = o.owner[f];
+            double aL = a[l], aR = f < nif ? a[o.nbr[f]] : aL;
+            double DVar = aR - aL;
+            for (int side = 0; side < (f < nif ? 2 : 1); ++side) {
+                int c = side == 0 ? l : o.nbr[f];
+                double sgn = vel0[c] > 0.0 ? 1.0 : -1.0;
+                double wk = 0.0;
+                if (side == 0) {
+                    wk += sgn * nrm[3 * c] * (-d[0]); wk += sgn * nrm[3 * c + 1] * (-d[1]);
+                    wk += sgn * nrm[3 * c + 2] * (-d[2]);
+                } else {
+                    wk += sgn * nrm[3 * c] * (d[0]); wk += sgn * nrm[3 * c + 1] * (d[1]);
+                    wk += sgn * nrm[3 * c + 2] * (d[2]);
+                }
+                wk = pow(1.0 * wk, 1.0);
+                wk = std::max(epsilon2, wk);
+                double* m = &M[6 * (size_t)c];
+                for (int ii = 0, num = 0; ii < 3; ++ii)
+                    for (int jj = ii; jj < 3; ++jj) m[num++] += wk * d[ii] * d[jj];
+                for (int ii = 0; ii < 3; ++ii) rhs[3 * (size_t)c + ii] += wk * d[ii] * DVar;
+            }
+        }
+        for (int i = 0; i < nc; ++i) {
+            double inv[6];
+            inv_sym3(&M[6 * (size_t)i], inv);
+            const double* t = &rhs[3 * (size_t)i];
+            double g0 = inv[0] * t[0] + inv[1] * t[1] + inv[2] * t[2];
+            double g1 = inv[1] * t[0] + inv[3] * t[1] + inv[4] * t[2];
+            double g2 = inv[2] * t[0] + inv[4] * t[1] + inv[5] * t[2];
+            if (!iface[i]) {
+                double mg = sqrt(pow(g0, 2.0) + pow(g1, 2.0) + pow(g2, 2.0));
+                double resi = CFL_LS * tau[i] * vel0[i] * (1.0 - mg);
+                a[i] += resi;
+            }
+        }
+    }
+    o.LS = a;
+    // normals and their divergence (curvature.cpp:725-2434)
+    lsq_grad_input(o, a, g);
+    for (int i = 0; i < nc; ++i) {
+        double mg = 0.0;
+        for (int k = 0; k < 3; ++k) mg += g[3 * i + k] * g[3 * i + k];
+        mg = sqrt(mg);
+        for (int k = 0; k < 3; ++k) nrm[3 * i + k] = mg > 1.e-15 ? g[3 * i + k] / mg : 0.0;
+    }
+    std::fill(M.begin(), M.end(), 0.0);
+    std::fill(rhs.begin(), rhs.end(), 0.0);
+    for (int i = 0; i < nc; ++i)
+        for (int k = o.sptr[i]; k < o.sptr[i + 1]; ++k) {
+            int j = o.sidx[k];
+            double d[3] = {o.cxyz[3 * j] - o.cxyz[3 * i], o.cxyz[3 * j + 1] - o.cxyz[3 * i + 1],
+                           o.cxyz[3 * j + 2] - o.cxyz[3 * i + 2]};
+            double* m = &M[6 * (size_t)i];
+            for (int ii = 0, num = 0; ii < 3; ++ii)
+                for (int jj = ii; jj < 3; ++jj) m[num++] += 1.0 * d[ii] * d[jj];
+        }
+    for (int f = nif; f < nf; ++f) {
+        int i = o.owner[f];
+        double d[3] = {o.fxyz[3 * f] - o.cxyz[3 * i], o.fxyz[3 * f + 1] - o.cxyz[3 * i + 1],
+                       o.fxyz[3 * f + 2] - o.cxyz[3 * i + 2]};
+        double* m = &M[6 * (size_t)i];
+        for (int ii = 0, num = 0; ii < 3; ++ii)
+            for (int jj = ii; jj < 3; ++jj) m[num++] += 1.0 * d[ii] * d[jj];
+    }
+    for (int i = 0; i < nc; ++i)
+        for (int k = o.sptr[i]; k < o.sptr[i + 1]; ++k) {
+            int j = o.sidx[k];
+            for (int ii = 0; ii < 3; ++ii) {
+                double d = o.cxyz[3 * j + ii] - o.cxyz[3 * i + ii];
+                double dv = nrm[3 * j + ii] - nrm[3 * i + ii];
+                rhs[3 * (size_t)i + ii] += 1.0 * d * dv;
+            }
+        }
+    o.kappa.assign(nc, 0.0);
+    for (int i = 0; i < nc; ++i) {
+        double inv[6];
+        inv_sym3(&M[6 * (size_t)i], inv);
+        const double* t = &rhs[3 * (size_t)i];
+        double g0 = inv[0] * t[0] + inv[1] * t[1] + inv[2] * t[2];
+        double g1 = inv[1] * t[0] + inv[3] * t[1] + inv[4] * t[2];
+        double g2 = inv[2] * t[0] + inv[4] * t[1] + inv[5] * t[2];
+        o.kappa[i] = -(g0 + g1 + g2);
+    }
+}
+
+// calcGaussGreen (gradient.cpp:1778-1985) of VF with the previous gradient
+// `grad` as skew correction, boundary values by the VF boundary type
+void gauss_green_vf(const Oracle& o, vector<double>& grad) {
+    int nc = o.nc;
+    vector<double> gt(3 * (size_t)nc, 0.0);
+    for (int f = 0; f < o.nif; ++f) {
+        int l = o.owner[f], r = o.nbr[f];
+        double wCL = o.fwC[f], wCR = 1.0 - wCL;
+        const double* sk = &o.fskew[3 * (size_t)f];
+        double v = wCL * o.VF[l] + wCR * o.VF[r];
+        for (int a = 0; a < 3; ++a) v += wCL * grad[3 * l + a] * sk[a];
+        for (int a = 0; a < 3; ++a) v += wCR * grad[3 * r + a] * sk[a];
+        for (int j = 0; j < 3; ++j) {
+            gt[3 * l + j] += v * o.fnrm[3 * f + j] * o.farea[f] / o.cvol[l];
+            gt[3 * r + j] -= v * o.fnrm[3 * f + j] * o.farea[f] / o.cvol[r];
+        }
+    }
+    for (const Patch& pb : o.patches)
+        for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
+            int c = o.owner[f];
+            double v = o.VF[c];
+            if (pb.vf_type == FIXED_VALUE) v = pb.vf_val;
+            else if (pb.vf_type == ZERO_GRADIENT) {
+                v += grad[3 * c] * (o.fxyz[3 * f] - o.cxyz[3 * c]);
+                v += grad[3 * c + 1] * (o.fxyz[3 * f + 1] - o.cxyz[3 * c + 1]);
+                v += grad[3 * c + 2] * (o.fxyz[3 * f + 2] - o.cxyz[3 * c + 2]);
+            } else v = 0.5 * (o.VF[c] + o.RVF[f]);
+            for (int j = 0; j < 3; ++j) gt[3 * c + j] += v * o.fnrm[3 * f + j] * o.farea[f] / o.cvol[c];
+        }
+    grad = gt;
+}
+
+// calcSourceSurfaceTension (source/surfaceTension.cpp:7-229): curvature, then
+// sigma kappa grad(alpha) with grad(alpha) = one Green-Gauss pass over the LSQ
+// gradient.  With sigma = 0 every use of kappa vanishes and the pipeline is
+// skipped (sources zero).
+void source_surface_tension(Oracle& o) {
+    for (int k = 0; k < 4; ++k) o.srcS[k].assign(o.nc, 0.0);
+    if (o.sigma == 0.0) { o.kappa.assign(o.nc, 0.0); return; }
+    curvature(o);
+    vector<double> g;
+    lsq_grad(o, o.VF, o.RVF, g);
+    gauss_green_vf(o, g);
+    for (int i = 0; i < o.nc; ++i) {
+        double s = o.sigma, kp = o.kappa[i];
+        o.srcS[0][i] = (s * kp * g[3 * i]);
+        o.srcS[1][i] = (s * kp * g[3 * i + 1]);
+        o.srcS[2][i] = (s * kp * g[3 * i + 2]);
+        o.srcS[3][i] = s * kp * (g[3 * i] * o.U[i] + g[3 * i + 1] * o.V[i] + g[3 * i + 2] * o.W[i]);
+    }
+}
+
 // calcVolfracEq assembly (eqVolfrac.cpp:9-765)
 void assemble_vof(Oracle& o) {
     int nc = o.nc;
@@ -1080,8 +1290,9 @@ double time_step(Oracle& o, int adjust) {
     return co;
 }
 
-// one call of incompressiblePressureBased with nSpeciesEq = 0
-// (pressureBased.cpp:6-413); sigma must be 0 (no curvature in this oracle).
+// one call of incompressiblePressureBased (pressureBased.cpp:6-413): time
+// levels, time step, n_vof VOF iterations, body-force sources, then ncorr
+// momentum + pressure iterations.
 void pressure_based_step(Oracle& o, int ncorr, double* norms) {
     if (o.iterReal == 0) cell_eos_transport(o);
     if (o.iterReal == 0) { o.Qm0 = o.U; o.Qm1 = o.V; o.Qm2 = o.W; o.Qm3 = o.VF; }
@@ -1098,8 +1309,7 @@ void pressure_based_step(Oracle& o, int ncorr, double* norms) {
     recon_faces(o);
     minmax(o, o.VF, o.maxVF, o.minVF);
     source_gravity(o);
-    for (int k = 0; k < 4; ++k) o.srcS[k].assign(o.nc, 0.0);
-    o.kappa.assign(o.nc, 0.0);
+    source_surface_tension(o);
     for (int it = 0; it < ncorr; ++it) {
         recon_faces(o);
         assemble_momentum(o);
@@ -1298,7 +1508,8 @@ static vector<double>* field(Oracle& o, const char* nm) {
     F("b0", B0) F("b1", B1) F("b2", B2) F("x0", X0) F("x1", X1) F("x2", X2) F("vals", Avals)
     F("faceRP", RP) F("faceRU", RU) F("faceRV", RV) F("faceRW", RW) F("faceRVF", RVF)
     F("faceLU", LU) F("faceLV", LV) F("faceLW", LW) F("faceLRho", LRho) F("faceLVF", LVF)
-    F("oldVF", oldVF) F("Qm3", Qm3) F("courant", co_cell)
+    F("oldVF", oldVF) F("Qm3", Qm3) F("courant", co_cell) F("kappa", kappa) F("LS", LS)
+    F("srcS0", srcS[0]) F("srcS1", srcS[1]) F("srcS2", srcS[2]) F("srcS3", srcS[3])
 #undef F
     return nullptr;
 }
@@ -1339,6 +1550,8 @@ int orc_set(void* h, const char* nm, const double* in) {
 void orc_eos(void* h) { cell_eos_transport(*(Oracle*)h); }
 void orc_recon(void* h) { recon_faces(*(Oracle*)h); }
 void orc_gravity(void* h) { source_gravity(*(Oracle*)h); }
+void orc_surface_tension(void* h) { source_surface_tension(*(Oracle*)h); }
+void orc_curvature(void* h) { curvature(*(Oracle*)h); }
 void orc_lsq(void* h, const double* phi, const double* faceR, double* grad) {
     Oracle& o = *(Oracle*)h;
     vector<double> p(phi, phi + o.nc), fr(faceR, faceR + o.nf), gr;

@@ -294,7 +294,9 @@ __global__ void __launch_bounds__(TPB) k_lsq1(
     int nc, int nif, const int* __restrict__ st_ptr, const int* __restrict__ st_idx,
     const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx, const double* __restrict__ cxyz,
     const double* __restrict__ fxyz, const double* __restrict__ lsq, const double* __restrict__ phi,
-    const double* __restrict__ faceR, double* __restrict__ g) {
+    const double* __restrict__ faceR, int foff, double* __restrict__ g) {
+    // faceR: boundary values indexed by face - foff; null: "input" mode with
+    // cn = -1, a zero boundary difference (gradient.cpp:1018-1022)
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double xi = cxyz[3 * (size_t)i], yi = cxyz[3 * (size_t)i + 1], zi = cxyz[3 * (size_t)i + 2];
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__(TPB) k_lsq1(
         if (f < nif) continue;
         double dx = fxyz[3 * (size_t)f] - xi, dy = fxyz[3 * (size_t)f + 1] - yi, dz = fxyz[3 * (size_t)f + 2] - zi;
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
-        double d = faceR[f] - p;
+        double d = faceR ? faceR[f - foff] - p : 0.0;
         t0 += wk * dx * d; t1 += wk * dy * d; t2 += wk * dz * d;
     }
     const double* C = lsq + 6 * (size_t)i;
@@ -853,6 +855,220 @@ __global__ void __launch_bounds__(TPB) k_vof_update(int nc, double urf, const do
     block_sum2(s2, 0.0, part);
 }
 
+// ------------------------------------------------------------- curvature --
+// calcCurvature (source/curvature.cpp:7-3537), live part: level-set
+// reinitialisation of alpha - 1/2, then kappa = -div(n).
+
+// inverse of a packed symmetric 3x3 matrix by cofactors, with the reference's
+// determinant expression (curvature.cpp:2964-2979)
+__device__ inline void inv_sym3(const double* v, double* o) {
+    double detA = v[0] * v[3] * v[5] + v[1] * v[4] * v[2] + v[2] * v[1] * v[4] - v[0] * v[4] * v[4] -
+                  v[2] * v[3] * v[2] - v[1] * v[1] * v[5];
+    o[0] = (v[3] * v[5] - v[4] * v[4]) / detA;
+    o[1] = (v[2] * v[4] - v[1] * v[5]) / detA;
+    o[2] = (v[1] * v[4] - v[2] * v[3]) / detA;
+    o[3] = (v[0] * v[5] - v[2] * v[2]) / detA;
+    o[4] = (v[2] * v[1] - v[0] * v[4]) / detA;
+    o[5] = (v[0] * v[3] - v[1] * v[1]) / detA;
+}
+
+// static: inverse of the unweighted least-squares matrix over the vertex
+// stencil and the cell's boundary faces (curvature.cpp:2742-2981)
+__global__ void __launch_bounds__(TPB) k_kappa_lsq_init(
+    int nc, const int* __restrict__ st_ptr, const int* __restrict__ st_idx, const int* __restrict__ bf_ptr,
+    const int* __restrict__ bf_idx, const double* __restrict__ cxyz, const double* __restrict__ fxyz,
+    double* __restrict__ out) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double m[6] = {0, 0, 0, 0, 0, 0};
+    const double* ci = cxyz + 3 * (size_t)i;
+    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) {
+        const double* cj = cxyz + 3 * (size_t)st_idx[q];
+        double d[3] = {cj[0] - ci[0], cj[1] - ci[1], cj[2] - ci[2]};
+        for (int a = 0, num = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b) m[num++] += 1.0 * d[a] * d[b];
+    }
+    for (int q = bf_ptr[i]; q < bf_ptr[i + 1]; ++q) {
+        const double* fc = fxyz + 3 * (size_t)bf_idx[q];
+        double d[3] = {fc[0] - ci[0], fc[1] - ci[1], fc[2] - ci[2]};
+        for (int a = 0, num = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b) m[num++] += 1.0 * d[a] * d[b];
+    }
+    double inv[6];
+    inv_sym3(m, inv);
+    for (int k = 0; k < 6; ++k) out[6 * (size_t)i + k] = inv[k];
+}
+
+// interface cells (alpha - 1/2 changes sign between the cell and one of its
+// face values, curvature.cpp:40-117); ls = alpha - 1/2
+__global__ void __launch_bounds__(TPB) k_curv_iface(
+    int nc, int nif, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ owner, const int* __restrict__ nbr, const double* __restrict__ wCv,
+    const double* __restrict__ VF, int8_t* __restrict__ iface, double* __restrict__ ls) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double a = VF[i];
+    int flag = 0;
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
+        (void)own;
+        double fa = a;
+        if (f < nif) { double w = wCv[f]; fa = w * VF[owner[f]] + (1.0 - w) * VF[nbr[f]]; }
+        if ((a - 0.5) * (fa - 0.5) <= 0.0) flag = 1;
+    }
+    iface[i] = (int8_t)flag;
+    ls[i] = a - 0.5;
+}
+
+// interface cells with |grad ls|^2 > 1 are rescaled; pseudo-time step and
+// sign velocity of the reinitialisation (curvature.cpp:119-177)
+__global__ void __launch_bounds__(TPB) k_curv_prep(
+    int nc, const int8_t* __restrict__ iface, const double* __restrict__ g, const double* __restrict__ vol,
+    const double* __restrict__ maxA, double* __restrict__ ls, double* __restrict__ tau,
+    double* __restrict__ vel0) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double a = ls[i];
+    if (iface[i]) {
+        const double* gi = g + 3 * (size_t)i;
+        double m = 0.0;
+        m += gi[0] * gi[0]; m += gi[1] * gi[1]; m += gi[2] * gi[2];
+        if (m > 1.0) { a = a / m; ls[i] = a; }
+    }
+    double v = vol[i];
+    double t = v / maxA[i] / 1.0;
+    tau[i] = smin(t, pow(v, 0.3));
+    double q = fabs(a) * pow(v, 0.333);
+    vel0[i] = a / sqrt(a * a + q * q);
+}
+
+// unit normal grad/|grad|; zero where |grad| is 0 (mode 0, curvature.cpp:
+// 309-319) or not above 1e-15 (mode 1, curvature.cpp:738-750)
+__global__ void __launch_bounds__(TPB) k_curv_normal(int nc, int mode, const double* __restrict__ g,
+                                                     double* __restrict__ nrm) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const double* gi = g + 3 * (size_t)i;
+    double mg = sqrt(gi[0] * gi[0] + gi[1] * gi[1] + gi[2] * gi[2]);
+    bool ok = mode == 0 ? mg != 0.0 : mg > 1.e-15;
+    for (int k = 0; k < 3; ++k) nrm[3 * (size_t)i + k] = ok ? gi[k] / mg : 0.0;
+}
+
+// one reinitialisation step (curvature.cpp:203-722): the sign-weighted face
+// least squares of ls (weights max(1e-5, s n.d) from the cell's side),
+// solved per cell, then ls += CFL tau s (1 - |grad ls|) outside the
+// interface cells.  Reads ls_in, writes ls_out.
+__global__ void __launch_bounds__(TPB) k_curv_reinit(
+    int nc, int nif, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ owner, const int* __restrict__ nbr, const double* __restrict__ cxyz,
+    const double* __restrict__ fxyz, const double* __restrict__ nrm, const double* __restrict__ vel0,
+    const double* __restrict__ tau, const int8_t* __restrict__ iface, const double* __restrict__ ls_in,
+    double* __restrict__ ls_out) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const double eps2 = 1.e-5, cfl = 0.1;
+    double ai = ls_in[i];
+    double s = vel0[i] > 0.0 ? 1.0 : -1.0;
+    double n0 = nrm[3 * (size_t)i], n1 = nrm[3 * (size_t)i + 1], n2 = nrm[3 * (size_t)i + 2];
+    double m[6] = {0, 0, 0, 0, 0, 0}, r[3] = {0, 0, 0};
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
+        double d[3], DVar;
+        if (f < nif) {
+            int o = owner[f], nb = nbr[f];
+            for (int a = 0; a < 3; ++a) d[a] = cxyz[3 * (size_t)nb + a] - cxyz[3 * (size_t)o + a];
+            DVar = ls_in[nb] - ls_in[o];
+        } else {
+            for (int a = 0; a < 3; ++a) d[a] = fxyz[3 * (size_t)f + a] - cxyz[3 * (size_t)i + a];
+            DVar = ai - ai;
+        }
+        double wk = 0.0;
+        if (own) { wk += s * n0 * (-d[0]); wk += s * n1 * (-d[1]); wk += s * n2 * (-d[2]); }
+        else { wk += s * n0 * (d[0]); wk += s * n1 * (d[1]); wk += s * n2 * (d[2]); }
+        wk = smax(eps2, wk);
+        for (int a = 0, num = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b) m[num++] += wk * d[a] * d[b];
+        for (int a = 0; a < 3; ++a) r[a] += wk * d[a] * DVar;
+    }
+    double inv[6];
+    inv_sym3(m, inv);
+    double g0 = inv[0] * r[0] + inv[1] * r[1] + inv[2] * r[2];
+    double g1 = inv[1] * r[0] + inv[3] * r[1] + inv[4] * r[2];
+    double g2 = inv[2] * r[0] + inv[4] * r[1] + inv[5] * r[2];
+    if (!iface[i]) {
+        double mg = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        ai += cfl * tau[i] * vel0[i] * (1.0 - mg);
+    }
+    ls_out[i] = ai;
+}
+
+// kappa = -div(n) with the unweighted stencil least squares of each normal
+// component (curvature.cpp:2982-3072, 3074-3077)
+__global__ void __launch_bounds__(TPB) k_curv_kappa(
+    int nc, const int* __restrict__ st_ptr, const int* __restrict__ st_idx, const double* __restrict__ cxyz,
+    const double* __restrict__ nrm, const double* __restrict__ kinv, double* __restrict__ kappa) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const double* ci = cxyz + 3 * (size_t)i;
+    const double* ni = nrm + 3 * (size_t)i;
+    double r[3] = {0, 0, 0};
+    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) {
+        int j = st_idx[q];
+        for (int a = 0; a < 3; ++a) {
+            double d = cxyz[3 * (size_t)j + a] - ci[a];
+            double dv = nrm[3 * (size_t)j + a] - ni[a];
+            r[a] += 1.0 * d * dv;
+        }
+    }
+    const double* v = kinv + 6 * (size_t)i;
+    double g0 = v[0] * r[0] + v[1] * r[1] + v[2] * r[2];
+    double g1 = v[1] * r[0] + v[3] * r[1] + v[4] * r[2];
+    double g2 = v[2] * r[0] + v[4] * r[1] + v[5] * r[2];
+    kappa[i] = -(g0 + g1 + g2);
+}
+
+// calcSourceSurfaceTension (source/surfaceTension.cpp:7-229): one
+// calcGaussGreen pass of alpha (gradient.cpp:1778-1985) over its LSQ
+// gradient g, then sigma kappa grad(alpha) and sigma kappa grad(alpha).U
+__global__ void __launch_bounds__(TPB) k_st_source(
+    int nc, int nif, CaseK k, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ owner, const int* __restrict__ nbr, const int* __restrict__ bpatch,
+    const semo_patch_bc* __restrict__ bcs, const double* __restrict__ wCv, const double* __restrict__ skew,
+    const double* __restrict__ nrm, const double* __restrict__ area, const double* __restrict__ vol,
+    const double* __restrict__ fxyz, const double* __restrict__ cxyz, const double* __restrict__ VF,
+    const double* __restrict__ bVF, const double* __restrict__ g, const double* __restrict__ kappa,
+    const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
+    double* __restrict__ srcS) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double vi = vol[i];
+    double gt[3] = {0, 0, 0};
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
+        const double* n = nrm + 3 * (size_t)f;
+        double A = area[f];
+        if (f < nif) {
+            int o = owner[f], nb = nbr[f];
+            double wCL = wCv[f], wCR = 1.0 - wCL;
+            const double* sk = skew + 3 * (size_t)f;
+            double v = wCL * VF[o] + wCR * VF[nb];
+            for (int a = 0; a < 3; ++a) v += wCL * g[3 * (size_t)o + a] * sk[a];
+            for (int a = 0; a < 3; ++a) v += wCR * g[3 * (size_t)nb + a] * sk[a];
+            if (own) for (int a = 0; a < 3; ++a) gt[a] += v * n[a] * A / vi;
+            else for (int a = 0; a < 3; ++a) gt[a] -= v * n[a] * A / vi;
+            continue;
+        }
+        int j = f - nif;
+        semo_patch_bc bc = bcs[bpatch[j]];
+        double v = VF[i];
+        if (bc.vf_type == SEMO_BC_FIXED_VALUE) v = bc.vf_value;
+        else if (bc.vf_type == SEMO_BC_ZERO_GRADIENT) {
+            for (int a = 0; a < 3; ++a) v += g[3 * (size_t)i + a] * (fxyz[3 * (size_t)f + a] - cxyz[3 * (size_t)i + a]);
+        } else v = 0.5 * (VF[i] + bVF[j]);
+        for (int a = 0; a < 3; ++a) gt[a] += v * n[a] * A / vi;
+    }
+    double s = k.sigma, kp = kappa[i];
+    for (int a = 0; a < 3; ++a) srcS[a * (size_t)nc + i] = (s * kp * gt[a]);
+    srcS[3 * (size_t)nc + i] = s * kp * (gt[0] * U[i] + gt[1] * V[i] + gt[2] * W[i]);
+}
+
 // ------------------------------------------- p' gradient and correction --
 // eqPressure.cpp:686-809: two Green-Gauss passes of p' with skew correction
 __global__ void __launch_bounds__(TPB) k_gg_face(int nif, const int* __restrict__ owner,
@@ -1126,6 +1342,51 @@ void assemble_vof(semo_ctx* c) {
     c->last_eq = 2;
 }
 
+// calcSourceSurfaceTension with calcCurvature (curvature.cpp:7-3537,
+// surfaceTension.cpp:7-229).  With sigma = 0 every use of kappa vanishes, so
+// the pipeline is skipped and the sources are zero.
+void surface_tension(semo_ctx* c) {
+    if (c->cs.sigma == 0.0) { c->srcS.zero(c->stream); c->kappa.zero(c->stream); return; }
+    int n = c->nc;
+    for (DBuf<double>* b : {&c->ls, &c->ls2, &c->tau_ls, &c->vel0_ls}) b->ensure(n);
+    c->cnrm.ensure(3 * (size_t)n); c->cgrad.ensure(3 * (size_t)n); c->iface.ensure(n);
+    dim3 gb(nblk(n)), tb(TPB);
+    ProfScope ps(c, "curvature", (double)n * 8 * (6 * 40 + 60));
+    if (c->kinv.n != 6 * (size_t)n) {
+        c->kinv.alloc(6 * (size_t)n);
+        hipLaunchKernelGGL(k_kappa_lsq_init, gb, tb, 0, c->stream, n, c->d_st_ptr.p, c->d_st_idx.p, c->d_bf_ptr.p,
+                           c->d_bf_idx.p, c->c_xyz.p, c->f_xyz.p, c->kinv.p);
+    }
+    auto lsq_input = [&](const double* phi, double* g) {
+        hipLaunchKernelGGL(k_lsq1, gb, tb, 0, c->stream, n, c->nif, c->d_st_ptr.p, c->d_st_idx.p, c->d_cf_ptr.p,
+                           c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p, phi, (const double*)nullptr, 0, g);
+    };
+    hipLaunchKernelGGL(k_curv_iface, gb, tb, 0, c->stream, n, c->nif, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p,
+                       c->d_nbr.p, c->f_wC.p, c->VF.p, c->iface.p, c->ls.p);
+    lsq_input(c->ls.p, c->cgrad.p);
+    hipLaunchKernelGGL(k_curv_prep, gb, tb, 0, c->stream, n, c->iface.p, c->cgrad.p, c->c_vol.p, c->c_maxA.p,
+                       c->ls.p, c->tau_ls.p, c->vel0_ls.p);
+    for (int it = 0; it < 4; ++it) {                       // LSReinitIterMax (curvature.cpp:16)
+        lsq_input(c->ls.p, c->cgrad.p);
+        hipLaunchKernelGGL(k_curv_normal, gb, tb, 0, c->stream, n, 0, c->cgrad.p, c->cnrm.p);
+        hipLaunchKernelGGL(k_curv_reinit, gb, tb, 0, c->stream, n, c->nif, c->d_cf_sl.p, c->d_cf_ell.p,
+                           c->d_owner.p, c->d_nbr.p, c->c_xyz.p, c->f_xyz.p, c->cnrm.p, c->vel0_ls.p, c->tau_ls.p,
+                           c->iface.p, c->ls.p, c->ls2.p);
+        std::swap(c->ls, c->ls2);
+    }
+    lsq_input(c->ls.p, c->cgrad.p);
+    hipLaunchKernelGGL(k_curv_normal, gb, tb, 0, c->stream, n, 1, c->cgrad.p, c->cnrm.p);
+    hipLaunchKernelGGL(k_curv_kappa, gb, tb, 0, c->stream, n, c->d_st_ptr.p, c->d_st_idx.p, c->c_xyz.p, c->cnrm.p,
+                       c->kinv.p, c->kappa.p);
+    // grad(alpha): LSQ with the boundary values, then one Green-Gauss pass
+    hipLaunchKernelGGL(k_lsq1, gb, tb, 0, c->stream, n, c->nif, c->d_st_ptr.p, c->d_st_idx.p, c->d_cf_ptr.p,
+                       c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p, c->VF.p, c->bVF.p, c->nif, c->cgrad.p);
+    hipLaunchKernelGGL(k_st_source, gb, tb, 0, c->stream, n, c->nif, case_k(c), c->d_cf_sl.p, c->d_cf_ell.p,
+                       c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_wC.p, c->f_skew.p, c->f_nrm.p,
+                       c->f_area.p, c->c_vol.p, c->f_xyz.p, c->c_xyz.p, c->VF.p, c->bVF.p, c->cgrad.p, c->kappa.p,
+                       c->U.p, c->V.p, c->W.p, c->srcS.p);
+}
+
 // calcTimeStepFromCorantNumber / calcCorantNumberForPrint: returns the
 // Courant number; with adjust the case's time step becomes maxCo times the
 // smallest time scale, capped at maxTimeStep
@@ -1193,7 +1454,7 @@ double pressure_correct(semo_ctx* c) {
 void lsq_standalone(semo_ctx* c, const double* phi, const double* faceR, double* grad) {
     hipLaunchKernelGGL(k_lsq1, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->d_st_ptr.p,
                        c->d_st_idx.p, c->d_cf_ptr.p, c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p, phi,
-                       faceR, grad);
+                       faceR, 0, grad);
 }
 
 void minmax_standalone(semo_ctx* c, const double* phi, double* mx, double* mn) {

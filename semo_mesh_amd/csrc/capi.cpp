@@ -92,6 +92,7 @@ DBuf<double>* cell_field(semo_ctx* c, int f, int& comps) {
         case SEMO_F_MAXW: return &c->mxW; case SEMO_F_MINW: return &c->mnW;
         case SEMO_F_ALPHA_P: return &c->alphaP;
         case SEMO_F_QM3: return &c->Qm3; case SEMO_F_OLDVF: return &c->oldVF;
+        case SEMO_F_KAPPA: return &c->kappa; case SEMO_F_LS: return &c->ls;
         case SEMO_F_GRADP: comps = 3; return &c->gP; case SEMO_F_GRADU: comps = 3; return &c->gU;
         case SEMO_F_GRADV: comps = 3; return &c->gV; case SEMO_F_GRADW: comps = 3; return &c->gW;
         case SEMO_F_GRADVF: comps = 3; return &c->gVF;
@@ -135,6 +136,7 @@ void step_setup(semo_ctx* c) {
 void step_sources(semo_ctx* c) {
     recon_boundary(c);
     source_gravity(c);
+    surface_tension(c);
 }
 
 // one VOF iteration (pressureBased.cpp:104-127)
@@ -258,7 +260,7 @@ int semo_set_case(semo_ctx* c, const semo_case* cs, const semo_patch_bc* bcs) {
         need_mesh(c);
         SEMO_CHECK(cs->dt > 0 && cs->rho[0] > 0 && cs->rho[1] > 0 && cs->mu[0] > 0 && cs->mu[1] > 0,
                    SEMO_EINVAL, "case: dt, rho and mu must be positive");
-        SEMO_CHECK(cs->sigma == 0.0, SEMO_EINVAL, "surface tension (curvature) is not available on this path");
+        SEMO_CHECK(cs->sigma >= 0.0, SEMO_EINVAL, "case: sigma must be non-negative");
         SEMO_CHECK(cs->n_vof >= 0 && cs->urf_vf > 0.0, SEMO_EINVAL, "case: n_vof >= 0 and urf_vf > 0");
         SEMO_CHECK(!cs->adjust_dt || (cs->max_co > 0.0 && cs->max_dt > 0.0), SEMO_EINVAL,
                    "case: maxCo and maxTimeStep must be positive");
@@ -300,6 +302,14 @@ int semo_field_set(semo_ctx* c, int f, const double* h, int64_t n) {
         need_mesh(c);
         int comps;
         DBuf<double>* d = cell_field(c, f, comps);
+        if (f >= SEMO_F_SRCS0 && f <= SEMO_F_SRCS2) {   // one component of the surface-tension source
+            SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
+            DBuf<double> t;
+            to_dev(c, t, h, 1);
+            copy_d(c, c->srcS.p + (size_t)(f - SEMO_F_SRCS0) * c->nc, t.p, c->nc);
+            SEMO_HIP(hipStreamSynchronize(c->stream));
+            return;
+        }
         if (f >= SEMO_F_SRCG0 && f <= SEMO_F_SRCG2) {   // one component of the gravity source
             SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
             DBuf<double> t;
@@ -322,6 +332,11 @@ int semo_field_get(semo_ctx* c, int f, double* h, int64_t n) {
         if (f >= SEMO_F_SRCG0 && f <= SEMO_F_SRCG2) {
             SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
             from_dev(c, c->srcG.p + (size_t)(f - SEMO_F_SRCG0) * c->nc, h, 1);
+            return;
+        }
+        if (f >= SEMO_F_SRCS0 && f <= SEMO_F_SRCS2) {
+            SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
+            from_dev(c, c->srcS.p + (size_t)(f - SEMO_F_SRCS0) * c->nc, h, 1);
             return;
         }
         if (f >= SEMO_F_X0 && f <= SEMO_F_X2) {
@@ -404,6 +419,26 @@ int semo_calc_volfrac_eq(semo_ctx* c, double* norm, semo_solve_stats* st) {
         double nv = volfrac_eq(c, &c->last_stats_vf);
         if (st) *st = c->last_stats_vf;
         if (norm) *norm = nv;
+    });
+}
+
+int semo_calc_source_gravity(semo_ctx* c) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        recon_boundary(c);
+        source_gravity(c);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int semo_calc_source_surface_tension(semo_ctx* c) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        recon_boundary(c);
+        surface_tension(c);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
     });
 }
 

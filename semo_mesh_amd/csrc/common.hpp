@@ -219,6 +219,10 @@ struct semo_ctx {
     semo::DBuf<double> Qm3;                    // alpha two time levels back (controls.Qm[3])
     semo::DBuf<double> c_maxA;                 // largest face area per cell (static)
     semo::DBuf<double> fVFL, fVFR;             // MSTACS face volume fractions [nif]
+    // curvature: level set (2 buffers), pseudo-time step, sign velocity,
+    // normals, gradient scratch, static normal-divergence LSQ inverse
+    semo::DBuf<double> ls, ls2, tau_ls, vel0_ls, cnrm, cgrad, kinv;
+    semo::DBuf<int8_t> iface;
     semo::DBuf<double> gP, gU, gV, gW, gVF;   // [3n] interleaved
     semo::DBuf<double> mxP, mnP, mxU, mnU, mxV, mnV, mxW, mnW;
     semo::DBuf<double> bP, bUL, bUR, bVF, bMu, bRho;   // boundary faces (bUL/bUR: [3][nbf])
@@ -255,6 +259,7 @@ void assemble_momentum(semo_ctx* c);
 void assemble_pressure(semo_ctx* c);
 void assemble_vof(semo_ctx* c);
 double time_step(semo_ctx* c, int adjust);
+void surface_tension(semo_ctx* c);
 double vof_update(semo_ctx* c);
 double momentum_update(semo_ctx* c);
 double pressure_correct(semo_ctx* c);

@@ -135,6 +135,14 @@ class GpuSolver:
         self.last_stats = st
         return n.value
 
+    def calcSourceGravity(self):
+        self._ok(self._L.semo_calc_source_gravity(self._h))
+
+    def calcSourceSurfaceTension(self):
+        """Curvature and sigma kappa grad(alpha) from the current fields."""
+        self._ok(self._L.semo_calc_source_surface_tension(self._h))
+        return 0.0
+
     def calcTimeStep(self, adjust: bool = True):
         """calcTimeStepFromCorantNumber (adjust) / calcCorantNumberForPrint:
         returns (time step in use, Courant number)."""

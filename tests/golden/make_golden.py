@@ -52,7 +52,9 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
                 continue
             if base.startswith("call") and not keep_calls:
                 continue
-            if base.endswith(("_gradVF", "_kappa")) or "_srcS" in base:
+            if base.endswith("_gradVF"):
+                continue
+            if (base.endswith("_kappa") or "_srcS" in base) and case.sigma == 0.0:
                 continue
             if base.endswith(VOF_ONLY) and not case.n_vof:
                 continue
@@ -68,7 +70,7 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
         correctors=np.array(correctors), case_g=np.array(case.g),
         case_n_vof=np.array(case.n_vof), case_urf_vf=np.array(case.urf_vf),
         case_adjust_dt=np.array(int(case.adjust_dt)), case_max_co=np.array(case.max_co),
-        case_max_dt=np.array(case.max_dt)))
+        case_max_dt=np.array(case.max_dt), case_sigma=np.array(case.sigma)))
     if init is not None:
         arrs["init"] = np.asarray(init)
     path = os.path.join(HERE, name + ".npz")
@@ -140,9 +142,22 @@ def _vof_tet():
     run_case("vof_tet433_2x1", m, c, 2, 1, vof_init(m, 6))
 
 
+def _sigma_box():
+    # surface tension: curvature of a jittered-box droplet, sigma = 0.07
+    m = hex_mesh(7, 6, 6, jitter=0.1, seed=31, ref_face=False)
+    c = dam_break_case(dt=1e-3, n_vof=1, urf_vf=1.0)
+    c.sigma = 0.07
+    nc = m.n_cells
+    cen = np.stack([m.points[m.cell_pts[m.cell_ptr[i]:m.cell_ptr[i + 1]]].mean(0) for i in range(nc)])
+    r = np.linalg.norm(cen - np.array([0.5, 0.5, 0.45]), axis=1)
+    init = vof_init(m, 12)
+    init[4 * nc:] = np.clip((0.32 - r) / 0.15 + 0.5, 0.0, 1.0)     # smeared droplet
+    run_case("sigma_box766_2x1", m, c, 2, 1, init)
+
+
 CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "cavity8_plain": _cavity8, "orifice_lid": _orifice, "vof_box654_2x2": _vof_box654,
-         "vof_tet433_2x1": _vof_tet}
+         "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box}
 
 
 def main(names):

@@ -134,7 +134,7 @@ def _eq_of_call(d, k):
 
 
 @pytest.mark.parametrize("name", ["cavity4_jitter_random", "box654_jitter_2x2", "orifice_lid",
-                                  "vof_box654_2x2", "vof_tet433_2x1"])
+                                  "vof_box654_2x2", "vof_tet433_2x1", "sigma_box766_2x1"])
 def test_assembly_replay_vs_reference(name):
     """The reference's own COO systems, replayed from its call-time fields.
     Momentum and pressure systems are bit-exact.  Volume-fraction systems
@@ -163,6 +163,16 @@ def test_assembly_replay_vs_reference(name):
         if any(case.g):
             for q in range(3):
                 g.set_field(f"srcG{q}", d[p + f"srcG{q}"])
+        if case.sigma:
+            # curvature and surface tension, once per time step after the VOF
+            # loop (pressureBased.cpp:130); the reference inverts the level-set
+            # least-squares matrices by SVD, the device by cofactors: round-off
+            if eq == "momentum" and _eq_of_call(d, k - 1) == "volume_fraction":
+                g.calcSourceSurfaceTension()
+                for f in ("kappa", "srcS0", "srcS1", "srcS2"):
+                    assert _rel(g.get_field(f), d[p + f]) <= 1e-12, p + f
+            for f in ("kappa", "srcS0", "srcS1", "srcS2"):
+                g.set_field(f, d[p + f])
         g.assemble(eq)
         gr, gc, gv, gb = g.matrix(eq)
         n = mesh.n_cells
@@ -312,6 +322,46 @@ def test_vof_steps_vs_oracle(oracle_lib, precision, tol):
         scale = np.linalg.norm(b) if f in ("P", "VF") else vel
         assert np.linalg.norm(a - b) <= tol * scale, (f, np.linalg.norm(a - b) / scale)
     assert 0.0 < o.get("VF").mean() < 1.0
+    g.close()
+
+
+def _droplet(mesh, centre, radius, width):
+    nc = mesh.n_cells
+    ptr, pts = mesh.cell_ptr, mesh.cell_pts
+    cen = np.stack([mesh.points[pts[ptr[i]:ptr[i + 1]]].mean(0) for i in range(nc)])
+    r = np.linalg.norm(cen - np.asarray(centre), axis=1)
+    return np.clip((radius - r) / width + 0.5, 0.0, 1.0)
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+@pytest.mark.parametrize("reorder", [False, True])
+def test_surface_tension_vs_oracle(oracle_lib, kind, reorder):
+    """Curvature (level-set reinitialisation + normal divergence) and the
+    surface-tension source of a smeared droplet, against the oracle.  Both
+    use the same cofactor inverses; they differ in libm (pow(V, 0.3),
+    pow(V, 0.333)), so the bar is round-off: 1e-12 relative.  A renumbered
+    upload changes every summation order, and kappa is a second derivative
+    of the reinitialised level set, which amplifies round-off: 1e-9 there."""
+    from semo_mesh_amd.cases import dam_break_case
+    from semo_mesh_amd.mesh import tet_mesh
+    mesh = (hex_mesh(9, 8, 7, jitter=0.1, seed=13, ref_face=False) if kind == "hex"
+            else tet_mesh(5, 4, 4, jitter=0.1, seed=13))
+    case = dam_break_case(dt=1e-3); case.sigma = 0.07
+    vf = _droplet(mesh, (0.5, 0.5, 0.45), 0.3, 0.15)
+    rng = np.random.default_rng(4)
+    nc = mesh.n_cells
+    o = oracle_lib.Oracle(mesh, case)
+    g = _gpu(mesh, case, reorder)
+    for f, v in (("VF", vf), ("U", 0.1 * rng.normal(size=nc)), ("V", 0.1 * rng.normal(size=nc)),
+                 ("W", 0.1 * rng.normal(size=nc))):
+        o.set(f, v); g.set_field(f, v)
+    o.eos(); o.recon(); o.surface_tension()
+    g.calcSourceSurfaceTension()
+    tol = 1e-9 if reorder else 1e-12
+    for f in ("kappa", "srcS0", "srcS1", "srcS2"):
+        a, b = g.get_field(f), o.get(f)
+        assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b), (f, _rel(a, b))
+    assert np.abs(o.get("kappa")).max() > 1.0
     g.close()
 
 

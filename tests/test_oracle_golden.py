@@ -77,15 +77,27 @@ def test_assembly_replay_bit_exact(name, oracle_lib):
             o.set_dt(d[p + "dt"][0])
         o.recon()
         o.gravity()
+        first_of_step = "momentum" in p and "volume_fraction" in cl[i - 1]
         if any(case.g):
             # the source is computed once per time step, after the VOF loop
             # (pressureBased.cpp:129-130): it matches a recomputation at the
             # first momentum call of the step; later calls use the stored one
-            if "momentum" in p and "volume_fraction" in cl[i - 1]:
+            if first_of_step:
                 for k in range(3):
                     assert np.array_equal(o.get(f"srcG{k}"), d[p + f"srcG{k}"]), p + f"srcG{k}"
             for k in range(3):
                 o.set(f"srcG{k}", d[p + f"srcG{k}"])
+        if case.sigma:
+            # curvature + surface-tension source, once per step: equal to
+            # round-off (the reference inverts the level-set least-squares
+            # matrices by SVD, the oracle by cofactors)
+            if first_of_step:
+                o.surface_tension()
+                for f in ("kappa", "srcS0", "srcS1", "srcS2"):
+                    b = d[p + f]
+                    assert np.linalg.norm(o.get(f) - b) <= 1e-13 * np.linalg.norm(b), p + f
+            for f in ("kappa", "srcS0", "srcS1", "srcS2"):
+                o.set(f, d[p + f])
         if "volume_fraction" in p:
             o.mstacs()
             for f in ("faceLVF", "faceRVF"):
