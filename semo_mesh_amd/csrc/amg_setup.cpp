@@ -176,14 +176,20 @@ int choose_lanes(int n, const std::vector<int>& ptr, size_t padded) {
     return lanes;
 }
 
+// epos (optional): device element index of every off-diagonal entry, in the
+// order split_diag visits them
 template <class T>
-void upload_csr(DevSell<T>& M, const HCsr& A, bool square, hipStream_t s) {
+void upload_csr(DevSell<T>& M, const HCsr& A, bool square, hipStream_t s, std::vector<int>* epos = nullptr) {
     std::vector<int> ptr, col; std::vector<double> val, diag;
     if (square) split_diag(A, ptr, col, val, diag);
     else { ptr = A.ptr; col = A.col; val = A.val; }
     std::vector<int> pos;
     SellPattern p = sell_from_csr(A.n, A.m, ptr, col, &pos);
     int lanes = choose_lanes(A.n, ptr, p.col.size());
+    if (epos) {
+        epos->resize(col.size());
+        for (size_t k = 0; k < col.size(); ++k) (*epos)[k] = lanes ? (int)k : pos[k];
+    }
     if (lanes) {
         csr_upload<T>(M, A.n, A.m, ptr, col, val, diag, lanes, s);
     } else {
@@ -196,7 +202,9 @@ void upload_csr(DevSell<T>& M, const HCsr& A, bool square, hipStream_t s) {
 
 }  // namespace
 
-void amg_setup_host(semo_ctx* c, Amg& amg, const HCsr& A0) {
+// src_slot (optional): for every entry of A0, the slot of the device system
+// it came from (-1 on the diagonal); lets level 0 be refreshed on the device
+void amg_setup_host(semo_ctx* c, Amg& amg, const HCsr& A0, const std::vector<int>* src_slot = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = c->stream;
     amg.lv.clear();
@@ -213,8 +221,21 @@ void amg_setup_host(semo_ctx* c, Amg& amg, const HCsr& A0) {
             SEMO_CHECK(d != 0.0, SEMO_EZERO_PIVOT, "zero diagonal in the AMG operator");
             dinv[i] = 1.0 / d;
         }
-        upload_csr<double>(L.A64, A, true, s);
+        std::vector<int> epos;
+        upload_csr<double>(L.A64, A, true, s, level == 0 && src_slot ? &epos : nullptr);
         upload_csr<float>(L.A32, A, true, s);
+        if (level == 0 && src_slot) {
+            // device element -> source slot, for refreshing level 0's values
+            std::vector<int> map(L.A64.val.n, -1);
+            size_t k = 0;
+            for (int i = 0; i < A.n; ++i)
+                for (int q = A.ptr[i]; q < A.ptr[i + 1]; ++q)
+                    if (!(A.col[q] == i && A.n == A.m)) map[epos[k++]] = (*src_slot)[q];
+            amg.l0_map.upload(map, s);
+            SEMO_HIP(hipStreamSynchronize(s));
+        } else if (level == 0) {
+            amg.l0_map.release();
+        }
         L.dinv64.upload(dinv, s);
         std::vector<float> df(dinv.begin(), dinv.end());
         L.dinv32.upload(df, s);
@@ -328,24 +349,25 @@ void amg_build(semo_ctx* c, Amg& amg, const DevSell<double>& Ad) {
         std::equal(diag.begin(), diag.end(), amg.key.begin() + val.size()))
         return;
     HCsr A; A.n = A.m = c->nc; A.ptr.assign(c->nc + 1, 0);
+    std::vector<int> slot;
     for (int s = 0; s < p.nslices; ++s) {
         int w = (p.sl_ptr[s + 1] - p.sl_ptr[s]) / SL;
         for (int lane = 0; lane < SL; ++lane) {
             int r = s * SL + lane;
             if (r >= c->nc) break;
-            std::vector<std::pair<int, double>> row;
-            row.push_back({r, diag[r]});
+            std::vector<std::pair<int, std::pair<double, int>>> row;
+            row.push_back({r, {diag[r], -1}});
             for (int k = 0; k < w; ++k) {
                 int e = p.sl_ptr[s] + k * SL + lane;
                 if (p.col[e] == r) continue;     // padding
-                row.push_back({p.col[e], val[e]});
+                row.push_back({p.col[e], {val[e], e}});
             }
             std::sort(row.begin(), row.end());
-            for (auto& q : row) { A.col.push_back(q.first); A.val.push_back(q.second); }
+            for (auto& q : row) { A.col.push_back(q.first); A.val.push_back(q.second.first); slot.push_back(q.second.second); }
             A.ptr[r + 1] = (int)A.col.size();
         }
     }
-    amg_setup_host(c, amg, A);
+    amg_setup_host(c, amg, A, &slot);
     amg.key = val;
     amg.key.insert(amg.key.end(), diag.begin(), diag.end());
 }

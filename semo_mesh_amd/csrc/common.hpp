@@ -154,6 +154,12 @@ struct Amg {
     double setup_ms = 0;
     bool built = false;
     std::vector<double> key;    // host copy of the fine values the hierarchy was built from
+    // level-0 refresh: device element of the level-0 operator -> slot of the
+    // assembled system (-1: padding); Krylov iterations right after the last
+    // full build and of the last solve (rebuild policy)
+    DBuf<int> l0_map;
+    int iters_ref = -1, last_iters = -1;
+    bool fresh = false;         // the last solve used a freshly built hierarchy
 };
 
 }  // namespace semo

@@ -605,6 +605,27 @@ int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const do
     return std::min(it, maxit);
 }
 
+// level-0 operator of the hierarchy refreshed from the assembled system
+// (same pattern, new values): off-diagonals through the slot map, diagonal,
+// D^-1, FP32 copies
+__global__ void __launch_bounds__(TPB) k_refresh_offdiag(size_t nv, const int* __restrict__ map,
+                                                         const double* __restrict__ src, double* __restrict__ v64,
+                                                         float* __restrict__ v32) {
+    size_t e = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= nv) return;
+    int m = map[e];
+    double v = m >= 0 ? src[m] : 0.0;
+    v64[e] = v; v32[e] = (float)v;
+}
+__global__ void __launch_bounds__(TPB) k_refresh_diag(int n, const double* __restrict__ d, double* __restrict__ d64,
+                                                      float* __restrict__ d32, double* __restrict__ di64,
+                                                      float* __restrict__ di32) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = d[i], iv = 1.0 / v;
+    d64[i] = v; d32[i] = (float)v; di64[i] = iv; di32[i] = (float)iv;
+}
+
 __global__ void __launch_bounds__(TPB) k_dinv(int n, const double* __restrict__ d, double* __restrict__ o) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i < n) o[i] = 1.0 / d[i];
@@ -686,6 +707,29 @@ void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* s
         SEMO_HIP(hipStreamSynchronize(c->stream));
         rebuild = f != 0;
     }
+    // New values on the same pattern (a variable-density flow): refresh the
+    // finest level on the device and keep the coarse levels, as long as the
+    // flexible CG converges within 1.5x (+3) the iterations it needed right
+    // after the last full set-up; otherwise, or without a slot map, rebuild.
+    // The outer Krylov method always uses the current operator, so this only
+    // affects the preconditioner's quality, never the solution.
+    bool refresh = false;
+    if (rebuild && amg.built && amg.l0_map.n == amg.lv[0].A64.val.n && amg.l0_map.n > 0 && amg.iters_ref >= 0 &&
+        amg.last_iters <= std::max((3 * amg.iters_ref) / 2, amg.iters_ref + 3)) {
+        refresh = true;
+        rebuild = false;
+    }
+    if (refresh) {
+        AmgLevel& L0 = amg.lv[0];
+        size_t nv = L0.A64.val.n;
+        hipLaunchKernelGGL(k_refresh_offdiag, dim3((unsigned)((nv + TPB - 1) / TPB)), dim3(TPB), 0, c->stream, nv,
+                           amg.l0_map.p, c->Amat.val.p, L0.A64.val.p, L0.A32.val.p);
+        hipLaunchKernelGGL(k_refresh_diag, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->Amat.diag.p,
+                           L0.A64.diag.p, L0.A32.diag.p, L0.dinv64.p, L0.dinv32.p);
+        SEMO_HIP(hipMemcpyAsync(c->amg_key_val.p, c->Amat.val.p, c->Amat.val.bytes(), hipMemcpyDeviceToDevice, c->stream));
+        SEMO_HIP(hipMemcpyAsync(c->amg_key_diag.p, c->Amat.diag.p, c->Amat.diag.bytes(), hipMemcpyDeviceToDevice, c->stream));
+        amg.fresh = false;
+    }
     if (rebuild) {
         amg.built = false;
         amg_build(c, amg, c->Amat);
@@ -693,10 +737,13 @@ void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* s
         c->amg_key_diag.alloc(c->nc);
         SEMO_HIP(hipMemcpyAsync(c->amg_key_val.p, c->Amat.val.p, c->Amat.val.bytes(), hipMemcpyDeviceToDevice, c->stream));
         SEMO_HIP(hipMemcpyAsync(c->amg_key_diag.p, c->Amat.diag.p, c->Amat.diag.bytes(), hipMemcpyDeviceToDevice, c->stream));
+        amg.fresh = true;
     }
     reduction_alloc(c, c->nc);
     double rr = 0;
     int it = fpcg(c, amg, c->Amat, b, x, c->sp.tol_p, c->sp.maxit_p, c->sp.precision == SEMO_PREC_MIXED, &rr);
+    if (amg.fresh) { amg.iters_ref = it; amg.fresh = false; }
+    amg.last_iters = it;
     if (st) {
         st->iters[0] = it; st->rel_res[0] = rr; st->n_levels = (int)amg.lv.size(); st->setup_ms = amg.setup_ms;
     }
