@@ -8,7 +8,7 @@
 // Every function states the reference file:line it restates.  The loops keep
 // the reference's per-cell accumulation order and the left-to-right grouping
 // of its arithmetic, so that on the same mesh the results are bit-identical to
-// the compiled reference (pinned in tests/test_oracle_ref.py against fixtures
+// the compiled reference (pinned in tests/test_oracle_golden.py against fixtures
 // the compiled reference wrote).  The linear solves are this file's own
 // (ILU0-preconditioned BiCGStab to a tight tolerance); SEMO uses AMGCL, so the
 // solved fields agree to the solver tolerance, not bit for bit.
