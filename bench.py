@@ -99,6 +99,79 @@ def dam_break_leg(steps, warmup, nx=96):
             "alpha_min": float(vf1.min()), "alpha_max": float(vf1.max())}
 
 
+def adjoint_leg(n=64, n_iter=20, reps=3):
+    """Discrete adjoint of the 262k-cell configuration: jittered n^3 cavity,
+    rho = 1, mu = 0.01, dt = 0.05, Darcy alpha_p ~ U[0, 10) seeded, 20 outer
+    iterations of one time step (checkpointed, recomputed in reverse),
+    J = int p dA (lid) - int p dA (bottom), solves to 1e-8 (mixed pressure
+    path).  One gradient = forward + reverse sweep; grads/s = 1 / its wall
+    time (forward timed alone on the same warmed-up context for comparison).
+    A directional central difference (h = 1e-5, solves at 1e-12) checks the
+    gradient at this size."""
+    from semo_mesh_amd.cases import cavity_case
+    from semo_mesh_amd.mesh import cavity_mesh
+    from semo_mesh_amd.solver import GpuSolver
+    m = cavity_mesh(n, jitter=0.1, seed=20261205)
+    case = cavity_case(dt=0.05)
+    case.rho = (1.0, 1.0)
+    case.mu = (0.01, 0.01)
+    alpha = 10.0 * np.random.default_rng(20261205).random(m.n_cells)
+    W = {"zmax": 1.0, "zmin": -1.0}
+
+    def ctx(a):
+        g = GpuSolver(0)
+        g.upload_mesh(m, reorder=True)
+        g.set_case(case)
+        g.set_solver("mixed", tol_p=1e-8, tol_u=1e-8, maxit_p=1000, maxit_u=1000)
+        g.set_field("alphaP", a)
+        return g
+
+    def reset(g):
+        for f in ("U", "V", "W", "P"):
+            g.set_field(f, np.zeros(m.n_cells))
+
+    g = ctx(alpha)
+    g.adjointAlphaP(2, W)                       # warm-up: AMG hierarchy, work arrays
+    times, ftimes = [], []
+    for _ in range(reps):
+        reset(g)
+        t0 = time.perf_counter()
+        J, grad = g.adjointAlphaP(n_iter, W)
+        times.append(time.perf_counter() - t0)
+        reset(g)
+        t0 = time.perf_counter()
+        g.incompressiblePressureBased(n_iter)
+        ftimes.append(time.perf_counter() - t0)
+    g.close()
+    t, t_fwd = min(times), min(ftimes)
+    # gradient check at this size: adjoint and central differences with the
+    # solves at 1e-12, so that the difference quotient sees the discrete map
+    tight = dict(tol_p=1e-12, tol_u=1e-12, maxit_p=1000, maxit_u=1000)
+
+    def J_of(a):
+        g = ctx(a)
+        g.set_solver("mixed", **tight)
+        g.incompressiblePressureBased(n_iter)
+        j = g.objective(W)
+        g.close()
+        return j
+
+    g = ctx(alpha)
+    g.set_solver("mixed", **tight)
+    _, grad_t = g.adjointAlphaP(n_iter, W)
+    g.close()
+    v = np.random.default_rng(7).standard_normal(m.n_cells)
+    h = 1e-5
+    fd = (J_of(alpha + h * v) - J_of(alpha - h * v)) / (2 * h)
+    return {"workload": f"discrete adjoint, jittered {n}^3 cavity ({m.n_cells} cells), Darcy alpha_p ~ U[0,10), "
+                        f"{n_iter} checkpointed SIMPLE iterations, J = int p dA(lid) - int p dA(bottom)",
+            "cells": m.n_cells, "iterations": n_iter, "value": round(1.0 / t, 3), "unit": "grads/s",
+            "ms_per_gradient": round(1e3 * t, 1), "forward_ms": round(1e3 * t_fwd, 1),
+            "gradient_over_forward": round(t / t_fwd, 2), "J": J,
+            "directional_fd_rel_err": float(abs(grad_t @ v - fd) / abs(fd)),
+            "fd_check": "directional central difference, h = 1e-5, solves at 1e-12"}
+
+
 def cpu_baseline(n=48):
     """The compiled reference (SEMO + AMGCL, 1 MPI rank, no OpenMP) on a
     bounded sample: one SIMPLE iteration of the same cavity at n^3."""
@@ -142,6 +215,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-vof", action="store_true", help="skip the two-phase dam-break leg")
     ap.add_argument("--vof-steps", type=int, default=50)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB): the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -224,6 +298,9 @@ def main():
     if not args.no_vof:
         out["vof_dam_break"] = dam_break_leg(args.vof_steps, 2)
         print(f"[bench] dam break: {out['vof_dam_break']['ms_per_step']} ms/step", file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
+        print(f"[bench] adjoint: {out['adjoint']['ms_per_gradient']} ms per gradient", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -194,6 +194,56 @@ int semo_calc_time_step(semo_ctx* ctx, int adjust, double* dt, double* courant);
  * per corrector.                                                               */
 int semo_incompressible_pressure_based(semo_ctx* ctx, int n_correctors, double* norms);
 
+/* Block-coupled pressure-velocity iteration: replaces calcCoupledEq
+ * (src/solvers/eqCoupled.cpp:12-1250, commented out in the reference; its
+ * design: 4 unknowns per cell p', u', v', w', B_n = 4 at :26, variable-major
+ * global index at :249-305, one solveAMGCL call at :1246).  One outer
+ * iteration: boundary values, momentum and pressure assembly, the 4x4-block
+ * system (momentum matrix + linearised pressure force; pressure-correction
+ * matrix + velocity-correction flux), FP64 BiCGStab with an FP32 block-AMG
+ * preconditioner to semo_solver_params.tol_u / maxit_u, then
+ * p += urf_p p', U += urf_mom u' (clipped at limit).  *norm as
+ * semo_calc_momentum_eqs.  semo_assemble(ctx, 3) assembles the block system
+ * without solving.                                                            */
+int semo_calc_coupled_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
+
+/* One time step with the block-coupled iteration in place of the
+ * momentum / pressure correctors: the set-up of
+ * semo_incompressible_pressure_based (time levels, dt, VOF, sources), then
+ * n_iter semo_calc_coupled_eq iterations; norms[k] per iteration.             */
+int semo_coupled_pressure_based(semo_ctx* ctx, int n_iter, double* norms);
+
+/* The last assembled block system: block COO in caller numbering (the n
+ * diagonal blocks first, then the off-diagonal blocks row by row), blocks
+ * row-major 4x4 over (p', u', v', w'), and the right-hand side variable-major
+ * rhs[var * n_cells + cell] (eqCoupled.cpp:249-305).  Call with null arrays
+ * to get *nnzb first.                                                          */
+int semo_coupled_matrix_get(semo_ctx* ctx, int64_t* nnzb, int32_t* rows, int32_t* cols, double* blocks,
+                            double* rhs);
+
+/* y = A x and the solve A x = b with the last assembled block system; x, y,
+ * b variable-major [4][n_cells], caller numbering.                           */
+int semo_coupled_spmv(semo_ctx* ctx, const double* x, double* y);
+int semo_coupled_solve(semo_ctx* ctx, const double* b, double* x, semo_solve_stats* st);
+
+/* Discrete adjoint.  No counterpart in the reference: SEMO has no adjoint.
+ * Runs n_iter outer iterations exactly as
+ * semo_incompressible_pressure_based(ctx, n_iter, NULL) (single phase:
+ * sigma = 0, g = 0, n_vof = 0, fixed dt, equal phase properties), with the
+ * Darcy term of field SEMO_F_ALPHA_P (momentum diagonal += alpha_p V, right
+ * side -= alpha_p V U; zero if never set).  Returns J of the final state,
+ *   J = sum_patches patch_weight[p] * sum_{faces f of p} A_f p_f
+ * (p_f: the boundary face pressure; fixedValue patches contribute their
+ * value), and dJ/dalpha_p for every cell (caller numbering, n_cells
+ * doubles), by reverse-mode differentiation of the discrete iterations
+ * (checkpoint per iteration, recompute, transposed solves).  The context is
+ * left in the forward's final state.                                         */
+int semo_adjoint_alpha_p(semo_ctx* ctx, int n_iter, const double* patch_weight, double* J, double* dJ_dalpha);
+
+/* J (as above) of the current pressure field.  No counterpart in the
+ * reference; the objective of semo_adjoint_alpha_p.                          */
+int semo_objective_pressure(semo_ctx* ctx, const double* patch_weight, double* J);
+
 /* Statistics of the last momentum (eq 0) / pressure (eq 1) / VOF (eq 2) solve. */
 int semo_get_solve_stats(const semo_ctx* ctx, int eq, semo_solve_stats* st);
 

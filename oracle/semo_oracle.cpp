@@ -66,6 +66,9 @@ struct Oracle {
     vector<double> P, U, V, W, VF, MF, Rho, Mu;
     vector<double> oldP, oldU, oldV, oldW, oldVF, oldRho, Qm0, Qm1, Qm2, Qm3;
     vector<double> srcG[4], srcS[4], kappa, LS;
+    // Darcy porosity alpha_p of the adjoint harness (no counterpart in the
+    // reference): momentum diagonal += alpha_p V, right sides -= alpha_p V U
+    vector<double> alphaP;
     vector<double> gP, gU, gV, gW, gVF;    // [3n]
     vector<double> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
     // face left/right values (src/solvers/reconIncom.cpp:501-1035)
@@ -528,6 +531,9 @@ void assemble_momentum(Oracle& o) {
         o.B2[i] = -R * (1.5 * o.W[i] - 2.0 * o.oldW[i] + 0.5 * o.Qm2[i]) * vol / dt;
         o.B0[i] += vol * o.srcG[0][i]; o.B1[i] += vol * o.srcG[1][i]; o.B2[i] += vol * o.srcG[2][i];
         o.B0[i] += vol * o.srcS[0][i]; o.B1[i] += vol * o.srcS[1][i]; o.B2[i] += vol * o.srcS[2][i];
+        double ap = o.alphaP[i] * vol;
+        o.Avals[i] += ap;
+        o.B0[i] -= ap * o.U[i]; o.B1[i] -= ap * o.V[i]; o.B2[i] -= ap * o.W[i];
     }
     double* B[3] = {o.B0.data(), o.B1.data(), o.B2.data()};
     for (int f = 0; f < o.nif; ++f) {            // eqMomentum.cpp:331-787
@@ -1449,7 +1455,7 @@ void* orc_create(int np, int nc, int nf, int nif, const double* pts, const int* 
     z(o->Qm0); z(o->Qm1); z(o->Qm2); z(o->Qm3, 1.0);
     z(o->oldP); z(o->oldU); z(o->oldV); z(o->oldW); z(o->oldVF, 1.0); z(o->oldRho);
     for (int k = 0; k < 4; ++k) { z(o->srcG[k]); z(o->srcS[k]); }
-    z(o->kappa);
+    z(o->kappa); z(o->alphaP);
     return o;
 }
 
@@ -1510,6 +1516,7 @@ static vector<double>* field(Oracle& o, const char* nm) {
     F("faceLU", LU) F("faceLV", LV) F("faceLW", LW) F("faceLRho", LRho) F("faceLVF", LVF)
     F("oldVF", oldVF) F("Qm3", Qm3) F("courant", co_cell) F("kappa", kappa) F("LS", LS)
     F("srcS0", srcS[0]) F("srcS1", srcS[1]) F("srcS2", srcS[2]) F("srcS3", srcS[3])
+    F("alphaP", alphaP)
 #undef F
     return nullptr;
 }

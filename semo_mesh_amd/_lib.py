@@ -70,7 +70,8 @@ EXPORTED = [
     "semo_rcm", "semo_set_case", "semo_set_solver", "semo_field_set", "semo_field_get",
     "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_calc_volfrac_eq", "semo_calc_time_step",
     "semo_calc_source_gravity", "semo_calc_source_surface_tension",
-    "semo_incompressible_pressure_based",
+    "semo_incompressible_pressure_based", "semo_adjoint_alpha_p", "semo_objective_pressure",
+    "semo_calc_coupled_eq", "semo_coupled_pressure_based", "semo_coupled_matrix_get", "semo_coupled_spmv", "semo_coupled_solve",
     "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
 ]
@@ -107,6 +108,13 @@ def lib():
         "semo_calc_source_gravity": (C.c_int, [vp]),
         "semo_calc_source_surface_tension": (C.c_int, [vp]),
         "semo_incompressible_pressure_based": (C.c_int, [vp, C.c_int, dp]),
+        "semo_adjoint_alpha_p": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+        "semo_objective_pressure": (C.c_int, [vp, dp, dp]),
+        "semo_calc_coupled_eq": (C.c_int, [vp, dp, vp]),
+        "semo_coupled_pressure_based": (C.c_int, [vp, C.c_int, dp]),
+        "semo_coupled_matrix_get": (C.c_int, [vp, vp, vp, vp, dp, dp]),
+        "semo_coupled_spmv": (C.c_int, [vp, dp, dp]),
+        "semo_coupled_solve": (C.c_int, [vp, dp, dp, vp]),
         "semo_get_solve_stats": (C.c_int, [vp, C.c_int, C.POINTER(SolveStats)]),
         "semo_assemble": (C.c_int, [vp, C.c_int]),
         "semo_matrix_get": (C.c_int, [vp, C.POINTER(i64), vp, vp, vp, vp]),

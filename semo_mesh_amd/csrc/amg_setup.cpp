@@ -372,6 +372,47 @@ void amg_build(semo_ctx* c, Amg& amg, const DevSell<double>& Ad) {
     amg.key.insert(amg.key.end(), diag.begin(), diag.end());
 }
 
+// Aggregation hierarchy of a scalar operator (CSR, diagonal included) for the
+// block AMG of the coupled system: per level the fine -> aggregate map, from
+// plain aggregation on the strength graph (eps halved per level) and the
+// unsmoothed Galerkin product P^T A P (sums of the entries between two
+// aggregates) as the next level's scalar operator.  Stops when a level has
+// at most min_coarse rows or aggregation no longer coarsens.
+std::vector<std::vector<int>> aggregate_hierarchy(int n, const std::vector<int>& ptr, const std::vector<int>& col,
+                                                  const std::vector<double>& val, int min_coarse) {
+    std::vector<std::vector<int>> maps;
+    HCsr A; A.n = A.m = n; A.ptr = ptr; A.col = col; A.val = val;
+    double eps = 0.08;
+    while (A.n > min_coarse) {
+        std::vector<int> agg;
+        int na = aggregate(A, eps, agg);
+        if (na >= A.n || na == 0 || na > (A.n * 9) / 10) break;
+        // coarse scalar operator: entry (agg[i], agg[j]) sums A(i, j)
+        HCsr Ac; Ac.n = Ac.m = na; Ac.ptr.assign(na + 1, 0);
+        std::vector<std::vector<int>> members(na);
+        for (int i = 0; i < A.n; ++i) members[agg[i]].push_back(i);
+        std::vector<int> mark(na, -1);
+        std::vector<double> acc(na, 0.0);
+        std::vector<int> cols;
+        for (int I = 0; I < na; ++I) {
+            cols.clear();
+            for (int i : members[I])
+                for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+                    int J = agg[A.col[k]];
+                    if (mark[J] != I) { mark[J] = I; acc[J] = 0.0; cols.push_back(J); }
+                    acc[J] += A.val[k];
+                }
+            std::sort(cols.begin(), cols.end());
+            for (int J : cols) { Ac.col.push_back(J); Ac.val.push_back(acc[J]); }
+            Ac.ptr[I + 1] = (int)Ac.col.size();
+        }
+        maps.push_back(std::move(agg));
+        A = std::move(Ac);
+        eps *= 0.5;
+    }
+    return maps;
+}
+
 // standalone system (COO in caller numbering) -> hierarchy
 void amg_build_csr(semo_ctx* c, Amg& amg, int n, const std::vector<int>& ptr,
                    const std::vector<int>& col, const std::vector<double>& val) {

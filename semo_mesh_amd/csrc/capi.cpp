@@ -164,6 +164,17 @@ double pressure_eq(semo_ctx* c, semo_solve_stats* st) {
 }
 
 }  // namespace
+
+// one time step's set-up and its outer iterations, for the adjoint driver
+void simple_setup(semo_ctx* c) {
+    step_setup(c);
+    c->courant = time_step(c, c->cs.adjust_dt);
+    for (int k = 0; k < c->cs.n_vof; ++k) volfrac_eq(c, &c->last_stats_vf);
+    step_sources(c);
+}
+double simple_momentum(semo_ctx* c, semo_solve_stats* st) { return momentum_eq(c, st); }
+double simple_pressure(semo_ctx* c, semo_solve_stats* st) { return pressure_eq(c, st); }
+
 }  // namespace semo
 
 using namespace semo;
@@ -196,6 +207,10 @@ int semo_ctx_destroy(semo_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     bool own = c->own_stream;
     hipStream_t s = c->stream;
+    adjoint_free(c->adj);
+    c->adj = nullptr;
+    coupled_free(c->cpl);
+    c->cpl = nullptr;
     delete c;
     if (own) (void)hipStreamDestroy(s);
     return SEMO_OK;
@@ -457,16 +472,131 @@ int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
     if (!c || ncorr < 1) return SEMO_EINVAL;
     return guard(c, [&] {
         need_case(c);
-        step_setup(c);
-        c->courant = time_step(c, c->cs.adjust_dt);     // pressureBased.cpp:77-90
-        for (int k = 0; k < c->cs.n_vof; ++k) volfrac_eq(c, &c->last_stats_vf);
-        step_sources(c);
+        simple_setup(c);      // pressureBased.cpp:17-130: time levels, dt, VOF, sources
         for (int it = 0; it < ncorr; ++it) {
             double n0 = momentum_eq(c, &c->last_stats_mom);
             double n1 = pressure_eq(c, &c->last_stats_p);
             if (norms) { norms[2 * it] = n0; norms[2 * it + 1] = n1; }
         }
         c->iterReal++;
+    });
+}
+
+int semo_adjoint_alpha_p(semo_ctx* c, int n_iter, const double* patch_weight, double* J, double* dJ_dalpha) {
+    if (!c || n_iter < 1 || !patch_weight) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        DBuf<double> g;
+        g.alloc(c->nc);
+        double j = adjoint_run(c, n_iter, patch_weight, g.p);
+        if (J) *J = j;
+        if (dJ_dalpha) from_dev(c, g.p, dJ_dalpha, 1);
+    });
+}
+
+// ---- block-coupled (BSR4) system ------------------------------------------------
+namespace {
+// var-major [4][n] caller numbering <-> interleaved [n][4] device numbering
+void blk_to_dev(semo_ctx* c, const double* h, DBuf<double>& d) {
+    size_t n = c->nc;
+    std::vector<double> t(4 * n);
+    for (size_t i = 0; i < n; ++i)
+        for (int q = 0; q < 4; ++q) t[4 * i + q] = h[q * n + c->perm[i]];
+    d.upload(t, c->stream);
+}
+void blk_from_dev(semo_ctx* c, const double* d, double* h) {
+    size_t n = c->nc;
+    std::vector<double> t(4 * n);
+    SEMO_HIP(hipMemcpyAsync(t.data(), d, t.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; ++i)
+        for (int q = 0; q < 4; ++q) h[q * n + c->perm[i]] = t[4 * i + q];
+}
+}  // namespace
+
+int semo_calc_coupled_eq(semo_ctx* c, double* norm, semo_solve_stats* st) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        recon_boundary(c);
+        assemble_coupled(c);
+        semo_solve_stats s{};
+        coupled_solve_assembled(c, &s);
+        if (st) *st = s;
+        double nv = coupled_update(c);
+        if (norm) *norm = nv;
+    });
+}
+
+int semo_coupled_pressure_based(semo_ctx* c, int n_iter, double* norms) {
+    if (!c || n_iter < 1) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        simple_setup(c);
+        for (int it = 0; it < n_iter; ++it) {
+            recon_boundary(c);
+            assemble_coupled(c);
+            coupled_solve_assembled(c, &c->last_stats_mom);
+            double nv = coupled_update(c);
+            if (norms) norms[it] = nv;
+        }
+        c->iterReal++;
+    });
+}
+
+int semo_coupled_matrix_get(semo_ctx* c, int64_t* nnzb, int32_t* rows, int32_t* cols, double* blocks,
+                            double* rhs) {
+    if (!c || !nnzb) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        std::vector<int> r, q;
+        std::vector<double> b, h;
+        coupled_export(c, r, q, b, h);
+        int64_t nz = (int64_t)r.size();
+        if (rows || cols || blocks || rhs)
+            SEMO_CHECK(*nnzb >= nz, SEMO_EINVAL, "coupled matrix: buffers smaller than nnzb");
+        *nnzb = nz;
+        size_t n = c->nc;
+        for (int64_t k = 0; k < nz; ++k) {
+            if (rows) rows[k] = c->perm[r[k]];
+            if (cols) cols[k] = c->perm[q[k]];
+        }
+        if (blocks) std::copy(b.begin(), b.end(), blocks);
+        if (rhs)
+            for (size_t i = 0; i < n; ++i)
+                for (int v = 0; v < 4; ++v) rhs[v * n + c->perm[i]] = h[4 * i + v];
+    });
+}
+
+int semo_coupled_spmv(semo_ctx* c, const double* x, double* y) {
+    if (!c || !x || !y) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        DBuf<double> dx, dy;
+        blk_to_dev(c, x, dx);
+        dy.alloc(4 * (size_t)c->nc);
+        coupled_spmv(c, dx.p, dy.p);
+        blk_from_dev(c, dy.p, y);
+    });
+}
+
+int semo_coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st) {
+    if (!c || !b || !x) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        DBuf<double> db, dx;
+        blk_to_dev(c, b, db);
+        dx.alloc(4 * (size_t)c->nc);
+        coupled_solve(c, db.p, dx.p, st);
+        blk_from_dev(c, dx.p, x);
+    });
+}
+
+int semo_objective_pressure(semo_ctx* c, const double* patch_weight, double* J) {
+    if (!c || !patch_weight || !J) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        *J = objective(c, patch_weight);
     });
 }
 
@@ -480,11 +610,13 @@ int semo_assemble(semo_ctx* c, int eq) {
     if (!c) return SEMO_EINVAL;
     return guard(c, [&] {
         need_case(c);
-        SEMO_CHECK(eq >= 0 && eq <= 2, SEMO_EINVAL, "eq must be 0 (momentum), 1 (pressure) or 2 (volume fraction)");
+        SEMO_CHECK(eq >= 0 && eq <= 3, SEMO_EINVAL,
+                   "eq must be 0 (momentum), 1 (pressure), 2 (volume fraction) or 3 (coupled)");
         recon_boundary(c);
         if (eq == 0) assemble_momentum(c);
         else if (eq == 1) assemble_pressure(c);
-        else assemble_vof(c);
+        else if (eq == 2) assemble_vof(c);
+        else assemble_coupled(c);
         SEMO_HIP(hipStreamSynchronize(c->stream));
     });
 }

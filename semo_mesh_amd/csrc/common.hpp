@@ -162,6 +162,9 @@ struct Amg {
     bool fresh = false;         // the last solve used a freshly built hierarchy
 };
 
+struct AdjBufs;                 // adjoint.hip: checkpoints and adjoint work arrays
+struct Coupled;                 // coupled.hip: block (4x4) system and its AMG
+
 }  // namespace semo
 
 // ------------------------------------------------------------------ context --
@@ -245,6 +248,8 @@ struct semo_ctx {
     semo::DBuf<double> kv64;                   // Krylov work (fp64)
     semo::DBuf<float> kv32;
     semo_solve_stats last_stats_mom{}, last_stats_p{}, last_stats_vf{};
+    semo::AdjBufs* adj = nullptr;
+    semo::Coupled* cpl = nullptr;
 };
 
 namespace semo {
@@ -283,4 +288,24 @@ void reduction_alloc(semo_ctx* c, long n);
 void amg_build(semo_ctx* c, Amg& amg, const DevSell<double>& A);
 void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* st);
 void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st);
+std::vector<std::vector<int>> aggregate_hierarchy(int n, const std::vector<int>& ptr, const std::vector<int>& col,
+                                                  const std::vector<double>& val, int min_coarse);
+// coupled.hip
+struct Coupled;
+void coupled_free(Coupled* p);
+void assemble_coupled(semo_ctx* c);
+void coupled_solve(semo_ctx* c, const double* b_dev, double* x_dev, semo_solve_stats* st);
+void coupled_solve_assembled(semo_ctx* c, semo_solve_stats* st);
+double coupled_update(semo_ctx* c);
+void coupled_spmv(semo_ctx* c, const double* x_dev, double* y_dev);
+void coupled_export(semo_ctx* c, std::vector<int>& rows, std::vector<int>& cols, std::vector<double>& blocks,
+                    std::vector<double>& rhs);
+// capi.cpp: the forward driver's pieces
+void simple_setup(semo_ctx* c);
+double simple_momentum(semo_ctx* c, semo_solve_stats* st);
+double simple_pressure(semo_ctx* c, semo_solve_stats* st);
+// adjoint.hip
+double adjoint_run(semo_ctx* c, int n_iter, const double* wpatch, double* grad_dev);
+double objective(semo_ctx* c, const double* wpatch);
+void adjoint_free(AdjBufs* b);
 }  // namespace semo

@@ -1,0 +1,661 @@
+// Block-coupled pressure-velocity solve: one 4x4 block per cell pair (BSR4).
+//
+// SEMO's calcCoupledEq (src/solvers/eqCoupled.cpp:12-1250) is commented out in
+// the reference: only its design survives -- four unknowns per cell
+// (p', u', v', w'), B_n = 4 (eqCoupled.cpp:26), global index
+// startCellGlobal*B_n + n*var + i, i.e. variable-major (eqCoupled.cpp:249-305),
+// a continuity row with the p' Laplacian and momentum rows with upwind
+// convection plus the face-interpolated p' gradient (eqCoupled.cpp:470-640),
+// one solveAMGCL call (:1246).  This file builds that system from the live
+// assembly of this library, so its blocks are exactly:
+//   momentum rows (k = 1..3): the momentum matrix of calcMomentumEqs on the
+//     (k, k) entries, plus the linearised pressure force: for a face f with
+//     owner P and neighbour N, row P gets +n_k A (wC p'_P + (1-wC) p'_N), row
+//     N the opposite; a zeroGradient-p boundary face adds n_k A p'_P;
+//   continuity row (0): the pressure-correction matrix of calcPressureEq on
+//     (0, 0), plus the face flux of the velocity correction,
+//     +A (wC n.u'_P + (1-wC) n.u'_N) in row P and the opposite in row N
+//     (zeroGradient / outflowing inletOutlet boundary faces add A n.u'_P);
+//   right-hand sides: the momentum and pressure right-hand sides.
+// The solution updates p += urf_p p', U += urf_u u'.  At a converged outer
+// iteration both right-hand sides vanish, so the coupled and the segregated
+// (SIMPLE) iterations share their fixed point.
+//
+// Storage (device numbering): block CSR, diagonal blocks apart, each block 16
+// contiguous values row-major; vectors interleaved [n][4] = (p, u, v, w).
+// Solver: FP64 BiCGStab, right-preconditioned by an FP32 block AMG --
+// aggregates of the scalar pressure operator, tentative (piecewise constant)
+// prolongation times I_4, Galerkin block sums refreshed on the device from
+// precomputed contribution lists, damped block-Jacobi smoothing (4x4
+// inverses), block-Jacobi sweeps on the coarsest level.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "kern.hpp"
+
+namespace semo {
+
+struct BLevel {
+    int n = 0;
+    int64_t nnzb = 0;                       // off-diagonal blocks
+    DBuf<int> ptr, col;
+    DBuf<float> val, diag, dinv;            // FP32 blocks (the smoother's operator)
+    DBuf<float> x, x2, b, r;                // [n][4]
+    // restriction: members of each aggregate of this level (this level = coarse)
+    DBuf<int> mptr, mem;                    // CSR over this level's rows: fine rows
+    DBuf<int> agg;                          // finer level's row -> this level's row
+    // Galerkin refresh: this level's blocks (diag first, then off-diagonal)
+    // as sums of the finer level's blocks (diag blocks coded ~i)
+    DBuf<int> cptr, clist;
+};
+
+struct Coupled {
+    bool built = false;
+    int n = 0;
+    // fine level (device numbering), FP64 + FP32
+    std::vector<int> h_ptr, h_col;
+    DBuf<int> ptr, col, fslot_own, fslot_nbr;   // per internal face: block index in row owner / neighbour
+    DBuf<double> val, diag, rhs;                // [nnzb][16], [n][16], [n][4]
+    DBuf<double> mval, mdiag, mrhs, pval, pdiag, prhs;   // scalar momentum / pressure systems
+    DBuf<double> x;                             // [n][4] last solution
+    std::vector<BLevel> lv;                     // lv[0] = fine level FP32 copy
+    DBuf<double> kv;                            // Krylov vectors
+    DBuf<double> part;
+    int last_iters = 0;
+    double last_relres = 0.0;
+};
+
+void coupled_free(Coupled* p) { delete p; }
+
+namespace {
+
+constexpr int NSWEEP = 2;          // pre / post smoothing sweeps
+constexpr int NCOARSE = 8;         // sweeps on the coarsest level
+constexpr float OMEGA = 0.6f;      // block-Jacobi damping
+
+// ---------------------------------------------------------------- assembly --
+__global__ void __launch_bounds__(TPB) k_cpl_row(
+    int nc, int nif, int nbf, const int* __restrict__ cf_sl, const int* __restrict__ cf_ell,
+    const int* __restrict__ owner, const int* __restrict__ nbr, const int* __restrict__ bpatch,
+    const semo_patch_bc* __restrict__ bcs, const double* __restrict__ nrm, const double* __restrict__ area,
+    const double* __restrict__ wCv, const int* __restrict__ slot_own, const int* __restrict__ slot_nbr,
+    const int* __restrict__ fso, const int* __restrict__ fsn, const double* __restrict__ mval,
+    const double* __restrict__ mdiag, const double* __restrict__ pval, const double* __restrict__ pdiag,
+    const double* __restrict__ mrhs, const double* __restrict__ prhs, const double* __restrict__ U,
+    const double* __restrict__ V, const double* __restrict__ W, const int* __restrict__ bptr,
+    double* __restrict__ val, double* __restrict__ diag, double* __restrict__ rhs) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    for (int e = bptr[i]; e < bptr[i + 1]; ++e)
+        for (int q = 0; q < 16; ++q) val[16 * (size_t)e + q] = 0.0;
+    double D[16];
+    for (int q = 0; q < 16; ++q) D[q] = 0.0;
+    double uc[3] = {U[i], V[i], W[i]};
+    CELL_FACES_BEGIN(i, cf_sl, cf_ell, f, own)
+        const double* n = nrm + 3 * (size_t)f;
+        double A = area[f];
+        if (f < nif) {
+            double wC = wCv[f];
+            double* B = val + 16 * (size_t)(own ? fso[f] : fsn[f]);
+            double a = own ? mval[slot_own[f]] : mval[slot_nbr[f]];
+            double q = own ? pval[slot_own[f]] : pval[slot_nbr[f]];
+            B[0] += q;
+            for (int k = 1; k < 4; ++k) B[5 * k] += a;
+            double wi = own ? wC : 1.0 - wC, wj = 1.0 - wi, sg = own ? 1.0 : -1.0;
+            for (int k = 1; k < 4; ++k) {
+                double g = sg * n[k - 1] * A;
+                D[k] += g * wi;       // continuity row, u'_k of this cell
+                B[k] += g * wj;
+                D[4 * k] += g * wi;   // momentum row k, p' of this cell
+                B[4 * k] += g * wj;
+            }
+            continue;
+        }
+        semo_patch_bc bc = bcs[bpatch[f - nif]];
+        if (bc.p_type != SEMO_BC_FIXED_VALUE)
+            for (int k = 1; k < 4; ++k) D[4 * k] += n[k - 1] * A;
+        bool flux = bc.u_type == SEMO_BC_ZERO_GRADIENT ||
+                    (bc.u_type == SEMO_BC_INLET_OUTLET && uc[0] * n[0] + uc[1] * n[1] + uc[2] * n[2] > 0.0);
+        if (flux)
+            for (int k = 1; k < 4; ++k) D[k] += n[k - 1] * A;
+    }
+    D[0] += pdiag[i];
+    for (int k = 1; k < 4; ++k) D[5 * k] += mdiag[i];
+    for (int q = 0; q < 16; ++q) diag[16 * (size_t)i + q] = D[q];
+    size_t N = (size_t)nc;
+    rhs[4 * (size_t)i] = prhs[i];
+    for (int k = 0; k < 3; ++k) rhs[4 * (size_t)i + 1 + k] = mrhs[k * N + i];
+}
+
+// inverse of a 4x4 block, Gauss-Jordan with partial pivoting in FP64
+__device__ inline void inv4(const double* a, double* o) {
+    double m[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int q = 0; q < 4; ++q) { m[r][q] = a[4 * r + q]; m[r][4 + q] = r == q ? 1.0 : 0.0; }
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r) if (fabs(m[r][c]) > fabs(m[p][c])) p = r;
+        if (p != c)
+            for (int q = 0; q < 8; ++q) { double t = m[c][q]; m[c][q] = m[p][q]; m[p][q] = t; }
+        double piv = m[c][c];
+        double ip = piv != 0.0 ? 1.0 / piv : 0.0;
+        for (int q = 0; q < 8; ++q) m[c][q] *= ip;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            double fct = m[r][c];
+            for (int q = 0; q < 8; ++q) m[r][q] -= fct * m[c][q];
+        }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int q = 0; q < 4; ++q) o[4 * r + q] = m[r][4 + q];
+}
+
+// fine FP32 copy, block inverses
+__global__ void __launch_bounds__(TPB) k_cpl_f32(size_t nv, const double* __restrict__ v, float* __restrict__ o) {
+    size_t e = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (e < nv) o[e] = (float)v[e];
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_cpl_dinv(int n, const T* __restrict__ diag, float* __restrict__ dinv) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a[16], o[16];
+    for (int q = 0; q < 16; ++q) a[q] = (double)diag[16 * (size_t)i + q];
+    inv4(a, o);
+    for (int q = 0; q < 16; ++q) dinv[16 * (size_t)i + q] = (float)o[q];
+}
+
+// Galerkin block sums: entry e of the coarse level (e < n: diagonal block e,
+// else off-diagonal block e - n) = sum of the listed finer blocks
+__global__ void __launch_bounds__(TPB) k_cpl_galerkin(int64_t ne, int nc, const int* __restrict__ cptr,
+                                                      const int* __restrict__ clist,
+                                                      const float* __restrict__ fdiag,
+                                                      const float* __restrict__ fval, float* __restrict__ diag,
+                                                      float* __restrict__ val) {
+    int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    float s[16];
+    for (int q = 0; q < 16; ++q) s[q] = 0.0f;
+    for (int k = cptr[e]; k < cptr[e + 1]; ++k) {
+        int src = clist[k];
+        const float* b = src < 0 ? fdiag + 16 * (size_t)(~src) : fval + 16 * (size_t)src;
+        for (int q = 0; q < 16; ++q) s[q] += b[q];
+    }
+    float* o = e < nc ? diag + 16 * (size_t)e : val + 16 * (size_t)(e - nc);
+    for (int q = 0; q < 16; ++q) o[q] = s[q];
+}
+
+// ------------------------------------------------------------- block SpMV --
+// OP 0: y = A x;  OP 1: y = b - A x
+template <class T, int OP>
+__global__ void __launch_bounds__(TPB) k_bsr(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                             const T* __restrict__ val, const T* __restrict__ diag,
+                                             const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T s[4] = {0, 0, 0, 0};
+    const T* d = diag + 16 * (size_t)i;
+    const T* xi = x + 4 * (size_t)i;
+    for (int r = 0; r < 4; ++r) s[r] += d[4 * r] * xi[0] + d[4 * r + 1] * xi[1] + d[4 * r + 2] * xi[2] + d[4 * r + 3] * xi[3];
+    for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+        const T* a = val + 16 * (size_t)e;
+        const T* xj = x + 4 * (size_t)col[e];
+        T x0 = xj[0], x1 = xj[1], x2 = xj[2], x3 = xj[3];
+        for (int r = 0; r < 4; ++r) s[r] += a[4 * r] * x0 + a[4 * r + 1] * x1 + a[4 * r + 2] * x2 + a[4 * r + 3] * x3;
+    }
+    T* yo = y + 4 * (size_t)i;
+    if (OP == 0) { for (int r = 0; r < 4; ++r) yo[r] = s[r]; }
+    else { const T* bi = b + 4 * (size_t)i; for (int r = 0; r < 4; ++r) yo[r] = bi[r] - s[r]; }
+}
+
+// damped block Jacobi: xo = x + w D^-1 (b - A x); x == null: start from 0
+__global__ void __launch_bounds__(TPB) k_bjac(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+                                              const float* __restrict__ val, const float* __restrict__ diag,
+                                              const float* __restrict__ dinv, const float* __restrict__ x,
+                                              const float* __restrict__ b, float w, float* __restrict__ xo) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* bi = b + 4 * (size_t)i;
+    float r[4] = {bi[0], bi[1], bi[2], bi[3]};
+    float xi[4] = {0, 0, 0, 0};
+    if (x) {
+        for (int q = 0; q < 4; ++q) xi[q] = x[4 * (size_t)i + q];
+        const float* d = diag + 16 * (size_t)i;
+        for (int q = 0; q < 4; ++q)
+            r[q] -= d[4 * q] * xi[0] + d[4 * q + 1] * xi[1] + d[4 * q + 2] * xi[2] + d[4 * q + 3] * xi[3];
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+            const float* a = val + 16 * (size_t)e;
+            const float* xj = x + 4 * (size_t)col[e];
+            float x0 = xj[0], x1 = xj[1], x2 = xj[2], x3 = xj[3];
+            for (int q = 0; q < 4; ++q) r[q] -= a[4 * q] * x0 + a[4 * q + 1] * x1 + a[4 * q + 2] * x2 + a[4 * q + 3] * x3;
+        }
+    }
+    const float* m = dinv + 16 * (size_t)i;
+    for (int q = 0; q < 4; ++q)
+        xo[4 * (size_t)i + q] = xi[q] + w * (m[4 * q] * r[0] + m[4 * q + 1] * r[1] + m[4 * q + 2] * r[2] + m[4 * q + 3] * r[3]);
+}
+
+// restriction by aggregate sums (tentative P^T), gathered per coarse row
+__global__ void __launch_bounds__(TPB) k_brestrict(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
+                                                   const float* __restrict__ rf, float* __restrict__ bc) {
+    int I = xcd_block() * blockDim.x + threadIdx.x;
+    if (I >= nc) return;
+    float s[4] = {0, 0, 0, 0};
+    for (int k = mptr[I]; k < mptr[I + 1]; ++k) {
+        const float* r = rf + 4 * (size_t)mem[k];
+        for (int q = 0; q < 4; ++q) s[q] += r[q];
+    }
+    for (int q = 0; q < 4; ++q) bc[4 * (size_t)I + q] = s[q];
+}
+
+// x += P xc
+__global__ void __launch_bounds__(TPB) k_bprolong(int n, const int* __restrict__ agg, const float* __restrict__ xc,
+                                                  float* __restrict__ x) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* c = xc + 4 * (size_t)agg[i];
+    for (int q = 0; q < 4; ++q) x[4 * (size_t)i + q] += c[q];
+}
+
+__global__ void __launch_bounds__(TPB) k_d2f(size_t n, const double* __restrict__ a, float* __restrict__ o) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (float)a[i];
+}
+__global__ void __launch_bounds__(TPB) k_f2d(size_t n, const float* __restrict__ a, double* __restrict__ o) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (double)a[i];
+}
+
+// ----------------------------------------------------------- Krylov kernels --
+// two dot products, block partials (fixed order): part[2b] = a.b, part[2b+1] = c.d
+__global__ void __launch_bounds__(TPB) k_dot2v(size_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                               const double* __restrict__ c, const double* __restrict__ d,
+                                               double* __restrict__ part) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    for (size_t k = i; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        s0 += a[k] * b[k];
+        if (c) s1 += c[k] * d[k];
+    }
+    block_sum2(s0, s1, part);
+}
+
+// y = a x + b y + c z   (z may be null)
+__global__ void __launch_bounds__(TPB) k_axpbypcz(size_t n, double a, const double* __restrict__ x, double b,
+                                                  double* __restrict__ y, double cz, const double* __restrict__ z) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = a * x[i] + b * y[i];
+    if (z) v += cz * z[i];
+    y[i] = v;
+}
+
+// P += urf_p x0, U += urf_u x(1..3), clipped; partial sums of |dx|^2 and |x_new|^2
+__global__ void __launch_bounds__(TPB) k_cpl_update(int nc, double urf_p, double urf_u, double lim,
+                                                    const double* __restrict__ x, double* __restrict__ P,
+                                                    double* __restrict__ U, double* __restrict__ V,
+                                                    double* __restrict__ W, double* __restrict__ part) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double nd = 0.0, np = 0.0;
+    if (i < nc) {
+        const double* xi = x + 4 * (size_t)i;
+        double* f[4] = {P, U, V, W};
+        for (int q = 0; q < 4; ++q) {
+            double d = (q == 0 ? urf_p : urf_u) * xi[q];
+            double v = f[q][i] + d;
+            if (v <= -lim) v = -lim;
+            if (v >= lim) v = lim;
+            f[q][i] = v;
+            nd += d * d; np += v * v;
+        }
+    }
+    block_sum2(nd, np, part);
+}
+
+// --------------------------------------------------------------- host side --
+int nb_of(size_t n) { return (int)std::min<size_t>((n + TPB - 1) / TPB, 2048); }
+
+double2 dot2(semo_ctx* c, Coupled& C, size_t n, const double* a, const double* b, const double* cc,
+             const double* d) {
+    int nb = nb_of(n);
+    hipLaunchKernelGGL(k_dot2v, dim3(nb), dim3(TPB), 0, c->stream, n, a, b, cc, d, C.part.p);
+    std::vector<double> h(2 * (size_t)nb);
+    SEMO_HIP(hipMemcpyAsync(h.data(), C.part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    double2 r{0.0, 0.0};
+    for (int k = 0; k < nb; ++k) { r.x += h[2 * k]; r.y += h[2 * k + 1]; }
+    return r;
+}
+
+void axpbypcz(semo_ctx* c, size_t n, double a, const double* x, double b, double* y, double cz = 0.0,
+              const double* z = nullptr) {
+    hipLaunchKernelGGL(k_axpbypcz, dim3(nblk((long)n)), dim3(TPB), 0, c->stream, n, a, x, b, y, cz, z);
+}
+
+void bjac(semo_ctx* c, BLevel& L, const float* x, float* xo) {
+    hipLaunchKernelGGL(k_bjac, dim3(nblk(L.n)), dim3(TPB), 0, c->stream, L.n, L.ptr.p, L.col.p, L.val.p, L.diag.p,
+                       L.dinv.p, x, L.b.p, OMEGA, xo);
+}
+
+// V-cycle on level l for the right-hand side in lv[l].b; result in lv[l].x
+void vcycle(semo_ctx* c, Coupled& C, size_t l) {
+    BLevel& L = C.lv[l];
+    bool last = l + 1 == C.lv.size();
+    int nsw = last ? NCOARSE : NSWEEP;
+    // x = 0 start, then sweeps alternating between x and x2
+    bjac(c, L, nullptr, L.x.p);
+    for (int s = 1; s < nsw; ++s) {
+        bjac(c, L, L.x.p, L.x2.p);
+        std::swap(L.x, L.x2);
+    }
+    if (last) return;
+    BLevel& Cn = C.lv[l + 1];
+    hipLaunchKernelGGL((k_bsr<float, 1>), dim3(nblk(L.n)), dim3(TPB), 0, c->stream, L.n, L.ptr.p, L.col.p, L.val.p,
+                       L.diag.p, L.x.p, L.b.p, L.r.p);
+    hipLaunchKernelGGL(k_brestrict, dim3(nblk(Cn.n)), dim3(TPB), 0, c->stream, Cn.n, Cn.mptr.p, Cn.mem.p, L.r.p,
+                       Cn.b.p);
+    vcycle(c, C, l + 1);
+    hipLaunchKernelGGL(k_bprolong, dim3(nblk(L.n)), dim3(TPB), 0, c->stream, L.n, Cn.agg.p, Cn.x.p, L.x.p);
+    for (int s = 0; s < nsw; ++s) {
+        bjac(c, L, L.x.p, L.x2.p);
+        std::swap(L.x, L.x2);
+    }
+}
+
+// z = M^-1 r (FP64 in/out, FP32 V-cycle)
+void precond(semo_ctx* c, Coupled& C, const double* r, double* z) {
+    BLevel& L = C.lv[0];
+    size_t n4 = 4 * (size_t)L.n;
+    hipLaunchKernelGGL(k_d2f, dim3(nblk((long)n4)), dim3(TPB), 0, c->stream, n4, r, L.b.p);
+    vcycle(c, C, 0);
+    hipLaunchKernelGGL(k_f2d, dim3(nblk((long)n4)), dim3(TPB), 0, c->stream, n4, L.x.p, z);
+}
+
+void spmv64(semo_ctx* c, Coupled& C, const double* x, double* y) {
+    hipLaunchKernelGGL((k_bsr<double, 0>), dim3(nblk(C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p, C.col.p,
+                       C.val.p, C.diag.p, x, (const double*)nullptr, y);
+}
+
+// fine pattern, face -> block maps and the aggregation hierarchy (once per mesh)
+void coupled_setup(semo_ctx* c, Coupled& C) {
+    int n = c->nc, nif = c->nif;
+    C.n = n;
+    std::vector<int> ptr(n + 1, 0);
+    for (int f = 0; f < nif; ++f) { ptr[c->h_owner[f] + 1]++; ptr[c->h_nbr[f] + 1]++; }
+    for (int i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+    std::vector<std::pair<int, int>> ent(ptr[n]);
+    {
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        for (int f = 0; f < nif; ++f) {
+            ent[pos[c->h_owner[f]]++] = {c->h_nbr[f], f};
+            ent[pos[c->h_nbr[f]]++] = {c->h_owner[f], ~f};
+        }
+    }
+    std::vector<int> col(ptr[n]), fso(std::max(nif, 1)), fsn(std::max(nif, 1));
+    for (int i = 0; i < n; ++i) {
+        std::sort(ent.begin() + ptr[i], ent.begin() + ptr[i + 1]);
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+            col[k] = ent[k].first;
+            int f = ent[k].second;
+            if (f >= 0) fso[f] = k; else fsn[~f] = k;
+        }
+    }
+    C.h_ptr = ptr; C.h_col = col;
+    hipStream_t s = c->stream;
+    C.ptr.upload(ptr, s); C.col.upload(col, s); C.fslot_own.upload(fso, s); C.fslot_nbr.upload(fsn, s);
+    size_t nnzb = col.size();
+    C.val.alloc(16 * std::max<size_t>(nnzb, 1)); C.diag.alloc(16 * (size_t)n); C.rhs.alloc(4 * (size_t)n);
+    C.x.alloc(4 * (size_t)n); C.x.zero(s);
+    size_t nv = c->pat.col.size();
+    for (DBuf<double>* b : {&C.mval, &C.pval}) b->alloc(std::max<size_t>(nv, 1));
+    for (DBuf<double>* b : {&C.mdiag, &C.pdiag, &C.prhs}) b->alloc(n);
+    C.mrhs.alloc(3 * (size_t)n);
+    C.kv.alloc(8 * 4 * (size_t)n);
+    C.part.alloc(2 * 2048 + 64);
+
+    // aggregation hierarchy from the scalar pressure operator (geometry only)
+    std::vector<double> pv(nv), pd(n);
+    c->Amat.val.download(pv.data(), nv, s);
+    c->Amat.diag.download(pd.data(), n, s);
+    SEMO_HIP(hipStreamSynchronize(s));
+    std::vector<int> sptr(n + 1, 0), scol;
+    std::vector<double> sval;
+    {
+        // scalar operator in CSR from the sliced-ELL pattern + the face slots
+        std::vector<std::vector<std::pair<int, double>>> rows(n);
+        for (int i = 0; i < n; ++i) rows[i].push_back({i, pd[i]});
+        for (int f = 0; f < nif; ++f) {
+            rows[c->h_owner[f]].push_back({c->h_nbr[f], pv[c->h_slot_own[f]]});
+            rows[c->h_nbr[f]].push_back({c->h_owner[f], pv[c->h_slot_nbr[f]]});
+        }
+        for (int i = 0; i < n; ++i) {
+            std::sort(rows[i].begin(), rows[i].end());
+            for (auto& q : rows[i]) { scol.push_back(q.first); sval.push_back(q.second); }
+            sptr[i + 1] = (int)scol.size();
+        }
+    }
+    std::vector<std::vector<int>> maps = aggregate_hierarchy(n, sptr, scol, sval, 256);
+    C.lv.clear();
+    C.lv.resize(maps.size() + 1);
+    // level 0: the fine pattern
+    {
+        BLevel& L = C.lv[0];
+        L.n = n; L.nnzb = (int64_t)nnzb;
+        L.ptr.upload(ptr, s); L.col.upload(col, s);
+        L.val.alloc(16 * std::max<size_t>(nnzb, 1)); L.diag.alloc(16 * (size_t)n); L.dinv.alloc(16 * (size_t)n);
+    }
+    std::vector<int> fptr = ptr, fcol = col;
+    for (size_t l = 0; l < maps.size(); ++l) {
+        const std::vector<int>& agg = maps[l];
+        int nf = (int)agg.size();
+        int na = 0;
+        for (int a : agg) na = std::max(na, a + 1);
+        BLevel& L = C.lv[l + 1];
+        L.n = na;
+        // members
+        std::vector<int> mptr(na + 1, 0), mem(nf);
+        for (int i = 0; i < nf; ++i) mptr[agg[i] + 1]++;
+        for (int I = 0; I < na; ++I) mptr[I + 1] += mptr[I];
+        {
+            std::vector<int> pos(mptr.begin(), mptr.end() - 1);
+            for (int i = 0; i < nf; ++i) mem[pos[agg[i]]++] = i;
+        }
+        // coarse pattern and contribution lists
+        std::vector<std::vector<std::pair<int, int>>> contrib(na);   // (coarse col, fine source)
+        for (int i = 0; i < nf; ++i) {
+            int I = agg[i];
+            contrib[I].push_back({I, ~i});
+            for (int e = fptr[i]; e < fptr[i + 1]; ++e) contrib[I].push_back({agg[fcol[e]], e});
+        }
+        std::vector<int> cptr_row(na + 1, 0), ccol;
+        std::vector<std::vector<int>> lists_diag(na), lists_off;
+        std::vector<int> off_row;
+        for (int I = 0; I < na; ++I) {
+            auto& v = contrib[I];
+            std::sort(v.begin(), v.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                return a.first < b.first;
+            });
+            for (size_t k = 0; k < v.size();) {
+                size_t k2 = k;
+                std::vector<int> lst;
+                while (k2 < v.size() && v[k2].first == v[k].first) lst.push_back(v[k2++].second);
+                std::sort(lst.begin(), lst.end());
+                if (v[k].first == I) lists_diag[I] = lst;
+                else { ccol.push_back(v[k].first); lists_off.push_back(lst); }
+                k = k2;
+            }
+            cptr_row[I + 1] = (int)ccol.size();
+        }
+        std::vector<int> cptr(1, 0), clist;
+        for (int I = 0; I < na; ++I) {
+            clist.insert(clist.end(), lists_diag[I].begin(), lists_diag[I].end());
+            cptr.push_back((int)clist.size());
+        }
+        for (auto& lst : lists_off) {
+            clist.insert(clist.end(), lst.begin(), lst.end());
+            cptr.push_back((int)clist.size());
+        }
+        L.nnzb = (int64_t)ccol.size();
+        L.ptr.upload(cptr_row, s);
+        L.col.upload(ccol.empty() ? std::vector<int>{0} : ccol, s);
+        L.mptr.upload(mptr, s); L.mem.upload(mem, s); L.agg.upload(agg, s);
+        L.cptr.upload(cptr, s); L.clist.upload(clist, s);
+        L.val.alloc(16 * std::max<size_t>(ccol.size(), 1)); L.diag.alloc(16 * (size_t)na);
+        L.dinv.alloc(16 * (size_t)na);
+        fptr = cptr_row; fcol = ccol;
+    }
+    for (BLevel& L : C.lv)
+        for (DBuf<float>* b : {&L.x, &L.x2, &L.b, &L.r}) b->alloc(4 * (size_t)L.n + 64);
+    SEMO_HIP(hipStreamSynchronize(s));
+    C.built = true;
+}
+
+// FP32 copies, Galerkin sums and block inverses of every level
+void refresh_levels(semo_ctx* c, Coupled& C) {
+    BLevel& L0 = C.lv[0];
+    size_t nv = 16 * (size_t)L0.nnzb, nd = 16 * (size_t)L0.n;
+    if (nv) hipLaunchKernelGGL(k_cpl_f32, dim3(nblk((long)nv)), dim3(TPB), 0, c->stream, nv, C.val.p, L0.val.p);
+    hipLaunchKernelGGL(k_cpl_f32, dim3(nblk((long)nd)), dim3(TPB), 0, c->stream, nd, C.diag.p, L0.diag.p);
+    hipLaunchKernelGGL((k_cpl_dinv<double>), dim3(nblk(L0.n)), dim3(TPB), 0, c->stream, L0.n, C.diag.p, L0.dinv.p);
+    for (size_t l = 1; l < C.lv.size(); ++l) {
+        BLevel& F = C.lv[l - 1];
+        BLevel& L = C.lv[l];
+        int64_t ne = L.n + L.nnzb;
+        hipLaunchKernelGGL(k_cpl_galerkin, dim3(nblk((long)ne)), dim3(TPB), 0, c->stream, ne, L.n, L.cptr.p,
+                           L.clist.p, F.diag.p, F.val.p, L.diag.p, L.val.p);
+        hipLaunchKernelGGL((k_cpl_dinv<float>), dim3(nblk(L.n)), dim3(TPB), 0, c->stream, L.n, L.diag.p, L.dinv.p);
+    }
+}
+
+}  // namespace
+
+// momentum + pressure assembly of the current state, then the block system
+void assemble_coupled(semo_ctx* c) {
+    if (!c->cpl) c->cpl = new Coupled();
+    Coupled& C = *c->cpl;
+    int n = c->nc;
+    size_t nv = c->pat.col.size();
+    assemble_pressure(c);
+    if (!C.built) coupled_setup(c, C);
+    copy_d(c, C.pval.p, c->Amat.val.p, nv);
+    copy_d(c, C.pdiag.p, c->Amat.diag.p, n);
+    copy_d(c, C.prhs.p, c->Bvec.p, n);
+    assemble_momentum(c);
+    copy_d(c, C.mval.p, c->Amat.val.p, nv);
+    copy_d(c, C.mdiag.p, c->Amat.diag.p, n);
+    copy_d(c, C.mrhs.p, c->Bvec.p, 3 * (size_t)n);
+    hipLaunchKernelGGL(k_cpl_row, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->nif, c->nbf, c->d_cf_sl.p,
+                       c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p,
+                       c->f_wC.p, c->d_slot_own.p, c->d_slot_nbr.p, C.fslot_own.p, C.fslot_nbr.p, C.mval.p,
+                       C.mdiag.p, C.pval.p, C.pdiag.p, C.mrhs.p, C.prhs.p, c->U.p, c->V.p, c->W.p, C.ptr.p, C.val.p,
+                       C.diag.p, C.rhs.p);
+    refresh_levels(c, C);
+    c->last_eq = 3;
+}
+
+void coupled_spmv(semo_ctx* c, const double* x, double* y) {
+    SEMO_CHECK(c->cpl && c->cpl->built, SEMO_ESTATE, "no coupled system assembled");
+    spmv64(c, *c->cpl, x, y);
+}
+
+// right-preconditioned BiCGStab, x0 = 0, stop at ||b - A x|| <= tol ||b||
+void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st) {
+    SEMO_CHECK(c->cpl && c->cpl->built, SEMO_ESTATE, "no coupled system assembled");
+    Coupled& C = *c->cpl;
+    size_t N = 4 * (size_t)C.n;
+    double* r = C.kv.p; double* rh = r + N; double* p = rh + N; double* v = p + N; double* ph = v + N;
+    double* s = ph + N; double* sh = s + N; double* t = sh + N;
+    SEMO_HIP(hipMemsetAsync(x, 0, N * sizeof(double), c->stream));
+    copy_d(c, r, b, N); copy_d(c, rh, b, N);
+    SEMO_HIP(hipMemsetAsync(p, 0, N * sizeof(double), c->stream));
+    SEMO_HIP(hipMemsetAsync(v, 0, N * sizeof(double), c->stream));
+    double bb = dot2(c, C, N, b, b, nullptr, nullptr).x;
+    int it = 0;
+    double rr = bb;
+    if (bb > 0.0) {
+        double nb = std::sqrt(bb), rho = 1.0, alpha = 1.0, omega = 1.0;
+        double tol = c->sp.tol_u;
+        for (it = 1; it <= c->sp.maxit_u; ++it) {
+            double rho1 = dot2(c, C, N, rh, r, nullptr, nullptr).x;
+            if (rho1 == 0.0) break;
+            double beta = (rho1 / rho) * (alpha / omega);
+            // p = r + beta (p - omega v)
+            axpbypcz(c, N, 1.0, r, beta, p, -beta * omega, v);
+            precond(c, C, p, ph);
+            spmv64(c, C, ph, v);
+            alpha = rho1 / dot2(c, C, N, rh, v, nullptr, nullptr).x;
+            // s = r - alpha v
+            copy_d(c, s, r, N);
+            axpbypcz(c, N, -alpha, v, 1.0, s);
+            precond(c, C, s, sh);
+            spmv64(c, C, sh, t);
+            double2 ts = dot2(c, C, N, t, s, t, t);
+            omega = ts.y > 0.0 ? ts.x / ts.y : 0.0;
+            // x += alpha ph + omega sh; r = s - omega t
+            axpbypcz(c, N, alpha, ph, 1.0, x, omega, sh);
+            copy_d(c, r, s, N);
+            axpbypcz(c, N, -omega, t, 1.0, r);
+            rho = rho1;
+            rr = dot2(c, C, N, r, r, nullptr, nullptr).x;
+            if (!(std::sqrt(rr) > tol * nb) || omega == 0.0) break;
+        }
+        it = std::min(it, c->sp.maxit_u);
+        // true residual
+        hipLaunchKernelGGL((k_bsr<double, 1>), dim3(nblk(C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p, C.col.p,
+                           C.val.p, C.diag.p, x, b, r);
+        rr = dot2(c, C, N, r, r, nullptr, nullptr).x;
+        C.last_relres = std::sqrt(rr) / nb;
+    } else {
+        C.last_relres = 0.0;
+    }
+    C.last_iters = it;
+    if (st) {
+        *st = semo_solve_stats{};
+        st->iters[0] = it;
+        st->rel_res[0] = C.last_relres;
+        st->n_levels = (int)C.lv.size();
+    }
+}
+
+void coupled_solve_assembled(semo_ctx* c, semo_solve_stats* st) {
+    SEMO_CHECK(c->cpl && c->cpl->built, SEMO_ESTATE, "no coupled system assembled");
+    coupled_solve(c, c->cpl->rhs.p, c->cpl->x.p, st);
+}
+
+double coupled_update(semo_ctx* c) {
+    Coupled& C = *c->cpl;
+    int nb = nblk(c->nc);
+    hipLaunchKernelGGL(k_cpl_update, dim3(nb), dim3(TPB), 0, c->stream, c->nc, c->cs.urf_p, c->cs.urf_mom,
+                       c->cs.limit, C.x.p, c->P.p, c->U.p, c->V.p, c->W.p, c->red.p);
+    std::vector<double> h(2 * (size_t)nb);
+    SEMO_HIP(hipMemcpyAsync(h.data(), c->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    double a = 0, b = 0;
+    for (int k = 0; k < nb; ++k) { a += h[2 * k]; b += h[2 * k + 1]; }
+    return b > 0.0 ? std::sqrt(a) / std::sqrt(b) : std::sqrt(a);
+}
+
+// block COO (diagonal blocks first, then row by row) and rhs, device numbering
+void coupled_export(semo_ctx* c, std::vector<int>& rows, std::vector<int>& cols, std::vector<double>& blocks,
+                    std::vector<double>& rhs) {
+    SEMO_CHECK(c->cpl && c->cpl->built, SEMO_ESTATE, "no coupled system assembled");
+    Coupled& C = *c->cpl;
+    int n = C.n;
+    size_t nnzb = C.h_col.size();
+    std::vector<double> d(16 * (size_t)n), v(16 * nnzb);
+    rhs.resize(4 * (size_t)n);
+    C.diag.download(d.data(), d.size(), c->stream);
+    if (nnzb) C.val.download(v.data(), v.size(), c->stream);
+    C.rhs.download(rhs.data(), rhs.size(), c->stream);
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    rows.clear(); cols.clear(); blocks.clear();
+    for (int i = 0; i < n; ++i) { rows.push_back(i); cols.push_back(i); }
+    blocks = d;
+    for (int i = 0; i < n; ++i)
+        for (int k = C.h_ptr[i]; k < C.h_ptr[i + 1]; ++k) { rows.push_back(i); cols.push_back(C.h_col[k]); }
+    blocks.insert(blocks.end(), v.begin(), v.end());
+}
+
+}  // namespace semo

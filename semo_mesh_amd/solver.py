@@ -150,10 +150,73 @@ class GpuSolver:
         self._ok(self._L.semo_calc_time_step(self._h, 1 if adjust else 0, C.byref(dt), C.byref(co)))
         return dt.value, co.value
 
+    def calcCoupledEq(self):
+        """One block-coupled (p', u', v', w') outer iteration (the design of
+        the reference's commented-out calcCoupledEq); returns the velocity
+        correction norm, solve statistics in last_stats."""
+        n = C.c_double(); st = _lib.SolveStats()
+        self._ok(self._L.semo_calc_coupled_eq(self._h, C.byref(n), C.byref(st)))
+        self.last_stats = st
+        return n.value
+
+    def coupledPressureBased(self, n_iter: int = 1):
+        """One time step with n_iter block-coupled outer iterations."""
+        norms = np.zeros(n_iter)
+        self._ok(self._L.semo_coupled_pressure_based(self._h, n_iter, _ptr(norms)))
+        return norms
+
+    def coupled_matrix(self):
+        """Block COO of the last assembled coupled system: rows, cols,
+        blocks (nnzb, 4, 4), rhs (4, n) variable-major."""
+        nnzb = C.c_int64(0)
+        self._ok(self._L.semo_coupled_matrix_get(self._h, C.byref(nnzb), None, None, None, None))
+        r = np.empty(nnzb.value, dtype=np.int32); c = np.empty(nnzb.value, dtype=np.int32)
+        b = np.empty((nnzb.value, 4, 4)); rhs = np.empty((4, self.mesh.n_cells))
+        self._ok(self._L.semo_coupled_matrix_get(self._h, C.byref(nnzb), _ptr(r), _ptr(c), _ptr(b), _ptr(rhs)))
+        return r, c, b, rhs
+
+    def coupled_spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(4, -1)
+        y = np.empty_like(x)
+        self._ok(self._L.semo_coupled_spmv(self._h, _ptr(x), _ptr(y)))
+        return y
+
+    def coupled_solve(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(4, -1)
+        x = np.empty_like(b); st = _lib.SolveStats()
+        self._ok(self._L.semo_coupled_solve(self._h, _ptr(b), _ptr(x), C.byref(st)))
+        self.last_stats = st
+        return x
+
     def incompressiblePressureBased(self, n_correctors: int = 1):
         norms = np.zeros(2 * n_correctors)
         self._ok(self._L.semo_incompressible_pressure_based(self._h, n_correctors, _ptr(norms)))
         return norms
+
+    # ---- discrete adjoint (no counterpart in the reference) -------------------------------
+    def _patch_weights(self, weights):
+        w = np.zeros(len(self.mesh.patch_names))
+        for name, v in weights.items():
+            w[self.mesh.patch(name)] = v
+        return w
+
+    def objective(self, weights: dict) -> float:
+        """J = sum over patches of weight * sum_f A_f p_f (face pressure of
+        the boundary reconstruction) for the current pressure field."""
+        w = self._patch_weights(weights)
+        J = C.c_double()
+        self._ok(self._L.semo_objective_pressure(self._h, _ptr(w), C.byref(J)))
+        return J.value
+
+    def adjointAlphaP(self, n_iter: int, weights: dict):
+        """Runs n_iter outer iterations (as incompressiblePressureBased) and
+        returns (J of the final state, dJ/dalpha_p per cell) by reverse-mode
+        differentiation of the discrete iterations."""
+        w = self._patch_weights(weights)
+        J = C.c_double()
+        g = np.empty(self.mesh.n_cells)
+        self._ok(self._L.semo_adjoint_alpha_p(self._h, n_iter, _ptr(w), C.byref(J), _ptr(g)))
+        return J.value, g
 
     def solve_stats(self, eq: str):
         """Iterations / final relative residuals / AMG levels of the last
@@ -164,7 +227,8 @@ class GpuSolver:
                     setup_ms=st.setup_ms)
 
     def assemble(self, eq: str):
-        self._ok(self._L.semo_assemble(self._h, {"momentum": 0, "pressure": 1, "volume_fraction": 2}[eq]))
+        self._ok(self._L.semo_assemble(self._h, {"momentum": 0, "pressure": 1, "volume_fraction": 2,
+                                                 "coupled": 3}[eq]))
 
     def matrix(self, eq: str):
         """COO (rows, cols, vals) and right-hand sides of the last assembly."""
