@@ -96,3 +96,14 @@ def dam_break_case(dt: float = 1e-4, n_vof: int = 1, urf_vf: float = 1.0) -> Cas
                           vf_type=BC_INLET_OUTLET, vf_value=0.0)
     return Case(rho=(1000.0, 1.2), mu=(1e-3, 1.8e-5), g=(0.0, 0.0, -9.81), dt=dt,
                 urf_vf=urf_vf, n_vof=n_vof, bcs=bcs)
+
+
+def channel_case(u_in: float, dt: float, rho: float = 1.177, mu: float = 1.85e-5) -> Case:
+    """Channel flow: inlet (xmin) fixedValue U = (u_in, 0, 0), outlet (xmax)
+    fixedValue p = 0 with zeroGradient U, the four side walls noSlip;
+    single phase (air at 300 K by default)."""
+    wall = PatchBC()
+    bcs = {n: wall for n in ("ymin", "ymax", "zmin", "zmax")}
+    bcs["xmin"] = PatchBC(u_type=BC_FIXED_VALUE, u_value=(u_in, 0.0, 0.0))
+    bcs["xmax"] = PatchBC(p_type=BC_FIXED_VALUE, p_value=0.0, u_type=BC_ZERO_GRADIENT)
+    return Case(rho=(rho, rho), mu=(mu, mu), dt=dt, bcs=bcs)

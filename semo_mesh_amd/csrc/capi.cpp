@@ -211,6 +211,8 @@ int semo_ctx_destroy(semo_ctx* c) {
     c->adj = nullptr;
     coupled_free(c->cpl);
     c->cpl = nullptr;
+    if (c->pin) (void)hipHostFree(c->pin);
+    for (hipEvent_t e : c->pev) if (e) (void)hipEventDestroy(e);
     delete c;
     if (own) (void)hipStreamDestroy(s);
     return SEMO_OK;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -128,6 +129,10 @@ struct DevSell {
     int lanes = 0;              // 0: sliced ELL; 4..64: CSR, lanes per row
     int64_t nnz = 0;            // real entries incl. diagonal
     DBuf<int> sl_ptr, col;
+    // sliced ELL of a square operator whose columns all lie within +-32767
+    // of their row (RCM bandwidth): column - row as int16, read instead of
+    // col (2 B per entry instead of 4); empty otherwise
+    DBuf<int16_t> dcol;
     DBuf<T> val, diag;
     size_t padded() const { return col.n; }
 };
@@ -248,11 +253,54 @@ struct semo_ctx {
     semo::DBuf<double> kv64;                   // Krylov work (fp64)
     semo::DBuf<float> kv32;
     semo_solve_stats last_stats_mom{}, last_stats_p{}, last_stats_vf{};
+    double* pin = nullptr;                     // pinned Krylov residual slots [4]
+    hipEvent_t pev[4] = {};
     semo::AdjBufs* adj = nullptr;
     semo::Coupled* cpl = nullptr;
 };
 
 namespace semo {
+// ------------------------------------------------- Krylov convergence tests --
+// Iteration k's squared residual is copied into pinned slot k % 4 behind an
+// event.  ConvCheck reads it only once iteration k + 1 has been issued, so the
+// device has queued work while the host waits -- except near convergence:
+// when the residual history predicts that iteration k meets the tolerance,
+// k is read at once (a short stall instead of a wasted iteration).
+inline void pipe_init(semo_ctx* c) {
+    if (c->pin) return;
+    SEMO_HIP(hipHostMalloc((void**)&c->pin, 4 * sizeof(double), hipHostMallocDefault));
+    for (hipEvent_t& e : c->pev) SEMO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+inline void pipe_push(semo_ctx* c, const double* dscalar, int k) {
+    SEMO_HIP(hipMemcpyAsync(c->pin + (k & 3), dscalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipEventRecord(c->pev[k & 3], c->stream));
+}
+inline double pipe_pop(semo_ctx* c, int k) {
+    SEMO_HIP(hipEventSynchronize(c->pev[k & 3]));
+    return ((volatile double*)c->pin)[k & 3];
+}
+struct ConvCheck {
+    semo_ctx* c;
+    double lim;                 // stop when the squared residual <= lim (NaN stops too)
+    int read = 0;               // last iteration whose residual was read
+    double r1, r2 = -1.0;       // last two residuals read (r1 starts as |b|^2)
+    ConvCheck(semo_ctx* c_, double lim_, double bb) : c(c_), lim(lim_), r1(bb) { pipe_init(c_); }
+    void take(int k) { double v = pipe_pop(c, k); r2 = r1; r1 = v; read = k; }
+    // after iteration `it` pushed its residual: true = stop
+    bool done(int it) {
+        if (read < it - 1) {
+            take(it - 1);
+            if (!(r1 > lim)) return true;         // it - 1 converged; `it` ran one step further
+        }
+        double pred = r1 * (r2 > 0.0 ? std::min(r1 / r2, 1.0) : 0.1);
+        if (pred <= 4.0 * lim) {
+            take(it);
+            return !(r1 > lim);
+        }
+        return false;
+    }
+};
+
 // mesh.cpp
 void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder);
 void genrcm(int n, const int* ptr, const int* adj, int* perm, int* inv);

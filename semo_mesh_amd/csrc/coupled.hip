@@ -61,7 +61,7 @@ struct Coupled {
     DBuf<double> x;                             // [n][4] last solution
     std::vector<BLevel> lv;                     // lv[0] = fine level FP32 copy
     DBuf<double> kv;                            // Krylov vectors
-    DBuf<double> part;
+    DBuf<double> part, red, sc;                 // host-summed partials; device partials, scalars
     int last_iters = 0;
     double last_relres = 0.0;
 };
@@ -73,6 +73,12 @@ namespace {
 constexpr int NSWEEP = 2;          // pre / post smoothing sweeps
 constexpr int NCOARSE = 8;         // sweeps on the coarsest level
 constexpr float OMEGA = 0.6f;      // block-Jacobi damping
+#ifndef SEMO_CPL_OVERCORR
+#define SEMO_CPL_OVERCORR 1.5f
+#endif
+// coarse-correction scaling of plain aggregation (AMGCL's over_interp,
+// lib/amgcl/coarsening/aggregation.hpp:74, default 1.5)
+constexpr float OVERCORR = SEMO_CPL_OVERCORR;
 
 // ---------------------------------------------------------------- assembly --
 __global__ void __launch_bounds__(TPB) k_cpl_row(
@@ -250,13 +256,13 @@ __global__ void __launch_bounds__(TPB) k_brestrict(int nc, const int* __restrict
     for (int q = 0; q < 4; ++q) bc[4 * (size_t)I + q] = s[q];
 }
 
-// x += P xc
+// x += w P xc (w: over-correction of the piecewise-constant interpolation)
 __global__ void __launch_bounds__(TPB) k_bprolong(int n, const int* __restrict__ agg, const float* __restrict__ xc,
-                                                  float* __restrict__ x) {
+                                                  float w, float* __restrict__ x) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* c = xc + 4 * (size_t)agg[i];
-    for (int q = 0; q < 4; ++q) x[4 * (size_t)i + q] += c[q];
+    for (int q = 0; q < 4; ++q) x[4 * (size_t)i + q] += w * c[q];
 }
 
 __global__ void __launch_bounds__(TPB) k_d2f(size_t n, const double* __restrict__ a, float* __restrict__ o) {
@@ -282,16 +288,6 @@ __global__ void __launch_bounds__(TPB) k_dot2v(size_t n, const double* __restric
     block_sum2(s0, s1, part);
 }
 
-// y = a x + b y + c z   (z may be null)
-__global__ void __launch_bounds__(TPB) k_axpbypcz(size_t n, double a, const double* __restrict__ x, double b,
-                                                  double* __restrict__ y, double cz, const double* __restrict__ z) {
-    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double v = a * x[i] + b * y[i];
-    if (z) v += cz * z[i];
-    y[i] = v;
-}
-
 // P += urf_p x0, U += urf_u x(1..3), clipped; partial sums of |dx|^2 and |x_new|^2
 __global__ void __launch_bounds__(TPB) k_cpl_update(int nc, double urf_p, double urf_u, double lim,
                                                     const double* __restrict__ x, double* __restrict__ P,
@@ -314,8 +310,85 @@ __global__ void __launch_bounds__(TPB) k_cpl_update(int nc, double urf_p, double
     block_sum2(nd, np, part);
 }
 
+// device scalars of the block BiCGStab (zero denominators give zero steps,
+// so that an iteration issued past convergence leaves x unchanged)
+enum { C_RHO = 0, C_ALPHA, C_OMEGA, C_BETA, C_RR, C_BB, C_N };
+enum { F_STORE_BB = 0, F_RHO, F_ALPHA, F_OMEGA, F_RR };
+
+__global__ void __launch_bounds__(TPB) k_cfinish(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                                 int what) {
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < nb; i += TPB) { a0 += part[2 * i]; a1 += part[2 * i + 1]; }
+    __shared__ double sh[2][TPB];
+    sh[0][threadIdx.x] = a0; sh[1][threadIdx.x] = a1;
+    __syncthreads();
+    for (int w = TPB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { sh[0][threadIdx.x] += sh[0][threadIdx.x + w]; sh[1][threadIdx.x] += sh[1][threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double d0 = sh[0][0], d1 = sh[1][0];
+    switch (what) {
+    case F_STORE_BB: sc[C_BB] = d0; sc[C_RR] = d0; sc[C_RHO] = 1.0; sc[C_ALPHA] = 1.0; sc[C_OMEGA] = 1.0; break;
+    case F_RHO:      // d0 = rh.r
+        sc[C_BETA] = (sc[C_RHO] != 0.0 && sc[C_OMEGA] != 0.0) ? (d0 / sc[C_RHO]) * (sc[C_ALPHA] / sc[C_OMEGA]) : 0.0;
+        sc[C_RHO] = d0;
+        break;
+    case F_ALPHA: sc[C_ALPHA] = d0 != 0.0 ? sc[C_RHO] / d0 : 0.0; break;     // d0 = rh.v
+    case F_OMEGA: sc[C_OMEGA] = d1 != 0.0 ? d0 / d1 : 0.0; break;            // d0 = t.s, d1 = t.t
+    case F_RR: sc[C_RR] = d0; break;
+    default: break;
+    }
+}
+
+// p = r + beta (p - omega v)
+__global__ void __launch_bounds__(TPB) k_cb_p(size_t n, const double* __restrict__ sc, const double* __restrict__ r,
+                                              const double* __restrict__ v, double* __restrict__ p) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p[i] = r[i] + sc[C_BETA] * (p[i] - sc[C_OMEGA] * v[i]);
+}
+// s = r - alpha v
+__global__ void __launch_bounds__(TPB) k_cb_s(size_t n, const double* __restrict__ sc, const double* __restrict__ r,
+                                              const double* __restrict__ v, double* __restrict__ s) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s[i] = r[i] - sc[C_ALPHA] * v[i];
+}
+// x += alpha ph + omega sh; r = s - omega t; partials of r.r
+__global__ void __launch_bounds__(TPB) k_cb_x(size_t n, const double* __restrict__ sc, const double* __restrict__ ph,
+                                              const double* __restrict__ sh, const double* __restrict__ s,
+                                              const double* __restrict__ t, double* __restrict__ x,
+                                              double* __restrict__ r, double* __restrict__ part) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    double rr = 0.0;
+    if (i < n) {
+        double a = sc[C_ALPHA], w = sc[C_OMEGA];
+        x[i] += a * ph[i] + w * sh[i];
+        double ri = s[i] - w * t[i];
+        r[i] = ri;
+        rr = ri * ri;
+    }
+    block_sum2(rr, 0.0, part);
+}
+// r = rh = b, x = p = v = 0; partials of b.b
+__global__ void __launch_bounds__(TPB) k_cb_init(size_t n, const double* __restrict__ b, double* __restrict__ x,
+                                                 double* __restrict__ r, double* __restrict__ rh,
+                                                 double* __restrict__ p, double* __restrict__ v,
+                                                 double* __restrict__ part) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    double bb = 0.0;
+    if (i < n) {
+        double bi = b[i];
+        r[i] = bi; rh[i] = bi; x[i] = 0.0; p[i] = 0.0; v[i] = 0.0;
+        bb = bi * bi;
+    }
+    block_sum2(bb, 0.0, part);
+}
+
 // --------------------------------------------------------------- host side --
 int nb_of(size_t n) { return (int)std::min<size_t>((n + TPB - 1) / TPB, 2048); }
+
 
 double2 dot2(semo_ctx* c, Coupled& C, size_t n, const double* a, const double* b, const double* cc,
              const double* d) {
@@ -327,11 +400,6 @@ double2 dot2(semo_ctx* c, Coupled& C, size_t n, const double* a, const double* b
     double2 r{0.0, 0.0};
     for (int k = 0; k < nb; ++k) { r.x += h[2 * k]; r.y += h[2 * k + 1]; }
     return r;
-}
-
-void axpbypcz(semo_ctx* c, size_t n, double a, const double* x, double b, double* y, double cz = 0.0,
-              const double* z = nullptr) {
-    hipLaunchKernelGGL(k_axpbypcz, dim3(nblk((long)n)), dim3(TPB), 0, c->stream, n, a, x, b, y, cz, z);
 }
 
 void bjac(semo_ctx* c, BLevel& L, const float* x, float* xo) {
@@ -357,7 +425,8 @@ void vcycle(semo_ctx* c, Coupled& C, size_t l) {
     hipLaunchKernelGGL(k_brestrict, dim3(nblk(Cn.n)), dim3(TPB), 0, c->stream, Cn.n, Cn.mptr.p, Cn.mem.p, L.r.p,
                        Cn.b.p);
     vcycle(c, C, l + 1);
-    hipLaunchKernelGGL(k_bprolong, dim3(nblk(L.n)), dim3(TPB), 0, c->stream, L.n, Cn.agg.p, Cn.x.p, L.x.p);
+    hipLaunchKernelGGL(k_bprolong, dim3(nblk(L.n)), dim3(TPB), 0, c->stream, L.n, Cn.agg.p, Cn.x.p, OVERCORR,
+                       L.x.p);
     for (int s = 0; s < nsw; ++s) {
         bjac(c, L, L.x.p, L.x2.p);
         std::swap(L.x, L.x2);
@@ -414,6 +483,8 @@ void coupled_setup(semo_ctx* c, Coupled& C) {
     C.mrhs.alloc(3 * (size_t)n);
     C.kv.alloc(8 * 4 * (size_t)n);
     C.part.alloc(2 * 2048 + 64);
+    C.red.alloc(2 * (size_t)nblk(4 * (long)n) + 64);
+    C.sc.alloc(16);
 
     // aggregation hierarchy from the scalar pressure operator (geometry only)
     std::vector<double> pv(nv), pd(n);
@@ -560,52 +631,55 @@ void coupled_spmv(semo_ctx* c, const double* x, double* y) {
     spmv64(c, *c->cpl, x, y);
 }
 
-// right-preconditioned BiCGStab, x0 = 0, stop at ||b - A x|| <= tol ||b||
+// right-preconditioned BiCGStab, x0 = 0, stop at ||b - A x|| <= tol ||b||.
+// The Krylov scalars stay on the device; the host reads one residual per
+// iteration, one iteration behind (pinned slot + event), so the stream never
+// drains while it waits.
 void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st) {
     SEMO_CHECK(c->cpl && c->cpl->built, SEMO_ESTATE, "no coupled system assembled");
     Coupled& C = *c->cpl;
     size_t N = 4 * (size_t)C.n;
     double* r = C.kv.p; double* rh = r + N; double* p = rh + N; double* v = p + N; double* ph = v + N;
-    double* s = ph + N; double* sh = s + N; double* t = sh + N;
-    SEMO_HIP(hipMemsetAsync(x, 0, N * sizeof(double), c->stream));
-    copy_d(c, r, b, N); copy_d(c, rh, b, N);
-    SEMO_HIP(hipMemsetAsync(p, 0, N * sizeof(double), c->stream));
-    SEMO_HIP(hipMemsetAsync(v, 0, N * sizeof(double), c->stream));
-    double bb = dot2(c, C, N, b, b, nullptr, nullptr).x;
+    double* sv = ph + N; double* sh = sv + N; double* t = sh + N;
+    double* sc = C.sc.p;
+    int nbv = nblk((long)N);
+    auto fin = [&](int nb, int what) {
+        hipLaunchKernelGGL(k_cfinish, dim3(1), dim3(TPB), 0, c->stream, nb, C.red.p, sc, what);
+    };
+    auto dot = [&](const double* a_, const double* b_, const double* c_, const double* d_, int what) {
+        int nb = nb_of(N);
+        hipLaunchKernelGGL(k_dot2v, dim3(nb), dim3(TPB), 0, c->stream, N, a_, b_, c_, d_, C.red.p);
+        fin(nb, what);
+    };
+    hipLaunchKernelGGL(k_cb_init, dim3(nbv), dim3(TPB), 0, c->stream, N, b, x, r, rh, p, v, C.red.p);
+    fin(nbv, F_STORE_BB);
+    double bb;
+    SEMO_HIP(hipMemcpyAsync(&bb, sc + C_BB, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
     int it = 0;
-    double rr = bb;
     if (bb > 0.0) {
-        double nb = std::sqrt(bb), rho = 1.0, alpha = 1.0, omega = 1.0;
-        double tol = c->sp.tol_u;
+        double nb = std::sqrt(bb), tol = c->sp.tol_u;
+        ConvCheck cc(c, tol * tol * bb, bb);
         for (it = 1; it <= c->sp.maxit_u; ++it) {
-            double rho1 = dot2(c, C, N, rh, r, nullptr, nullptr).x;
-            if (rho1 == 0.0) break;
-            double beta = (rho1 / rho) * (alpha / omega);
-            // p = r + beta (p - omega v)
-            axpbypcz(c, N, 1.0, r, beta, p, -beta * omega, v);
+            dot(rh, r, nullptr, nullptr, F_RHO);
+            hipLaunchKernelGGL(k_cb_p, dim3(nbv), dim3(TPB), 0, c->stream, N, sc, r, v, p);
             precond(c, C, p, ph);
             spmv64(c, C, ph, v);
-            alpha = rho1 / dot2(c, C, N, rh, v, nullptr, nullptr).x;
-            // s = r - alpha v
-            copy_d(c, s, r, N);
-            axpbypcz(c, N, -alpha, v, 1.0, s);
-            precond(c, C, s, sh);
+            dot(rh, v, nullptr, nullptr, F_ALPHA);
+            hipLaunchKernelGGL(k_cb_s, dim3(nbv), dim3(TPB), 0, c->stream, N, sc, r, v, sv);
+            precond(c, C, sv, sh);
             spmv64(c, C, sh, t);
-            double2 ts = dot2(c, C, N, t, s, t, t);
-            omega = ts.y > 0.0 ? ts.x / ts.y : 0.0;
-            // x += alpha ph + omega sh; r = s - omega t
-            axpbypcz(c, N, alpha, ph, 1.0, x, omega, sh);
-            copy_d(c, r, s, N);
-            axpbypcz(c, N, -omega, t, 1.0, r);
-            rho = rho1;
-            rr = dot2(c, C, N, r, r, nullptr, nullptr).x;
-            if (!(std::sqrt(rr) > tol * nb) || omega == 0.0) break;
+            dot(t, sv, t, t, F_OMEGA);
+            hipLaunchKernelGGL(k_cb_x, dim3(nbv), dim3(TPB), 0, c->stream, N, sc, ph, sh, sv, t, x, r, C.red.p);
+            fin(nbv, F_RR);
+            pipe_push(c, sc + C_RR, it);
+            if (cc.done(it)) break;
         }
         it = std::min(it, c->sp.maxit_u);
         // true residual
         hipLaunchKernelGGL((k_bsr<double, 1>), dim3(nblk(C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p, C.col.p,
                            C.val.p, C.diag.p, x, b, r);
-        rr = dot2(c, C, N, r, r, nullptr, nullptr).x;
+        double rr = dot2(c, C, N, r, r, nullptr, nullptr).x;
         C.last_relres = std::sqrt(rr) / nb;
     } else {
         C.last_relres = 0.0;

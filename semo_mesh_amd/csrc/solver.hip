@@ -77,17 +77,22 @@ enum { W_STORE = 0,      // s[slot] = d0, s[slot+1] = d1
 __device__ void finish_scalars(int what, int slot, const double (&d)[2], double* s) {
     switch (what) {
     case W_STORE: s[slot] = d[0]; s[slot + 1] = d[1]; break;
-    case W_FCG_ALPHA: s[S_PQ] = d[0]; s[S_ALPHA] = s[S_RZ] / d[0]; break;
-    case W_FCG_BETA: s[S_BETA] = (d[0] - d[1]) / s[S_RZ]; s[S_RZ] = d[0]; s[S_ZRO] = d[1]; break;
+    // zero denominators (an exactly solved system) give zero steps, so that
+    // an iteration issued past convergence leaves x unchanged
+    case W_FCG_ALPHA: s[S_PQ] = d[0]; s[S_ALPHA] = d[0] != 0.0 ? s[S_RZ] / d[0] : 0.0; break;
+    case W_FCG_BETA:
+        s[S_BETA] = s[S_RZ] != 0.0 ? (d[0] - d[1]) / s[S_RZ] : 0.0;
+        s[S_RZ] = d[0]; s[S_ZRO] = d[1];
+        break;
     case W_BICG_INIT:
         s[S_X] = d[0]; s[S_RR] = d[0]; s[S_RHO] = d[0];
         s[S_BETA] = 0.0; s[S_ALPHA] = 1.0; s[S_OMEGA] = 1.0;
         break;
-    case W_BICG_ALPHA: s[S_AV] = d[0]; s[S_ALPHA] = s[S_RHO] / d[0]; break;
+    case W_BICG_ALPHA: s[S_AV] = d[0]; s[S_ALPHA] = d[0] != 0.0 ? s[S_RHO] / d[0] : 0.0; break;
     case W_BICG_OMEGA: s[S_TT] = d[0]; s[S_TS] = d[1]; s[S_OMEGA] = d[0] != 0.0 ? d[1] / d[0] : 0.0; break;
     case W_BICG_RHO:
         s[S_RR] = d[0];
-        s[S_BETA] = (d[1] / s[S_RHO]) * (s[S_ALPHA] / s[S_OMEGA]);
+        s[S_BETA] = (s[S_RHO] != 0.0 && s[S_OMEGA] != 0.0) ? (d[1] / s[S_RHO]) * (s[S_ALPHA] / s[S_OMEGA]) : 0.0;
         s[S_RHO] = d[1];
         break;
     default: break;
@@ -100,15 +105,18 @@ struct Red {                 // where a launch leaves its block partials
 
 __device__ inline void reduce2(double (&v)[2], const Red& rd) { block_partials(v, rd.part); }
 
-// ordered sum of nb partial pairs, then the scalar update `what`
-__global__ void __launch_bounds__(TPB) k_finish(int nb, const double* __restrict__ part, double* __restrict__ s,
-                                                int what, int slot) {
+// ordered sum of nb partial pairs, then the scalar update `what`; one
+// 1024-thread block, so each thread streams only a few partials
+constexpr int FTPB = 1024;
+__global__ void __launch_bounds__(FTPB) k_finish(int nb, const double* __restrict__ part, double* __restrict__ s,
+                                                 int what, int slot) {
     double a0 = 0.0, a1 = 0.0;
-    for (int i = threadIdx.x; i < nb; i += TPB) { a0 += part[2 * i]; a1 += part[2 * i + 1]; }
-    __shared__ double sh[2][TPB];
+    const double2* p2 = reinterpret_cast<const double2*>(part);
+    for (int i = threadIdx.x; i < nb; i += FTPB) { double2 v = p2[i]; a0 += v.x; a1 += v.y; }
+    __shared__ double sh[2][FTPB];
     sh[0][threadIdx.x] = a0; sh[1][threadIdx.x] = a1;
     __syncthreads();
-    for (int w = TPB / 2; w > 0; w >>= 1) {
+    for (int w = FTPB / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) {
             sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
             sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
@@ -144,6 +152,7 @@ struct RowArgs {
     int n;
     const int* __restrict__ ptr;    // slice offsets (sliced ELL) or row offsets (CSR)
     const int* __restrict__ col;
+    const int16_t* __restrict__ dcol;   // sliced ELL: column - row (null: use col)
     const T* __restrict__ val;
     const T* __restrict__ diag;     // null: no diagonal part
     const T* __restrict__ x;        // gathered operand
@@ -168,7 +177,11 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         row = gid;
         int s = row >> 6, lane = row & 63;
         if (s >= ((a.n + 63) >> 6)) return;
-        for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64) acc += a.val[e] * a.x[a.col[e]];
+        if (a.dcol) {
+            for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64) acc += a.val[e] * a.x[row + a.dcol[e]];
+        } else {
+            for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64) acc += a.val[e] * a.x[a.col[e]];
+        }
         if (row >= a.n) return;
     } else {
         row = gid / G;
@@ -207,6 +220,7 @@ template <class T, int OP>
 void row_launch(semo_ctx* c, const DevSell<T>& A, RowArgs<T> a, const char* tag, double bytes) {
     if (A.n == 0) return;
     a.n = A.n; a.ptr = A.sl_ptr.p; a.col = A.col.p; a.val = A.val.p; a.diag = A.diag.p;
+    a.dcol = (A.lanes == 0 && A.dcol.n) ? A.dcol.p : nullptr;
     hipEvent_t ev{};
     bool prof = c->prof.on && tag;
     if (prof) c->prof.begin(c->stream, ev);
@@ -228,7 +242,9 @@ void row_launch(semo_ctx* c, const DevSell<T>& A, RowArgs<T> a, const char* tag,
 template <class T>
 double op_bytes(const DevSell<T>& A, int vec) {
     double ptrs = A.lanes ? 4.0 * (A.n + 1) : 4.0 * (A.nslices + 1);
-    return (double)A.nnz * (sizeof(T) + 4) + ptrs + (double)A.n * sizeof(T) * vec;
+    double idx = (A.lanes == 0 && A.dcol.n) ? 2.0 : 4.0;
+    double nd = A.diag.n ? (double)A.n : 0.0;          // diagonal entries: a value, no index
+    return ((double)A.nnz - nd) * (sizeof(T) + idx) + nd * sizeof(T) + ptrs + (double)A.n * sizeof(T) * vec;
 }
 
 template <class T, int OP>
@@ -240,6 +256,7 @@ void spmv(semo_ctx* c, const DevSell<T>& A, const T* x, const T* b, T* y, const 
 
 // y = A x (sliced ELL, FP64) with d0 = (a0 ? a0 : y).y and d1 = a1.y
 __global__ void __launch_bounds__(TPB) k_spmv_dot(int n, const int* __restrict__ sl_ptr, const int* __restrict__ col,
+                                                  const int16_t* __restrict__ dcol,
                                                   const double* __restrict__ val, const double* __restrict__ diag,
                                                   const double* __restrict__ x, double* __restrict__ y,
                                                   const double* __restrict__ a0, const double* __restrict__ a1, Red rd) {
@@ -248,7 +265,11 @@ __global__ void __launch_bounds__(TPB) k_spmv_dot(int n, const int* __restrict__
     double v[2] = {0.0, 0.0};
     if (s < ((n + 63) >> 6)) {
         double acc = 0.0;
-        for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) acc += val[e] * x[col[e]];
+        if (dcol) {
+            for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) acc += val[e] * x[row + dcol[e]];
+        } else {
+            for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) acc += val[e] * x[col[e]];
+        }
         if (row < n) {
             acc += diag[row] * x[row];
             y[row] = acc;
@@ -347,6 +368,165 @@ __global__ void __launch_bounds__(TPB) k_bicg_x(int n, const double* __restrict_
         w[0] = ri * ri; w[1] = rh[i] * ri;
     }
     reduce2(w, rd);
+}
+
+// ------------------------------------------ three right-hand sides at once --
+// The momentum components U, V, W share one matrix: BiCGStab on the three
+// systems runs in lockstep, so every matrix pass serves all three.  Vectors
+// are [3][n] (component k at +k*n); the Krylov scalars of component k live
+// in slots Q_* + k of the device scalar array.
+enum { Q_RHO = 32, Q_ALPHA = 35, Q_OMEGA = 38, Q_BETA = 41, Q_BB = 44, Q_RR = 47, Q_MAXREL = 50 };
+enum { F3_INIT = 0, F3_ALPHA, F3_OMEGA, F3_RHO, F3_STORE };
+
+__device__ inline void block_partials6(double (&v)[6], double* __restrict__ part) {
+    __shared__ double sh[6][TPB / 64];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[q] = wave_sum(v[q]);
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 6; ++q) sh[q][threadIdx.x >> 6] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) {
+            double t = 0.0;
+            for (int w = 0; w < TPB / 64; ++w) t += sh[q][w];
+            part[6 * blockIdx.x + q] = t;
+        }
+}
+
+__global__ void __launch_bounds__(1024) k_finish6(int nb, const double* __restrict__ part, double* __restrict__ s,
+                                                  int what) {
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nb; i += 1024)
+        for (int q = 0; q < 6; ++q) a[q] += part[6 * i + q];
+    __shared__ double sh[6][1024];
+    for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double d[6];
+    for (int q = 0; q < 6; ++q) d[q] = sh[q][0];
+    double mx = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        switch (what) {
+        case F3_INIT:
+            s[Q_BB + k] = d[k]; s[Q_RR + k] = d[k]; s[Q_RHO + k] = d[k];
+            s[Q_ALPHA + k] = 1.0; s[Q_OMEGA + k] = 1.0; s[Q_BETA + k] = 0.0;
+            break;
+        case F3_ALPHA: s[Q_ALPHA + k] = d[k] != 0.0 ? s[Q_RHO + k] / d[k] : 0.0; break;
+        case F3_OMEGA: s[Q_OMEGA + k] = d[3 + k] != 0.0 ? d[k] / d[3 + k] : 0.0; break;
+        case F3_RHO: {
+            s[Q_RR + k] = d[k];
+            double rho = s[Q_RHO + k], om = s[Q_OMEGA + k];
+            s[Q_BETA + k] = (rho != 0.0 && om != 0.0) ? (d[3 + k] / rho) * (s[Q_ALPHA + k] / om) : 0.0;
+            s[Q_RHO + k] = d[3 + k];
+            double bb = s[Q_BB + k];
+            double rel = bb > 0.0 ? d[k] / bb : 0.0;
+            mx = (rel > mx || rel != rel) ? rel : mx;
+            break;
+        }
+        default: s[Q_RR + k] = d[k]; break;
+        }
+    }
+    if (what == F3_RHO) s[Q_MAXREL] = mx;
+}
+
+__global__ void __launch_bounds__(TPB) k_bicg3_init(int n, const double* __restrict__ b, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ rh,
+                                                    double* __restrict__ p, double* __restrict__ v,
+                                                    double* __restrict__ part) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double w[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n)
+        for (int k = 0; k < 3; ++k) {
+            size_t j = (size_t)k * n + i;
+            double bi = b[j];
+            x[j] = 0.0; r[j] = bi; rh[j] = bi; p[j] = 0.0; v[j] = 0.0;
+            w[k] = bi * bi;
+        }
+    block_partials6(w, part);
+}
+// p = r + beta (p - omega v) ; ph = D^-1 p
+__global__ void __launch_bounds__(TPB) k_bicg3_p(int n, const double* __restrict__ s, const double* __restrict__ r,
+                                                 const double* __restrict__ v, const double* __restrict__ dinv,
+                                                 double* __restrict__ p, double* __restrict__ ph) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double di = dinv[i];
+    for (int k = 0; k < 3; ++k) {
+        size_t j = (size_t)k * n + i;
+        double pi = r[j] + s[Q_BETA + k] * (p[j] - s[Q_OMEGA + k] * v[j]);
+        p[j] = pi; ph[j] = di * pi;
+    }
+}
+// sv = r - alpha v ; sh = D^-1 sv
+__global__ void __launch_bounds__(TPB) k_bicg3_s(int n, const double* __restrict__ s, const double* __restrict__ r,
+                                                 const double* __restrict__ v, const double* __restrict__ dinv,
+                                                 double* __restrict__ sv, double* __restrict__ sh) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double di = dinv[i];
+    for (int k = 0; k < 3; ++k) {
+        size_t j = (size_t)k * n + i;
+        double si = r[j] - s[Q_ALPHA + k] * v[j];
+        sv[j] = si; sh[j] = di * si;
+    }
+}
+// x += alpha ph + omega sh ; r = sv - omega t ; partials r.r (0..2), rh.r (3..5)
+__global__ void __launch_bounds__(TPB) k_bicg3_x(int n, const double* __restrict__ s, const double* __restrict__ ph,
+                                                 const double* __restrict__ sh, const double* __restrict__ sv,
+                                                 const double* __restrict__ t, const double* __restrict__ rh,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 double* __restrict__ part) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double w[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n)
+        for (int k = 0; k < 3; ++k) {
+            size_t j = (size_t)k * n + i;
+            x[j] += s[Q_ALPHA + k] * ph[j] + s[Q_OMEGA + k] * sh[j];
+            double ri = sv[j] - s[Q_OMEGA + k] * t[j];
+            r[j] = ri;
+            w[k] = ri * ri; w[3 + k] = rh[j] * ri;
+        }
+    block_partials6(w, part);
+}
+// y_k = A x_k on the fine sliced-ELL operator for k = 0..2, with the dots
+// MODE 0: a_k.y_k (0..2);  MODE 1: y_k.a_k (0..2), y_k.y_k (3..5);
+// MODE 2: y_k = a_k - A x_k and y_k.y_k (0..2)
+template <int MODE>
+__global__ void __launch_bounds__(TPB) k_spmv3(int n, const int* __restrict__ sl_ptr, const int* __restrict__ col,
+                                               const int16_t* __restrict__ dcol, const double* __restrict__ val,
+                                               const double* __restrict__ diag, const double* __restrict__ x,
+                                               double* __restrict__ y, const double* __restrict__ a,
+                                               double* __restrict__ part) {
+    int row = xcd_block() * blockDim.x + threadIdx.x;
+    int s = row >> 6, lane = row & 63;
+    double w[6] = {0, 0, 0, 0, 0, 0};
+    size_t N = (size_t)n;
+    if (s < ((n + 63) >> 6)) {
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) {
+            int j = dcol ? row + dcol[e] : col[e];
+            double v = val[e];
+            acc0 += v * x[j]; acc1 += v * x[N + j]; acc2 += v * x[2 * N + j];
+        }
+        if (row < n) {
+            double d = diag[row];
+            double acc[3] = {acc0 + d * x[row], acc1 + d * x[N + row], acc2 + d * x[2 * N + row]};
+            for (int k = 0; k < 3; ++k) {
+                size_t jj = (size_t)k * N + row;
+                double yk = MODE == 2 ? a[jj] - acc[k] : acc[k];
+                y[jj] = yk;
+                if (MODE == 0) w[k] = a[jj] * yk;
+                else if (MODE == 1) { w[k] = yk * a[jj]; w[3 + k] = yk * yk; }
+                else w[k] = yk * yk;
+            }
+        }
+    }
+    block_partials6(w, part);
 }
 
 // ---------------------------------------------------------- Chebyshev ----
@@ -506,9 +686,10 @@ double host_scalar(semo_ctx* c, int idx) {
 
 Red red_of(semo_ctx* c) { return Red{c->red.p}; }
 
+
 // sums the partials of the previous launch (nb blocks) and applies `what`
 void finish(semo_ctx* c, int nb, int what, int slot = 0) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, c->stream, nb, c->red.p, c->scal.p, what, slot);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FTPB), 0, c->stream, nb, c->red.p, c->scal.p, what, slot);
 }
 
 // grid-stride dot products a.b (+ c.d), finished as `what`
@@ -530,9 +711,10 @@ void spmv_dot(semo_ctx* c, const DevSell<double>& A, const double* x, double* y,
     bool prof = c->prof.on;
     if (prof) c->prof.begin(c->stream, ev);
     int nb = nblk((long)A.nslices * 64);
-    hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(TPB), 0, c->stream, A.n, A.sl_ptr.p, A.col.p, A.val.p,
+    hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(TPB), 0, c->stream, A.n, A.sl_ptr.p, A.col.p,
+                       A.dcol.n ? A.dcol.p : nullptr, A.val.p,
                        A.diag.p, x, y, a0, a1, red_of(c));
-    if (prof) c->prof.end(c->stream, "spmv_fp64", ev, op_bytes(A, 3 + (a0 ? 1 : 0) + (a1 ? 1 : 0)));
+    if (prof) c->prof.end(c->stream, "spmv_fp64", ev, op_bytes(A, 2 + (a0 ? 1 : 0) + (a1 ? 1 : 0)));
     finish(c, nb, what);
 }
 
@@ -554,13 +736,14 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
     hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, z, p);
     dot_store(c, n, r, z, nullptr, nullptr, S_RZ);
     int it = 0;
+    ConvCheck cc(c, tol * tol * bb, bb);
     for (it = 1; it <= maxit; ++it) {
         spmv_dot(c, A, p, q, p, nullptr, W_FCG_ALPHA);                    // q = A p, alpha
         hipLaunchKernelGGL(k_fcg_update, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->scal.p, x, r, rold, p, q,
                            red_of(c));
         finish(c, nblk(n), W_STORE, S_RR);
-        double rr = host_scalar(c, S_RR);
-        if (std::sqrt(rr) <= tol * nb) break;
+        pipe_push(c, c->scal.p + S_RR, it);
+        if (cc.done(it) || it == maxit) break;
         if (mixed) precondition<float>(c, amg, r, z); else precondition<double>(c, amg, r, z);
         dot_what(c, n, r, z, z, rold, W_FCG_BETA);                        // r.z, z.r_old -> beta
         hipLaunchKernelGGL(k_xpby, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->scal.p, (int)S_BETA, z, p);
@@ -588,6 +771,7 @@ int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const do
     if (bb == 0.0) { *relres = 0.0; return 0; }
     double nb = std::sqrt(bb);
     int it;
+    ConvCheck cc(c, tol * tol * bb, bb);
     for (it = 1; it <= maxit; ++it) {
         hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, r, v, dinv, p, ph);
         spmv_dot(c, A, ph, v, rh, nullptr, W_BICG_ALPHA);                 // v = A ph, alpha
@@ -596,13 +780,83 @@ int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const do
         hipLaunchKernelGGL(k_bicg_x, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, ph, sh, sv, t, rh, x, r,
                            red_of(c));
         finish(c, nblk(n), W_BICG_RHO);
-        double rr = host_scalar(c, S_RR);
-        if (!(std::sqrt(rr) > tol * nb)) break;
+        pipe_push(c, c->scal.p + S_RR, it);
+        if (cc.done(it)) break;
     }
     spmv<double, OP_RES>(c, A, x, b, r);
     dot_store(c, n, r, r, nullptr, nullptr, S_X);
     *relres = std::sqrt(host_scalar(c, S_X)) / nb;
     return std::min(it, maxit);
+}
+
+// BiCGStab on the three momentum components in lockstep (Jacobi, x0 = 0);
+// stops when every component has ||r_k|| <= tol ||b_k||
+void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const double* b, double* x, double tol,
+               int maxit, semo_solve_stats* st) {
+    SEMO_CHECK(A.lanes == 0, SEMO_EINVAL, "Krylov operator must be sliced ELL");
+    int n = A.n;
+    size_t ln = 3 * (size_t)n + 64;
+    c->kv64.ensure(8 * ln);
+    double* r = c->kv64.p; double* rh = r + ln; double* p = rh + ln; double* v = p + ln; double* sv = v + ln;
+    double* t = sv + ln; double* ph = t + ln; double* sh = ph + ln;
+    double* s = c->scal.p;
+    int nb = nblk(n), nbs = nblk((long)A.nslices * 64);
+    size_t need = 6 * (size_t)std::max(nb, nbs) + 64;
+    if (c->red.n < need) c->red.alloc(need);
+    double* part = c->red.p;
+    const int16_t* dc = A.dcol.n ? A.dcol.p : nullptr;
+    auto fin = [&](int blocks, int what) {
+        hipLaunchKernelGGL(k_finish6, dim3(1), dim3(1024), 0, c->stream, blocks, part, s, what);
+    };
+    auto spmv3 = [&](int mode, const double* xin, double* y, const double* a) {
+        hipEvent_t ev{};
+        bool prof = c->prof.on;
+        if (prof) c->prof.begin(c->stream, ev);
+        if (mode == 0)
+            hipLaunchKernelGGL(k_spmv3<0>, dim3(nbs), dim3(TPB), 0, c->stream, n, A.sl_ptr.p, A.col.p, dc, A.val.p,
+                               A.diag.p, xin, y, a, part);
+        else if (mode == 1)
+            hipLaunchKernelGGL(k_spmv3<1>, dim3(nbs), dim3(TPB), 0, c->stream, n, A.sl_ptr.p, A.col.p, dc, A.val.p,
+                               A.diag.p, xin, y, a, part);
+        else
+            hipLaunchKernelGGL(k_spmv3<2>, dim3(nbs), dim3(TPB), 0, c->stream, n, A.sl_ptr.p, A.col.p, dc, A.val.p,
+                               A.diag.p, xin, y, a, part);
+        // matrix once, three gathered operands, three results, three dot operands
+        if (prof) c->prof.end(c->stream, "spmv3_fp64", ev, op_bytes(A, 9));
+    };
+    hipLaunchKernelGGL(k_bicg3_init, dim3(nb), dim3(TPB), 0, c->stream, n, b, x, r, rh, p, v, part);
+    fin(nb, F3_INIT);
+    double bb[3];
+    SEMO_HIP(hipMemcpyAsync(bb, s + Q_BB, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    int it = 0;
+    if (bb[0] > 0.0 || bb[1] > 0.0 || bb[2] > 0.0) {
+        ConvCheck cc(c, tol * tol, 1.0);
+        for (it = 1; it <= maxit; ++it) {
+            hipLaunchKernelGGL(k_bicg3_p, dim3(nb), dim3(TPB), 0, c->stream, n, s, r, v, dinv, p, ph);
+            spmv3(0, ph, v, rh);                                   // v = A ph, rh.v
+            fin(nbs, F3_ALPHA);
+            hipLaunchKernelGGL(k_bicg3_s, dim3(nb), dim3(TPB), 0, c->stream, n, s, r, v, dinv, sv, sh);
+            spmv3(1, sh, t, sv);                                   // t = A sh, t.s, t.t
+            fin(nbs, F3_OMEGA);
+            hipLaunchKernelGGL(k_bicg3_x, dim3(nb), dim3(TPB), 0, c->stream, n, s, ph, sh, sv, t, rh, x, r, part);
+            fin(nb, F3_RHO);
+            pipe_push(c, s + Q_MAXREL, it);
+            if (cc.done(it)) break;
+        }
+        it = std::min(it, maxit);
+    }
+    // true residuals in FP64
+    spmv3(2, x, r, b);
+    fin(nbs, F3_STORE);
+    double rr[3];
+    SEMO_HIP(hipMemcpyAsync(rr, s + Q_RR, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    if (st)
+        for (int k = 0; k < 3; ++k) {
+            st->iters[k] = bb[k] > 0.0 ? it : 0;
+            st->rel_res[k] = bb[k] > 0.0 ? std::sqrt(rr[k] / bb[k]) : 0.0;
+        }
 }
 
 // level-0 operator of the hierarchy refreshed from the assembled system
@@ -641,6 +895,21 @@ void sell_upload(DevSell<T>& M, const SellPattern& p, const std::vector<double>&
     M.nnz = p.nnz_offdiag + (int64_t)diag.size();
     M.sl_ptr.upload(p.sl_ptr, s);
     M.col.upload(p.col, s);
+    // column - row as int16 when every entry allows it; entries of the rows
+    // past n in the last slice (value 0) point at row n - 1
+    M.dcol.release();
+    if (p.n == p.ncols && p.n > 0) {
+        std::vector<int16_t> d(p.col.size());
+        bool ok = true;
+        for (int sl = 0; sl < p.nslices && ok; ++sl)
+            for (int e = p.sl_ptr[sl]; e < p.sl_ptr[sl + 1]; ++e) {
+                int row = sl * SL + ((e - p.sl_ptr[sl]) & (SL - 1));
+                long dv = row < p.n ? (long)p.col[e] - row : (long)(p.n - 1) - row;
+                if (dv < -32767 || dv > 32767) { ok = false; break; }
+                d[e] = (int16_t)dv;
+            }
+        if (ok) M.dcol.upload(d, s);
+    }
     std::vector<T> v(vals.begin(), vals.end());
     M.val.upload(v, s);
     if (!diag.empty()) {
@@ -757,6 +1026,10 @@ void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st) {
     DBuf<double>& dinv = c->dinv_mom;
     dinv.alloc(n + 64);
     hipLaunchKernelGGL(k_dinv, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->Amat.diag.p, dinv.p);
+    if (nrhs == 3) {
+        bicgstab3(c, c->Amat, dinv.p, c->Bvec.p, c->Xvec.p, c->sp.tol_u, c->sp.maxit_u, st);
+        return;
+    }
     for (int k = 0; k < nrhs; ++k) {
         double rr = 0;
         int it = bicgstab(c, c->Amat, dinv.p, c->Bvec.p + (size_t)k * n, c->Xvec.p + (size_t)k * n, c->sp.tol_u,
