@@ -167,12 +167,18 @@ void split_diag(const HCsr& A, std::vector<int>& ptr, std::vector<int>& col, std
 // of similar length; CSR with `lanes` lanes per row (about four entries per
 // lane) for long rows (restriction, Galerkin coarse operators) or rows whose
 // lengths vary so much that ELL padding would waste more than half the reads.
+#ifndef SEMO_ELL_MAXAVG
+#define SEMO_ELL_MAXAVG 10.0
+#endif
+#ifndef SEMO_LANE_DIV
+#define SEMO_LANE_DIV 4.0
+#endif
 int choose_lanes(int n, const std::vector<int>& ptr, size_t padded) {
     size_t nnz = ptr.empty() ? 0 : (size_t)ptr[n];
     double avg = n ? (double)nnz / n : 0.0;
-    if (avg <= 10.0 && (double)padded <= 1.5 * (double)std::max<size_t>(nnz, 1)) return 0;
+    if (avg <= SEMO_ELL_MAXAVG && (double)padded <= 1.5 * (double)std::max<size_t>(nnz, 1)) return 0;
     int lanes = 4;
-    while (lanes < 64 && 4.0 * lanes < avg) lanes *= 2;
+    while (lanes < 64 && SEMO_LANE_DIV * lanes < avg) lanes *= 2;
     return lanes;
 }
 
@@ -205,6 +211,7 @@ void upload_csr(DevSell<T>& M, const HCsr& A, bool square, hipStream_t s, std::v
 // src_slot (optional): for every entry of A0, the slot of the device system
 // it came from (-1 on the diagonal); lets level 0 be refreshed on the device
 void amg_setup_host(semo_ctx* c, Amg& amg, const HCsr& A0, const std::vector<int>* src_slot = nullptr) {
+    amg.drop_graph();             // the level buffers are reallocated below
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = c->stream;
     amg.lv.clear();

@@ -5,6 +5,8 @@
 #include <cstring>
 #include <functional>
 
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace semo {
@@ -193,6 +195,8 @@ int semo_ctx_create(int device, void* stream, semo_ctx** out) {
         c->device = device;
         if (stream) c->stream = (hipStream_t)stream;
         else { SEMO_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)); c->own_stream = true; }
+        const char* gv = std::getenv("SEMO_GPU_GRAPHS");
+        c->graphs = !(gv && gv[0] == '0');
         c->sp.precision = SEMO_PREC_FP64; c->sp.tol_p = 1e-8; c->sp.tol_u = 1e-8;
         c->sp.maxit_p = 500; c->sp.maxit_u = 500; c->sp.cheb_degree = 2; c->sp.coarse_size = 1000;
     });
@@ -232,7 +236,11 @@ int semo_mesh_upload(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
     return guard(c, [&] {
         SEMO_HIP(hipSetDevice(c->device));
         c->has_mesh = false; c->has_case = false; c->iterReal = 0;
-        c->amg_p = Amg();
+        c->amg_p.reset();
+        coupled_free(c->cpl);          // mesh-bound block pattern and work arrays
+        c->cpl = nullptr;
+        adjoint_free(c->adj);
+        c->adj = nullptr;
         mesh_setup(c, m, reorder);
         geometry_compute(c);
         fields_alloc(c);

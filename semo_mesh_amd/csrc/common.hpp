@@ -165,6 +165,25 @@ struct Amg {
     DBuf<int> l0_map;
     int iters_ref = -1, last_iters = -1;
     bool fresh = false;         // the last solve used a freshly built hierarchy
+    // one V-cycle captured as a HIP graph (replayed per Krylov iteration):
+    // valid for the hierarchy's buffers and the operands it was captured with
+    hipGraphExec_t vg = nullptr;
+    const double* vg_r = nullptr;
+    double* vg_z = nullptr;
+    int vg_deg = 0, vg_prec = -1;
+    void drop_graph() {
+        if (vg) (void)hipGraphExecDestroy(vg);
+        vg = nullptr; vg_r = nullptr; vg_z = nullptr; vg_prec = -1;
+    }
+    void reset() {
+        drop_graph();
+        lv.clear(); nc_dense = 0; coarse_inv.release(); setup_ms = 0; built = false; key.clear();
+        l0_map.release(); iters_ref = last_iters = -1; fresh = false;
+    }
+    Amg() = default;
+    Amg(const Amg&) = delete;
+    Amg& operator=(const Amg&) = delete;
+    ~Amg() { drop_graph(); }
 };
 
 struct AdjBufs;                 // adjoint.hip: checkpoints and adjoint work arrays
@@ -177,6 +196,7 @@ struct semo_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    bool graphs = true;                    // replay V-cycles as HIP graphs (SEMO_GPU_GRAPHS=0: off)
     std::string err;
     semo::Profiler prof;
 
@@ -326,6 +346,7 @@ void lsq_standalone(semo_ctx* c, const double* d_phi, const double* d_faceR, dou
 void minmax_standalone(semo_ctx* c, const double* d_phi, double* d_mx, double* d_mn);
 void copy_d(semo_ctx* c, double* dst, const double* src, size_t n);
 // solver.hip
+void sell_narrow(const SellPattern& p, DBuf<int16_t>& d, hipStream_t s);
 template <class T> void sell_upload(DevSell<T>& M, const SellPattern& p, const std::vector<double>& vals,
                                     const std::vector<double>& diag, hipStream_t s);
 template <class T> void csr_upload(DevSell<T>& M, int n, int ncols, const std::vector<int>& ptr,

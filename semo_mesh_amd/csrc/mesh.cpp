@@ -389,6 +389,7 @@ void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
     c->Amat.n = nc; c->Amat.ncols = nc; c->Amat.nslices = c->pat.nslices; c->Amat.nnz = nc + c->pat.nnz_offdiag;
     c->Amat.sl_ptr.upload(c->pat.sl_ptr, s);
     c->Amat.col.upload(c->pat.col, s);
+    sell_narrow(c->pat, c->Amat.dcol, s);
     c->Amat.val.alloc(c->pat.col.size());
     c->Amat.val.zero(s);
     c->Amat.diag.alloc(nc);

@@ -280,6 +280,36 @@ __global__ void __launch_bounds__(TPB) k_spmv_dot(int n, const int* __restrict__
     reduce2(v, rd);
 }
 
+// FCG direction update fused into its product: pn = z + beta po (beta =
+// s[S_BETA]), q = A pn, partials pn.q.  po and pn are distinct buffers:
+// neighbours read po while the row writes pn.
+__global__ void __launch_bounds__(TPB) k_spmv_xpby(int n, const int* __restrict__ sl_ptr,
+                                                   const int* __restrict__ col, const int16_t* __restrict__ dcol,
+                                                   const double* __restrict__ val, const double* __restrict__ diag,
+                                                   const double* __restrict__ s, const double* __restrict__ z,
+                                                   const double* __restrict__ po, double* __restrict__ pn,
+                                                   double* __restrict__ q, Red rd) {
+    int row = xcd_block() * blockDim.x + threadIdx.x;
+    int sl = row >> 6, lane = row & 63;
+    double v[2] = {0.0, 0.0};
+    if (sl < ((n + 63) >> 6)) {
+        double beta = s[S_BETA];
+        double acc = 0.0;
+        for (int e = sl_ptr[sl] + lane; e < sl_ptr[sl + 1]; e += 64) {
+            int j = dcol ? row + dcol[e] : col[e];
+            acc += val[e] * (z[j] + beta * po[j]);
+        }
+        if (row < n) {
+            double pr = z[row] + beta * po[row];
+            pn[row] = pr;
+            acc += diag[row] * pr;
+            q[row] = acc;
+            v[0] = pr * acc;
+        }
+    }
+    reduce2(v, rd);
+}
+
 // ------------------------------------------------------- Krylov vectors --
 // x = 0, r = b, d0 = b.b
 __global__ void __launch_bounds__(TPB) k_fcg_init(int n, const double* __restrict__ b, double* __restrict__ x,
@@ -307,14 +337,6 @@ __global__ void __launch_bounds__(TPB) k_fcg_update(int n, const double* __restr
         v[0] = ri * ri;
     }
     reduce2(v, rd);
-}
-
-// p = z + s[sidx] p
-__global__ void __launch_bounds__(TPB) k_xpby(int n, const double* __restrict__ s, int sidx,
-                                              const double* __restrict__ z, double* __restrict__ p) {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    p[i] = z[i] + s[sidx] * p[i];
 }
 
 template <class TA, class TB>
@@ -670,11 +692,30 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
     cheb_smooth<T>(c, L, b, degree, false, nullptr, zo, tg);
 }
 
-// z = M^-1 r with one V-cycle in precision T
+// z = M^-1 r with one V-cycle in precision T.  The V-cycle is a fixed
+// sequence of ~22 kernels over fixed buffers, so it is captured once as a HIP
+// graph and replayed: one submission instead of ~22.  It holds no memset and
+// no library call.  Profiling (per-kernel events) runs it eagerly.
 template <class T>
 void precondition(semo_ctx* c, Amg& amg, const double* r64, double* z64) {
     int deg = c->sp.cheb_degree > 0 ? c->sp.cheb_degree : 2;
-    vcycle<T>(c, amg, 0, deg, r64, z64);
+    if (!c->graphs || c->prof.on) {
+        vcycle<T>(c, amg, 0, deg, r64, z64);
+        return;
+    }
+    int prec = sizeof(T) == 4 ? 1 : 0;
+    if (!amg.vg || amg.vg_r != r64 || amg.vg_z != z64 || amg.vg_deg != deg || amg.vg_prec != prec) {
+        amg.drop_graph();
+        hipGraph_t g = nullptr;
+        SEMO_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        vcycle<T>(c, amg, 0, deg, r64, z64);
+        SEMO_HIP(hipStreamEndCapture(c->stream, &g));
+        hipError_t e = hipGraphInstantiate(&amg.vg, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        SEMO_HIP(e);
+        amg.vg_r = r64; amg.vg_z = z64; amg.vg_deg = deg; amg.vg_prec = prec;
+    }
+    SEMO_HIP(hipGraphLaunch(amg.vg, c->stream));
 }
 
 double host_scalar(semo_ctx* c, int idx) {
@@ -737,8 +778,22 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
     dot_store(c, n, r, z, nullptr, nullptr, S_RZ);
     int it = 0;
     ConvCheck cc(c, tol * tol * bb, bb);
+    double* p2 = rold + ln;                      // second direction buffer
     for (it = 1; it <= maxit; ++it) {
-        spmv_dot(c, A, p, q, p, nullptr, W_FCG_ALPHA);                    // q = A p, alpha
+        if (it == 1) {
+            spmv_dot(c, A, p, q, p, nullptr, W_FCG_ALPHA);                // q = A p, alpha
+        } else {                                                          // p = z + beta p; q = A p, alpha
+            hipEvent_t ev{};
+            bool prof = c->prof.on;
+            if (prof) c->prof.begin(c->stream, ev);
+            int nbs = nblk((long)A.nslices * 64);
+            hipLaunchKernelGGL(k_spmv_xpby, dim3(nbs), dim3(TPB), 0, c->stream, A.n, A.sl_ptr.p, A.col.p,
+                               A.dcol.n ? A.dcol.p : nullptr, A.val.p, A.diag.p, c->scal.p, z, p, p2, q,
+                               red_of(c));
+            if (prof) c->prof.end(c->stream, "spmv_xpby_fp64", ev, op_bytes(A, 4));
+            finish(c, nbs, W_FCG_ALPHA);
+            std::swap(p, p2);
+        }
         hipLaunchKernelGGL(k_fcg_update, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->scal.p, x, r, rold, p, q,
                            red_of(c));
         finish(c, nblk(n), W_STORE, S_RR);
@@ -746,7 +801,6 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
         if (cc.done(it) || it == maxit) break;
         if (mixed) precondition<float>(c, amg, r, z); else precondition<double>(c, amg, r, z);
         dot_what(c, n, r, z, z, rold, W_FCG_BETA);                        // r.z, z.r_old -> beta
-        hipLaunchKernelGGL(k_xpby, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->scal.p, (int)S_BETA, z, p);
     }
     // true residual in FP64
     spmv<double, OP_RES>(c, A, x, b, r);
@@ -822,7 +876,7 @@ void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const 
             hipLaunchKernelGGL(k_spmv3<2>, dim3(nbs), dim3(TPB), 0, c->stream, n, A.sl_ptr.p, A.col.p, dc, A.val.p,
                                A.diag.p, xin, y, a, part);
         // matrix once, three gathered operands, three results, three dot operands
-        if (prof) c->prof.end(c->stream, "spmv3_fp64", ev, op_bytes(A, 9));
+        if (prof) c->prof.end(c->stream, mode == 2 ? "spmv3_res_fp64" : "spmv3_fp64", ev, op_bytes(A, 9));
     };
     hipLaunchKernelGGL(k_bicg3_init, dim3(nb), dim3(TPB), 0, c->stream, n, b, x, r, rh, p, v, part);
     fin(nb, F3_INIT);
@@ -895,21 +949,7 @@ void sell_upload(DevSell<T>& M, const SellPattern& p, const std::vector<double>&
     M.nnz = p.nnz_offdiag + (int64_t)diag.size();
     M.sl_ptr.upload(p.sl_ptr, s);
     M.col.upload(p.col, s);
-    // column - row as int16 when every entry allows it; entries of the rows
-    // past n in the last slice (value 0) point at row n - 1
-    M.dcol.release();
-    if (p.n == p.ncols && p.n > 0) {
-        std::vector<int16_t> d(p.col.size());
-        bool ok = true;
-        for (int sl = 0; sl < p.nslices && ok; ++sl)
-            for (int e = p.sl_ptr[sl]; e < p.sl_ptr[sl + 1]; ++e) {
-                int row = sl * SL + ((e - p.sl_ptr[sl]) & (SL - 1));
-                long dv = row < p.n ? (long)p.col[e] - row : (long)(p.n - 1) - row;
-                if (dv < -32767 || dv > 32767) { ok = false; break; }
-                d[e] = (int16_t)dv;
-            }
-        if (ok) M.dcol.upload(d, s);
-    }
+    sell_narrow(p, M.dcol, s);
     std::vector<T> v(vals.begin(), vals.end());
     M.val.upload(v, s);
     if (!diag.empty()) {
@@ -922,6 +962,24 @@ template void sell_upload<double>(DevSell<double>&, const SellPattern&, const st
                                   const std::vector<double>&, hipStream_t);
 template void sell_upload<float>(DevSell<float>&, const SellPattern&, const std::vector<double>&,
                                  const std::vector<double>&, hipStream_t);
+
+// column - row as int16 when every entry of a square pattern allows it;
+// entries of the rows past n in the last slice (value 0) point at row n - 1.
+// Leaves d empty otherwise.
+void sell_narrow(const SellPattern& p, DBuf<int16_t>& d, hipStream_t s) {
+    d.release();
+    if (p.n != p.ncols || p.n == 0) return;
+    std::vector<int16_t> h(p.col.size());
+    for (int sl = 0; sl < p.nslices; ++sl)
+        for (int e = p.sl_ptr[sl]; e < p.sl_ptr[sl + 1]; ++e) {
+            int row = sl * SL + ((e - p.sl_ptr[sl]) & (SL - 1));
+            long dv = row < p.n ? (long)p.col[e] - row : (long)(p.n - 1) - row;
+            if (dv < -32767 || dv > 32767) return;
+            h[e] = (int16_t)dv;
+        }
+    d.upload(h, s);
+    SEMO_HIP(hipStreamSynchronize(s));
+}
 
 template <class T>
 void csr_upload(DevSell<T>& M, int n, int ncols, const std::vector<int>& ptr, const std::vector<int>& col,
