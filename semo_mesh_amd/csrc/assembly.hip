@@ -203,11 +203,8 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
     double mx[4] = {ph[0], ph[1], ph[2], ph[3]}, mn[4] = {ph[0], ph[1], ph[2], ph[3]};
     int s = i >> 6, lane = i & 63;
     int e0 = st_sl[s], e1 = st_sl[s + 1];
-    for (int e = e0 + lane; e < e1; e += 64) {
-        int j = st_ell[e];
-        if (j < 0) break;
-        const double2* o = reinterpret_cast<const double2*>(cpk + 8 * (size_t)j);
-        double2 o0 = o[0], o1 = o[1], o2 = o[2], o3 = o[3];
+    // one stencil neighbour's terms, in stencil order
+    auto add = [&](const double2& o0, const double2& o1, const double2& o2, const double2& o3) {
         double dx = o0.x - xi, dy = o0.y - yi, dz = o1.x - zi;
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
         double pj[5] = {o1.y, o2.x, o2.y, o3.x, o3.y};
@@ -218,6 +215,23 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) { mx[v] = smax(mx[v], pj[v]); mn[v] = smin(mn[v], pj[v]); }
+    };
+    // two neighbours per trip: both records are requested before the first
+    // is summed, so two gathers are in flight per lane
+    for (int e = e0 + lane; e < e1; e += 128) {
+        int j0 = st_ell[e];
+        if (j0 < 0) break;
+        int j1 = e + 64 < e1 ? st_ell[e + 64] : -1;
+        const double2* o = reinterpret_cast<const double2*>(cpk + 8 * (size_t)j0);
+        double2 a0 = o[0], a1 = o[1], a2 = o[2], a3 = o[3];
+        double2 b0, b1, b2, b3;
+        if (j1 >= 0) {
+            const double2* q = reinterpret_cast<const double2*>(cpk + 8 * (size_t)j1);
+            b0 = q[0]; b1 = q[1]; b2 = q[2]; b3 = q[3];
+        }
+        add(a0, a1, a2, a3);
+        if (j1 < 0) break;
+        add(b0, b1, b2, b3);
     }
     for (int q = bf_ptr[i]; q < bf_ptr[i + 1]; ++q) {
         int f = bf_idx[q];

@@ -198,7 +198,7 @@ int semo_ctx_create(int device, void* stream, semo_ctx** out) {
         const char* gv = std::getenv("SEMO_GPU_GRAPHS");
         c->graphs = !(gv && gv[0] == '0');
         c->sp.precision = SEMO_PREC_FP64; c->sp.tol_p = 1e-8; c->sp.tol_u = 1e-8;
-        c->sp.maxit_p = 500; c->sp.maxit_u = 500; c->sp.cheb_degree = 2; c->sp.coarse_size = 1000;
+        c->sp.maxit_p = 500; c->sp.maxit_u = 500; c->sp.cheb_degree = 1; c->sp.coarse_size = 1000;
     });
     if (rc != SEMO_OK) { delete c; return rc; }
     *out = c;
@@ -315,7 +315,7 @@ int semo_set_solver(semo_ctx* c, const semo_solver_params* p) {
         SEMO_CHECK(p->precision == SEMO_PREC_FP64 || p->precision == SEMO_PREC_MIXED, SEMO_EINVAL, "precision");
         bool rebuild = p->coarse_size != c->sp.coarse_size;
         c->sp = *p;
-        if (c->sp.cheb_degree <= 0) c->sp.cheb_degree = 2;
+        if (c->sp.cheb_degree <= 0) c->sp.cheb_degree = 1;
         if (c->sp.coarse_size <= 0) c->sp.coarse_size = 1000;
         if (rebuild) c->amg_p.built = false;
     });

@@ -194,53 +194,55 @@ __global__ void __launch_bounds__(TPB) k_cpl_galerkin(int64_t ne, int nc, const 
 }
 
 // ------------------------------------------------------------- block SpMV --
+// Four lanes per block row: lane q owns output component q and reads row q of
+// every block (4 contiguous values), so a quad reads each 16-value block as
+// one contiguous run; x_j (4 values) is shared by the quad.
 // OP 0: y = A x;  OP 1: y = b - A x
 template <class T, int OP>
 __global__ void __launch_bounds__(TPB) k_bsr(int n, const int* __restrict__ ptr, const int* __restrict__ col,
                                              const T* __restrict__ val, const T* __restrict__ diag,
                                              const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y) {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
+    int gid = xcd_block() * blockDim.x + threadIdx.x;
+    int i = gid >> 2, q = gid & 3;
     if (i >= n) return;
-    T s[4] = {0, 0, 0, 0};
-    const T* d = diag + 16 * (size_t)i;
+    const T* d = diag + 16 * (size_t)i + 4 * q;
     const T* xi = x + 4 * (size_t)i;
-    for (int r = 0; r < 4; ++r) s[r] += d[4 * r] * xi[0] + d[4 * r + 1] * xi[1] + d[4 * r + 2] * xi[2] + d[4 * r + 3] * xi[3];
+    T s = d[0] * xi[0] + d[1] * xi[1] + d[2] * xi[2] + d[3] * xi[3];
     for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
-        const T* a = val + 16 * (size_t)e;
+        const T* a = val + 16 * (size_t)e + 4 * q;
         const T* xj = x + 4 * (size_t)col[e];
-        T x0 = xj[0], x1 = xj[1], x2 = xj[2], x3 = xj[3];
-        for (int r = 0; r < 4; ++r) s[r] += a[4 * r] * x0 + a[4 * r + 1] * x1 + a[4 * r + 2] * x2 + a[4 * r + 3] * x3;
+        s += a[0] * xj[0] + a[1] * xj[1] + a[2] * xj[2] + a[3] * xj[3];
     }
-    T* yo = y + 4 * (size_t)i;
-    if (OP == 0) { for (int r = 0; r < 4; ++r) yo[r] = s[r]; }
-    else { const T* bi = b + 4 * (size_t)i; for (int r = 0; r < 4; ++r) yo[r] = bi[r] - s[r]; }
+    size_t o = 4 * (size_t)i + q;
+    y[o] = OP == 0 ? s : b[o] - s;
 }
 
-// damped block Jacobi: xo = x + w D^-1 (b - A x); x == null: start from 0
+// damped block Jacobi: xo = x + w D^-1 (b - A x); x == null: start from 0.
+// Lane q forms residual component q; the quad exchanges the four components
+// to apply row q of the 4x4 inverse.
 __global__ void __launch_bounds__(TPB) k_bjac(int n, const int* __restrict__ ptr, const int* __restrict__ col,
                                               const float* __restrict__ val, const float* __restrict__ diag,
                                               const float* __restrict__ dinv, const float* __restrict__ x,
                                               const float* __restrict__ b, float w, float* __restrict__ xo) {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* bi = b + 4 * (size_t)i;
-    float r[4] = {bi[0], bi[1], bi[2], bi[3]};
-    float xi[4] = {0, 0, 0, 0};
+    int gid = xcd_block() * blockDim.x + threadIdx.x;
+    int i = gid >> 2, q = gid & 3;
+    if (i >= n) return;                  // whole quads leave together
+    size_t o = 4 * (size_t)i + q;
+    float r = b[o], xq = 0.0f;
     if (x) {
-        for (int q = 0; q < 4; ++q) xi[q] = x[4 * (size_t)i + q];
-        const float* d = diag + 16 * (size_t)i;
-        for (int q = 0; q < 4; ++q)
-            r[q] -= d[4 * q] * xi[0] + d[4 * q + 1] * xi[1] + d[4 * q + 2] * xi[2] + d[4 * q + 3] * xi[3];
+        xq = x[o];
+        const float* d = diag + 16 * (size_t)i + 4 * q;
+        const float* xi = x + 4 * (size_t)i;
+        r -= d[0] * xi[0] + d[1] * xi[1] + d[2] * xi[2] + d[3] * xi[3];
         for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
-            const float* a = val + 16 * (size_t)e;
+            const float* a = val + 16 * (size_t)e + 4 * q;
             const float* xj = x + 4 * (size_t)col[e];
-            float x0 = xj[0], x1 = xj[1], x2 = xj[2], x3 = xj[3];
-            for (int q = 0; q < 4; ++q) r[q] -= a[4 * q] * x0 + a[4 * q + 1] * x1 + a[4 * q + 2] * x2 + a[4 * q + 3] * x3;
+            r -= a[0] * xj[0] + a[1] * xj[1] + a[2] * xj[2] + a[3] * xj[3];
         }
     }
-    const float* m = dinv + 16 * (size_t)i;
-    for (int q = 0; q < 4; ++q)
-        xo[4 * (size_t)i + q] = xi[q] + w * (m[4 * q] * r[0] + m[4 * q + 1] * r[1] + m[4 * q + 2] * r[2] + m[4 * q + 3] * r[3]);
+    float r0 = __shfl(r, 0, 4), r1 = __shfl(r, 1, 4), r2 = __shfl(r, 2, 4), r3 = __shfl(r, 3, 4);
+    const float* m = dinv + 16 * (size_t)i + 4 * q;
+    xo[o] = xq + w * (m[0] * r0 + m[1] * r1 + m[2] * r2 + m[3] * r3);
 }
 
 // restriction by aggregate sums (tentative P^T), gathered per coarse row
@@ -403,8 +405,8 @@ double2 dot2(semo_ctx* c, Coupled& C, size_t n, const double* a, const double* b
 }
 
 void bjac(semo_ctx* c, BLevel& L, const float* x, float* xo) {
-    hipLaunchKernelGGL(k_bjac, dim3(nblk(L.n)), dim3(TPB), 0, c->stream, L.n, L.ptr.p, L.col.p, L.val.p, L.diag.p,
-                       L.dinv.p, x, L.b.p, OMEGA, xo);
+    hipLaunchKernelGGL(k_bjac, dim3(nblk(4 * (long)L.n)), dim3(TPB), 0, c->stream, L.n, L.ptr.p, L.col.p, L.val.p,
+                       L.diag.p, L.dinv.p, x, L.b.p, OMEGA, xo);
 }
 
 // V-cycle on level l for the right-hand side in lv[l].b; result in lv[l].x
@@ -420,8 +422,8 @@ void vcycle(semo_ctx* c, Coupled& C, size_t l) {
     }
     if (last) return;
     BLevel& Cn = C.lv[l + 1];
-    hipLaunchKernelGGL((k_bsr<float, 1>), dim3(nblk(L.n)), dim3(TPB), 0, c->stream, L.n, L.ptr.p, L.col.p, L.val.p,
-                       L.diag.p, L.x.p, L.b.p, L.r.p);
+    hipLaunchKernelGGL((k_bsr<float, 1>), dim3(nblk(4 * (long)L.n)), dim3(TPB), 0, c->stream, L.n, L.ptr.p, L.col.p,
+                       L.val.p, L.diag.p, L.x.p, L.b.p, L.r.p);
     hipLaunchKernelGGL(k_brestrict, dim3(nblk(Cn.n)), dim3(TPB), 0, c->stream, Cn.n, Cn.mptr.p, Cn.mem.p, L.r.p,
                        Cn.b.p);
     vcycle(c, C, l + 1);
@@ -443,7 +445,7 @@ void precond(semo_ctx* c, Coupled& C, const double* r, double* z) {
 }
 
 void spmv64(semo_ctx* c, Coupled& C, const double* x, double* y) {
-    hipLaunchKernelGGL((k_bsr<double, 0>), dim3(nblk(C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p, C.col.p,
+    hipLaunchKernelGGL((k_bsr<double, 0>), dim3(nblk(4 * (long)C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p, C.col.p,
                        C.val.p, C.diag.p, x, (const double*)nullptr, y);
 }
 
@@ -677,8 +679,8 @@ void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st
         }
         it = std::min(it, c->sp.maxit_u);
         // true residual
-        hipLaunchKernelGGL((k_bsr<double, 1>), dim3(nblk(C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p, C.col.p,
-                           C.val.p, C.diag.p, x, b, r);
+        hipLaunchKernelGGL((k_bsr<double, 1>), dim3(nblk(4 * (long)C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p,
+                           C.col.p, C.val.p, C.diag.p, x, b, r);
         double rr = dot2(c, C, N, r, r, nullptr, nullptr).x;
         C.last_relres = std::sqrt(rr) / nb;
     } else {

@@ -76,7 +76,7 @@ class GpuSolver:
         self._ok(self._L.semo_set_case(self._h, C.byref(cs), bcs))
 
     def set_solver(self, precision="fp64", tol_p=1e-8, tol_u=1e-8, maxit_p=500, maxit_u=500,
-                   cheb_degree=2, coarse_size=1000):
+                   cheb_degree=1, coarse_size=1000):
         p = _lib.SolverParams(PREC[precision], tol_p, tol_u, maxit_p, maxit_u, cheb_degree, coarse_size)
         self._ok(self._L.semo_set_solver(self._h, C.byref(p)))
 
