@@ -184,6 +184,11 @@ __global__ void __launch_bounds__(TPB) k_pack_xyz(int nc, const double* __restri
     o[0] = cxyz[3 * (size_t)i]; o[1] = cxyz[3 * (size_t)i + 1]; o[2] = cxyz[3 * (size_t)i + 2];
 }
 
+// NF = 5: P, U, V, W, VF.  NF = 3: U, V, W only (the pressure stage of an
+// outer iteration, whose P and VF are those of its momentum stage, so their
+// gradients from that stage are reused bit for bit); bounds of U, V, W, and
+// of P when with_p.
+template <int NF>
 __global__ void __launch_bounds__(TPB) k_grad_minmax(
     int nc, int nif, int nbf, int with_p, const int* __restrict__ st_sl, const int* __restrict__ st_ell,
     const int* __restrict__ bf_ptr, const int* __restrict__ bf_idx, const double* __restrict__ cpk,
@@ -209,12 +214,12 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
         double pj[5] = {o1.y, o2.x, o2.y, o3.x, o3.y};
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
+        for (int v = (NF == 5 ? 0 : 1); v < (NF == 5 ? 5 : 4); ++v) {
             double d = pj[v] - ph[v];
             t[v][0] += wk * dx * d; t[v][1] += wk * dy * d; t[v][2] += wk * dz * d;
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) { mx[v] = smax(mx[v], pj[v]); mn[v] = smin(mn[v], pj[v]); }
+        for (int v = (NF == 5 ? 0 : 1); v < 4; ++v) { mx[v] = smax(mx[v], pj[v]); mn[v] = smin(mn[v], pj[v]); }
     };
     // two neighbours per trip: both records are requested before the first
     // is summed, so two gathers are in flight per lane
@@ -238,9 +243,10 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
         int j = f - nif;
         double dx = fxyz[3 * (size_t)f] - xi, dy = fxyz[3 * (size_t)f + 1] - yi, dz = fxyz[3 * (size_t)f + 2] - zi;
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
-        double fr[5] = {bP[j], bUR[j], bUR[(size_t)nbf + j], bUR[2 * (size_t)nbf + j], bVF[j]};
+        double fr[5] = {NF == 5 ? bP[j] : 0.0, bUR[j], bUR[(size_t)nbf + j], bUR[2 * (size_t)nbf + j],
+                        NF == 5 ? bVF[j] : 0.0};
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
+        for (int v = (NF == 5 ? 0 : 1); v < (NF == 5 ? 5 : 4); ++v) {
             double d = fr[v] - ph[v];
             t[v][0] += wk * dx * d; t[v][1] += wk * dy * d; t[v][2] += wk * dz * d;
         }
@@ -248,13 +254,13 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
     const double* C = lsq + 6 * (size_t)i;
     double* out[5] = {gP, gU, gV, gW, gVF};
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
+    for (int v = (NF == 5 ? 0 : 1); v < (NF == 5 ? 5 : 4); ++v) {
         double r0 = 0.0; r0 += C[0] * t[v][0]; r0 += C[1] * t[v][1]; r0 += C[2] * t[v][2];
         double r1 = 0.0; r1 += C[1] * t[v][0]; r1 += C[3] * t[v][1]; r1 += C[4] * t[v][2];
         double r2 = 0.0; r2 += C[2] * t[v][0]; r2 += C[4] * t[v][1]; r2 += C[5] * t[v][2];
         out[v][3 * (size_t)i] = r0; out[v][3 * (size_t)i + 1] = r1; out[v][3 * (size_t)i + 2] = r2;
     }
-    if (with_p) { mxP[i] = mx[0]; mnP[i] = mn[0]; }
+    if (NF == 5 && with_p) { mxP[i] = mx[0]; mnP[i] = mn[0]; }
     mxU[i] = mx[1]; mnU[i] = mn[1]; mxV[i] = mx[2]; mnV[i] = mn[2]; mxW[i] = mx[3]; mnW[i] = mn[3];
 }
 
@@ -1136,29 +1142,35 @@ void source_gravity(semo_ctx* c) {
                        c->bUL.p, c->bUR.p, c->bVF.p, c->nbf, c->srcG.p);
 }
 
-void grad_minmax(semo_ctx* c, bool with_p) {
+void grad_minmax(semo_ctx* c, bool with_p, bool uvw_only) {
     {
         ProfScope ps(c, "pack5", 80.0 * c->nc);
         hipLaunchKernelGGL(k_pack5, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->P.p, c->U.p, c->V.p,
                            c->W.p, c->VF.p, c->cpk.p);
     }
     // algorithmic bytes of a k-field LSQ gradient: 4 S (stencil indices) +
-    // k (8n in + 24n out) + 48n (coefficients) + 24n (centroids), k = 5, plus
-    // the stencil bounds written (6 or 8 arrays) and the boundary faces
-    double n = c->nc, S = (double)c->d_st_idx.n;
-    double bytes = 4.0 * S + 5.0 * 32.0 * n + 48.0 * n + 24.0 * n + (with_p ? 64.0 : 48.0) * n +
-                   (double)c->nbf * (24.0 + 40.0 + 4.0);
-    ProfScope ps(c, "grad_minmax", bytes);
-    hipLaunchKernelGGL(k_grad_minmax, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf,
-                       with_p ? 1 : 0, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p,
-                       c->f_xyz.p, c->c_lsq.p, c->bP.p, c->bUR.p,
-                       c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p, c->gVF.p, c->mxP.p, c->mnP.p, c->mxU.p,
-                       c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
+    // the neighbours' records (read once: 64n) + k 24n (gradients written) +
+    // 48n (coefficients), k = 5 or 3, plus the stencil bounds written (6 or 8
+    // arrays) and the boundary faces
+    double n = c->nc, S = (double)c->d_st_idx.n, k = uvw_only ? 3.0 : 5.0;
+    double bytes = 4.0 * S + 64.0 * n + k * 24.0 * n + 48.0 * n + (with_p ? 64.0 : 48.0) * n +
+                   (double)c->nbf * (24.0 + 8.0 * k + 4.0);
+    ProfScope ps(c, uvw_only ? "grad_minmax_uvw" : "grad_minmax", bytes);
+    if (uvw_only)
+        hipLaunchKernelGGL(k_grad_minmax<3>, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, 0,
+                           c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p, c->f_xyz.p,
+                           c->c_lsq.p, c->bP.p, c->bUR.p, c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p, c->gVF.p,
+                           c->mxP.p, c->mnP.p, c->mxU.p, c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
+    else
+        hipLaunchKernelGGL(k_grad_minmax<5>, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf,
+                           with_p ? 1 : 0, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p,
+                           c->f_xyz.p, c->c_lsq.p, c->bP.p, c->bUR.p, c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p,
+                           c->gVF.p, c->mxP.p, c->mnP.p, c->mxU.p, c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
 }
 
 void assemble_momentum(semo_ctx* c) {
     CaseK k = case_k(c);
-    grad_minmax(c, true);
+    grad_minmax(c, true, false);
     if (c->nif) {
         double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 8 * 8) + (double)c->nc * 8 * (4 + 12 + 8 + 2);
         ProfScope ps(c, "mom_face", bytes);
@@ -1179,9 +1191,9 @@ void assemble_momentum(semo_ctx* c) {
     c->last_eq = 0;
 }
 
-void assemble_pressure(semo_ctx* c) {
+void assemble_pressure(semo_ctx* c, bool after_momentum) {
     CaseK k = case_k(c);
-    grad_minmax(c, false);
+    grad_minmax(c, false, after_momentum);
     if (c->nif) {
         double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 2 * 8) + (double)c->nc * 8 * (4 + 12 + 6 + 2);
         ProfScope ps(c, "p_face", bytes);
@@ -1204,7 +1216,7 @@ void assemble_pressure(semo_ctx* c) {
 // anyway (same field, same boundary values).
 void assemble_vof(semo_ctx* c) {
     CaseK k = case_k(c);
-    grad_minmax(c, false);
+    grad_minmax(c, false, false);
     if (c->nif) {
         {
             ProfScope ps(c, "mstacs_face", (double)c->nif * (8 * 8 + 8 * 3 + 16) + (double)c->nc * 8 * 12);

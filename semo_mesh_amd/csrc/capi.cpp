@@ -158,9 +158,9 @@ double momentum_eq(semo_ctx* c, semo_solve_stats* st) {
     return momentum_update(c);
 }
 
-double pressure_eq(semo_ctx* c, semo_solve_stats* st) {
+double pressure_eq(semo_ctx* c, semo_solve_stats* st, bool after_momentum = false) {
     recon_boundary(c);
-    assemble_pressure(c);
+    assemble_pressure(c, after_momentum);
     solve_pressure(c, c->Bvec.p, c->Xvec.p, st);
     return pressure_correct(c);
 }
@@ -175,7 +175,7 @@ void simple_setup(semo_ctx* c) {
     step_sources(c);
 }
 double simple_momentum(semo_ctx* c, semo_solve_stats* st) { return momentum_eq(c, st); }
-double simple_pressure(semo_ctx* c, semo_solve_stats* st) { return pressure_eq(c, st); }
+double simple_pressure(semo_ctx* c, semo_solve_stats* st) { return pressure_eq(c, st, true); }
 
 }  // namespace semo
 
@@ -485,7 +485,7 @@ int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
         simple_setup(c);      // pressureBased.cpp:17-130: time levels, dt, VOF, sources
         for (int it = 0; it < ncorr; ++it) {
             double n0 = momentum_eq(c, &c->last_stats_mom);
-            double n1 = pressure_eq(c, &c->last_stats_p);
+            double n1 = pressure_eq(c, &c->last_stats_p, true);
             if (norms) { norms[2 * it] = n0; norms[2 * it + 1] = n1; }
         }
         c->iterReal++;

@@ -333,9 +333,13 @@ void fields_alloc(semo_ctx* c);
 void eos_transport(semo_ctx* c);
 void recon_boundary(semo_ctx* c);
 void source_gravity(semo_ctx* c);
-void grad_minmax(semo_ctx* c, bool with_p_minmax);
+// uvw_only: gradients of U, V, W only; P and VF gradients are kept from the
+// last full pass (valid when P, VF and their boundary values are unchanged)
+void grad_minmax(semo_ctx* c, bool with_p_minmax, bool uvw_only = false);
 void assemble_momentum(semo_ctx* c);
-void assemble_pressure(semo_ctx* c);
+// after_momentum: called by an outer iteration right after its momentum
+// stage (P and VF unchanged since), so the P gradient of that stage is reused
+void assemble_pressure(semo_ctx* c, bool after_momentum = false);
 void assemble_vof(semo_ctx* c);
 double time_step(semo_ctx* c, int adjust);
 void surface_tension(semo_ctx* c);

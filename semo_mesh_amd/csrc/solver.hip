@@ -145,7 +145,9 @@ enum { OP_Y = 0,       // y = A x
        OP_RES = 1,     // y = b - A x
        OP_ADD = 2,     // y += A x
        OP_CHEB = 3,    // Chebyshev step: x = d_k, y = d_k+1, r -= D^-1 A d_k, xo += d_k+1
-       OP_CHEBX = 4 }; // Chebyshev start from x != 0: r = D^-1 (b - A x), y = r c1, xo = x + y
+       OP_CHEBX = 4,   // Chebyshev start from x != 0: r = D^-1 (b - A x), y = r c1, xo = x + y
+       OP_J1RES = 5 }; // degree-1 smoothing from x = 0 fused with the residual: x = c1 D^-1 b
+                       // (b from bsrc when given, stored as T), y = b - A x; x_j formed on the fly
 
 template <class T>
 struct RowArgs {
@@ -162,8 +164,22 @@ struct RowArgs {
     T* __restrict__ r;
     T* __restrict__ xo;
     double* __restrict__ zo;        // optional FP64 copy of the new iterate
+    const double* __restrict__ bsrc;    // OP_J1RES: FP64 right-hand side to convert (or null)
     T c1, c2;
 };
+
+// the operand the row kernels gather: x_j, or for OP_J1RES the smoothed
+// iterate c1 D^-1 b at j, formed exactly as k_cheb_init forms it
+template <class T, int OP>
+__device__ inline T row_operand(const RowArgs<T>& a, int j) {
+    if constexpr (OP == OP_J1RES) {
+        T bj = a.bsrc ? (T)a.bsrc[j] : a.b[j];
+        T rj = a.dinv[j] * bj;
+        return rj * a.c1;
+    } else {
+        return a.x[j];
+    }
+}
 
 // G == 0: sliced ELL, one lane per row.  G > 0: CSR, G consecutive lanes per
 // row, strided partial sums combined by a fixed shuffle tree (the result is
@@ -178,19 +194,29 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         int s = row >> 6, lane = row & 63;
         if (s >= ((a.n + 63) >> 6)) return;
         if (a.dcol) {
-            for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64) acc += a.val[e] * a.x[row + a.dcol[e]];
+            for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64)
+                acc += a.val[e] * row_operand<T, OP>(a, row + a.dcol[e]);
         } else {
-            for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64) acc += a.val[e] * a.x[a.col[e]];
+            for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64) acc += a.val[e] * row_operand<T, OP>(a, a.col[e]);
         }
         if (row >= a.n) return;
     } else {
         row = gid / G;
         int sub = gid & (G - 1);
         if (row >= a.n) return;
-        for (int e = a.ptr[row] + sub; e < a.ptr[row + 1]; e += G) acc += a.val[e] * a.x[a.col[e]];
+        for (int e = a.ptr[row] + sub; e < a.ptr[row + 1]; e += G) acc += a.val[e] * row_operand<T, OP>(a, a.col[e]);
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, G);
         if (sub) return;
+    }
+    if constexpr (OP == OP_J1RES) {
+        T bi = a.bsrc ? (T)a.bsrc[row] : a.b[row];
+        T xi = row_operand<T, OP>(a, row);
+        if (a.bsrc) const_cast<T*>(a.b)[row] = bi;
+        a.xo[row] = xi;
+        if (a.diag) acc += a.diag[row] * xi;
+        a.y[row] = bi - acc;
+        return;
     }
     if (a.diag) acc += a.diag[row] * a.x[row];
     if constexpr (OP == OP_Y) {
@@ -683,9 +709,18 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
     AmgLevel& C = amg.lv[l + 1];
     Tags tg = level_tags(l, sizeof(T) == 8);
     T* t = vt<T>(L);
-    cheb_smooth<T>(c, L, b, degree, true, bsrc, nullptr, tg);
     const DevSell<T>& A = opA<T>(L);
-    spmv<T, OP_RES>(c, A, bx<T>(L).p, b, t, tg.resid);
+    if (degree == 1) {
+        // pre-smoothing from x = 0 and the residual in one pass
+        double hi = 1.1 * L.lmax, lo = hi / 30.0;
+        RowArgs<T> a{};
+        a.b = b; a.bsrc = bsrc; a.dinv = vdinv<T>(L); a.c1 = (T)(1.0 / (0.5 * (hi + lo)));
+        a.xo = bx<T>(L).p; a.y = t;
+        row_launch<T, OP_J1RES>(c, A, a, tg.resid, op_bytes(A, 5) + (bsrc ? 8.0 * n : 0.0));
+    } else {
+        cheb_smooth<T>(c, L, b, degree, true, bsrc, nullptr, tg);
+        spmv<T, OP_RES>(c, A, bx<T>(L).p, b, t, tg.resid);
+    }
     spmv<T, OP_Y>(c, opR<T>(L), t, (const T*)nullptr, vb<T>(C), tg.restrict_);
     vcycle<T>(c, amg, l + 1, degree, nullptr, nullptr);
     spmv<T, OP_ADD>(c, opP<T>(L), bx<T>(C).p, (const T*)nullptr, bx<T>(L).p, tg.prolong);
