@@ -188,9 +188,16 @@ __global__ void __launch_bounds__(TPB) k_pack_xyz(int nc, const double* __restri
 // outer iteration, whose P and VF are those of its momentum stage, so their
 // gradients from that stage are reused bit for bit); bounds of U, V, W, and
 // of P when with_p.
-template <int NF>
+//
+// The stencil is read as sliced ELL: int32 cell ids (IDX = int, -1 pads) or,
+// when every neighbour lies within +-32767 of its cell, int16 offsets
+// neighbour - cell (IDX = int16_t, -32768 pads).  The stencil indices, the
+// LSQ coefficients and the outputs are touched once per launch and move with
+// non-temporal loads and stores, so that they do not push the neighbour
+// records, which every cell's stencil re-reads, out of the L2.
+template <int NF, class IDX, int UNR = 2>
 __global__ void __launch_bounds__(TPB) k_grad_minmax(
-    int nc, int nif, int nbf, int with_p, const int* __restrict__ st_sl, const int* __restrict__ st_ell,
+    int nc, int nif, int nbf, int with_p, const int* __restrict__ st_sl, const IDX* __restrict__ st_ell,
     const int* __restrict__ bf_ptr, const int* __restrict__ bf_idx, const double* __restrict__ cpk,
     const double* __restrict__ fxyz, const double* __restrict__ lsq,
     const double* __restrict__ bP, const double* __restrict__ bUR,
@@ -221,22 +228,39 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
 #pragma unroll
         for (int v = (NF == 5 ? 0 : 1); v < 4; ++v) { mx[v] = smax(mx[v], pj[v]); mn[v] = smin(mn[v], pj[v]); }
     };
-    // two neighbours per trip: both records are requested before the first
-    // is summed, so two gathers are in flight per lane
-    for (int e = e0 + lane; e < e1; e += 128) {
-        int j0 = st_ell[e];
-        if (j0 < 0) break;
-        int j1 = e + 64 < e1 ? st_ell[e + 64] : -1;
-        const double2* o = reinterpret_cast<const double2*>(cpk + 8 * (size_t)j0);
-        double2 a0 = o[0], a1 = o[1], a2 = o[2], a3 = o[3];
-        double2 b0, b1, b2, b3;
-        if (j1 >= 0) {
-            const double2* q = reinterpret_cast<const double2*>(cpk + 8 * (size_t)j1);
-            b0 = q[0]; b1 = q[1]; b2 = q[2]; b3 = q[3];
+    // UNR neighbours per trip, all records requested before the first is
+    // summed, and the next trip's stencil indices requested before this
+    // trip's sums.  Padding (-1) follows a cell's last neighbour.  (Keeping
+    // the record loads a whole trip ahead measured slower: the compiler then
+    // waits for every outstanding load at the top of the loop.)
+    auto nbr = [&](int k) -> int {
+        int v = __builtin_nontemporal_load(st_ell + k);
+        if constexpr (sizeof(IDX) == 2) return v == -32768 ? -1 : i + v;
+        else return v;
+    };
+    int e = e0 + lane;
+    int jn[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) jn[u] = e + 64 * u < e1 ? nbr(e + 64 * u) : -1;
+    while (jn[0] >= 0) {
+        int j[UNR];
+        double2 rec[UNR][4];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            j[u] = jn[u];
+            if (j[u] >= 0) {
+                const double2* o = reinterpret_cast<const double2*>(cpk + 8 * (size_t)j[u]);
+                rec[u][0] = o[0]; rec[u][1] = o[1]; rec[u][2] = o[2]; rec[u][3] = o[3];
+            }
         }
-        add(a0, a1, a2, a3);
-        if (j1 < 0) break;
-        add(b0, b1, b2, b3);
+        e += 64 * UNR;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) jn[u] = (j[UNR - 1] >= 0 && e + 64 * u < e1) ? nbr(e + 64 * u) : -1;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (j[u] < 0) break;
+            add(rec[u][0], rec[u][1], rec[u][2], rec[u][3]);
+        }
     }
     for (int q = bf_ptr[i]; q < bf_ptr[i + 1]; ++q) {
         int f = bf_idx[q];
@@ -251,17 +275,24 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
             t[v][0] += wk * dx * d; t[v][1] += wk * dy * d; t[v][2] += wk * dz * d;
         }
     }
-    const double* C = lsq + 6 * (size_t)i;
+    const double* Cp = lsq + 6 * (size_t)i;
+    double C[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) C[k] = __builtin_nontemporal_load(Cp + k);
     double* out[5] = {gP, gU, gV, gW, gVF};
 #pragma unroll
     for (int v = (NF == 5 ? 0 : 1); v < (NF == 5 ? 5 : 4); ++v) {
         double r0 = 0.0; r0 += C[0] * t[v][0]; r0 += C[1] * t[v][1]; r0 += C[2] * t[v][2];
         double r1 = 0.0; r1 += C[1] * t[v][0]; r1 += C[3] * t[v][1]; r1 += C[4] * t[v][2];
         double r2 = 0.0; r2 += C[2] * t[v][0]; r2 += C[4] * t[v][1]; r2 += C[5] * t[v][2];
-        out[v][3 * (size_t)i] = r0; out[v][3 * (size_t)i + 1] = r1; out[v][3 * (size_t)i + 2] = r2;
+        __builtin_nontemporal_store(r0, out[v] + 3 * (size_t)i);
+        __builtin_nontemporal_store(r1, out[v] + 3 * (size_t)i + 1);
+        __builtin_nontemporal_store(r2, out[v] + 3 * (size_t)i + 2);
     }
-    if (NF == 5 && with_p) { mxP[i] = mx[0]; mnP[i] = mn[0]; }
-    mxU[i] = mx[1]; mnU[i] = mn[1]; mxV[i] = mx[2]; mnV[i] = mn[2]; mxW[i] = mx[3]; mnW[i] = mn[3];
+    if (NF == 5 && with_p) { __builtin_nontemporal_store(mx[0], mxP + i); __builtin_nontemporal_store(mn[0], mnP + i); }
+    __builtin_nontemporal_store(mx[1], mxU + i); __builtin_nontemporal_store(mn[1], mnU + i);
+    __builtin_nontemporal_store(mx[2], mxV + i); __builtin_nontemporal_store(mn[2], mnV + i);
+    __builtin_nontemporal_store(mx[3], mxW + i); __builtin_nontemporal_store(mn[3], mnW + i);
 }
 
 // standalone calcLeastSquare of one field
@@ -1152,20 +1183,24 @@ void grad_minmax(semo_ctx* c, bool with_p, bool uvw_only) {
     // the neighbours' records (read once: 64n) + k 24n (gradients written) +
     // 48n (coefficients), k = 5 or 3, plus the stencil bounds written (6 or 8
     // arrays) and the boundary faces
+    bool narrow = c->d_st_dl.n > 0;
     double n = c->nc, S = (double)c->d_st_idx.n, k = uvw_only ? 3.0 : 5.0;
-    double bytes = 4.0 * S + 64.0 * n + k * 24.0 * n + 48.0 * n + (with_p ? 64.0 : 48.0) * n +
+    double bytes = (narrow ? 2.0 : 4.0) * S + 64.0 * n + k * 24.0 * n + 48.0 * n + (with_p ? 64.0 : 48.0) * n +
                    (double)c->nbf * (24.0 + 8.0 * k + 4.0);
     ProfScope ps(c, uvw_only ? "grad_minmax_uvw" : "grad_minmax", bytes);
-    if (uvw_only)
-        hipLaunchKernelGGL(k_grad_minmax<3>, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, 0,
-                           c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p, c->f_xyz.p,
-                           c->c_lsq.p, c->bP.p, c->bUR.p, c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p, c->gVF.p,
-                           c->mxP.p, c->mnP.p, c->mxU.p, c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
-    else
-        hipLaunchKernelGGL(k_grad_minmax<5>, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf,
-                           with_p ? 1 : 0, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p,
+    auto go = [&](auto kern, const auto* idx) {
+        hipLaunchKernelGGL(kern, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf,
+                           (with_p && !uvw_only) ? 1 : 0, c->d_st_sl.p, idx, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p,
                            c->f_xyz.p, c->c_lsq.p, c->bP.p, c->bUR.p, c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p,
                            c->gVF.p, c->mxP.p, c->mnP.p, c->mxU.p, c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
+    };
+    if (narrow) {
+        if (uvw_only) go(k_grad_minmax<3, int16_t>, c->d_st_dl.p);
+        else go(k_grad_minmax<5, int16_t>, c->d_st_dl.p);
+    } else {
+        if (uvw_only) go(k_grad_minmax<3, int>, c->d_st_ell.p);
+        else go(k_grad_minmax<5, int>, c->d_st_ell.p);
+    }
 }
 
 void assemble_momentum(semo_ctx* c) {

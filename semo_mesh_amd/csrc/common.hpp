@@ -217,6 +217,9 @@ struct semo_ctx {
     // the same stencil as sliced ELL: 64-cell slices, entry k of cell
     // 64s+lane at st_sl[s] + 64k + lane, padding -1 after the last entry
     semo::DBuf<int> d_st_sl, d_st_ell;
+    // the same slices as int16 offsets neighbour - cell, -32768 padding; empty
+    // when some neighbour lies further than 32767 from its cell
+    semo::DBuf<int16_t> d_st_dl;
     // cell -> faces as sliced ELL (ascending faces; ~f for a face the cell
     // is the neighbour of; padding INT_MIN) and cell -> boundary faces (CSR)
     semo::DBuf<int> d_cf_sl, d_cf_ell, d_bf_ptr, d_bf_idx;

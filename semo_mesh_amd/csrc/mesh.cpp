@@ -369,6 +369,19 @@ void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
         std::vector<int> sl, ell;
         to_sliced(st_ptr, st_idx, -1, [](int, int j) { return j; }, sl, ell);
         c->d_st_sl.upload(sl, s); c->d_st_ell.upload(ell, s);
+        {
+            std::vector<int16_t> dl(ell.size());
+            bool fits = true;
+            for (int sl_i = 0; sl_i + 1 < (int)sl.size() && fits; ++sl_i)
+                for (int e = sl[sl_i]; e < sl[sl_i + 1]; ++e) {
+                    int r = sl_i * SL + ((e - sl[sl_i]) & (SL - 1));
+                    long d = ell[e] < 0 ? -32768 : (long)ell[e] - r;
+                    if (ell[e] >= 0 && (d < -32767 || d > 32767)) { fits = false; break; }
+                    dl[e] = (int16_t)d;
+                }
+            c->d_st_dl.release();
+            if (fits) c->d_st_dl.upload(dl, s);
+        }
         SEMO_HIP(hipStreamSynchronize(s));
         to_sliced(c->h_cf_ptr, c->h_cf_idx, INT32_MIN,
                   [&](int r, int f) { return c->h_owner[f] == r ? f : ~f; }, sl, ell);

@@ -656,7 +656,7 @@ Tags level_tags(int l, bool fp64) {
     return Tags{t, t, t, t, t};
 }
 
-// Chebyshev polynomial smoother in D^-1 A on [lmax/30, 1.1 lmax] for the
+// Chebyshev polynomial smoother in D^-1 A on [lmax/10, 1.1 lmax] for the
 // iterate held in the level's x buffer.  x_zero: start from x = 0 (no SpMV
 // for the start).  bsrc: FP64 right-hand side to convert into b first.  zo:
 // FP64 copy of the final iterate.
@@ -666,7 +666,7 @@ void cheb_smooth(semo_ctx* c, AmgLevel& L, T* b, int degree, bool x_zero, const 
     const DevSell<T>& A = opA<T>(L);
     int n = L.n;
     T* dA = vt<T>(L); T* dB = vu<T>(L); T* r = vr<T>(L);
-    double hi = 1.1 * L.lmax, lo = hi / 30.0;
+    double hi = 1.1 * L.lmax, lo = hi / 10.0;
     double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
     double sigma = theta / delta, rho = 1.0 / sigma;
     if (x_zero) {
@@ -712,7 +712,7 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
     const DevSell<T>& A = opA<T>(L);
     if (degree == 1) {
         // pre-smoothing from x = 0 and the residual in one pass
-        double hi = 1.1 * L.lmax, lo = hi / 30.0;
+        double hi = 1.1 * L.lmax, lo = hi / 10.0;
         RowArgs<T> a{};
         a.b = b; a.bsrc = bsrc; a.dinv = vdinv<T>(L); a.c1 = (T)(1.0 / (0.5 * (hi + lo)));
         a.xo = bx<T>(L).p; a.y = t;
