@@ -171,6 +171,7 @@ struct Amg {
     const double* vg_r = nullptr;
     double* vg_z = nullptr;
     int vg_deg = 0, vg_prec = -1;
+    bool vg_pre = false;
     void drop_graph() {
         if (vg) (void)hipGraphExecDestroy(vg);
         vg = nullptr; vg_r = nullptr; vg_z = nullptr; vg_prec = -1;

@@ -233,11 +233,14 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         a.xo[row] = xn;
         if (a.zo) a.zo[row] = (double)xn;
     } else {
+        // r, y and xo may be null: a degree-1 smoother reads none of r and
+        // y afterwards, and at the finest level only the FP64 copy zo
         T ri = a.dinv[row] * (a.b[row] - acc);
         T di = ri * a.c1;
-        a.r[row] = ri; a.y[row] = di;
+        if (a.r) a.r[row] = ri;
+        if (a.y) a.y[row] = di;
         T xn = a.x[row] + di;
-        a.xo[row] = xn;
+        if (a.xo) a.xo[row] = xn;
         if (a.zo) a.zo[row] = (double)xn;
     }
 }
@@ -338,11 +341,30 @@ __global__ void __launch_bounds__(TPB) k_spmv_xpby(int n, const int* __restrict_
 
 // ------------------------------------------------------- Krylov vectors --
 // x = 0, r = b, d0 = b.b
+// Level-0 input of the next FP32 V-cycle, written by the kernel that
+// produces the residual r (null b32: not used): b32 = (float) r and the
+// degree-1 smoothed start x32 = c1 D^-1 b32, rounded as the V-cycle's own
+// fused smoothing step forms them.
+struct Prep {
+    float* __restrict__ b32;
+    float* __restrict__ x32;
+    const float* __restrict__ dinv32;
+    float c1;
+};
+__device__ inline void prep_store(const Prep& pp, int i, double ri) {
+    if (pp.b32) {
+        float bf = (float)ri;
+        pp.b32[i] = bf;
+        float rj = pp.dinv32[i] * bf;
+        pp.x32[i] = rj * pp.c1;
+    }
+}
+
 __global__ void __launch_bounds__(TPB) k_fcg_init(int n, const double* __restrict__ b, double* __restrict__ x,
-                                                  double* __restrict__ r, Red rd) {
+                                                  double* __restrict__ r, Red rd, Prep pp) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
-    if (i < n) { double bi = b[i]; x[i] = 0.0; r[i] = bi; v[0] = bi * bi; }
+    if (i < n) { double bi = b[i]; x[i] = 0.0; r[i] = bi; v[0] = bi * bi; prep_store(pp, i, bi); }
     reduce2(v, rd);
 }
 
@@ -350,7 +372,7 @@ __global__ void __launch_bounds__(TPB) k_fcg_init(int n, const double* __restric
 __global__ void __launch_bounds__(TPB) k_fcg_update(int n, const double* __restrict__ s, double* __restrict__ x,
                                                     double* __restrict__ r, double* __restrict__ rold,
                                                     const double* __restrict__ p, const double* __restrict__ q,
-                                                    Red rd) {
+                                                    Red rd, Prep pp) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (i < n) {
@@ -361,6 +383,7 @@ __global__ void __launch_bounds__(TPB) k_fcg_update(int n, const double* __restr
         ri -= a * q[i];
         r[i] = ri;
         v[0] = ri * ri;
+        prep_store(pp, i, ri);
     }
     reduce2(v, rd);
 }
@@ -660,23 +683,35 @@ Tags level_tags(int l, bool fp64) {
 // iterate held in the level's x buffer.  x_zero: start from x = 0 (no SpMV
 // for the start).  bsrc: FP64 right-hand side to convert into b first.  zo:
 // FP64 copy of the final iterate.
+// Chebyshev interval [lo, hi] of a level's smoother
+inline void cheb_interval(const AmgLevel& L, double& lo, double& hi) { hi = 1.1 * L.lmax; lo = hi / 10.0; }
+
 template <class T>
 void cheb_smooth(semo_ctx* c, AmgLevel& L, T* b, int degree, bool x_zero, const double* bsrc, double* zo,
                  const Tags& tg) {
     const DevSell<T>& A = opA<T>(L);
     int n = L.n;
     T* dA = vt<T>(L); T* dB = vu<T>(L); T* r = vr<T>(L);
-    double hi = 1.1 * L.lmax, lo = hi / 10.0;
+    double hi, lo;
+    cheb_interval(L, lo, hi);
     double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
     double sigma = theta / delta, rho = 1.0 / sigma;
     if (x_zero) {
         hipLaunchKernelGGL(k_cheb_init<T>, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, vdinv<T>(L), b, bsrc,
                            (T)(1.0 / theta), r, dA, bx<T>(L).p, degree == 1 ? zo : nullptr);
+    } else if (degree == 1) {
+        // the only step: r and y are dead, and with zo (the finest level)
+        // so is the T iterate, which keeps the level's x buffer in place
+        RowArgs<T> a{};
+        a.x = bx<T>(L).p; a.b = b; a.dinv = vdinv<T>(L); a.zo = zo; a.c1 = (T)(1.0 / theta);
+        if (!zo) a.xo = bw<T>(L).p;
+        row_launch<T, OP_CHEBX>(c, A, a, tg.chebx, op_bytes(A, 3) + (zo ? 8.0 * n : (double)sizeof(T) * n));
+        if (!zo) std::swap(bx<T>(L), bw<T>(L));
     } else {
         RowArgs<T> a{};
         a.x = bx<T>(L).p; a.b = b; a.y = dA; a.dinv = vdinv<T>(L); a.r = r; a.xo = bw<T>(L).p;
-        a.zo = degree == 1 ? zo : nullptr; a.c1 = (T)(1.0 / theta);
-        row_launch<T, OP_CHEBX>(c, A, a, tg.chebx, op_bytes(A, 7));
+        a.c1 = (T)(1.0 / theta);
+        row_launch<T, OP_CHEBX>(c, A, a, tg.chebx, op_bytes(A, 6));
         std::swap(bx<T>(L), bw<T>(L));       // the new iterate is the level's x from here on
     }
     T* x = bx<T>(L).p;
@@ -694,8 +729,10 @@ void cheb_smooth(semo_ctx* c, AmgLevel& L, T* b, int degree, bool x_zero, const 
 
 // one V-cycle for level l on b; the result is left in the level's x buffer
 // (and, at the finest level, converted into zo)
+// prepared (finest level, degree 1): b and the smoothed start x of level 0
+// were written by the Krylov kernel that produced the residual (Prep)
 template <class T>
-void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double* zo) {
+void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double* zo, bool prepared = false) {
     AmgLevel& L = amg.lv[l];
     int n = L.n;
     T* b = vb<T>(L);
@@ -710,9 +747,12 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
     Tags tg = level_tags(l, sizeof(T) == 8);
     T* t = vt<T>(L);
     const DevSell<T>& A = opA<T>(L);
-    if (degree == 1) {
+    if (degree == 1 && l == 0 && prepared) {
+        spmv<T, OP_RES>(c, A, bx<T>(L).p, b, t, tg.resid);
+    } else if (degree == 1) {
         // pre-smoothing from x = 0 and the residual in one pass
-        double hi = 1.1 * L.lmax, lo = hi / 10.0;
+        double hi, lo;
+        cheb_interval(L, lo, hi);
         RowArgs<T> a{};
         a.b = b; a.bsrc = bsrc; a.dinv = vdinv<T>(L); a.c1 = (T)(1.0 / (0.5 * (hi + lo)));
         a.xo = bx<T>(L).p; a.y = t;
@@ -731,24 +771,27 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
 // sequence of ~22 kernels over fixed buffers, so it is captured once as a HIP
 // graph and replayed: one submission instead of ~22.  It holds no memset and
 // no library call.  Profiling (per-kernel events) runs it eagerly.
+int cheb_degree(const semo_ctx* c) { return c->sp.cheb_degree > 0 ? c->sp.cheb_degree : 2; }
+
 template <class T>
-void precondition(semo_ctx* c, Amg& amg, const double* r64, double* z64) {
-    int deg = c->sp.cheb_degree > 0 ? c->sp.cheb_degree : 2;
+void precondition(semo_ctx* c, Amg& amg, const double* r64, double* z64, bool prepared = false) {
+    int deg = cheb_degree(c);
     if (!c->graphs || c->prof.on) {
-        vcycle<T>(c, amg, 0, deg, r64, z64);
+        vcycle<T>(c, amg, 0, deg, r64, z64, prepared);
         return;
     }
     int prec = sizeof(T) == 4 ? 1 : 0;
-    if (!amg.vg || amg.vg_r != r64 || amg.vg_z != z64 || amg.vg_deg != deg || amg.vg_prec != prec) {
+    if (!amg.vg || amg.vg_r != r64 || amg.vg_z != z64 || amg.vg_deg != deg || amg.vg_prec != prec ||
+        amg.vg_pre != prepared) {
         amg.drop_graph();
         hipGraph_t g = nullptr;
         SEMO_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        vcycle<T>(c, amg, 0, deg, r64, z64);
+        vcycle<T>(c, amg, 0, deg, r64, z64, prepared);
         SEMO_HIP(hipStreamEndCapture(c->stream, &g));
         hipError_t e = hipGraphInstantiate(&amg.vg, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         SEMO_HIP(e);
-        amg.vg_r = r64; amg.vg_z = z64; amg.vg_deg = deg; amg.vg_prec = prec;
+        amg.vg_r = r64; amg.vg_z = z64; amg.vg_deg = deg; amg.vg_prec = prec; amg.vg_pre = prepared;
     }
     SEMO_HIP(hipGraphLaunch(amg.vg, c->stream));
 }
@@ -803,12 +846,26 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
     size_t ln = (size_t)n + 64;
     c->kv64.ensure(8 * ln);
     double* r = c->kv64.p; double* z = r + ln; double* p = z + ln; double* q = p + ln; double* rold = q + ln;
-    hipLaunchKernelGGL(k_fcg_init, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, x, r, red_of(c));
+    // mixed precision, degree-1 smoothing, at least two levels: the kernels
+    // that produce r also write the FP32 V-cycle's level-0 input
+    Prep pp{};
+    bool prep = mixed && cheb_degree(c) == 1 && amg.lv.size() >= 2 && amg.lv[0].n == n;
+    if (prep) {
+        AmgLevel& L0 = amg.lv[0];
+        double hi, lo;
+        cheb_interval(L0, lo, hi);
+        pp.b32 = vb<float>(L0); pp.x32 = bx<float>(L0).p; pp.dinv32 = vdinv<float>(L0);
+        pp.c1 = (float)(1.0 / (0.5 * (hi + lo)));
+    }
+    auto pre = [&]() {
+        if (mixed) precondition<float>(c, amg, r, z, prep); else precondition<double>(c, amg, r, z);
+    };
+    hipLaunchKernelGGL(k_fcg_init, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, x, r, red_of(c), pp);
     finish(c, nblk(n), W_STORE, S_X);
     double bb = host_scalar(c, S_X);
     if (bb == 0.0) { *relres = 0.0; return 0; }
     double nb = std::sqrt(bb);
-    if (mixed) precondition<float>(c, amg, r, z); else precondition<double>(c, amg, r, z);
+    pre();
     hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, z, p);
     dot_store(c, n, r, z, nullptr, nullptr, S_RZ);
     int it = 0;
@@ -830,11 +887,11 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
             std::swap(p, p2);
         }
         hipLaunchKernelGGL(k_fcg_update, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->scal.p, x, r, rold, p, q,
-                           red_of(c));
+                           red_of(c), pp);
         finish(c, nblk(n), W_STORE, S_RR);
         pipe_push(c, c->scal.p + S_RR, it);
         if (cc.done(it) || it == maxit) break;
-        if (mixed) precondition<float>(c, amg, r, z); else precondition<double>(c, amg, r, z);
+        pre();
         dot_what(c, n, r, z, z, rold, W_FCG_BETA);                        // r.z, z.r_old -> beta
     }
     // true residual in FP64
