@@ -181,6 +181,37 @@ __device__ inline T row_operand(const RowArgs<T>& a, int j) {
     }
 }
 
+// entries per trip of the row sums: 4 on sliced-ELL slices (rows of ~6-7
+// entries), 8 per lane of the CSR kernels (coarse rows of 30-60 entries over
+// avg/4 lanes); measured best of 1/4/8 and 1/2/4/8 on MI355X
+constexpr int SELL_UNR = 4, CSR_UNR = 8;
+// One lane's sum over its row of a sliced-ELL slice, entries e0, e0 + 64, ...
+// below e1, in entry order: acc += val[e] * opnd(col(e)).  U entries per
+// trip: their indices and values are loaded first, then their gathers are
+// issued together, so the gathers' latency is paid once per trip instead of
+// once per entry.  The sum is the same as the one-entry loop's, bit for bit.
+// (STRIDE 64; the CSR kernels use it with STRIDE = lanes per row.)
+template <int U, int STRIDE, class T, class ColF, class OpF>
+__device__ inline T sell_row_sum(int e0, int e1, const T* __restrict__ val, ColF colf, OpF opnd) {
+    T acc = 0;
+    for (int e = e0; e < e1; e += STRIDE * U) {
+        int j[U];
+        T v[U], x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int ee = e + STRIDE * u;
+            j[u] = ee < e1 ? colf(ee) : -1;
+            v[u] = ee < e1 ? val[ee] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = j[u] >= 0 ? opnd(j[u]) : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) acc += v[u] * x[u];
+    }
+    return acc;
+}
+
 // G == 0: sliced ELL, one lane per row.  G > 0: CSR, G consecutive lanes per
 // row, strided partial sums combined by a fixed shuffle tree (the result is
 // deterministic for a given G).
@@ -193,18 +224,20 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         row = gid;
         int s = row >> 6, lane = row & 63;
         if (s >= ((a.n + 63) >> 6)) return;
-        if (a.dcol) {
-            for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64)
-                acc += a.val[e] * row_operand<T, OP>(a, row + a.dcol[e]);
-        } else {
-            for (int e = a.ptr[s] + lane; e < a.ptr[s + 1]; e += 64) acc += a.val[e] * row_operand<T, OP>(a, a.col[e]);
-        }
+        auto opnd = [&](int j) { return row_operand<T, OP>(a, j); };
+        if (a.dcol)
+            acc = sell_row_sum<SELL_UNR, 64>(a.ptr[s] + lane, a.ptr[s + 1], a.val,
+                                              [&](int e) { return row + (int)a.dcol[e]; }, opnd);
+        else
+            acc = sell_row_sum<SELL_UNR, 64>(a.ptr[s] + lane, a.ptr[s + 1], a.val,
+                                              [&](int e) { return a.col[e]; }, opnd);
         if (row >= a.n) return;
     } else {
         row = gid / G;
         int sub = gid & (G - 1);
         if (row >= a.n) return;
-        for (int e = a.ptr[row] + sub; e < a.ptr[row + 1]; e += G) acc += a.val[e] * row_operand<T, OP>(a, a.col[e]);
+        acc = sell_row_sum<CSR_UNR, G, T>(a.ptr[row] + sub, a.ptr[row + 1], a.val, [&](int e) { return a.col[e]; },
+                                            [&](int j) { return row_operand<T, OP>(a, j); });
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, G);
         if (sub) return;
@@ -293,12 +326,11 @@ __global__ void __launch_bounds__(TPB) k_spmv_dot(int n, const int* __restrict__
     int s = row >> 6, lane = row & 63;
     double v[2] = {0.0, 0.0};
     if (s < ((n + 63) >> 6)) {
-        double acc = 0.0;
-        if (dcol) {
-            for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) acc += val[e] * x[row + dcol[e]];
-        } else {
-            for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) acc += val[e] * x[col[e]];
-        }
+        auto opnd = [&](int j) { return x[j]; };
+        double acc = dcol ? sell_row_sum<SELL_UNR, 64>(sl_ptr[s] + lane, sl_ptr[s + 1], val,
+                                                        [&](int e) { return row + (int)dcol[e]; }, opnd)
+                          : sell_row_sum<SELL_UNR, 64>(sl_ptr[s] + lane, sl_ptr[s + 1], val,
+                                                        [&](int e) { return col[e]; }, opnd);
         if (row < n) {
             acc += diag[row] * x[row];
             y[row] = acc;
@@ -323,11 +355,11 @@ __global__ void __launch_bounds__(TPB) k_spmv_xpby(int n, const int* __restrict_
     double v[2] = {0.0, 0.0};
     if (sl < ((n + 63) >> 6)) {
         double beta = s[S_BETA];
-        double acc = 0.0;
-        for (int e = sl_ptr[sl] + lane; e < sl_ptr[sl + 1]; e += 64) {
-            int j = dcol ? row + dcol[e] : col[e];
-            acc += val[e] * (z[j] + beta * po[j]);
-        }
+        auto opnd = [&](int j) { return z[j] + beta * po[j]; };
+        double acc = dcol ? sell_row_sum<SELL_UNR, 64>(sl_ptr[sl] + lane, sl_ptr[sl + 1], val,
+                                                        [&](int e) { return row + (int)dcol[e]; }, opnd)
+                          : sell_row_sum<SELL_UNR, 64>(sl_ptr[sl] + lane, sl_ptr[sl + 1], val,
+                                                        [&](int e) { return col[e]; }, opnd);
         if (row < n) {
             double pr = z[row] + beta * po[row];
             pn[row] = pr;
@@ -340,7 +372,6 @@ __global__ void __launch_bounds__(TPB) k_spmv_xpby(int n, const int* __restrict_
 }
 
 // ------------------------------------------------------- Krylov vectors --
-// x = 0, r = b, d0 = b.b
 // Level-0 input of the next FP32 V-cycle, written by the kernel that
 // produces the residual r (null b32: not used): b32 = (float) r and the
 // degree-1 smoothed start x32 = c1 D^-1 b32, rounded as the V-cycle's own
@@ -360,6 +391,7 @@ __device__ inline void prep_store(const Prep& pp, int i, double ri) {
     }
 }
 
+// x = 0, r = b, d0 = b.b
 __global__ void __launch_bounds__(TPB) k_fcg_init(int n, const double* __restrict__ b, double* __restrict__ x,
                                                   double* __restrict__ r, Red rd, Prep pp) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
