@@ -349,16 +349,15 @@ __global__ void __launch_bounds__(TPB) k_mom_face(
     const int* __restrict__ slot_nbr, double* __restrict__ aval, double* __restrict__ s) {
     int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nif) return;
-    int l = in.owner[f], r = in.nbr[f];
-    double area = in.area[f];
-    const double* n = in.nrm + 3 * (size_t)f;
-    const double* e = in.ePN + 3 * (size_t)f;
-    const double* sk = in.skew + 3 * (size_t)f;
-    double wCL = in.wC[f], wCR = 1.0 - wCL;
+    int l = ldnt(in.owner + f), r = ldnt(in.nbr + f);
+    double area = ldnt(in.area + f);
+    double n[3], e[3], sk[3];
+    ldnt3(in.nrm, f, n); ldnt3(in.ePN, f, e); ldnt3(in.skew, f, sk);
+    double wCL = ldnt(in.wC + f), wCR = 1.0 - wCL;
     double muF = 1.0 / (wCL / Mu[l] + wCR / Mu[r]);
     double RL = in.Rho[l], RR = in.Rho[r];
-    double dPN = in.magPN[f], alpha = in.alpha[f];
-    double UnF = face_unf(in, k, f, l, r);
+    double dPN = ldnt(in.magPN + f), alpha = ldnt(in.alpha + f);
+    double UnF = face_unf<true>(in, k, f, l, r);
     double wL = (UnF > 0.0) ? 1.0 : 0.0, wR = 1.0 - wL;
     double PL = in.P[l], PR = in.P[r];
     double uL[3] = {in.U[l], in.V[l], in.W[l]}, uR[3] = {in.U[r], in.V[r], in.W[r]};
@@ -374,10 +373,10 @@ __global__ void __launch_bounds__(TPB) k_mom_face(
     double Y = (wR * RR * UnF * area - alpha * muF / dPN * area);
     double an = -(wL * RR * UnF * area + alpha * muF / dPN * area);
     // entries with |v| < DBL_MIN are dropped (eqMomentum.cpp:699-711)
-    aval[slot_own[f]] = fabs(ao) < 2.2250738585072014e-308 ? 0.0 : ao;
-    aval[slot_nbr[f]] = fabs(an) < 2.2250738585072014e-308 ? 0.0 : an;
-    s[f] = X;
-    s[(size_t)nif + f] = Y;
+    aval[ldnt(slot_own + f)] = fabs(ao) < 2.2250738585072014e-308 ? 0.0 : ao;
+    aval[ldnt(slot_nbr + f)] = fabs(an) < 2.2250738585072014e-308 ? 0.0 : an;
+    stnt(s + f, X);
+    stnt(s + (size_t)nif + f, Y);
     const double* gl[3] = {in.gU + 3 * (size_t)l, in.gV + 3 * (size_t)l, in.gW + 3 * (size_t)l};
     const double* gr[3] = {in.gU + 3 * (size_t)r, in.gV + 3 * (size_t)r, in.gW + 3 * (size_t)r};
     for (int v = 0; v < 3; ++v) {
@@ -387,8 +386,8 @@ __global__ void __launch_bounds__(TPB) k_mom_face(
         for (int a = 0; a < 3; ++a) df += (muF * (wCL * gl[v][a] + wCR * gr[v][a]) * (n[a] - alpha * e[a]) * area);
         double dL = ((gl[v][0] * n[0] + gl[v][1] * n[1] + gl[v][2] * n[2]) * muF * area);
         double dR = ((gr[v][0] * n[0] + gr[v][1] * n[1] + gr[v][2] * n[2]) * muF * area);
-        s[(size_t)(2 + v) * nif + f] = (cL - df - dL);
-        s[(size_t)(5 + v) * nif + f] = (cR - df - dR);
+        stnt(s + (size_t)(2 + v) * nif + f, (cL - df - dL));
+        stnt(s + (size_t)(5 + v) * nif + f, (cR - df - dR));
     }
 }
 

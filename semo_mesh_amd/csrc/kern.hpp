@@ -72,11 +72,27 @@ struct FaceIn {
     int nc;
 };
 
+// load / store of data touched once per launch (per-face geometry, face
+// scratch): non-temporal, so that it does not push the gathered cell data out
+// of the L2
+template <class T> __device__ inline T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+template <class T> __device__ inline void stnt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+template <bool NT, class T> __device__ inline T ld(const T* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT = true>
+__device__ inline void ldnt3(const double* p, size_t f, double* o) {
+    o[0] = ld<NT>(p + 3 * f); o[1] = ld<NT>(p + 3 * f + 1); o[2] = ld<NT>(p + 3 * f + 2);
+}
+
+// NT: the face geometry is read non-temporally (measured faster in the
+// momentum face kernel, slower in the pressure face kernel)
+template <bool NT = false>
 __device__ inline double face_unf(const FaceIn& in, const CaseK& k, int f, int l, int r) {
-    const double* n = in.nrm + 3 * (size_t)f;
-    const double* e = in.ePN + 3 * (size_t)f;
-    const double* sk = in.skew + 3 * (size_t)f;
-    double wCL = in.wC[f], wCR = 1.0 - wCL, alpha = in.alpha[f], dPN = in.magPN[f];
+    double n[3], e[3], sk[3];
+    ldnt3<NT>(in.nrm, f, n); ldnt3<NT>(in.ePN, f, e); ldnt3<NT>(in.skew, f, sk);
+    double wCL = ld<NT>(in.wC + f), wCR = 1.0 - wCL, alpha = ld<NT>(in.alpha + f), dPN = ld<NT>(in.magPN + f);
     double uL[3] = {in.U[l], in.V[l], in.W[l]}, uR[3] = {in.U[r], in.V[r], in.W[r]};
     const double* gl[3] = {in.gU + 3 * (size_t)l, in.gV + 3 * (size_t)l, in.gW + 3 * (size_t)l};
     const double* gr[3] = {in.gU + 3 * (size_t)r, in.gV + 3 * (size_t)r, in.gW + 3 * (size_t)r};
