@@ -146,8 +146,10 @@ enum { OP_Y = 0,       // y = A x
        OP_ADD = 2,     // y += A x
        OP_CHEB = 3,    // Chebyshev step: x = d_k, y = d_k+1, r -= D^-1 A d_k, xo += d_k+1
        OP_CHEBX = 4,   // Chebyshev start from x != 0: r = D^-1 (b - A x), y = r c1, xo = x + y
-       OP_J1RES = 5 }; // degree-1 smoothing from x = 0 fused with the residual: x = c1 D^-1 b
+       OP_J1RES = 5,   // degree-1 smoothing from x = 0 fused with the residual: x = c1 D^-1 b
                        // (b from bsrc when given, stored as T), y = b - A x; x_j formed on the fly
+       OP_YPREP = 6 }; // y = A x and xo = c1 D^-1 y (restriction writing the next level's b and
+                       // its degree-1 smoothed start, rounded as OP_J1RES forms them)
 
 template <class T>
 struct RowArgs {
@@ -252,7 +254,11 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         return;
     }
     if (a.diag) acc += a.diag[row] * a.x[row];
-    if constexpr (OP == OP_Y) {
+    if constexpr (OP == OP_YPREP) {
+        a.y[row] = acc;
+        T rj = a.dinv[row] * acc;
+        a.xo[row] = rj * a.c1;
+    } else if constexpr (OP == OP_Y) {
         a.y[row] = acc;
     } else if constexpr (OP == OP_RES) {
         a.y[row] = a.b[row] - acc;
@@ -761,8 +767,9 @@ void cheb_smooth(semo_ctx* c, AmgLevel& L, T* b, int degree, bool x_zero, const 
 
 // one V-cycle for level l on b; the result is left in the level's x buffer
 // (and, at the finest level, converted into zo)
-// prepared (finest level, degree 1): b and the smoothed start x of level 0
-// were written by the Krylov kernel that produced the residual (Prep)
+// prepared (degree 1): b and the smoothed start x of level l were written by
+// the kernel that produced b (the Krylov residual update at level 0, the
+// restriction below it)
 template <class T>
 void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double* zo, bool prepared = false) {
     AmgLevel& L = amg.lv[l];
@@ -779,7 +786,7 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
     Tags tg = level_tags(l, sizeof(T) == 8);
     T* t = vt<T>(L);
     const DevSell<T>& A = opA<T>(L);
-    if (degree == 1 && l == 0 && prepared) {
+    if (degree == 1 && prepared) {
         spmv<T, OP_RES>(c, A, bx<T>(L).p, b, t, tg.resid);
     } else if (degree == 1) {
         // pre-smoothing from x = 0 and the residual in one pass
@@ -793,8 +800,20 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
         cheb_smooth<T>(c, L, b, degree, true, bsrc, nullptr, tg);
         spmv<T, OP_RES>(c, A, bx<T>(L).p, b, t, tg.resid);
     }
-    spmv<T, OP_Y>(c, opR<T>(L), t, (const T*)nullptr, vb<T>(C), tg.restrict_);
-    vcycle<T>(c, amg, l + 1, degree, nullptr, nullptr);
+    // a coarse level that smooths (not the coarsest) gets its b and its
+    // degree-1 smoothed start from the restriction
+    bool prep_next = degree == 1 && l + 2 < (int)amg.lv.size();
+    if (prep_next) {
+        double chi, clo;
+        cheb_interval(C, clo, chi);
+        RowArgs<T> a{};
+        a.x = t; a.y = vb<T>(C); a.dinv = vdinv<T>(C); a.xo = bx<T>(C).p; a.c1 = (T)(1.0 / (0.5 * (chi + clo)));
+        row_launch<T, OP_YPREP>(c, opR<T>(L), a, tg.restrict_,
+                                op_bytes(opR<T>(L), 0) + (double)sizeof(T) * (n + 3.0 * C.n));
+    } else {
+        spmv<T, OP_Y>(c, opR<T>(L), t, (const T*)nullptr, vb<T>(C), tg.restrict_);
+    }
+    vcycle<T>(c, amg, l + 1, degree, nullptr, nullptr, prep_next);
     spmv<T, OP_ADD>(c, opP<T>(L), bx<T>(C).p, (const T*)nullptr, bx<T>(L).p, tg.prolong);
     cheb_smooth<T>(c, L, b, degree, false, nullptr, zo, tg);
 }
