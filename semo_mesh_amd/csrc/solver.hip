@@ -382,11 +382,16 @@ __global__ void __launch_bounds__(TPB) k_spmv_xpby(int n, const int* __restrict_
 // produces the residual r (null b32: not used): b32 = (float) r and the
 // degree-1 smoothed start x32 = c1 D^-1 b32, rounded as the V-cycle's own
 // fused smoothing step forms them.
+// (b64 / x64: the same for an FP64 V-cycle.)
 struct Prep {
     float* __restrict__ b32;
     float* __restrict__ x32;
     const float* __restrict__ dinv32;
     float c1;
+    double* __restrict__ b64;
+    double* __restrict__ x64;
+    const double* __restrict__ dinv64;
+    double c1d;
 };
 __device__ inline void prep_store(const Prep& pp, int i, double ri) {
     if (pp.b32) {
@@ -394,6 +399,10 @@ __device__ inline void prep_store(const Prep& pp, int i, double ri) {
         pp.b32[i] = bf;
         float rj = pp.dinv32[i] * bf;
         pp.x32[i] = rj * pp.c1;
+    } else if (pp.b64) {
+        pp.b64[i] = ri;
+        double rj = pp.dinv64[i] * ri;
+        pp.x64[i] = rj * pp.c1d;
     }
 }
 
@@ -900,16 +909,21 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
     // mixed precision, degree-1 smoothing, at least two levels: the kernels
     // that produce r also write the FP32 V-cycle's level-0 input
     Prep pp{};
-    bool prep = mixed && cheb_degree(c) == 1 && amg.lv.size() >= 2 && amg.lv[0].n == n;
+    bool prep = cheb_degree(c) == 1 && amg.lv.size() >= 2 && amg.lv[0].n == n;
     if (prep) {
         AmgLevel& L0 = amg.lv[0];
         double hi, lo;
         cheb_interval(L0, lo, hi);
-        pp.b32 = vb<float>(L0); pp.x32 = bx<float>(L0).p; pp.dinv32 = vdinv<float>(L0);
-        pp.c1 = (float)(1.0 / (0.5 * (hi + lo)));
+        if (mixed) {
+            pp.b32 = vb<float>(L0); pp.x32 = bx<float>(L0).p; pp.dinv32 = vdinv<float>(L0);
+            pp.c1 = (float)(1.0 / (0.5 * (hi + lo)));
+        } else {
+            pp.b64 = vb<double>(L0); pp.x64 = bx<double>(L0).p; pp.dinv64 = vdinv<double>(L0);
+            pp.c1d = 1.0 / (0.5 * (hi + lo));
+        }
     }
     auto pre = [&]() {
-        if (mixed) precondition<float>(c, amg, r, z, prep); else precondition<double>(c, amg, r, z);
+        if (mixed) precondition<float>(c, amg, r, z, prep); else precondition<double>(c, amg, r, z, prep);
     };
     hipLaunchKernelGGL(k_fcg_init, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, x, r, red_of(c), pp);
     finish(c, nblk(n), W_STORE, S_X);
