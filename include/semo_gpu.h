@@ -156,7 +156,14 @@ typedef enum {
     SEMO_G_FACE_MAGPN, SEMO_G_FACE_ALPHAF, SEMO_G_FACE_EPN, SEMO_G_FACE_SKEW,
     /* face volume fractions varL/varR[fVF] after the last VOF assembly (MSTACS
        on internal faces, boundary values on boundary faces), read-only      */
-    SEMO_F_FACE_VFL = 96, SEMO_F_FACE_VFR
+    SEMO_F_FACE_VFL = 96, SEMO_F_FACE_VFR,
+    /* per-face values of fixedValue-U boundaries, [3][n_faces - n_internal_faces]
+       (component-major, boundary faces in the caller's face order), settable:
+       a fixedValue-U face takes these values instead of its patch's u_value
+       (the form SEMO's UDF inlet profiles take, reconIncom.cpp:707-727);
+       semo_field_set with n = 0 clears them.  The orifice profile of
+       reconIncom.cpp:707-727 still applies on top.                          */
+    SEMO_F_FACE_UFIX = 98
 } semo_field;
 /* host <-> device copies in the caller's numbering; n = element count       */
 int semo_field_set(semo_ctx* ctx, int field, const double* host, int64_t n);

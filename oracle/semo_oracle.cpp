@@ -69,6 +69,11 @@ struct Oracle {
     // Darcy porosity alpha_p of the adjoint harness (no counterpart in the
     // reference): momentum diagonal += alpha_p V, right sides -= alpha_p V U
     vector<double> alphaP;
+    // per-face fixedValue-U values, [3][nbf] over the boundary faces in face
+    // order (an inlet profile; the reference's form of one is a UDF,
+    // reconIncom.cpp:707-727); used once set (ufix_on)
+    vector<double> ufix;
+    bool ufix_on = false;
     vector<double> gP, gU, gV, gW, gVF;    // [3n]
     vector<double> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
     // face left/right values (src/solvers/reconIncom.cpp:501-1035)
@@ -363,7 +368,12 @@ void recon_faces(Oracle& o) {
             o.RP[f] = o.LP[f];
             // velocity (reconIncom.cpp:642-852)
             if (pb.u_type == FIXED_VALUE) {
-                o.LU[f] = pb.u_val[0]; o.LV[f] = pb.u_val[1]; o.LW[f] = pb.u_val[2];
+                double uf[3] = {pb.u_val[0], pb.u_val[1], pb.u_val[2]};
+                if (o.ufix_on) {
+                    size_t nb = (size_t)(o.nf - o.nif), j = (size_t)(f - o.nif);
+                    for (int a = 0; a < 3; ++a) uf[a] = o.ufix[a * nb + j];
+                }
+                o.LU[f] = uf[0]; o.LV[f] = uf[1]; o.LW[f] = uf[2];
                 // live orifice profile (reconIncom.cpp:707-727)
                 for (int k = 0; k < 24; ++k) {
                     double cen = 0.0006 * (double)k;
@@ -376,7 +386,7 @@ void recon_faces(Oracle& o) {
                         o.LW[f] = 0.0 * sin(th) + 0.1 * cos(th);
                     }
                 }
-                o.RU[f] = pb.u_val[0]; o.RV[f] = pb.u_val[1]; o.RW[f] = pb.u_val[2];
+                o.RU[f] = uf[0]; o.RV[f] = uf[1]; o.RW[f] = uf[2];
             } else if (pb.u_type == ZERO_GRADIENT) {
                 o.LU[f] = o.U[c]; o.LV[f] = o.V[c]; o.LW[f] = o.W[c];
                 o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
@@ -1456,6 +1466,7 @@ void* orc_create(int np, int nc, int nf, int nif, const double* pts, const int* 
     z(o->oldP); z(o->oldU); z(o->oldV); z(o->oldW); z(o->oldVF, 1.0); z(o->oldRho);
     for (int k = 0; k < 4; ++k) { z(o->srcG[k]); z(o->srcS[k]); }
     z(o->kappa); z(o->alphaP);
+    o->ufix.assign(3 * (size_t)(o->nf - o->nif), 0.0);
     return o;
 }
 
@@ -1516,7 +1527,7 @@ static vector<double>* field(Oracle& o, const char* nm) {
     F("faceLU", LU) F("faceLV", LV) F("faceLW", LW) F("faceLRho", LRho) F("faceLVF", LVF)
     F("oldVF", oldVF) F("Qm3", Qm3) F("courant", co_cell) F("kappa", kappa) F("LS", LS)
     F("srcS0", srcS[0]) F("srcS1", srcS[1]) F("srcS2", srcS[2]) F("srcS3", srcS[3])
-    F("alphaP", alphaP)
+    F("alphaP", alphaP) F("ufix", ufix)
 #undef F
     return nullptr;
 }
@@ -1550,6 +1561,7 @@ int orc_set(void* h, const char* nm, const double* in) {
     vector<double>* v = field(o, nm);
     if (!v) return -1;
     memcpy(v->data(), in, v->size() * 8);
+    if (v == &o.ufix) o.ufix_on = true;
     return 0;
 }
 

@@ -26,10 +26,10 @@ F = dict(P=0, U=1, V=2, W=3, VF=4, Rho=5, mu=6, oldU=7, oldV=8, oldW=9, Qm0=10, 
          x0=48, x1=49, x2=50, b0=56, b1=57, b2=58,
          cell_volume=64, cell_xyz=65, cell_lsq=66,
          face_area=80, face_normal=81, face_xyz=82, face_wC=83, face_magPN=84, face_alphaF=85,
-         face_ePN=86, face_skew=87, faceLVF=96, faceRVF=97)
+         face_ePN=86, face_skew=87, faceLVF=96, faceRVF=97, face_Ufix=98)
 COMPONENTS = dict(gradP=3, gradU=3, gradV=3, gradW=3, gradVF=3, cell_xyz=3, cell_lsq=6,
                   face_normal=3, face_xyz=3, face_ePN=3, face_skew=3)
-FACE_FIELDS = {k for k in F if k.startswith("face")}
+FACE_FIELDS = {k for k in F if k.startswith("face") and k != "face_Ufix"}
 
 
 class MeshDesc(C.Structure):

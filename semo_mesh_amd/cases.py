@@ -9,6 +9,8 @@ conditions.
 """
 from __future__ import annotations
 
+import numpy as np
+
 from dataclasses import dataclass, field
 
 # boundary-condition type codes shared with the C ABI (include/semo_gpu.h)
@@ -107,3 +109,28 @@ def channel_case(u_in: float, dt: float, rho: float = 1.177, mu: float = 1.85e-5
     bcs["xmin"] = PatchBC(u_type=BC_FIXED_VALUE, u_value=(u_in, 0.0, 0.0))
     bcs["xmax"] = PatchBC(p_type=BC_FIXED_VALUE, p_value=0.0, u_type=BC_ZERO_GRADIENT)
     return Case(rho=(rho, rho), mu=(mu, mu), dt=dt, bcs=bcs)
+
+
+def inlet_profile(mesh, patch: str, u_in: float, amplitude: float = 0.02, seed: int = 20261206):
+    """Per-face fixedValue-U values ([3][n_boundary_faces], for
+    GpuSolver.set_boundary_velocity) with a seeded sinusoidal perturbation
+    on one inlet patch: u = u_in (1 + amplitude sin(2 pi (y/Ly + a)) sin(2 pi
+    (z/Lz + b))) along +x, a and b seeded phases; every other boundary face
+    gets (u_in, 0, 0) (ignored unless it is fixedValue-U)."""
+    nif = mesh.n_internal_faces
+    nb = mesh.n_faces - nif
+    u = np.zeros((3, nb))
+    u[0] = u_in
+    b = mesh.patch(patch)
+    f0, nf = int(mesh.patch_start[b]), int(mesh.patch_nfaces[b])
+    ptr = np.asarray(mesh.face_ptr[f0:f0 + nf + 1], dtype=np.int64)
+    pts = mesh.points[np.asarray(mesh.face_pts[ptr[0]:ptr[-1]])]
+    cen = np.add.reduceat(pts, ptr[:-1] - ptr[0], axis=0) / np.diff(ptr)[:, None]
+    lo, hi = mesh.points.min(axis=0), mesh.points.max(axis=0)
+    ph = np.random.default_rng(seed).random(2)
+    y = (cen[:, 1] - lo[1]) / (hi[1] - lo[1])
+    z = (cen[:, 2] - lo[2]) / (hi[2] - lo[2])
+    u[0, f0 - nif:f0 - nif + nf] = u_in * (1.0 + amplitude * np.sin(2 * np.pi * (y + ph[0]))
+                                           * np.sin(2 * np.pi * (z + ph[1])))
+    return u
+

@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(TPB) k_recon_boundary(
     const int* __restrict__ owner, const double* __restrict__ nrm, const double* __restrict__ fxyz,
     const double* __restrict__ P, const double* __restrict__ U, const double* __restrict__ V,
     const double* __restrict__ W, const double* __restrict__ VF, const double* __restrict__ Mu,
-    double* __restrict__ bP, double* __restrict__ bUL, double* __restrict__ bUR,
+    const double* __restrict__ ufix, double* __restrict__ bP, double* __restrict__ bUL, double* __restrict__ bUR,
     double* __restrict__ bVF, double* __restrict__ bMu, double* __restrict__ bRho) {
     int j = xcd_block() * blockDim.x + threadIdx.x;
     if (j >= nbf) return;
@@ -57,7 +57,8 @@ __global__ void __launch_bounds__(TPB) k_recon_boundary(
     double uc[3] = {U[c], V[c], W[c]};
     switch (bc.u_type) {
         case SEMO_BC_FIXED_VALUE: {
-            for (int a = 0; a < 3; ++a) { uL[a] = bc.u_value[a]; uR[a] = bc.u_value[a]; }
+            // per-face values when given, else the patch's uniform value
+            for (int a = 0; a < 3; ++a) uL[a] = uR[a] = ufix ? ufix[a * (size_t)nbf + j] : bc.u_value[a];
             // live orifice profile on fixedValue-U faces (reconIncom.cpp:707-727)
             double fx = fxyz[3 * (size_t)f];
             for (int q = 0; q < 24; ++q) {
@@ -1159,8 +1160,8 @@ void recon_boundary(semo_ctx* c) {
     ProfScope ps(c, "recon_boundary", 120.0 * c->nbf);
     hipLaunchKernelGGL(k_recon_boundary, dim3(nblk(c->nbf)), dim3(TPB), 0, c->stream, c->nbf, c->nif, k,
                        c->d_bpatch.p, c->d_bcs.p, c->d_owner.p, c->f_nrm.p, c->f_xyz.p, c->P.p, c->U.p,
-                       c->V.p, c->W.p, c->VF.p, c->Mu.p, c->bP.p, c->bUL.p, c->bUR.p, c->bVF.p, c->bMu.p,
-                       c->bRho.p);
+                       c->V.p, c->W.p, c->VF.p, c->Mu.p, c->b_ufix.n ? c->b_ufix.p : nullptr, c->bP.p, c->bUL.p,
+                       c->bUR.p, c->bVF.p, c->bMu.p, c->bRho.p);
 }
 
 void source_gravity(semo_ctx* c) {

@@ -237,6 +237,7 @@ int semo_mesh_upload(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
         SEMO_HIP(hipSetDevice(c->device));
         c->has_mesh = false; c->has_case = false; c->iterReal = 0;
         c->amg_p.reset();
+        c->b_ufix.release(); c->h_ufix.clear();
         coupled_free(c->cpl);          // mesh-bound block pattern and work arrays
         c->cpl = nullptr;
         adjoint_free(c->adj);
@@ -322,9 +323,25 @@ int semo_set_solver(semo_ctx* c, const semo_solver_params* p) {
 }
 
 int semo_field_set(semo_ctx* c, int f, const double* h, int64_t n) {
-    if (!c || !h) return SEMO_EINVAL;
+    if (!c || (!h && !(f == SEMO_F_FACE_UFIX && n == 0))) return SEMO_EINVAL;
     return guard(c, [&] {
         need_mesh(c);
+        if (f == SEMO_F_FACE_UFIX) {      // [3][nbf], caller's boundary-face order
+            if (n == 0) { c->b_ufix.release(); c->h_ufix.clear(); return; }
+            size_t nb = (size_t)c->nbf;
+            SEMO_CHECK(n == (int64_t)(3 * nb), SEMO_EINVAL, "field size");
+            for (int64_t q = 0; q < n; ++q)
+                SEMO_CHECK(std::isfinite(h[q]), SEMO_EINVAL, "non-finite boundary velocity");
+            c->h_ufix.assign(h, h + n);
+            std::vector<double> d(3 * nb);
+            for (size_t j = 0; j < nb; ++j) {
+                size_t o = (size_t)c->fperm[c->nif + j] - (size_t)c->nif;   // caller's boundary-face index
+                for (int a = 0; a < 3; ++a) d[a * nb + j] = h[a * nb + o];
+            }
+            c->b_ufix.upload(d, c->stream);
+            SEMO_HIP(hipStreamSynchronize(c->stream));
+            return;
+        }
         int comps;
         DBuf<double>* d = cell_field(c, f, comps);
         if (f >= SEMO_F_SRCS0 && f <= SEMO_F_SRCS2) {   // one component of the surface-tension source
@@ -354,6 +371,12 @@ int semo_field_get(semo_ctx* c, int f, double* h, int64_t n) {
     if (!c || !h) return SEMO_EINVAL;
     return guard(c, [&] {
         need_mesh(c);
+        if (f == SEMO_F_FACE_UFIX) {
+            SEMO_CHECK(!c->h_ufix.empty(), SEMO_ESTATE, "no per-face boundary velocity set");
+            SEMO_CHECK(n == (int64_t)c->h_ufix.size(), SEMO_EINVAL, "field size");
+            std::copy(c->h_ufix.begin(), c->h_ufix.end(), h);
+            return;
+        }
         if (f >= SEMO_F_SRCG0 && f <= SEMO_F_SRCG2) {
             SEMO_CHECK(n == c->nc, SEMO_EINVAL, "field size");
             from_dev(c, c->srcG.p + (size_t)(f - SEMO_F_SRCG0) * c->nc, h, 1);

@@ -221,6 +221,10 @@ struct semo_ctx {
     // the same slices as int16 offsets neighbour - cell, -32768 padding; empty
     // when some neighbour lies further than 32767 from its cell
     semo::DBuf<int16_t> d_st_dl;
+    // per-face fixedValue-U values [3][nbf] (device face order), empty: the
+    // patches' uniform values
+    semo::DBuf<double> b_ufix;
+    std::vector<double> h_ufix;     // as set, in the caller's numbering
     // cell -> faces as sliced ELL (ascending faces; ~f for a face the cell
     // is the neighbour of; padding INT_MIN) and cell -> boundary faces (CSR)
     semo::DBuf<int> d_cf_sl, d_cf_ell, d_bf_ptr, d_bf_idx;

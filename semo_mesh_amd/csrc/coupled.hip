@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(TPB) k_cpl_update(int nc, double urf_p, double
 // device scalars of the block BiCGStab (zero denominators give zero steps,
 // so that an iteration issued past convergence leaves x unchanged)
 enum { C_RHO = 0, C_ALPHA, C_OMEGA, C_BETA, C_RR, C_BB, C_N };
-enum { F_STORE_BB = 0, F_RHO, F_ALPHA, F_OMEGA, F_RR };
+enum { F_STORE_BB = 0, F_RHO, F_ALPHA, F_OMEGA, F_RR, F_RESTART };
 
 __global__ void __launch_bounds__(TPB) k_cfinish(int nb, const double* __restrict__ part, double* __restrict__ sc,
                                                  int what) {
@@ -339,6 +339,7 @@ __global__ void __launch_bounds__(TPB) k_cfinish(int nb, const double* __restric
     case F_ALPHA: sc[C_ALPHA] = d0 != 0.0 ? sc[C_RHO] / d0 : 0.0; break;     // d0 = rh.v
     case F_OMEGA: sc[C_OMEGA] = d1 != 0.0 ? d0 / d1 : 0.0; break;            // d0 = t.s, d1 = t.t
     case F_RR: sc[C_RR] = d0; break;
+    case F_RESTART: sc[C_RR] = d0; sc[C_RHO] = 1.0; sc[C_ALPHA] = 1.0; sc[C_OMEGA] = 1.0; break;
     default: break;
     }
 }
@@ -369,6 +370,20 @@ __global__ void __launch_bounds__(TPB) k_cb_x(size_t n, const double* __restrict
         x[i] += a * ph[i] + w * sh[i];
         double ri = s[i] - w * t[i];
         r[i] = ri;
+        rr = ri * ri;
+    }
+    block_sum2(rr, 0.0, part);
+}
+// restart from the current x (r = b - A x already formed): rh = r,
+// p = v = 0; partials of r.r
+__global__ void __launch_bounds__(TPB) k_cb_restart(size_t n, const double* __restrict__ r, double* __restrict__ rh,
+                                                    double* __restrict__ p, double* __restrict__ v,
+                                                    double* __restrict__ part) {
+    size_t i = (size_t)xcd_block() * blockDim.x + threadIdx.x;
+    double rr = 0.0;
+    if (i < n) {
+        double ri = r[i];
+        rh[i] = ri; p[i] = 0.0; v[i] = 0.0;
         rr = ri * ri;
     }
     block_sum2(rr, 0.0, part);
@@ -636,7 +651,11 @@ void coupled_spmv(semo_ctx* c, const double* x, double* y) {
 // right-preconditioned BiCGStab, x0 = 0, stop at ||b - A x|| <= tol ||b||.
 // The Krylov scalars stay on the device; the host reads one residual per
 // iteration, one iteration behind (pinned slot + event), so the stream never
-// drains while it waits.
+// drains while it waits.  Restarted from the current iterate (true residual,
+// new shadow vector) when RESTART_AFTER iterations have not halved the
+// smallest residual seen since the last (re)start: in transients the
+// unrestarted iteration was seen to stall for hundreds of iterations.
+constexpr int RESTART_AFTER = 30;
 void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st) {
     SEMO_CHECK(c->cpl && c->cpl->built, SEMO_ESTATE, "no coupled system assembled");
     Coupled& C = *c->cpl;
@@ -662,6 +681,8 @@ void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st
     if (bb > 0.0) {
         double nb = std::sqrt(bb), tol = c->sp.tol_u;
         ConvCheck cc(c, tol * tol * bb, bb);
+        double best = bb;
+        int since = 0;
         for (it = 1; it <= c->sp.maxit_u; ++it) {
             dot(rh, r, nullptr, nullptr, F_RHO);
             hipLaunchKernelGGL(k_cb_p, dim3(nbv), dim3(TPB), 0, c->stream, N, sc, r, v, p);
@@ -676,6 +697,15 @@ void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st
             fin(nbv, F_RR);
             pipe_push(c, sc + C_RR, it);
             if (cc.done(it)) break;
+            if (cc.r1 < 0.25 * best) { best = cc.r1; since = 0; }       // squared residuals
+            else if (++since >= RESTART_AFTER) {
+                hipLaunchKernelGGL((k_bsr<double, 1>), dim3(nblk(4 * (long)C.n)), dim3(TPB), 0, c->stream, C.n,
+                                   C.ptr.p, C.col.p, C.val.p, C.diag.p, x, b, r);
+                hipLaunchKernelGGL(k_cb_restart, dim3(nbv), dim3(TPB), 0, c->stream, N, r, rh, p, v, C.red.p);
+                fin(nbv, F_RESTART);
+                since = 0;
+                best = cc.r1;
+            }
         }
         it = std::min(it, c->sp.maxit_u);
         // true residual

@@ -82,8 +82,21 @@ class GpuSolver:
 
     # ---- fields ---------------------------------------------------------------------------
     def _size(self, name):
+        if name == "face_Ufix":
+            return 3 * (self.mesh.n_faces - self.mesh.n_internal_faces)
         n = self.mesh.n_faces if name in FACE_FIELDS else self.mesh.n_cells
         return n * COMPONENTS.get(name, 1)
+
+    def set_boundary_velocity(self, values):
+        """Per-face values of the fixedValue-U boundaries: values[3][n_b] over
+        the boundary faces in the mesh's face order (faces n_internal_faces ..
+        n_faces-1); a fixedValue-U face then takes its entry instead of its
+        patch's uniform value.  None clears them."""
+        if values is None:
+            self._ok(self._L.semo_field_set(self._h, F["face_Ufix"], None, 0))
+            return
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(3, -1)
+        self.set_field("face_Ufix", a)
 
     def set_field(self, name, values):
         a = np.ascontiguousarray(values, dtype=np.float64).ravel()
@@ -93,6 +106,8 @@ class GpuSolver:
         n = self._size(name)
         a = np.empty(n, dtype=np.float64)
         self._ok(self._L.semo_field_get(self._h, F[name], _ptr(a), n))
+        if name == "face_Ufix":
+            return a.reshape(3, -1)
         c = COMPONENTS.get(name, 1)
         return a.reshape(-1, c) if c > 1 else a
 
