@@ -296,28 +296,62 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
     __builtin_nontemporal_store(mx[3], mxW + i); __builtin_nontemporal_store(mn[3], mnW + i);
 }
 
-// standalone calcLeastSquare of one field
-__global__ void __launch_bounds__(TPB) k_lsq1(
-    int nc, int nif, const int* __restrict__ st_ptr, const int* __restrict__ st_idx,
-    const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx, const double* __restrict__ cxyz,
-    const double* __restrict__ fxyz, const double* __restrict__ lsq, const double* __restrict__ phi,
-    const double* __restrict__ faceR, int foff, double* __restrict__ g) {
-    // faceR: boundary values indexed by face - foff; null: "input" mode with
-    // cn = -1, a zero boundary difference (gradient.cpp:1018-1022)
+// one field's LSQ gradient, calcLeastSquare "cellVertex" (gradient.cpp:
+// 761-1159): the vertex stencil read as sliced ELL (int32 ids or int16
+// offsets, as in k_grad_minmax) over packed records xph = [x, y, z, phi] of
+// 32 B, two neighbours per trip, then the cell's boundary faces in face
+// order.  faceR: boundary values indexed by face - foff; null: "input" mode
+// with cn = -1, a zero boundary difference (gradient.cpp:1018-1022).  The
+// sums run in stencil order, as the reference's.
+__global__ void __launch_bounds__(TPB) k_pack_xphi(int nc, const double* __restrict__ cxyz,
+                                                   const double* __restrict__ phi, double* __restrict__ xph) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
-    double xi = cxyz[3 * (size_t)i], yi = cxyz[3 * (size_t)i + 1], zi = cxyz[3 * (size_t)i + 2];
-    double p = phi[i], t0 = 0, t1 = 0, t2 = 0;
-    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) {
-        int j = st_idx[q];
-        double dx = cxyz[3 * (size_t)j] - xi, dy = cxyz[3 * (size_t)j + 1] - yi, dz = cxyz[3 * (size_t)j + 2] - zi;
+    double2* o = reinterpret_cast<double2*>(xph + 4 * (size_t)i);
+    o[0] = make_double2(cxyz[3 * (size_t)i], cxyz[3 * (size_t)i + 1]);
+    o[1] = make_double2(cxyz[3 * (size_t)i + 2], phi[i]);
+}
+
+template <class IDX>
+__global__ void __launch_bounds__(TPB) k_lsq1(
+    int nc, const int* __restrict__ st_sl, const IDX* __restrict__ st_ell, const int* __restrict__ bf_ptr,
+    const int* __restrict__ bf_idx, const double* __restrict__ xph, const double* __restrict__ fxyz,
+    const double* __restrict__ lsq, const double* __restrict__ faceR, int foff, double* __restrict__ g) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const double2* me = reinterpret_cast<const double2*>(xph + 4 * (size_t)i);
+    double2 m0 = me[0], m1 = me[1];
+    double xi = m0.x, yi = m0.y, zi = m1.x, p = m1.y;
+    double t0 = 0, t1 = 0, t2 = 0;
+    auto add = [&](const double2& o0, const double2& o1) {
+        double dx = o0.x - xi, dy = o0.y - yi, dz = o1.x - zi;
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
-        double d = phi[j] - p;
+        double d = o1.y - p;
         t0 += wk * dx * d; t1 += wk * dy * d; t2 += wk * dz * d;
+    };
+    int s = i >> 6, lane = i & 63;
+    int e0 = st_sl[s], e1 = st_sl[s + 1];
+    auto nbr = [&](int k) -> int {
+        int v = __builtin_nontemporal_load(st_ell + k);
+        if constexpr (sizeof(IDX) == 2) return v == -32768 ? -1 : i + v;
+        else return v;
+    };
+    for (int e = e0 + lane; e < e1; e += 128) {
+        int j0 = nbr(e);
+        if (j0 < 0) break;
+        int j1 = e + 64 < e1 ? nbr(e + 64) : -1;
+        const double2* o = reinterpret_cast<const double2*>(xph + 4 * (size_t)j0);
+        double2 a0 = o[0], a1 = o[1], b0, b1;
+        if (j1 >= 0) {
+            const double2* q = reinterpret_cast<const double2*>(xph + 4 * (size_t)j1);
+            b0 = q[0]; b1 = q[1];
+        }
+        add(a0, a1);
+        if (j1 < 0) break;
+        add(b0, b1);
     }
-    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
-        int f = cf_idx[q];
-        if (f < nif) continue;
+    for (int q = bf_ptr[i]; q < bf_ptr[i + 1]; ++q) {
+        int f = bf_idx[q];
         double dx = fxyz[3 * (size_t)f] - xi, dy = fxyz[3 * (size_t)f + 1] - yi, dz = fxyz[3 * (size_t)f + 2] - zi;
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
         double d = faceR ? faceR[f - foff] - p : 0.0;
@@ -937,22 +971,57 @@ __global__ void __launch_bounds__(TPB) k_curv_reinit(
 }
 
 // kappa = -div(n) with the unweighted stencil least squares of each normal
-// component (curvature.cpp:2982-3072, 3074-3077)
-__global__ void __launch_bounds__(TPB) k_curv_kappa(
-    int nc, const int* __restrict__ st_ptr, const int* __restrict__ st_idx, const double* __restrict__ cxyz,
-    const double* __restrict__ nrm, const double* __restrict__ kinv, double* __restrict__ kappa) {
+// component (curvature.cpp:2982-3072, 3074-3077); the stencil as sliced ELL
+// over packed records xn = [x, y, z, nx, ny, nz, -, -] of 64 B, two
+// neighbours per trip, sums in stencil order
+__global__ void __launch_bounds__(TPB) k_pack_xn(int nc, const double* __restrict__ cxyz,
+                                                 const double* __restrict__ nrm, double* __restrict__ xn) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
-    const double* ci = cxyz + 3 * (size_t)i;
-    const double* ni = nrm + 3 * (size_t)i;
+    double2* o = reinterpret_cast<double2*>(xn + 8 * (size_t)i);
+    o[0] = make_double2(cxyz[3 * (size_t)i], cxyz[3 * (size_t)i + 1]);
+    o[1] = make_double2(cxyz[3 * (size_t)i + 2], nrm[3 * (size_t)i]);
+    o[2] = make_double2(nrm[3 * (size_t)i + 1], nrm[3 * (size_t)i + 2]);
+}
+
+template <class IDX>
+__global__ void __launch_bounds__(TPB) k_curv_kappa(
+    int nc, const int* __restrict__ st_sl, const IDX* __restrict__ st_ell, const double* __restrict__ xn,
+    const double* __restrict__ kinv, double* __restrict__ kappa) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const double2* me = reinterpret_cast<const double2*>(xn + 8 * (size_t)i);
+    double2 m0 = me[0], m1 = me[1], m2 = me[2];
+    double ci[3] = {m0.x, m0.y, m1.x}, ni[3] = {m1.y, m2.x, m2.y};
     double r[3] = {0, 0, 0};
-    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) {
-        int j = st_idx[q];
+    auto add = [&](const double2& o0, const double2& o1, const double2& o2) {
+        double cj[3] = {o0.x, o0.y, o1.x}, nj[3] = {o1.y, o2.x, o2.y};
         for (int a = 0; a < 3; ++a) {
-            double d = cxyz[3 * (size_t)j + a] - ci[a];
-            double dv = nrm[3 * (size_t)j + a] - ni[a];
+            double d = cj[a] - ci[a];
+            double dv = nj[a] - ni[a];
             r[a] += 1.0 * d * dv;
         }
+    };
+    int s = i >> 6, lane = i & 63;
+    int e0 = st_sl[s], e1 = st_sl[s + 1];
+    auto nbr = [&](int k) -> int {
+        int v = __builtin_nontemporal_load(st_ell + k);
+        if constexpr (sizeof(IDX) == 2) return v == -32768 ? -1 : i + v;
+        else return v;
+    };
+    for (int e = e0 + lane; e < e1; e += 128) {
+        int j0 = nbr(e);
+        if (j0 < 0) break;
+        int j1 = e + 64 < e1 ? nbr(e + 64) : -1;
+        const double2* o = reinterpret_cast<const double2*>(xn + 8 * (size_t)j0);
+        double2 a0 = o[0], a1 = o[1], a2 = o[2], b0, b1, b2;
+        if (j1 >= 0) {
+            const double2* q = reinterpret_cast<const double2*>(xn + 8 * (size_t)j1);
+            b0 = q[0]; b1 = q[1]; b2 = q[2];
+        }
+        add(a0, a1, a2);
+        if (j1 < 0) break;
+        add(b0, b1, b2);
     }
     const double* v = kinv + 6 * (size_t)i;
     double g0 = v[0] * r[0] + v[1] * r[1] + v[2] * r[2];
@@ -1276,6 +1345,22 @@ void assemble_vof(semo_ctx* c) {
     c->last_eq = 2;
 }
 
+// one field's LSQ gradient (k_lsq1): packs [x, y, z, phi], then the
+// stencil pass; faceR indexed by face - foff, null for a zero boundary
+// difference
+void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double* g) {
+    int n = c->nc;
+    c->xph.ensure(4 * (size_t)n);
+    dim3 gb(nblk(n)), tb(TPB);
+    hipLaunchKernelGGL(k_pack_xphi, gb, tb, 0, c->stream, n, c->c_xyz.p, phi, c->xph.p);
+    if (c->d_st_dl.n)
+        hipLaunchKernelGGL(k_lsq1<int16_t>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
+                           c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g);
+    else
+        hipLaunchKernelGGL(k_lsq1<int>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
+                           c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g);
+}
+
 // calcSourceSurfaceTension with calcCurvature (curvature.cpp:7-3537,
 // surfaceTension.cpp:7-229).  With sigma = 0 every use of kappa vanishes, so
 // the pipeline is skipped and the sources are zero.
@@ -1291,10 +1376,7 @@ void surface_tension(semo_ctx* c) {
         hipLaunchKernelGGL(k_kappa_lsq_init, gb, tb, 0, c->stream, n, c->d_st_ptr.p, c->d_st_idx.p, c->d_bf_ptr.p,
                            c->d_bf_idx.p, c->c_xyz.p, c->f_xyz.p, c->kinv.p);
     }
-    auto lsq_input = [&](const double* phi, double* g) {
-        hipLaunchKernelGGL(k_lsq1, gb, tb, 0, c->stream, n, c->nif, c->d_st_ptr.p, c->d_st_idx.p, c->d_cf_ptr.p,
-                           c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p, phi, (const double*)nullptr, 0, g);
-    };
+    auto lsq_input = [&](const double* phi, double* g) { lsq1(c, phi, nullptr, 0, g); };
     hipLaunchKernelGGL(k_curv_iface, gb, tb, 0, c->stream, n, c->nif, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p,
                        c->d_nbr.p, c->f_wC.p, c->VF.p, c->iface.p, c->ls.p);
     lsq_input(c->ls.p, c->cgrad.p);
@@ -1310,11 +1392,16 @@ void surface_tension(semo_ctx* c) {
     }
     lsq_input(c->ls.p, c->cgrad.p);
     hipLaunchKernelGGL(k_curv_normal, gb, tb, 0, c->stream, n, 1, c->cgrad.p, c->cnrm.p);
-    hipLaunchKernelGGL(k_curv_kappa, gb, tb, 0, c->stream, n, c->d_st_ptr.p, c->d_st_idx.p, c->c_xyz.p, c->cnrm.p,
-                       c->kinv.p, c->kappa.p);
+    c->xn.ensure(8 * (size_t)n);
+    hipLaunchKernelGGL(k_pack_xn, gb, tb, 0, c->stream, n, c->c_xyz.p, c->cnrm.p, c->xn.p);
+    if (c->d_st_dl.n)
+        hipLaunchKernelGGL(k_curv_kappa<int16_t>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_dl.p, c->xn.p,
+                           c->kinv.p, c->kappa.p);
+    else
+        hipLaunchKernelGGL(k_curv_kappa<int>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_ell.p, c->xn.p,
+                           c->kinv.p, c->kappa.p);
     // grad(alpha): LSQ with the boundary values, then one Green-Gauss pass
-    hipLaunchKernelGGL(k_lsq1, gb, tb, 0, c->stream, n, c->nif, c->d_st_ptr.p, c->d_st_idx.p, c->d_cf_ptr.p,
-                       c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p, c->VF.p, c->bVF.p, c->nif, c->cgrad.p);
+    lsq1(c, c->VF.p, c->bVF.p, c->nif, c->cgrad.p);
     hipLaunchKernelGGL(k_st_source, gb, tb, 0, c->stream, n, c->nif, case_k(c), c->d_cf_sl.p, c->d_cf_ell.p,
                        c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_wC.p, c->f_skew.p, c->f_nrm.p,
                        c->f_area.p, c->c_vol.p, c->f_xyz.p, c->c_xyz.p, c->VF.p, c->bVF.p, c->cgrad.p, c->kappa.p,
@@ -1386,9 +1473,7 @@ double pressure_correct(semo_ctx* c) {
 }
 
 void lsq_standalone(semo_ctx* c, const double* phi, const double* faceR, double* grad) {
-    hipLaunchKernelGGL(k_lsq1, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->d_st_ptr.p,
-                       c->d_st_idx.p, c->d_cf_ptr.p, c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p, phi,
-                       faceR, 0, grad);
+    lsq1(c, phi, faceR, 0, grad);
 }
 
 void minmax_standalone(semo_ctx* c, const double* phi, double* mx, double* mn) {

@@ -224,6 +224,9 @@ struct semo_ctx {
     // per-face fixedValue-U values [3][nbf] (device face order), empty: the
     // patches' uniform values
     semo::DBuf<double> b_ufix;
+    // packed gather records of the one-field LSQ ([x, y, z, phi]) and of the
+    // curvature divergence ([x, y, z, n, -, -])
+    semo::DBuf<double> xph, xn;
     std::vector<double> h_ufix;     // as set, in the caller's numbering
     // cell -> faces as sliced ELL (ascending faces; ~f for a face the cell
     // is the neighbour of; padding INT_MIN) and cell -> boundary faces (CSR)
