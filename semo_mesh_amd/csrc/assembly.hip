@@ -312,7 +312,9 @@ __global__ void __launch_bounds__(TPB) k_pack_xphi(int nc, const double* __restr
     o[1] = make_double2(cxyz[3 * (size_t)i + 2], phi[i]);
 }
 
-template <class IDX>
+// (8 neighbours per trip measured best of 2/4/8 on the 1.3 M-tet dam break,
+// whose vertex stencils are large)
+template <class IDX, int UNR = 8>
 __global__ void __launch_bounds__(TPB) k_lsq1(
     int nc, const int* __restrict__ st_sl, const IDX* __restrict__ st_ell, const int* __restrict__ bf_ptr,
     const int* __restrict__ bf_idx, const double* __restrict__ xph, const double* __restrict__ fxyz,
@@ -323,12 +325,6 @@ __global__ void __launch_bounds__(TPB) k_lsq1(
     double2 m0 = me[0], m1 = me[1];
     double xi = m0.x, yi = m0.y, zi = m1.x, p = m1.y;
     double t0 = 0, t1 = 0, t2 = 0;
-    auto add = [&](const double2& o0, const double2& o1) {
-        double dx = o0.x - xi, dy = o0.y - yi, dz = o1.x - zi;
-        double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
-        double d = o1.y - p;
-        t0 += wk * dx * d; t1 += wk * dy * d; t2 += wk * dz * d;
-    };
     int s = i >> 6, lane = i & 63;
     int e0 = st_sl[s], e1 = st_sl[s + 1];
     auto nbr = [&](int k) -> int {
@@ -336,19 +332,35 @@ __global__ void __launch_bounds__(TPB) k_lsq1(
         if constexpr (sizeof(IDX) == 2) return v == -32768 ? -1 : i + v;
         else return v;
     };
-    for (int e = e0 + lane; e < e1; e += 128) {
-        int j0 = nbr(e);
-        if (j0 < 0) break;
-        int j1 = e + 64 < e1 ? nbr(e + 64) : -1;
-        const double2* o = reinterpret_cast<const double2*>(xph + 4 * (size_t)j0);
-        double2 a0 = o[0], a1 = o[1], b0, b1;
-        if (j1 >= 0) {
-            const double2* q = reinterpret_cast<const double2*>(xph + 4 * (size_t)j1);
-            b0 = q[0]; b1 = q[1];
+    // UNR neighbours per trip, records requested before the first is summed,
+    // the next trip's indices before this trip's sums (padding -1 follows a
+    // cell's last neighbour)
+    int e = e0 + lane;
+    int jn[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) jn[u] = e + 64 * u < e1 ? nbr(e + 64 * u) : -1;
+    while (jn[0] >= 0) {
+        int j[UNR];
+        double2 r0[UNR], r1[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            j[u] = jn[u];
+            if (j[u] >= 0) {
+                const double2* o = reinterpret_cast<const double2*>(xph + 4 * (size_t)j[u]);
+                r0[u] = o[0]; r1[u] = o[1];
+            }
         }
-        add(a0, a1);
-        if (j1 < 0) break;
-        add(b0, b1);
+        e += 64 * UNR;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) jn[u] = (j[UNR - 1] >= 0 && e + 64 * u < e1) ? nbr(e + 64 * u) : -1;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (j[u] < 0) break;
+            double dx = r0[u].x - xi, dy = r0[u].y - yi, dz = r1[u].x - zi;
+            double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+            double d = r1[u].y - p;
+            t0 += wk * dx * d; t1 += wk * dy * d; t2 += wk * dz * d;
+        }
     }
     for (int q = bf_ptr[i]; q < bf_ptr[i + 1]; ++q) {
         int f = bf_idx[q];
