@@ -217,15 +217,36 @@ __device__ inline T sell_row_sum(int e0, int e1, const T* __restrict__ val, ColF
 // G == 0: sliced ELL, one lane per row.  G > 0: CSR, G consecutive lanes per
 // row, strided partial sums combined by a fixed shuffle tree (the result is
 // deterministic for a given G).
+// the row's own operands of the epilogue: diagonal and x (for the
+// diagonal term, and x for the Chebyshev start), b, D^-1
+template <class T>
+struct Own {
+    T d, x, b, dv;
+};
+template <class T, int OP>
+__device__ inline Own<T> load_own(const RowArgs<T>& a, int row) {
+    Own<T> o{};
+    if constexpr (OP != OP_J1RES) {
+        if (a.diag) o.d = a.diag[row];
+        if (a.diag || OP == OP_CHEBX || OP == OP_CHEB) o.x = a.x[row];
+    }
+    if constexpr (OP == OP_RES || OP == OP_CHEBX) o.b = a.b[row];
+    if constexpr (OP == OP_YPREP || OP == OP_CHEB || OP == OP_CHEBX) o.dv = a.dinv[row];
+    return o;
+}
+
 template <class T, int G, int OP>
 __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
     int gid = xcd_block() * TPB + threadIdx.x;
     int row;
     T acc = 0;
+    Own<T> own{};
     if constexpr (G == 0) {
         row = gid;
         int s = row >> 6, lane = row & 63;
         if (s >= ((a.n + 63) >> 6)) return;
+        // the row's own operands are requested before the gathers
+        if (row < a.n) own = load_own<T, OP>(a, row);
         auto opnd = [&](int j) { return row_operand<T, OP>(a, j); };
         if (a.dcol)
             acc = sell_row_sum<SELL_UNR, 64>(a.ptr[s] + lane, a.ptr[s + 1], a.val,
@@ -243,6 +264,7 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, G);
         if (sub) return;
+        own = load_own<T, OP>(a, row);
     }
     if constexpr (OP == OP_J1RES) {
         T bi = a.bsrc ? (T)a.bsrc[row] : a.b[row];
@@ -253,20 +275,20 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         a.y[row] = bi - acc;
         return;
     }
-    if (a.diag) acc += a.diag[row] * a.x[row];
+    if (a.diag) acc += own.d * own.x;
     if constexpr (OP == OP_YPREP) {
         a.y[row] = acc;
-        T rj = a.dinv[row] * acc;
+        T rj = own.dv * acc;
         a.xo[row] = rj * a.c1;
     } else if constexpr (OP == OP_Y) {
         a.y[row] = acc;
     } else if constexpr (OP == OP_RES) {
-        a.y[row] = a.b[row] - acc;
+        a.y[row] = own.b - acc;
     } else if constexpr (OP == OP_ADD) {
         a.y[row] += acc;
     } else if constexpr (OP == OP_CHEB) {
-        T ri = a.r[row] - a.dinv[row] * acc;
-        T di = a.c1 * a.x[row] + a.c2 * ri;
+        T ri = a.r[row] - own.dv * acc;
+        T di = a.c1 * own.x + a.c2 * ri;
         a.r[row] = ri; a.y[row] = di;
         T xn = a.xo[row] + di;
         a.xo[row] = xn;
@@ -274,11 +296,11 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
     } else {
         // r, y and xo may be null: a degree-1 smoother reads none of r and
         // y afterwards, and at the finest level only the FP64 copy zo
-        T ri = a.dinv[row] * (a.b[row] - acc);
+        T ri = own.dv * (own.b - acc);
         T di = ri * a.c1;
         if (a.r) a.r[row] = ri;
         if (a.y) a.y[row] = di;
-        T xn = a.x[row] + di;
+        T xn = own.x + di;
         if (a.xo) a.xo[row] = xn;
         if (a.zo) a.zo[row] = (double)xn;
     }
@@ -361,15 +383,18 @@ __global__ void __launch_bounds__(TPB) k_spmv_xpby(int n, const int* __restrict_
     double v[2] = {0.0, 0.0};
     if (sl < ((n + 63) >> 6)) {
         double beta = s[S_BETA];
+        // the row's own operands are requested before the gathers
+        bool own = row < n;
+        double zr = own ? z[row] : 0.0, por = own ? po[row] : 0.0, dr = own ? diag[row] : 0.0;
         auto opnd = [&](int j) { return z[j] + beta * po[j]; };
         double acc = dcol ? sell_row_sum<SELL_UNR, 64>(sl_ptr[sl] + lane, sl_ptr[sl + 1], val,
                                                         [&](int e) { return row + (int)dcol[e]; }, opnd)
                           : sell_row_sum<SELL_UNR, 64>(sl_ptr[sl] + lane, sl_ptr[sl + 1], val,
                                                         [&](int e) { return col[e]; }, opnd);
-        if (row < n) {
-            double pr = z[row] + beta * po[row];
+        if (own) {
+            double pr = zr + beta * por;
             pn[row] = pr;
-            acc += diag[row] * pr;
+            acc += dr * pr;
             q[row] = acc;
             v[0] = pr * acc;
         }
