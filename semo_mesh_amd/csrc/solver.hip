@@ -354,13 +354,15 @@ __global__ void __launch_bounds__(TPB) k_spmv_dot(int n, const int* __restrict__
     int s = row >> 6, lane = row & 63;
     double v[2] = {0.0, 0.0};
     if (s < ((n + 63) >> 6)) {
+        bool own = row < n;     // the row's own operands before the gathers
+        double dr = own ? diag[row] : 0.0, xr = own ? x[row] : 0.0;
         auto opnd = [&](int j) { return x[j]; };
         double acc = dcol ? sell_row_sum<SELL_UNR, 64>(sl_ptr[s] + lane, sl_ptr[s + 1], val,
                                                         [&](int e) { return row + (int)dcol[e]; }, opnd)
                           : sell_row_sum<SELL_UNR, 64>(sl_ptr[s] + lane, sl_ptr[s + 1], val,
                                                         [&](int e) { return col[e]; }, opnd);
-        if (row < n) {
-            acc += diag[row] * x[row];
+        if (own) {
+            acc += dr * xr;
             y[row] = acc;
             v[0] = (a0 ? a0[row] : acc) * acc;
             if (a1) v[1] = a1[row] * acc;
@@ -650,15 +652,17 @@ __global__ void __launch_bounds__(TPB) k_spmv3(int n, const int* __restrict__ sl
     double w[6] = {0, 0, 0, 0, 0, 0};
     size_t N = (size_t)n;
     if (s < ((n + 63) >> 6)) {
+        bool own = row < n;     // the row's own operands before the gathers
+        double d = own ? diag[row] : 0.0;
+        double xr0 = own ? x[row] : 0.0, xr1 = own ? x[N + row] : 0.0, xr2 = own ? x[2 * N + row] : 0.0;
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
         for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) {
             int j = dcol ? row + dcol[e] : col[e];
             double v = val[e];
             acc0 += v * x[j]; acc1 += v * x[N + j]; acc2 += v * x[2 * N + j];
         }
-        if (row < n) {
-            double d = diag[row];
-            double acc[3] = {acc0 + d * x[row], acc1 + d * x[N + row], acc2 + d * x[2 * N + row]};
+        if (own) {
+            double acc[3] = {acc0 + d * xr0, acc1 + d * xr1, acc2 + d * xr2};
             for (int k = 0; k < 3; ++k) {
                 size_t jj = (size_t)k * N + row;
                 double yk = MODE == 2 ? a[jj] - acc[k] : acc[k];
