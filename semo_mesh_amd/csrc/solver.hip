@@ -146,10 +146,8 @@ enum { OP_Y = 0,       // y = A x
        OP_ADD = 2,     // y += A x
        OP_CHEB = 3,    // Chebyshev step: x = d_k, y = d_k+1, r -= D^-1 A d_k, xo += d_k+1
        OP_CHEBX = 4,   // Chebyshev start from x != 0: r = D^-1 (b - A x), y = r c1, xo = x + y
-       OP_J1RES = 5,   // degree-1 smoothing from x = 0 fused with the residual: x = c1 D^-1 b
-                       // (b from bsrc when given, stored as T), y = b - A x; x_j formed on the fly
        OP_YPREP = 6 }; // y = A x and xo = c1 D^-1 y (restriction writing the next level's b and
-                       // its degree-1 smoothed start, rounded as OP_J1RES forms them)
+                       // its degree-1 smoothed start, rounded as k_cheb_init forms it)
 
 template <class T>
 struct RowArgs {
@@ -166,22 +164,8 @@ struct RowArgs {
     T* __restrict__ r;
     T* __restrict__ xo;
     double* __restrict__ zo;        // optional FP64 copy of the new iterate
-    const double* __restrict__ bsrc;    // OP_J1RES: FP64 right-hand side to convert (or null)
     T c1, c2;
 };
-
-// the operand the row kernels gather: x_j, or for OP_J1RES the smoothed
-// iterate c1 D^-1 b at j, formed exactly as k_cheb_init forms it
-template <class T, int OP>
-__device__ inline T row_operand(const RowArgs<T>& a, int j) {
-    if constexpr (OP == OP_J1RES) {
-        T bj = a.bsrc ? (T)a.bsrc[j] : a.b[j];
-        T rj = a.dinv[j] * bj;
-        return rj * a.c1;
-    } else {
-        return a.x[j];
-    }
-}
 
 // entries per trip of the row sums: 4 on sliced-ELL slices (rows of ~6-7
 // entries), 8 per lane of the CSR kernels (coarse rows of 30-60 entries over
@@ -226,10 +210,8 @@ struct Own {
 template <class T, int OP>
 __device__ inline Own<T> load_own(const RowArgs<T>& a, int row) {
     Own<T> o{};
-    if constexpr (OP != OP_J1RES) {
-        if (a.diag) o.d = a.diag[row];
-        if (a.diag || OP == OP_CHEBX || OP == OP_CHEB) o.x = a.x[row];
-    }
+    if (a.diag) o.d = a.diag[row];
+    if (a.diag || OP == OP_CHEBX || OP == OP_CHEB) o.x = a.x[row];
     if constexpr (OP == OP_RES || OP == OP_CHEBX) o.b = a.b[row];
     if constexpr (OP == OP_YPREP || OP == OP_CHEB || OP == OP_CHEBX) o.dv = a.dinv[row];
     return o;
@@ -247,7 +229,7 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         if (s >= ((a.n + 63) >> 6)) return;
         // the row's own operands are requested before the gathers
         if (row < a.n) own = load_own<T, OP>(a, row);
-        auto opnd = [&](int j) { return row_operand<T, OP>(a, j); };
+        auto opnd = [&](int j) { return a.x[j]; };
         if (a.dcol)
             acc = sell_row_sum<SELL_UNR, 64>(a.ptr[s] + lane, a.ptr[s + 1], a.val,
                                               [&](int e) { return row + (int)a.dcol[e]; }, opnd);
@@ -260,20 +242,11 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         int sub = gid & (G - 1);
         if (row >= a.n) return;
         acc = sell_row_sum<CSR_UNR, G, T>(a.ptr[row] + sub, a.ptr[row + 1], a.val, [&](int e) { return a.col[e]; },
-                                            [&](int j) { return row_operand<T, OP>(a, j); });
+                                            [&](int j) { return a.x[j]; });
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, G);
         if (sub) return;
         own = load_own<T, OP>(a, row);
-    }
-    if constexpr (OP == OP_J1RES) {
-        T bi = a.bsrc ? (T)a.bsrc[row] : a.b[row];
-        T xi = row_operand<T, OP>(a, row);
-        if (a.bsrc) const_cast<T*>(a.b)[row] = bi;
-        a.xo[row] = xi;
-        if (a.diag) acc += a.diag[row] * xi;
-        a.y[row] = bi - acc;
-        return;
     }
     if (a.diag) acc += own.d * own.x;
     if constexpr (OP == OP_YPREP) {
@@ -826,14 +799,6 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
     const DevSell<T>& A = opA<T>(L);
     if (degree == 1 && prepared) {
         spmv<T, OP_RES>(c, A, bx<T>(L).p, b, t, tg.resid);
-    } else if (degree == 1) {
-        // pre-smoothing from x = 0 and the residual in one pass
-        double hi, lo;
-        cheb_interval(L, lo, hi);
-        RowArgs<T> a{};
-        a.b = b; a.bsrc = bsrc; a.dinv = vdinv<T>(L); a.c1 = (T)(1.0 / (0.5 * (hi + lo)));
-        a.xo = bx<T>(L).p; a.y = t;
-        row_launch<T, OP_J1RES>(c, A, a, tg.resid, op_bytes(A, 5) + (bsrc ? 8.0 * n : 0.0));
     } else {
         cheb_smooth<T>(c, L, b, degree, true, bsrc, nullptr, tg);
         spmv<T, OP_RES>(c, A, bx<T>(L).p, b, t, tg.resid);
