@@ -169,18 +169,15 @@ void split_diag(const HCsr& A, std::vector<int>& ptr, std::vector<int>& col, std
 // of similar length; CSR with `lanes` lanes per row (about four entries per
 // lane) for long rows (restriction, Galerkin coarse operators) or rows whose
 // lengths vary so much that ELL padding would waste more than half the reads.
-#ifndef SEMO_ELL_MAXAVG
-#define SEMO_ELL_MAXAVG 10.0
-#endif
-#ifndef SEMO_LANE_DIV
-#define SEMO_LANE_DIV 4.0
-#endif
+// (Average 10 and four entries per lane measured best against averages up
+// to 64 for ELL and two or eight entries per lane.)
+constexpr double ELL_MAX_AVG = 10.0, ENTRIES_PER_LANE = 4.0;
 int choose_lanes(int n, const std::vector<int>& ptr, size_t padded) {
     size_t nnz = ptr.empty() ? 0 : (size_t)ptr[n];
     double avg = n ? (double)nnz / n : 0.0;
-    if (avg <= SEMO_ELL_MAXAVG && (double)padded <= 1.5 * (double)std::max<size_t>(nnz, 1)) return 0;
+    if (avg <= ELL_MAX_AVG && (double)padded <= 1.5 * (double)std::max<size_t>(nnz, 1)) return 0;
     int lanes = 4;
-    while (lanes < 64 && SEMO_LANE_DIV * lanes < avg) lanes *= 2;
+    while (lanes < 64 && ENTRIES_PER_LANE * lanes < avg) lanes *= 2;
     return lanes;
 }
 

@@ -73,12 +73,9 @@ namespace {
 constexpr int NSWEEP = 2;          // pre / post smoothing sweeps
 constexpr int NCOARSE = 8;         // sweeps on the coarsest level
 constexpr float OMEGA = 0.6f;      // block-Jacobi damping
-#ifndef SEMO_CPL_OVERCORR
-#define SEMO_CPL_OVERCORR 1.5f
-#endif
 // coarse-correction scaling of plain aggregation (AMGCL's over_interp,
 // lib/amgcl/coarsening/aggregation.hpp:74, default 1.5)
-constexpr float OVERCORR = SEMO_CPL_OVERCORR;
+constexpr float OVERCORR = 1.5f;
 
 // ---------------------------------------------------------------- assembly --
 __global__ void __launch_bounds__(TPB) k_cpl_row(
