@@ -101,3 +101,12 @@ def test_rcm_rejects_bad_graph():
     from semo_mesh_amd.solver import rcm
     with pytest.raises(_lib.SemoError):
         rcm(np.array([0, 1], dtype=np.int32), np.array([5], dtype=np.int32))
+
+
+def test_autograd_module_imports_without_a_gpu():
+    """The torch.autograd harness of the adjoint is importable on a host
+    without a GPU (its forward/backward need one)."""
+    import torch
+    from semo_mesh_amd.autograd import PressureObjective, pressure_objective
+    assert issubclass(PressureObjective, torch.autograd.Function)
+    assert callable(pressure_objective)

@@ -124,3 +124,44 @@ def test_pressure_matrix_symmetric_laplacian(oracle_lib):
     pref = m.owner[m.patch_start[m.patch("pref")]]
     assert np.all(np.abs(np.delete(rs, pref)) < 1e-12 * np.abs(v).max())
     assert rs[pref] > 0
+
+
+def test_inlet_profile_shape_and_bounds():
+    """cases.inlet_profile: [3][n_boundary_faces], the perturbation (amplitude
+    2 %) only on the named patch, (u_in, 0, 0) elsewhere."""
+    from semo_mesh_amd.cases import inlet_profile
+    m = hex_mesh(8, 4, 4, lo=(1.0, 0.0, 0.0), hi=(5.0, 1.0, 1.0), jitter=0.1, seed=3, ref_face=False)
+    u = inlet_profile(m, "xmin", 2.0, amplitude=0.02, seed=7)
+    nb = m.n_faces - m.n_internal_faces
+    assert u.shape == (3, nb)
+    b = m.patch("xmin")
+    s = int(m.patch_start[b]) - m.n_internal_faces
+    sl = slice(s, s + int(m.patch_nfaces[b]))
+    inlet = u[0, sl]
+    assert np.all(np.abs(inlet / 2.0 - 1.0) <= 0.02 + 1e-15) and np.ptp(inlet) > 0.0
+    assert np.all(u[1:] == 0.0)
+    rest = np.delete(u[0], np.arange(nb)[sl])
+    assert np.all(rest == 2.0)
+    assert np.array_equal(u, inlet_profile(m, "xmin", 2.0, amplitude=0.02, seed=7))
+
+
+def test_oracle_uniform_face_velocity_is_the_patch_value(oracle_lib):
+    """The oracle's per-face fixedValue-U values, set to the patch value on
+    every boundary face, reproduce the uniform inlet bit for bit; a profile
+    changes the solution."""
+    from semo_mesh_amd.cases import channel_case, inlet_profile
+    m = hex_mesh(8, 3, 3, lo=(1.0, 0.0, 0.0), hi=(5.0, 1.0, 1.0), jitter=0.1, seed=5, ref_face=False)
+    case = channel_case(1.0, dt=0.02, rho=1.0, mu=0.01)
+    nb = m.n_faces - m.n_internal_faces
+    uni = np.zeros((3, nb)); uni[0] = 1.0
+    out = []
+    for ufix in (None, uni, inlet_profile(m, "xmin", 1.0)):
+        o = oracle_lib.Oracle(m, case)
+        o.set_solver(tol=1e-14)
+        o.set("U", np.full(m.n_cells, 1.0))
+        if ufix is not None:
+            o.set("ufix", ufix.ravel())
+        o.step(correctors=2)
+        out.append(o.get("U"))
+    assert np.array_equal(out[0], out[1])
+    assert not np.array_equal(out[0], out[2])
