@@ -178,6 +178,15 @@ __global__ void __launch_bounds__(TPB) k_pack5(int nc, const double* __restrict_
     o[3] = P[i]; o[4] = U[i]; o[5] = V[i]; o[6] = W[i]; o[7] = VF[i];
 }
 
+// U, V, W only (the pressure stage: P and VF are unchanged since the last pack)
+__global__ void __launch_bounds__(TPB) k_pack3(int nc, const double* __restrict__ U, const double* __restrict__ V,
+                                               const double* __restrict__ W, double* __restrict__ cpk) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double* o = cpk + 8 * (size_t)i;
+    o[4] = U[i]; o[5] = V[i]; o[6] = W[i];
+}
+
 __global__ void __launch_bounds__(TPB) k_pack_xyz(int nc, const double* __restrict__ cxyz, double* __restrict__ cpk) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
@@ -1255,7 +1264,11 @@ void source_gravity(semo_ctx* c) {
 }
 
 void grad_minmax(semo_ctx* c, bool with_p, bool uvw_only) {
-    {
+    if (uvw_only) {
+        ProfScope ps(c, "pack3", 48.0 * c->nc);
+        hipLaunchKernelGGL(k_pack3, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->U.p, c->V.p, c->W.p,
+                           c->cpk.p);
+    } else {
         ProfScope ps(c, "pack5", 80.0 * c->nc);
         hipLaunchKernelGGL(k_pack5, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->P.p, c->U.p, c->V.p,
                            c->W.p, c->VF.p, c->cpk.p);
