@@ -1297,9 +1297,12 @@ void grad_minmax(semo_ctx* c, bool with_p, bool uvw_only) {
     }
 }
 
-void assemble_momentum(semo_ctx* c) {
+void assemble_momentum(semo_ctx* c, bool after_vof) {
     CaseK k = case_k(c);
-    grad_minmax(c, true, false);
+    // right after the VOF stage only alpha has changed: its gradient alone is
+    // recomputed; those of P, U, V, W and their bounds are the VOF stage's
+    if (after_vof) lsq1(c, c->VF.p, c->bVF.p, c->nif, c->gVF.p);
+    else grad_minmax(c, true, false);
     if (c->nif) {
         double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 8 * 8) + (double)c->nc * 8 * (4 + 12 + 8 + 2);
         ProfScope ps(c, "mom_face", bytes);
@@ -1345,7 +1348,7 @@ void assemble_pressure(semo_ctx* c, bool after_momentum) {
 // anyway (same field, same boundary values).
 void assemble_vof(semo_ctx* c) {
     CaseK k = case_k(c);
-    grad_minmax(c, false, false);
+    grad_minmax(c, true, false);       // with the P bounds the momentum stage after it reuses
     if (c->nif) {
         {
             ProfScope ps(c, "mstacs_face", (double)c->nif * (8 * 8 + 8 * 3 + 16) + (double)c->nc * 8 * 12);

@@ -151,9 +151,9 @@ double volfrac_eq(semo_ctx* c, semo_solve_stats* st) {
     return nv;
 }
 
-double momentum_eq(semo_ctx* c, semo_solve_stats* st) {
+double momentum_eq(semo_ctx* c, semo_solve_stats* st, bool after_vof = false) {
     recon_boundary(c);
-    assemble_momentum(c);
+    assemble_momentum(c, after_vof);
     solve_momentum(c, 3, st);
     return momentum_update(c);
 }
@@ -507,7 +507,7 @@ int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
         need_case(c);
         simple_setup(c);      // pressureBased.cpp:17-130: time levels, dt, VOF, sources
         for (int it = 0; it < ncorr; ++it) {
-            double n0 = momentum_eq(c, &c->last_stats_mom);
+            double n0 = momentum_eq(c, &c->last_stats_mom, it == 0 && c->cs.n_vof > 0);
             double n1 = pressure_eq(c, &c->last_stats_p, true);
             if (norms) { norms[2 * it] = n0; norms[2 * it + 1] = n1; }
         }

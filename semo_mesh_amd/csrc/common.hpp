@@ -347,7 +347,12 @@ void source_gravity(semo_ctx* c);
 // uvw_only: gradients of U, V, W only; P and VF gradients are kept from the
 // last full pass (valid when P, VF and their boundary values are unchanged)
 void grad_minmax(semo_ctx* c, bool with_p_minmax, bool uvw_only = false);
-void assemble_momentum(semo_ctx* c);
+// after_vof: called by the outer iteration right after its VOF stage(s), which
+// left P, U, V, W unchanged: only the alpha gradient is recomputed
+void assemble_momentum(semo_ctx* c, bool after_vof = false);
+// one field's LSQ gradient over the vertex stencil and the boundary faces
+// (faceR indexed by face - foff; null: zero boundary difference)
+void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double* g);
 // after_momentum: called by an outer iteration right after its momentum
 // stage (P and VF unchanged since), so the P gradient of that stage is reused
 void assemble_pressure(semo_ctx* c, bool after_momentum = false);
