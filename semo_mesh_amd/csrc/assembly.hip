@@ -1297,12 +1297,15 @@ void grad_minmax(semo_ctx* c, bool with_p, bool uvw_only) {
     }
 }
 
-void assemble_momentum(semo_ctx* c, bool after_vof) {
+void assemble_momentum(semo_ctx* c, bool after_vof, bool alpha_grad_ready) {
     CaseK k = case_k(c);
     // right after the VOF stage only alpha has changed: its gradient alone is
     // recomputed; those of P, U, V, W and their bounds are the VOF stage's
-    if (after_vof) lsq1(c, c->VF.p, c->bVF.p, c->nif, c->gVF.p);
-    else grad_minmax(c, true, false);
+    if (after_vof) {
+        if (!alpha_grad_ready) lsq1(c, c->VF.p, c->bVF.p, c->nif, c->gVF.p);
+    } else {
+        grad_minmax(c, true, false);
+    }
     if (c->nif) {
         double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 8 * 8) + (double)c->nc * 8 * (4 + 12 + 8 + 2);
         ProfScope ps(c, "mom_face", bytes);
@@ -1392,8 +1395,8 @@ void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double*
 // calcSourceSurfaceTension with calcCurvature (curvature.cpp:7-3537,
 // surfaceTension.cpp:7-229).  With sigma = 0 every use of kappa vanishes, so
 // the pipeline is skipped and the sources are zero.
-void surface_tension(semo_ctx* c) {
-    if (c->cs.sigma == 0.0) { c->srcS.zero(c->stream); c->kappa.zero(c->stream); return; }
+bool surface_tension(semo_ctx* c) {
+    if (c->cs.sigma == 0.0) { c->srcS.zero(c->stream); c->kappa.zero(c->stream); return false; }
     int n = c->nc;
     for (DBuf<double>* b : {&c->ls, &c->ls2, &c->tau_ls, &c->vel0_ls}) b->ensure(n);
     c->cnrm.ensure(3 * (size_t)n); c->cgrad.ensure(3 * (size_t)n); c->iface.ensure(n);
@@ -1428,12 +1431,14 @@ void surface_tension(semo_ctx* c) {
     else
         hipLaunchKernelGGL(k_curv_kappa<int>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_ell.p, c->xn.p,
                            c->kinv.p, c->kappa.p);
-    // grad(alpha): LSQ with the boundary values, then one Green-Gauss pass
-    lsq1(c, c->VF.p, c->bVF.p, c->nif, c->cgrad.p);
+    // grad(alpha): LSQ with the boundary values (into the alpha gradient of
+    // the assembly, which is the same), then one Green-Gauss pass
+    lsq1(c, c->VF.p, c->bVF.p, c->nif, c->gVF.p);
     hipLaunchKernelGGL(k_st_source, gb, tb, 0, c->stream, n, c->nif, case_k(c), c->d_cf_sl.p, c->d_cf_ell.p,
                        c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_wC.p, c->f_skew.p, c->f_nrm.p,
-                       c->f_area.p, c->c_vol.p, c->f_xyz.p, c->c_xyz.p, c->VF.p, c->bVF.p, c->cgrad.p, c->kappa.p,
+                       c->f_area.p, c->c_vol.p, c->f_xyz.p, c->c_xyz.p, c->VF.p, c->bVF.p, c->gVF.p, c->kappa.p,
                        c->U.p, c->V.p, c->W.p, c->srcS.p);
+    return true;
 }
 
 // calcTimeStepFromCorantNumber / calcCorantNumberForPrint: returns the

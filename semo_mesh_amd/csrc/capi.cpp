@@ -135,10 +135,10 @@ void step_setup(semo_ctx* c) {
 
 // after the VOF loop: face values and the body-force source of the step
 // (pressureBased.cpp:128-130)
-void step_sources(semo_ctx* c) {
+bool step_sources(semo_ctx* c) {
     recon_boundary(c);
     source_gravity(c);
-    surface_tension(c);
+    return surface_tension(c);
 }
 
 // one VOF iteration (pressureBased.cpp:104-127)
@@ -151,9 +151,9 @@ double volfrac_eq(semo_ctx* c, semo_solve_stats* st) {
     return nv;
 }
 
-double momentum_eq(semo_ctx* c, semo_solve_stats* st, bool after_vof = false) {
+double momentum_eq(semo_ctx* c, semo_solve_stats* st, bool after_vof = false, bool alpha_grad_ready = false) {
     recon_boundary(c);
-    assemble_momentum(c, after_vof);
+    assemble_momentum(c, after_vof, alpha_grad_ready);
     solve_momentum(c, 3, st);
     return momentum_update(c);
 }
@@ -168,11 +168,11 @@ double pressure_eq(semo_ctx* c, semo_solve_stats* st, bool after_momentum = fals
 }  // namespace
 
 // one time step's set-up and its outer iterations, for the adjoint driver
-void simple_setup(semo_ctx* c) {
+bool simple_setup(semo_ctx* c) {
     step_setup(c);
     c->courant = time_step(c, c->cs.adjust_dt);
     for (int k = 0; k < c->cs.n_vof; ++k) volfrac_eq(c, &c->last_stats_vf);
-    step_sources(c);
+    return step_sources(c);
 }
 double simple_momentum(semo_ctx* c, semo_solve_stats* st) { return momentum_eq(c, st); }
 double simple_pressure(semo_ctx* c, semo_solve_stats* st) { return pressure_eq(c, st, true); }
@@ -505,9 +505,11 @@ int semo_incompressible_pressure_based(semo_ctx* c, int ncorr, double* norms) {
     if (!c || ncorr < 1) return SEMO_EINVAL;
     return guard(c, [&] {
         need_case(c);
-        simple_setup(c);      // pressureBased.cpp:17-130: time levels, dt, VOF, sources
+        // pressureBased.cpp:17-130: time levels, dt, VOF, sources
+        bool alpha_grad = simple_setup(c);
         for (int it = 0; it < ncorr; ++it) {
-            double n0 = momentum_eq(c, &c->last_stats_mom, it == 0 && c->cs.n_vof > 0);
+            bool first = it == 0 && c->cs.n_vof > 0;
+            double n0 = momentum_eq(c, &c->last_stats_mom, first, first && alpha_grad);
             double n1 = pressure_eq(c, &c->last_stats_p, true);
             if (norms) { norms[2 * it] = n0; norms[2 * it + 1] = n1; }
         }

@@ -348,8 +348,9 @@ void source_gravity(semo_ctx* c);
 // last full pass (valid when P, VF and their boundary values are unchanged)
 void grad_minmax(semo_ctx* c, bool with_p_minmax, bool uvw_only = false);
 // after_vof: called by the outer iteration right after its VOF stage(s), which
-// left P, U, V, W unchanged: only the alpha gradient is recomputed
-void assemble_momentum(semo_ctx* c, bool after_vof = false);
+// left P, U, V, W unchanged: only the alpha gradient is recomputed, unless
+// alpha_grad_ready (the step's surface-tension source computed it already)
+void assemble_momentum(semo_ctx* c, bool after_vof = false, bool alpha_grad_ready = false);
 // one field's LSQ gradient over the vertex stencil and the boundary faces
 // (faceR indexed by face - foff; null: zero boundary difference)
 void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double* g);
@@ -358,7 +359,8 @@ void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double*
 void assemble_pressure(semo_ctx* c, bool after_momentum = false);
 void assemble_vof(semo_ctx* c);
 double time_step(semo_ctx* c, int adjust);
-void surface_tension(semo_ctx* c);
+// true when it computed the alpha gradient (into gVF): sigma != 0
+bool surface_tension(semo_ctx* c);
 double vof_update(semo_ctx* c);
 double momentum_update(semo_ctx* c);
 double pressure_correct(semo_ctx* c);
@@ -390,7 +392,8 @@ void coupled_spmv(semo_ctx* c, const double* x_dev, double* y_dev);
 void coupled_export(semo_ctx* c, std::vector<int>& rows, std::vector<int>& cols, std::vector<double>& blocks,
                     std::vector<double>& rhs);
 // capi.cpp: the forward driver's pieces
-void simple_setup(semo_ctx* c);
+// returns whether the step's sources left the alpha gradient in gVF
+bool simple_setup(semo_ctx* c);
 double simple_momentum(semo_ctx* c, semo_solve_stats* st);
 double simple_pressure(semo_ctx* c, semo_solve_stats* st);
 // adjoint.hip
