@@ -1,0 +1,120 @@
+"""OpenFOAM polyMesh reader/writer (semo_mesh_amd.polymesh), CPU only.
+
+The reference ships no polyMesh case, so the reader is pinned by the
+loader's rules (src/load/loadOpenFoam.cpp) on a hand-written file set and by
+round trips through the writer: the read-back mesh has the same points,
+faces, owner/neighbour lists and patches, each cell the same point set, and
+the oracle's geometry and one SIMPLE iteration on it agree with the
+original mesh's (cell point order only changes summation orders).
+"""
+import os
+
+import numpy as np
+
+from semo_mesh_amd.cases import cavity_case
+from semo_mesh_amd.mesh import hex_mesh, tet_mesh
+from semo_mesh_amd.polymesh import read_polymesh, write_polymesh
+
+
+def test_round_trip_hex_and_tet(tmp_path):
+    for k, m in enumerate((hex_mesh(4, 3, 2, jitter=0.1, seed=3), tet_mesh(2, 2, 1, jitter=0.1))):
+        d = os.path.join(tmp_path, f"m{k}")
+        write_polymesh(m, d)
+        r = read_polymesh(d)
+        assert np.array_equal(r.points, m.points)
+        for a in ("owner", "neighbour", "face_ptr", "face_pts", "patch_start", "patch_nfaces"):
+            assert np.array_equal(getattr(r, a), getattr(m, a)), a
+        assert r.patch_names == m.patch_names
+        assert r.n_cells == m.n_cells
+        for c in range(m.n_cells):
+            a = set(r.cell_pts[r.cell_ptr[c]:r.cell_ptr[c + 1]].tolist())
+            b = set(m.cell_pts[m.cell_ptr[c]:m.cell_ptr[c + 1]].tolist())
+            assert a == b, c
+
+
+def test_cell_points_first_appearance_order(tmp_path):
+    m = hex_mesh(2, 1, 1)
+    d = os.path.join(tmp_path, "m")
+    write_polymesh(m, d)
+    r = read_polymesh(d)
+    # cell 0's list starts with the points of its first face, in face order
+    f0 = [f for f in range(m.n_faces) if m.owner[f] == 0 or (f < m.n_internal_faces and m.neighbour[f] == 0)][0]
+    first = m.face_pts[m.face_ptr[f0]:m.face_ptr[f0 + 1]].tolist()
+    assert r.cell_pts[r.cell_ptr[0]:r.cell_ptr[0] + len(first)].tolist() == first
+
+
+def test_oracle_on_read_mesh_matches(tmp_path, oracle_lib):
+    m = hex_mesh(5, 4, 3, jitter=0.1, seed=11, ref_face=True)
+    d = os.path.join(tmp_path, "m")
+    write_polymesh(m, d)
+    r = read_polymesh(d)
+    case = cavity_case(dt=0.05)
+    case.rho = (1.0, 1.0)
+    case.mu = (0.01, 0.01)
+    out = []
+    for mesh in (m, r):
+        o = oracle_lib.Oracle(mesh, case)
+        o.set_solver(tol=1e-14)
+        geo = {f: o.get(f).copy() for f in ("cell_volume", "cell_xyz", "face_area", "face_wC")}
+        o.step(correctors=1)
+        out.append((geo, {f: o.get(f) for f in ("P", "U", "V", "W")}))
+    for f in out[0][0]:
+        a, b = out[0][0][f], out[1][0][f]
+        assert np.allclose(a, b, rtol=1e-13, atol=1e-15), f
+    for f in out[0][1]:
+        a, b = out[0][1][f], out[1][1][f]
+        assert np.linalg.norm(a - b) <= 1e-10 * max(np.linalg.norm(a), 1e-300), f
+
+
+FOAM_HEADER = """/*--------------------------------*- C++ -*----------------------------------*\\
+  =========                 |
+\\*---------------------------------------------------------------------------*/
+FoamFile
+{{
+    version     2.0;
+    format      ascii;
+    class       {cls};
+    object      {obj};
+}}
+// * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * //
+
+"""
+
+
+def test_hand_written_polymesh(tmp_path):
+    """Two unit cubes side by side in OpenFOAM's own layout: headers, comment
+    lines, a face spread over two lines and a boundary block with inGroups."""
+    d = str(tmp_path)
+    pts = [(0, 0, 0), (1, 0, 0), (2, 0, 0), (0, 1, 0), (1, 1, 0), (2, 1, 0),
+           (0, 0, 1), (1, 0, 1), (2, 0, 1), (0, 1, 1), (1, 1, 1), (2, 1, 1)]
+    faces = ["4(1 4 10 7)",
+             "4(0 6 9 3)", "4(2 5 11 8)",
+             "4(0 1 7 6)", "4(1 2 8 7)", "4(3 9 10 4)", "4(4 10 11 5)",
+             "4(0 3 4 1)", "4(1 4 5 2)", "4(6 7 10 9)", "4(7 8 11 10)"]
+    owner = [0, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1]
+    with open(os.path.join(d, "points"), "w") as fh:
+        fh.write(FOAM_HEADER.format(cls="vectorField", obj="points"))
+        fh.write(f"{len(pts)}\n(\n" + "\n".join(f"({x} {y} {z})" for x, y, z in pts) + "\n)\n\n// end\n")
+    with open(os.path.join(d, "faces"), "w") as fh:
+        fh.write(FOAM_HEADER.format(cls="faceList", obj="faces"))
+        body = faces[:1] + ["4(0 6", "9 3)"] + faces[2:]
+        fh.write(f"{len(faces)}\n(\n" + "\n".join(body) + "\n)\n")
+    with open(os.path.join(d, "owner"), "w") as fh:
+        fh.write(FOAM_HEADER.format(cls="labelList", obj="owner"))
+        fh.write(f"{len(owner)}\n(\n" + "\n".join(map(str, owner)) + "\n)\n")
+    with open(os.path.join(d, "neighbour"), "w") as fh:
+        fh.write(FOAM_HEADER.format(cls="labelList", obj="neighbour"))
+        fh.write("1\n(\n1\n)\n")
+    with open(os.path.join(d, "boundary"), "w") as fh:
+        fh.write(FOAM_HEADER.format(cls="polyBoundaryMesh", obj="boundary"))
+        fh.write("2\n(\n    ends\n    {\n        type            patch;\n        nFaces          2;\n"
+                 "        startFace       1;\n    }\n    walls\n    {\n        type            wall;\n"
+                 "        inGroups        1(wall);\n        nFaces          8;\n        startFace       3;\n"
+                 "    }\n)\n")
+    m = read_polymesh(d)
+    assert m.n_points == 12 and m.n_faces == 11 and m.n_internal_faces == 1 and m.n_cells == 2
+    assert m.face_pts[m.face_ptr[1]:m.face_ptr[2]].tolist() == [0, 6, 9, 3]
+    assert m.patch_names == ["ends", "walls"]
+    assert m.patch_start.tolist() == [1, 3] and m.patch_nfaces.tolist() == [2, 8]
+    assert m.cell_pts[m.cell_ptr[0]:m.cell_ptr[1]].tolist()[:4] == [1, 4, 10, 7]
+    assert sorted(m.cell_pts[m.cell_ptr[1]:m.cell_ptr[2]].tolist()) == [1, 2, 4, 5, 7, 8, 10, 11]
