@@ -1,15 +1,17 @@
-"""OpenFOAM polyMesh reader/writer (semo_mesh_amd.polymesh), CPU only.
+"""OpenFOAM polyMesh reader/writer (semo_mesh_amd.polymesh).
 
 The reference ships no polyMesh case, so the reader is pinned by the
 loader's rules (src/load/loadOpenFoam.cpp) on a hand-written file set and by
 round trips through the writer: the read-back mesh has the same points,
 faces, owner/neighbour lists and patches, each cell the same point set, and
 the oracle's geometry and one SIMPLE iteration on it agree with the
-original mesh's (cell point order only changes summation orders).
+original mesh's (cell point order only changes summation orders).  One
+GPU test runs a read-back mesh through the GPU path against the oracle.
 """
 import os
 
 import numpy as np
+import pytest
 
 from semo_mesh_amd.cases import cavity_case
 from semo_mesh_amd.mesh import hex_mesh, tet_mesh
@@ -118,3 +120,29 @@ def test_hand_written_polymesh(tmp_path):
     assert m.patch_start.tolist() == [1, 3] and m.patch_nfaces.tolist() == [2, 8]
     assert m.cell_pts[m.cell_ptr[0]:m.cell_ptr[1]].tolist()[:4] == [1, 4, 10, 7]
     assert sorted(m.cell_pts[m.cell_ptr[1]:m.cell_ptr[2]].tolist()) == [1, 2, 4, 5, 7, 8, 10, 11]
+
+
+@pytest.mark.gpu
+def test_gpu_step_on_read_mesh(tmp_path, oracle_lib):
+    """A mesh read from a polyMesh folder through the GPU path: two SIMPLE
+    iterations against the oracle on the same mesh (FP64, 1e-10)."""
+    from semo_mesh_amd.solver import GpuSolver
+    m = hex_mesh(6, 5, 4, jitter=0.1, seed=19, ref_face=True)
+    d = os.path.join(tmp_path, "m")
+    write_polymesh(m, d)
+    r = read_polymesh(d)
+    case = cavity_case(dt=0.05)
+    case.rho = (1.0, 1.0)
+    case.mu = (0.01, 0.01)
+    g = GpuSolver(0)
+    g.upload_mesh(r, reorder=True)
+    g.set_case(case)
+    g.set_solver("fp64", tol_p=1e-13, tol_u=1e-13, maxit_p=500, maxit_u=500)
+    g.incompressiblePressureBased(2)
+    o = oracle_lib.Oracle(r, case)
+    o.set_solver(tol=1e-14)
+    o.step(correctors=2)
+    for f in ("P", "U", "V", "W"):
+        a, b = g.get_field(f), o.get(f)
+        assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b), f
+    g.close()
