@@ -87,7 +87,15 @@ int semo_rcm(int32_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t* per
  * Boundary types (SEMO_Boundary::type strings, src/solvers/reconIncom.cpp:
  * 591-1032): */
 enum { SEMO_BC_ZERO_GRADIENT = 0, SEMO_BC_FIXED_VALUE = 1, SEMO_BC_NO_SLIP = 2,
-       SEMO_BC_SLIP = 3, SEMO_BC_SURFACE_NORMAL_FIXED_VALUE = 4, SEMO_BC_INLET_OUTLET = 5 };
+       SEMO_BC_SLIP = 3, SEMO_BC_SURFACE_NORMAL_FIXED_VALUE = 4, SEMO_BC_INLET_OUTLET = 5,
+       /* p only: the owner cell's pressure where its Mach number |U|/C
+          exceeds 1, else p_value; the pressure equation's boundary diagonal
+          is added only above Mach 1 (reconIncom.cpp:618-631,
+          eqPressure.cpp:625-632).  C is field SEMO_F_C.                      */
+       SEMO_BC_SWITCH = 6,
+       /* volume fraction only: per-face values of the nearest given point
+          (semo_set_vf_function_points; reconIncom.cpp:929-1019)             */
+       SEMO_BC_FUNCTION = 7 };
 
 typedef struct {
     int32_t p_type;  double p_value;
@@ -114,6 +122,14 @@ typedef struct {
 /* Replaces the controls/species read by SEMO_Controls_Builder
  * (src/controls/build.cpp:7-623) for this path; bcs has n_patches entries.   */
 int semo_set_case(semo_ctx* ctx, const semo_case* c, const semo_patch_bc* bcs);
+
+/* Points of the volume-fraction "function" boundary: replaces the read of
+ * ./VF.csv in reconIncomZeroOrder (src/solvers/reconIncom.cpp:929-1019).
+ * pts[4*k .. 4*k+3] = (VF, x, y, z) of row k (the CSV's columns 3-6).  Every
+ * boundary face takes the VF of the nearest point (squared distance; the first
+ * of equal distances; points at squared distance >= 1e9 are never taken).
+ * Call after semo_mesh_upload; n = 0 clears the points.                      */
+int semo_set_vf_function_points(semo_ctx* ctx, int32_t n, const double* pts);
 
 /* ---- linear solver configuration --------------------------------------------------
  * Replaces the hard-coded AMGCL set-ups of solveAMGCL (src/solvers/
@@ -148,6 +164,11 @@ typedef enum {
     SEMO_F_KAPPA, SEMO_F_SRCS0, SEMO_F_SRCS1, SEMO_F_SRCS2,   /* curvature, sigma kappa grad(alpha) */
     SEMO_F_LS,                                              /* reinitialised level set (read) */
     SEMO_F_GRADP = 32, SEMO_F_GRADU, SEMO_F_GRADV, SEMO_F_GRADW, SEMO_F_GRADVF, /* [n*3] */
+    /* cell speed of sound, computed at the first time step from the mixture
+       EOS as calcCellEOSVF sets it at load (eos.cpp:160-293) and never
+       updated by the incompressible iteration; read by SEMO_BC_SWITCH.
+       Settable after that step.                                             */
+    SEMO_F_C = 40,
     SEMO_F_X0 = 48, SEMO_F_X1, SEMO_F_X2,                  /* last solution     */
     SEMO_F_B0 = 56, SEMO_F_B1, SEMO_F_B2,                  /* last right sides  */
     /* geometry (read-only): cell arrays, then face arrays (original face order) */
@@ -267,6 +288,14 @@ int semo_matrix_get(semo_ctx* ctx, int64_t* nnz, int32_t* rows, int32_t* cols, d
  * phi[n] cell values, face_r[n_faces] right-face values (used at boundary
  * faces), grad[3n].                                                            */
 int semo_least_square(semo_ctx* ctx, const double* phi, const double* face_r, double* grad);
+/* calcGaussGreen(mesh, cn, fn, gradient) (src/math/gradient.cpp:1778-1985):
+ * field = SEMO_F_P, _U, _V, _W or _VF names cn, whose patch types and values
+ * (semo_set_case) give the boundary faces' values (fixedValue: the patch
+ * value; zeroGradient: phi_P + grad_P.(x_f - x_P); other types: the mean of
+ * phi_P and face_r).  phi[n] cell values, face_r[n_faces] the right face
+ * values varR[fn] (read at boundary faces), grad[3n] in: the gradient used
+ * for the skewness correction and the extrapolation, out: the new gradient. */
+int semo_gauss_green(semo_ctx* ctx, int field, const double* phi, const double* face_r, double* grad);
 /* setCellVarMinMax (src/solvers/cellVarMinMax.cpp:8-72)                        */
 int semo_cell_min_max(semo_ctx* ctx, const double* phi, double* vmax, double* vmin);
 

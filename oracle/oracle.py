@@ -29,6 +29,7 @@ def lib():
         L.orc_create.argtypes = [C.c_int] * 4 + [_D, _I, _I, _I, _I, _I, _I, C.c_int, _I, _I]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_bc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _D, C.c_int, C.c_double]
+        L.orc_set_vf_points.argtypes = [C.c_void_p, C.c_int, _D]
         L.orc_set_case.argtypes = [C.c_void_p, _D, _D, C.c_double, _D, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_double]
         L.orc_set_solver.argtypes = [C.c_void_p, C.c_double, C.c_int]
@@ -54,6 +55,7 @@ def lib():
             getattr(L, nm).argtypes = [C.c_void_p]
         L.orc_lsq.argtypes = [C.c_void_p, _D, _D, _D]
         L.orc_minmax.argtypes = [C.c_void_p, _D, _D, _D]
+        L.orc_gauss_green.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
         L.orc_solve.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.orc_momentum_update.restype = C.c_double
         L.orc_momentum_update.argtypes = [C.c_void_p]
@@ -106,6 +108,10 @@ class Oracle:
             b = case.bcs[nm]
             L.orc_set_bc(self._h, i, b.p_type, b.p_value, b.u_type,
                          np.array(b.u_value, dtype=np.float64), b.vf_type, b.vf_value)
+        pts = getattr(case, "vf_points", None)
+        if pts is not None:
+            a = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 4)
+            L.orc_set_vf_points(self._h, a.shape[0], a)
 
     def set_solver(self, tol=1e-13, maxit=5000):
         lib().orc_set_solver(self._h, tol, maxit)
@@ -137,6 +143,13 @@ class Oracle:
         g = np.empty(3 * self.mesh.n_cells)
         lib().orc_lsq(self._h, np.ascontiguousarray(phi, dtype=np.float64),
                       np.ascontiguousarray(face_r, dtype=np.float64), g)
+        return g.reshape(-1, 3)
+
+    def gauss_green(self, field, phi, face_r, grad):
+        sel = {"P": 0, "U": 1, "V": 2, "W": 3, "VF": 4}[field]
+        g = np.ascontiguousarray(grad, dtype=np.float64).reshape(-1).copy()
+        lib().orc_gauss_green(self._h, sel, np.ascontiguousarray(phi, dtype=np.float64),
+                              np.ascontiguousarray(face_r, dtype=np.float64), g)
         return g.reshape(-1, 3)
 
     def minmax(self, phi):

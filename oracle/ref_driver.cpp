@@ -362,6 +362,27 @@ int main(int argc, char** argv) {
     }
     auto t3 = std::chrono::steady_clock::now();
     dump_state(mesh, "final_");
+    {   // calcGaussGreen (gradient.cpp:1778-1985) on the final state, one call
+        // per field, from the field's last LSQ gradient, with the face right
+        // values it reads
+        struct GG { int cn, fn; const char* name; };
+        GG ggs[] = {{controls.P, controls.fP, "P"}, {controls.U, controls.fU, "U"},
+                    {controls.V, controls.fV, "V"}, {controls.W, controls.fW, "W"},
+                    {controls.VF[0], controls.fVF[0], "VF"}};
+        for (const GG& q : ggs) {
+            vector<vector<double>> g = mesh.cellsGradientVar[q.cn];
+            g.resize(nc, vector<double>(3, 0.0));
+            vector<double> gin(3 * nc), fr(mesh.faces.size());
+            for (size_t i = 0; i < nc; ++i) for (int k = 0; k < 3; ++k) gin[3 * i + k] = g[i][k];
+            for (size_t f = 0; f < mesh.faces.size(); ++f) fr[f] = mesh.faces[f].varR[q.fn];
+            math.calcGaussGreen(mesh, q.cn, q.fn, g);
+            vector<double> gout(3 * nc);
+            for (size_t i = 0; i < nc; ++i) for (int k = 0; k < 3; ++k) gout[3 * i + k] = g[i][k];
+            dump(string("gg_in_") + q.name, gin);
+            dump(string("gg_faceR_") + q.name, fr);
+            dump(string("gg_out_") + q.name, gout);
+        }
+    }
     double ts = std::chrono::duration<double>(t1 - t0).count();
     double tr = std::chrono::duration<double>(t3 - t2).count();
     FILE* f = fopen((g_out + "/timing.txt").c_str(), "w");

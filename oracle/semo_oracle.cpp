@@ -28,7 +28,9 @@ using std::vector;
 namespace {
 
 enum BcType { ZERO_GRADIENT = 0, FIXED_VALUE = 1, NO_SLIP = 2, SLIP = 3,
-              SURFACE_NORMAL_FIXED_VALUE = 4, INLET_OUTLET = 5 };
+              SURFACE_NORMAL_FIXED_VALUE = 4, INLET_OUTLET = 5,
+              SWITCH = 6,        // P: owner value above Mach 1, else the fixed value
+              FUNCTION = 7 };    // VF: nearest point of ./VF.csv
 
 struct Patch {
     int start, nfaces;
@@ -74,6 +76,13 @@ struct Oracle {
     // reconIncom.cpp:707-727); used once set (ufix_on)
     vector<double> ufix;
     bool ufix_on = false;
+    // cell speed of sound of the mixture EOS, set once at the first step the
+    // way calcCellEOSVF sets it at load (eos.cpp:160-293); read by the P
+    // "switch" boundary (reconIncom.cpp:618-631, eqPressure.cpp:625-632)
+    vector<double> Csnd;
+    // VF "function" boundary values per boundary face (reconIncom.cpp:929-
+    // 1019): the VF of the nearest point of the set given by orc_set_vf_points
+    vector<double> vffun;
     vector<double> gP, gU, gV, gW, gVF;    // [3n]
     vector<double> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
     // face left/right values (src/solvers/reconIncom.cpp:501-1035)
@@ -345,10 +354,41 @@ void cell_eos_transport(Oracle& o) {
     }
 }
 
+// Speed of sound of the two-phase constant-density mixture, as calcCellEOSVF
+// sets cell.var[C] at load (eos.cpp:160-293; the const branch gives every
+// phase drho/dP = drho/dT = dh/dP = dh/dT = 1e8, eos.cpp:184-192).  The
+// incompressible iteration never updates it.
+void cell_sound_speed(Oracle& o) {
+    o.Csnd.assign(o.nc, 0.0);
+    for (int i = 0; i < o.nc; ++i) {
+        double a = std::max(0.0, std::min(1.0, o.VF[i]));
+        double vf[2] = {std::max(0.0, std::min(1.0, a)), 0.0};
+        vf[1] = std::max(0.0, std::min(1.0, 1.0 - a));
+        double rho = 0.0;
+        for (int k = 0; k < 2; ++k) rho += vf[k] * o.rho[k];
+        double mf[2];
+        mf[0] = std::max(0.0, std::min(1.0, vf[0] * o.rho[0] / rho));
+        mf[1] = std::max(0.0, std::min(1.0, 1.0 - mf[0]));
+        double drdp = 0.0, drdt = 0.0, dhdp = 0.0, dhdt = 0.0;
+        for (int k = 0; k < 2; ++k) {
+            drdp += vf[k] * 1.0e+8; drdt += vf[k] * 1.0e+8;
+            dhdp += mf[k] * 1.0e+8; dhdt += mf[k] * 1.0e+8;
+        }
+        double c2 = drdp + 1.0 / rho * drdt / dhdt * (1.0 - rho * dhdp);
+        o.Csnd[i] = sqrt(1.0 / c2);
+    }
+}
+
+// cell Mach number of the P "switch" boundary (reconIncom.cpp:620-625)
+double mach_of(const Oracle& o, int c) {
+    return sqrt(pow(o.U[c], 2.0) + pow(o.V[c], 2.0) + pow(o.W[c], 2.0)) / o.Csnd[c];
+}
+
 // setIncomValuesLeftRightFace -> reconIncomZeroOrder + calcOtherDataIncomFromEOS
 // (reconIncom.cpp:9-67, 440-496, 501-1035)
 void recon_faces(Oracle& o) {
     int nf = o.nf;
+    if ((int)o.Csnd.size() != o.nc) cell_sound_speed(o);   // before the first step
     auto sz = [&](vector<double>& v) { v.assign(nf, 0.0); };
     sz(o.LP); sz(o.RP); sz(o.LU); sz(o.RU); sz(o.LV); sz(o.RV); sz(o.LW); sz(o.RW);
     sz(o.LVF); sz(o.RVF); sz(o.Lmu); sz(o.Rmu); sz(o.LRho); sz(o.RRho);
@@ -363,8 +403,10 @@ void recon_faces(Oracle& o) {
         for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
             int c = o.owner[f];
             const double* n = &o.fnrm[3 * f];
-            // pressure (reconIncom.cpp:600-617)
-            if (pb.p_type == FIXED_VALUE) o.LP[f] = pb.p_val; else o.LP[f] = o.P[c];
+            // pressure (reconIncom.cpp:600-632)
+            if (pb.p_type == FIXED_VALUE) o.LP[f] = pb.p_val;
+            else if (pb.p_type == SWITCH) o.LP[f] = mach_of(o, c) > 1.0 ? o.P[c] : pb.p_val;
+            else o.LP[f] = o.P[c];
             o.RP[f] = o.LP[f];
             // velocity (reconIncom.cpp:642-852)
             if (pb.u_type == FIXED_VALUE) {
@@ -408,8 +450,9 @@ void recon_faces(Oracle& o) {
                 else { o.LU[f] = nv * n[0]; o.LV[f] = nv * n[1]; o.LW[f] = nv * n[2]; }
                 o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
             }
-            // volume fraction (reconIncom.cpp:882-928)
+            // volume fraction (reconIncom.cpp:882-1019)
             if (pb.vf_type == FIXED_VALUE) o.LVF[f] = pb.vf_val;
+            else if (pb.vf_type == FUNCTION) o.LVF[f] = o.vffun[f - o.nif];
             else if (pb.vf_type == INLET_OUTLET) {
                 double own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
                 o.LVF[f] = own > 0.0 ? o.VF[c] : pb.vf_val;
@@ -683,6 +726,7 @@ void assemble_pressure(Oracle& o) {
             double RhoL = o.LRho[f];
             double t1 = 1.0 / RhoL * o.dt;
             double cPd = pb.p_type == FIXED_VALUE ? 1.0 : 0.0;
+            if (pb.p_type == SWITCH) cPd = mach_of(o, c) > 1.0 ? 1.0 : 0.0;     // eqPressure.cpp:625-632
             double dPN = 0.5 * o.fmagPN[f], alpha = o.falpha[f];
             if (pb.p_type == FIXED_VALUE) {
                 UnF -= alpha * t1 * (o.RP[f] - o.P[c]) / dPN;
@@ -947,18 +991,20 @@ void curvature(Oracle& o) {
     }
 }
 
-// calcGaussGreen (gradient.cpp:1778-1985) of VF with the previous gradient
-// `grad` as skew correction, boundary values by the VF boundary type
-void gauss_green_vf(const Oracle& o, vector<double>& grad) {
-    int nc = o.nc;
-    vector<double> gt(3 * (size_t)nc, 0.0);
+// calcGaussGreen(mesh, cn, fn, gradient) (gradient.cpp:1778-1985) of one cell
+// field, one rank: internal faces in face order (skew-corrected linear face
+// value), then the boundary patches by the field's type (sel: 0 = P, 1..3 =
+// U, V, W, 4 = VF); faceR: varR[fn] per face; grad in: the old gradient,
+// out: the new one.
+void gauss_green(Oracle& o, int sel, const double* phi, const double* faceR, double* grad) {
+    vector<double> gt(3 * (size_t)o.nc, 0.0);
     for (int f = 0; f < o.nif; ++f) {
         int l = o.owner[f], r = o.nbr[f];
         double wCL = o.fwC[f], wCR = 1.0 - wCL;
         const double* sk = &o.fskew[3 * (size_t)f];
-        double v = wCL * o.VF[l] + wCR * o.VF[r];
-        for (int a = 0; a < 3; ++a) v += wCL * grad[3 * l + a] * sk[a];
-        for (int a = 0; a < 3; ++a) v += wCR * grad[3 * r + a] * sk[a];
+        double v = wCL * phi[l] + wCR * phi[r];
+        v += wCL * grad[3 * l] * sk[0]; v += wCL * grad[3 * l + 1] * sk[1]; v += wCL * grad[3 * l + 2] * sk[2];
+        v += wCR * grad[3 * r] * sk[0]; v += wCR * grad[3 * r + 1] * sk[1]; v += wCR * grad[3 * r + 2] * sk[2];
         for (int j = 0; j < 3; ++j) {
             gt[3 * l + j] += v * o.fnrm[3 * f + j] * o.farea[f] / o.cvol[l];
             gt[3 * r + j] -= v * o.fnrm[3 * f + j] * o.farea[f] / o.cvol[r];
@@ -967,16 +1013,18 @@ void gauss_green_vf(const Oracle& o, vector<double>& grad) {
     for (const Patch& pb : o.patches)
         for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
             int c = o.owner[f];
-            double v = o.VF[c];
-            if (pb.vf_type == FIXED_VALUE) v = pb.vf_val;
-            else if (pb.vf_type == ZERO_GRADIENT) {
+            int type = sel == 0 ? pb.p_type : sel == 4 ? pb.vf_type : pb.u_type;
+            double val = sel == 0 ? pb.p_val : sel == 4 ? pb.vf_val : pb.u_val[sel - 1];
+            double v = phi[c];
+            if (type == FIXED_VALUE) v = val;
+            else if (type == ZERO_GRADIENT) {
                 v += grad[3 * c] * (o.fxyz[3 * f] - o.cxyz[3 * c]);
                 v += grad[3 * c + 1] * (o.fxyz[3 * f + 1] - o.cxyz[3 * c + 1]);
                 v += grad[3 * c + 2] * (o.fxyz[3 * f + 2] - o.cxyz[3 * c + 2]);
-            } else v = 0.5 * (o.VF[c] + o.RVF[f]);
+            } else v = 0.5 * (phi[c] + faceR[f]);
             for (int j = 0; j < 3; ++j) gt[3 * c + j] += v * o.fnrm[3 * f + j] * o.farea[f] / o.cvol[c];
         }
-    grad = gt;
+    std::copy(gt.begin(), gt.end(), grad);
 }
 
 // calcSourceSurfaceTension (source/surfaceTension.cpp:7-229): curvature, then
@@ -989,7 +1037,7 @@ void source_surface_tension(Oracle& o) {
     curvature(o);
     vector<double> g;
     lsq_grad(o, o.VF, o.RVF, g);
-    gauss_green_vf(o, g);
+    gauss_green(o, 4, o.VF.data(), o.RVF.data(), g.data());
     for (int i = 0; i < o.nc; ++i) {
         double s = o.sigma, kp = o.kappa[i];
         o.srcS[0][i] = (s * kp * g[3 * i]);
@@ -1310,6 +1358,7 @@ double time_step(Oracle& o, int adjust) {
 // levels, time step, n_vof VOF iterations, body-force sources, then ncorr
 // momentum + pressure iterations.
 void pressure_based_step(Oracle& o, int ncorr, double* norms) {
+    if (o.iterReal == 0) cell_sound_speed(o);
     if (o.iterReal == 0) cell_eos_transport(o);
     if (o.iterReal == 0) { o.Qm0 = o.U; o.Qm1 = o.V; o.Qm2 = o.W; o.Qm3 = o.VF; }
     else { o.Qm0 = o.oldU; o.Qm1 = o.oldV; o.Qm2 = o.oldW; o.Qm3 = o.oldVF; }
@@ -1472,6 +1521,26 @@ void* orc_create(int np, int nc, int nf, int nif, const double* pts, const int* 
 
 void orc_destroy(void* h) { delete (Oracle*)h; }
 
+// VF "function" boundary values: for every boundary face, the VF of the
+// nearest of the n points (vf, x, y, z) -- the rows of ./VF.csv, columns 3-6
+// (reconIncom.cpp:929-1019): squared distance by pow(.., 2.0), first point of
+// the smallest distance below 1e9 wins
+void orc_set_vf_points(void* h, int n, const double* p4) {
+    Oracle& o = *(Oracle*)h;
+    int nb = o.nf - o.nif;
+    o.vffun.assign(nb, 0.0);
+    for (int j = 0; j < nb; ++j) {
+        const double* x = &o.fxyz[3 * (size_t)(o.nif + j)];
+        double best = 1.e9, v = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double* q = p4 + 4 * (size_t)k;
+            double d2 = (pow(x[0] - q[1], 2.0) + pow(x[1] - q[2], 2.0) + pow(x[2] - q[3], 2.0));
+            if (d2 < best) { best = d2; v = q[0]; }
+        }
+        o.vffun[j] = v;
+    }
+}
+
 void orc_set_bc(void* h, int b, int p_type, double p_val, int u_type, const double* u_val,
                 int vf_type, double vf_val) {
     Oracle& o = *(Oracle*)h;
@@ -1527,7 +1596,7 @@ static vector<double>* field(Oracle& o, const char* nm) {
     F("faceLU", LU) F("faceLV", LV) F("faceLW", LW) F("faceLRho", LRho) F("faceLVF", LVF)
     F("oldVF", oldVF) F("Qm3", Qm3) F("courant", co_cell) F("kappa", kappa) F("LS", LS)
     F("srcS0", srcS[0]) F("srcS1", srcS[1]) F("srcS2", srcS[2]) F("srcS3", srcS[3])
-    F("alphaP", alphaP) F("ufix", ufix)
+    F("alphaP", alphaP) F("ufix", ufix) F("C", Csnd)
 #undef F
     return nullptr;
 }
@@ -1582,6 +1651,9 @@ void orc_minmax(void* h, const double* phi, double* mx, double* mn) {
     vector<double> p(phi, phi + o.nc), a, b;
     minmax(o, p, a, b);
     memcpy(mx, a.data(), a.size() * 8); memcpy(mn, b.data(), b.size() * 8);
+}
+void orc_gauss_green(void* h, int sel, const double* phi, const double* faceR, double* grad) {
+    gauss_green(*(Oracle*)h, sel, phi, faceR, grad);
 }
 void orc_assemble_momentum(void* h) { assemble_momentum(*(Oracle*)h); }
 void orc_assemble_pressure(void* h) { assemble_pressure(*(Oracle*)h); }

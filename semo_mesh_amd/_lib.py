@@ -23,6 +23,7 @@ STATUS = {0: "SEMO_OK", 1: "SEMO_EINVAL", 2: "SEMO_EINVAL_BC", 3: "SEMO_ESINGULA
 F = dict(P=0, U=1, V=2, W=3, VF=4, Rho=5, mu=6, oldU=7, oldV=8, oldW=9, Qm0=10, Qm1=11, Qm2=12,
          srcG0=13, srcG1=14, srcG2=15, maxP=16, minP=17, maxU=18, minU=19, maxV=20, minV=21,
          maxW=22, minW=23, alphaP=24, Qm3=25, oldVF=26, kappa=27, srcS0=28, srcS1=29, srcS2=30, LS=31, gradP=32, gradU=33, gradV=34, gradW=35, gradVF=36,
+         C=40,
          x0=48, x1=49, x2=50, b0=56, b1=57, b2=58,
          cell_volume=64, cell_xyz=65, cell_lsq=66,
          face_area=80, face_normal=81, face_xyz=82, face_wC=83, face_magPN=84, face_alphaF=85,
@@ -67,12 +68,12 @@ class SolveStats(C.Structure):
 EXPORTED = [
     "semo_ctx_create", "semo_ctx_destroy", "semo_ctx_last_error", "semo_ctx_device_bytes",
     "semo_mesh_upload", "semo_mesh_get_perm", "semo_mesh_get_face_map", "semo_mesh_get_connectivity",
-    "semo_rcm", "semo_set_case", "semo_set_solver", "semo_field_set", "semo_field_get",
+    "semo_rcm", "semo_set_case", "semo_set_vf_function_points", "semo_set_solver", "semo_field_set", "semo_field_get",
     "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_calc_volfrac_eq", "semo_calc_time_step",
     "semo_calc_source_gravity", "semo_calc_source_surface_tension",
     "semo_incompressible_pressure_based", "semo_adjoint_alpha_p", "semo_objective_pressure",
     "semo_calc_coupled_eq", "semo_coupled_pressure_based", "semo_coupled_matrix_get", "semo_coupled_spmv", "semo_coupled_solve",
-    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_solve_coo",
+    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_gauss_green", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
 ]
 
@@ -98,6 +99,7 @@ def lib():
         "semo_mesh_get_connectivity": (C.c_int, [vp, vp, vp]),
         "semo_rcm": (C.c_int, [i32, vp, vp, vp, vp]),
         "semo_set_case": (C.c_int, [vp, C.POINTER(CaseDesc), vp]),
+        "semo_set_vf_function_points": (C.c_int, [vp, i32, dp]),
         "semo_set_solver": (C.c_int, [vp, C.POINTER(SolverParams)]),
         "semo_field_set": (C.c_int, [vp, C.c_int, dp, i64]),
         "semo_field_get": (C.c_int, [vp, C.c_int, dp, i64]),
@@ -120,6 +122,7 @@ def lib():
         "semo_matrix_get": (C.c_int, [vp, C.POINTER(i64), vp, vp, vp, vp]),
         "semo_least_square": (C.c_int, [vp, dp, dp, dp]),
         "semo_cell_min_max": (C.c_int, [vp, dp, dp, dp]),
+        "semo_gauss_green": (C.c_int, [vp, C.c_int, dp, dp, dp]),
         "semo_solve_coo": (C.c_int, [vp, C.c_char_p, i32, i64, vp, vp, vp, i32, dp, dp,
                                      C.POINTER(SolveStats)]),
         "semo_profile_enable": (C.c_int, [vp, C.c_int]),

@@ -20,10 +20,12 @@ BC_NO_SLIP = 2
 BC_SLIP = 3
 BC_SURFACE_NORMAL_FIXED_VALUE = 4
 BC_INLET_OUTLET = 5
+BC_SWITCH = 6             # pressure: owner value above Mach 1, else fixed (reconIncom.cpp:618-631)
+BC_FUNCTION = 7           # volume fraction: nearest point of ./VF.csv (reconIncom.cpp:929-1019)
 BC_NAMES = {BC_ZERO_GRADIENT: "zeroGradient", BC_FIXED_VALUE: "fixedValue",
             BC_NO_SLIP: "noSlip", BC_SLIP: "slip",
             BC_SURFACE_NORMAL_FIXED_VALUE: "surfaceNormalFixedValue",
-            BC_INLET_OUTLET: "inletOutlet"}
+            BC_INLET_OUTLET: "inletOutlet", BC_SWITCH: "switch", BC_FUNCTION: "function"}
 
 
 @dataclass
@@ -53,6 +55,17 @@ class Case:
     max_co: float = 1.0       # maxCo
     max_dt: float = 1.0       # maxTimeStep
     bcs: dict = field(default_factory=dict)   # patch name -> PatchBC
+    # points (VF, x, y, z) of the volume-fraction "function" boundary: the
+    # rows of SEMO's ./VF.csv, columns 3-6 (reconIncom.cpp:929-1019)
+    vf_points: np.ndarray | None = None
+
+    def write_vf_csv(self, path: str) -> None:
+        """./VF.csv as reconIncomZeroOrder reads it: a header line, then
+        comma-separated rows whose columns 3-6 are VF, x, y, z."""
+        with open(path, "w") as fh:
+            fh.write("a,b,c,VF,x,y,z\n")
+            for k, (vf, x, y, z) in enumerate(np.asarray(self.vf_points, dtype=np.float64).reshape(-1, 4)):
+                fh.write(f"{k},0,0,{float(vf)!r},{float(x)!r},{float(y)!r},{float(z)!r}\n")
 
     def write_ref_case(self, path: str, steps: int = 1, correctors: int = 1,
                        init: str | None = None) -> None:

@@ -663,6 +663,9 @@ double adjoint_run(semo_ctx* c, int n_iter, const double* wpatch_h, double* grad
                "adjoint: single phase with a fixed time step only (n_vof = 0, adjust_dt = 0)");
     SEMO_CHECK(c->cs.rho[0] == c->cs.rho[1] && c->cs.mu[0] == c->cs.mu[1], SEMO_EINVAL,
                "adjoint: both phases must have the same rho and mu");
+    for (const semo_patch_bc& b : c->bcs)
+        SEMO_CHECK(b.p_type != SEMO_BC_SWITCH, SEMO_EINVAL_BC,
+                   "adjoint: the Mach-switched pressure boundary is not differentiated");
     if (!c->adj) c->adj = new AdjBufs();
     AdjBufs& B = *c->adj;
     check_stencil_symmetric(c, B);

@@ -36,6 +36,64 @@ __global__ void __launch_bounds__(TPB) k_eos(int nc, CaseK k, double* __restrict
     Mu[i] = m + 0.0;
 }
 
+// Speed of sound of the constant-density mixture, as calcCellEOSVF sets
+// cell.var[C] at load (eos.cpp:160-293): every "const" phase has drho/dP =
+// drho/dT = dh/dP = dh/dT = 1e8 (eos.cpp:184-192), mixed by volume (rho
+// derivatives) and mass fraction (enthalpy derivatives).
+__global__ void __launch_bounds__(TPB) k_sound_speed(int nc, CaseK k, const double* __restrict__ VF,
+                                                     double* __restrict__ Cs) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double a = clip01(VF[i]);
+    double v0 = clip01(a), v1 = clip01(1.0 - a);
+    double rho = 0.0; rho += v0 * k.rho0; rho += v1 * k.rho1;
+    double m0 = clip01(v0 * k.rho0 / rho), m1 = clip01(1.0 - m0);
+    double drdp = 0.0, drdt = 0.0, dhdp = 0.0, dhdt = 0.0;
+    drdp += v0 * 1.0e+8; drdt += v0 * 1.0e+8; dhdp += m0 * 1.0e+8; dhdt += m0 * 1.0e+8;
+    drdp += v1 * 1.0e+8; drdt += v1 * 1.0e+8; dhdp += m1 * 1.0e+8; dhdt += m1 * 1.0e+8;
+    double c2 = drdp + 1.0 / rho * drdt / dhdt * (1.0 - rho * dhdp);
+    Cs[i] = sqrt(1.0 / c2);
+}
+
+// Mach number |U|/C of cell c, the P "switch" test (reconIncom.cpp:620-625,
+// eqPressure.cpp:626-630)
+__device__ inline double cell_mach(int c, const double* __restrict__ U, const double* __restrict__ V,
+                                   const double* __restrict__ W, const double* __restrict__ Cs) {
+    double u = U[c], v = V[c], w = W[c];
+    return sqrt(u * u + v * v + w * w) / Cs[c];
+}
+
+// VF "function" boundary (reconIncom.cpp:929-1019): the VF of the nearest of
+// the given points to each boundary face centroid; the first point of the
+// smallest squared distance below 1e9 wins (none: 0)
+__global__ void __launch_bounds__(TPB) k_vf_nearest(int nbf, int nif, const double* __restrict__ fxyz, int np,
+                                                    const double* __restrict__ p4, double* __restrict__ out) {
+    int j = xcd_block() * blockDim.x + threadIdx.x;
+    if (j >= nbf) return;
+    const double* x = fxyz + 3 * ((size_t)nif + j);
+    double x0 = x[0], x1 = x[1], x2 = x[2];
+    double best = 1.e9, v = 0.0;
+    for (int q = 0; q < np; ++q) {
+        const double* p = p4 + 4 * (size_t)q;
+        double d0 = x0 - p[1], d1 = x1 - p[2], d2 = x2 - p[3];
+        double dist = (d0 * d0 + d1 * d1 + d2 * d2);
+        if (dist < best) { best = dist; v = p[0]; }
+    }
+    out[j] = v;
+}
+
+// flags[0] / flags[1]: some boundary P / VF value differs from the saved one
+// (a null current array skips that comparison)
+__global__ void __launch_bounds__(TPB) k_bchanged(int nbf, const double* __restrict__ bP,
+                                                  const double* __restrict__ bP_g, const double* __restrict__ bVF,
+                                                  const double* __restrict__ bVF_g, int* __restrict__ flags) {
+    int j = xcd_block() * blockDim.x + threadIdx.x;
+    if (j >= nbf) return;
+    if (bP && bP[j] != bP_g[j]) flags[0] = 1;
+    if (bVF && bVF[j] != bVF_g[j]) flags[1] = 1;
+}
+__global__ void k_clear2(int* __restrict__ flags) { flags[0] = 0; flags[1] = 0; }
+
 // ------------------------------------------------------ boundary face values --
 // reconIncomZeroOrder boundary dispatch (reconIncom.cpp:591-1032) and the face
 // density of calcOtherDataIncomFromEOS (reconIncom.cpp:445-493)
@@ -44,7 +102,8 @@ __global__ void __launch_bounds__(TPB) k_recon_boundary(
     const int* __restrict__ owner, const double* __restrict__ nrm, const double* __restrict__ fxyz,
     const double* __restrict__ P, const double* __restrict__ U, const double* __restrict__ V,
     const double* __restrict__ W, const double* __restrict__ VF, const double* __restrict__ Mu,
-    const double* __restrict__ ufix, double* __restrict__ bP, double* __restrict__ bUL, double* __restrict__ bUR,
+    const double* __restrict__ ufix, const double* __restrict__ Cs, const double* __restrict__ vffun,
+    double* __restrict__ bP, double* __restrict__ bUL, double* __restrict__ bUR,
     double* __restrict__ bVF, double* __restrict__ bMu, double* __restrict__ bRho) {
     int j = xcd_block() * blockDim.x + threadIdx.x;
     if (j >= nbf) return;
@@ -52,7 +111,9 @@ __global__ void __launch_bounds__(TPB) k_recon_boundary(
     int c = owner[f];
     semo_patch_bc bc = bcs[bpatch[j]];
     const double* n = nrm + 3 * (size_t)f;
-    bP[j] = bc.p_type == SEMO_BC_FIXED_VALUE ? bc.p_value : P[c];
+    if (bc.p_type == SEMO_BC_FIXED_VALUE) bP[j] = bc.p_value;
+    else if (bc.p_type == SEMO_BC_SWITCH) bP[j] = cell_mach(c, U, V, W, Cs) > 1.0 ? P[c] : bc.p_value;
+    else bP[j] = P[c];
     double uL[3], uR[3];
     double uc[3] = {U[c], V[c], W[c]};
     switch (bc.u_type) {
@@ -94,6 +155,7 @@ __global__ void __launch_bounds__(TPB) k_recon_boundary(
     for (int a = 0; a < 3; ++a) { bUL[a * (size_t)nbf + j] = uL[a]; bUR[a * (size_t)nbf + j] = uR[a]; }
     double vf;
     if (bc.vf_type == SEMO_BC_FIXED_VALUE) vf = bc.vf_value;
+    else if (bc.vf_type == SEMO_BC_FUNCTION) vf = vffun[j];
     else if (bc.vf_type == SEMO_BC_INLET_OUTLET) {
         double own = uc[0] * n[0] + uc[1] * n[1] + uc[2] * n[2];
         vf = own > 0.0 ? VF[c] : bc.vf_value;
@@ -312,8 +374,11 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
 // order.  faceR: boundary values indexed by face - foff; null: "input" mode
 // with cn = -1, a zero boundary difference (gradient.cpp:1018-1022).  The
 // sums run in stencil order, as the reference's.
+// gate (optional): skip the pass unless *gate != 0
 __global__ void __launch_bounds__(TPB) k_pack_xphi(int nc, const double* __restrict__ cxyz,
-                                                   const double* __restrict__ phi, double* __restrict__ xph) {
+                                                   const double* __restrict__ phi, double* __restrict__ xph,
+                                                   const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double2* o = reinterpret_cast<double2*>(xph + 4 * (size_t)i);
@@ -327,7 +392,9 @@ template <class IDX, int UNR = 8>
 __global__ void __launch_bounds__(TPB) k_lsq1(
     int nc, const int* __restrict__ st_sl, const IDX* __restrict__ st_ell, const int* __restrict__ bf_ptr,
     const int* __restrict__ bf_idx, const double* __restrict__ xph, const double* __restrict__ fxyz,
-    const double* __restrict__ lsq, const double* __restrict__ faceR, int foff, double* __restrict__ g) {
+    const double* __restrict__ lsq, const double* __restrict__ faceR, int foff, double* __restrict__ g,
+    const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     const double2* me = reinterpret_cast<const double2*>(xph + 4 * (size_t)i);
@@ -570,7 +637,8 @@ __global__ void __launch_bounds__(TPB) k_p_cell(
     const double* __restrict__ magPN, const double* __restrict__ alphaF, const double* __restrict__ P,
     const double* __restrict__ gP, const double* __restrict__ bP, const double* __restrict__ bUL,
     const double* __restrict__ bUR, const double* __restrict__ bRho, const double* __restrict__ s,
-    double* __restrict__ diag, double* __restrict__ B) {
+    const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
+    const double* __restrict__ Cs, double* __restrict__ diag, double* __restrict__ B) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double d = 0.0, b = 0.0;
@@ -594,6 +662,8 @@ __global__ void __launch_bounds__(TPB) k_p_cell(
             cPd = 1.0;
             UnF -= alpha * t1 * (bP[j] - P[i]) / dPN;
             for (int a = 0; a < 3; ++a) UnF += alpha * t1 * gP[3 * (size_t)i + a] * e[a];
+        } else if (bc.p_type == SEMO_BC_SWITCH) {
+            cPd = cell_mach(i, U, V, W, Cs) > 1.0 ? 1.0 : 0.0;
         }
         d += cPd * (alpha * t1 / dPN * A);
         b -= (UnF * A);
@@ -1137,6 +1207,52 @@ __global__ void __launch_bounds__(TPB) k_gg_cell(
 }
 
 
+// calcGaussGreen (gradient.cpp:1778-1985) of one cell field, gathered per
+// cell in ascending face order (the order in which the reference's face loops
+// accumulate into each cell).  Internal faces: wC phi_P + (1 - wC) phi_N plus
+// the skewness correction with the input gradient.  Boundary faces by the
+// field's patch type: fixedValue -> the patch value, zeroGradient -> phi_P +
+// grad_P . (x_f - x_P), any other type -> the mean of phi_P and the face's
+// right value.  sel: 0 = P, 1..3 = U, V, W, 4 = VF.
+__global__ void __launch_bounds__(TPB) k_gauss_green(
+    int nc, int nif, int sel, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
+    const int* __restrict__ owner, const int* __restrict__ nbr, const int* __restrict__ bpatch,
+    const semo_patch_bc* __restrict__ bcs, const double* __restrict__ nrm, const double* __restrict__ area,
+    const double* __restrict__ wCv, const double* __restrict__ skew, const double* __restrict__ fxyz,
+    const double* __restrict__ cxyz, const double* __restrict__ vol, const double* __restrict__ phi,
+    const double* __restrict__ faceR, const double* __restrict__ gin, double* __restrict__ gout) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double g[3] = {0, 0, 0};
+    double vi = vol[i];
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
+        const double* n = nrm + 3 * (size_t)f;
+        double A = area[f];
+        if (f < nif) {
+            int l = owner[f], r = nbr[f];
+            double wCL = wCv[f], wCR = 1.0 - wCL;
+            const double* sk = skew + 3 * (size_t)f;
+            double v = wCL * phi[l] + wCR * phi[r];
+            for (int a = 0; a < 3; ++a) v += wCL * gin[3 * (size_t)l + a] * sk[a];
+            for (int a = 0; a < 3; ++a) v += wCR * gin[3 * (size_t)r + a] * sk[a];
+            if (own) { for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi; }
+            else { for (int a = 0; a < 3; ++a) g[a] -= v * n[a] * A / vi; }
+        } else {
+            int j = f - nif;
+            semo_patch_bc bc = bcs[bpatch[j]];
+            int type = sel == 0 ? bc.p_type : sel == 4 ? bc.vf_type : bc.u_type;
+            double val = sel == 0 ? bc.p_value : sel == 4 ? bc.vf_value : bc.u_value[sel - 1];
+            double v = phi[i];
+            if (type == SEMO_BC_FIXED_VALUE) v = val;
+            else if (type == SEMO_BC_ZERO_GRADIENT) {
+                for (int a = 0; a < 3; ++a) v += gin[3 * (size_t)i + a] * (fxyz[3 * (size_t)f + a] - cxyz[3 * (size_t)i + a]);
+            } else v = 0.5 * (phi[i] + faceR[j]);
+            for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi;
+        }
+    }
+    for (int a = 0; a < 3; ++a) gout[3 * (size_t)i + a] = g[a];
+}
+
 __device__ inline double clipv(double v, double lim) {
     if (v <= -lim) v = -lim;
     if (v >= lim) v = lim;
@@ -1244,14 +1360,44 @@ void eos_transport(semo_ctx* c) {
     hipLaunchKernelGGL(k_eos, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, k, c->VF.p, c->Rho.p, c->Mu.p);
 }
 
+// cell.var[C] as calcCellEOSVF leaves it at load
+void sound_speed(semo_ctx* c) {
+    c->Csnd.alloc(c->nc);
+    hipLaunchKernelGGL(k_sound_speed, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, case_k(c), c->VF.p,
+                       c->Csnd.p);
+    c->csnd_valid = true;
+}
+
 void recon_boundary(semo_ctx* c) {
+    if (!c->csnd_valid) sound_speed(c);
     if (c->nbf == 0) return;
+    for (const semo_patch_bc& b : c->bcs)
+        SEMO_CHECK(b.vf_type != SEMO_BC_FUNCTION || c->b_vffun.n == (size_t)c->nbf, SEMO_EINVAL_BC,
+                   "volume-fraction function boundary without points (semo_set_vf_function_points)");
     CaseK k = case_k(c);
     ProfScope ps(c, "recon_boundary", 120.0 * c->nbf);
     hipLaunchKernelGGL(k_recon_boundary, dim3(nblk(c->nbf)), dim3(TPB), 0, c->stream, c->nbf, c->nif, k,
                        c->d_bpatch.p, c->d_bcs.p, c->d_owner.p, c->f_nrm.p, c->f_xyz.p, c->P.p, c->U.p,
-                       c->V.p, c->W.p, c->VF.p, c->Mu.p, c->b_ufix.n ? c->b_ufix.p : nullptr, c->bP.p, c->bUL.p,
-                       c->bUR.p, c->bVF.p, c->bMu.p, c->bRho.p);
+                       c->V.p, c->W.p, c->VF.p, c->Mu.p, c->b_ufix.n ? c->b_ufix.p : nullptr, c->Csnd.p,
+                       c->b_vffun.p, c->bP.p, c->bUL.p, c->bUR.p, c->bVF.p, c->bMu.p, c->bRho.p);
+}
+
+void gauss_green_standalone(semo_ctx* c, int sel, const double* phi, const double* faceR_b, const double* gin,
+                            double* gout) {
+    hipLaunchKernelGGL(k_gauss_green, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, sel, c->d_cf_sl.p,
+                       c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p,
+                       c->f_wC.p, c->f_skew.p, c->f_xyz.p, c->c_xyz.p, c->c_vol.p, phi, faceR_b, gin, gout);
+}
+
+void vf_function_values(semo_ctx* c, int n, const double* pts4) {
+    if (n == 0) { c->b_vffun.release(); return; }
+    if (c->nbf == 0) return;
+    DBuf<double> p;
+    p.upload(pts4, 4 * (size_t)n, c->stream);
+    c->b_vffun.alloc(c->nbf);
+    hipLaunchKernelGGL(k_vf_nearest, dim3(nblk(c->nbf)), dim3(TPB), 0, c->stream, c->nbf, c->nif, c->f_xyz.p, n,
+                       p.p, c->b_vffun.p);
+    SEMO_HIP(hipStreamSynchronize(c->stream));   // p is freed on return
 }
 
 void source_gravity(semo_ctx* c) {
@@ -1295,6 +1441,32 @@ void grad_minmax(semo_ctx* c, bool with_p, bool uvw_only) {
         if (uvw_only) go(k_grad_minmax<3, int>, c->d_st_ell.p);
         else go(k_grad_minmax<5, int>, c->d_st_ell.p);
     }
+    // boundary values the P and VF gradients were computed with, where a
+    // boundary type makes them depend on U (checked by a later U, V, W-only
+    // pass, which keeps those gradients)
+    if (!uvw_only && c->nbf) {
+        BcDeps dep = bc_deps(c);
+        if (dep.p) { c->bP_g.ensure(c->nbf); copy_d(c, c->bP_g.p, c->bP.p, c->nbf); }
+        if (dep.vf) { c->bVF_g.ensure(c->nbf); copy_d(c, c->bVF_g.p, c->bVF.p, c->nbf); }
+    }
+}
+
+// A U, V, W-only pass keeps the P and VF gradients of the last full pass.
+// Their boundary values can have moved with U since (P "switch": the owner's
+// Mach number, VF inletOutlet: the owner's outflow): a gradient whose
+// boundary values changed is recomputed on the device, decided there, so
+// no host round trip.  (Same kernels and summation order as the full pass.)
+void refresh_u_dependent_grads(semo_ctx* c) {
+    if (!c->nbf) return;
+    BcDeps dep = bc_deps(c);
+    if (!dep.p && !dep.vf) return;
+    c->d_bflag.ensure(2);
+    hipLaunchKernelGGL(k_clear2, dim3(1), dim3(1), 0, c->stream, c->d_bflag.p);
+    hipLaunchKernelGGL(k_bchanged, dim3(nblk(c->nbf)), dim3(TPB), 0, c->stream, c->nbf, dep.p ? c->bP.p : nullptr,
+                       c->bP_g.p, dep.vf ? c->bVF.p : nullptr, c->bVF_g.p, c->d_bflag.p);
+    ProfScope ps(c, "lsq1_bchanged", 0.0);
+    if (dep.p) lsq1(c, c->P.p, c->bP.p, c->nif, c->gP.p, c->d_bflag.p);
+    if (dep.vf) lsq1(c, c->VF.p, c->bVF.p, c->nif, c->gVF.p, c->d_bflag.p + 1);
 }
 
 void assemble_momentum(semo_ctx* c, bool after_vof, bool alpha_grad_ready) {
@@ -1303,6 +1475,9 @@ void assemble_momentum(semo_ctx* c, bool after_vof, bool alpha_grad_ready) {
     // recomputed; those of P, U, V, W and their bounds are the VOF stage's
     if (after_vof) {
         if (!alpha_grad_ready) lsq1(c, c->VF.p, c->bVF.p, c->nif, c->gVF.p);
+        // gVF now goes with the current boundary values (the surface-tension
+        // source computed its copy from the same ones)
+        if (c->nbf && bc_deps(c).vf) { c->bVF_g.ensure(c->nbf); copy_d(c, c->bVF_g.p, c->bVF.p, c->nbf); }
     } else {
         grad_minmax(c, true, false);
     }
@@ -1329,6 +1504,7 @@ void assemble_momentum(semo_ctx* c, bool after_vof, bool alpha_grad_ready) {
 void assemble_pressure(semo_ctx* c, bool after_momentum) {
     CaseK k = case_k(c);
     grad_minmax(c, false, after_momentum);
+    if (after_momentum) refresh_u_dependent_grads(c);
     if (c->nif) {
         double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 2 * 8) + (double)c->nc * 8 * (4 + 12 + 6 + 2);
         ProfScope ps(c, "p_face", bytes);
@@ -1340,7 +1516,8 @@ void assemble_pressure(semo_ctx* c, bool after_momentum) {
         hipLaunchKernelGGL(k_p_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
                            c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p,
                            c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->P.p, c->gP.p, c->bP.p,
-                           c->bUL.p, c->bUR.p, c->bRho.p, c->fscr.p, c->Amat.diag.p, c->Bvec.p);
+                           c->bUL.p, c->bUR.p, c->bRho.p, c->fscr.p, c->U.p, c->V.p, c->W.p, c->Csnd.p,
+                           c->Amat.diag.p, c->Bvec.p);
     }
     c->last_eq = 1;
 }
@@ -1379,17 +1556,17 @@ void assemble_vof(semo_ctx* c) {
 // one field's LSQ gradient (k_lsq1): packs [x, y, z, phi], then the
 // stencil pass; faceR indexed by face - foff, null for a zero boundary
 // difference
-void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double* g) {
+void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double* g, const int* gate) {
     int n = c->nc;
     c->xph.ensure(4 * (size_t)n);
     dim3 gb(nblk(n)), tb(TPB);
-    hipLaunchKernelGGL(k_pack_xphi, gb, tb, 0, c->stream, n, c->c_xyz.p, phi, c->xph.p);
+    hipLaunchKernelGGL(k_pack_xphi, gb, tb, 0, c->stream, n, c->c_xyz.p, phi, c->xph.p, gate);
     if (c->d_st_dl.n)
         hipLaunchKernelGGL(k_lsq1<int16_t>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
-                           c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g);
+                           c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
     else
         hipLaunchKernelGGL(k_lsq1<int>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
-                           c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g);
+                           c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
 }
 
 // calcSourceSurfaceTension with calcCurvature (curvature.cpp:7-3537,

@@ -95,6 +95,7 @@ DBuf<double>* cell_field(semo_ctx* c, int f, int& comps) {
         case SEMO_F_ALPHA_P: return &c->alphaP;
         case SEMO_F_QM3: return &c->Qm3; case SEMO_F_OLDVF: return &c->oldVF;
         case SEMO_F_KAPPA: return &c->kappa; case SEMO_F_LS: return &c->ls;
+        case SEMO_F_C: return &c->Csnd;
         case SEMO_F_GRADP: comps = 3; return &c->gP; case SEMO_F_GRADU: comps = 3; return &c->gU;
         case SEMO_F_GRADV: comps = 3; return &c->gV; case SEMO_F_GRADW: comps = 3; return &c->gW;
         case SEMO_F_GRADVF: comps = 3; return &c->gVF;
@@ -120,7 +121,10 @@ int guard(semo_ctx* c, const std::function<void()>& fn) {
 }
 
 void step_setup(semo_ctx* c) {
-    // pressureBased.cpp:17-69: first-step EOS/transport, time levels
+    // the speed of sound of the loaded state (calcCellEOSVF, main/
+    // incompressible/pressureBased.cpp:79), then pressureBased.cpp:17-69:
+    // first-step EOS/transport, time levels
+    if (c->iterReal == 0 && !c->csnd_user) sound_speed(c);
     if (c->iterReal == 0) eos_transport(c);
     const DBuf<double>& q0 = c->iterReal == 0 ? c->U : c->oldU;
     const DBuf<double>& q1 = c->iterReal == 0 ? c->V : c->oldV;
@@ -238,6 +242,7 @@ int semo_mesh_upload(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
         c->has_mesh = false; c->has_case = false; c->iterReal = 0;
         c->amg_p.reset();
         c->b_ufix.release(); c->h_ufix.clear();
+        c->b_vffun.release(); c->Csnd.release(); c->csnd_valid = c->csnd_user = false;
         coupled_free(c->cpl);          // mesh-bound block pattern and work arrays
         c->cpl = nullptr;
         adjoint_free(c->adj);
@@ -290,14 +295,16 @@ int semo_set_case(semo_ctx* c, const semo_case* cs, const semo_patch_bc* bcs) {
         SEMO_CHECK(cs->n_vof >= 0 && cs->urf_vf > 0.0, SEMO_EINVAL, "case: n_vof >= 0 and urf_vf > 0");
         SEMO_CHECK(!cs->adjust_dt || (cs->max_co > 0.0 && cs->max_dt > 0.0), SEMO_EINVAL,
                    "case: maxCo and maxTimeStep must be positive");
+        SEMO_CHECK(bcs || c->npatch == 0, SEMO_EINVAL, "case: null boundary-condition array");
         for (int b = 0; b < c->npatch; ++b) {
             const semo_patch_bc& p = bcs[b];
-            SEMO_CHECK(p.p_type == SEMO_BC_ZERO_GRADIENT || p.p_type == SEMO_BC_FIXED_VALUE, SEMO_EINVAL_BC,
+            SEMO_CHECK(p.p_type == SEMO_BC_ZERO_GRADIENT || p.p_type == SEMO_BC_FIXED_VALUE ||
+                           p.p_type == SEMO_BC_SWITCH, SEMO_EINVAL_BC,
                        "pressure boundary type not defined");
             SEMO_CHECK(p.u_type >= SEMO_BC_ZERO_GRADIENT && p.u_type <= SEMO_BC_INLET_OUTLET, SEMO_EINVAL_BC,
                        "velocity boundary type not defined");
             SEMO_CHECK(p.vf_type == SEMO_BC_ZERO_GRADIENT || p.vf_type == SEMO_BC_FIXED_VALUE ||
-                           p.vf_type == SEMO_BC_INLET_OUTLET, SEMO_EINVAL_BC,
+                           p.vf_type == SEMO_BC_INLET_OUTLET || p.vf_type == SEMO_BC_FUNCTION, SEMO_EINVAL_BC,
                        "volume-fraction boundary type not defined");
         }
         c->cs = *cs;
@@ -309,15 +316,26 @@ int semo_set_case(semo_ctx* c, const semo_case* cs, const semo_patch_bc* bcs) {
     });
 }
 
+int semo_set_vf_function_points(semo_ctx* c, int32_t n, const double* pts) {
+    if (!c || n < 0 || (n > 0 && !pts)) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        for (int64_t k = 0; k < 4 * (int64_t)n; ++k)
+            SEMO_CHECK(std::isfinite(pts[k]), SEMO_EINVAL, "non-finite volume-fraction point");
+        vf_function_values(c, n, pts);
+    });
+}
+
 int semo_set_solver(semo_ctx* c, const semo_solver_params* p) {
     if (!c || !p) return SEMO_EINVAL;
     return guard(c, [&] {
         SEMO_CHECK(p->tol_p > 0 && p->tol_u > 0 && p->maxit_p > 0 && p->maxit_u > 0, SEMO_EINVAL, "solver params");
         SEMO_CHECK(p->precision == SEMO_PREC_FP64 || p->precision == SEMO_PREC_MIXED, SEMO_EINVAL, "precision");
-        bool rebuild = p->coarse_size != c->sp.coarse_size;
+        int coarse = p->coarse_size > 0 ? p->coarse_size : 1000;      // 0: the default
+        bool rebuild = coarse != c->sp.coarse_size;
         c->sp = *p;
         if (c->sp.cheb_degree <= 0) c->sp.cheb_degree = 1;
-        if (c->sp.coarse_size <= 0) c->sp.coarse_size = 1000;
+        c->sp.coarse_size = coarse;
         if (rebuild) c->amg_p.built = false;
     });
 }
@@ -364,6 +382,7 @@ int semo_field_set(semo_ctx* c, int f, const double* h, int64_t n) {
         SEMO_CHECK(n == (int64_t)c->nc * comps, SEMO_EINVAL, "field size");
         to_dev(c, *d, h, comps);
         SEMO_HIP(hipStreamSynchronize(c->stream));
+        if (f == SEMO_F_C) c->csnd_valid = c->csnd_user = true;
     });
 }
 
@@ -705,6 +724,25 @@ int semo_least_square(semo_ctx* c, const double* phi, const double* face_r, doub
         dg.alloc(3 * (size_t)c->nc);
         lsq_standalone(c, dphi.p, dfr.p, dg.p);
         from_dev(c, dg.p, grad, 3);
+    });
+}
+
+int semo_gauss_green(semo_ctx* c, int field, const double* phi, const double* face_r, double* grad) {
+    if (!c || !phi || !face_r || !grad) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        int sel = field == SEMO_F_P ? 0 : field == SEMO_F_U ? 1 : field == SEMO_F_V ? 2 : field == SEMO_F_W ? 3
+                : field == SEMO_F_VF ? 4 : -1;
+        SEMO_CHECK(sel >= 0, SEMO_EINVAL, "Green-Gauss: field must be P, U, V, W or VF");
+        DBuf<double> dphi, dfr, gin, gout;
+        to_dev(c, dphi, phi, 1);
+        to_dev(c, gin, grad, 3);
+        std::vector<double> fr(std::max(c->nbf, 1));
+        for (int j = 0; j < c->nbf; ++j) fr[j] = face_r[c->fperm[c->nif + j]];
+        dfr.upload(fr, c->stream);
+        gout.alloc(3 * (size_t)c->nc);
+        gauss_green_standalone(c, sel, dphi.p, dfr.p, gin.p, gout.p);
+        from_dev(c, gout.p, grad, 3);
     });
 }
 

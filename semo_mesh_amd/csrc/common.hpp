@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -166,15 +167,24 @@ struct Amg {
     int iters_ref = -1, last_iters = -1;
     bool fresh = false;         // the last solve used a freshly built hierarchy
     // one V-cycle captured as a HIP graph (replayed per Krylov iteration):
-    // valid for the hierarchy's buffers and the operands it was captured with
+    // valid for the hierarchy's buffers and the operands it was captured
+    // with.  vg_sig lists every device address the captured kernels take
+    // (operands, level vectors and operators), sorted: a replay whose current
+    // list differs re-captures, so a reallocated or swapped buffer is never
+    // reached through a stale graph.
     hipGraphExec_t vg = nullptr;
-    const double* vg_r = nullptr;
-    double* vg_z = nullptr;
+    hipStream_t vg_stream = nullptr;      // stream of the exec's launches
+    std::vector<const void*> vg_sig;
     int vg_deg = 0, vg_prec = -1;
     bool vg_pre = false;
+    // The exec's last launch may still be queued: wait for its stream before
+    // the exec (and, in the callers, the buffers it references) goes away.
     void drop_graph() {
-        if (vg) (void)hipGraphExecDestroy(vg);
-        vg = nullptr; vg_r = nullptr; vg_z = nullptr; vg_prec = -1;
+        if (vg) {
+            if (vg_stream) (void)hipStreamSynchronize(vg_stream);
+            (void)hipGraphExecDestroy(vg);
+        }
+        vg = nullptr; vg_stream = nullptr; vg_sig.clear(); vg_prec = -1;
     }
     void reset() {
         drop_graph();
@@ -228,6 +238,9 @@ struct semo_ctx {
     // curvature divergence ([x, y, z, n, -, -])
     semo::DBuf<double> xph, xn;
     std::vector<double> h_ufix;     // as set, in the caller's numbering
+    // per-face values of the volume-fraction "function" boundary [nbf]
+    // (device face order), empty until semo_set_vf_function_points
+    semo::DBuf<double> b_vffun;
     // cell -> faces as sliced ELL (ascending faces; ~f for a face the cell
     // is the neighbour of; padding INT_MIN) and cell -> boundary faces (CSR)
     semo::DBuf<int> d_cf_sl, d_cf_ell, d_bf_ptr, d_bf_idx;
@@ -262,6 +275,12 @@ struct semo_ctx {
     semo::DBuf<double> P, U, V, W, VF, Rho, Mu, oldP, oldU, oldV, oldW, oldVF, Qm0, Qm1, Qm2,
         srcG, srcS, kappa, alphaP;
     semo::DBuf<double> Qm3;                    // alpha two time levels back (controls.Qm[3])
+    // speed of sound of the mixture (cell.var[C]): computed from the volume
+    // fraction at the start of the first time step (the state SEMO loads),
+    // or by the first boundary reconstruction before that; csnd_user: set by
+    // the caller, never recomputed
+    semo::DBuf<double> Csnd;
+    bool csnd_valid = false, csnd_user = false;
     semo::DBuf<double> c_maxA;                 // largest face area per cell (static)
     semo::DBuf<double> fVFL, fVFR;             // MSTACS face volume fractions [nif]
     // curvature: level set (2 buffers), pseudo-time step, sign velocity,
@@ -271,6 +290,10 @@ struct semo_ctx {
     semo::DBuf<double> gP, gU, gV, gW, gVF;   // [3n] interleaved
     semo::DBuf<double> mxP, mnP, mxU, mnU, mxV, mnV, mxW, mnW;
     semo::DBuf<double> bP, bUL, bUR, bVF, bMu, bRho;   // boundary faces (bUL/bUR: [3][nbf])
+    // bP / bVF as the last full gradient pass used them (U-dependent boundary
+    // types only), and the device flags of their change
+    semo::DBuf<double> bP_g, bVF_g;
+    semo::DBuf<int> d_bflag;
     semo::DBuf<double> gpp, gtmp;              // p' Green-Gauss gradient
     semo::DBuf<double> red;                    // reduction partials
     semo::DBuf<double> scal;                   // device scalars
@@ -310,6 +333,19 @@ inline double pipe_pop(semo_ctx* c, int k) {
     SEMO_HIP(hipEventSynchronize(c->pev[k & 3]));
     return ((volatile double*)c->pin)[k & 3];
 }
+// Final test of a solve, on its true FP64 relative residual: a non-finite
+// residual (NaN/Inf in the system or the iterates) or one above
+// max(10 tol, 1e-12) is a failure, whatever stopped the iteration.  (The
+// 1e-12 floor admits the FP64 round-off level of solves asked for tighter
+// tolerances.)
+inline void check_converged(double rel, double tol, const char* what) {
+    if (!std::isfinite(rel))
+        throw Error(SEMO_ENOCONV, std::string(what) + " solve produced a non-finite residual");
+    if (rel > std::max(10.0 * tol, 1e-12))
+        throw Error(SEMO_ENOCONV, std::string(what) + " solve did not converge (relative residual " +
+                                      std::to_string(rel) + ")");
+}
+
 struct ConvCheck {
     semo_ctx* c;
     double lim;                 // stop when the squared residual <= lim (NaN stops too)
@@ -343,6 +379,13 @@ void geometry_compute(semo_ctx* c);
 void fields_alloc(semo_ctx* c);
 void eos_transport(semo_ctx* c);
 void recon_boundary(semo_ctx* c);
+void sound_speed(semo_ctx* c);
+// nearest-point VF of every boundary face for the "function" boundary
+void vf_function_values(semo_ctx* c, int n, const double* pts4_host);
+// calcGaussGreen of one field: sel 0 = P, 1..3 = U, V, W, 4 = VF (its patch
+// types and values); faceR_b: right face values of the boundary faces [nbf]
+void gauss_green_standalone(semo_ctx* c, int sel, const double* phi, const double* faceR_b, const double* gin,
+                            double* gout);
 void source_gravity(semo_ctx* c);
 // uvw_only: gradients of U, V, W only; P and VF gradients are kept from the
 // last full pass (valid when P, VF and their boundary values are unchanged)
@@ -353,7 +396,19 @@ void grad_minmax(semo_ctx* c, bool with_p_minmax, bool uvw_only = false);
 void assemble_momentum(semo_ctx* c, bool after_vof = false, bool alpha_grad_ready = false);
 // one field's LSQ gradient over the vertex stencil and the boundary faces
 // (faceR indexed by face - foff; null: zero boundary difference)
-void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double* g);
+// gate (optional device flag): the pass runs only where *gate != 0
+void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double* g, const int* gate = nullptr);
+// boundary types whose face values depend on the owner's velocity: P
+// "switch" (Mach number), VF inletOutlet (flow direction)
+struct BcDeps { bool p = false, vf = false; };
+inline BcDeps bc_deps(const semo_ctx* c) {
+    BcDeps d;
+    for (const semo_patch_bc& b : c->bcs) {
+        d.p = d.p || b.p_type == SEMO_BC_SWITCH;
+        d.vf = d.vf || b.vf_type == SEMO_BC_INLET_OUTLET;
+    }
+    return d;
+}
 // after_momentum: called by an outer iteration right after its momentum
 // stage (P and VF unchanged since), so the P gradient of that stage is reused
 void assemble_pressure(semo_ctx* c, bool after_momentum = false);

@@ -618,6 +618,9 @@ void refresh_levels(semo_ctx* c, Coupled& C) {
 
 // momentum + pressure assembly of the current state, then the block system
 void assemble_coupled(semo_ctx* c) {
+    for (const semo_patch_bc& b : c->bcs)
+        SEMO_CHECK(b.p_type != SEMO_BC_SWITCH, SEMO_EINVAL_BC,
+                   "coupled system: the Mach-switched pressure boundary has no block form here");
     if (!c->cpl) c->cpl = new Coupled();
     Coupled& C = *c->cpl;
     int n = c->nc;
@@ -720,6 +723,7 @@ void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st
         st->rel_res[0] = C.last_relres;
         st->n_levels = (int)C.lv.size();
     }
+    check_converged(bb > 0.0 ? C.last_relres : bb, c->sp.tol_u, "coupled");
 }
 
 void coupled_solve_assembled(semo_ctx* c, semo_solve_stats* st) {

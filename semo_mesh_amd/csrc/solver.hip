@@ -828,6 +828,33 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
 int cheb_degree(const semo_ctx* c) { return c->sp.cheb_degree > 0 ? c->sp.cheb_degree : 2; }
 
 template <class T>
+void sig_op(std::vector<const void*>& s, const DevSell<T>& M) {
+    s.push_back(M.sl_ptr.p); s.push_back(M.col.p); s.push_back(M.dcol.p); s.push_back(M.val.p);
+    s.push_back(M.diag.p);
+}
+// every device address a V-cycle in precision T passes to its kernels
+template <class T>
+std::vector<const void*> vcycle_signature(const Amg& amg, const double* r64, const double* z64) {
+    std::vector<const void*> s{r64, z64, amg.coarse_inv.p};
+    for (const AmgLevel& L : amg.lv) {
+        if (sizeof(T) == 8) {
+            for (const void* p : {(const void*)L.x64.p, (const void*)L.b64.p, (const void*)L.r64.p,
+                                  (const void*)L.t64.p, (const void*)L.u64.p, (const void*)L.w64.p,
+                                  (const void*)L.dinv64.p})
+                s.push_back(p);
+        } else {
+            for (const void* p : {(const void*)L.x32.p, (const void*)L.b32.p, (const void*)L.r32.p,
+                                  (const void*)L.t32.p, (const void*)L.u32.p, (const void*)L.w32.p,
+                                  (const void*)L.dinv32.p})
+                s.push_back(p);
+        }
+        sig_op(s, opA<T>(L)); sig_op(s, opP<T>(L)); sig_op(s, opR<T>(L));
+    }
+    std::sort(s.begin(), s.end());
+    return s;
+}
+
+template <class T>
 void precondition(semo_ctx* c, Amg& amg, const double* r64, double* z64, bool prepared = false) {
     int deg = cheb_degree(c);
     if (!c->graphs || c->prof.on) {
@@ -835,7 +862,8 @@ void precondition(semo_ctx* c, Amg& amg, const double* r64, double* z64, bool pr
         return;
     }
     int prec = sizeof(T) == 4 ? 1 : 0;
-    if (!amg.vg || amg.vg_r != r64 || amg.vg_z != z64 || amg.vg_deg != deg || amg.vg_prec != prec ||
+    std::vector<const void*> sig = vcycle_signature<T>(amg, r64, z64);
+    if (!amg.vg || amg.vg_stream != c->stream || amg.vg_sig != sig || amg.vg_deg != deg || amg.vg_prec != prec ||
         amg.vg_pre != prepared) {
         amg.drop_graph();
         hipGraph_t g = nullptr;
@@ -845,7 +873,12 @@ void precondition(semo_ctx* c, Amg& amg, const double* r64, double* z64, bool pr
         hipError_t e = hipGraphInstantiate(&amg.vg, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         SEMO_HIP(e);
-        amg.vg_r = r64; amg.vg_z = z64; amg.vg_deg = deg; amg.vg_prec = prec; amg.vg_pre = prepared;
+        amg.vg_stream = c->stream; amg.vg_deg = deg; amg.vg_prec = prec; amg.vg_pre = prepared;
+        // the capture swapped some levels' x buffers (out-of-place smoothing
+        // steps): the graph holds the addresses from before the swaps, the
+        // same set
+        amg.vg_sig = vcycle_signature<T>(amg, r64, z64);
+        SEMO_CHECK(amg.vg_sig == sig, SEMO_EHIP, "V-cycle capture changed its buffer set");
     }
     SEMO_HIP(hipGraphLaunch(amg.vg, c->stream));
 }
@@ -1057,11 +1090,11 @@ void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const 
     double rr[3];
     SEMO_HIP(hipMemcpyAsync(rr, s + Q_RR, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SEMO_HIP(hipStreamSynchronize(c->stream));
-    if (st)
-        for (int k = 0; k < 3; ++k) {
-            st->iters[k] = bb[k] > 0.0 ? it : 0;
-            st->rel_res[k] = bb[k] > 0.0 ? std::sqrt(rr[k] / bb[k]) : 0.0;
-        }
+    for (int k = 0; k < 3; ++k) {
+        double rel = bb[k] > 0.0 ? std::sqrt(rr[k] / bb[k]) : 0.0;
+        if (st) { st->iters[k] = bb[k] > 0.0 ? it : 0; st->rel_res[k] = rel; }
+        check_converged(std::isfinite(bb[k]) ? rel : bb[k], tol, "momentum");
+    }
 }
 
 // level-0 operator of the hierarchy refreshed from the assembled system
@@ -1225,8 +1258,7 @@ void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* s
     if (st) {
         st->iters[0] = it; st->rel_res[0] = rr; st->n_levels = (int)amg.lv.size(); st->setup_ms = amg.setup_ms;
     }
-    SEMO_CHECK(rr <= std::max(c->sp.tol_p, 1e-15) * 10.0 || it < c->sp.maxit_p, SEMO_ENOCONV,
-               "pressure solve did not converge");
+    check_converged(rr, c->sp.tol_p, "pressure");
 }
 
 void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st) {
@@ -1244,6 +1276,7 @@ void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st) {
         int it = bicgstab(c, c->Amat, dinv.p, c->Bvec.p + (size_t)k * n, c->Xvec.p + (size_t)k * n, c->sp.tol_u,
                           c->sp.maxit_u, &rr);
         if (st) { st->iters[k] = it; st->rel_res[k] = rr; }
+        check_converged(rr, c->sp.tol_u, "momentum/volume-fraction");
     }
 }
 
@@ -1251,8 +1284,12 @@ void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st) {
 void solve_coo(semo_ctx* c, const char* eq, int n, int64_t nnz, const int* rows, const int* cols,
                const double* vals, int nrhs, const double* b, double* x, semo_solve_stats* st) {
     SEMO_CHECK(n > 0 && nnz >= n && nrhs >= 1 && nrhs <= 4, SEMO_EINVAL, "solve_coo sizes");
-    for (int64_t k = 0; k < nnz; ++k)
+    for (int64_t k = 0; k < nnz; ++k) {
         SEMO_CHECK(rows[k] >= 0 && rows[k] < n && cols[k] >= 0 && cols[k] < n, SEMO_EINVAL, "COO index out of range");
+        SEMO_CHECK(std::isfinite(vals[k]), SEMO_EINVAL, "COO value is not finite");
+    }
+    for (int64_t k = 0; k < (int64_t)n * nrhs; ++k)
+        SEMO_CHECK(std::isfinite(b[k]), SEMO_EINVAL, "right-hand side is not finite");
     // CSR with summed duplicates, sorted columns
     std::vector<int> ptr(n + 1, 0);
     for (int64_t k = 0; k < nnz; ++k) ptr[rows[k] + 1]++;
@@ -1301,7 +1338,10 @@ void solve_coo(semo_ctx* c, const char* eq, int n, int64_t nnz, const int* rows,
             int it = fpcg(c, amg, A, db.p + (size_t)k * n, dx.p + (size_t)k * n, c->sp.tol_p, c->sp.maxit_p,
                           c->sp.precision == SEMO_PREC_MIXED, &rr);
             if (st) { st->iters[k] = it; st->rel_res[k] = rr; st->n_levels = (int)amg.lv.size(); st->setup_ms = amg.setup_ms; }
+            check_converged(rr, c->sp.tol_p, "pressure");
         }
+        // the local hierarchy and its graph go out of scope here
+        SEMO_HIP(hipStreamSynchronize(c->stream));
     } else {
         DBuf<double> dinv; dinv.alloc(n + 64);
         hipLaunchKernelGGL(k_dinv, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, A.diag.p, dinv.p);
@@ -1309,6 +1349,7 @@ void solve_coo(semo_ctx* c, const char* eq, int n, int64_t nnz, const int* rows,
             double rr = 0;
             int it = bicgstab(c, A, dinv.p, db.p + (size_t)k * n, dx.p + (size_t)k * n, c->sp.tol_u, c->sp.maxit_u, &rr);
             if (st) { st->iters[k] = it; st->rel_res[k] = rr; }
+            check_converged(rr, c->sp.tol_u, "solve");
         }
     }
     dx.download(x, (size_t)n * nrhs, c->stream);

@@ -332,3 +332,22 @@ def tet_mesh(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
                 cell_pts=tets.astype(np.int32).ravel(),
                 patch_names=names, patch_start=np.array(starts, dtype=np.int32),
                 patch_nfaces=np.array(counts, dtype=np.int32), shape=(nx, ny, nz))
+
+
+def rotated(mesh: Mesh, seed: int, shift=(0.0, 0.0, 0.0)) -> Mesh:
+    """The mesh turned by a seeded proper rotation about the origin, then
+    shifted: every face normal of a box mesh gets three non-zero components,
+    and the topology, point orders and patches are unchanged (a proper
+    rotation keeps every face's right-hand normal pointing owner ->
+    neighbour).  The rotation is the Q factor of a seeded Gaussian matrix with
+    positive diagonal of R and det +1."""
+    a = np.random.default_rng(seed).standard_normal((3, 3))
+    q, r = np.linalg.qr(a)
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0.0:
+        q[:, 0] = -q[:, 0]
+    pts = np.asarray(mesh.points, dtype=np.float64) @ q.T + np.asarray(shift, dtype=np.float64)
+    return Mesh(points=pts, owner=mesh.owner, neighbour=mesh.neighbour, face_ptr=mesh.face_ptr,
+                face_pts=mesh.face_pts, cell_ptr=mesh.cell_ptr, cell_pts=mesh.cell_pts,
+                patch_names=list(mesh.patch_names), patch_start=mesh.patch_start,
+                patch_nfaces=mesh.patch_nfaces, shape=mesh.shape)

@@ -74,6 +74,10 @@ class GpuSolver:
             bcs[i] = _lib.PatchBC(b.p_type, b.p_value, b.u_type, (C.c_double * 3)(*b.u_value),
                                   b.vf_type, b.vf_value)
         self._ok(self._L.semo_set_case(self._h, C.byref(cs), bcs))
+        pts = getattr(case, "vf_points", None)
+        if pts is not None:
+            a = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 4)
+            self._ok(self._L.semo_set_vf_function_points(self._h, a.shape[0], _ptr(a)))
 
     def set_solver(self, precision="fp64", tol_p=1e-8, tol_u=1e-8, maxit_p=500, maxit_u=500,
                    cheb_degree=1, coarse_size=1000):
@@ -262,6 +266,17 @@ class GpuSolver:
         fr = np.ascontiguousarray(face_r, dtype=np.float64)
         g = np.empty(3 * self.mesh.n_cells)
         self._ok(self._L.semo_least_square(self._h, _ptr(phi), _ptr(fr), _ptr(g)))
+        return g.reshape(-1, 3)
+
+    def calcGaussGreen(self, field, phi, face_r, gradient):
+        """calcGaussGreen(mesh, cn, fn, gradient): field names cn ("P", "U",
+        "V", "W" or "VF", whose patch types give the boundary values), phi the
+        cell values, face_r the right face values, gradient the input
+        gradient (skewness correction, extrapolation); returns the new one."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        fr = np.ascontiguousarray(face_r, dtype=np.float64)
+        g = np.ascontiguousarray(gradient, dtype=np.float64).reshape(-1).copy()
+        self._ok(self._L.semo_gauss_green(self._h, F[field], _ptr(phi), _ptr(fr), _ptr(g)))
         return g.reshape(-1, 3)
 
     def setCellVarMinMax(self, phi):

@@ -22,8 +22,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from semo_mesh_amd.cases import cavity_case, dam_break_case  # noqa: E402
-from semo_mesh_amd.mesh import cavity_mesh, hex_mesh, tet_mesh  # noqa: E402
+from semo_mesh_amd.cases import (BC_FIXED_VALUE, BC_FUNCTION, BC_INLET_OUTLET, BC_NO_SLIP,  # noqa: E402
+                                  BC_SLIP, BC_SURFACE_NORMAL_FIXED_VALUE, BC_SWITCH, BC_ZERO_GRADIENT,
+                                  PatchBC, cavity_case, dam_break_case)
+from semo_mesh_amd.mesh import cavity_mesh, hex_mesh, rotated, tet_mesh  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
 
@@ -40,9 +42,11 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
             np.ascontiguousarray(init, dtype=np.float64).tofile(ip)
         case.write_ref_case(os.path.join(d, "case.txt"), steps=steps,
                             correctors=correctors, init=ip)
+        if case.vf_points is not None:        # read as ./VF.csv (reconIncom.cpp:931)
+            case.write_vf_csv(os.path.join(d, "VF.csv"))
         out = os.path.join(d, "out")
         r = subprocess.run([REF, os.path.join(d, "mesh.raw"), os.path.join(d, "case.txt"), out],
-                           capture_output=True, text=True, timeout=600)
+                           capture_output=True, text=True, timeout=600, cwd=d)
         if "SEMO_REF_DONE" not in r.stdout:
             raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
         arrs = {}
@@ -51,6 +55,8 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
             if ext not in (".f64", ".i32"):
                 continue
             if base.startswith("call") and not keep_calls:
+                continue
+            if base.startswith("gg_") and not name.startswith("bc_"):
                 continue
             if base.endswith("_gradVF"):
                 continue
@@ -70,7 +76,15 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
         correctors=np.array(correctors), case_g=np.array(case.g),
         case_n_vof=np.array(case.n_vof), case_urf_vf=np.array(case.urf_vf),
         case_adjust_dt=np.array(int(case.adjust_dt)), case_max_co=np.array(case.max_co),
-        case_max_dt=np.array(case.max_dt), case_sigma=np.array(case.sigma)))
+        case_max_dt=np.array(case.max_dt), case_sigma=np.array(case.sigma),
+        # the patch table: (p, u, vf) types and (p, u0, u1, u2, vf) values
+        case_bc_types=np.array([[case.bcs[nm].p_type, case.bcs[nm].u_type, case.bcs[nm].vf_type]
+                                for nm in mesh.patch_names], dtype=np.int32),
+        case_bc_values=np.array([[case.bcs[nm].p_value, *case.bcs[nm].u_value, case.bcs[nm].vf_value]
+                                 for nm in mesh.patch_names], dtype=np.float64),
+        case_urf=np.array([case.urf_mom, case.urf_p, case.urf_pu])))
+    if case.vf_points is not None:
+        arrs["case_vf_points"] = np.asarray(case.vf_points, dtype=np.float64)
     if init is not None:
         arrs["init"] = np.asarray(init)
     path = os.path.join(HERE, name + ".npz")
@@ -155,9 +169,51 @@ def _sigma_box():
     run_case("sigma_box766_2x1", m, c, 2, 1, init)
 
 
+def _bc_open():
+    # the velocity boundaries slip, surfaceNormalFixedValue and inletOutlet
+    # (reconIncom.cpp:751-852, eqMomentum.cpp:841-864) and the Mach-switched
+    # pressure outlet (reconIncom.cpp:618-631, eqPressure.cpp:625-632) on a
+    # jittered box; rho = 1 gives the constant-density sound speed C = 1, so
+    # the random velocities put boundary cells on both sides of Mach 1.  The
+    # box is rotated (every boundary normal has three non-zero components)
+    # and shifted to x > 1, clear of the orifice window of
+    # reconIncom.cpp:707-727.
+    m = rotated(hex_mesh(6, 5, 4, jitter=0.1, seed=41), seed=42, shift=(3.0, 0.0, 0.0))
+    c = cavity_case(); c.rho = (1.0, 1.0); c.mu = (0.05, 0.05); c.dt = 0.02
+    c.bcs["xmin"] = PatchBC(u_type=BC_INLET_OUTLET, u_value=(-0.8, 0.0, 0.0))
+    c.bcs["xmax"] = PatchBC(p_type=BC_SWITCH, p_value=0.5, u_type=BC_ZERO_GRADIENT)
+    c.bcs["ymin"] = PatchBC(u_type=BC_SLIP)
+    c.bcs["ymax"] = PatchBC(u_type=BC_SURFACE_NORMAL_FIXED_VALUE, u_value=(0.3, 0.0, 0.0))
+    run_case("bc_open_box654_2x2", m, c, 2, 2, random_init(m.n_cells, 5))
+
+
+def _bc_vof():
+    # two-phase: the volume-fraction "function" boundary (nearest point of
+    # ./VF.csv, reconIncom.cpp:929-1019; eqVolfrac.cpp:663-665), inletOutlet
+    # U and VF, a Mach-switched pressure boundary whose sound speed follows
+    # the mixture (air cells: C ~ 1.1), slip and surfaceNormalFixedValue walls
+    m = hex_mesh(6, 5, 4, jitter=0.1, seed=43, ref_face=False)
+    c = dam_break_case(dt=2e-3, n_vof=1, urf_vf=0.8)
+    c.bcs["xmin"] = PatchBC(u_type=BC_INLET_OUTLET, u_value=(-0.4, 0.0, 0.0),
+                            vf_type=BC_FUNCTION)
+    c.bcs["xmax"] = PatchBC(p_type=BC_SWITCH, p_value=0.0, u_type=BC_ZERO_GRADIENT,
+                            vf_type=BC_INLET_OUTLET, vf_value=0.0)
+    c.bcs["ymin"] = PatchBC(u_type=BC_SLIP)
+    c.bcs["ymax"] = PatchBC(u_type=BC_SURFACE_NORMAL_FIXED_VALUE, u_value=(0.0, 0.0, 0.0),
+                            vf_type=BC_FUNCTION)
+    rng = np.random.default_rng(44)
+    pts = np.column_stack([rng.random(40), rng.random((40, 3))])
+    c.vf_points = pts
+    init = vof_init(m, 45)
+    nc = m.n_cells
+    init[nc:4 * nc] *= 15.0            # speeds of O(1): some air cells above Mach 1
+    run_case("bc_vof_box654_2x1", m, c, 2, 1, init)
+
+
 CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "cavity8_plain": _cavity8, "orifice_lid": _orifice, "vof_box654_2x2": _vof_box654,
-         "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box}
+         "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box,
+         "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof}
 
 
 def main(names):

@@ -149,6 +149,6 @@ def test_coupled_and_simple_share_the_fixed_point():
         fa, fb = a.get_field(f), b.get_field(f)
         rel = np.linalg.norm(fa - fb) / np.linalg.norm(fa)
         print(f, rel)
-        assert rel <= 1e-6, f
+        assert rel <= 1e-9, f
     a.close()
     b.close()

@@ -134,7 +134,8 @@ def _eq_of_call(d, k):
 
 
 @pytest.mark.parametrize("name", ["cavity4_jitter_random", "box654_jitter_2x2", "orifice_lid",
-                                  "vof_box654_2x2", "vof_tet433_2x1", "sigma_box766_2x1"])
+                                  "vof_box654_2x2", "vof_tet433_2x1", "sigma_box766_2x1",
+                                  "bc_open_box654_2x2", "bc_vof_box654_2x1"])
 def test_assembly_replay_vs_reference(name):
     """The reference's own COO systems, replayed from its call-time fields.
     Momentum and pressure systems are bit-exact.  Volume-fraction systems
@@ -193,7 +194,8 @@ def test_assembly_replay_vs_reference(name):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["box654_jitter_2x2", "vof_tet433_2x1"])
+@pytest.mark.parametrize("name", ["box654_jitter_2x2", "vof_tet433_2x1", "bc_open_box654_2x2",
+                                  "bc_vof_box654_2x1"])
 def test_assembly_replay_renumbered(name):
     """The same replay on an RCM-renumbered upload, where faces whose owner
     changes are flipped: every system agrees to round-off (summation orders
@@ -399,4 +401,22 @@ def test_error_paths():
     with pytest.raises(SemoError) as e:
         g.set_case(case)
     assert e.value.code == 2
+    g.close()
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+@pytest.mark.parametrize("name", ["bc_open_box654_2x2", "bc_vof_box654_2x1"])
+def test_gauss_green_vs_reference(name, reorder):
+    """semo_gauss_green (calcGaussGreen, gradient.cpp:1778-1985) for every
+    field against the reference's own call on the fixture's final state:
+    bit-exact on the un-renumbered mesh, round-off after RCM renumbering."""
+    d, mesh = load_golden(name)
+    g = _gpu(mesh, golden_case(d), reorder)
+    for f in ("P", "U", "V", "W", "VF"):
+        a = g.calcGaussGreen(f, d["final_" + f], d["gg_faceR_" + f], d["gg_in_" + f]).ravel()
+        b = d["gg_out_" + f]
+        if reorder:
+            assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), f
+        else:
+            assert np.array_equal(a, b), f
     g.close()

@@ -73,3 +73,32 @@ def test_cavity_step_gradients_match_oracle(oracle_lib):
         o.step(correctors=2)
         _compare(g, o)
     g.close()
+
+
+@pytest.mark.parametrize("name", ["bc_open_box654_2x2", "bc_vof_box654_2x1"])
+def test_u_dependent_boundaries_refresh_reused_gradients(oracle_lib, name):
+    """Boundaries whose face value follows the owner's velocity (P "switch",
+    VF inletOutlet) change between the momentum and the pressure stage when
+    the momentum update moves U across their threshold: the pressure stage
+    must then recompute the P / alpha gradient it would otherwise keep.  The
+    random initial fields of these fixtures move boundary cells across Mach 1
+    and across the in/outflow switch within the first corrector."""
+    from conftest import golden_case, load_golden
+    from semo_mesh_amd.solver import GpuSolver
+    d, m = load_golden(name)
+    case = golden_case(d)
+    nc = m.n_cells
+    g = GpuSolver(0)
+    g.upload_mesh(m, reorder=False)
+    g.set_case(case)
+    g.set_solver("fp64", tol_p=1e-13, tol_u=1e-13, maxit_p=500, maxit_u=500)
+    o = oracle_lib.Oracle(m, case)
+    o.set_solver(tol=1e-14)
+    for k, f in enumerate(("P", "U", "V", "W", "VF")):
+        g.set_field(f, d["init"][k * nc:(k + 1) * nc])
+        o.set(f, d["init"][k * nc:(k + 1) * nc])
+    for _ in range(2):
+        g.incompressiblePressureBased(int(d["correctors"]))
+        o.step(correctors=int(d["correctors"]))
+        _compare(g, o)
+    g.close()

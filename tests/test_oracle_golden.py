@@ -165,3 +165,16 @@ def test_full_simple_run_matches_reference(name, oracle_lib):
         a, b = o.get(f), d["final_" + f]
         scale = np.linalg.norm(b) if f in ("P", "VF") else vel
         assert np.linalg.norm(a - b) <= 1e-7 * scale, f
+
+
+@pytest.mark.parametrize("name", ["bc_open_box654_2x2", "bc_vof_box654_2x1"])
+@pytest.mark.parametrize("field", ["P", "U", "V", "W", "VF"])
+def test_gauss_green_bit_exact(name, field, oracle_lib):
+    """calcGaussGreen (gradient.cpp:1778-1985) for each field on the final
+    state of the boundary-condition fixtures, whose patches carry every type
+    (fixedValue, zeroGradient and the mean-of-cell-and-face branch for noSlip,
+    slip, surfaceNormalFixedValue, inletOutlet, switch, function)."""
+    d, mesh = load_golden(name)
+    o = oracle_lib.Oracle(mesh, golden_case(d))
+    g = o.gauss_green(field, d["final_" + field], d["gg_faceR_" + field], d["gg_in_" + field])
+    assert np.array_equal(g.ravel(), d["gg_out_" + field])
