@@ -416,7 +416,10 @@ def test_gauss_green_vs_reference(name, reorder):
         a = g.calcGaussGreen(f, d["final_" + f], d["gg_faceR_" + f], d["gg_in_" + f]).ravel()
         b = d["gg_out_" + f]
         if reorder:
-            assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), f
+            # round-off of sums of terms of size |phi| A / V ~ |phi| / h, h ~ 0.1 (a
+            # constant field has a zero gradient made of round-off)
+            scale = np.abs(b).max() + np.abs(d["final_" + f]).max() * 10.0
+            assert np.abs(a - b).max() <= 1e-12 * scale, f
         else:
             assert np.array_equal(a, b), f
     g.close()
