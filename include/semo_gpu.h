@@ -238,7 +238,10 @@ int semo_calc_coupled_eq(semo_ctx* ctx, double* norm, semo_solve_stats* st);
 /* One time step with the block-coupled iteration in place of the
  * momentum / pressure correctors: the set-up of
  * semo_incompressible_pressure_based (time levels, dt, VOF, sources), then
- * n_iter semo_calc_coupled_eq iterations; norms[k] per iteration.             */
+ * n_iter semo_calc_coupled_eq iterations; norms[k] per iteration.  Afterwards
+ * semo_get_solve_stats(ctx, 0, .) holds the last block solve, with iters[1]
+ * and iters[2] the sum and the maximum of the Krylov iterations over the
+ * step's outer iterations.                                                    */
 int semo_coupled_pressure_based(semo_ctx* ctx, int n_iter, double* norms);
 
 /* The last assembled block system: block COO in caller numbering (the n

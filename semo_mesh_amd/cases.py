@@ -124,6 +124,24 @@ def channel_case(u_in: float, dt: float, rho: float = 1.177, mu: float = 1.85e-5
     return Case(rho=(rho, rho), mu=(mu, mu), dt=dt, bcs=bcs)
 
 
+def channel_workload(nx: int = 256, ny: int = 64, nz: int = 64):
+    """The block-coupled channel configuration: a 4 x 1 x 1 box of nx x ny x
+    nz jittered hexes (seed 20261204) spanning x in [1, 5] -- clear of the
+    reference's live inlet UDF, which rewrites fixedValue-U faces within
+    1.6e-4 of x = 0.0006 k, k = 0..23 (reconIncom.cpp:707-727) -- air
+    (rho 1.177, mu 1.85e-5), inlet U = Mach 0.3 at 300 K (gamma 1.4) with a
+    seeded sinusoidal perturbation of 2 % amplitude, outlet p = 0, noSlip
+    walls, dt at Courant 1 on the inlet velocity, under-relaxation 0.7 (u)
+    and 0.3 (p).  Returns (mesh, case, u_in, per-face inlet values)."""
+    from .mesh import hex_mesh
+    m = hex_mesh(nx, ny, nz, lo=(1.0, 0.0, 0.0), hi=(5.0, 1.0, 1.0), jitter=0.1, seed=20261204,
+                 ref_face=False)
+    u_in = 0.3 * (1.4 * 287.05 * 300.0) ** 0.5
+    case = channel_case(u_in, dt=(4.0 / nx) / u_in)
+    case.urf_mom, case.urf_p, case.urf_pu = 0.7, 0.3, 0.3
+    return m, case, u_in, inlet_profile(m, "xmin", u_in, amplitude=0.02)
+
+
 def inlet_profile(mesh, patch: str, u_in: float, amplitude: float = 0.02, seed: int = 20261206):
     """Per-face fixedValue-U values ([3][n_boundary_faces], for
     GpuSolver.set_boundary_velocity) with a seeded sinusoidal perturbation

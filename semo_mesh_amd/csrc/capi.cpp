@@ -587,13 +587,18 @@ int semo_coupled_pressure_based(semo_ctx* c, int n_iter, double* norms) {
     return guard(c, [&] {
         need_case(c);
         simple_setup(c);
+        int sum = 0, mx = 0;
         for (int it = 0; it < n_iter; ++it) {
             recon_boundary(c);
             assemble_coupled(c);
             coupled_solve_assembled(c, &c->last_stats_mom);
+            sum += c->last_stats_mom.iters[0];
+            mx = std::max(mx, c->last_stats_mom.iters[0]);
             double nv = coupled_update(c);
             if (norms) norms[it] = nv;
         }
+        c->last_stats_mom.iters[1] = sum;
+        c->last_stats_mom.iters[2] = mx;
         c->iterReal++;
     });
 }

@@ -433,6 +433,10 @@ void reduction_alloc(semo_ctx* c, long n);
 // amg_setup.cpp / solver.hip
 void amg_build(semo_ctx* c, Amg& amg, const DevSell<double>& A);
 void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* st);
+// pressure hierarchy for c->Amat (build / device refresh of level 0 while
+// keep_coarse(amg)), and one V-cycle of it
+void pressure_amg_update(semo_ctx* c, bool (*keep_coarse)(const Amg&));
+void pressure_vcycle(semo_ctx* c, const double* r64, double* z64, bool fp32);
 void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st);
 std::vector<std::vector<int>> aggregate_hierarchy(int n, const std::vector<int>& ptr, const std::vector<int>& col,
                                                   const std::vector<double>& val, int min_coarse);

@@ -1200,8 +1200,10 @@ void reduction_alloc(semo_ctx* c, long n) {
 }
 
 // ----------------------------------------------------- equation solves ----
-// pressure: hierarchy (re)built when the operator's values changed
-void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* st) {
+// The pressure hierarchy for the operator in c->Amat: built when missing;
+// when the values changed, the finest level is refreshed on the device while
+// keep_coarse says the coarse levels still serve, else rebuilt.
+void pressure_amg_update(semo_ctx* c, bool (*keep_coarse)(const Amg&)) {
     Amg& amg = c->amg_p;
     bool rebuild = !amg.built;
     if (amg.built) {
@@ -1225,8 +1227,7 @@ void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* s
     // The outer Krylov method always uses the current operator, so this only
     // affects the preconditioner's quality, never the solution.
     bool refresh = false;
-    if (rebuild && amg.built && amg.l0_map.n == amg.lv[0].A64.val.n && amg.l0_map.n > 0 && amg.iters_ref >= 0 &&
-        amg.last_iters <= std::max((3 * amg.iters_ref) / 2, amg.iters_ref + 3)) {
+    if (rebuild && amg.built && amg.l0_map.n == amg.lv[0].A64.val.n && amg.l0_map.n > 0 && keep_coarse(amg)) {
         refresh = true;
         rebuild = false;
     }
@@ -1250,6 +1251,26 @@ void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* s
         SEMO_HIP(hipMemcpyAsync(c->amg_key_diag.p, c->Amat.diag.p, c->Amat.diag.bytes(), hipMemcpyDeviceToDevice, c->stream));
         amg.fresh = true;
     }
+}
+
+// z = M^-1 r: one V-cycle of the pressure hierarchy, FP32 (fp32) or FP64,
+// FP64 in and out (the Schur-complement step of the coupled solver)
+void pressure_vcycle(semo_ctx* c, const double* r64, double* z64, bool fp32) {
+    SEMO_CHECK(c->amg_p.built, SEMO_ESTATE, "pressure hierarchy not built");
+    if (fp32) precondition<float>(c, c->amg_p, r64, z64, false);
+    else precondition<double>(c, c->amg_p, r64, z64, false);
+}
+
+// flexible CG: the coarse levels serve while the iteration count stays
+// within 1.5x (+3) of the count right after the last full set-up
+bool fcg_keeps_coarse(const Amg& amg) {
+    return amg.iters_ref >= 0 && amg.last_iters <= std::max((3 * amg.iters_ref) / 2, amg.iters_ref + 3);
+}
+
+// pressure: hierarchy (re)built when the operator's values changed
+void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* st) {
+    Amg& amg = c->amg_p;
+    pressure_amg_update(c, fcg_keeps_coarse);
     reduction_alloc(c, c->nc);
     double rr = 0;
     int it = fpcg(c, amg, c->Amat, b, x, c->sp.tol_p, c->sp.maxit_p, c->sp.precision == SEMO_PREC_MIXED, &rr);
