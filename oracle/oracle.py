@@ -195,3 +195,93 @@ def genrcm(ptr, adj):
     lib().orc_genrcm(n, np.ascontiguousarray(ptr, dtype=np.int32),
                      np.ascontiguousarray(adj, dtype=np.int32), perm, inv)
     return perm, inv
+
+
+_LIB_AD = None
+
+
+def lib_ad():
+    """liborc_ad.so: the oracle compiled with the taped reverse-mode type
+    (the CPU adjoint)."""
+    global _LIB_AD
+    if _LIB_AD is None:
+        path = os.path.join(_HERE, "liborc_ad.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liborc_ad.so is not built (make -f oracle/Makefile)")
+        L = C.CDLL(path)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int] * 4 + [_D, _I, _I, _I, _I, _I, _I, C.c_int, _I, _I]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_bc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _D, C.c_int, C.c_double]
+        L.orc_set_case.argtypes = [C.c_void_p, _D, _D, C.c_double, _D, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double]
+        L.orc_set_solver.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.orc_ad_set.argtypes = [C.c_void_p, C.c_char_p, _D]
+        L.orc_ad_get.argtypes = [C.c_void_p, C.c_char_p, _D]
+        L.orc_ad_forward.restype = C.c_double
+        L.orc_ad_forward.argtypes = [C.c_void_p, C.c_int, _D]
+        L.orc_ad_adjoint.restype = C.c_double
+        L.orc_ad_adjoint.argtypes = [C.c_void_p, C.c_int, _D, _D]
+        _LIB_AD = L
+    return _LIB_AD
+
+
+class AdjointOracle:
+    """The CPU adjoint: one time step of n_iter SIMPLE outer iterations of a
+    single-phase case with the Darcy term alpha_p (momentum diagonal +=
+    alpha_p V, right sides -= alpha_p V U), its objective J = sum_patches
+    w_p sum_f A_f p_f, and dJ/dalpha_p by reverse-mode differentiation of the
+    oracle's own assembly (ad.hpp)."""
+
+    def __init__(self, mesh, case):
+        L = lib_ad()
+        self.mesh = mesh
+        self._h = L.orc_create(mesh.n_points, mesh.n_cells, mesh.n_faces, mesh.n_internal_faces,
+                               np.ascontiguousarray(mesh.points, dtype=np.float64),
+                               np.ascontiguousarray(mesh.owner, dtype=np.int32),
+                               np.ascontiguousarray(mesh.neighbour, dtype=np.int32),
+                               np.ascontiguousarray(mesh.face_ptr, dtype=np.int32),
+                               np.ascontiguousarray(mesh.face_pts, dtype=np.int32),
+                               np.ascontiguousarray(mesh.cell_ptr, dtype=np.int32),
+                               np.ascontiguousarray(mesh.cell_pts, dtype=np.int32),
+                               len(mesh.patch_names),
+                               np.ascontiguousarray(mesh.patch_start, dtype=np.int32),
+                               np.ascontiguousarray(mesh.patch_nfaces, dtype=np.int32))
+        L.orc_set_case(self._h, np.array(case.rho, dtype=np.float64), np.array(case.mu, dtype=np.float64),
+                       case.sigma, np.array(case.g, dtype=np.float64), case.dt, case.urf_mom,
+                       case.urf_p, case.urf_pu, case.limit)
+        for i, nm in enumerate(mesh.patch_names):
+            b = case.bcs[nm]
+            L.orc_set_bc(self._h, i, b.p_type, b.p_value, b.u_type,
+                         np.array(b.u_value, dtype=np.float64), b.vf_type, b.vf_value)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib_ad().orc_destroy(self._h)
+            self._h = None
+
+    def set_solver(self, tol=1e-14, maxit=20000):
+        lib_ad().orc_set_solver(self._h, tol, maxit)
+
+    def set(self, name, arr):
+        if lib_ad().orc_ad_set(self._h, name.encode(), np.ascontiguousarray(arr, dtype=np.float64)) != 0:
+            raise KeyError(name)
+
+    def get(self, name):
+        out = np.empty(self.mesh.n_cells)
+        if lib_ad().orc_ad_get(self._h, name.encode(), out) != 0:
+            raise KeyError(name)
+        return out
+
+    def _w(self, weights):
+        return np.array([float(weights.get(nm, 0.0)) for nm in self.mesh.patch_names])
+
+    def forward(self, n_iter, weights):
+        """J after one time step of n_iter outer iterations (the state advances)."""
+        return lib_ad().orc_ad_forward(self._h, n_iter, self._w(weights))
+
+    def adjoint(self, n_iter, weights):
+        """(J, dJ/dalpha_p) of one time step of n_iter outer iterations."""
+        g = np.empty(self.mesh.n_cells)
+        J = lib_ad().orc_ad_adjoint(self._h, n_iter, self._w(weights), g)
+        return J, g

@@ -20,12 +20,28 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <string>
 #include <vector>
 
 using std::vector;
 
+// The arithmetic type of everything that depends on the flow state: double,
+// or -- in the CPU-adjoint build (-DORC_AD, liborc_ad.so) -- the taped
+// reverse-mode type of ad.hpp, which evaluates the same IEEE operations.
+#ifdef ORC_AD
+#include "ad.hpp"
+using Real = ad::adouble;
+using ad::val;
+#else
+using Real = double;
+#endif
+inline double val(double x) { return x; }
+
 namespace {
+
+// std::max(0.0, std::min(1.0, a)) of the reference's clipping
+inline Real clamp01(const Real& a) { return std::max(Real(0.0), std::min(Real(1.0), a)); }
 
 enum BcType { ZERO_GRADIENT = 0, FIXED_VALUE = 1, NO_SLIP = 2, SLIP = 3,
               SURFACE_NORMAL_FIXED_VALUE = 4, INLET_OUTLET = 5,
@@ -65,12 +81,12 @@ struct Oracle {
     int iterReal = 0;
 
     // ---- fields (cell) -------------------------------------------------------
-    vector<double> P, U, V, W, VF, MF, Rho, Mu;
-    vector<double> oldP, oldU, oldV, oldW, oldVF, oldRho, Qm0, Qm1, Qm2, Qm3;
-    vector<double> srcG[4], srcS[4], kappa, LS;
+    vector<Real> P, U, V, W, VF, MF, Rho, Mu;
+    vector<Real> oldP, oldU, oldV, oldW, oldVF, oldRho, Qm0, Qm1, Qm2, Qm3;
+    vector<Real> srcG[4], srcS[4], kappa, LS;
     // Darcy porosity alpha_p of the adjoint harness (no counterpart in the
     // reference): momentum diagonal += alpha_p V, right sides -= alpha_p V U
-    vector<double> alphaP;
+    vector<Real> alphaP;
     // per-face fixedValue-U values, [3][nbf] over the boundary faces in face
     // order (an inlet profile; the reference's form of one is a UDF,
     // reconIncom.cpp:707-727); used once set (ufix_on)
@@ -79,20 +95,20 @@ struct Oracle {
     // cell speed of sound of the mixture EOS, set once at the first step the
     // way calcCellEOSVF sets it at load (eos.cpp:160-293); read by the P
     // "switch" boundary (reconIncom.cpp:618-631, eqPressure.cpp:625-632)
-    vector<double> Csnd;
+    vector<Real> Csnd;
     // VF "function" boundary values per boundary face (reconIncom.cpp:929-
     // 1019): the VF of the nearest point of the set given by orc_set_vf_points
     vector<double> vffun;
-    vector<double> gP, gU, gV, gW, gVF;    // [3n]
-    vector<double> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
+    vector<Real> gP, gU, gV, gW, gVF;    // [3n]
+    vector<Real> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
     // face left/right values (src/solvers/reconIncom.cpp:501-1035)
-    vector<double> LP, RP, LU, RU, LV, RV, LW, RW, LVF, RVF, Lmu, Rmu, LRho, RRho;
+    vector<Real> LP, RP, LU, RU, LV, RV, LW, RW, LVF, RVF, Lmu, Rmu, LRho, RRho;
     vector<double> co_cell;            // per-cell Courant number of the last MSTACS pass
     double vof_norm = 0.0;             // return value of the last calcVolfracEq
 
     // ---- last assembled system -------------------------------------------------
     vector<int> Arows, Acols;
-    vector<double> Avals, B0, B1, B2, X0, X1, X2;
+    vector<Real> Avals, B0, B1, B2, X0, X1, X2;
     int last_iters = 0; double last_res = 0;
     double tol = 1e-13; int maxit = 5000;
 };
@@ -291,22 +307,22 @@ void build_geometry(Oracle& o) {
 // ------------------------------------------------------------------------
 // calcLeastSquare(mesh,"cellVertex","1st","cell",cn,fn,...) (gradient.cpp:593-1159)
 // phi: cell values; bR: face right value at boundary faces (varR[fn]).
-void lsq_grad(const Oracle& o, const vector<double>& phi, const vector<double>& faceR,
-              vector<double>& grad) {
+void lsq_grad(const Oracle& o, const vector<Real>& phi, const vector<Real>& faceR,
+              vector<Real>& grad) {
     int nc = o.nc;
-    vector<double> gt(3 * (size_t)nc, 0.0);
+    vector<Real> gt(3 * (size_t)nc, 0.0);
     for (int i = 0; i < nc; ++i)
         for (int k = o.sptr[i]; k < o.sptr[i + 1]; ++k) {
             int j = o.sidx[k];
             double dx = o.cxyz[3 * j] - o.cxyz[3 * i], dy = o.cxyz[3 * j + 1] - o.cxyz[3 * i + 1],
                    dz = o.cxyz[3 * j + 2] - o.cxyz[3 * i + 2];
             double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
-            double d = phi[j] - phi[i];
+            Real d = phi[j] - phi[i];
             gt[3 * i] += wk * dx * d; gt[3 * i + 1] += wk * dy * d; gt[3 * i + 2] += wk * dz * d;
         }
     for (int f = o.nif; f < o.nf; ++f) {
         int i = o.owner[f];
-        double d = faceR[f] - phi[i];
+        Real d = faceR[f] - phi[i];
         double dx = o.fxyz[3 * f] - o.cxyz[3 * i], dy = o.fxyz[3 * f + 1] - o.cxyz[3 * i + 1],
                dz = o.fxyz[3 * f + 2] - o.cxyz[3 * i + 2];
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
@@ -315,19 +331,19 @@ void lsq_grad(const Oracle& o, const vector<double>& phi, const vector<double>& 
     grad.assign(3 * (size_t)nc, 0.0);
     for (int i = 0; i < nc; ++i) {
         const double* c = &o.lsq[6 * (size_t)i];
-        const double* t = &gt[3 * (size_t)i];
-        double r0 = 0.0; r0 += c[0] * t[0]; r0 += c[1] * t[1]; r0 += c[2] * t[2];
-        double r1 = 0.0; r1 += c[1] * t[0]; r1 += c[3] * t[1]; r1 += c[4] * t[2];
-        double r2 = 0.0; r2 += c[2] * t[0]; r2 += c[4] * t[1]; r2 += c[5] * t[2];
+        const Real* t = &gt[3 * (size_t)i];
+        Real r0 = 0.0; r0 += c[0] * t[0]; r0 += c[1] * t[1]; r0 += c[2] * t[2];
+        Real r1 = 0.0; r1 += c[1] * t[0]; r1 += c[3] * t[1]; r1 += c[4] * t[2];
+        Real r2 = 0.0; r2 += c[2] * t[0]; r2 += c[4] * t[1]; r2 += c[5] * t[2];
         grad[3 * i] = r0; grad[3 * i + 1] = r1; grad[3 * i + 2] = r2;
     }
 }
 
 // setCellVarMinMax (src/solvers/cellVarMinMax.cpp:8-24)
-void minmax(const Oracle& o, const vector<double>& phi, vector<double>& mx, vector<double>& mn) {
-    mx.assign(o.nc, 0); mn.assign(o.nc, 0);
+void minmax(const Oracle& o, const vector<Real>& phi, vector<Real>& mx, vector<Real>& mn) {
+    mx.assign(o.nc, 0.0); mn.assign(o.nc, 0.0);
     for (int i = 0; i < o.nc; ++i) {
-        double a = phi[i], b = phi[i];
+        Real a = phi[i], b = phi[i];
         for (int k = o.sptr[i]; k < o.sptr[i + 1]; ++k) {
             a = std::max(a, phi[o.sidx[k]]); b = std::min(b, phi[o.sidx[k]]);
         }
@@ -338,18 +354,18 @@ void minmax(const Oracle& o, const vector<double>& phi, vector<double>& mx, vect
 // calcIncomCellEOSVF (eos.cpp:10-151) + calcCellTransport (transport.cpp:9-98)
 void cell_eos_transport(Oracle& o) {
     for (int i = 0; i < o.nc; ++i) {
-        double a0 = std::max(0.0, std::min(1.0, o.VF[i]));
-        double v0 = std::max(0.0, std::min(1.0, a0));
-        double v1 = std::max(0.0, std::min(1.0, 1.0 - a0));
-        double r = 0.0; r += v0 * o.rho[0]; r += v1 * o.rho[1];
+        Real a0 = clamp01(o.VF[i]);
+        Real v0 = clamp01(a0);
+        Real v1 = clamp01(1.0 - a0);
+        Real r = 0.0; r += v0 * o.rho[0]; r += v1 * o.rho[1];
         o.Rho[i] = r;
-        double m0 = std::max(0.0, std::min(1.0, v0 * o.rho[0] / r));
+        Real m0 = clamp01(v0 * o.rho[0] / r);
         o.MF[i] = m0;
         o.VF[i] = v0;
     }
     for (int i = 0; i < o.nc; ++i) {
-        double v0 = o.VF[i], v1 = 1.0 - o.VF[i];
-        double m = 0.0; m += v0 * o.mu[0]; m += v1 * o.mu[1];
+        Real v0 = o.VF[i], v1 = 1.0 - o.VF[i];
+        Real m = 0.0; m += v0 * o.mu[0]; m += v1 * o.mu[1];
         o.Mu[i] = m + 0.0;
     }
 }
@@ -361,26 +377,26 @@ void cell_eos_transport(Oracle& o) {
 void cell_sound_speed(Oracle& o) {
     o.Csnd.assign(o.nc, 0.0);
     for (int i = 0; i < o.nc; ++i) {
-        double a = std::max(0.0, std::min(1.0, o.VF[i]));
-        double vf[2] = {std::max(0.0, std::min(1.0, a)), 0.0};
-        vf[1] = std::max(0.0, std::min(1.0, 1.0 - a));
-        double rho = 0.0;
+        Real a = clamp01(o.VF[i]);
+        Real vf[2] = {clamp01(a), 0.0};
+        vf[1] = clamp01(1.0 - a);
+        Real rho = 0.0;
         for (int k = 0; k < 2; ++k) rho += vf[k] * o.rho[k];
-        double mf[2];
-        mf[0] = std::max(0.0, std::min(1.0, vf[0] * o.rho[0] / rho));
-        mf[1] = std::max(0.0, std::min(1.0, 1.0 - mf[0]));
-        double drdp = 0.0, drdt = 0.0, dhdp = 0.0, dhdt = 0.0;
+        Real mf[2];
+        mf[0] = clamp01(vf[0] * o.rho[0] / rho);
+        mf[1] = clamp01(1.0 - mf[0]);
+        Real drdp = 0.0, drdt = 0.0, dhdp = 0.0, dhdt = 0.0;
         for (int k = 0; k < 2; ++k) {
             drdp += vf[k] * 1.0e+8; drdt += vf[k] * 1.0e+8;
             dhdp += mf[k] * 1.0e+8; dhdt += mf[k] * 1.0e+8;
         }
-        double c2 = drdp + 1.0 / rho * drdt / dhdt * (1.0 - rho * dhdp);
+        Real c2 = drdp + 1.0 / rho * drdt / dhdt * (1.0 - rho * dhdp);
         o.Csnd[i] = sqrt(1.0 / c2);
     }
 }
 
 // cell Mach number of the P "switch" boundary (reconIncom.cpp:620-625)
-double mach_of(const Oracle& o, int c) {
+Real mach_of(const Oracle& o, int c) {
     return sqrt(pow(o.U[c], 2.0) + pow(o.V[c], 2.0) + pow(o.W[c], 2.0)) / o.Csnd[c];
 }
 
@@ -389,7 +405,7 @@ double mach_of(const Oracle& o, int c) {
 void recon_faces(Oracle& o) {
     int nf = o.nf;
     if ((int)o.Csnd.size() != o.nc) cell_sound_speed(o);   // before the first step
-    auto sz = [&](vector<double>& v) { v.assign(nf, 0.0); };
+    auto sz = [&](vector<Real>& v) { v.assign(nf, 0.0); };
     sz(o.LP); sz(o.RP); sz(o.LU); sz(o.RU); sz(o.LV); sz(o.RV); sz(o.LW); sz(o.RW);
     sz(o.LVF); sz(o.RVF); sz(o.Lmu); sz(o.Rmu); sz(o.LRho); sz(o.RRho);
     for (int f = 0; f < o.nif; ++f) {
@@ -405,7 +421,7 @@ void recon_faces(Oracle& o) {
             const double* n = &o.fnrm[3 * f];
             // pressure (reconIncom.cpp:600-632)
             if (pb.p_type == FIXED_VALUE) o.LP[f] = pb.p_val;
-            else if (pb.p_type == SWITCH) o.LP[f] = mach_of(o, c) > 1.0 ? o.P[c] : pb.p_val;
+            else if (pb.p_type == SWITCH) o.LP[f] = mach_of(o, c) > 1.0 ? o.P[c] : Real(pb.p_val);
             else o.LP[f] = o.P[c];
             o.RP[f] = o.LP[f];
             // velocity (reconIncom.cpp:642-852)
@@ -433,8 +449,8 @@ void recon_faces(Oracle& o) {
                 o.LU[f] = o.U[c]; o.LV[f] = o.V[c]; o.LW[f] = o.W[c];
                 o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
             } else if (pb.u_type == SLIP) {
-                double u = o.U[c], v = o.V[c], w = o.W[c];
-                double nv = u * n[0] + v * n[1] + w * n[2];
+                Real u = o.U[c], v = o.V[c], w = o.W[c];
+                Real nv = u * n[0] + v * n[1] + w * n[2];
                 o.LU[f] = u - nv * n[0]; o.LV[f] = v - nv * n[1]; o.LW[f] = w - nv * n[2];
                 o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
             } else if (pb.u_type == NO_SLIP) {
@@ -445,7 +461,7 @@ void recon_faces(Oracle& o) {
                 o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
             } else if (pb.u_type == INLET_OUTLET) {
                 double nv = pb.u_val[0];
-                double own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
+                Real own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
                 if (own > 0.0) { o.LU[f] = o.U[c]; o.LV[f] = o.V[c]; o.LW[f] = o.W[c]; }
                 else { o.LU[f] = nv * n[0]; o.LV[f] = nv * n[1]; o.LW[f] = nv * n[2]; }
                 o.RU[f] = o.LU[f]; o.RV[f] = o.LV[f]; o.RW[f] = o.LW[f];
@@ -454,8 +470,8 @@ void recon_faces(Oracle& o) {
             if (pb.vf_type == FIXED_VALUE) o.LVF[f] = pb.vf_val;
             else if (pb.vf_type == FUNCTION) o.LVF[f] = o.vffun[f - o.nif];
             else if (pb.vf_type == INLET_OUTLET) {
-                double own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
-                o.LVF[f] = own > 0.0 ? o.VF[c] : pb.vf_val;
+                Real own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
+                o.LVF[f] = own > 0.0 ? o.VF[c] : Real(pb.vf_val);
             } else o.LVF[f] = o.VF[c];
             o.RVF[f] = o.LVF[f];
             o.Lmu[f] = o.Mu[c]; o.Rmu[f] = o.Lmu[f];             // reconIncom.cpp:1024-1028
@@ -464,10 +480,10 @@ void recon_faces(Oracle& o) {
     // face density from the mixture EOS (reconIncom.cpp:445-493, eos.cpp:439-489)
     for (int f = 0; f < nf; ++f) {
         for (int side = 0; side < 2; ++side) {
-            double a = side == 0 ? o.LVF[f] : o.RVF[f];
-            double v0 = std::max(0.0, std::min(1.0, a));
-            double v1 = std::max(0.0, std::min(1.0, 1.0 - a));
-            double r = 0.0; r += v0 * o.rho[0]; r += v1 * o.rho[1];
+            Real a = side == 0 ? o.LVF[f] : o.RVF[f];
+            Real v0 = clamp01(a);
+            Real v1 = clamp01(1.0 - a);
+            Real r = 0.0; r += v0 * o.rho[0]; r += v1 * o.rho[1];
             (side == 0 ? o.LRho : o.RRho)[f] = r;
         }
     }
@@ -479,10 +495,10 @@ void source_gravity(Oracle& o) {
     for (int f = 0; f < o.nf; ++f) {
         double wCL = o.fwC[f], wCR = 1.0 - wCL;
         const double* n = &o.fnrm[3 * f];
-        double UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
-        double UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
-        double UnF = wCL * UnL + wCR * UnR;
-        double RhoF = 1.0 / (wCL / o.LRho[f] + wCR / o.RRho[f]);
+        Real UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
+        Real UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
+        Real UnF = wCL * UnL + wCR * UnR;
+        Real RhoF = 1.0 / (wCL / o.LRho[f] + wCR / o.RRho[f]);
         double gL = 0.0, gR = 0.0;
         for (int a = 0; a < 3; ++a) { gL += o.g[a] * o.fvecPF[3 * f + a]; gR += o.g[a] * o.fvecNF[3 * f + a]; }
         int l = o.owner[f];
@@ -499,14 +515,14 @@ void source_gravity(Oracle& o) {
 // Rhie-Chow face velocity of the assembly face loops:
 // eqMomentum.cpp:519-602, eqPressure.cpp:416-501,
 // surfaceNormalVelocity.cpp:125-223 (Pressure, Gravity, SurfTens).
-double face_UnF(const Oracle& o, int f) {
+Real face_UnF(const Oracle& o, int f) {
     int l = o.owner[f], r = o.nbr[f];
     const double* n = &o.fnrm[3 * f];
     const double* e = &o.fePN[3 * f];
     const double* sk = &o.fskew[3 * f];
     double wCL = o.fwC[f], wCR = 1.0 - wCL;
     double alpha = o.falpha[f], dPN = o.fmagPN[f];
-    double ULs = o.U[l], VLs = o.V[l], WLs = o.W[l], URs = o.U[r], VRs = o.V[r], WRs = o.W[r];
+    Real ULs = o.U[l], VLs = o.V[l], WLs = o.W[l], URs = o.U[r], VRs = o.V[r], WRs = o.W[r];
     for (int a = 0; a < 3; ++a) {
         ULs += o.gU[3 * l + a] * sk[a]; URs += o.gU[3 * r + a] * sk[a];
         VLs += o.gV[3 * l + a] * sk[a]; VRs += o.gV[3 * r + a] * sk[a];
@@ -515,12 +531,12 @@ double face_UnF(const Oracle& o, int f) {
     ULs = std::max(o.minU[l], std::min(o.maxU[l], ULs)); URs = std::max(o.minU[r], std::min(o.maxU[r], URs));
     VLs = std::max(o.minV[l], std::min(o.maxV[l], VLs)); VRs = std::max(o.minV[r], std::min(o.maxV[r], VRs));
     WLs = std::max(o.minW[l], std::min(o.maxW[l], WLs)); WRs = std::max(o.minW[r], std::min(o.maxW[r], WRs));
-    double UnL = ULs * n[0] + VLs * n[1] + WLs * n[2];
-    double UnR = URs * n[0] + VRs * n[1] + WRs * n[2];
-    double UnF = wCL * UnL + wCR * UnR;
-    double RhoL = o.Rho[l], RhoR = o.Rho[r];
-    double Rs = 1.0 / (wCL / RhoL + wCR / RhoR);
-    double t1 = o.dt / Rs;
+    Real UnL = ULs * n[0] + VLs * n[1] + WLs * n[2];
+    Real UnR = URs * n[0] + VRs * n[1] + WRs * n[2];
+    Real UnF = wCL * UnL + wCR * UnR;
+    Real RhoL = o.Rho[l], RhoR = o.Rho[r];
+    Real Rs = 1.0 / (wCL / RhoL + wCR / RhoR);
+    Real t1 = o.dt / Rs;
     // pressure
     UnF -= alpha * t1 * (o.P[r] - o.P[l]) / dPN;
     for (int a = 0; a < 3; ++a) {
@@ -576,7 +592,8 @@ void assemble_momentum(Oracle& o) {
     o.B0.assign(nc, 0.0); o.B1.assign(nc, 0.0); o.B2.assign(nc, 0.0);
     double dt = o.dt;
     for (int i = 0; i < nc; ++i) {               // eqMomentum.cpp:223-326
-        double R = o.Rho[i], vol = o.cvol[i];
+        Real R = o.Rho[i];
+        double vol = o.cvol[i];
         o.Arows[i] = i; o.Acols[i] = i;
         o.Avals[i] = R * 1.5 * vol / dt;
         o.B0[i] = -R * (1.5 * o.U[i] - 2.0 * o.oldU[i] + 0.5 * o.Qm0[i]) * vol / dt;
@@ -584,11 +601,11 @@ void assemble_momentum(Oracle& o) {
         o.B2[i] = -R * (1.5 * o.W[i] - 2.0 * o.oldW[i] + 0.5 * o.Qm2[i]) * vol / dt;
         o.B0[i] += vol * o.srcG[0][i]; o.B1[i] += vol * o.srcG[1][i]; o.B2[i] += vol * o.srcG[2][i];
         o.B0[i] += vol * o.srcS[0][i]; o.B1[i] += vol * o.srcS[1][i]; o.B2[i] += vol * o.srcS[2][i];
-        double ap = o.alphaP[i] * vol;
+        Real ap = o.alphaP[i] * vol;
         o.Avals[i] += ap;
         o.B0[i] -= ap * o.U[i]; o.B1[i] -= ap * o.V[i]; o.B2[i] -= ap * o.W[i];
     }
-    double* B[3] = {o.B0.data(), o.B1.data(), o.B2.data()};
+    Real* B[3] = {o.B0.data(), o.B1.data(), o.B2.data()};
     for (int f = 0; f < o.nif; ++f) {            // eqMomentum.cpp:331-787
         int l = o.owner[f], r = o.nbr[f];
         double area = o.farea[f];
@@ -596,16 +613,16 @@ void assemble_momentum(Oracle& o) {
         const double* e = &o.fePN[3 * f];
         const double* sk = &o.fskew[3 * f];
         double wCL = o.fwC[f], wCR = 1.0 - wCL;
-        double muF = 1.0 / (wCL / o.Lmu[f] + wCR / o.Rmu[f]);
-        double RhoL = o.Rho[l], RhoR = o.Rho[r];
+        Real muF = 1.0 / (wCL / o.Lmu[f] + wCR / o.Rmu[f]);
+        Real RhoL = o.Rho[l], RhoR = o.Rho[r];
         double dPN = o.fmagPN[f], alpha = o.falpha[f];
-        double UnF = face_UnF(o, f);
+        Real UnF = face_UnF(o, f);
         double wL = (UnF > 0.0) ? 1.0 : 0.0, wR = 1.0 - wL;
-        double PL = o.P[l], PR = o.P[r];
-        double PF = wCL * PL + wCR * PR;
-        double UF = wL * o.LU[f] + wR * o.RU[f];
-        double VFv = wL * o.LV[f] + wR * o.RV[f];
-        double WF = wL * o.LW[f] + wR * o.RW[f];
+        Real PL = o.P[l], PR = o.P[r];
+        Real PF = wCL * PL + wCR * PR;
+        Real UF = wL * o.LU[f] + wR * o.RU[f];
+        Real VFv = wL * o.LV[f] + wR * o.RV[f];
+        Real WF = wL * o.LW[f] + wR * o.RW[f];
         PF = wCL * PL + wCR * PR;
         for (int a = 0; a < 3; ++a) {
             PF += wCL * o.gP[3 * l + a] * sk[a];
@@ -614,27 +631,27 @@ void assemble_momentum(Oracle& o) {
         PF = std::max(o.minP[l], std::min(o.maxP[l], PF));
         PF = std::max(o.minP[r], std::min(o.maxP[r], PF));
         o.Avals[l] += (wL * RhoL * UnF * area + alpha * muF / dPN * area);
-        double v = (wR * RhoL * UnF * area - alpha * muF / dPN * area);
-        if (!(std::abs(v) < std::numeric_limits<double>::min())) {
+        Real v = (wR * RhoL * UnF * area - alpha * muF / dPN * area);
+        if (!(fabs(v) < std::numeric_limits<double>::min())) {
             o.Arows.push_back(l); o.Acols.push_back(r); o.Avals.push_back(v);
         }
         o.Avals[r] -= (wR * RhoR * UnF * area - alpha * muF / dPN * area);
         v = -(wL * RhoR * UnF * area + alpha * muF / dPN * area);
-        if (!(std::abs(v) < std::numeric_limits<double>::min())) {
+        if (!(fabs(v) < std::numeric_limits<double>::min())) {
             o.Arows.push_back(r); o.Acols.push_back(l); o.Avals.push_back(v);
         }
-        double Fv[3] = {UF, VFv, WF};
-        double uL[3] = {o.U[l], o.V[l], o.W[l]}, uR[3] = {o.U[r], o.V[r], o.W[r]};
-        const double* gL[3] = {&o.gU[3 * l], &o.gV[3 * l], &o.gW[3 * l]};
-        const double* gR[3] = {&o.gU[3 * r], &o.gV[3 * r], &o.gW[3 * r]};
+        Real Fv[3] = {UF, VFv, WF};
+        Real uL[3] = {o.U[l], o.V[l], o.W[l]}, uR[3] = {o.U[r], o.V[r], o.W[r]};
+        const Real* gL[3] = {&o.gU[3 * l], &o.gV[3 * l], &o.gW[3 * l]};
+        const Real* gR[3] = {&o.gU[3 * r], &o.gV[3 * r], &o.gW[3 * r]};
         for (int k = 0; k < 3; ++k) {
-            double cL = (RhoL * Fv[k] * UnF + PF * n[k]) * area;
-            double cR = (RhoR * Fv[k] * UnF + PF * n[k]) * area;
-            double df = alpha * muF * (uR[k] - uL[k]) / dPN * area;
+            Real cL = (RhoL * Fv[k] * UnF + PF * n[k]) * area;
+            Real cR = (RhoR * Fv[k] * UnF + PF * n[k]) * area;
+            Real df = alpha * muF * (uR[k] - uL[k]) / dPN * area;
             for (int a = 0; a < 3; ++a)
                 df += (muF * (wCL * gL[k][a] + wCR * gR[k][a]) * (n[a] - alpha * e[a]) * area);
-            double dL = ((gL[k][0] * n[0] + gL[k][1] * n[1] + gL[k][2] * n[2]) * muF * area);
-            double dR = ((gR[k][0] * n[0] + gR[k][1] * n[1] + gR[k][2] * n[2]) * muF * area);
+            Real dL = ((gL[k][0] * n[0] + gL[k][1] * n[1] + gL[k][2] * n[2]) * muF * area);
+            Real dR = ((gR[k][0] * n[0] + gR[k][1] * n[1] + gR[k][2] * n[2]) * muF * area);
             B[k][l] -= (cL - df - dL);
             B[k][r] += (cR - df - dR);
         }
@@ -645,31 +662,31 @@ void assemble_momentum(Oracle& o) {
             double area = o.farea[f];
             const double* n = &o.fnrm[3 * f];
             const double* e = &o.fePN[3 * f];
-            double UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
-            double UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
-            double UnF = 0.5 * UnL + 0.5 * UnR;
-            double RhoL = o.LRho[f];
-            double t1 = 1.0 / RhoL * o.dt;
-            double Fv[3] = {0.5 * o.LU[f] + 0.5 * o.RU[f], 0.5 * o.LV[f] + 0.5 * o.RV[f],
-                            0.5 * o.LW[f] + 0.5 * o.RW[f]};
-            double PF = 0.5 * o.LP[f] + 0.5 * o.RP[f];
+            Real UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
+            Real UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
+            Real UnF = 0.5 * UnL + 0.5 * UnR;
+            Real RhoL = o.LRho[f];
+            Real t1 = 1.0 / RhoL * o.dt;
+            Real Fv[3] = {0.5 * o.LU[f] + 0.5 * o.RU[f], 0.5 * o.LV[f] + 0.5 * o.RV[f],
+                          0.5 * o.LW[f] + 0.5 * o.RW[f]};
+            Real PF = 0.5 * o.LP[f] + 0.5 * o.RP[f];
             double cU = 0, cUd = 0;
             if (pb.u_type == FIXED_VALUE || pb.u_type == NO_SLIP || pb.u_type == SURFACE_NORMAL_FIXED_VALUE) { cU = 0; cUd = 1; }
             else if (pb.u_type == ZERO_GRADIENT) { cU = 1; cUd = 0; }
             else if (pb.u_type == SLIP) { cU = 0; cUd = 0; }
             else if (pb.u_type == INLET_OUTLET) {
-                double own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
+                Real own = o.U[c] * n[0] + o.V[c] * n[1] + o.W[c] * n[2];
                 cU = own > 0.0 ? 1.0 : 0.0; cUd = own > 0.0 ? 0.0 : 1.0;
             }
-            double muF = 0.5 * o.Lmu[f] + 0.5 * o.Rmu[f];
-            double org[3][2] = {{o.U[c], o.RU[f]}, {o.V[c], o.RV[f]}, {o.W[c], o.RW[f]}};
+            Real muF = 0.5 * o.Lmu[f] + 0.5 * o.Rmu[f];
+            Real org[3][2] = {{o.U[c], o.RU[f]}, {o.V[c], o.RV[f]}, {o.W[c], o.RW[f]}};
             double dPN = 0.5 * o.fmagPN[f], alpha = o.falpha[f];
             if (pb.p_type == FIXED_VALUE) {
                 UnF -= alpha * t1 * (pb.p_val - o.P[c]) / dPN;
                 for (int a = 0; a < 3; ++a) UnF += alpha * t1 * o.gP[3 * c + a] * e[a];
             }
-            const double* gc[3] = {&o.gU[3 * c], &o.gV[3 * c], &o.gW[3 * c]};
-            double dn[3];
+            const Real* gc[3] = {&o.gU[3 * c], &o.gV[3 * c], &o.gW[3 * c]};
+            Real dn[3];
             for (int k = 0; k < 3; ++k) {
                 dn[k] = 0.0;
                 dn[k] += alpha * (org[k][1] - org[k][0]) / dPN;
@@ -702,11 +719,11 @@ void assemble_pressure(Oracle& o) {
         int l = o.owner[f], r = o.nbr[f];
         double area = o.farea[f];
         double wCL = o.fwC[f], wCR = 1.0 - wCL;
-        double RhoL = o.Rho[l], RhoR = o.Rho[r];
-        double Rs = 1.0 / (wCR / RhoL + wCR / RhoR);       // eqPressure.cpp:396
-        double t1 = o.dt / Rs;
+        Real RhoL = o.Rho[l], RhoR = o.Rho[r];
+        Real Rs = 1.0 / (wCR / RhoL + wCR / RhoR);       // eqPressure.cpp:396
+        Real t1 = o.dt / Rs;
         double dPN = o.fmagPN[f], alpha = o.falpha[f];
-        double UnF = face_UnF(o, f);
+        Real UnF = face_UnF(o, f);
         o.Avals[l] += (+alpha * t1 / dPN * area);
         o.Arows.push_back(l); o.Acols.push_back(r); o.Avals.push_back((-alpha * t1 / dPN * area));
         o.Avals[r] -= (-alpha * t1 / dPN * area);
@@ -720,11 +737,11 @@ void assemble_pressure(Oracle& o) {
             double area = o.farea[f];
             const double* n = &o.fnrm[3 * f];
             const double* e = &o.fePN[3 * f];
-            double UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
-            double UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
-            double UnF = 0.5 * UnL + 0.5 * UnR;
-            double RhoL = o.LRho[f];
-            double t1 = 1.0 / RhoL * o.dt;
+            Real UnL = o.LU[f] * n[0] + o.LV[f] * n[1] + o.LW[f] * n[2];
+            Real UnR = o.RU[f] * n[0] + o.RV[f] * n[1] + o.RW[f] * n[2];
+            Real UnF = 0.5 * UnL + 0.5 * UnR;
+            Real RhoL = o.LRho[f];
+            Real t1 = 1.0 / RhoL * o.dt;
             double cPd = pb.p_type == FIXED_VALUE ? 1.0 : 0.0;
             if (pb.p_type == SWITCH) cPd = mach_of(o, c) > 1.0 ? 1.0 : 0.0;     // eqPressure.cpp:625-632
             double dPN = 0.5 * o.fmagPN[f], alpha = o.falpha[f];
@@ -738,6 +755,7 @@ void assemble_pressure(Oracle& o) {
     }
 }
 
+#ifndef ORC_AD   // two-phase stages: not differentiated (the adjoint is single phase)
 // ------------------------------------------------------------------------
 // setIncomValuesLeftRightFaceWithVfMSTACS (reconIncom.cpp:345-430) with
 // calcMSTACS (NVD.cpp:1098-1862), one rank: zero-order face values, the LSQ
@@ -1108,6 +1126,7 @@ void assemble_vof(Oracle& o) {
         }
     }
 }
+#endif  // ORC_AD
 
 // ------------------------------------------------------------------------
 // linear solve: COO -> CSR (solveAMGCL.cpp:206-251), ILU0 + BiCGStab
@@ -1226,18 +1245,71 @@ int bicgstab(const Csr& A, const Ilu0& M, const vector<double>& b, vector<double
     return it;
 }
 
-void solve_last(Oracle& o, int nrhs) {
-    Csr A = coo_to_csr(o.nc, o.Arows, o.Acols, o.Avals);
-    Ilu0 M; M.factor(A);
-    vector<double>* B[3] = {&o.B0, &o.B1, &o.B2};
-    vector<double>* X[3] = {&o.X0, &o.X1, &o.X2};
-    for (int k = 0; k < nrhs; ++k) {
-        double rr; o.last_iters = bicgstab(A, M, *B[k], *X[k], o.tol, o.maxit, rr);
-        o.last_res = std::max(k == 0 ? 0.0 : o.last_res, rr);
-    }
+vector<double> values(const vector<Real>& v) {
+    vector<double> d(v.size());
+    for (size_t k = 0; k < v.size(); ++k) d[k] = val(v[k]);
+    return d;
 }
 
-// momentum update (eqMomentum.cpp:1029-1119)
+void solve_last(Oracle& o, int nrhs) {
+    Csr A = coo_to_csr(o.nc, o.Arows, o.Acols, values(o.Avals));
+    Ilu0 M; M.factor(A);
+    vector<Real>* B[3] = {&o.B0, &o.B1, &o.B2};
+    vector<Real>* X[3] = {&o.X0, &o.X1, &o.X2};
+    vector<double> x[3];
+    for (int k = 0; k < nrhs; ++k) {
+        double rr; o.last_iters = bicgstab(A, M, values(*B[k]), x[k], o.tol, o.maxit, rr);
+        o.last_res = std::max(k == 0 ? 0.0 : o.last_res, rr);
+    }
+#ifndef ORC_AD
+    for (int k = 0; k < nrhs; ++k) X[k]->swap(x[k]);
+#else
+    // x = A^-1 b as one step of the tape: the reverse step solves
+    // A^T lam = x_bar, then b_bar += lam and a_bar(k) -= lam[row k] x[col k]
+    // for every COO entry k (the implicit-function rule of an exact solve)
+    ad::Tape& T = ad::tape();
+    int n = o.nc;
+    for (int k = 0; k < nrhs; ++k) {
+        X[k]->resize(n);
+        for (int i = 0; i < n; ++i) (*X[k])[i] = ad::adouble::var(x[k][i]);
+    }
+    if (!T.on) return;
+    struct SolveRec {
+        int n, nrhs; double tol; int maxit;
+        vector<int> rows, cols, aidx, bidx, xidx;
+        vector<double> avals, x;
+    };
+    auto rec = std::make_shared<SolveRec>();
+    rec->n = n; rec->nrhs = nrhs; rec->tol = o.tol; rec->maxit = o.maxit;
+    rec->rows = o.Arows; rec->cols = o.Acols; rec->avals = values(o.Avals);
+    rec->aidx.resize(o.Avals.size());
+    for (size_t q = 0; q < o.Avals.size(); ++q) rec->aidx[q] = o.Avals[q].i;
+    for (int k = 0; k < nrhs; ++k)
+        for (int i = 0; i < n; ++i) {
+            rec->bidx.push_back((*B[k])[i].i); rec->xidx.push_back((*X[k])[i].i); rec->x.push_back(x[k][i]);
+        }
+    T.push_custom([rec](vector<double>& adj) {
+        int n = rec->n;
+        Csr At = coo_to_csr(n, rec->cols, rec->rows, rec->avals);
+        Ilu0 Mt; Mt.factor(At);
+        for (int k = 0; k < rec->nrhs; ++k) {
+            vector<double> xb(n), lam;
+            for (int i = 0; i < n; ++i) xb[i] = adj[rec->xidx[(size_t)k * n + i]];
+            double rr;
+            bicgstab(At, Mt, xb, lam, rec->tol, rec->maxit, rr);
+            for (int i = 0; i < n; ++i) {
+                int b = rec->bidx[(size_t)k * n + i];
+                if (b >= 0) adj[b] += lam[i];
+            }
+            const double* xk = &rec->x[(size_t)k * n];
+            for (size_t q = 0; q < rec->rows.size(); ++q)
+                if (rec->aidx[q] >= 0) adj[rec->aidx[q]] -= lam[rec->rows[q]] * xk[rec->cols[q]];
+        }
+    });
+#endif
+}
+
+#ifndef ORC_AD
 // VF += vofVofURF dVF, clipped to [0,1]; returns sqrt(sum dVF^2)
 // (eqVolfrac.cpp:770-783)
 double vof_update(Oracle& o) {
@@ -1250,17 +1322,20 @@ double vof_update(Oracle& o) {
     return sqrt(nd);
 }
 
+#endif  // ORC_AD
+
+// momentum update (eqMomentum.cpp:1029-1119)
 double momentum_update(Oracle& o) {
     double nd = 0, np = 0;
     for (int i = 0; i < o.nc; ++i) {
         o.U[i] += o.urf_u * o.X0[i]; o.V[i] += o.urf_u * o.X1[i]; o.W[i] += o.urf_u * o.X2[i];
-        double* q[3] = {&o.U[i], &o.V[i], &o.W[i]};
+        Real* q[3] = {&o.U[i], &o.V[i], &o.W[i]};
         for (int k = 0; k < 3; ++k) {
             if (*q[k] <= -o.lim) *q[k] = -o.lim;
             if (*q[k] >= o.lim) *q[k] = o.lim;
         }
-        nd += pow(o.X0[i], 2.0); nd += pow(o.X1[i], 2.0); nd += pow(o.X2[i], 2.0);
-        np += pow(o.U[i], 2.0); np += pow(o.V[i], 2.0); np += pow(o.W[i], 2.0);
+        nd += pow(val(o.X0[i]), 2.0); nd += pow(val(o.X1[i]), 2.0); nd += pow(val(o.X2[i]), 2.0);
+        np += pow(val(o.U[i]), 2.0); np += pow(val(o.V[i]), 2.0); np += pow(val(o.W[i]), 2.0);
     }
     return sqrt(nd) / sqrt(np);
 }
@@ -1268,14 +1343,14 @@ double momentum_update(Oracle& o) {
 // p' Green-Gauss gradient x2 and the correction (eqPressure.cpp:686-883)
 double pressure_correct(Oracle& o) {
     int nc = o.nc;
-    const vector<double>& pp = o.X0;
-    vector<double> gp(3 * (size_t)nc, 0.0), tmp(3 * (size_t)nc);
+    const vector<Real>& pp = o.X0;
+    vector<Real> gp(3 * (size_t)nc, 0.0), tmp(3 * (size_t)nc);
     for (int pass = 0; pass < 2; ++pass) {
-        std::fill(tmp.begin(), tmp.end(), 0.0);
+        std::fill(tmp.begin(), tmp.end(), Real(0.0));
         for (int f = 0; f < o.nif; ++f) {
             int l = o.owner[f], r = o.nbr[f];
             double wCL = o.fwC[f], wCR = 1.0 - wCL;
-            double vF = wCL * pp[l] + wCR * pp[r];
+            Real vF = wCL * pp[l] + wCR * pp[r];
             const double* sk = &o.fskew[3 * f];
             for (int a = 0; a < 3; ++a) {
                 vF += wCL * gp[3 * l + a] * sk[a];
@@ -1288,7 +1363,7 @@ double pressure_correct(Oracle& o) {
         for (const Patch& pb : o.patches)
             for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
                 int c = o.owner[f];
-                double vF = pp[c];
+                Real vF = pp[c];
                 if (pb.p_type == FIXED_VALUE) vF = 0.0;
                 const double* n = &o.fnrm[3 * f];
                 for (int a = 0; a < 3; ++a) tmp[3 * c + a] += vF * n[a] * o.farea[f] / o.cvol[c];
@@ -1298,21 +1373,23 @@ double pressure_correct(Oracle& o) {
     double nd = 0, np = 0;
     for (int i = 0; i < nc; ++i) {
         o.P[i] += o.urf_p * pp[i];
-        double t1 = o.dt / o.Rho[i];
-        double rU = o.urf_pu * t1 * gp[3 * i], rV = o.urf_pu * t1 * gp[3 * i + 1],
-               rW = o.urf_pu * t1 * gp[3 * i + 2];
+        Real t1 = o.dt / o.Rho[i];
+        Real rU = o.urf_pu * t1 * gp[3 * i], rV = o.urf_pu * t1 * gp[3 * i + 1],
+             rW = o.urf_pu * t1 * gp[3 * i + 2];
         o.U[i] -= rU; o.V[i] -= rV; o.W[i] -= rW;
-        double* q[4] = {&o.P[i], &o.U[i], &o.V[i], &o.W[i]};
+        Real* q[4] = {&o.P[i], &o.U[i], &o.V[i], &o.W[i]};
         for (int k = 0; k < 4; ++k) {
             if (*q[k] <= -o.lim) *q[k] = -o.lim;
             if (*q[k] >= o.lim) *q[k] = o.lim;
         }
-        nd += pow(pp[i], 2.0); nd += pow(rU, 2.0); nd += pow(rV, 2.0); nd += pow(rW, 2.0);
-        np += pow(o.P[i], 2.0); np += pow(o.U[i], 2.0); np += pow(o.V[i], 2.0); np += pow(o.W[i], 2.0);
+        nd += pow(val(pp[i]), 2.0); nd += pow(val(rU), 2.0); nd += pow(val(rV), 2.0); nd += pow(val(rW), 2.0);
+        np += pow(val(o.P[i]), 2.0); np += pow(val(o.U[i]), 2.0); np += pow(val(o.V[i]), 2.0);
+        np += pow(val(o.W[i]), 2.0);
     }
     return sqrt(nd) / sqrt(np);
 }
 
+#ifndef ORC_AD
 // calcTimeStepFromCorantNumber (adjust != 0, timestep.cpp:260-330) or
 // calcCorantNumberForPrint (timestep.cpp:202-253): the smallest of the
 // convective, viscous, gravity and capillary time scales over all cells.
@@ -1388,6 +1465,7 @@ void pressure_based_step(Oracle& o, int ncorr, double* norms) {
     }
     o.iterReal++;
 }
+#endif  // ORC_AD
 
 // ------------------------------------------------------------------------
 // GENRCM (George & Liu, SPARSPAK) as called by lib_RCM (src/math/RCM.cpp:34-210,
@@ -1509,7 +1587,7 @@ void* orc_create(int np, int nc, int nf, int nif, const double* pts, const int* 
         o->patches.push_back(p);
     }
     build_geometry(*o);
-    auto z = [&](vector<double>& v, double x = 0.0) { v.assign(nc, x); };
+    auto z = [&](auto& v, double x = 0.0) { v.assign(nc, x); };
     z(o->P); z(o->U); z(o->V); z(o->W); z(o->VF, 1.0); z(o->MF, 1.0); z(o->Rho); z(o->Mu);
     z(o->Qm0); z(o->Qm1); z(o->Qm2); z(o->Qm3, 1.0);
     z(o->oldP); z(o->oldU); z(o->oldV); z(o->oldW); z(o->oldVF, 1.0); z(o->oldRho);
@@ -1560,6 +1638,7 @@ void orc_set_case(void* h, const double* rho, const double* mu, double sigma, co
 }
 
 // two-phase controls: vofVofURF and nSpeciesEq (VOF iterations per step)
+#ifndef ORC_AD
 void orc_set_vof(void* h, double urf_vf, int n_vof) {
     Oracle& o = *(Oracle*)h;
     o.urf_vf = urf_vf; o.n_vof = n_vof;
@@ -1667,6 +1746,146 @@ void orc_assemble_vof(void* h) { assemble_vof(*(Oracle*)h); }
 double orc_vof_update(void* h) { return vof_update(*(Oracle*)h); }
 double orc_vof_norm(void* h) { return ((Oracle*)h)->vof_norm; }
 void orc_step(void* h, int ncorr, double* norms) { pressure_based_step(*(Oracle*)h, ncorr, norms); }
+
+#else  // ORC_AD: the CPU adjoint (liborc_ad.so)
+
+void orc_set_solver(void* h, double tol, int maxit) {
+    Oracle& o = *(Oracle*)h; o.tol = tol; o.maxit = maxit;
+}
+
+// P, U, V, W or alphaP from doubles / to doubles
+int orc_ad_set(void* h, const char* nm, const double* in) {
+    Oracle& o = *(Oracle*)h;
+    std::string s(nm);
+    vector<Real>* v = s == "P" ? &o.P : s == "U" ? &o.U : s == "V" ? &o.V : s == "W" ? &o.W
+                    : s == "alphaP" ? &o.alphaP : nullptr;
+    if (!v) return -1;
+    for (int i = 0; i < o.nc; ++i) (*v)[i] = in[i];
+    return 0;
+}
+int orc_ad_get(void* h, const char* nm, double* out) {
+    Oracle& o = *(Oracle*)h;
+    std::string s(nm);
+    const vector<Real>* v = s == "P" ? &o.P : s == "U" ? &o.U : s == "V" ? &o.V : s == "W" ? &o.W : nullptr;
+    if (!v) return -1;
+    for (int i = 0; i < o.nc; ++i) out[i] = val((*v)[i]);
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// the set-up of one time step of pressure_based_step for a single-phase case
+// with a fixed time step and no body forces (the adjoint's scope)
+void ad_step_setup(Oracle& o) {
+    if (o.iterReal == 0) { cell_sound_speed(o); cell_eos_transport(o); }
+    if (o.iterReal == 0) { o.Qm0 = o.U; o.Qm1 = o.V; o.Qm2 = o.W; o.Qm3 = o.VF; }
+    else { o.Qm0 = o.oldU; o.Qm1 = o.oldV; o.Qm2 = o.oldW; o.Qm3 = o.oldVF; }
+    o.oldP = o.P; o.oldU = o.U; o.oldV = o.V; o.oldW = o.W; o.oldVF = o.VF; o.oldRho = o.Rho;
+    recon_faces(o);
+    source_gravity(o);
+    for (int k = 0; k < 4; ++k) o.srcS[k].assign(o.nc, 0.0);
+    o.kappa.assign(o.nc, 0.0);
+}
+// one outer iteration (pressureBased.cpp:139-212 for one corrector)
+void ad_iteration(Oracle& o) {
+    recon_faces(o);
+    assemble_momentum(o);
+    solve_last(o, 3);
+    momentum_update(o);
+    recon_faces(o);
+    assemble_pressure(o);
+    solve_last(o, 1);
+    pressure_correct(o);
+}
+// J = sum_patches w_p sum_f A_f p_f (p_f: a fixedValue patch's value, else the
+// owner's pressure) and dJ/dP
+double ad_objective(const Oracle& o, const double* w, vector<double>* dJdP) {
+    double J = 0.0;
+    if (dJdP) dJdP->assign(o.nc, 0.0);
+    for (size_t b = 0; b < o.patches.size(); ++b) {
+        const Patch& pb = o.patches[b];
+        for (int f = pb.start; f < pb.start + pb.nfaces; ++f) {
+            int c = o.owner[f];
+            J += w[b] * o.farea[f] * (pb.p_type == FIXED_VALUE ? pb.p_val : val(o.P[c]));
+            if (dJdP && pb.p_type != FIXED_VALUE) (*dJdP)[c] += w[b] * o.farea[f];
+        }
+    }
+    return J;
+}
+}  // namespace
+
+extern "C" {
+
+// J after one time step of n_iter outer iterations from the current state
+// (the state advances; no tape)
+double orc_ad_forward(void* h, int n_iter, const double* w) {
+    Oracle& o = *(Oracle*)h;
+    ad::tape().on = false;
+    ad_step_setup(o);
+    for (int it = 0; it < n_iter; ++it) ad_iteration(o);
+    o.iterReal++;
+    return ad_objective(o, w, nullptr);
+}
+
+// J and dJ/dalpha_p of that step by reverse-mode differentiation: the state
+// at the start of every outer iteration is kept (U, V, W, P); the sweep runs
+// backwards, re-taping one iteration from its checkpoint with the checkpoint
+// and alpha_p as independent variables, seeding the iteration's outputs with
+// the adjoint of the state after it, and reversing the tape.  The old time
+// levels are constants of the step.  Leaves the forward's final state.
+double orc_ad_adjoint(void* h, int n_iter, const double* w, double* grad) {
+    Oracle& o = *(Oracle*)h;
+    ad::Tape& T = ad::tape();
+    T.on = false; T.clear();
+    int n = o.nc;
+    ad_step_setup(o);
+    vector<vector<double>> ck(n_iter);
+    for (int it = 0; it < n_iter; ++it) {
+        vector<double>& c = ck[it];
+        c.resize(4 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            c[i] = val(o.U[i]); c[n + i] = val(o.V[i]); c[2 * n + i] = val(o.W[i]); c[3 * n + i] = val(o.P[i]);
+        }
+        ad_iteration(o);
+    }
+    o.iterReal++;
+    vector<double> fin[4] = {values(o.U), values(o.V), values(o.W), values(o.P)};
+    vector<double> bar[4];
+    for (int q = 0; q < 3; ++q) bar[q].assign(n, 0.0);
+    double J = ad_objective(o, w, &bar[3]);
+    vector<double> alpha = values(o.alphaP), abar(n, 0.0);
+    vector<Real>* st[4] = {&o.U, &o.V, &o.W, &o.P};
+    for (int it = n_iter - 1; it >= 0; --it) {
+        T.clear(); T.on = true;
+        vector<int> in[4], ain(n);
+        for (int q = 0; q < 4; ++q) {
+            in[q].resize(n);
+            for (int i = 0; i < n; ++i) {
+                (*st[q])[i] = ad::adouble::var(ck[it][(size_t)q * n + i]);
+                in[q][i] = (*st[q])[i].i;
+            }
+        }
+        for (int i = 0; i < n; ++i) { o.alphaP[i] = ad::adouble::var(alpha[i]); ain[i] = o.alphaP[i].i; }
+        ad_iteration(o);
+        T.on = false;
+        vector<double> adj(T.next, 0.0);
+        for (int q = 0; q < 4; ++q)
+            for (int i = 0; i < n; ++i)
+                if ((*st[q])[i].i >= 0) adj[(*st[q])[i].i] += bar[q][i];
+        T.reverse(adj);
+        for (int q = 0; q < 4; ++q)
+            for (int i = 0; i < n; ++i) bar[q][i] = adj[in[q][i]];
+        for (int i = 0; i < n; ++i) abar[i] += adj[ain[i]];
+    }
+    T.clear();
+    for (int q = 0; q < 4; ++q)
+        for (int i = 0; i < n; ++i) (*st[q])[i] = fin[q][i];
+    for (int i = 0; i < n; ++i) { o.alphaP[i] = alpha[i]; grad[i] = abar[i]; }
+    return J;
+}
+
+#endif  // ORC_AD
 
 // RCM of an arbitrary graph: adjacency CSR (0-based), perm[new] = old,
 // inv[old] = new (lib_RCM's cell.RCM / cell.invRCM).
