@@ -190,3 +190,39 @@ def test_adjoint_vs_cpu_adjoint(oracle_lib, n, physics):
           f"max |dJ/dalpha| {np.abs(gc).max():.3e}")
     assert abs(J - Jc) <= 1e-10 * abs(Jc)
     assert rel <= 1e-10
+
+
+def test_darcy_term_definition():
+    """The Darcy term in closed form, independent of the oracle: on one
+    state, the momentum system assembled with alpha_p = a differs from the
+    one with alpha_p = 0 by a_i V_i on the diagonal and -a_i V_i U_i
+    (and V, W) on the right sides, and nowhere else."""
+    from semo_mesh_amd.solver import GpuSolver
+    mesh = cavity_mesh(6, jitter=0.1, seed=41)
+    case = _case()
+    rng = np.random.default_rng(43)
+    n = mesh.n_cells
+    state = {f: rng.standard_normal(n) for f in ("U", "V", "W", "P")}
+    a = 10.0 * rng.random(n)
+    out = []
+    for alpha in (np.zeros(n), a):
+        g = GpuSolver(0)
+        g.upload_mesh(mesh, reorder=True)
+        g.set_case(case)
+        for f, v in state.items():
+            g.set_field(f, v)
+        g.set_field("alphaP", alpha)
+        g.assemble("momentum")
+        out.append(g.matrix("momentum"))
+        vol = g.get_field("cell_volume")
+        g.close()
+    (r0, c0, v0, b0), (r1, c1, v1, b1) = out
+    assert np.array_equal(r0, r1) and np.array_equal(c0, c1)
+    # the diagonal comes first (device order, caller's cell numbers)
+    assert np.array_equal(r0[:n], c0[:n]) and np.array_equal(np.sort(r0[:n]), np.arange(n))
+    d0 = np.empty(n); d1 = np.empty(n)
+    d0[r0[:n]] = v0[:n]; d1[r0[:n]] = v1[:n]
+    assert np.allclose(d1 - d0, a * vol, rtol=1e-12, atol=1e-12 * np.abs(d0).max())
+    assert np.array_equal(v0[n:], v1[n:])
+    for k, f in enumerate(("U", "V", "W")):
+        assert np.allclose(b1[k] - b0[k], -a * vol * state[f], rtol=1e-9, atol=1e-12 * np.abs(b0[k]).max())
