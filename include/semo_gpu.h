@@ -291,6 +291,18 @@ int semo_matrix_get(semo_ctx* ctx, int64_t* nnz, int32_t* rows, int32_t* cols, d
  * phi[n] cell values, face_r[n_faces] right-face values (used at boundary
  * faces), grad[3n].                                                            */
 int semo_least_square(semo_ctx* ctx, const double* phi, const double* face_r, double* grad);
+/* semo_least_square without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops).  Device pointers in the library's device numbering
+ * (semo_mesh_get_perm: cell k of the device is the caller's cell perm[k];
+ * semo_mesh_get_face_map for faces): d_phi[n_cells], d_face_r the right
+ * values of the boundary faces [n_faces - n_internal_faces] in device face
+ * order, d_grad[3 n_cells] out.  Issued on the context's stream, not
+ * synchronised.  Its scratch is one static device array per GPU for up to
+ * 2097152 cells (64 MiB, held until the process ends); larger meshes get
+ * SEMO_EINVAL.  Calls on one GPU share that scratch: concurrent calls on one
+ * GPU (from different contexts or streams) race.                            */
+int semo_least_square_noalloc(semo_ctx* ctx, const double* d_phi, const double* d_face_r, double* d_grad);
 /* calcGaussGreen(mesh, cn, fn, gradient) (src/math/gradient.cpp:1778-1985):
  * field = SEMO_F_P, _U, _V, _W or _VF names cn, whose patch types and values
  * (semo_set_case) give the boundary faces' values (fixedValue: the patch

@@ -73,7 +73,7 @@ EXPORTED = [
     "semo_calc_source_gravity", "semo_calc_source_surface_tension",
     "semo_incompressible_pressure_based", "semo_adjoint_alpha_p", "semo_objective_pressure",
     "semo_calc_coupled_eq", "semo_coupled_pressure_based", "semo_coupled_matrix_get", "semo_coupled_spmv", "semo_coupled_solve",
-    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_cell_min_max", "semo_gauss_green", "semo_solve_coo",
+    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_least_square_noalloc", "semo_cell_min_max", "semo_gauss_green", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
 ]
 
@@ -121,6 +121,7 @@ def lib():
         "semo_assemble": (C.c_int, [vp, C.c_int]),
         "semo_matrix_get": (C.c_int, [vp, C.POINTER(i64), vp, vp, vp, vp]),
         "semo_least_square": (C.c_int, [vp, dp, dp, dp]),
+        "semo_least_square_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "semo_cell_min_max": (C.c_int, [vp, dp, dp, dp]),
         "semo_gauss_green": (C.c_int, [vp, C.c_int, dp, dp, dp]),
         "semo_solve_coo": (C.c_int, [vp, C.c_char_p, i32, i64, vp, vp, vp, i32, dp, dp,

@@ -13,6 +13,7 @@
 // per-cell order in which SEMO's face loops scatter (eqMomentum.cpp:331-1008,
 // eqPressure.cpp:246-673) -- and adds the boundary faces inline.
 #include <climits>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -1684,6 +1685,47 @@ double pressure_correct(semo_ctx* c) {
 
 void lsq_standalone(semo_ctx* c, const double* phi, const double* faceR, double* grad) {
     lsq1(c, phi, faceR, 0, grad);
+}
+
+// ------------------------------------------ LSQ gradient without allocation --
+// Scratch of semo_least_square_noalloc: the packed [x, y, z, phi] records the
+// one-field LSQ pass gathers (32 B per cell), for up to NOALLOC_MAX_CELLS
+// cells: 64 MiB, which every GPU the library runs on holds from module load
+// until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is a stand-in, not a device address): it uses the
+// address hipGetSymbolAddress gives for the current device.
+static __device__ __align__(256) double scratch[4 * (size_t)NOALLOC_MAX_CELLS];
+
+// this device's copy of scratch, looked up once per device with that device
+// current (the context's creation calls it)
+double* noalloc_scratch(int device) {
+    static std::mutex mu;
+    static double* ptr[64] = {};
+    SEMO_CHECK(device >= 0 && device < 64, SEMO_EINVAL, "device index");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        void* p = nullptr;
+        SEMO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[device] = static_cast<double*>(p);
+    }
+    return ptr[device];
+}
+
+void lsq1_noalloc(semo_ctx* c, const double* phi, const double* faceR_b, double* grad) {
+    SEMO_CHECK(c->nc <= NOALLOC_MAX_CELLS, SEMO_EINVAL,
+               "semo_least_square_noalloc: more cells than its scratch holds (2097152)");
+    SEMO_CHECK(c->noalloc_xph, SEMO_ESTATE, "no-allocation scratch not set up");
+    int n = c->nc;
+    dim3 gb(nblk(n)), tb(TPB);
+    double* xph = c->noalloc_xph;
+    hipLaunchKernelGGL(k_pack_xphi, gb, tb, 0, c->stream, n, c->c_xyz.p, phi, xph, (const int*)nullptr);
+    if (c->d_st_dl.n)
+        hipLaunchKernelGGL(k_lsq1<int16_t>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
+                           c->d_bf_idx.p, xph, c->f_xyz.p, c->c_lsq.p, faceR_b, c->nif, grad, (const int*)nullptr);
+    else
+        hipLaunchKernelGGL(k_lsq1<int>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
+                           c->d_bf_idx.p, xph, c->f_xyz.p, c->c_lsq.p, faceR_b, c->nif, grad, (const int*)nullptr);
+    SEMO_HIP(hipGetLastError());
 }
 
 void minmax_standalone(semo_ctx* c, const double* phi, double* mx, double* mn) {

@@ -201,6 +201,7 @@ int semo_ctx_create(int device, void* stream, semo_ctx** out) {
         else { SEMO_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)); c->own_stream = true; }
         const char* gv = std::getenv("SEMO_GPU_GRAPHS");
         c->graphs = !(gv && gv[0] == '0');
+        c->noalloc_xph = noalloc_scratch(device);
         c->sp.precision = SEMO_PREC_FP64; c->sp.tol_p = 1e-8; c->sp.tol_u = 1e-8;
         c->sp.maxit_p = 500; c->sp.maxit_u = 500; c->sp.cheb_degree = 1; c->sp.coarse_size = 1000;
     });
@@ -748,6 +749,14 @@ int semo_gauss_green(semo_ctx* c, int field, const double* phi, const double* fa
         gout.alloc(3 * (size_t)c->nc);
         gauss_green_standalone(c, sel, dphi.p, dfr.p, gin.p, gout.p);
         from_dev(c, gout.p, grad, 3);
+    });
+}
+
+int semo_least_square_noalloc(semo_ctx* c, const double* d_phi, const double* d_face_r, double* d_grad) {
+    if (!c || !d_phi || !d_face_r || !d_grad) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        lsq1_noalloc(c, d_phi, d_face_r, d_grad);
     });
 }
 

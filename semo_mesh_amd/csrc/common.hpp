@@ -311,6 +311,8 @@ struct semo_ctx {
     hipEvent_t pev[4] = {};
     semo::AdjBufs* adj = nullptr;
     semo::Coupled* cpl = nullptr;
+    // this device's copy of the no-allocation entry point's scratch
+    double* noalloc_xph = nullptr;
 };
 
 namespace semo {
@@ -420,6 +422,10 @@ double vof_update(semo_ctx* c);
 double momentum_update(semo_ctx* c);
 double pressure_correct(semo_ctx* c);
 void lsq_standalone(semo_ctx* c, const double* d_phi, const double* d_faceR, double* d_grad);
+// the LSQ gradient on the static scratch (semo_least_square_noalloc)
+constexpr int NOALLOC_MAX_CELLS = 1 << 21;
+double* noalloc_scratch(int device);
+void lsq1_noalloc(semo_ctx* c, const double* phi, const double* faceR_b, double* grad);
 void minmax_standalone(semo_ctx* c, const double* d_phi, double* d_mx, double* d_mn);
 void copy_d(semo_ctx* c, double* dst, const double* src, size_t n);
 // solver.hip

@@ -268,6 +268,13 @@ class GpuSolver:
         self._ok(self._L.semo_least_square(self._h, _ptr(phi), _ptr(fr), _ptr(g)))
         return g.reshape(-1, 3)
 
+    def least_square_noalloc(self, d_phi: int, d_face_r: int, d_grad: int):
+        """semo_least_square_noalloc on device addresses (device numbering,
+        boundary face values in device face order); issued on the context's
+        stream, not synchronised."""
+        self._ok(self._L.semo_least_square_noalloc(self._h, C.c_void_p(d_phi), C.c_void_p(d_face_r),
+                                                   C.c_void_p(d_grad)))
+
     def calcGaussGreen(self, field, phi, face_r, gradient):
         """calcGaussGreen(mesh, cn, fn, gradient): field names cn ("P", "U",
         "V", "W" or "VF", whose patch types give the boundary values), phi the
