@@ -14,6 +14,11 @@
 #  * math.cpp uses the Intel compiler's built-in __int64 type and a redeclared
 #    default argument; g++ accepts both with -D__int64="long long" -fpermissive.
 #  * Every solveAMGCL call is intercepted with -Wl,--wrap (see ref_driver.cpp).
+#  * semo_gpu_eq / semo_gpu_step: the same driver and reference objects with
+#    the C++ bridge of semo_mesh_amd/bridge linked in place of the equation
+#    files (eq: SEMO's own incompressiblePressureBased, its equation calls on
+#    the GPU) or of those and pressureBased.cpp too (step: the whole step on
+#    the GPU, fields resident), against semo_mesh_amd/libsemo_gpu.so.
 REF ?= /root/reference
 HERE := $(dir $(lastword $(MAKEFILE_LIST)))
 OUT := $(HERE)_ref
@@ -36,7 +41,14 @@ OBJS := $(foreach s,$(SRCS),$(OBJ)/$(subst /,_,$(s)).o)
 SOLVE1 := _ZN20SEMO_Solvers_Builder10solveAMGCLENSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEER17SEMO_Mesh_BuilderiRSt6vectorIiSaIiEESB_RS8_IdSaIdEESE_SE_
 SOLVE3 := $(SOLVE1)SE_SE_SE_SE_
 
-all: $(OUT)/semo_ref $(OUT)/rcm_ref $(OUT)/lib/.stamp
+BRIDGE := $(HERE)../semo_mesh_amd/bridge
+GPULIB := $(HERE)../semo_mesh_amd/libsemo_gpu.so
+EQ_OBJS := $(OBJ)/solvers_eqMomentum.o $(OBJ)/solvers_eqPressure.o $(OBJ)/solvers_eqVolfrac.o
+BOBJS_EQ := $(filter-out $(EQ_OBJS),$(OBJS))
+BOBJS_STEP := $(filter-out $(OBJ)/solvers_pressureBased.o,$(BOBJS_EQ))
+MPILIBS := $(MPIDIR)/lib/libmpicxx.so.12 $(MPIDIR)/lib/libmpi.so.12
+
+all: $(OUT)/semo_ref $(OUT)/rcm_ref $(OUT)/lib/.stamp $(OUT)/semo_gpu_eq $(OUT)/semo_gpu_step
 
 $(OBJ)/.dir:
 	mkdir -p $(OBJ) && touch $@
@@ -54,6 +66,21 @@ $(OUT)/semo_ref: $(OBJS) $(OBJ)/ref_driver.o
 	$(CXX) -o $@ $^ -Wl,--wrap=$(SOLVE1) -Wl,--wrap=$(SOLVE3) \
 	  $(MPIDIR)/lib/libmpicxx.so.12 $(MPIDIR)/lib/libmpi.so.12 -lz \
 	  -Wl,-rpath,'$$ORIGIN/lib' -Wl,--allow-shlib-undefined
+
+$(OBJ)/gpuBridge.o: $(BRIDGE)/gpuBridge.cpp $(BRIDGE)/gpuBridge.h $(HERE)../include/semo_gpu.h $(OBJ)/.dir
+	$(CXX) $(CXXFLAGS) -I$(HERE)../include -I$(BRIDGE) -c $< -o $@
+
+$(OBJ)/gpuBridgeStep.o: $(BRIDGE)/gpuBridgeStep.cpp $(BRIDGE)/gpuBridge.h $(HERE)../include/semo_gpu.h $(OBJ)/.dir
+	$(CXX) $(CXXFLAGS) -I$(HERE)../include -I$(BRIDGE) -c $< -o $@
+
+$(OUT)/semo_gpu_eq: $(BOBJS_EQ) $(OBJ)/gpuBridge.o $(OBJ)/ref_driver.o $(GPULIB)
+	$(CXX) -o $@ $(BOBJS_EQ) $(OBJ)/gpuBridge.o $(OBJ)/ref_driver.o -Wl,--wrap=$(SOLVE1) -Wl,--wrap=$(SOLVE3) \
+	  $(GPULIB) $(MPILIBS) -lz -Wl,-rpath,'$$ORIGIN/lib:$$ORIGIN/../../semo_mesh_amd' -Wl,--allow-shlib-undefined
+
+$(OUT)/semo_gpu_step: $(BOBJS_STEP) $(OBJ)/gpuBridge.o $(OBJ)/gpuBridgeStep.o $(OBJ)/ref_driver.o $(GPULIB)
+	$(CXX) -o $@ $(BOBJS_STEP) $(OBJ)/gpuBridge.o $(OBJ)/gpuBridgeStep.o $(OBJ)/ref_driver.o \
+	  -Wl,--wrap=$(SOLVE1) -Wl,--wrap=$(SOLVE3) \
+	  $(GPULIB) $(MPILIBS) -lz -Wl,-rpath,'$$ORIGIN/lib:$$ORIGIN/../../semo_mesh_amd' -Wl,--allow-shlib-undefined
 
 $(OBJ)/rcm_ref_main.o: $(HERE)rcm_ref_main.cpp $(OBJ)/.dir
 	$(CXX) $(CXXFLAGS) -c $< -o $@

@@ -69,23 +69,26 @@ class Case:
 
     def write_ref_case(self, path: str, steps: int = 1, correctors: int = 1,
                        init: str | None = None) -> None:
+        """The case file of oracle/ref_driver.cpp (numbers as shortest
+        round-trip decimals, read back exactly by operator>>)."""
+        r = lambda x: repr(float(x))  # noqa: E731
         with open(path, "w") as fh:
-            fh.write(f"dt {self.dt!r}\nrho {self.rho[0]!r} {self.rho[1]!r}\n")
-            fh.write(f"mu {self.mu[0]!r} {self.mu[1]!r}\nsigma {self.sigma!r}\n")
-            fh.write("g %r %r %r\n" % tuple(self.g))
-            fh.write(f"urf {self.urf_mom!r} {self.urf_p!r} {self.urf_pu!r}\n")
-            fh.write(f"limit {self.limit!r}\nsteps {steps}\ncorrectors {correctors}\n")
+            fh.write(f"dt {r(self.dt)}\nrho {r(self.rho[0])} {r(self.rho[1])}\n")
+            fh.write(f"mu {r(self.mu[0])} {r(self.mu[1])}\nsigma {r(self.sigma)}\n")
+            fh.write("g %s %s %s\n" % tuple(r(x) for x in self.g))
+            fh.write(f"urf {r(self.urf_mom)} {r(self.urf_p)} {r(self.urf_pu)}\n")
+            fh.write(f"limit {r(self.limit)}\nsteps {int(steps)}\ncorrectors {int(correctors)}\n")
             if self.n_vof:
-                fh.write(f"vof {self.n_vof}\nurfvf {self.urf_vf!r}\n")
+                fh.write(f"vof {int(self.n_vof)}\nurfvf {r(self.urf_vf)}\n")
             if self.adjust_dt:
-                fh.write(f"adjustdt 1 {self.max_co!r} {self.max_dt!r}\n")
+                fh.write(f"adjustdt 1 {r(self.max_co)} {r(self.max_dt)}\n")
             if init:
                 fh.write(f"init {init}\n")
             for name, b in self.bcs.items():
-                fh.write("bc %s %s %r %s %r %r %r %s %r\n" % (
-                    name, BC_NAMES[b.p_type], b.p_value, BC_NAMES[b.u_type],
-                    b.u_value[0], b.u_value[1], b.u_value[2],
-                    BC_NAMES[b.vf_type], b.vf_value))
+                fh.write("bc %s %s %s %s %s %s %s %s %s\n" % (
+                    name, BC_NAMES[b.p_type], r(b.p_value), BC_NAMES[b.u_type],
+                    r(b.u_value[0]), r(b.u_value[1]), r(b.u_value[2]),
+                    BC_NAMES[b.vf_type], r(b.vf_value)))
 
 
 def cavity_case(dt: float = 0.01, lid=(1.0, 0.0, 0.0)) -> Case:

@@ -211,6 +211,8 @@ def test_darcy_term_definition():
         g.set_case(case)
         for f, v in state.items():
             g.set_field(f, v)
+        g.set_field("Rho", np.full(n, case.rho[0]))     # as the first step's EOS would
+        g.set_field("mu", np.full(n, case.mu[0]))
         g.set_field("alphaP", alpha)
         g.assemble("momentum")
         out.append(g.matrix("momentum"))
