@@ -46,6 +46,7 @@ def run_path(mesh, case, precision, steps, warmup, profile_kernels=True, cheb_de
     for _ in range(warmup):                       # the first one builds the AMG hierarchy
         g.incompressiblePressureBased(1)
     t_warm = time.perf_counter() - t0
+    amg_setup_ms = g.solve_stats("pressure")["setup_ms"]
     # timed pass: the library as a caller runs it (V-cycles replayed as graphs)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -65,7 +66,8 @@ def run_path(mesh, case, precision, steps, warmup, profile_kernels=True, cheb_de
         prof = g.profile_stats()
         g.profile(False)
     g.close()
-    return dict(seconds=t, upload_s=t_upload, warmup_s=t_warm, prof=prof, fields=fields, seconds_kernel_pass=t_prof)
+    return dict(seconds=t, upload_s=t_upload, warmup_s=t_warm, amg_setup_ms=amg_setup_ms, prof=prof, fields=fields,
+                seconds_kernel_pass=t_prof)
 
 
 def dam_break_leg(steps, warmup, nx=96):
@@ -93,11 +95,12 @@ def dam_break_leg(steps, warmup, nx=96):
     g.set_field("VF", vf)
     for _ in range(warmup):
         g.incompressiblePressureBased(1)
-    vf0 = float(g.get_field("VF").sum())
+    m0, vol = g.volume_integral("VF")          # sum alpha_i V_i, FP64 on the device
     t0 = time.perf_counter()
     for _ in range(steps):
         g.incompressiblePressureBased(1)
     t = time.perf_counter() - t0
+    m1, _ = g.volume_integral("VF")
     vf1 = g.get_field("VF")
     g.close()
     return {"workload": f"VOF dam break, {nx}x{nx // 2}x{nx // 2} cubes x 6 tets ({nc} cells), "
@@ -105,7 +108,9 @@ def dam_break_leg(steps, warmup, nx=96):
             "cells": nc, "steps": steps, "warmup": warmup,
             "value": round(nc * steps / t / 1e6, 3), "unit": "Mcell-step/s",
             "ms_per_step": round(1e3 * t / steps, 3), "mesh_s": round(t_mesh, 1),
-            "alpha_sum_change_rel": float(abs(vf1.sum() - vf0) / vf0),
+            "water_volume_m3": [m0, m1], "domain_volume_m3": vol,
+            "water_volume_change_rel": abs(m1 - m0) / m0,
+            "mass_check": "sum alpha_i V_i in FP64, reduced on the device, before and after the timed steps",
             "alpha_min": float(vf1.min()), "alpha_max": float(vf1.max())}
 
 
@@ -120,7 +125,10 @@ def adjoint_leg(n=64, n_iter=20, reps=3, fd_cells=32):
     central differences at fd_cells seeded cells with FP64 solves at 1e-14,
     each cell's quotient at h = 1 and h/2 (their spread bounds the truncation
     error; Richardson (4 D(h/2) - D(h)) / 3 removes the h^2 term) and at the
-    configuration's h = 1e-6.  At rho 1000 the Darcy term (alpha_p <= 10) is
+    configuration's h = 1e-6; and directional derivatives dJ/dalpha . d over
+    three seeded whole-field directions d, against central differences at
+    h = 1 and 1/2 with Richardson.  The torch.autograd path (forward +
+    backward on a CUDA tensor) is timed too.  At rho 1000 the Darcy term (alpha_p <= 10) is
     a 1e-4 perturbation of the momentum diagonal 1.5 rho V / dt, so a
     per-cell dJ/dalpha_p is ~1e-12 against J ~ 1e-3: J's round-off (~1e-17)
     swamps a quotient at h = 1e-6 (it resolves ~1e-5 of it), while J is
@@ -166,6 +174,7 @@ def adjoint_leg(n=64, n_iter=20, reps=3, fd_cells=32):
 
     t, t_fwd = timed("fp64")
     t_mx, t_fwd_mx = timed("mixed")
+    t_torch = torch_path(m, alpha, n_iter, W, ctx, reset, reps)
     # gradient check: one context at tight tolerance, reused for every J
     g = ctx("fp64", 1e-14)
     reset(g)
@@ -187,6 +196,22 @@ def adjoint_leg(n=64, n_iter=20, reps=3, fd_cells=32):
             e[c] = step
             q.append((J_of(alpha + e) - J_of(alpha - e)) / (2 * step))
         rows.append([int(c), float(grad[c])] + [float(x) for x in q])
+        if len(rows) % 8 == 0:
+            print(f"[bench] adjoint fd check: {len(rows)}/{fd_cells} cells", file=sys.stderr, flush=True)
+    # directional derivatives: dJ/dalpha . d against central differences of
+    # J(alpha + h d) over whole-field directions d, whose J changes are large
+    # against J's round-off (a per-cell change is not)
+    rng = np.random.default_rng(11)
+    u = rng.random(m.n_cells)
+    dirs = {"U[0,1)": u, "U[-1,1)": 2.0 * rng.random(m.n_cells) - 1.0,
+            "sign(dJ) U[0,1)": np.sign(grad) * rng.random(m.n_cells)}
+    drows = {}
+    for name, d in dirs.items():
+        q = [(J_of(alpha + hh * d) - J_of(alpha - hh * d)) / (2 * hh) for hh in (1.0, 0.5)]
+        gd = float(grad @ d)
+        rich = (4.0 * q[1] - q[0]) / 3.0
+        drows[name] = {"adjoint": gd, "fd_h1": q[0], "fd_h0.5": q[1], "rel_err_richardson": abs(gd - rich) / abs(rich),
+                       "rel_err_h0.5": abs(gd - q[1]) / abs(q[1])}
     g.close()
     r = np.array(rows)
     adj, d1, d2, d6 = r[:, 1], r[:, 2], r[:, 3], r[:, 4]
@@ -208,7 +233,35 @@ def adjoint_leg(n=64, n_iter=20, reps=3, fd_cells=32):
                          "max_rel_err_h_1e-6": float(np.max(np.abs(adj - d6) / scale)),
                          "max_rel_spread_h_vs_h_half": float(np.max(np.abs(d1 - d2) / scale)),
                          "per_cell": "[cell, adjoint, D(1), D(0.5), D(1e-6)]",
-                         "rows": rows}}
+                         "rows": rows,
+                         "directional": drows,
+                         "max_rel_err_directional": max(v["rel_err_richardson"] for v in drows.values())},
+            "torch_autograd": {"grads_per_s": round(1.0 / t_torch, 3), "ms_per_gradient": round(1e3 * t_torch, 1),
+                               "path": "semo_mesh_amd.autograd.pressure_objective on a CUDA tensor, forward + "
+                                       "backward, one persistent solver (FP64 to 1e-8)"}}
+
+
+def torch_path(m, alpha, n_iter, W, ctx, reset, reps):
+    """Wall time of one gradient through the torch.autograd Function: J and
+    J.backward() on a CUDA tensor, one persistent solver."""
+    import torch
+    from semo_mesh_amd.autograd import pressure_objective
+    g = ctx("fp64", 1e-8)
+    a = torch.tensor(alpha, dtype=torch.float64, device="cuda", requires_grad=True)
+    try:
+        times = []
+        for r in range(reps + 1):                # the first builds the AMG hierarchy
+            a.grad = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            J = pressure_objective(a, g, n_iter, W, reset)
+            J.backward()
+            torch.cuda.synchronize()
+            if r:
+                times.append(time.perf_counter() - t0)
+    finally:
+        g.close()
+    return min(times)
 
 
 def coupled_leg(nx=256, ny=64, nz=64, n_iter=10, reps=2):
@@ -277,7 +330,10 @@ def cpu_baseline(n=48):
                 return dict(value=m.n_cells / run_s / 1e6, unit="Mcell-iter/s", cores=1, kind="reference",
                             sample=f"compiled SEMO+AMGCL, 1 rank, one SIMPLE iteration of the jittered "
                                    f"{n}^3 cavity ({m.n_cells} cells), {run_s:.2f} s; mesh geometry / "
-                                   f"LSQ / RCM set-up excluded")
+                                   f"LSQ / RCM set-up excluded",
+                            extrapolated=f"the per-cell rate of the {n}^3 sample stands for the 128^3 "
+                                         f"configuration (not timed there: the reference has no OpenMP, "
+                                         f"and its multi-rank path needs a prebuilt METIS)")
         # fallback: the repo's CPU oracle (single thread)
         from oracle.oracle import Oracle
         o = Oracle(m, cavity_case())
@@ -321,6 +377,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(domain decomposition over RCCL is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if not args.no_adjoint:
+        # the adjoint leg times the torch.autograd path: torch's libraries
+        # must be in the process before libsemo_gpu.so
+        import semo_mesh_amd.autograd  # noqa: F401
 
     from semo_mesh_amd.cases import cavity_case
     from semo_mesh_amd.mesh import cavity_mesh
@@ -350,6 +410,11 @@ def main():
                                "1 SIMPLE iteration per step, solves to 1e-8",
                    "cells": nc, "faces": mesh.n_faces, "seed": 20261202},
     }
+    out["setup"] = {"upload_geometry_rcm_s": round(resmx["upload_s"], 3),
+                    "first_steps_s": round(resmx["warmup_s"], 3), "first_steps": args.warmup,
+                    "amg_setup_ms": round(resmx["amg_setup_ms"], 1),
+                    "note": "one-time: mesh upload, RCM, geometry and LSQ set-up on the device; the first "
+                            "steps build the AMG hierarchy on the host (single thread) and capture the V-cycle graph"}
     if res64 is not None:
         v64 = nc * args.steps / res64["seconds"] / 1e6
         out["fp64"] = {"value": round(v64, 3), "ms_per_step": round(1e3 * res64["seconds"] / args.steps, 3)}
@@ -378,7 +443,19 @@ def main():
                            "bytes_per_launch": per_launch, "launches": st["count"],
                            "timing": "HIP event pair per launch inside the steps"}
         out["kernels"] = {k: {"count": v["count"], "ms": round(v["ms"], 3),
+                              "GB_per_step": round(v["bytes"] / args.steps / 1e9, 4),
                               "GBs": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()}
+        # the whole step against the HBM roofline: algorithmic bytes of every
+        # kernel of a step (every launch is tagged) over the timed step
+        step_bytes = sum(v["bytes"] for v in prof.values()) / args.steps
+        tagged_ms = sum(v["ms"] for v in prof.values()) / args.steps
+        out["hbm_per_step"] = {
+            "algorithmic_GB": round(step_bytes / 1e9, 3),
+            "frac_over_step": round(step_bytes / (out["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "kernel_ms": round(tagged_ms, 3),
+            "frac_over_kernel_time": round(step_bytes / (tagged_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "note": "kernel_ms: sum of the per-launch HIP event times of the kernel pass (launches issued "
+                    "one by one); ms_per_step - kernel_ms is host time and launch gaps"}
         traffic = _measured_traffic(name)
         if traffic:
             out["roofline"]["traffic"] = traffic["bytes"]

@@ -275,6 +275,12 @@ int semo_adjoint_alpha_p(semo_ctx* ctx, int n_iter, const double* patch_weight, 
  * reference; the objective of semo_adjoint_alpha_p.                          */
 int semo_objective_pressure(semo_ctx* ctx, const double* patch_weight, double* J);
 
+/* sum_i field_i V_i over the cells in FP64, reduced on the device (the
+ * same value on every run); *volume (optional) gets sum_i V_i.  No
+ * counterpart in the reference: the alpha mass balance of two-phase runs
+ * (field SEMO_F_VF: the first phase's volume).                               */
+int semo_volume_integral(semo_ctx* ctx, int field, double* integral, double* volume);
+
 /* Statistics of the last momentum (eq 0) / pressure (eq 1) / VOF (eq 2) solve. */
 int semo_get_solve_stats(const semo_ctx* ctx, int eq, semo_solve_stats* st);
 

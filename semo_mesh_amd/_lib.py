@@ -72,6 +72,7 @@ EXPORTED = [
     "semo_calc_momentum_eqs", "semo_calc_pressure_eq", "semo_calc_volfrac_eq", "semo_calc_time_step",
     "semo_calc_source_gravity", "semo_calc_source_surface_tension",
     "semo_incompressible_pressure_based", "semo_adjoint_alpha_p", "semo_objective_pressure",
+    "semo_volume_integral",
     "semo_calc_coupled_eq", "semo_coupled_pressure_based", "semo_coupled_matrix_get", "semo_coupled_spmv", "semo_coupled_solve",
     "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_least_square_noalloc", "semo_cell_min_max", "semo_gauss_green", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
@@ -112,6 +113,7 @@ def lib():
         "semo_incompressible_pressure_based": (C.c_int, [vp, C.c_int, dp]),
         "semo_adjoint_alpha_p": (C.c_int, [vp, C.c_int, dp, dp, dp]),
         "semo_objective_pressure": (C.c_int, [vp, dp, dp]),
+        "semo_volume_integral": (C.c_int, [vp, C.c_int, dp, dp]),
         "semo_calc_coupled_eq": (C.c_int, [vp, dp, vp]),
         "semo_coupled_pressure_based": (C.c_int, [vp, C.c_int, dp]),
         "semo_coupled_matrix_get": (C.c_int, [vp, vp, vp, vp, dp, dp]),

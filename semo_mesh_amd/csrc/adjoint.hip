@@ -580,7 +580,7 @@ struct AdjBufs {
 namespace {
 
 void dcopy(semo_ctx* c, double* d, const double* s, size_t n) {
-    if (n) hipLaunchKernelGGL(k_adj_copy, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, c->stream, n, d, s);
+    if (n) SEMO_LAUNCH(c, "adj_copy", 16.0 * n, k_adj_copy, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), n, d, s);
 }
 
 void check_stencil_symmetric(semo_ctx* c, AdjBufs& B) {
@@ -630,9 +630,10 @@ void lsq_recon_reverse(semo_ctx* c, AdjBufs& B, const double* U, const double* V
     int n = c->nc;
     dim3 gb(nblk(n)), tb(TPB);
     int nb = with_p ? 4 : 3;
-    hipLaunchKernelGGL(k_adj_argminmax, gb, tb, 0, c->stream, n, nb, c->d_st_ptr.p, c->d_st_idx.p, U, V, W, P,
-                       B.amx.p, B.amn.p);
-    hipLaunchKernelGGL(k_adj_lsq_h, gb, tb, 0, c->stream, n, 4, c->c_lsq.p, B.gUb.p, B.gVb.p, B.gWb.p, B.gPb.p);
+    SEMO_LAUNCH(c, "adj_argminmax", 4.0 * n + 4.0 * (double)c->d_st_idx.n + 16.0 * nb * n, k_adj_argminmax, gb, tb, n,
+                nb, c->d_st_ptr.p, c->d_st_idx.p, U, V, W, P, B.amx.p, B.amn.p);
+    SEMO_LAUNCH(c, "adj_lsq_h", 48.0 * n + 192.0 * n, k_adj_lsq_h, gb, tb, n, 4, c->c_lsq.p, B.gUb.p, B.gVb.p,
+                B.gWb.p, B.gPb.p);
     LsqT t;
     size_t N = n;
     t.h[0] = B.gUb.p; t.h[1] = B.gVb.p; t.h[2] = B.gWb.p; t.h[3] = B.gPb.p;
@@ -640,10 +641,13 @@ void lsq_recon_reverse(semo_ctx* c, AdjBufs& B, const double* U, const double* V
     for (int v = 0; v < 4; ++v) { t.mxb[v] = m + 2 * v * N; t.mnb[v] = m + (2 * v + 1) * N; }
     t.amx = B.amx.p; t.amn = B.amn.p;
     t.out[0] = uB; t.out[1] = vB; t.out[2] = wB; t.out[3] = pB;
-    hipLaunchKernelGGL(k_adj_lsq_t, gb, tb, 0, c->stream, n, c->nif, c->nbf, 4, nb, c->d_st_ptr.p, c->d_st_idx.p,
-                       c->d_bf_ptr.p, c->d_bf_idx.p, c->c_xyz.p, c->f_xyz.p, t, B.bURb.p, B.bPb.p);
-    hipLaunchKernelGGL(k_adj_recon, gb, tb, 0, c->stream, n, c->nif, c->nbf, c->d_bf_ptr.p, c->d_bf_idx.p,
-                       c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, U, V, W, B.bULb.p, B.bURb.p, B.bPb.p, uB, vB, wB, pB);
+    SEMO_LAUNCH(c, "adj_lsq_t",
+                4.0 * n + 4.0 * (double)c->d_st_idx.n + 24.0 * n + 96.0 * n + 64.0 * n + 44.0 * c->nbf, k_adj_lsq_t,
+                gb, tb, n, c->nif, c->nbf, 4, nb, c->d_st_ptr.p, c->d_st_idx.p, c->d_bf_ptr.p, c->d_bf_idx.p,
+                c->c_xyz.p, c->f_xyz.p, t, B.bURb.p, B.bPb.p);
+    SEMO_LAUNCH(c, "adj_recon", 4.0 * n + 160.0 * c->nbf + 64.0 * n, k_adj_recon, gb, tb, n, c->nif, c->nbf,
+                c->d_bf_ptr.p, c->d_bf_idx.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, U, V, W, B.bULb.p, B.bURb.p,
+                B.bPb.p, uB, vB, wB, pB);
 }
 
 void zero_stage(semo_ctx* c, AdjBufs& B) {
@@ -701,8 +705,8 @@ double adjoint_run(semo_ctx* c, int n_iter, const double* wpatch_h, double* grad
     // objective of the final state
     double J = 0.0;
     if (nbf) {
-        hipLaunchKernelGGL(k_adj_objective, dim3(nblk(nbf)), tb, 0, c->stream, nbf, nif, c->d_bpatch.p, c->d_bcs.p,
-                           B.wpatch.p, c->d_owner.p, c->f_area.p, c->P.p, B.jf.p);
+        SEMO_LAUNCH(c, "adj_objective", 40.0 * nbf, k_adj_objective, dim3(nblk(nbf)), tb, nbf, nif, c->d_bpatch.p,
+                    c->d_bcs.p, B.wpatch.p, c->d_owner.p, c->f_area.p, c->P.p, B.jf.p);
         std::vector<double> h(nbf);
         SEMO_HIP(hipMemcpyAsync(h.data(), B.jf.p, nbf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         SEMO_HIP(hipStreamSynchronize(c->stream));
@@ -714,8 +718,8 @@ double adjoint_run(semo_ctx* c, int n_iter, const double* wpatch_h, double* grad
     dcopy(c, fin.p + 3 * N, c->P.p, N);
     // ---- seed ----
     for (DBuf<double>* b : {&B.aU, &B.aV, &B.aW, &B.nU, &B.nV, &B.nW, &B.nP, &B.alphab}) b->zero(c->stream);
-    hipLaunchKernelGGL(k_adj_seed, gb, tb, 0, c->stream, n, nif, c->d_bf_ptr.p, c->d_bf_idx.p, c->d_bpatch.p,
-                       c->d_bcs.p, B.wpatch.p, c->f_area.p, B.aP.p);
+    SEMO_LAUNCH(c, "adj_seed", 40.0 * nbf + 8.0 * n, k_adj_seed, gb, tb, n, nif, c->d_bf_ptr.p, c->d_bf_idx.p,
+                c->d_bpatch.p, c->d_bcs.p, B.wpatch.p, c->f_area.p, B.aP.p);
     // ---- reverse sweep ----
     for (int it = n_iter - 1; it >= 0; --it) {
         const double* ck = B.ck.p + 4 * N * (size_t)it;
@@ -743,17 +747,17 @@ double adjoint_run(semo_ctx* c, int n_iter, const double* wpatch_h, double* grad
         recon_boundary(c);
         assemble_pressure(c);
         // -- reverse of the p / U correction and the two Green-Gauss passes --
-        hipLaunchKernelGGL(k_adj_pcorrect, gb, tb, 0, c->stream, n, k, c->Rho.p, nx + 3 * N, nx, nx + N, nx + 2 * N,
-                           B.aP.p, B.aU.p, B.aV.p, B.aW.p, B.ppb.p, B.nP.p, B.u1U.p, B.u1V.p, B.u1W.p, B.g2b.p);
+        SEMO_LAUNCH(c, "adj_pcorrect", 128.0 * n, k_adj_pcorrect, gb, tb, n, k, c->Rho.p, nx + 3 * N, nx, nx + N,
+                    nx + 2 * N, B.aP.p, B.aU.p, B.aV.p, B.aW.p, B.ppb.p, B.nP.p, B.u1U.p, B.u1V.p, B.u1W.p, B.g2b.p);
         for (int pass = 1; pass >= 0; --pass) {
             double* gcur = pass == 1 ? B.g2b.p : B.g1b.p;
             double* gprev = pass == 1 ? B.g1b.p : nullptr;
             if (nif)
-                hipLaunchKernelGGL(k_adj_gg_face, dim3(nblk(nif)), tb, 0, c->stream, nif, c->d_owner.p, c->d_nbr.p,
-                                   c->f_nrm.p, c->f_area.p, c->c_vol.p, gcur, B.vfb.p);
-            hipLaunchKernelGGL(k_adj_gg_cell, gb, tb, 0, c->stream, n, nif, c->d_cf_sl.p, c->d_cf_ell.p,
-                               c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->c_vol.p, c->f_wC.p,
-                               c->f_skew.p, gcur, B.vfb.p, B.ppb.p, gprev);
+                SEMO_LAUNCH(c, "adj_gg_face", 64.0 * nif + 32.0 * n, k_adj_gg_face, dim3(nblk(nif)), tb, nif,
+                            c->d_owner.p, c->d_nbr.p, c->f_nrm.p, c->f_area.p, c->c_vol.p, gcur, B.vfb.p);
+            SEMO_LAUNCH(c, "adj_gg_cell", cell_faces_bytes(c, 72.0) + 80.0 * n, k_adj_gg_cell, gb, tb, n, nif,
+                        c->d_cf_sl.p, c->d_cf_ell.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->c_vol.p,
+                        c->f_wC.p, c->f_skew.p, gcur, B.vfb.p, B.ppb.p, gprev);
         }
         // pressure adjoint: the Laplacian is symmetric
         solve_pressure(c, B.ppb.p, B.lam.p, &st);
@@ -761,28 +765,28 @@ double adjoint_run(semo_ctx* c, int n_iter, const double* wpatch_h, double* grad
         zero_stage(c, B);
         {
             FaceIn in = stage_in(c, B.U1.p, B.U1.p + N, B.U1.p + 2 * N, P0, nullptr, nullptr);
-            hipLaunchKernelGGL(k_adj_p_cell, gb, tb, 0, c->stream, n, nif, nbf, k, in, c->d_cf_sl.p, c->d_cf_ell.p,
-                               c->d_bpatch.p, c->d_bcs.p, c->bRho.p, B.lam.p,
-                               stage_out(c, B, B.u1U.p, B.u1V.p, B.u1W.p, B.nP.p, false));
+            SEMO_LAUNCH(c, "adj_p_cell", cell_faces_bytes(c, 64.0) + 96.0 * n, k_adj_p_cell, gb, tb, n, nif, nbf, k,
+                        in, c->d_cf_sl.p, c->d_cf_ell.p, c->d_bpatch.p, c->d_bcs.p, c->bRho.p, B.lam.p,
+                        stage_out(c, B, B.u1U.p, B.u1V.p, B.u1W.p, B.nP.p, false));
         }
         lsq_recon_reverse(c, B, B.U1.p, B.U1.p + N, B.U1.p + 2 * N, P0, false, B.u1U.p, B.u1V.p, B.u1W.p, B.nP.p);
         // momentum update reverse -> transposed momentum solves
-        hipLaunchKernelGGL(k_adj_mom_update, gb, tb, 0, c->stream, n, k, B.U1.p, B.U1.p + N, B.U1.p + 2 * N,
-                           B.u1U.p, B.u1V.p, B.u1W.p, c->Bvec.p, B.nU.p, B.nV.p, B.nW.p);
+        SEMO_LAUNCH(c, "adj_mom_update", 80.0 * n, k_adj_mom_update, gb, tb, n, k, B.U1.p, B.U1.p + N, B.U1.p + 2 * N,
+                    B.u1U.p, B.u1V.p, B.u1W.p, c->Bvec.p, B.nU.p, B.nV.p, B.nW.p);
         dcopy(c, c->Amat.val.p, B.mval.p, c->Amat.val.n);
         if (nif)
-            hipLaunchKernelGGL(k_adj_transpose, dim3(nblk(nif)), tb, 0, c->stream, nif, c->d_slot_own.p,
-                               c->d_slot_nbr.p, B.mval.p, c->Amat.val.p);
+            SEMO_LAUNCH(c, "adj_transpose", 40.0 * nif, k_adj_transpose, dim3(nblk(nif)), tb, nif, c->d_slot_own.p,
+                        c->d_slot_nbr.p, B.mval.p, c->Amat.val.p);
         dcopy(c, c->Amat.diag.p, B.mdiag.p, N);
         solve_momentum(c, 3, &st);
         // momentum assembly reverse into U / P / alpha_p
         zero_stage(c, B);
         {
             FaceIn in = stage_in(c, U0, V0, W0, P0, B.mg.p, B.mbnd.p);
-            hipLaunchKernelGGL(k_adj_mom_cell, gb, tb, 0, c->stream, n, nif, nbf, k, in, c->d_cf_sl.p,
-                               c->d_cf_ell.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p, c->Mu.p, c->alphaP.p,
-                               B.mbUL.p, B.mbUR.p, B.mbP.p, c->bMu.p, c->bRho.p, c->Xvec.p, B.mX.p,
-                               stage_out(c, B, B.nU.p, B.nV.p, B.nW.p, B.nP.p, true), B.alphab.p);
+            SEMO_LAUNCH(c, "adj_mom_cell", cell_faces_bytes(c, 96.0) + 200.0 * n, k_adj_mom_cell, gb, tb, n, nif, nbf,
+                        k, in, c->d_cf_sl.p, c->d_cf_ell.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p, c->Mu.p,
+                        c->alphaP.p, B.mbUL.p, B.mbUR.p, B.mbP.p, c->bMu.p, c->bRho.p, c->Xvec.p, B.mX.p,
+                        stage_out(c, B, B.nU.p, B.nV.p, B.nW.p, B.nP.p, true), B.alphab.p);
         }
         lsq_recon_reverse(c, B, U0, V0, W0, P0, true, B.nU.p, B.nV.p, B.nW.p, B.nP.p);
         // the start adjoint becomes the next (earlier) iteration's end adjoint
@@ -804,8 +808,8 @@ double objective(semo_ctx* c, const double* wpatch_h) {
     if (!nbf) return 0.0;
     B.jf.ensure(nbf);
     B.wpatch.upload(std::vector<double>(wpatch_h, wpatch_h + c->npatch), c->stream);
-    hipLaunchKernelGGL(k_adj_objective, dim3(nblk(nbf)), dim3(TPB), 0, c->stream, nbf, c->nif, c->d_bpatch.p,
-                       c->d_bcs.p, B.wpatch.p, c->d_owner.p, c->f_area.p, c->P.p, B.jf.p);
+    SEMO_LAUNCH(c, "adj_objective", 40.0 * nbf, k_adj_objective, dim3(nblk(nbf)), dim3(TPB), nbf, c->nif,
+                c->d_bpatch.p, c->d_bcs.p, B.wpatch.p, c->d_owner.p, c->f_area.p, c->P.p, B.jf.p);
     std::vector<double> h(nbf);
     SEMO_HIP(hipMemcpyAsync(h.data(), B.jf.p, nbf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SEMO_HIP(hipStreamSynchronize(c->stream));

@@ -1300,6 +1300,20 @@ __global__ void __launch_bounds__(TPB) k_mom_update(int nc, CaseK k, const doubl
     block_sum2(nd, np, part);
 }
 
+// block partials of sum phi_i V_i and of sum V_i
+__global__ void __launch_bounds__(TPB) k_vol_integral(int nc, const double* __restrict__ phi,
+                                                      const double* __restrict__ vol, double* __restrict__ part) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    double a = 0.0, v = 0.0;
+    if (i < nc) { v = vol[i]; a = phi[i] * v; }
+    block_sum2(a, v, part);
+}
+
+__global__ void k_zero(size_t n, uint32_t* __restrict__ d) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = 0u;
+}
+
 __global__ void k_copy(size_t n, double* __restrict__ d, const double* __restrict__ s) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = s[i];
@@ -1315,17 +1329,22 @@ double finish_ratio(semo_ctx* c, int nb) {
     return sqrt(a) / sqrt(b);
 }
 
-struct ProfScope {
-    semo_ctx* c; const char* name; double bytes; hipEvent_t a{};
-    ProfScope(semo_ctx* c_, const char* n, double b) : c(c_), name(n), bytes(b) {
-        if (c->prof.on) c->prof.begin(c->stream, a);
-    }
-    ~ProfScope() { if (c->prof.on) c->prof.end(c->stream, name, a, bytes); }
-};
-
 }  // namespace
 
 // ---------------------------------------------------------------- drivers --
+// algorithmic bytes of one launch (each operand read once, each result
+// written once; face data counted once although a cell-loop kernel reaches
+// an internal face from both of its cells)
+namespace {
+// k_lsq1: stencil indices, the neighbours' [x, y, z, phi] records, the LSQ
+// coefficients, the gradient written, the boundary faces' centroid and value
+double lsq1_bytes(const semo_ctx* c) {
+    double n = c->nc, S = (double)c->d_st_idx.n, nb = c->nbf;
+    return (c->d_st_dl.n ? 2.0 : 4.0) * S + 4.0 * (n / 64 + 1) + 32.0 * n + 48.0 * n + 24.0 * n + 4.0 * n +
+           nb * (4.0 + 24.0 + 8.0);
+}
+}  // namespace
+
 void fields_alloc(semo_ctx* c) {
     size_t n = c->nc, nb = c->nbf;
     for (DBuf<double>* b : {&c->P, &c->U, &c->V, &c->W, &c->VF, &c->Rho, &c->Mu, &c->oldP, &c->oldU,
@@ -1346,26 +1365,26 @@ void fields_alloc(semo_ctx* c) {
     c->VF.upload(one, c->stream);
     c->oldVF.upload(one, c->stream);
     c->cpk.alloc(8 * n);
-    hipLaunchKernelGGL(k_pack_xyz, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->c_xyz.p, c->cpk.p);
+    SEMO_LAUNCH(c, "setup_pack_xyz", 88.0 * c->nc, k_pack_xyz, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->c_xyz.p,
+                c->cpk.p);
     c->Qm3.alloc(n); c->Qm3.zero(c->stream);
     c->fVFL.alloc(std::max(c->nif, 1)); c->fVFR.alloc(std::max(c->nif, 1));
     c->c_maxA.alloc(n);
-    hipLaunchKernelGGL(k_cell_max_area, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->d_cf_sl.p,
-                       c->d_cf_ell.p, c->f_area.p, c->c_maxA.p);
+    SEMO_LAUNCH(c, "setup_cell_max_area", cell_faces_bytes(c, 8.0) + 8.0 * c->nc, k_cell_max_area, dim3(nblk(c->nc)),
+                dim3(TPB), c->nc, c->d_cf_sl.p, c->d_cf_ell.p, c->f_area.p, c->c_maxA.p);
     SEMO_HIP(hipStreamSynchronize(c->stream));   // host sources above are temporaries
 }
 
 void eos_transport(semo_ctx* c) {
     CaseK k = case_k(c);
-    ProfScope ps(c, "eos", 24.0 * c->nc);
-    hipLaunchKernelGGL(k_eos, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, k, c->VF.p, c->Rho.p, c->Mu.p);
+    SEMO_LAUNCH(c, "eos", 32.0 * c->nc, k_eos, dim3(nblk(c->nc)), dim3(TPB), c->nc, k, c->VF.p, c->Rho.p, c->Mu.p);
 }
 
 // cell.var[C] as calcCellEOSVF leaves it at load
 void sound_speed(semo_ctx* c) {
     c->Csnd.alloc(c->nc);
-    hipLaunchKernelGGL(k_sound_speed, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, case_k(c), c->VF.p,
-                       c->Csnd.p);
+    SEMO_LAUNCH(c, "sound_speed", 16.0 * c->nc, k_sound_speed, dim3(nblk(c->nc)), dim3(TPB), c->nc, case_k(c),
+                c->VF.p, c->Csnd.p);
     c->csnd_valid = true;
 }
 
@@ -1376,18 +1395,18 @@ void recon_boundary(semo_ctx* c) {
         SEMO_CHECK(b.vf_type != SEMO_BC_FUNCTION || c->b_vffun.n == (size_t)c->nbf, SEMO_EINVAL_BC,
                    "volume-fraction function boundary without points (semo_set_vf_function_points)");
     CaseK k = case_k(c);
-    ProfScope ps(c, "recon_boundary", 120.0 * c->nbf);
-    hipLaunchKernelGGL(k_recon_boundary, dim3(nblk(c->nbf)), dim3(TPB), 0, c->stream, c->nbf, c->nif, k,
-                       c->d_bpatch.p, c->d_bcs.p, c->d_owner.p, c->f_nrm.p, c->f_xyz.p, c->P.p, c->U.p,
-                       c->V.p, c->W.p, c->VF.p, c->Mu.p, c->b_ufix.n ? c->b_ufix.p : nullptr, c->Csnd.p,
-                       c->b_vffun.p, c->bP.p, c->bUL.p, c->bUR.p, c->bVF.p, c->bMu.p, c->bRho.p);
+    SEMO_LAUNCH(c, "recon_boundary", 120.0 * c->nbf, k_recon_boundary, dim3(nblk(c->nbf)), dim3(TPB), c->nbf, c->nif,
+                k, c->d_bpatch.p, c->d_bcs.p, c->d_owner.p, c->f_nrm.p, c->f_xyz.p, c->P.p, c->U.p, c->V.p, c->W.p,
+                c->VF.p, c->Mu.p, c->b_ufix.n ? c->b_ufix.p : nullptr, c->Csnd.p, c->b_vffun.p, c->bP.p, c->bUL.p,
+                c->bUR.p, c->bVF.p, c->bMu.p, c->bRho.p);
 }
 
 void gauss_green_standalone(semo_ctx* c, int sel, const double* phi, const double* faceR_b, const double* gin,
                             double* gout) {
-    hipLaunchKernelGGL(k_gauss_green, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, sel, c->d_cf_sl.p,
-                       c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p,
-                       c->f_wC.p, c->f_skew.p, c->f_xyz.p, c->c_xyz.p, c->c_vol.p, phi, faceR_b, gin, gout);
+    SEMO_LAUNCH(c, "gauss_green", cell_faces_bytes(c, 8.0 + 24.0 + 8.0 + 8.0 + 24.0) + 8.0 * c->nbf + 80.0 * c->nc,
+                k_gauss_green, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, sel, c->d_cf_sl.p, c->d_cf_ell.p,
+                c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->f_wC.p, c->f_skew.p,
+                c->f_xyz.p, c->c_xyz.p, c->c_vol.p, phi, faceR_b, gin, gout);
 }
 
 void vf_function_values(semo_ctx* c, int n, const double* pts4) {
@@ -1396,29 +1415,27 @@ void vf_function_values(semo_ctx* c, int n, const double* pts4) {
     DBuf<double> p;
     p.upload(pts4, 4 * (size_t)n, c->stream);
     c->b_vffun.alloc(c->nbf);
-    hipLaunchKernelGGL(k_vf_nearest, dim3(nblk(c->nbf)), dim3(TPB), 0, c->stream, c->nbf, c->nif, c->f_xyz.p, n,
-                       p.p, c->b_vffun.p);
+    SEMO_LAUNCH(c, "vf_function_values", 32.0 * c->nbf + 32.0 * n, k_vf_nearest, dim3(nblk(c->nbf)), dim3(TPB),
+                c->nbf, c->nif, c->f_xyz.p, n, p.p, c->b_vffun.p);
     SEMO_HIP(hipStreamSynchronize(c->stream));   // p is freed on return
 }
 
 void source_gravity(semo_ctx* c) {
     CaseK k = case_k(c);
-    if (k.g0 == 0.0 && k.g1 == 0.0 && k.g2 == 0.0) { c->srcG.zero(c->stream); return; }
-    hipLaunchKernelGGL(k_gravity, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, k,
-                       c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_nbr.p, c->f_nrm.p, c->f_area.p,
-                       c->f_wC.p, c->f_pf.p, c->f_nf.p, c->c_vol.p, c->U.p, c->V.p, c->W.p, c->VF.p,
-                       c->bUL.p, c->bUR.p, c->bVF.p, c->nbf, c->srcG.p);
+    if (k.g0 == 0.0 && k.g1 == 0.0 && k.g2 == 0.0) { zero_d(c, c->srcG.p, c->srcG.bytes()); return; }
+    SEMO_LAUNCH(c, "gravity", cell_faces_bytes(c, 8.0 + 24.0 + 8.0 + 8.0 + 48.0) + 72.0 * c->nbf + 72.0 * c->nc,
+                k_gravity, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, k, c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p,
+                c->d_nbr.p, c->f_nrm.p, c->f_area.p, c->f_wC.p, c->f_pf.p, c->f_nf.p, c->c_vol.p, c->U.p, c->V.p,
+                c->W.p, c->VF.p, c->bUL.p, c->bUR.p, c->bVF.p, c->nbf, c->srcG.p);
 }
 
 void grad_minmax(semo_ctx* c, bool with_p, bool uvw_only) {
     if (uvw_only) {
-        ProfScope ps(c, "pack3", 48.0 * c->nc);
-        hipLaunchKernelGGL(k_pack3, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->U.p, c->V.p, c->W.p,
-                           c->cpk.p);
+        SEMO_LAUNCH(c, "pack3", 48.0 * c->nc, k_pack3, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->U.p, c->V.p, c->W.p,
+                    c->cpk.p);
     } else {
-        ProfScope ps(c, "pack5", 80.0 * c->nc);
-        hipLaunchKernelGGL(k_pack5, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->P.p, c->U.p, c->V.p,
-                           c->W.p, c->VF.p, c->cpk.p);
+        SEMO_LAUNCH(c, "pack5", 80.0 * c->nc, k_pack5, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->P.p, c->U.p, c->V.p,
+                    c->W.p, c->VF.p, c->cpk.p);
     }
     // algorithmic bytes of a k-field LSQ gradient: 4 S (stencil indices) +
     // the neighbours' records (read once: 64n) + k 24n (gradients written) +
@@ -1428,12 +1445,12 @@ void grad_minmax(semo_ctx* c, bool with_p, bool uvw_only) {
     double n = c->nc, S = (double)c->d_st_idx.n, k = uvw_only ? 3.0 : 5.0;
     double bytes = (narrow ? 2.0 : 4.0) * S + 64.0 * n + k * 24.0 * n + 48.0 * n + (with_p ? 64.0 : 48.0) * n +
                    (double)c->nbf * (24.0 + 8.0 * k + 4.0);
-    ProfScope ps(c, uvw_only ? "grad_minmax_uvw" : "grad_minmax", bytes);
+    const char* tag = uvw_only ? "grad_minmax_uvw" : "grad_minmax";
     auto go = [&](auto kern, const auto* idx) {
-        hipLaunchKernelGGL(kern, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf,
-                           (with_p && !uvw_only) ? 1 : 0, c->d_st_sl.p, idx, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p,
-                           c->f_xyz.p, c->c_lsq.p, c->bP.p, c->bUR.p, c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p,
-                           c->gVF.p, c->mxP.p, c->mnP.p, c->mxU.p, c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
+        SEMO_LAUNCH(c, tag, bytes, kern, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, c->nbf,
+                    (with_p && !uvw_only) ? 1 : 0, c->d_st_sl.p, idx, c->d_bf_ptr.p, c->d_bf_idx.p, c->cpk.p,
+                    c->f_xyz.p, c->c_lsq.p, c->bP.p, c->bUR.p, c->bVF.p, c->gP.p, c->gU.p, c->gV.p, c->gW.p, c->gVF.p,
+                    c->mxP.p, c->mnP.p, c->mxU.p, c->mnU.p, c->mxV.p, c->mnV.p, c->mxW.p, c->mnW.p);
     };
     if (narrow) {
         if (uvw_only) go(k_grad_minmax<3, int16_t>, c->d_st_dl.p);
@@ -1462,10 +1479,9 @@ void refresh_u_dependent_grads(semo_ctx* c) {
     BcDeps dep = bc_deps(c);
     if (!dep.p && !dep.vf) return;
     c->d_bflag.ensure(2);
-    hipLaunchKernelGGL(k_clear2, dim3(1), dim3(1), 0, c->stream, c->d_bflag.p);
-    hipLaunchKernelGGL(k_bchanged, dim3(nblk(c->nbf)), dim3(TPB), 0, c->stream, c->nbf, dep.p ? c->bP.p : nullptr,
-                       c->bP_g.p, dep.vf ? c->bVF.p : nullptr, c->bVF_g.p, c->d_bflag.p);
-    ProfScope ps(c, "lsq1_bchanged", 0.0);
+    SEMO_LAUNCH(c, "lsq1_refresh_check", 8.0, k_clear2, dim3(1), dim3(1), c->d_bflag.p);
+    SEMO_LAUNCH(c, "lsq1_refresh_check", 32.0 * c->nbf, k_bchanged, dim3(nblk(c->nbf)), dim3(TPB), c->nbf,
+                dep.p ? c->bP.p : nullptr, c->bP_g.p, dep.vf ? c->bVF.p : nullptr, c->bVF_g.p, c->d_bflag.p);
     if (dep.p) lsq1(c, c->P.p, c->bP.p, c->nif, c->gP.p, c->d_bflag.p);
     if (dep.vf) lsq1(c, c->VF.p, c->bVF.p, c->nif, c->gVF.p, c->d_bflag.p + 1);
 }
@@ -1484,20 +1500,17 @@ void assemble_momentum(semo_ctx* c, bool after_vof, bool alpha_grad_ready) {
     }
     if (c->nif) {
         double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 8 * 8) + (double)c->nc * 8 * (4 + 12 + 8 + 2);
-        ProfScope ps(c, "mom_face", bytes);
-        hipLaunchKernelGGL(k_mom_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, k, face_in(c),
-                           c->Mu.p, c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
+        SEMO_LAUNCH(c, "mom_face", bytes, k_mom_face, dim3(nblk(c->nif)), dim3(TPB), c->nif, k, face_in(c), c->Mu.p,
+                    c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
     }
     {
         double bytes = (double)c->nc * 8 * (1 + 1 + 3 + 3 + 3 + 4 + 4) + 8.0 * 8 * c->nif + 4.0 * c->h_cf_idx.size();
-        ProfScope ps(c, "mom_cell", bytes);
-        hipLaunchKernelGGL(k_mom_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
-                           c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p,
-                           c->f_nrm.p, c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->Rho.p, c->P.p,
-                           c->U.p, c->V.p, c->W.p, c->oldU.p, c->oldV.p, c->oldW.p, c->Qm0.p, c->Qm1.p, c->Qm2.p,
-                           c->srcG.p, c->srcS.p, c->alphaP.n ? c->alphaP.p : nullptr, c->gP.p, c->gU.p, c->gV.p,
-                           c->gW.p, c->bP.p, c->bUL.p, c->bUR.p, c->bMu.p, c->bRho.p, c->fscr.p, c->Amat.diag.p,
-                           c->Bvec.p);
+        SEMO_LAUNCH(c, "mom_cell", bytes, k_mom_cell, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, c->nbf, k,
+                    c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p, c->f_nrm.p,
+                    c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->Rho.p, c->P.p, c->U.p, c->V.p, c->W.p,
+                    c->oldU.p, c->oldV.p, c->oldW.p, c->Qm0.p, c->Qm1.p, c->Qm2.p, c->srcG.p, c->srcS.p,
+                    c->alphaP.n ? c->alphaP.p : nullptr, c->gP.p, c->gU.p, c->gV.p, c->gW.p, c->bP.p, c->bUL.p,
+                    c->bUR.p, c->bMu.p, c->bRho.p, c->fscr.p, c->Amat.diag.p, c->Bvec.p);
     }
     c->last_eq = 0;
 }
@@ -1508,17 +1521,15 @@ void assemble_pressure(semo_ctx* c, bool after_momentum) {
     if (after_momentum) refresh_u_dependent_grads(c);
     if (c->nif) {
         double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 2 * 8) + (double)c->nc * 8 * (4 + 12 + 6 + 2);
-        ProfScope ps(c, "p_face", bytes);
-        hipLaunchKernelGGL(k_p_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, k, face_in(c),
-                           c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
+        SEMO_LAUNCH(c, "p_face", bytes, k_p_face, dim3(nblk(c->nif)), dim3(TPB), c->nif, k, face_in(c),
+                    c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
     }
     {
-        ProfScope ps(c, "p_cell", (double)c->nc * 16 + 16.0 * c->nif + 4.0 * c->h_cf_idx.size());
-        hipLaunchKernelGGL(k_p_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
-                           c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p,
-                           c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->P.p, c->gP.p, c->bP.p,
-                           c->bUL.p, c->bUR.p, c->bRho.p, c->fscr.p, c->U.p, c->V.p, c->W.p, c->Csnd.p,
-                           c->Amat.diag.p, c->Bvec.p);
+        SEMO_LAUNCH(c, "p_cell", (double)c->nc * 16 + 16.0 * c->nif + 4.0 * c->h_cf_idx.size(), k_p_cell,
+                    dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, c->nbf, k, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p,
+                    c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p,
+                    c->P.p, c->gP.p, c->bP.p, c->bUL.p, c->bUR.p, c->bRho.p, c->fscr.p, c->U.p, c->V.p, c->W.p,
+                    c->Csnd.p, c->Amat.diag.p, c->Bvec.p);
     }
     c->last_eq = 1;
 }
@@ -1532,24 +1543,21 @@ void assemble_vof(semo_ctx* c) {
     grad_minmax(c, true, false);       // with the P bounds the momentum stage after it reuses
     if (c->nif) {
         {
-            ProfScope ps(c, "mstacs_face", (double)c->nif * (8 * 8 + 8 * 3 + 16) + (double)c->nc * 8 * 12);
-            hipLaunchKernelGGL(k_mstacs_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, k,
-                               c->d_owner.p, c->d_nbr.p, c->f_wC.p, c->f_nrm.p, c->f_skew.p, c->c_xyz.p,
-                               c->c_vol.p, c->c_maxA.p, c->U.p, c->V.p, c->W.p, c->VF.p, c->gVF.p, c->fVFL.p,
-                               c->fVFR.p);
+            SEMO_LAUNCH(c, "mstacs_face", (double)c->nif * (8 * 8 + 8 * 3 + 16) + (double)c->nc * 8 * 12,
+                        k_mstacs_face, dim3(nblk(c->nif)), dim3(TPB), c->nif, k, c->d_owner.p, c->d_nbr.p, c->f_wC.p,
+                        c->f_nrm.p, c->f_skew.p, c->c_xyz.p, c->c_vol.p, c->c_maxA.p, c->U.p, c->V.p, c->W.p, c->VF.p,
+                        c->gVF.p, c->fVFL.p, c->fVFR.p);
         }
         double bytes = (double)c->nif * (13 * 8 + 8 + 8 + 16 + 3 * 8) + (double)c->nc * 8 * (4 + 12 + 6 + 2);
-        ProfScope ps(c, "vof_face", bytes);
-        hipLaunchKernelGGL(k_vof_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, k, face_in(c),
-                           c->fVFL.p, c->fVFR.p, c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
+        SEMO_LAUNCH(c, "vof_face", bytes, k_vof_face, dim3(nblk(c->nif)), dim3(TPB), c->nif, k, face_in(c), c->fVFL.p,
+                    c->fVFR.p, c->d_slot_own.p, c->d_slot_nbr.p, c->Amat.val.p, c->fscr.p);
     }
     {
-        ProfScope ps(c, "vof_cell", (double)c->nc * 8 * 6 + 24.0 * c->nif + 4.0 * c->h_cf_idx.size());
-        hipLaunchKernelGGL(k_vof_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->nbf, k,
-                           c->d_cf_sl.p, c->d_cf_ell.p, c->d_bpatch.p, c->d_bcs.p, c->c_vol.p, c->f_nrm.p,
-                           c->f_ePN.p, c->f_area.p, c->f_magPN.p, c->f_alpha.p, c->P.p, c->U.p, c->V.p, c->W.p,
-                           c->VF.p, c->oldVF.p, c->Qm3.p, c->gP.p, c->bVF.p, c->bUL.p, c->bUR.p, c->bRho.p,
-                           c->fscr.p, c->Amat.diag.p, c->Bvec.p);
+        SEMO_LAUNCH(c, "vof_cell", (double)c->nc * 8 * 6 + 24.0 * c->nif + 4.0 * c->h_cf_idx.size(), k_vof_cell,
+                    dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, c->nbf, k, c->d_cf_sl.p, c->d_cf_ell.p,
+                    c->d_bpatch.p, c->d_bcs.p, c->c_vol.p, c->f_nrm.p, c->f_ePN.p, c->f_area.p, c->f_magPN.p,
+                    c->f_alpha.p, c->P.p, c->U.p, c->V.p, c->W.p, c->VF.p, c->oldVF.p, c->Qm3.p, c->gP.p, c->bVF.p,
+                    c->bUL.p, c->bUR.p, c->bRho.p, c->fscr.p, c->Amat.diag.p, c->Bvec.p);
     }
     c->last_eq = 2;
 }
@@ -1561,61 +1569,64 @@ void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double*
     int n = c->nc;
     c->xph.ensure(4 * (size_t)n);
     dim3 gb(nblk(n)), tb(TPB);
-    hipLaunchKernelGGL(k_pack_xphi, gb, tb, 0, c->stream, n, c->c_xyz.p, phi, c->xph.p, gate);
+    SEMO_LAUNCH(c, "lsq1_pack", 64.0 * n, k_pack_xphi, gb, tb, n, c->c_xyz.p, phi, c->xph.p, gate);
     if (c->d_st_dl.n)
-        hipLaunchKernelGGL(k_lsq1<int16_t>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
-                           c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
+        SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int16_t>, gb, tb, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
+                    c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
     else
-        hipLaunchKernelGGL(k_lsq1<int>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
-                           c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
+        SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int>, gb, tb, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
+                    c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
 }
 
 // calcSourceSurfaceTension with calcCurvature (curvature.cpp:7-3537,
 // surfaceTension.cpp:7-229).  With sigma = 0 every use of kappa vanishes, so
 // the pipeline is skipped and the sources are zero.
 bool surface_tension(semo_ctx* c) {
-    if (c->cs.sigma == 0.0) { c->srcS.zero(c->stream); c->kappa.zero(c->stream); return false; }
+    if (c->cs.sigma == 0.0) {
+        zero_d(c, c->srcS.p, c->srcS.bytes());
+        zero_d(c, c->kappa.p, c->kappa.bytes());
+        return false;
+    }
     int n = c->nc;
     for (DBuf<double>* b : {&c->ls, &c->ls2, &c->tau_ls, &c->vel0_ls}) b->ensure(n);
     c->cnrm.ensure(3 * (size_t)n); c->cgrad.ensure(3 * (size_t)n); c->iface.ensure(n);
     dim3 gb(nblk(n)), tb(TPB);
-    ProfScope ps(c, "curvature", (double)n * 8 * (6 * 40 + 60));
     if (c->kinv.n != 6 * (size_t)n) {
         c->kinv.alloc(6 * (size_t)n);
-        hipLaunchKernelGGL(k_kappa_lsq_init, gb, tb, 0, c->stream, n, c->d_st_ptr.p, c->d_st_idx.p, c->d_bf_ptr.p,
-                           c->d_bf_idx.p, c->c_xyz.p, c->f_xyz.p, c->kinv.p);
+        SEMO_LAUNCH(c, "setup_kappa_lsq", 4.0 * (double)c->d_st_idx.n + 24.0 * n + 48.0 * n, k_kappa_lsq_init, gb, tb,
+                    n, c->d_st_ptr.p, c->d_st_idx.p, c->d_bf_ptr.p, c->d_bf_idx.p, c->c_xyz.p, c->f_xyz.p, c->kinv.p);
     }
     auto lsq_input = [&](const double* phi, double* g) { lsq1(c, phi, nullptr, 0, g); };
-    hipLaunchKernelGGL(k_curv_iface, gb, tb, 0, c->stream, n, c->nif, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p,
-                       c->d_nbr.p, c->f_wC.p, c->VF.p, c->iface.p, c->ls.p);
+    SEMO_LAUNCH(c, "curv_iface", cell_faces_bytes(c, 16.0) + 17.0 * n, k_curv_iface, gb, tb, n, c->nif, c->d_cf_sl.p,
+                c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p, c->f_wC.p, c->VF.p, c->iface.p, c->ls.p);
     lsq_input(c->ls.p, c->cgrad.p);
-    hipLaunchKernelGGL(k_curv_prep, gb, tb, 0, c->stream, n, c->iface.p, c->cgrad.p, c->c_vol.p, c->c_maxA.p,
-                       c->ls.p, c->tau_ls.p, c->vel0_ls.p);
+    SEMO_LAUNCH(c, "curv_prep", 73.0 * n, k_curv_prep, gb, tb, n, c->iface.p, c->cgrad.p, c->c_vol.p, c->c_maxA.p,
+                c->ls.p, c->tau_ls.p, c->vel0_ls.p);
     for (int it = 0; it < 4; ++it) {                       // LSReinitIterMax (curvature.cpp:16)
         lsq_input(c->ls.p, c->cgrad.p);
-        hipLaunchKernelGGL(k_curv_normal, gb, tb, 0, c->stream, n, 0, c->cgrad.p, c->cnrm.p);
-        hipLaunchKernelGGL(k_curv_reinit, gb, tb, 0, c->stream, n, c->nif, c->d_cf_sl.p, c->d_cf_ell.p,
-                           c->d_owner.p, c->d_nbr.p, c->c_xyz.p, c->f_xyz.p, c->cnrm.p, c->vel0_ls.p, c->tau_ls.p,
-                           c->iface.p, c->ls.p, c->ls2.p);
+        SEMO_LAUNCH(c, "curv_normal", 48.0 * n, k_curv_normal, gb, tb, n, 0, c->cgrad.p, c->cnrm.p);
+        SEMO_LAUNCH(c, "curv_reinit", cell_faces_bytes(c, 32.0) + 97.0 * n, k_curv_reinit, gb, tb, n, c->nif,
+                    c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p, c->c_xyz.p, c->f_xyz.p, c->cnrm.p,
+                    c->vel0_ls.p, c->tau_ls.p, c->iface.p, c->ls.p, c->ls2.p);
         std::swap(c->ls, c->ls2);
     }
     lsq_input(c->ls.p, c->cgrad.p);
-    hipLaunchKernelGGL(k_curv_normal, gb, tb, 0, c->stream, n, 1, c->cgrad.p, c->cnrm.p);
+    SEMO_LAUNCH(c, "curv_normal", 48.0 * n, k_curv_normal, gb, tb, n, 1, c->cgrad.p, c->cnrm.p);
     c->xn.ensure(8 * (size_t)n);
-    hipLaunchKernelGGL(k_pack_xn, gb, tb, 0, c->stream, n, c->c_xyz.p, c->cnrm.p, c->xn.p);
+    SEMO_LAUNCH(c, "curv_pack", 112.0 * n, k_pack_xn, gb, tb, n, c->c_xyz.p, c->cnrm.p, c->xn.p);
     if (c->d_st_dl.n)
-        hipLaunchKernelGGL(k_curv_kappa<int16_t>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_dl.p, c->xn.p,
-                           c->kinv.p, c->kappa.p);
+        SEMO_LAUNCH(c, "curv_kappa", 2.0 * (double)c->d_st_idx.n + 48.0 * n + 48.0 * n + 8.0 * n,
+                    k_curv_kappa<int16_t>, gb, tb, n, c->d_st_sl.p, c->d_st_dl.p, c->xn.p, c->kinv.p, c->kappa.p);
     else
-        hipLaunchKernelGGL(k_curv_kappa<int>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_ell.p, c->xn.p,
-                           c->kinv.p, c->kappa.p);
+        SEMO_LAUNCH(c, "curv_kappa", 4.0 * (double)c->d_st_idx.n + 48.0 * n + 48.0 * n + 8.0 * n, k_curv_kappa<int>,
+                    gb, tb, n, c->d_st_sl.p, c->d_st_ell.p, c->xn.p, c->kinv.p, c->kappa.p);
     // grad(alpha): LSQ with the boundary values (into the alpha gradient of
     // the assembly, which is the same), then one Green-Gauss pass
     lsq1(c, c->VF.p, c->bVF.p, c->nif, c->gVF.p);
-    hipLaunchKernelGGL(k_st_source, gb, tb, 0, c->stream, n, c->nif, case_k(c), c->d_cf_sl.p, c->d_cf_ell.p,
-                       c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_wC.p, c->f_skew.p, c->f_nrm.p,
-                       c->f_area.p, c->c_vol.p, c->f_xyz.p, c->c_xyz.p, c->VF.p, c->bVF.p, c->gVF.p, c->kappa.p,
-                       c->U.p, c->V.p, c->W.p, c->srcS.p);
+    SEMO_LAUNCH(c, "st_source", cell_faces_bytes(c, 8.0 + 8.0 + 24.0 + 24.0 + 8.0) + 40.0 * c->nbf + 128.0 * n,
+                k_st_source, gb, tb, n, c->nif, case_k(c), c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p,
+                c->d_bpatch.p, c->d_bcs.p, c->f_wC.p, c->f_skew.p, c->f_nrm.p, c->f_area.p, c->c_vol.p, c->f_xyz.p,
+                c->c_xyz.p, c->VF.p, c->bVF.p, c->gVF.p, c->kappa.p, c->U.p, c->V.p, c->W.p, c->srcS.p);
     return true;
 }
 
@@ -1625,9 +1636,9 @@ bool surface_tension(semo_ctx* c) {
 double time_step(semo_ctx* c, int adjust) {
     CaseK k = case_k(c);
     int nb = nblk(c->nc);
-    hipLaunchKernelGGL(k_time_scale, dim3(nb), dim3(TPB), 0, c->stream, c->nc, k, adjust, c->d_cf_sl.p,
-                       c->d_cf_ell.p, c->f_area.p, c->c_vol.p, c->U.p, c->V.p, c->W.p, c->Mu.p, c->Rho.p,
-                       c->red.p);
+    SEMO_LAUNCH(c, "time_scale", cell_faces_bytes(c, 8.0) + 56.0 * c->nc + 16.0 * nb, k_time_scale, dim3(nb),
+                dim3(TPB), c->nc, k, adjust, c->d_cf_sl.p, c->d_cf_ell.p, c->f_area.p, c->c_vol.p, c->U.p, c->V.p,
+                c->W.p, c->Mu.p, c->Rho.p, c->red.p);
     std::vector<double> h(2 * (size_t)nb);
     SEMO_HIP(hipMemcpyAsync(h.data(), c->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SEMO_HIP(hipStreamSynchronize(c->stream));
@@ -1646,8 +1657,8 @@ double time_step(semo_ctx* c, int adjust) {
 // VF += vofVofURF dVF, clipped; returns sqrt(sum dVF^2) (eqVolfrac.cpp:770-783)
 double vof_update(semo_ctx* c) {
     int nb = nblk(c->nc);
-    hipLaunchKernelGGL(k_vof_update, dim3(nb), dim3(TPB), 0, c->stream, c->nc, c->cs.urf_vf, c->Xvec.p, c->VF.p,
-                       c->red.p);
+    SEMO_LAUNCH(c, "vof_update", 24.0 * c->nc + 16.0 * nb, k_vof_update, dim3(nb), dim3(TPB), c->nc, c->cs.urf_vf,
+                c->Xvec.p, c->VF.p, c->red.p);
     std::vector<double> h(2 * (size_t)nb);
     SEMO_HIP(hipMemcpyAsync(h.data(), c->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SEMO_HIP(hipStreamSynchronize(c->stream));
@@ -1659,27 +1670,27 @@ double vof_update(semo_ctx* c) {
 double momentum_update(semo_ctx* c) {
     CaseK k = case_k(c);
     int nb = nblk(c->nc);
-    hipLaunchKernelGGL(k_mom_update, dim3(nb), dim3(TPB), 0, c->stream, c->nc, k, c->Xvec.p, c->U.p, c->V.p,
-                       c->W.p, c->red.p);
+    SEMO_LAUNCH(c, "mom_update", (3 * 8.0 + 3 * 16.0) * c->nc + 16.0 * nb, k_mom_update, dim3(nb), dim3(TPB), c->nc,
+                k, c->Xvec.p, c->U.p, c->V.p, c->W.p, c->red.p);
     return finish_ratio(c, nb);
 }
 
 double pressure_correct(semo_ctx* c) {
     CaseK k = case_k(c);
     const double* pp = c->Xvec.p;
-    c->gpp.zero(c->stream);
+    zero_d(c, c->gpp.p, c->gpp.bytes());
     for (int pass = 0; pass < 2; ++pass) {
         if (c->nif)
-            hipLaunchKernelGGL(k_gg_face, dim3(nblk(c->nif)), dim3(TPB), 0, c->stream, c->nif, c->d_owner.p,
-                               c->d_nbr.p, c->f_wC.p, c->f_skew.p, pp, c->gpp.p, c->fscr.p);
-        hipLaunchKernelGGL(k_gg_cell, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->nif, c->d_cf_sl.p,
-                           c->d_cf_ell.p, c->d_owner.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p,
-                           c->c_vol.p, pp, c->fscr.p, c->gtmp.p);
+            SEMO_LAUNCH(c, "gg_face", (8.0 + 8.0 + 24.0 + 8.0) * c->nif + 32.0 * c->nc, k_gg_face, dim3(nblk(c->nif)),
+                        dim3(TPB), c->nif, c->d_owner.p, c->d_nbr.p, c->f_wC.p, c->f_skew.p, pp, c->gpp.p, c->fscr.p);
+        SEMO_LAUNCH(c, "gg_cell", cell_faces_bytes(c, 24.0 + 8.0) + 8.0 * c->nif + 8.0 * c->nbf + 40.0 * c->nc,
+                    k_gg_cell, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p,
+                    c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->c_vol.p, pp, c->fscr.p, c->gtmp.p);
         std::swap(c->gpp, c->gtmp);
     }
     int nb = nblk(c->nc);
-    hipLaunchKernelGGL(k_p_correct, dim3(nb), dim3(TPB), 0, c->stream, c->nc, k, pp, c->gpp.p, c->Rho.p,
-                       c->P.p, c->U.p, c->V.p, c->W.p, c->red.p);
+    SEMO_LAUNCH(c, "p_correct", (8.0 + 24.0 + 8.0 + 16.0 + 3 * 16.0) * c->nc + 16.0 * nb, k_p_correct, dim3(nb),
+                dim3(TPB), c->nc, k, pp, c->gpp.p, c->Rho.p, c->P.p, c->U.p, c->V.p, c->W.p, c->red.p);
     return finish_ratio(c, nb);
 }
 
@@ -1718,24 +1729,47 @@ void lsq1_noalloc(semo_ctx* c, const double* phi, const double* faceR_b, double*
     int n = c->nc;
     dim3 gb(nblk(n)), tb(TPB);
     double* xph = c->noalloc_xph;
-    hipLaunchKernelGGL(k_pack_xphi, gb, tb, 0, c->stream, n, c->c_xyz.p, phi, xph, (const int*)nullptr);
+    SEMO_LAUNCH(c, "lsq1_pack", 64.0 * n, k_pack_xphi, gb, tb, n, c->c_xyz.p, phi, xph, (const int*)nullptr);
     if (c->d_st_dl.n)
-        hipLaunchKernelGGL(k_lsq1<int16_t>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
-                           c->d_bf_idx.p, xph, c->f_xyz.p, c->c_lsq.p, faceR_b, c->nif, grad, (const int*)nullptr);
+        SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int16_t>, gb, tb, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
+                    c->d_bf_idx.p, xph, c->f_xyz.p, c->c_lsq.p, faceR_b, c->nif, grad, (const int*)nullptr);
     else
-        hipLaunchKernelGGL(k_lsq1<int>, gb, tb, 0, c->stream, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
-                           c->d_bf_idx.p, xph, c->f_xyz.p, c->c_lsq.p, faceR_b, c->nif, grad, (const int*)nullptr);
+        SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int>, gb, tb, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
+                    c->d_bf_idx.p, xph, c->f_xyz.p, c->c_lsq.p, faceR_b, c->nif, grad, (const int*)nullptr);
     SEMO_HIP(hipGetLastError());
 }
 
 void minmax_standalone(semo_ctx* c, const double* phi, double* mx, double* mn) {
-    hipLaunchKernelGGL(k_minmax1, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->d_st_ptr.p,
-                       c->d_st_idx.p, phi, mx, mn);
+    SEMO_LAUNCH(c, "minmax1", 4.0 * (double)c->d_st_idx.n + 4.0 * c->nc + 24.0 * c->nc, k_minmax1, dim3(nblk(c->nc)),
+                dim3(TPB), c->nc, c->d_st_ptr.p, c->d_st_idx.p, phi, mx, mn);
+}
+
+// sum phi_i V_i in FP64 (block partials on the device, summed on the host in
+// block order: the same value on every run); *total_vol = sum V_i
+double volume_integral(semo_ctx* c, const double* phi, double* total_vol) {
+    int nb = nblk(c->nc);
+    SEMO_LAUNCH(c, "volume_integral", 16.0 * c->nc + 16.0 * nb, k_vol_integral, dim3(nb), dim3(TPB), c->nc, phi,
+                c->c_vol.p, c->red.p);
+    std::vector<double> h(2 * (size_t)nb);
+    SEMO_HIP(hipMemcpyAsync(h.data(), c->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+    double a = 0.0, v = 0.0;
+    for (int q = 0; q < nb; ++q) { a += h[2 * q]; v += h[2 * q + 1]; }
+    if (total_vol) *total_vol = v;
+    return a;
+}
+
+void zero_d(semo_ctx* c, void* p, size_t bytes) {
+    size_t n = bytes / 4;
+    SEMO_CHECK(n * 4 == bytes, SEMO_EINVAL, "zero_d: size not a multiple of 4 bytes");
+    if (!n) return;
+    SEMO_LAUNCH(c, "fill", 4.0 * n, k_zero, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), n,
+                static_cast<uint32_t*>(p));
 }
 
 void copy_d(semo_ctx* c, double* dst, const double* src, size_t n) {
     if (!n) return;
-    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, c->stream, n, dst, src);
+    SEMO_LAUNCH(c, "copy", 16.0 * n, k_copy, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), n, dst, src);
 }
 
 }  // namespace semo

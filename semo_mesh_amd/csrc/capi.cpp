@@ -660,6 +660,18 @@ int semo_objective_pressure(semo_ctx* c, const double* patch_weight, double* J) 
     });
 }
 
+int semo_volume_integral(semo_ctx* c, int f, double* integral, double* volume) {
+    if (!c || !integral) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        int comps;
+        DBuf<double>* d = cell_field(c, f, comps);
+        SEMO_CHECK(d && comps == 1 && d->n >= (size_t)c->nc, SEMO_EINVAL, "not a one-component cell field");
+        reduction_alloc(c, (long)c->nc);
+        *integral = volume_integral(c, d->p, volume);
+    });
+}
+
 int semo_get_solve_stats(const semo_ctx* c, int eq, semo_solve_stats* st) {
     if (!c || !st || eq < 0 || eq > 2) return SEMO_EINVAL;
     *st = eq == 0 ? c->last_stats_mom : eq == 1 ? c->last_stats_p : c->last_stats_vf;

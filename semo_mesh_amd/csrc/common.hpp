@@ -316,6 +316,35 @@ struct semo_ctx {
 };
 
 namespace semo {
+// ------------------------------------------------------ tagged launches --
+// While the context's kernel profiler is on (V-cycle graphs are then issued
+// eagerly), a HIP event pair on the context's stream brackets the launches
+// of a scope; the tag accumulates their count, time and algorithmic bytes
+// (the compulsory HBM traffic of one launch: every operand read once, every
+// result written once).
+struct ProfScope {
+    semo_ctx* c; const char* name; double bytes; hipEvent_t a{};
+    ProfScope(semo_ctx* c_, const char* n, double b) : c(c_), name(n), bytes(b) {
+        if (c->prof.on && name) c->prof.begin(c->stream, a);
+    }
+    ~ProfScope() { if (c->prof.on && name) c->prof.end(c->stream, name, a, bytes); }
+    ProfScope(const ProfScope&) = delete;
+    ProfScope& operator=(const ProfScope&) = delete;
+};
+// algorithmic bytes of a cell loop over the cell's faces: the face lists and,
+// per face, `per_face` bytes of face data (counted once, although an
+// internal face is reached from both of its cells)
+inline double cell_faces_bytes(const semo_ctx* c, double per_face) {
+    return 4.0 * (double)c->h_cf_idx.size() + per_face * (double)c->nf;
+}
+// one kernel launch on the context's stream under a profiler tag; `bytes` is
+// evaluated only while the profiler is on
+#define SEMO_LAUNCH(c, tag, bytes, kern, grid, block, ...)                                  \
+    do {                                                                                    \
+        ::semo::ProfScope semo_ps_((c), (tag), (c)->prof.on ? (double)(bytes) : 0.0);       \
+        hipLaunchKernelGGL(kern, grid, block, 0, (c)->stream, __VA_ARGS__);                 \
+    } while (0)
+
 // ------------------------------------------------- Krylov convergence tests --
 // Iteration k's squared residual is copied into pinned slot k % 4 behind an
 // event.  ConvCheck reads it only once iteration k + 1 has been issued, so the
@@ -428,6 +457,10 @@ double* noalloc_scratch(int device);
 void lsq1_noalloc(semo_ctx* c, const double* phi, const double* faceR_b, double* grad);
 void minmax_standalone(semo_ctx* c, const double* d_phi, double* d_mx, double* d_mn);
 void copy_d(semo_ctx* c, double* dst, const double* src, size_t n);
+// `bytes` bytes at p (a multiple of 4) set to zero by a kernel on the
+// context's stream, tagged "fill" (a HIP memset would run untagged)
+void zero_d(semo_ctx* c, void* p, size_t bytes);
+double volume_integral(semo_ctx* c, const double* phi, double* total_vol);
 // solver.hip
 void sell_narrow(const SellPattern& p, DBuf<int16_t>& d, hipStream_t s);
 template <class T> void sell_upload(DevSell<T>& M, const SellPattern& p, const std::vector<double>& vals,

@@ -340,7 +340,7 @@ int nb_of(size_t n) { return (int)std::min<size_t>((n + TPB - 1) / TPB, 2048); }
 double2 dot2(semo_ctx* c, Coupled& C, size_t n, const double* a, const double* b, const double* cc,
              const double* d) {
     int nb = nb_of(n);
-    hipLaunchKernelGGL(k_dot2v, dim3(nb), dim3(TPB), 0, c->stream, n, a, b, cc, d, C.part.p);
+    SEMO_LAUNCH(c, "cpl_dot", (cc ? 32.0 : 16.0) * n, k_dot2v, dim3(nb), dim3(TPB), n, a, b, cc, d, C.part.p);
     std::vector<double> h(2 * (size_t)nb);
     SEMO_HIP(hipMemcpyAsync(h.data(), C.part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SEMO_HIP(hipStreamSynchronize(c->stream));
@@ -353,17 +353,18 @@ double2 dot2(semo_ctx* c, Coupled& C, size_t n, const double* a, const double* b
 // couplings and pressure V-cycle)
 void precond(semo_ctx* c, Coupled& C, const double* r, double* z) {
     int n = C.n;
-    hipLaunchKernelGGL(k_sp_u, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, r, C.mdiag.p, z);
-    hipLaunchKernelGGL(k_sp_schur_rhs, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, C.ptr.p, C.col.p, C.dg.p, r, z,
-                       C.tp.p);
+    SEMO_LAUNCH(c, "cpl_precond_u", 56.0 * n, k_sp_u, dim3(nblk(n)), dim3(TPB), n, r, C.mdiag.p, z);
+    SEMO_LAUNCH(c, "cpl_precond_schur_rhs", 28.0 * C.nnzb + 68.0 * n, k_sp_schur_rhs, dim3(nblk(n)), dim3(TPB), n,
+                C.ptr.p, C.col.p, C.dg.p, r, z, C.tp.p);
     pressure_vcycle(c, C.tp.p, C.zp.p, true);
-    hipLaunchKernelGGL(k_sp_correct, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, C.ptr.p, C.col.p, C.dg.p,
-                       C.mdiag.p, C.zp.p, z);
+    SEMO_LAUNCH(c, "cpl_precond_correct", 28.0 * C.nnzb + 100.0 * n, k_sp_correct, dim3(nblk(n)), dim3(TPB), n,
+                C.ptr.p, C.col.p, C.dg.p, C.mdiag.p, C.zp.p, z);
 }
 
 void spmv64(semo_ctx* c, Coupled& C, const double* x, double* y) {
-    hipLaunchKernelGGL((k_bsr<double, 0>), dim3(nblk(4 * (long)C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p, C.col.p,
-                       C.val.p, C.diag.p, x, (const double*)nullptr, y);
+    SEMO_LAUNCH(c, "cpl_bsr_spmv", 4.0 * C.n + 132.0 * C.nnzb + 128.0 * C.n + 64.0 * C.n, (k_bsr<double, 0>),
+                dim3(nblk(4 * (long)C.n)), dim3(TPB), C.n, C.ptr.p, C.col.p, C.val.p, C.diag.p, x,
+                (const double*)nullptr, y);
 }
 
 // fine pattern and face -> block maps (once per mesh)
@@ -436,13 +437,15 @@ void assemble_coupled(semo_ctx* c) {
     copy_d(c, C.mval.p, c->Amat.val.p, nv);
     copy_d(c, C.mdiag.p, c->Amat.diag.p, n);
     copy_d(c, C.mrhs.p, c->Bvec.p, 3 * (size_t)n);
-    hipLaunchKernelGGL(k_cpl_row, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->nif, c->nbf, c->d_cf_sl.p,
-                       c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p,
-                       c->f_wC.p, c->d_slot_own.p, c->d_slot_nbr.p, C.fslot_own.p, C.fslot_nbr.p, C.mval.p,
-                       C.mdiag.p, C.pval.p, C.pdiag.p, C.mrhs.p, C.prhs.p, c->U.p, c->V.p, c->W.p, C.ptr.p, C.val.p,
-                       C.diag.p, C.rhs.p);
+    SEMO_LAUNCH(c, "cpl_assemble",
+                cell_faces_bytes(c, 56.0) + 32.0 * c->nif + 128.0 * ((double)n + C.nnzb) + 112.0 * n, k_cpl_row,
+                dim3(nblk(n)), dim3(TPB), n, c->nif, c->nbf, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p,
+                c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->f_wC.p, c->d_slot_own.p, c->d_slot_nbr.p,
+                C.fslot_own.p, C.fslot_nbr.p, C.mval.p, C.mdiag.p, C.pval.p, C.pdiag.p, C.mrhs.p, C.prhs.p, c->U.p,
+                c->V.p, c->W.p, C.ptr.p, C.val.p, C.diag.p, C.rhs.p);
     int64_t nb = (int64_t)n + C.nnzb;
-    hipLaunchKernelGGL(k_cpl_dg, dim3(nblk((long)nb)), dim3(TPB), 0, c->stream, nb, n, C.diag.p, C.val.p, C.dg.p);
+    SEMO_LAUNCH(c, "cpl_couplings_fp32", 152.0 * nb, k_cpl_dg, dim3(nblk((long)nb)), dim3(TPB), nb, n, C.diag.p,
+                C.val.p, C.dg.p);
     c->last_eq = 3;
 }
 
@@ -468,14 +471,14 @@ void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st
     double* sc = C.sc.p;
     int nbv = nblk((long)N);
     auto fin = [&](int nb, int what) {
-        hipLaunchKernelGGL(k_cfinish, dim3(1), dim3(TPB), 0, c->stream, nb, C.red.p, sc, what);
+        SEMO_LAUNCH(c, "krylov_finish", 16.0 * nb + 8.0, k_cfinish, dim3(1), dim3(TPB), nb, C.red.p, sc, what);
     };
     auto dot = [&](const double* a_, const double* b_, const double* c_, const double* d_, int what) {
         int nb = nb_of(N);
-        hipLaunchKernelGGL(k_dot2v, dim3(nb), dim3(TPB), 0, c->stream, N, a_, b_, c_, d_, C.red.p);
+        SEMO_LAUNCH(c, "cpl_dot", (c_ ? 32.0 : 16.0) * N, k_dot2v, dim3(nb), dim3(TPB), N, a_, b_, c_, d_, C.red.p);
         fin(nb, what);
     };
-    hipLaunchKernelGGL(k_cb_init, dim3(nbv), dim3(TPB), 0, c->stream, N, b, x, r, rh, p, v, C.red.p);
+    SEMO_LAUNCH(c, "cpl_bicg_init", 48.0 * N, k_cb_init, dim3(nbv), dim3(TPB), N, b, x, r, rh, p, v, C.red.p);
     fin(nbv, F_STORE_BB);
     double bb;
     SEMO_HIP(hipMemcpyAsync(&bb, sc + C_BB, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -488,23 +491,25 @@ void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st
         int since = 0;
         for (it = 1; it <= c->sp.maxit_u; ++it) {
             dot(rh, r, nullptr, nullptr, F_RHO);
-            hipLaunchKernelGGL(k_cb_p, dim3(nbv), dim3(TPB), 0, c->stream, N, sc, r, v, p);
+            SEMO_LAUNCH(c, "cpl_bicg_p", 32.0 * N, k_cb_p, dim3(nbv), dim3(TPB), N, sc, r, v, p);
             precond(c, C, p, ph);
             spmv64(c, C, ph, v);
             dot(rh, v, nullptr, nullptr, F_ALPHA);
-            hipLaunchKernelGGL(k_cb_s, dim3(nbv), dim3(TPB), 0, c->stream, N, sc, r, v, sv);
+            SEMO_LAUNCH(c, "cpl_bicg_s", 24.0 * N, k_cb_s, dim3(nbv), dim3(TPB), N, sc, r, v, sv);
             precond(c, C, sv, sh);
             spmv64(c, C, sh, t);
             dot(t, sv, t, t, F_OMEGA);
-            hipLaunchKernelGGL(k_cb_x, dim3(nbv), dim3(TPB), 0, c->stream, N, sc, ph, sh, sv, t, x, r, C.red.p);
+            SEMO_LAUNCH(c, "cpl_bicg_x", 56.0 * N, k_cb_x, dim3(nbv), dim3(TPB), N, sc, ph, sh, sv, t, x, r, C.red.p);
             fin(nbv, F_RR);
             pipe_push(c, sc + C_RR, it);
             if (cc.done(it)) break;
             if (cc.r1 < 0.25 * best) { best = cc.r1; since = 0; }       // squared residuals
             else if (++since >= RESTART_AFTER) {
-                hipLaunchKernelGGL((k_bsr<double, 1>), dim3(nblk(4 * (long)C.n)), dim3(TPB), 0, c->stream, C.n,
-                                   C.ptr.p, C.col.p, C.val.p, C.diag.p, x, b, r);
-                hipLaunchKernelGGL(k_cb_restart, dim3(nbv), dim3(TPB), 0, c->stream, N, r, rh, p, v, C.red.p);
+                SEMO_LAUNCH(c, "cpl_bsr_resid", 4.0 * C.n + 132.0 * C.nnzb + 128.0 * C.n + 64.0 * C.n + 32.0 * C.n,
+                            (k_bsr<double, 1>), dim3(nblk(4 * (long)C.n)), dim3(TPB), C.n, C.ptr.p, C.col.p, C.val.p,
+                            C.diag.p, x, b, r);
+                SEMO_LAUNCH(c, "cpl_bicg_restart", 32.0 * N, k_cb_restart, dim3(nbv), dim3(TPB), N, r, rh, p, v,
+                            C.red.p);
                 fin(nbv, F_RESTART);
                 since = 0;
                 best = cc.r1;
@@ -512,8 +517,9 @@ void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st
         }
         it = std::min(it, c->sp.maxit_u);
         // true residual
-        hipLaunchKernelGGL((k_bsr<double, 1>), dim3(nblk(4 * (long)C.n)), dim3(TPB), 0, c->stream, C.n, C.ptr.p,
-                           C.col.p, C.val.p, C.diag.p, x, b, r);
+        SEMO_LAUNCH(c, "cpl_bsr_resid", 4.0 * C.n + 132.0 * C.nnzb + 128.0 * C.n + 64.0 * C.n + 32.0 * C.n,
+                    (k_bsr<double, 1>), dim3(nblk(4 * (long)C.n)), dim3(TPB), C.n, C.ptr.p, C.col.p, C.val.p,
+                    C.diag.p, x, b, r);
         double rr = dot2(c, C, N, r, r, nullptr, nullptr).x;
         C.last_relres = std::sqrt(rr) / nb;
     } else {
@@ -537,8 +543,8 @@ void coupled_solve_assembled(semo_ctx* c, semo_solve_stats* st) {
 double coupled_update(semo_ctx* c) {
     Coupled& C = *c->cpl;
     int nb = nblk(c->nc);
-    hipLaunchKernelGGL(k_cpl_update, dim3(nb), dim3(TPB), 0, c->stream, c->nc, c->cs.urf_p, c->cs.urf_mom,
-                       c->cs.limit, C.x.p, c->P.p, c->U.p, c->V.p, c->W.p, c->red.p);
+    SEMO_LAUNCH(c, "cpl_update", 96.0 * c->nc + 16.0 * nb, k_cpl_update, dim3(nb), dim3(TPB), c->nc, c->cs.urf_p,
+                c->cs.urf_mom, c->cs.limit, C.x.p, c->P.p, c->U.p, c->V.p, c->W.p, c->red.p);
     std::vector<double> h(2 * (size_t)nb);
     SEMO_HIP(hipMemcpyAsync(h.data(), c->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SEMO_HIP(hipStreamSynchronize(c->stream));

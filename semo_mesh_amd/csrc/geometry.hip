@@ -219,17 +219,20 @@ void geometry_compute(semo_ctx* c) {
     c->f_wC.alloc(nf); c->f_magPN.alloc(nf); c->f_alpha.alloc(nf); c->f_ePN.alloc(3 * (size_t)nf);
     c->f_skew.alloc(3 * (size_t)nf); c->f_pf.alloc(3 * (size_t)nf); c->f_nf.alloc(3 * (size_t)nf);
     c->c_vol.alloc(nc); c->c_xyz.alloc(3 * (size_t)nc); c->c_lsq.alloc(6 * (size_t)nc);
-    hipLaunchKernelGGL(k_face_polygon, dim3(nblk(nf)), dim3(TPB), 0, s, nf, c->d_fptr.p, c->d_fpts.p,
-                       c->d_pts.p, c->f_area.p, c->f_nrm.p, c->f_xyz.p, c->f_vsn.p, c->f_cc.p, c->d_flags.p);
-    hipLaunchKernelGGL(k_cell_volume, dim3(nblk(nc)), dim3(TPB), 0, s, nc, c->nif, c->d_cf_ptr.p,
-                       c->d_cf_idx.p, c->d_owner.p, c->f_vsn.p, c->f_cc.p, c->c_vol.p, c->c_xyz.p, c->d_flags.p);
+    SEMO_LAUNCH(c, "setup_face_polygon",
+                4.0 * nf + 4.0 * (double)c->d_fpts.n + 24.0 * (double)c->d_fpts.n + 128.0 * nf, k_face_polygon,
+                dim3(nblk(nf)), dim3(TPB), nf, c->d_fptr.p, c->d_fpts.p, c->d_pts.p, c->f_area.p, c->f_nrm.p,
+                c->f_xyz.p, c->f_vsn.p, c->f_cc.p, c->d_flags.p);
+    SEMO_LAUNCH(c, "setup_cell_volume", cell_faces_bytes(c, 32.0) + 32.0 * nc, k_cell_volume, dim3(nblk(nc)),
+                dim3(TPB), nc, c->nif, c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->f_vsn.p, c->f_cc.p, c->c_vol.p,
+                c->c_xyz.p, c->d_flags.p);
     c->d_fflip.upload(c->fflip, s);
-    hipLaunchKernelGGL(k_face_metrics, dim3(nblk(nf)), dim3(TPB), 0, s, nf, c->nif, c->d_owner.p,
-                       c->d_nbr.p, c->d_fflip.p, c->f_nrm.p, c->f_xyz.p, c->c_xyz.p, c->f_wC.p, c->f_magPN.p,
-                       c->f_alpha.p, c->f_ePN.p, c->f_skew.p, c->f_pf.p, c->f_nf.p);
-    hipLaunchKernelGGL(k_lsq_init, dim3(nblk(nc)), dim3(TPB), 0, s, nc, c->nif, c->d_st_ptr.p,
-                       c->d_st_idx.p, c->d_cf_ptr.p, c->d_cf_idx.p, c->c_xyz.p, c->f_xyz.p, c->c_lsq.p,
-                       c->d_flags.p);
+    SEMO_LAUNCH(c, "setup_face_metrics", (8.0 + 1.0 + 48.0 + 160.0) * nf + 24.0 * nc, k_face_metrics, dim3(nblk(nf)),
+                dim3(TPB), nf, c->nif, c->d_owner.p, c->d_nbr.p, c->d_fflip.p, c->f_nrm.p, c->f_xyz.p, c->c_xyz.p,
+                c->f_wC.p, c->f_magPN.p, c->f_alpha.p, c->f_ePN.p, c->f_skew.p, c->f_pf.p, c->f_nf.p);
+    SEMO_LAUNCH(c, "setup_lsq", 4.0 * (double)c->d_st_idx.n + 24.0 * nc + 48.0 * nc, k_lsq_init, dim3(nblk(nc)),
+                dim3(TPB), nc, c->nif, c->d_st_ptr.p, c->d_st_idx.p, c->d_cf_ptr.p, c->d_cf_idx.p, c->c_xyz.p,
+                c->f_xyz.p, c->c_lsq.p, c->d_flags.p);
     SEMO_HIP(hipGetLastError());
     int flags = 0;
     SEMO_HIP(hipMemcpyAsync(&flags, c->d_flags.p, sizeof(int), hipMemcpyDeviceToHost, s));

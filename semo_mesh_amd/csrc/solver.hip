@@ -284,9 +284,7 @@ void row_launch(semo_ctx* c, const DevSell<T>& A, RowArgs<T> a, const char* tag,
     if (A.n == 0) return;
     a.n = A.n; a.ptr = A.sl_ptr.p; a.col = A.col.p; a.val = A.val.p; a.diag = A.diag.p;
     a.dcol = (A.lanes == 0 && A.dcol.n) ? A.dcol.p : nullptr;
-    hipEvent_t ev{};
-    bool prof = c->prof.on && tag;
-    if (prof) c->prof.begin(c->stream, ev);
+    ProfScope ps(c, tag, bytes);
     long rows = A.n;
     switch (A.lanes) {
     case 0: hipLaunchKernelGGL((k_row<T, 0, OP>), dim3(nblk((long)A.nslices * 64)), dim3(TPB), 0, c->stream, a); break;
@@ -297,7 +295,6 @@ void row_launch(semo_ctx* c, const DevSell<T>& A, RowArgs<T> a, const char* tag,
     case 64: hipLaunchKernelGGL((k_row<T, 64, OP>), dim3(nblk(rows * 64)), dim3(TPB), 0, c->stream, a); break;
     default: throw Error(SEMO_EINVAL, "operator with an unsupported lane count");
     }
-    if (prof) c->prof.end(c->stream, tag, ev, bytes);
 }
 
 // algorithmic bytes of one pass over A: values + columns of the real
@@ -311,10 +308,10 @@ double op_bytes(const DevSell<T>& A, int vec) {
 }
 
 template <class T, int OP>
-void spmv(semo_ctx* c, const DevSell<T>& A, const T* x, const T* b, T* y, const char* tag = nullptr) {
+void spmv(semo_ctx* c, const DevSell<T>& A, const T* x, const T* b, T* y, const char* tag) {
     RowArgs<T> a{};
     a.x = x; a.b = b; a.y = y;
-    row_launch<T, OP>(c, A, a, tag, tag ? op_bytes(A, OP == OP_Y ? 2 : 3) : 0.0);
+    row_launch<T, OP>(c, A, a, tag, op_bytes(A, OP == OP_Y ? 2 : 3));
 }
 
 // y = A x (sliced ELL, FP64) with d0 = (a0 ? a0 : y).y and d1 = a1.y
@@ -746,8 +743,10 @@ void cheb_smooth(semo_ctx* c, AmgLevel& L, T* b, int degree, bool x_zero, const 
     double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
     double sigma = theta / delta, rho = 1.0 / sigma;
     if (x_zero) {
-        hipLaunchKernelGGL(k_cheb_init<T>, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, vdinv<T>(L), b, bsrc,
-                           (T)(1.0 / theta), r, dA, bx<T>(L).p, degree == 1 ? zo : nullptr);
+        SEMO_LAUNCH(c, tg.chebx,
+                    (double)sizeof(T) * 5 * n + (bsrc ? 8.0 * n : 0.0) + (degree == 1 && zo ? 8.0 * n : 0.0),
+                    k_cheb_init<T>, dim3(nblk(n)), dim3(TPB), n, vdinv<T>(L), b, bsrc, (T)(1.0 / theta), r, dA,
+                    bx<T>(L).p, degree == 1 ? zo : nullptr);
     } else if (degree == 1) {
         // the only step: r and y are dead, and with zo (the finest level)
         // so is the T iterate, which keeps the level's x buffer in place
@@ -787,10 +786,12 @@ void vcycle(semo_ctx* c, Amg& amg, int l, int degree, const double* bsrc, double
     int n = L.n;
     T* b = vb<T>(L);
     if (l + 1 == (int)amg.lv.size()) {     // coarsest: dense direct solve in FP64
-        if (bsrc) hipLaunchKernelGGL((k_cast<double, T>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, bsrc, b);
-        hipLaunchKernelGGL(k_dense_gemv<T>, dim3((n + 3) / 4), dim3(TPB), 0, c->stream, n, amg.coarse_inv.p, b,
-                           bx<T>(L).p);
-        if (zo) hipLaunchKernelGGL((k_cast<T, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, bx<T>(L).p, zo);
+        if (bsrc) SEMO_LAUNCH(c, (sizeof(T) == 8 ? "amg_coarse_fp64" : "amg_coarse_fp32"), (8.0 + sizeof(T)) * n,
+                              (k_cast<double, T>), dim3(nblk(n)), dim3(TPB), n, bsrc, b);
+        SEMO_LAUNCH(c, (sizeof(T) == 8 ? "amg_coarse_fp64" : "amg_coarse_fp32"), 8.0 * n * n + 2.0 * sizeof(T) * n,
+                    k_dense_gemv<T>, dim3((n + 3) / 4), dim3(TPB), n, amg.coarse_inv.p, b, bx<T>(L).p);
+        if (zo) SEMO_LAUNCH(c, (sizeof(T) == 8 ? "amg_coarse_fp64" : "amg_coarse_fp32"), (8.0 + sizeof(T)) * n,
+                            (k_cast<T, double>), dim3(nblk(n)), dim3(TPB), n, bx<T>(L).p, zo);
         return;
     }
     AmgLevel& C = amg.lv[l + 1];
@@ -895,14 +896,15 @@ Red red_of(semo_ctx* c) { return Red{c->red.p}; }
 
 // sums the partials of the previous launch (nb blocks) and applies `what`
 void finish(semo_ctx* c, int nb, int what, int slot = 0) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FTPB), 0, c->stream, nb, c->red.p, c->scal.p, what, slot);
+    SEMO_LAUNCH(c, "krylov_finish", 16.0 * nb + 8.0, k_finish, dim3(1), dim3(FTPB), nb, c->red.p, c->scal.p, what,
+                slot);
 }
 
 // grid-stride dot products a.b (+ c.d), finished as `what`
 void dot_what(semo_ctx* c, int n, const double* a, const double* b, const double* cc, const double* d, int what,
               int slot = 0) {
     int nb = rblk(n);
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(TPB), 0, c->stream, n, a, b, cc, d, red_of(c));
+    SEMO_LAUNCH(c, "krylov_dot", (cc ? 32.0 : 16.0) * n, k_dot, dim3(nb), dim3(TPB), n, a, b, cc, d, red_of(c));
     finish(c, nb, what, slot);
 }
 // ... stored into s[slot], s[slot+1]
@@ -913,14 +915,10 @@ void dot_store(semo_ctx* c, int n, const double* a, const double* b, const doubl
 // y = A x with dots of y (see k_spmv_dot), for the fine-level sliced-ELL operator
 void spmv_dot(semo_ctx* c, const DevSell<double>& A, const double* x, double* y, const double* a0,
               const double* a1, int what) {
-    hipEvent_t ev{};
-    bool prof = c->prof.on;
-    if (prof) c->prof.begin(c->stream, ev);
     int nb = nblk((long)A.nslices * 64);
-    hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(TPB), 0, c->stream, A.n, A.sl_ptr.p, A.col.p,
-                       A.dcol.n ? A.dcol.p : nullptr, A.val.p,
-                       A.diag.p, x, y, a0, a1, red_of(c));
-    if (prof) c->prof.end(c->stream, "spmv_fp64", ev, op_bytes(A, 2 + (a0 ? 1 : 0) + (a1 ? 1 : 0)));
+    SEMO_LAUNCH(c, "spmv_fp64", op_bytes(A, 2 + (a0 ? 1 : 0) + (a1 ? 1 : 0)), k_spmv_dot, dim3(nb), dim3(TPB),
+                A.n, A.sl_ptr.p, A.col.p, A.dcol.n ? A.dcol.p : nullptr, A.val.p, A.diag.p, x, y, a0, a1,
+                red_of(c));
     finish(c, nb, what);
 }
 
@@ -952,13 +950,14 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
     auto pre = [&]() {
         if (mixed) precondition<float>(c, amg, r, z, prep); else precondition<double>(c, amg, r, z, prep);
     };
-    hipLaunchKernelGGL(k_fcg_init, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, x, r, red_of(c), pp);
+    double prep_bytes = prep ? (mixed ? 8.0 : 16.0) * n : 0.0;     // the V-cycle's b and x written
+    SEMO_LAUNCH(c, "fcg_init", 24.0 * n + prep_bytes, k_fcg_init, dim3(nblk(n)), dim3(TPB), n, b, x, r, red_of(c), pp);
     finish(c, nblk(n), W_STORE, S_X);
     double bb = host_scalar(c, S_X);
     if (bb == 0.0) { *relres = 0.0; return 0; }
     double nb = std::sqrt(bb);
     pre();
-    hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(n)), dim3(TPB), 0, c->stream, n, z, p);
+    SEMO_LAUNCH(c, "fcg_copy", 16.0 * n, (k_cast<double, double>), dim3(nblk(n)), dim3(TPB), n, z, p);
     dot_store(c, n, r, z, nullptr, nullptr, S_RZ);
     int it = 0;
     ConvCheck cc(c, tol * tol * bb, bb);
@@ -967,19 +966,15 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
         if (it == 1) {
             spmv_dot(c, A, p, q, p, nullptr, W_FCG_ALPHA);                // q = A p, alpha
         } else {                                                          // p = z + beta p; q = A p, alpha
-            hipEvent_t ev{};
-            bool prof = c->prof.on;
-            if (prof) c->prof.begin(c->stream, ev);
             int nbs = nblk((long)A.nslices * 64);
-            hipLaunchKernelGGL(k_spmv_xpby, dim3(nbs), dim3(TPB), 0, c->stream, A.n, A.sl_ptr.p, A.col.p,
-                               A.dcol.n ? A.dcol.p : nullptr, A.val.p, A.diag.p, c->scal.p, z, p, p2, q,
-                               red_of(c));
-            if (prof) c->prof.end(c->stream, "spmv_xpby_fp64", ev, op_bytes(A, 4));
+            SEMO_LAUNCH(c, "spmv_xpby_fp64", op_bytes(A, 4), k_spmv_xpby, dim3(nbs), dim3(TPB), A.n, A.sl_ptr.p,
+                        A.col.p, A.dcol.n ? A.dcol.p : nullptr, A.val.p, A.diag.p, c->scal.p, z, p, p2, q,
+                        red_of(c));
             finish(c, nbs, W_FCG_ALPHA);
             std::swap(p, p2);
         }
-        hipLaunchKernelGGL(k_fcg_update, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->scal.p, x, r, rold, p, q,
-                           red_of(c), pp);
+        SEMO_LAUNCH(c, "fcg_update", 56.0 * n + prep_bytes, k_fcg_update, dim3(nblk(n)), dim3(TPB), n, c->scal.p, x,
+                    r, rold, p, q, red_of(c), pp);
         finish(c, nblk(n), W_STORE, S_RR);
         pipe_push(c, c->scal.p + S_RR, it);
         if (cc.done(it) || it == maxit) break;
@@ -987,7 +982,7 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
         dot_what(c, n, r, z, z, rold, W_FCG_BETA);                        // r.z, z.r_old -> beta
     }
     // true residual in FP64
-    spmv<double, OP_RES>(c, A, x, b, r);
+    spmv<double, OP_RES>(c, A, x, b, r, "true_resid_fp64");
     dot_store(c, n, r, r, nullptr, nullptr, S_X);
     *relres = std::sqrt(host_scalar(c, S_X)) / nb;
     return std::min(it, maxit);
@@ -1003,7 +998,7 @@ int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const do
     double* r = c->kv64.p; double* rh = r + ln; double* p = rh + ln; double* v = p + ln; double* sv = v + ln;
     double* t = sv + ln; double* ph = t + ln; double* sh = ph + ln;
     double* s = c->scal.p;
-    hipLaunchKernelGGL(k_bicg_init, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, b, x, r, rh, p, v, red_of(c));
+    SEMO_LAUNCH(c, "bicg_init", 48.0 * n, k_bicg_init, dim3(nblk(n)), dim3(TPB), n, b, x, r, rh, p, v, red_of(c));
     finish(c, nblk(n), W_BICG_INIT);
     double bb = host_scalar(c, S_X);
     if (bb == 0.0) { *relres = 0.0; return 0; }
@@ -1011,17 +1006,17 @@ int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const do
     int it;
     ConvCheck cc(c, tol * tol * bb, bb);
     for (it = 1; it <= maxit; ++it) {
-        hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, r, v, dinv, p, ph);
+        SEMO_LAUNCH(c, "bicg_p", 48.0 * n, k_bicg_p, dim3(nblk(n)), dim3(TPB), n, s, r, v, dinv, p, ph);
         spmv_dot(c, A, ph, v, rh, nullptr, W_BICG_ALPHA);                 // v = A ph, alpha
-        hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, r, v, dinv, sv, sh);
+        SEMO_LAUNCH(c, "bicg_s", 40.0 * n, k_bicg_s, dim3(nblk(n)), dim3(TPB), n, s, r, v, dinv, sv, sh);
         spmv_dot(c, A, sh, t, nullptr, sv, W_BICG_OMEGA);                 // t = A sh, omega
-        hipLaunchKernelGGL(k_bicg_x, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, s, ph, sh, sv, t, rh, x, r,
-                           red_of(c));
+        SEMO_LAUNCH(c, "bicg_x", 64.0 * n, k_bicg_x, dim3(nblk(n)), dim3(TPB), n, s, ph, sh, sv, t, rh, x, r,
+                    red_of(c));
         finish(c, nblk(n), W_BICG_RHO);
         pipe_push(c, c->scal.p + S_RR, it);
         if (cc.done(it)) break;
     }
-    spmv<double, OP_RES>(c, A, x, b, r);
+    spmv<double, OP_RES>(c, A, x, b, r, "true_resid_fp64");
     dot_store(c, n, r, r, nullptr, nullptr, S_X);
     *relres = std::sqrt(host_scalar(c, S_X)) / nb;
     return std::min(it, maxit);
@@ -1044,12 +1039,11 @@ void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const 
     double* part = c->red.p;
     const int16_t* dc = A.dcol.n ? A.dcol.p : nullptr;
     auto fin = [&](int blocks, int what) {
-        hipLaunchKernelGGL(k_finish6, dim3(1), dim3(1024), 0, c->stream, blocks, part, s, what);
+        SEMO_LAUNCH(c, "krylov_finish", 48.0 * blocks + 8.0, k_finish6, dim3(1), dim3(1024), blocks, part, s, what);
     };
     auto spmv3 = [&](int mode, const double* xin, double* y, const double* a) {
-        hipEvent_t ev{};
-        bool prof = c->prof.on;
-        if (prof) c->prof.begin(c->stream, ev);
+        // matrix once, three gathered operands, three results, three dot operands
+        ProfScope ps(c, mode == 2 ? "spmv3_res_fp64" : "spmv3_fp64", op_bytes(A, 9));
         if (mode == 0)
             hipLaunchKernelGGL(k_spmv3<0>, dim3(nbs), dim3(TPB), 0, c->stream, n, A.sl_ptr.p, A.col.p, dc, A.val.p,
                                A.diag.p, xin, y, a, part);
@@ -1059,10 +1053,8 @@ void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const 
         else
             hipLaunchKernelGGL(k_spmv3<2>, dim3(nbs), dim3(TPB), 0, c->stream, n, A.sl_ptr.p, A.col.p, dc, A.val.p,
                                A.diag.p, xin, y, a, part);
-        // matrix once, three gathered operands, three results, three dot operands
-        if (prof) c->prof.end(c->stream, mode == 2 ? "spmv3_res_fp64" : "spmv3_fp64", ev, op_bytes(A, 9));
     };
-    hipLaunchKernelGGL(k_bicg3_init, dim3(nb), dim3(TPB), 0, c->stream, n, b, x, r, rh, p, v, part);
+    SEMO_LAUNCH(c, "bicg3_init", 144.0 * n, k_bicg3_init, dim3(nb), dim3(TPB), n, b, x, r, rh, p, v, part);
     fin(nb, F3_INIT);
     double bb[3];
     SEMO_HIP(hipMemcpyAsync(bb, s + Q_BB, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1071,13 +1063,13 @@ void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const 
     if (bb[0] > 0.0 || bb[1] > 0.0 || bb[2] > 0.0) {
         ConvCheck cc(c, tol * tol, 1.0);
         for (it = 1; it <= maxit; ++it) {
-            hipLaunchKernelGGL(k_bicg3_p, dim3(nb), dim3(TPB), 0, c->stream, n, s, r, v, dinv, p, ph);
+            SEMO_LAUNCH(c, "bicg3_p", 128.0 * n, k_bicg3_p, dim3(nb), dim3(TPB), n, s, r, v, dinv, p, ph);
             spmv3(0, ph, v, rh);                                   // v = A ph, rh.v
             fin(nbs, F3_ALPHA);
-            hipLaunchKernelGGL(k_bicg3_s, dim3(nb), dim3(TPB), 0, c->stream, n, s, r, v, dinv, sv, sh);
+            SEMO_LAUNCH(c, "bicg3_s", 104.0 * n, k_bicg3_s, dim3(nb), dim3(TPB), n, s, r, v, dinv, sv, sh);
             spmv3(1, sh, t, sv);                                   // t = A sh, t.s, t.t
             fin(nbs, F3_OMEGA);
-            hipLaunchKernelGGL(k_bicg3_x, dim3(nb), dim3(TPB), 0, c->stream, n, s, ph, sh, sv, t, rh, x, r, part);
+            SEMO_LAUNCH(c, "bicg3_x", 192.0 * n, k_bicg3_x, dim3(nb), dim3(TPB), n, s, ph, sh, sv, t, rh, x, r, part);
             fin(nb, F3_RHO);
             pipe_push(c, s + Q_MAXREL, it);
             if (cc.done(it)) break;
@@ -1209,12 +1201,13 @@ void pressure_amg_update(semo_ctx* c, bool (*keep_coarse)(const Amg&)) {
     if (amg.built) {
         // device-side comparison with the values the hierarchy was built from
         DBuf<int>& flag = c->d_flag1;
-        flag.ensure(1); flag.zero(c->stream);
+        flag.ensure(1);
+        zero_d(c, flag.p, sizeof(int));
         size_t nv = c->Amat.val.n;
-        hipLaunchKernelGGL(k_neq, dim3((unsigned)((nv + TPB - 1) / TPB)), dim3(TPB), 0, c->stream, nv,
-                           c->Amat.val.p, c->amg_key_val.p, flag.p);
-        hipLaunchKernelGGL(k_neq, dim3((unsigned)((c->nc + TPB - 1) / TPB)), dim3(TPB), 0, c->stream,
-                           (size_t)c->nc, c->Amat.diag.p, c->amg_key_diag.p, flag.p);
+        SEMO_LAUNCH(c, "amg_values_changed", 16.0 * nv, k_neq, dim3((unsigned)((nv + TPB - 1) / TPB)), dim3(TPB), nv,
+                    c->Amat.val.p, c->amg_key_val.p, flag.p);
+        SEMO_LAUNCH(c, "amg_values_changed", 16.0 * c->nc, k_neq, dim3((unsigned)((c->nc + TPB - 1) / TPB)),
+                    dim3(TPB), (size_t)c->nc, c->Amat.diag.p, c->amg_key_diag.p, flag.p);
         int f = 0;
         SEMO_HIP(hipMemcpyAsync(&f, flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         SEMO_HIP(hipStreamSynchronize(c->stream));
@@ -1234,12 +1227,12 @@ void pressure_amg_update(semo_ctx* c, bool (*keep_coarse)(const Amg&)) {
     if (refresh) {
         AmgLevel& L0 = amg.lv[0];
         size_t nv = L0.A64.val.n;
-        hipLaunchKernelGGL(k_refresh_offdiag, dim3((unsigned)((nv + TPB - 1) / TPB)), dim3(TPB), 0, c->stream, nv,
-                           amg.l0_map.p, c->Amat.val.p, L0.A64.val.p, L0.A32.val.p);
-        hipLaunchKernelGGL(k_refresh_diag, dim3(nblk(c->nc)), dim3(TPB), 0, c->stream, c->nc, c->Amat.diag.p,
-                           L0.A64.diag.p, L0.A32.diag.p, L0.dinv64.p, L0.dinv32.p);
-        SEMO_HIP(hipMemcpyAsync(c->amg_key_val.p, c->Amat.val.p, c->Amat.val.bytes(), hipMemcpyDeviceToDevice, c->stream));
-        SEMO_HIP(hipMemcpyAsync(c->amg_key_diag.p, c->Amat.diag.p, c->Amat.diag.bytes(), hipMemcpyDeviceToDevice, c->stream));
+        SEMO_LAUNCH(c, "amg_refresh_L0", 24.0 * nv, k_refresh_offdiag, dim3((unsigned)((nv + TPB - 1) / TPB)),
+                    dim3(TPB), nv, amg.l0_map.p, c->Amat.val.p, L0.A64.val.p, L0.A32.val.p);
+        SEMO_LAUNCH(c, "amg_refresh_L0", 32.0 * c->nc, k_refresh_diag, dim3(nblk(c->nc)), dim3(TPB), c->nc,
+                    c->Amat.diag.p, L0.A64.diag.p, L0.A32.diag.p, L0.dinv64.p, L0.dinv32.p);
+        copy_d(c, c->amg_key_val.p, c->Amat.val.p, c->Amat.val.n);
+        copy_d(c, c->amg_key_diag.p, c->Amat.diag.p, c->Amat.diag.n);
         amg.fresh = false;
     }
     if (rebuild) {
@@ -1287,7 +1280,7 @@ void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st) {
     reduction_alloc(c, n);
     DBuf<double>& dinv = c->dinv_mom;
     dinv.alloc(n + 64);
-    hipLaunchKernelGGL(k_dinv, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, c->Amat.diag.p, dinv.p);
+    SEMO_LAUNCH(c, "jacobi_dinv", 16.0 * n, k_dinv, dim3(nblk(n)), dim3(TPB), n, c->Amat.diag.p, dinv.p);
     if (nrhs == 3) {
         bicgstab3(c, c->Amat, dinv.p, c->Bvec.p, c->Xvec.p, c->sp.tol_u, c->sp.maxit_u, st);
         return;
@@ -1365,7 +1358,7 @@ void solve_coo(semo_ctx* c, const char* eq, int n, int64_t nnz, const int* rows,
         SEMO_HIP(hipStreamSynchronize(c->stream));
     } else {
         DBuf<double> dinv; dinv.alloc(n + 64);
-        hipLaunchKernelGGL(k_dinv, dim3(nblk(n)), dim3(TPB), 0, c->stream, n, A.diag.p, dinv.p);
+        SEMO_LAUNCH(c, "jacobi_dinv", 16.0 * n, k_dinv, dim3(nblk(n)), dim3(TPB), n, A.diag.p, dinv.p);
         for (int k = 0; k < nrhs; ++k) {
             double rr = 0;
             int it = bicgstab(c, A, dinv.p, db.p + (size_t)k * n, dx.p + (size_t)k * n, c->sp.tol_u, c->sp.maxit_u, &rr);

@@ -227,6 +227,13 @@ class GpuSolver:
         self._ok(self._L.semo_objective_pressure(self._h, _ptr(w), C.byref(J)))
         return J.value
 
+    def volume_integral(self, name: str):
+        """(sum_i phi_i V_i, sum_i V_i) of a one-component cell field, in
+        FP64 on the device (VF: the first phase's volume)."""
+        a, v = C.c_double(), C.c_double()
+        self._ok(self._L.semo_volume_integral(self._h, F[name], C.byref(a), C.byref(v)))
+        return a.value, v.value
+
     def adjointAlphaP(self, n_iter: int, weights: dict):
         """Runs n_iter outer iterations (as incompressiblePressureBased) and
         returns (J of the final state, dJ/dalpha_p per cell) by reverse-mode

@@ -3,6 +3,8 @@
 The Function's backward is semo_adjoint_alpha_p.  Checked here:
   * J and alpha.grad from the autograd graph equal the adjoint entry point's
     own (J, dJ/dalpha) bit for bit, on a CUDA tensor;
+  * one solver serves repeated calls: after a reset the second call's J and
+    gradient equal the first's;
   * torch.autograd.gradcheck: the backward against torch's own central
     differences of the forward, every component, on a 4^3 jittered cavity
     (step 1e-4, solves at 1e-14; relative 1e-5 of the largest component).
@@ -37,21 +39,42 @@ def _factory(mesh):
     return make
 
 
+def _reset(g):
+    # the cavity at rest: the state a fresh solver starts from
+    n = g.mesh.n_cells
+    for f in ("U", "V", "W", "P"):
+        g.set_field(f, np.zeros(n))
+
+
 def test_autograd_matches_adjoint_entry_point():
     from semo_mesh_amd.autograd import pressure_objective
     mesh = cavity_mesh(8, jitter=0.1, seed=11)
     make = _factory(mesh)
     a0 = 10.0 * np.random.default_rng(5).random(mesh.n_cells)
     alpha = torch.tensor(a0, dtype=torch.float64, device="cuda", requires_grad=True)
-    J = pressure_objective(alpha, make, N_ITER, W)
-    J.backward()
-    g = make()
-    g.set_field("alphaP", a0)
-    J_ref, grad_ref = g.adjointAlphaP(N_ITER, W)
-    g.close()
-    assert J.device.type == "cuda" and alpha.grad.device.type == "cuda"
-    assert J.item() == J_ref
-    assert np.array_equal(alpha.grad.cpu().numpy(), grad_ref)
+    solver = make()
+    try:
+        J = pressure_objective(alpha, solver, N_ITER, W, _reset)
+        J.backward()
+        g = make()
+        g.set_field("alphaP", a0)
+        J_ref, grad_ref = g.adjointAlphaP(N_ITER, W)
+        g.close()
+        assert J.device.type == "cuda" and alpha.grad.device.type == "cuda"
+        assert J.item() == J_ref
+        assert np.array_equal(alpha.grad.cpu().numpy(), grad_ref)
+        # the same solver again, from the reset state
+        alpha.grad = None
+        J2 = pressure_objective(alpha, solver, N_ITER, W, _reset)
+        J2.backward()
+        assert abs(J2.item() - J_ref) <= 1e-12 * abs(J_ref)
+        assert np.abs(alpha.grad.cpu().numpy() - grad_ref).max() <= 1e-10 * np.abs(grad_ref).max()
+        # without a gradient: the plain iterations, the same J
+        with torch.no_grad():
+            J3 = pressure_objective(alpha.detach(), solver, N_ITER, W, _reset)
+        assert abs(J3.item() - J_ref) <= 1e-12 * abs(J_ref)
+    finally:
+        solver.close()
 
 
 def test_autograd_gradcheck():
@@ -65,5 +88,9 @@ def test_autograd_gradcheck():
     _, grad = g.adjointAlphaP(N_ITER, W)
     g.close()
     atol = 1e-5 * float(np.abs(grad).max())
-    assert torch.autograd.gradcheck(lambda a: pressure_objective(a, make, N_ITER, W), (alpha,),
-                                    eps=1e-4, atol=atol, rtol=1e-5)
+    solver = make()
+    try:
+        assert torch.autograd.gradcheck(lambda a: pressure_objective(a, solver, N_ITER, W, _reset), (alpha,),
+                                        eps=1e-4, atol=atol, rtol=1e-5)
+    finally:
+        solver.close()
