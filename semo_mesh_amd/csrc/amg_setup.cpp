@@ -364,7 +364,7 @@ void amg_build(semo_ctx* c, Amg& amg, const DevSell<double>& Ad) {
             std::vector<std::pair<int, std::pair<double, int>>> row;
             row.push_back({r, {diag[r], -1}});
             for (int k = 0; k < w; ++k) {
-                int e = p.sl_ptr[s] + k * SL + lane;
+                int e = sell_pos(p.sl_ptr[s], k, lane);
                 if (p.col[e] == r) continue;     // padding
                 row.push_back({p.col[e], {val[e], e}});
             }

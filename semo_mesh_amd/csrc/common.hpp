@@ -111,13 +111,21 @@ struct Profiler {
 
 // ------------------------------------------------------- sliced-ELL matrix ---
 // Off-diagonal entries in slices of 64 rows (one wavefront per slice, one lane
-// per row), column-major inside a slice, plus a separate diagonal.  Padding
+// per row), in entry pairs inside a slice (sell_pos), plus a separate diagonal.  Padding
 // entries have val = 0 and a column inside [0, ncols) (the row itself for
 // square matrices, 0 for rectangular ones).
 // Operators with long or ragged rows (AMG restriction and coarse levels) use
 // CSR instead (lanes > 0): sl_ptr then holds the n+1 row offsets and `lanes`
 // consecutive lanes of a wavefront share one row.
 constexpr int SL = 64;
+// Inside a slice the entries go in pairs: pair m (entries 2m, 2m+1 of each
+// row) of the 64 rows is 128 consecutive elements, row r's two at 2r, 2r+1,
+// so one 16-byte (FP64) or 8-byte (FP32) load per lane brings two entries of
+// its row and a wavefront's load is one contiguous 1 KB / 512 B run.  Slice
+// widths are even (a padding entry completes an odd one).
+inline int sell_pos(int base, int k, int lane) { return base + (k >> 1) * (2 * SL) + 2 * lane + (k & 1); }
+inline int sell_lane(int base, int e) { return ((e - base) >> 1) & (SL - 1); }
+inline int sell_entry(int base, int e) { return (((e - base) / (2 * SL)) << 1) | ((e - base) & 1); }
 struct SellPattern {
     int n = 0, ncols = 0, nslices = 0;
     std::vector<int> sl_ptr;    // element offset of each slice (nslices+1)

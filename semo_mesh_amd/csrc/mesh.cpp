@@ -139,6 +139,7 @@ SellPattern sell_from_csr(int n, int ncols, const std::vector<int>& ptr, const s
     for (int s = 0; s < p.nslices; ++s) {
         int w = 0;
         for (int r = s * SL; r < std::min(n, (s + 1) * SL); ++r) w = std::max(w, ptr[r + 1] - ptr[r]);
+        w += w & 1;                            // entry pairs
         p.sl_ptr[s + 1] = p.sl_ptr[s] + w * SL;
     }
     p.col.assign(p.sl_ptr[p.nslices], 0);
@@ -148,7 +149,7 @@ SellPattern sell_from_csr(int n, int ncols, const std::vector<int>& ptr, const s
         for (int lane = 0; lane < SL; ++lane) {
             int r = s * SL + lane;
             for (int k = 0; k < w; ++k) {
-                int e = p.sl_ptr[s] + k * SL + lane;
+                int e = sell_pos(p.sl_ptr[s], k, lane);
                 if (r < n && k < ptr[r + 1] - ptr[r]) {
                     p.col[e] = col[ptr[r] + k];
                     if (pos) (*pos)[ptr[r] + k] = e;

@@ -167,10 +167,10 @@ struct RowArgs {
     T c1, c2;
 };
 
-// entries per trip of the row sums: 4 on sliced-ELL slices (rows of ~6-7
-// entries), 8 per lane of the CSR kernels (coarse rows of 30-60 entries over
-// avg/4 lanes); measured best of 1/4/8 and 1/2/4/8 on MI355X
-constexpr int SELL_UNR = 4, CSR_UNR = 8;
+// entries per trip of the row sums: 2 pairs on sliced-ELL slices (rows of
+// ~6-7 entries), 8 per lane of the CSR kernels (coarse rows of 30-60 entries
+// over avg/4 lanes; measured best of 1/2/4/8 on MI355X)
+constexpr int SELL_PAIRS = 2, CSR_UNR = 8;
 // One lane's sum over its row of a sliced-ELL slice, entries e0, e0 + 64, ...
 // below e1, in entry order: acc += val[e] * opnd(col(e)).  U entries per
 // trip: their indices and values are loaded first, then their gathers are
@@ -196,6 +196,45 @@ __device__ inline T sell_row_sum(int e0, int e1, const T* __restrict__ val, ColF
             if (j[u] >= 0) acc += v[u] * x[u];
     }
     return acc;
+}
+
+// One lane's sum over its row of a sliced-ELL slice (entry pairs, sell_pos):
+// pair m of the row at base + 128 m + 2 lane, P pairs per trip, their values
+// and columns loaded (one load each per pair) before their gathers are
+// issued together.  acc += val * opnd(col) in entry order, bit for bit the
+// sum of the one-entry loop.  colf2(e) gives the columns of entries e, e + 1.
+template <int P, class T, class ColF2, class OpF>
+__device__ inline T sell2_row_sum(int base, int end, int lane, const T* __restrict__ val, ColF2 colf2, OpF opnd) {
+    using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+    T acc = 0;
+    for (int e = base + 2 * lane; e < end; e += 2 * SL * P) {
+        int2 j[P];
+        T2 v[P];
+        T x0[P], x1[P];
+#pragma unroll
+        for (int u = 0; u < P; ++u) {
+            int ee = e + 2 * SL * u;
+            bool in = ee < end;
+            j[u] = in ? colf2(ee) : make_int2(-1, -1);
+            if (in) v[u] = *reinterpret_cast<const T2*>(val + ee);
+        }
+#pragma unroll
+        for (int u = 0; u < P; ++u)
+            if (j[u].x >= 0) { x0[u] = opnd(j[u].x); x1[u] = opnd(j[u].y); }
+#pragma unroll
+        for (int u = 0; u < P; ++u)
+            if (j[u].x >= 0) { acc += v[u].x * x0[u]; acc += v[u].y * x1[u]; }
+    }
+    return acc;
+}
+// the two columns of entries e, e + 1 (e even) from int32 columns or from
+// int16 offsets column - row
+__device__ inline int2 sell2_col(const int* __restrict__ col, int e) {
+    return *reinterpret_cast<const int2*>(col + e);
+}
+__device__ inline int2 sell2_dcol(const int16_t* __restrict__ dcol, int e, int row) {
+    int w = *reinterpret_cast<const int*>(dcol + e);
+    return make_int2(row + (int)(int16_t)(w & 0xffff), row + (w >> 16));
 }
 
 // G == 0: sliced ELL, one lane per row.  G > 0: CSR, G consecutive lanes per
@@ -231,11 +270,11 @@ __global__ void __launch_bounds__(TPB) k_row(RowArgs<T> a) {
         if (row < a.n) own = load_own<T, OP>(a, row);
         auto opnd = [&](int j) { return a.x[j]; };
         if (a.dcol)
-            acc = sell_row_sum<SELL_UNR, 64>(a.ptr[s] + lane, a.ptr[s + 1], a.val,
-                                              [&](int e) { return row + (int)a.dcol[e]; }, opnd);
+            acc = sell2_row_sum<SELL_PAIRS>(a.ptr[s], a.ptr[s + 1], lane, a.val,
+                                            [&](int e) { return sell2_dcol(a.dcol, e, row); }, opnd);
         else
-            acc = sell_row_sum<SELL_UNR, 64>(a.ptr[s] + lane, a.ptr[s + 1], a.val,
-                                              [&](int e) { return a.col[e]; }, opnd);
+            acc = sell2_row_sum<SELL_PAIRS>(a.ptr[s], a.ptr[s + 1], lane, a.val,
+                                            [&](int e) { return sell2_col(a.col, e); }, opnd);
         if (row >= a.n) return;
     } else {
         row = gid / G;
@@ -327,10 +366,10 @@ __global__ void __launch_bounds__(TPB) k_spmv_dot(int n, const int* __restrict__
         bool own = row < n;     // the row's own operands before the gathers
         double dr = own ? diag[row] : 0.0, xr = own ? x[row] : 0.0;
         auto opnd = [&](int j) { return x[j]; };
-        double acc = dcol ? sell_row_sum<SELL_UNR, 64>(sl_ptr[s] + lane, sl_ptr[s + 1], val,
-                                                        [&](int e) { return row + (int)dcol[e]; }, opnd)
-                          : sell_row_sum<SELL_UNR, 64>(sl_ptr[s] + lane, sl_ptr[s + 1], val,
-                                                        [&](int e) { return col[e]; }, opnd);
+        double acc = dcol ? sell2_row_sum<SELL_PAIRS>(sl_ptr[s], sl_ptr[s + 1], lane, val,
+                                                      [&](int e) { return sell2_dcol(dcol, e, row); }, opnd)
+                          : sell2_row_sum<SELL_PAIRS>(sl_ptr[s], sl_ptr[s + 1], lane, val,
+                                                      [&](int e) { return sell2_col(col, e); }, opnd);
         if (own) {
             acc += dr * xr;
             y[row] = acc;
@@ -359,10 +398,10 @@ __global__ void __launch_bounds__(TPB) k_spmv_xpby(int n, const int* __restrict_
         bool own = row < n;
         double zr = own ? z[row] : 0.0, por = own ? po[row] : 0.0, dr = own ? diag[row] : 0.0;
         auto opnd = [&](int j) { return z[j] + beta * po[j]; };
-        double acc = dcol ? sell_row_sum<SELL_UNR, 64>(sl_ptr[sl] + lane, sl_ptr[sl + 1], val,
-                                                        [&](int e) { return row + (int)dcol[e]; }, opnd)
-                          : sell_row_sum<SELL_UNR, 64>(sl_ptr[sl] + lane, sl_ptr[sl + 1], val,
-                                                        [&](int e) { return col[e]; }, opnd);
+        double acc = dcol ? sell2_row_sum<SELL_PAIRS>(sl_ptr[sl], sl_ptr[sl + 1], lane, val,
+                                                      [&](int e) { return sell2_dcol(dcol, e, row); }, opnd)
+                          : sell2_row_sum<SELL_PAIRS>(sl_ptr[sl], sl_ptr[sl + 1], lane, val,
+                                                      [&](int e) { return sell2_col(col, e); }, opnd);
         if (own) {
             double pr = zr + beta * por;
             pn[row] = pr;
@@ -626,10 +665,11 @@ __global__ void __launch_bounds__(TPB) k_spmv3(int n, const int* __restrict__ sl
         double d = own ? diag[row] : 0.0;
         double xr0 = own ? x[row] : 0.0, xr1 = own ? x[N + row] : 0.0, xr2 = own ? x[2 * N + row] : 0.0;
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-        for (int e = sl_ptr[s] + lane; e < sl_ptr[s + 1]; e += 64) {
-            int j = dcol ? row + dcol[e] : col[e];
-            double v = val[e];
-            acc0 += v * x[j]; acc1 += v * x[N + j]; acc2 += v * x[2 * N + j];
+        for (int e = sl_ptr[s] + 2 * lane; e < sl_ptr[s + 1]; e += 2 * SL) {     // entry pairs
+            int2 j = dcol ? sell2_dcol(dcol, e, row) : sell2_col(col, e);
+            double2 v = *reinterpret_cast<const double2*>(val + e);
+            acc0 += v.x * x[j.x]; acc1 += v.x * x[N + j.x]; acc2 += v.x * x[2 * N + j.x];
+            acc0 += v.y * x[j.y]; acc1 += v.y * x[N + j.y]; acc2 += v.y * x[2 * N + j.y];
         }
         if (own) {
             double acc[3] = {acc0 + d * xr0, acc1 + d * xr1, acc2 + d * xr2};
@@ -1148,7 +1188,7 @@ void sell_narrow(const SellPattern& p, DBuf<int16_t>& d, hipStream_t s) {
     std::vector<int16_t> h(p.col.size());
     for (int sl = 0; sl < p.nslices; ++sl)
         for (int e = p.sl_ptr[sl]; e < p.sl_ptr[sl + 1]; ++e) {
-            int row = sl * SL + ((e - p.sl_ptr[sl]) & (SL - 1));
+            int row = sl * SL + sell_lane(p.sl_ptr[sl], e);
             long dv = row < p.n ? (long)p.col[e] - row : (long)(p.n - 1) - row;
             if (dv < -32767 || dv > 32767) return;
             h[e] = (int16_t)dv;
