@@ -14,6 +14,8 @@
 #  * math.cpp uses the Intel compiler's built-in __int64 type and a redeclared
 #    default argument; g++ accepts both with -D__int64="long long" -fpermissive.
 #  * Every solveAMGCL call is intercepted with -Wl,--wrap (see ref_driver.cpp).
+#  * semo_initial: SEMO's initial-field utility (src/utility/initial.cpp) built
+#    as it is, for the region-initialiser fixtures of tests/golden.
 #  * semo_gpu_eq / semo_gpu_step: the same driver and reference objects with
 #    the C++ bridge of semo_mesh_amd/bridge linked in place of the equation
 #    files (eq: SEMO's own incompressiblePressureBased, its equation calls on
@@ -35,7 +37,8 @@ SRCS := mesh/build mesh/geometric math/math math/gradient math/RCM mpi/build \
         solvers/solveAMGCL solvers/solvePETSc solvers/timestep solvers/eos solvers/transport \
         solvers/NVD solvers/cellVarMinMax solvers/surfaceNormalVelocity solvers/pressureBased \
         solvers/source/gravity solvers/source/surfaceTension solvers/source/curvature \
-        turbulenceModels/LES
+        turbulenceModels/LES \
+        save/save save/saveAscii save/saveBinary save/saveCompress save/saveBase64 save/saveMkdirs
 OBJS := $(foreach s,$(SRCS),$(OBJ)/$(subst /,_,$(s)).o)
 
 SOLVE1 := _ZN20SEMO_Solvers_Builder10solveAMGCLENSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEER17SEMO_Mesh_BuilderiRSt6vectorIiSaIiEESB_RS8_IdSaIdEESE_SE_
@@ -48,7 +51,7 @@ BOBJS_EQ := $(filter-out $(EQ_OBJS),$(OBJS))
 BOBJS_STEP := $(filter-out $(OBJ)/solvers_pressureBased.o,$(BOBJS_EQ))
 MPILIBS := $(MPIDIR)/lib/libmpicxx.so.12 $(MPIDIR)/lib/libmpi.so.12
 
-all: $(OUT)/semo_ref $(OUT)/rcm_ref $(OUT)/lib/.stamp $(OUT)/semo_gpu_eq $(OUT)/semo_gpu_step
+all: $(OUT)/semo_ref $(OUT)/rcm_ref $(OUT)/lib/.stamp $(OUT)/semo_gpu_eq $(OUT)/semo_gpu_step $(OUT)/semo_initial
 
 $(OBJ)/.dir:
 	mkdir -p $(OBJ) && touch $@
@@ -81,6 +84,14 @@ $(OUT)/semo_gpu_step: $(BOBJS_STEP) $(OBJ)/gpuBridge.o $(OBJ)/gpuBridgeStep.o $(
 	$(CXX) -o $@ $(BOBJS_STEP) $(OBJ)/gpuBridge.o $(OBJ)/gpuBridgeStep.o $(OBJ)/ref_driver.o \
 	  -Wl,--wrap=$(SOLVE1) -Wl,--wrap=$(SOLVE3) \
 	  $(GPULIB) $(MPILIBS) -lz -Wl,-rpath,'$$ORIGIN/lib:$$ORIGIN/../../semo_mesh_amd' -Wl,--allow-shlib-undefined
+
+# SEMO's initial-field utility (src/utility/initial.cpp, its own main) as it
+# is, for the region-initialiser fixtures
+$(OBJ)/utility_initial.o: $(REF)/src/utility/initial.cpp $(OBJ)/.dir
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/semo_initial: $(OBJS) $(OBJ)/utility_initial.o
+	$(CXX) -o $@ $^ $(MPILIBS) -lz -ldl -Wl,-rpath,'$$ORIGIN/lib' -Wl,--allow-shlib-undefined
 
 $(OBJ)/rcm_ref_main.o: $(HERE)rcm_ref_main.cpp $(OBJ)/.dir
 	$(CXX) $(CXXFLAGS) -c $< -o $@

@@ -21,12 +21,15 @@
 #include <vector>
 #include <chrono>
 #include <sys/stat.h>
+#include <unistd.h>
 
+#include "load/load.h"
 #include "mesh/build.h"
 #include "mesh/geometric.h"
 #include "controls/build.h"
 #include "solvers/build.h"
 #include "math/math.h"
+#include "save/save.h"
 
 using std::string;
 using std::vector;
@@ -147,15 +150,62 @@ struct PatchBC {
     double pval = 0, u[3] = {0, 0, 0}, vfval = 0;
 };
 
+// ---- VTU modes ---------------------------------------------------------------
+// savevtu FORMAT COMPRESSION: the initial state written by SEMO's own writer
+// (SEMO_Mesh_Save::vtu, src/save/save.cpp:48-767) into OUTDIR/save/0.25/,
+// with point levels i % 4, cell levels i % 3 and cell groups i % 5, time
+// 0.25, 17 significant digits in the ascii format.
+// loadvtu FOLDER: FOLDER/plot.0.vtu read by SEMO's own loader
+// (SEMO_Mesh_Load::vtu, src/load/load.cpp:114-837) and what it built dumped
+// (vtu_* arrays; mesh.raw is not read).
+static void dump_loaded(SEMO_Mesh_Builder& m, SEMO_Controls_Builder& c) {
+    vector<double> pts;
+    for (auto& p : m.points) { pts.push_back(p.x); pts.push_back(p.y); pts.push_back(p.z); }
+    dump("vtu_points", pts);
+    vector<int> own, nbr, fptr{0}, fpts, cptr{0}, cpts, cfptr{0}, cfs, plev, clev, cgrp, bst, bnf, bname;
+    for (auto& f : m.faces) {
+        own.push_back(f.owner); nbr.push_back(f.neighbour);
+        for (int q : f.points) fpts.push_back(q);
+        fptr.push_back((int)fpts.size());
+    }
+    for (auto& cl : m.cells) {
+        for (int q : cl.points) cpts.push_back(q);
+        cptr.push_back((int)cpts.size());
+        for (int q : cl.faces) cfs.push_back(q);
+        cfptr.push_back((int)cfs.size());
+        clev.push_back(cl.level); cgrp.push_back(cl.group);
+    }
+    for (auto& p : m.points) plev.push_back(p.level);
+    for (auto& b : m.boundary) {
+        bst.push_back(b.startFace); bnf.push_back(b.nFaces);
+        for (char ch : b.name) bname.push_back((int)ch);
+        bname.push_back(0);
+    }
+    dump("vtu_owner", own); dump("vtu_neighbour", nbr); dump("vtu_face_ptr", fptr); dump("vtu_face_pts", fpts);
+    dump("vtu_cell_ptr", cptr); dump("vtu_cell_pts", cpts); dump("vtu_cell_face_ptr", cfptr);
+    dump("vtu_cell_faces", cfs); dump("vtu_point_levels", plev); dump("vtu_cell_levels", clev);
+    dump("vtu_cell_groups", cgrp); dump("vtu_bc_start", bst); dump("vtu_bc_nfaces", bnf); dump("vtu_bc_names", bname);
+    dump_cellvar(m, c.P, "vtu_P"); dump_cellvar(m, c.U, "vtu_U"); dump_cellvar(m, c.V, "vtu_V");
+    dump_cellvar(m, c.W, "vtu_W"); dump_cellvar(m, c.T, "vtu_T"); dump_cellvar(m, c.VF[0], "vtu_VF");
+    dump_cellvar(m, c.MF[0], "vtu_MF");
+    dump("vtu_time", vector<double>{c.time});
+}
+
 int main(int argc, char** argv) {
     if (argc < 4) {
-        fprintf(stderr, "usage: %s mesh.raw case.txt outdir [nodump]\n", argv[0]);
+        fprintf(stderr, "usage: %s mesh.raw case.txt outdir [nodump | savevtu FORMAT COMPRESSION | "
+                        "loadvtu FOLDER]\n", argv[0]);
         return 2;
     }
     MPI::Init();
     g_out = argv[3];
     mkdir(g_out.c_str(), 0755);
     if (argc > 4 && !strcmp(argv[4], "nodump")) g_dump = false;
+    string mode = argc > 4 ? argv[4] : "";
+    if ((mode == "savevtu" && argc < 7) || (mode == "loadvtu" && argc < 6)) {
+        fprintf(stderr, "savevtu needs FORMAT COMPRESSION, loadvtu needs FOLDER\n");
+        return 2;
+    }
 
     // case file: "key value..." lines
     double dt = 0.01, rho[2] = {1000, 1000}, mu[2] = {1e-3, 1e-3}, sigma = 0, g[3] = {0, 0, 0};
@@ -216,6 +266,17 @@ int main(int argc, char** argv) {
     controls.iterVofMax = nvof;
     controls.turbType = "laminar";
     controls.startClock = 0.0;
+
+    if (mode == "loadvtu") {
+        SEMO_Mesh_Builder m2;
+        SEMO_Mesh_Load load;
+        load.vtu(argv[5], m2, controls, species);
+        dump_loaded(m2, controls);
+        printf("SEMO_REF_DONE loadvtu\n");
+        fflush(stdout);
+        MPI_Finalize();
+        return 0;
+    }
 
     // ---- mesh (load.cpp:352-438 semantics) ----
     SEMO_Mesh_Builder mesh;
@@ -303,6 +364,23 @@ int main(int argc, char** argv) {
         cl.var[controls.T] = 300.0;
         cl.var[controls.VF[0]] = init[4 * nc + i]; cl.var[controls.VF[1]] = 1.0 - init[4 * nc + i];
         cl.var[controls.MF[0]] = init[4 * nc + i]; cl.var[controls.MF[1]] = 1.0 - init[4 * nc + i];
+    }
+
+    if (mode == "savevtu") {
+        for (size_t i = 0; i < mesh.points.size(); ++i) mesh.points[i].level = (int)(i % 4);
+        for (size_t i = 0; i < nc; ++i) { mesh.cells[i].level = (int)(i % 3); mesh.cells[i].group = (int)(i % 5); }
+        controls.saveFormat = argv[5];
+        controls.saveCompression = atoi(argv[6]);
+        controls.writePrecision = 17;
+        controls.time = 0.25;
+        if (chdir(g_out.c_str()) != 0) { perror("chdir"); return 2; }
+        mkdir("save", 0755);
+        SEMO_Mesh_Save save;
+        save.vtu("./save/0.25/", mesh, controls, species);
+        printf("SEMO_REF_DONE savevtu\n");
+        fflush(stdout);
+        MPI_Finalize();
+        return 0;
     }
 
     SEMO_Solvers_Builder solvers;

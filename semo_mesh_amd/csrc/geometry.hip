@@ -36,29 +36,19 @@ __device__ inline double sqrt_x87(double x) {
 }
 
 // calcUnitNormals_Area3dPolygon (geometric.cpp:16-204): 5 passes of a fan
-// triangulation about the running area-weighted centroid.
-__global__ void __launch_bounds__(TPB) k_face_polygon(
-    int nf, const int* __restrict__ fptr, const int* __restrict__ fpts,
-    const double* __restrict__ pts, double* __restrict__ area, double* __restrict__ nrm,
-    double* __restrict__ fxyz, double* __restrict__ vsn_out, double* __restrict__ cc_out,
-    int* __restrict__ flags) {
-    int f = xcd_block() * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    int b0 = fptr[f], n = fptr[f + 1] - b0;
-    if (n < 3 || n > 16) { atomicOr(flags, FLAG_BAD_FACE); return; }
-    double X[16][3];
-    for (int k = 0; k < n; ++k) {
-        int p = fpts[b0 + k];
-        X[k][0] = pts[3 * (size_t)p]; X[k][1] = pts[3 * (size_t)p + 1]; X[k][2] = pts[3 * (size_t)p + 2];
-    }
+// triangulation about the running area-weighted centroid.  Faces of up to
+// 16 points keep their coordinates in registers; larger ones (polyhedral
+// meshes) read them from the point array at every use, the same arithmetic.
+template <class PtF>
+__device__ inline void face_polygon(int n, PtF X, double& ar, double& vs, double* un, double* xt, double* ct) {
     double avg[3];
     for (int a = 0; a < 3; ++a) {
         double s = 0.0;
-        for (int k = 0; k < n; ++k) s += X[k][a];
+        for (int k = 0; k < n; ++k) s += X(k, a);
         avg[a] = s / (double)n;
     }
-    double xt[3] = {0, 0, 0}, V0o[3] = {0, 0, 0}, un[3] = {0, 0, 0}, ct[3] = {0, 0, 0};
-    double ar = 0.0, vs = 0.0;
+    double V0o[3] = {0, 0, 0};
+    xt[0] = xt[1] = xt[2] = 0.0;
     for (int two = 0; two < 5; ++two) {
         un[0] = un[1] = un[2] = 0.0; ct[0] = ct[1] = ct[2] = 0.0; vs = 0.0;
         double V0[3];
@@ -67,16 +57,17 @@ __global__ void __launch_bounds__(TPB) k_face_polygon(
         double tot = 0.0;
         for (int i = 0; i < n; ++i) {
             int c = (i == n - 1) ? 0 : i + 1;
-            double A0 = X[i][0] - V0[0], A1 = X[i][1] - V0[1], A2 = X[i][2] - V0[2];
-            double B0 = X[c][0] - V0[0], B1 = X[c][1] - V0[1], B2 = X[c][2] - V0[2];
+            double Xi[3] = {X(i, 0), X(i, 1), X(i, 2)}, Xc[3] = {X(c, 0), X(c, 1), X(c, 2)};
+            double A0 = Xi[0] - V0[0], A1 = Xi[1] - V0[1], A2 = Xi[2] - V0[2];
+            double B0 = Xc[0] - V0[0], B1 = Xc[1] - V0[1], B2 = Xc[2] - V0[2];
             double Nv[3] = {0.5 * (A1 * B2 - A2 * B1), 0.5 * (A2 * B0 - A0 * B2), 0.5 * (A0 * B1 - A1 * B0)};
             un[0] += Nv[0]; un[1] += Nv[1]; un[2] += Nv[2];
             double ta = sqrt_x87(Nv[0] * Nv[0] + Nv[1] * Nv[1] + Nv[2] * Nv[2]);
-            for (int a = 0; a < 3; ++a) xt[a] += ((V0[a] + X[i][a] + X[c][a]) / 3.0) * ta;
+            for (int a = 0; a < 3; ++a) xt[a] += ((V0[a] + Xi[a] + Xc[a]) / 3.0) * ta;
             tot += ta;
             vs += V0[0] * Nv[0]; vs += V0[1] * Nv[1]; vs += V0[2] * Nv[2];
             for (int a = 0; a < 3; ++a) {
-                double s1 = V0[a] + X[i][a], s2 = X[i][a] + X[c][a], s3 = X[c][a] + V0[a];
+                double s1 = V0[a] + Xi[a], s2 = Xi[a] + Xc[a], s3 = Xc[a] + V0[a];
                 ct[a] += 2.0 * Nv[a] * (s1 * s1 + s2 * s2 + s3 * s3);
             }
         }
@@ -84,6 +75,30 @@ __global__ void __launch_bounds__(TPB) k_face_polygon(
         un[0] /= mag; un[1] /= mag; un[2] /= mag;
         ar = mag;
         for (int a = 0; a < 3; ++a) xt[a] /= tot;
+    }
+}
+
+constexpr int FACE_REG_POINTS = 16;
+
+__global__ void __launch_bounds__(TPB) k_face_polygon(
+    int nf, const int* __restrict__ fptr, const int* __restrict__ fpts,
+    const double* __restrict__ pts, double* __restrict__ area, double* __restrict__ nrm,
+    double* __restrict__ fxyz, double* __restrict__ vsn_out, double* __restrict__ cc_out,
+    int* __restrict__ flags) {
+    int f = xcd_block() * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    int b0 = fptr[f], n = fptr[f + 1] - b0;
+    if (n < 3) { atomicOr(flags, FLAG_BAD_FACE); return; }
+    double ar = 0.0, vs = 0.0, un[3], xt[3], ct[3];
+    if (n <= FACE_REG_POINTS) {
+        double R[FACE_REG_POINTS][3];
+        for (int k = 0; k < n; ++k) {
+            int p = fpts[b0 + k];
+            R[k][0] = pts[3 * (size_t)p]; R[k][1] = pts[3 * (size_t)p + 1]; R[k][2] = pts[3 * (size_t)p + 2];
+        }
+        face_polygon(n, [&](int k, int a) { return R[k][a]; }, ar, vs, un, xt, ct);
+    } else {
+        face_polygon(n, [&](int k, int a) { return pts[3 * (size_t)fpts[b0 + k] + a]; }, ar, vs, un, xt, ct);
     }
     if (!(ar >= 2.2250738585072014e-308)) atomicOr(flags, FLAG_ZERO_AREA);
     area[f] = ar;
@@ -238,7 +253,7 @@ void geometry_compute(semo_ctx* c) {
     SEMO_HIP(hipMemcpyAsync(&flags, c->d_flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
     SEMO_HIP(hipStreamSynchronize(s));
     c->f_vsn.release(); c->f_cc.release();
-    SEMO_CHECK(!(flags & FLAG_BAD_FACE), SEMO_EINVAL, "face with fewer than 3 or more than 16 points");
+    SEMO_CHECK(!(flags & FLAG_BAD_FACE), SEMO_EINVAL, "face with fewer than 3 points");
     SEMO_CHECK(!(flags & FLAG_ZERO_AREA), SEMO_EZERO_AREA, "zero face area");
     SEMO_CHECK(!(flags & FLAG_NEG_VOLUME), SEMO_ENEG_VOLUME, "non-positive cell volume");
     SEMO_CHECK(!(flags & FLAG_SINGULAR_LSQ), SEMO_ESINGULAR_LSQ, "singular least-squares matrix (detA = 0)");

@@ -174,7 +174,7 @@ void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
     for (int f = 0; f < nf; ++f) {
         SEMO_CHECK(m->owner[f] >= 0 && m->owner[f] < nc, SEMO_EINVAL, "owner out of range");
         int k = m->face_ptr[f + 1] - m->face_ptr[f];
-        SEMO_CHECK(k >= 3 && k <= 16, SEMO_EINVAL, "faces need 3..16 points");
+        SEMO_CHECK(k >= 3, SEMO_EINVAL, "faces need at least 3 points");
     }
     for (int f = 0; f < nif; ++f)
         SEMO_CHECK(m->neighbour[f] >= 0 && m->neighbour[f] < nc && m->neighbour[f] != m->owner[f],

@@ -351,3 +351,41 @@ def rotated(mesh: Mesh, seed: int, shift=(0.0, 0.0, 0.0)) -> Mesh:
                 face_pts=mesh.face_pts, cell_ptr=mesh.cell_ptr, cell_pts=mesh.cell_pts,
                 patch_names=list(mesh.patch_names), patch_start=mesh.patch_start,
                 patch_nfaces=mesh.patch_nfaces, shape=mesh.shape)
+
+
+def prism_stack_mesh(sides: int = 20, layers: int = 3, radius: float = 1.0, height: float = 0.3,
+                     jitter: float = 0.0, seed: int = DEFAULT_SEED) -> Mesh:
+    """`layers` prisms over a regular `sides`-gon stacked along z: polyhedral
+    cells whose caps (the internal faces between layers and the patches
+    "bottom" / "top") have `sides` points, side walls of quads (patch
+    "side").  Every point moves by a seeded U(-jitter, jitter) times the
+    polygon's edge length in each coordinate, so faces are not planar.
+    Faces point owner -> neighbour (internal: +z), boundary faces outward."""
+    rng = np.random.default_rng(seed)
+    th = 2.0 * np.pi * np.arange(sides) / sides
+    edge = 2.0 * radius * np.sin(np.pi / sides)
+    pts = np.array([[radius * np.cos(t), radius * np.sin(t), l * height]
+                    for l in range(layers + 1) for t in th])
+    pts += jitter * edge * rng.uniform(-1.0, 1.0, pts.shape)
+    ring = lambda l: [l * sides + i for i in range(sides)]     # noqa: E731  counter-clockwise seen from +z
+    faces, owner, nbr = [], [], []
+    for l in range(1, layers):                     # internal caps, owner below
+        faces.append(ring(l)); owner.append(l - 1); nbr.append(l)
+    names, starts, counts = ["bottom", "top", "side"], [], []
+    starts.append(len(faces)); faces.append(ring(0)[::-1]); owner.append(0); counts.append(1)
+    starts.append(len(faces)); faces.append(ring(layers)); owner.append(layers - 1); counts.append(1)
+    starts.append(len(faces))
+    for l in range(layers):
+        lo, hi = ring(l), ring(l + 1)
+        for i in range(sides):
+            j = (i + 1) % sides
+            faces.append([lo[i], lo[j], hi[j], hi[i]]); owner.append(l)
+    counts.append(len(faces) - starts[2])
+    face_ptr = np.concatenate([[0], np.cumsum([len(f) for f in faces])]).astype(np.int32)
+    cells = [ring(l) + ring(l + 1) for l in range(layers)]
+    cell_ptr = np.concatenate([[0], np.cumsum([len(c) for c in cells])]).astype(np.int32)
+    return Mesh(points=pts, owner=np.array(owner, dtype=np.int32), neighbour=np.array(nbr, dtype=np.int32),
+                face_ptr=face_ptr, face_pts=np.array([p for f in faces for p in f], dtype=np.int32),
+                cell_ptr=cell_ptr, cell_pts=np.array([p for c in cells for p in c], dtype=np.int32),
+                patch_names=names, patch_start=np.array(starts, dtype=np.int32),
+                patch_nfaces=np.array(counts, dtype=np.int32))

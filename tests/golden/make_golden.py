@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 from semo_mesh_amd.cases import (BC_FIXED_VALUE, BC_FUNCTION, BC_INLET_OUTLET, BC_NO_SLIP,  # noqa: E402
                                   BC_SLIP, BC_SURFACE_NORMAL_FIXED_VALUE, BC_SWITCH, BC_ZERO_GRADIENT,
                                   PatchBC, cavity_case, dam_break_case)
-from semo_mesh_amd.mesh import cavity_mesh, hex_mesh, rotated, tet_mesh  # noqa: E402
+from semo_mesh_amd.mesh import cavity_mesh, hex_mesh, prism_stack_mesh, rotated, tet_mesh  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
 
@@ -66,13 +66,11 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
                 continue
             arrs[base] = np.fromfile(os.path.join(out, fn),
                                      dtype=np.float64 if ext == ".f64" else np.int32)
+    arrs.update(mesh_arrays(mesh))
     arrs.update(dict(
-        mesh_points=mesh.points, mesh_owner=mesh.owner, mesh_neighbour=mesh.neighbour,
-        mesh_face_ptr=mesh.face_ptr, mesh_face_pts=mesh.face_pts, mesh_cell_ptr=mesh.cell_ptr,
-        mesh_cell_pts=mesh.cell_pts, mesh_patch_start=mesh.patch_start,
-        mesh_patch_nfaces=mesh.patch_nfaces, mesh_patch_names=np.array(mesh.patch_names),
         case_mu=np.array(case.mu), case_rho=np.array(case.rho), case_dt=np.array(case.dt),
-        case_lid=np.array(case.bcs["zmax"].u_value), steps=np.array(steps),
+        case_lid=np.array(case.bcs["zmax"].u_value if "zmax" in case.bcs else (0.0, 0.0, 0.0)),
+        steps=np.array(steps),
         correctors=np.array(correctors), case_g=np.array(case.g),
         case_n_vof=np.array(case.n_vof), case_urf_vf=np.array(case.urf_vf),
         case_adjust_dt=np.array(int(case.adjust_dt)), case_max_co=np.array(case.max_co),
@@ -90,6 +88,14 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, **arrs)
     print(f"{name}: {len(arrs)} arrays, {os.path.getsize(path) / 1024:.0f} KiB")
+
+
+def mesh_arrays(mesh):
+    """The mesh as fixture arrays (read back by conftest.load_golden)."""
+    return dict(mesh_points=mesh.points, mesh_owner=mesh.owner, mesh_neighbour=mesh.neighbour,
+                mesh_face_ptr=mesh.face_ptr, mesh_face_pts=mesh.face_pts, mesh_cell_ptr=mesh.cell_ptr,
+                mesh_cell_pts=mesh.cell_pts, mesh_patch_start=mesh.patch_start,
+                mesh_patch_nfaces=mesh.patch_nfaces, mesh_patch_names=np.array(mesh.patch_names))
 
 
 def random_init(nc, seed):
@@ -210,10 +216,137 @@ def _bc_vof():
     run_case("bc_vof_box654_2x1", m, c, 2, 1, init)
 
 
+def _poly20():
+    """Three stacked prisms over a jittered 20-gon: internal faces and caps
+    of 20 points (above the 16 the device keeps in registers), lid-driven
+    top (fixedValue p = 0, U = (1, 0, 0)), two SIMPLE steps."""
+    m = prism_stack_mesh(20, 3, jitter=0.05, seed=3)
+    c = cavity_case(dt=0.01)
+    c.mu = (0.01, 0.01)
+    c.bcs = {"bottom": PatchBC(), "side": PatchBC(),
+             "top": PatchBC(p_type=BC_FIXED_VALUE, p_value=0.0, u_type=BC_FIXED_VALUE, u_value=(1.0, 0.0, 0.0))}
+    run_case("poly20_stack3", m, c, 2, 1, random_init(m.n_cells, 81))
+
+
+# ------------------------------------------------------------------ VTU files --
+REF_TREE = "/root/reference"
+SEMO_INITIAL = os.path.join(ROOT, "oracle", "_ref", "semo_initial")
+
+
+def _vtu():
+    """SEMO's own writer (SEMO_Mesh_Save::vtu through semo_ref's savevtu
+    mode) on a seeded state of a jittered tet mesh, in its three encodings:
+    ascii (17 digits), binary, zlib level 6.  Stores the written files and
+    the state that went in (point levels i % 4, cell levels i % 3, cell
+    groups i % 5, time 0.25, as the driver sets them)."""
+    m = tet_mesh(2, 2, 1, jitter=0.1, seed=5)
+    nc = m.n_cells
+    init = random_init(nc, 71)
+    init[4 * nc:] = np.random.default_rng(72).random(nc)
+    case = cavity_case()
+    case.bcs = {nm: PatchBC() for nm in m.patch_names}
+    with tempfile.TemporaryDirectory() as d:
+        m.write_raw(os.path.join(d, "mesh.raw"))
+        ip = os.path.join(d, "init.f64")
+        np.ascontiguousarray(init, dtype=np.float64).tofile(ip)
+        case.write_ref_case(os.path.join(d, "case.txt"), init=ip)
+        for tag, fmt, comp in (("ascii", "ascii", 0), ("binary", "binary", 0), ("zlib", "binary", 6)):
+            out = os.path.join(d, "out_" + tag)
+            r = subprocess.run([REF, os.path.join(d, "mesh.raw"), os.path.join(d, "case.txt"), out, "savevtu",
+                                fmt, str(comp)], capture_output=True, text=True, timeout=120, cwd=d)
+            if "SEMO_REF_DONE" not in r.stdout:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            with open(os.path.join(out, "save", "0.25", "plot.0.vtu")) as fh:
+                text = fh.read()
+            with open(os.path.join(HERE, f"vtu_{tag}_tet221.vtu"), "w") as fh:
+                fh.write(text)
+    np.savez_compressed(os.path.join(HERE, "vtu_tet221.npz"), init=init, **mesh_arrays(m))
+    print("vtu_tet221: 3 files")
+
+
+INITIAL_SPECS = {
+    # name: (pressure, velocities, temperature, water) initialField bodies
+    "box_cyl_box_sphere": (
+        ["type boxToCell;", "value 0.5;", "box (0.2 0.1 -1) (0.7 0.8 0.55);", "value 2.5;"],
+        ["type cylinderToCell;", "value (0.1 0 0);", "points (0.1 0.2 0.3) (0.9 0.7 0.6);", "radius 0.3;",
+         "value (1 -2 0.5);"],
+        ["type boxToCell;", "value 300;", "box (0.5 0 0) (1 1 1);", "value 350;"],
+        ["type sphereToCell;", "value 0;", "points (0.4 0.5 0.45);", "radius 0.35;", "value 1;"]),
+    # (velocities sphereToCell is left out: the reference's parser throws on
+    # every input, initial.cpp:1077 erasing from an empty string)
+    "fixed_box_fixed_cyl": (
+        ["type fixedValue;", "value 101325;"],
+        ["type boxToCell;", "value (0 0 0);", "box (0.3 0.3 0.3) (0.8 0.9 0.7);", "value (0 0.75 -1.25);"],
+        ["type fixedValue;", "value 293.15;"],
+        ["type cylinderToCell;", "value 0.25;", "points (0 0.5 0.5) (1 0.5 0.5);", "radius 0.33;", "value 0.9;"]),
+    "box_fixed_box_box": (
+        ["type boxToCell;", "value -1;", "box (0 0 0) (0.5 0.5 0.5);", "value 1;"],
+        ["type fixedValue;", "value (3 2 1);"],
+        ["type boxToCell;", "value 280;", "box (0 0.5 0) (1 1 0.5);", "value 290;"],
+        ["type boxToCell;", "value 1;", "box (0 0 0.6) (1 1 1);", "value 0;"]),
+}
+
+
+def write_initial_file(path, body, patches):
+    """A constant/<field> file in SEMO's layout: the initialField block and a
+    zeroGradient boundaryField entry per patch (the utility matches the
+    patch names against the mesh's)."""
+    bf = "".join(f"\t{nm}\n\t{{\n\t\ttype\tzeroGradient;\n\t}}\n" for nm in patches)
+    with open(path, "w") as fh:
+        fh.write("initialField\n{\n" + "".join(f"\t{line}\n" for line in body) + "}\n\nboundaryField\n{\n"
+                 + bf + "}\n")
+
+
+def _initial():
+    """SEMO's initial-field utility (src/utility/initial.cpp, built as it is
+    into oracle/_ref/semo_initial) on a jittered 6^3 cube for each spec of
+    INITIAL_SPECS, in a case holding the reference's sample system/ and
+    constant/ files (saveFormat set to binary) with the spec's initialField
+    blocks, the mesh as ./grid/0/plot.0.vtu from write_grid_vtu.  Stores the
+    mesh, the specs and the fields the utility saved."""
+    import shutil
+    from semo_mesh_amd.vtu import read_vtu, write_grid_vtu
+    if not os.path.exists(SEMO_INITIAL):
+        raise SystemExit("build the reference first: make -f oracle/ref.mk REF=/root/reference")
+    m = cavity_mesh(6, jitter=0.15, seed=73)
+    nc = m.n_cells
+    out = {}
+    for name, spec in INITIAL_SPECS.items():
+        with tempfile.TemporaryDirectory() as d:
+            shutil.copytree(os.path.join(REF_TREE, "system"), os.path.join(d, "system"))
+            shutil.copytree(os.path.join(REF_TREE, "constant"), os.path.join(d, "constant"))
+            cd = os.path.join(d, "system", "controlDict")
+            with open(cd) as fh:
+                lines = fh.read().splitlines()
+            lines = ["saveFormat = binary" if ln.startswith("saveFormat") else
+                     "saveCompression = 0" if ln.startswith("saveCompression") else ln for ln in lines]
+            with open(cd, "w") as fh:
+                fh.write("\n".join(lines) + "\n")
+            for fname, body in zip(("pressure", "velocities", "temperature", "water", "air"),
+                                   spec + (["type fixedValue;", "value 0;"],)):
+                write_initial_file(os.path.join(d, "constant", fname), body, m.patch_names)
+            os.makedirs(os.path.join(d, "grid", "0"))
+            write_grid_vtu(os.path.join(d, "grid", "0", "plot.0.vtu"), m)
+            r = subprocess.run([SEMO_INITIAL], capture_output=True, text=True, timeout=120, cwd=d)
+            res = os.path.join(d, "save", "0", "plot.0.vtu")
+            if not os.path.exists(res):
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            _, f, _ = read_vtu(res)
+        out[name + "_P"] = f["pressure"]
+        out[name + "_UVW"] = f["velocity"]
+        out[name + "_T"] = f["temperature"]
+        out[name + "_VF"] = f["volume-fraction-water"]
+        out[name + "_MF"] = f["mass-fraction-water"]
+        out[name + "_spec"] = np.array(["\n".join(b) for b in spec])
+    np.savez_compressed(os.path.join(HERE, "initial_cube6.npz"), **out, **mesh_arrays(m))
+    print("initial_cube6:", len(INITIAL_SPECS), "specs")
+
+
 CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "cavity8_plain": _cavity8, "orifice_lid": _orifice, "vof_box654_2x2": _vof_box654,
          "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box,
-         "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof}
+         "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof,
+         "poly20_stack3": _poly20, "vtu_tet221": _vtu, "initial_cube6": _initial}
 
 
 def main(names):

@@ -1,18 +1,33 @@
-"""SEMO's plot.<rank>.vtu, ASCII (semo_mesh_amd.vtu), CPU only.
+"""SEMO's plot.<rank>.vtu in its three encodings (semo_mesh_amd.vtu), CPU
+only.
 
-The reference holds no .vtu case, so the reader is pinned by round trips of
-the writer and by the loader's rules (src/load/load.cpp:352-413): every mesh
-array and field comes back identical, the oracle's geometry and one SIMPLE
-iteration on the read-back mesh are bit-identical to the original's, and a
-face's points come from the first cell whose face stream lists it.
+Pinned by SEMO's own writer and loader, compiled from the reference
+(oracle/_ref/semo_ref's savevtu / loadvtu modes):
+
+* tests/golden/vtu_{ascii,binary,zlib}_tet221.vtu are SEMO_Mesh_Save::vtu's
+  output for a seeded state (ascii at 17 digits, binary, zlib level 6): the
+  reader gives back the mesh and every field, level and group exactly, and
+  the writer's binary and zlib encodings of the same arrays are the
+  reference's text, character for character;
+* SEMO_Mesh_Load::vtu reads the writer's files (every encoding) back to the
+  same mesh and fields (run where the reference is built);
+* round trips of the writer, the oracle bit-identical on a read-back mesh,
+  and the loader's rule that a face's points come from the first cell whose
+  face stream lists it (src/load/load.cpp:352-413).
 """
 import os
+import subprocess
 
 import numpy as np
+import pytest
 
+from conftest import GOLDEN, ROOT, load_golden
 from semo_mesh_amd.cases import cavity_case
 from semo_mesh_amd.mesh import hex_mesh, tet_mesh
-from semo_mesh_amd.vtu import read_vtu, write_vtu
+from semo_mesh_amd.vtu import read_vtu, read_vtu_arrays, semo_fields, write_grid_vtu, write_vtu
+
+SEMO_REF = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
+ENCODINGS = {"ascii": ("ascii", 0), "binary": ("binary", 0), "zlib": ("binary", 6)}
 
 ARRAYS = ("points", "owner", "neighbour", "face_ptr", "face_pts", "cell_ptr", "cell_pts",
           "patch_start", "patch_nfaces")
@@ -68,3 +83,146 @@ def test_face_points_from_the_first_cell(tmp_path):
     open(p, "w").write(head + 'Name="faces" format="ascii">\n' + stream + "\n" + rest)
     r, _, _ = read_vtu(p)
     assert r.face_pts[r.face_ptr[0]:r.face_ptr[1]].tolist() == f0
+
+
+def _tet221_state():
+    d, m = load_golden("vtu_tet221")
+    nc = m.n_cells
+    init = d["init"]
+    f = semo_fields(init[:nc], init[nc:2 * nc], init[2 * nc:3 * nc], init[3 * nc:4 * nc], init[4 * nc:])
+    lv = {"pointLevels": np.arange(m.n_points) % 4, "cellLevels": np.arange(nc) % 3,
+          "cellGroups": np.arange(nc) % 5}
+    return m, f, lv
+
+
+@pytest.mark.parametrize("enc", list(ENCODINGS))
+def test_reads_semo_written_file(enc):
+    m, f, lv = _tet221_state()
+    r, g, t, rl = read_vtu(os.path.join(GOLDEN, f"vtu_{enc}_tet221.vtu"), with_levels=True)
+    for a in ARRAYS:
+        assert np.array_equal(getattr(r, a), getattr(m, a)), a
+    assert r.patch_names == m.patch_names
+    assert t == 0.25
+    assert list(g) == list(f)
+    for k in f:
+        assert np.array_equal(g[k], f[k]), k
+    for k in lv:
+        assert np.array_equal(rl[k], lv[k]), k
+
+
+@pytest.mark.parametrize("enc", ["binary", "zlib"])
+def test_writer_encodes_as_semo(tmp_path, enc):
+    """Every binary / zlib DataArray of the writer's file is the text of the
+    reference's file for the same arrays."""
+    m, f, lv = _tet221_state()
+    p = os.path.join(tmp_path, "plot.0.vtu")
+    fmt, comp = ENCODINGS[enc]
+    write_vtu(p, m, f, time=0.25, fmt=fmt, compression=comp, point_levels=lv["pointLevels"],
+              cell_levels=lv["cellLevels"], cell_groups=lv["cellGroups"])
+
+    def texts(path):
+        import xml.etree.ElementTree as ET
+        return {da.get("Name"): "".join((da.text or "").split())
+                for da in ET.parse(path).getroot().iter("DataArray")}
+    ours, ref = texts(p), texts(os.path.join(GOLDEN, f"vtu_{enc}_tet221.vtu"))
+    assert set(ours) == set(ref)
+    for name in ref:
+        assert ours[name] == ref[name], name
+
+
+@pytest.mark.parametrize("enc", list(ENCODINGS))
+def test_round_trip_encodings(tmp_path, enc):
+    m = tet_mesh(3, 2, 2, jitter=0.1, seed=9)
+    rng = np.random.default_rng(4)
+    nc = m.n_cells
+    f = semo_fields(*(rng.standard_normal(nc) for _ in range(4)), VF=rng.random(nc))
+    p = os.path.join(tmp_path, "plot.0.vtu")
+    fmt, comp = ENCODINGS[enc]
+    write_vtu(p, m, f, time=1.5e-3, fmt=fmt, compression=comp, cell_groups=np.arange(nc))
+    r, g, t, lv = read_vtu(p, with_levels=True)
+    for a in ARRAYS:
+        assert np.array_equal(getattr(r, a), getattr(m, a)), a
+    assert t == 1.5e-3 and np.array_equal(lv["cellGroups"], np.arange(nc))
+    for k in f:
+        assert np.array_equal(g[k], f[k]), k
+
+
+def test_vtk_byte_count_header_and_blocks():
+    """Files from other VTK writers: a binary header holding the byte count,
+    and zlib data in several blocks, decode to the same values."""
+    import base64
+    import struct
+    import zlib
+    from semo_mesh_amd.vtu import _decode_binary, _decode_compressed
+    a = np.arange(10, dtype="<i4")
+    tok = base64.b64encode(struct.pack("<Q", a.nbytes) + a.tobytes()).decode()
+    assert np.array_equal(_decode_binary(tok, "<i4", "a"), a)
+    raw = np.linspace(0, 1, 1000).tobytes()
+    blocks = [raw[i:i + 3000] for i in range(0, len(raw), 3000)]
+    zs = [zlib.compress(b) for b in blocks]
+    hdr = struct.pack(f"<{3 + len(zs)}Q", len(zs), 3000, len(blocks[-1]), *[len(z) for z in zs])
+    tok = base64.b64encode(hdr).decode() + base64.b64encode(b"".join(zs)).decode()
+    assert np.array_equal(_decode_compressed(tok, "<f8", "b"), np.linspace(0, 1, 1000))
+    with pytest.raises(ValueError):
+        _decode_binary(base64.b64encode(struct.pack("<Q", 7) + a.tobytes()).decode(), "<i4", "a")
+
+
+@pytest.mark.parametrize("enc", list(ENCODINGS))
+def test_semo_loader_reads_our_file(tmp_path, enc):
+    if not os.path.exists(SEMO_REF):
+        pytest.skip("the compiled reference (oracle/_ref) is built only where the reference checkout is")
+    m = hex_mesh(3, 2, 2, jitter=0.1, seed=21)
+    nc = m.n_cells
+    rng = np.random.default_rng(8)
+    P, U, V, W = (rng.standard_normal(nc) for _ in range(4))
+    VF = rng.random(nc)
+    f = semo_fields(P, U, V, W, VF=VF, T=np.full(nc, 310.0))
+    folder = os.path.join(tmp_path, "save", "0.5")
+    os.makedirs(folder)
+    fmt, comp = ENCODINGS[enc]
+    write_vtu(os.path.join(folder, "plot.0.vtu"), m, f, time=0.5, fmt=fmt, compression=comp,
+              point_levels=np.arange(m.n_points) % 2, cell_levels=np.arange(nc) % 3)
+    case = cavity_case()
+    cp = os.path.join(tmp_path, "case.txt")
+    case.write_ref_case(cp)
+    # the loader also reads the patches' conditions from ./constant/<field>
+    os.makedirs(os.path.join(tmp_path, "constant"))
+    bf = "".join(f"\t{nm}\n\t{{\n\t\ttype\tzeroGradient;\n\t}}\n" for nm in m.patch_names)
+    for fname in ("pressure", "velocities", "temperature", "water", "air"):
+        with open(os.path.join(tmp_path, "constant", fname), "w") as fh:
+            fh.write("initialField\n{\n\ttype fixedValue;\n\tvalue 0;\n}\n\nboundaryField\n{\n" + bf + "}\n")
+    out = os.path.join(tmp_path, "out")
+    r = subprocess.run([SEMO_REF, cp, cp, out, "loadvtu", folder + "/"], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert "SEMO_REF_DONE" in r.stdout, r.stdout[-1500:] + r.stderr[-1500:]
+    rd = lambda n, t: np.fromfile(os.path.join(out, n + (".f64" if t == "f" else ".i32")),  # noqa: E731
+                                  dtype=np.float64 if t == "f" else np.int32)
+    assert np.array_equal(rd("vtu_points", "f").reshape(-1, 3), m.points)
+    nbr = np.full(m.n_faces, -1)
+    nbr[:m.n_internal_faces] = m.neighbour
+    assert np.array_equal(rd("vtu_owner", "i"), m.owner) and np.array_equal(rd("vtu_neighbour", "i"), nbr)
+    assert np.array_equal(rd("vtu_face_ptr", "i"), m.face_ptr)
+    assert np.array_equal(rd("vtu_face_pts", "i"), m.face_pts)
+    assert np.array_equal(rd("vtu_cell_ptr", "i"), m.cell_ptr)
+    assert np.array_equal(rd("vtu_cell_pts", "i"), m.cell_pts)
+    assert np.array_equal(rd("vtu_point_levels", "i"), np.arange(m.n_points) % 2)
+    assert np.array_equal(rd("vtu_cell_levels", "i"), np.arange(nc) % 3)
+    for nm, v in (("P", P), ("U", U), ("V", V), ("W", W), ("T", np.full(nc, 310.0)), ("VF", VF), ("MF", VF)):
+        assert np.array_equal(rd("vtu_" + nm, "f"), v), nm
+    assert rd("vtu_time", "f")[0] == 0.5
+    names = bytes(rd("vtu_bc_names", "i").astype(np.uint8)).decode().split("\0")[:-1]
+    assert names == m.patch_names
+    assert np.array_equal(rd("vtu_bc_start", "i"), m.patch_start)
+
+
+def test_grid_layout_round_trip(tmp_path):
+    """SEMO's mesh-only grid file (./grid/0/, saveOnlyMesh.cpp): the reader
+    takes its owner / neighbour / patch elements."""
+    m = tet_mesh(2, 2, 2, jitter=0.1, seed=2)
+    p = os.path.join(tmp_path, "plot.0.vtu")
+    write_grid_vtu(p, m)
+    r, g, t = read_vtu(p)
+    for a in ARRAYS:
+        assert np.array_equal(getattr(r, a), getattr(m, a)), a
+    assert r.patch_names == m.patch_names and g == {} and t == 0.0
+    assert "owner" in read_vtu_arrays(p)
