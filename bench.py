@@ -70,6 +70,26 @@ def run_path(mesh, case, precision, steps, warmup, profile_kernels=True, cheb_de
                 seconds_kernel_pass=t_prof)
 
 
+# stages of a SIMPLE iteration by kernel tag (every launch carries one)
+STAGES = (
+    ("pressure V-cycle", lambda t: t.startswith("amg_") and t not in ("amg_values_changed", "amg_refresh_L0")),
+    ("pressure Krylov (FP64)", lambda t: t.startswith(("fcg_", "spmv_fp64", "spmv_xpby", "true_resid", "krylov_"))),
+    ("momentum BiCGStab (FP64)", lambda t: t.startswith(("bicg3_", "spmv3_", "jacobi_dinv"))),
+    ("assembly, gradients, corrections", lambda t: True),
+)
+
+
+def stage_table(prof, steps):
+    """ms per step of each stage from the kernel pass's tags."""
+    out = {name: 0.0 for name, _ in STAGES}
+    for tag, v in prof.items():
+        for name, test in STAGES:
+            if test(tag):
+                out[name] += v["ms"]
+                break
+    return {k: round(v / steps, 3) for k, v in out.items()}
+
+
 def dam_break_leg(steps, warmup, nx=96):
     """Two-phase VOF dam break on the tetrahedral mesh of the 1.3M-cell
     configuration: 4 x 2 x 2 m box, nx x nx/2 x nx/2 jittered cubes split into
@@ -359,19 +379,17 @@ def main():
     ap.add_argument("--no-coupled", action="store_true", help="skip the block-coupled (BSR4) leg")
     ap.add_argument("--cheb-degree", type=int, default=1, help="Chebyshev smoother degree of the pressure AMG")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (read MIB, write MIB): the "
-                         "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
+                    help="only run the library's streaming copy of MIB MiB (read MIB, write MIB; "
+                         "1 + 3 launches): the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-        n = args.copy_probe * (1 << 20) // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            torch.mul(a, 1.0, out=b)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 3}))
+        from semo_mesh_amd.solver import GpuSolver
+        n = args.copy_probe * (1 << 20) // 8
+        g = GpuSolver(0)
+        ms = g.stream_copy(n, 3)
+        g.close()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 4,
+                          "ms_per_copy": ms}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -424,6 +442,10 @@ def main():
             a, b = resmx["fields"][f], res64["fields"][f]
             err[f] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
         out["mixed_vs_fp64_rel_l2"] = err
+        if res64["prof"] and resmx["prof"]:
+            out["stages_ms_per_step"] = {"fp64": stage_table(res64["prof"], args.steps),
+                                         "mixed": stage_table(resmx["prof"], args.steps),
+                                         "timing": "kernel pass (HIP event pair per launch)"}
     # dominant kernel by device time inside the timed mixed run
     prof = resmx["prof"]
     if prof:
@@ -456,6 +478,15 @@ def main():
             "frac_over_kernel_time": round(step_bytes / (tagged_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
             "note": "kernel_ms: sum of the per-launch HIP event times of the kernel pass (launches issued "
                     "one by one); ms_per_step - kernel_ms is host time and launch gaps"}
+        # the practical streaming ceiling: the library's own copy kernel on
+        # 1 GiB (1 GiB read + 1 GiB written), average of 10 copies
+        from semo_mesh_amd.solver import GpuSolver
+        gc = GpuSolver(0)
+        ms_copy = gc.stream_copy(1 << 27, 10)
+        gc.close()
+        copy_gbs = 2.0 * (1 << 30) / (ms_copy * 1e-3) / 1e9
+        out["roofline"]["copy_ceiling_GBs"] = round(copy_gbs, 1)
+        out["roofline"]["frac_of_copy"] = round(achieved / copy_gbs, 4)
         traffic = _measured_traffic(name)
         if traffic:
             out["roofline"]["traffic"] = traffic["bytes"]

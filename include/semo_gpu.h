@@ -334,6 +334,12 @@ int semo_solve_coo(semo_ctx* ctx, const char* equation, int32_t n, int64_t nnz,
 int semo_profile_enable(semo_ctx* ctx, int on);
 int semo_profile_reset(semo_ctx* ctx);
 /* i-th profiled kernel: name (<= 63 chars), launches, total ms, total bytes     */
+/* Average ms of one streaming copy of n doubles (read n, write n) by the
+ * library's own copy kernel, over reps copies on the context's stream: the
+ * practical HBM ceiling the kernels are compared with.  No counterpart in the
+ * reference (a measurement aid).                                              */
+int semo_stream_copy(semo_ctx* ctx, int64_t n, int reps, double* ms);
+
 int semo_profile_entry(semo_ctx* ctx, int i, char* name, int64_t* count, double* ms,
                        double* bytes);
 

@@ -75,7 +75,7 @@ EXPORTED = [
     "semo_volume_integral",
     "semo_calc_coupled_eq", "semo_coupled_pressure_based", "semo_coupled_matrix_get", "semo_coupled_spmv", "semo_coupled_solve",
     "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_least_square_noalloc", "semo_cell_min_max", "semo_gauss_green", "semo_solve_coo",
-    "semo_profile_enable", "semo_profile_reset", "semo_profile_entry",
+    "semo_profile_enable", "semo_profile_reset", "semo_profile_entry", "semo_stream_copy",
 ]
 
 
@@ -130,6 +130,7 @@ def lib():
                                      C.POINTER(SolveStats)]),
         "semo_profile_enable": (C.c_int, [vp, C.c_int]),
         "semo_profile_reset": (C.c_int, [vp]),
+        "semo_stream_copy": (C.c_int, [vp, i64, C.c_int, C.POINTER(C.c_double)]),
         "semo_profile_entry": (C.c_int, [vp, C.c_int, C.c_char_p, C.POINTER(i64),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     }

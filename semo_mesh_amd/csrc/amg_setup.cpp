@@ -17,6 +17,8 @@
 #include <cmath>
 #include <numeric>
 
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace semo {
@@ -172,10 +174,19 @@ void split_diag(const HCsr& A, std::vector<int>& ptr, std::vector<int>& col, std
 // (Average 10 and four entries per lane measured best against averages up
 // to 64 for ELL and two or eight entries per lane.)
 constexpr double ELL_MAX_AVG = 10.0, ENTRIES_PER_LANE = 4.0;
+// SEMO_GPU_ELL_MAX_AVG overrides the average row length up to which sliced
+// ELL is chosen (a tuning knob; the default is the measured best)
+double ell_max_avg() {
+    static const double v = [] {
+        const char* e = std::getenv("SEMO_GPU_ELL_MAX_AVG");
+        return e ? std::atof(e) : ELL_MAX_AVG;
+    }();
+    return v;
+}
 int choose_lanes(int n, const std::vector<int>& ptr, size_t padded) {
     size_t nnz = ptr.empty() ? 0 : (size_t)ptr[n];
     double avg = n ? (double)nnz / n : 0.0;
-    if (avg <= ELL_MAX_AVG && (double)padded <= 1.5 * (double)std::max<size_t>(nnz, 1)) return 0;
+    if (avg <= ell_max_avg() && (double)padded <= 1.5 * (double)std::max<size_t>(nnz, 1)) return 0;
     int lanes = 4;
     while (lanes < 64 && ENTRIES_PER_LANE * lanes < avg) lanes *= 2;
     return lanes;

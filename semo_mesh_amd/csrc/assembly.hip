@@ -1167,11 +1167,16 @@ __global__ void __launch_bounds__(TPB) k_st_source(
 }
 
 // ------------------------------------------- p' gradient and correction --
-// eqPressure.cpp:686-809: two Green-Gauss passes of p' with skew correction
+// eqPressure.cpp:686-809: two Green-Gauss passes of p' with skew correction.
+// The face pass writes each internal face's flux vector (v n_a) A, [nif][3],
+// so that the cell pass reads 24 B per face side instead of the normal, the
+// area and the value (40 B); the cell pass divides by its volume, the same
+// operations in the same order as ((v n_a) A) / V.
 __global__ void __launch_bounds__(TPB) k_gg_face(int nif, const int* __restrict__ owner,
                                                  const int* __restrict__ nbr, const double* __restrict__ wCv,
-                                                 const double* __restrict__ skew, const double* __restrict__ pp,
-                                                 const double* __restrict__ gp, double* __restrict__ vF) {
+                                                 const double* __restrict__ skew, const double* __restrict__ nrm,
+                                                 const double* __restrict__ area, const double* __restrict__ pp,
+                                                 const double* __restrict__ gp, double* __restrict__ fluxF) {
     int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nif) return;
     int l = owner[f], r = nbr[f];
@@ -1179,26 +1184,28 @@ __global__ void __launch_bounds__(TPB) k_gg_face(int nif, const int* __restrict_
     double v = wCL * pp[l] + wCR * pp[r];
     const double* sk = skew + 3 * (size_t)f;
     for (int a = 0; a < 3; ++a) { v += wCL * gp[3 * (size_t)l + a] * sk[a]; v += wCR * gp[3 * (size_t)r + a] * sk[a]; }
-    vF[f] = v;
+    const double* n = nrm + 3 * (size_t)f;
+    double A = area[f];
+    for (int a = 0; a < 3; ++a) fluxF[3 * (size_t)f + a] = v * n[a] * A;
 }
 
 __global__ void __launch_bounds__(TPB) k_gg_cell(
     int nc, int nif, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
     const int* __restrict__ owner, const int* __restrict__ bpatch, const semo_patch_bc* __restrict__ bcs,
     const double* __restrict__ nrm, const double* __restrict__ area, const double* __restrict__ vol,
-    const double* __restrict__ pp, const double* __restrict__ vF, double* __restrict__ gout) {
+    const double* __restrict__ pp, const double* __restrict__ fluxF, double* __restrict__ gout) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double g[3] = {0, 0, 0};
     double vi = vol[i];
     CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
-        const double* n = nrm + 3 * (size_t)f;
-        double A = area[f];
         if (f < nif) {
-            double v = vF[f];
-            if (own) { for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi; }
-            else { for (int a = 0; a < 3; ++a) g[a] -= v * n[a] * A / vi; }
+            const double* w = fluxF + 3 * (size_t)f;
+            if (own) { for (int a = 0; a < 3; ++a) g[a] += w[a] / vi; }
+            else { for (int a = 0; a < 3; ++a) g[a] -= w[a] / vi; }
         } else {
+            const double* n = nrm + 3 * (size_t)f;
+            double A = area[f];
             double v = pp[i];
             if (bcs[bpatch[f - nif]].p_type == SEMO_BC_FIXED_VALUE) v = 0.0;
             for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi;
@@ -1681,9 +1688,13 @@ double pressure_correct(semo_ctx* c) {
     zero_d(c, c->gpp.p, c->gpp.bytes());
     for (int pass = 0; pass < 2; ++pass) {
         if (c->nif)
-            SEMO_LAUNCH(c, "gg_face", (8.0 + 8.0 + 24.0 + 8.0) * c->nif + 32.0 * c->nc, k_gg_face, dim3(nblk(c->nif)),
-                        dim3(TPB), c->nif, c->d_owner.p, c->d_nbr.p, c->f_wC.p, c->f_skew.p, pp, c->gpp.p, c->fscr.p);
-        SEMO_LAUNCH(c, "gg_cell", cell_faces_bytes(c, 24.0 + 8.0) + 8.0 * c->nif + 8.0 * c->nbf + 40.0 * c->nc,
+            SEMO_LAUNCH(c, "gg_face", (8.0 + 8.0 + 24.0 + 24.0 + 8.0 + 24.0) * c->nif + 32.0 * c->nc, k_gg_face,
+                        dim3(nblk(c->nif)), dim3(TPB), c->nif, c->d_owner.p, c->d_nbr.p, c->f_wC.p, c->f_skew.p,
+                        c->f_nrm.p, c->f_area.p, pp, c->gpp.p, c->fscr.p);
+        // per cell: its face list, the internal faces' flux vectors, the
+        // boundary faces' normal and area
+        SEMO_LAUNCH(c, "gg_cell", 4.0 * (double)c->h_cf_idx.size() + 24.0 * c->nif + 32.0 * c->nbf +
+                        8.0 * c->nbf + 40.0 * c->nc,
                     k_gg_cell, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p,
                     c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->c_vol.p, pp, c->fscr.p, c->gtmp.p);
         std::swap(c->gpp, c->gtmp);

@@ -804,6 +804,28 @@ int semo_profile_reset(semo_ctx* c) {
     return guard(c, [&] { c->prof.flush(); c->prof.stats.clear(); });
 }
 
+int semo_stream_copy(semo_ctx* c, int64_t n, int reps, double* ms) {
+    if (!c || n <= 0 || reps <= 0 || !ms) return SEMO_EINVAL;
+    return guard(c, [&] {
+        DBuf<double> a, b;
+        a.alloc((size_t)n); b.alloc((size_t)n);
+        zero_d(c, a.p, a.bytes());
+        copy_d(c, b.p, a.p, (size_t)n);                 // first touch
+        hipEvent_t e0, e1;
+        SEMO_HIP(hipEventCreate(&e0));
+        SEMO_HIP(hipEventCreate(&e1));
+        SEMO_HIP(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r) copy_d(c, (r & 1) ? a.p : b.p, (r & 1) ? b.p : a.p, (size_t)n);
+        SEMO_HIP(hipEventRecord(e1, c->stream));
+        SEMO_HIP(hipEventSynchronize(e1));
+        float t = 0.0f;
+        SEMO_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = (double)t / reps;
+    });
+}
+
 int semo_profile_entry(semo_ctx* c, int i, char* name, int64_t* count, double* ms, double* bytes) {
     if (!c) return SEMO_EINVAL;
     return guard(c, [&] {

@@ -471,7 +471,7 @@ void coupled_solve(semo_ctx* c, const double* b, double* x, semo_solve_stats* st
     double* sc = C.sc.p;
     int nbv = nblk((long)N);
     auto fin = [&](int nb, int what) {
-        SEMO_LAUNCH(c, "krylov_finish", 16.0 * nb + 8.0, k_cfinish, dim3(1), dim3(TPB), nb, C.red.p, sc, what);
+        SEMO_LAUNCH(c, "cpl_finish", 16.0 * nb + 8.0, k_cfinish, dim3(1), dim3(TPB), nb, C.red.p, sc, what);
     };
     auto dot = [&](const double* a_, const double* b_, const double* c_, const double* d_, int what) {
         int nb = nb_of(N);

@@ -1079,7 +1079,7 @@ void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const 
     double* part = c->red.p;
     const int16_t* dc = A.dcol.n ? A.dcol.p : nullptr;
     auto fin = [&](int blocks, int what) {
-        SEMO_LAUNCH(c, "krylov_finish", 48.0 * blocks + 8.0, k_finish6, dim3(1), dim3(1024), blocks, part, s, what);
+        SEMO_LAUNCH(c, "bicg3_finish", 48.0 * blocks + 8.0, k_finish6, dim3(1), dim3(1024), blocks, part, s, what);
     };
     auto spmv3 = [&](int mode, const double* xin, double* y, const double* a) {
         // matrix once, three gathered operands, three results, three dot operands

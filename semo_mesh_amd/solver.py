@@ -313,6 +313,13 @@ class GpuSolver:
         return x if nrhs > 1 else x[0]
 
     # ---- profiling ---------------------------------------------------------------------------
+    def stream_copy(self, n: int, reps: int = 10) -> float:
+        """ms of one copy of n doubles by the library's copy kernel (average
+        of reps)."""
+        ms = C.c_double()
+        self._ok(self._L.semo_stream_copy(self._h, int(n), int(reps), C.byref(ms)))
+        return ms.value
+
     def profile(self, on=True):
         self._ok(self._L.semo_profile_enable(self._h, 1 if on else 0))
 
