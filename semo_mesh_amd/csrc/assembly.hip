@@ -453,6 +453,113 @@ __global__ void __launch_bounds__(TPB) k_lsq1(
     g[3 * (size_t)i] = r0; g[3 * (size_t)i + 1] = r1; g[3 * (size_t)i + 2] = r2;
 }
 
+// The same gradient with G lanes per cell (G = 2 or 4): lane q of a cell's
+// group gathers and weighs its stencil entries q, q + G, ..., so one load
+// instruction brings G neighbours of each of 64 / G cells (neighbouring
+// entries of a sorted stencil often share a cache line); the G terms of a
+// round are then broadcast inside the group (DPP quad permutes, no LDS) and
+// added on every lane in stencil order -- the sums, and so the gradient, are
+// bit for bit those of k_lsq1.  cnt: the stencil's CSR row pointers (entry
+// counts).
+template <int CTRL>
+__device__ inline double dpp_bcast(double v) {
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// quad_perm control that gives every lane of a G-lane group the value of the
+// group's lane SRC (G = 4: the quad; G = 2: each half of the quad)
+template <int G, int SRC>
+constexpr int group_bcast_ctrl() {
+    return G == 4 ? SRC * 0x55 : (SRC | SRC << 2 | (2 + SRC) << 4 | (2 + SRC) << 6);
+}
+template <class IDX, int G, int UNR>
+__global__ void __launch_bounds__(TPB) k_lsq1g(
+    int nc, const int* __restrict__ st_sl, const IDX* __restrict__ st_ell, const int* __restrict__ st_ptr,
+    const int* __restrict__ bf_ptr, const int* __restrict__ bf_idx, const double* __restrict__ xph,
+    const double* __restrict__ fxyz, const double* __restrict__ lsq, const double* __restrict__ faceR, int foff,
+    double* __restrict__ g, const int* __restrict__ gate) {
+    static_assert(G == 2 || G == 4, "lanes per cell");
+    if (gate && *gate == 0) return;
+    int gid = xcd_block() * blockDim.x + threadIdx.x;
+    int i = gid / G, q = gid & (G - 1);
+    if (i >= nc) return;      // a group's lanes leave together
+    const double2* me = reinterpret_cast<const double2*>(xph + 4 * (size_t)i);
+    double2 m0 = me[0], m1 = me[1];
+    double xi = m0.x, yi = m0.y, zi = m1.x, p = m1.y;
+    double t0 = 0, t1 = 0, t2 = 0;
+    int s = i >> 6, lane = i & 63;
+    int e0 = st_sl[s];
+    int cnt = st_ptr[i + 1] - st_ptr[i];
+    for (int kb = 0; kb < cnt; kb += G * UNR) {
+        int j[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            int k = kb + u * G + q;
+            if (k < cnt) {
+                int v = __builtin_nontemporal_load(st_ell + e0 + 64 * k + lane);
+                j[u] = sizeof(IDX) == 2 ? i + v : v;
+            } else {
+                j[u] = -1;
+            }
+        }
+        double2 r0[UNR], r1[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+            if (j[u] >= 0) {
+                const double2* o = reinterpret_cast<const double2*>(xph + 4 * (size_t)j[u]);
+                r0[u] = o[0]; r1[u] = o[1];
+            }
+        double a0[UNR], a1[UNR], a2[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            a0[u] = a1[u] = a2[u] = 0.0;
+            if (j[u] >= 0) {
+                double dx = r0[u].x - xi, dy = r0[u].y - yi, dz = r1[u].x - zi;
+                double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+                double d = r1[u].y - p;
+                a0[u] = wk * dx * d; a1[u] = wk * dy * d; a2[u] = wk * dz * d;
+            }
+        }
+        // stencil order: entry kb + u G + src is lane src's u-th term
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            int k = kb + u * G;
+            double b0 = dpp_bcast<group_bcast_ctrl<G, 0>()>(a0[u]);
+            double b1 = dpp_bcast<group_bcast_ctrl<G, 0>()>(a1[u]);
+            double b2 = dpp_bcast<group_bcast_ctrl<G, 0>()>(a2[u]);
+            if (k < cnt) { t0 += b0; t1 += b1; t2 += b2; }
+            b0 = dpp_bcast<group_bcast_ctrl<G, 1>()>(a0[u]);
+            b1 = dpp_bcast<group_bcast_ctrl<G, 1>()>(a1[u]);
+            b2 = dpp_bcast<group_bcast_ctrl<G, 1>()>(a2[u]);
+            if (k + 1 < cnt) { t0 += b0; t1 += b1; t2 += b2; }
+            if constexpr (G == 4) {
+                b0 = dpp_bcast<group_bcast_ctrl<G, 2>()>(a0[u]);
+                b1 = dpp_bcast<group_bcast_ctrl<G, 2>()>(a1[u]);
+                b2 = dpp_bcast<group_bcast_ctrl<G, 2>()>(a2[u]);
+                if (k + 2 < cnt) { t0 += b0; t1 += b1; t2 += b2; }
+                b0 = dpp_bcast<group_bcast_ctrl<G, 3>()>(a0[u]);
+                b1 = dpp_bcast<group_bcast_ctrl<G, 3>()>(a1[u]);
+                b2 = dpp_bcast<group_bcast_ctrl<G, 3>()>(a2[u]);
+                if (k + 3 < cnt) { t0 += b0; t1 += b1; t2 += b2; }
+            }
+        }
+    }
+    if (q) return;
+    for (int qq = bf_ptr[i]; qq < bf_ptr[i + 1]; ++qq) {
+        int f = bf_idx[qq];
+        double dx = fxyz[3 * (size_t)f] - xi, dy = fxyz[3 * (size_t)f + 1] - yi, dz = fxyz[3 * (size_t)f + 2] - zi;
+        double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
+        double d = faceR ? faceR[f - foff] - p : 0.0;
+        t0 += wk * dx * d; t1 += wk * dy * d; t2 += wk * dz * d;
+    }
+    const double* C = lsq + 6 * (size_t)i;
+    double r0 = 0.0; r0 += C[0] * t0; r0 += C[1] * t1; r0 += C[2] * t2;
+    double r1 = 0.0; r1 += C[1] * t0; r1 += C[3] * t1; r1 += C[4] * t2;
+    double r2 = 0.0; r2 += C[2] * t0; r2 += C[4] * t1; r2 += C[5] * t2;
+    g[3 * (size_t)i] = r0; g[3 * (size_t)i + 1] = r1; g[3 * (size_t)i + 2] = r2;
+}
+
 __global__ void __launch_bounds__(TPB) k_minmax1(int nc, const int* __restrict__ st_ptr,
                                                  const int* __restrict__ st_idx, const double* __restrict__ phi,
                                                  double* __restrict__ mx, double* __restrict__ mn) {
@@ -1167,16 +1274,11 @@ __global__ void __launch_bounds__(TPB) k_st_source(
 }
 
 // ------------------------------------------- p' gradient and correction --
-// eqPressure.cpp:686-809: two Green-Gauss passes of p' with skew correction.
-// The face pass writes each internal face's flux vector (v n_a) A, [nif][3],
-// so that the cell pass reads 24 B per face side instead of the normal, the
-// area and the value (40 B); the cell pass divides by its volume, the same
-// operations in the same order as ((v n_a) A) / V.
+// eqPressure.cpp:686-809: two Green-Gauss passes of p' with skew correction
 __global__ void __launch_bounds__(TPB) k_gg_face(int nif, const int* __restrict__ owner,
                                                  const int* __restrict__ nbr, const double* __restrict__ wCv,
-                                                 const double* __restrict__ skew, const double* __restrict__ nrm,
-                                                 const double* __restrict__ area, const double* __restrict__ pp,
-                                                 const double* __restrict__ gp, double* __restrict__ fluxF) {
+                                                 const double* __restrict__ skew, const double* __restrict__ pp,
+                                                 const double* __restrict__ gp, double* __restrict__ vF) {
     int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nif) return;
     int l = owner[f], r = nbr[f];
@@ -1184,28 +1286,26 @@ __global__ void __launch_bounds__(TPB) k_gg_face(int nif, const int* __restrict_
     double v = wCL * pp[l] + wCR * pp[r];
     const double* sk = skew + 3 * (size_t)f;
     for (int a = 0; a < 3; ++a) { v += wCL * gp[3 * (size_t)l + a] * sk[a]; v += wCR * gp[3 * (size_t)r + a] * sk[a]; }
-    const double* n = nrm + 3 * (size_t)f;
-    double A = area[f];
-    for (int a = 0; a < 3; ++a) fluxF[3 * (size_t)f + a] = v * n[a] * A;
+    vF[f] = v;
 }
 
 __global__ void __launch_bounds__(TPB) k_gg_cell(
     int nc, int nif, const int* __restrict__ cf_ptr, const int* __restrict__ cf_idx,
     const int* __restrict__ owner, const int* __restrict__ bpatch, const semo_patch_bc* __restrict__ bcs,
     const double* __restrict__ nrm, const double* __restrict__ area, const double* __restrict__ vol,
-    const double* __restrict__ pp, const double* __restrict__ fluxF, double* __restrict__ gout) {
+    const double* __restrict__ pp, const double* __restrict__ vF, double* __restrict__ gout) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double g[3] = {0, 0, 0};
     double vi = vol[i];
     CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
+        const double* n = nrm + 3 * (size_t)f;
+        double A = area[f];
         if (f < nif) {
-            const double* w = fluxF + 3 * (size_t)f;
-            if (own) { for (int a = 0; a < 3; ++a) g[a] += w[a] / vi; }
-            else { for (int a = 0; a < 3; ++a) g[a] -= w[a] / vi; }
+            double v = vF[f];
+            if (own) { for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi; }
+            else { for (int a = 0; a < 3; ++a) g[a] -= v * n[a] * A / vi; }
         } else {
-            const double* n = nrm + 3 * (size_t)f;
-            double A = area[f];
             double v = pp[i];
             if (bcs[bpatch[f - nif]].p_type == SEMO_BC_FIXED_VALUE) v = 0.0;
             for (int a = 0; a < 3; ++a) g[a] += v * n[a] * A / vi;
@@ -1569,6 +1669,39 @@ void assemble_vof(semo_ctx* c) {
     c->last_eq = 2;
 }
 
+// the stencil pass of the one-field LSQ gradient over packed records xph
+void lsq1_launch(semo_ctx* c, const double* xph, const double* faceR, int foff, double* g, const int* gate) {
+    int n = c->nc;
+    int G = c->lsq_lanes;     // 1: k_lsq1; 2, 4: k_lsq1g
+    dim3 tb(TPB), gb(nblk((long)n * G));
+    const int* sl = c->d_st_sl.p;
+    const int* bp = c->d_bf_ptr.p;
+    const int* bi = c->d_bf_idx.p;
+    if (c->d_st_dl.n) {
+        const int16_t* st = c->d_st_dl.p;
+        if (G == 4)
+            SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), (k_lsq1g<int16_t, 4, 2>), gb, tb, n, sl, st, c->d_st_ptr.p, bp, bi,
+                        xph, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
+        else if (G == 2)
+            SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), (k_lsq1g<int16_t, 2, 4>), gb, tb, n, sl, st, c->d_st_ptr.p, bp, bi,
+                        xph, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
+        else
+            SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int16_t>, gb, tb, n, sl, st, bp, bi, xph, c->f_xyz.p,
+                        c->c_lsq.p, faceR, foff, g, gate);
+    } else {
+        const int* st = c->d_st_ell.p;
+        if (G == 4)
+            SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), (k_lsq1g<int, 4, 2>), gb, tb, n, sl, st, c->d_st_ptr.p, bp, bi, xph,
+                        c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
+        else if (G == 2)
+            SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), (k_lsq1g<int, 2, 4>), gb, tb, n, sl, st, c->d_st_ptr.p, bp, bi, xph,
+                        c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
+        else
+            SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int>, gb, tb, n, sl, st, bp, bi, xph, c->f_xyz.p, c->c_lsq.p,
+                        faceR, foff, g, gate);
+    }
+}
+
 // one field's LSQ gradient (k_lsq1): packs [x, y, z, phi], then the
 // stencil pass; faceR indexed by face - foff, null for a zero boundary
 // difference
@@ -1577,12 +1710,7 @@ void lsq1(semo_ctx* c, const double* phi, const double* faceR, int foff, double*
     c->xph.ensure(4 * (size_t)n);
     dim3 gb(nblk(n)), tb(TPB);
     SEMO_LAUNCH(c, "lsq1_pack", 64.0 * n, k_pack_xphi, gb, tb, n, c->c_xyz.p, phi, c->xph.p, gate);
-    if (c->d_st_dl.n)
-        SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int16_t>, gb, tb, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
-                    c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
-    else
-        SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int>, gb, tb, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
-                    c->d_bf_idx.p, c->xph.p, c->f_xyz.p, c->c_lsq.p, faceR, foff, g, gate);
+    lsq1_launch(c, c->xph.p, faceR, foff, g, gate);
 }
 
 // calcSourceSurfaceTension with calcCurvature (curvature.cpp:7-3537,
@@ -1688,13 +1816,9 @@ double pressure_correct(semo_ctx* c) {
     zero_d(c, c->gpp.p, c->gpp.bytes());
     for (int pass = 0; pass < 2; ++pass) {
         if (c->nif)
-            SEMO_LAUNCH(c, "gg_face", (8.0 + 8.0 + 24.0 + 24.0 + 8.0 + 24.0) * c->nif + 32.0 * c->nc, k_gg_face,
-                        dim3(nblk(c->nif)), dim3(TPB), c->nif, c->d_owner.p, c->d_nbr.p, c->f_wC.p, c->f_skew.p,
-                        c->f_nrm.p, c->f_area.p, pp, c->gpp.p, c->fscr.p);
-        // per cell: its face list, the internal faces' flux vectors, the
-        // boundary faces' normal and area
-        SEMO_LAUNCH(c, "gg_cell", 4.0 * (double)c->h_cf_idx.size() + 24.0 * c->nif + 32.0 * c->nbf +
-                        8.0 * c->nbf + 40.0 * c->nc,
+            SEMO_LAUNCH(c, "gg_face", (8.0 + 8.0 + 24.0 + 8.0) * c->nif + 32.0 * c->nc, k_gg_face, dim3(nblk(c->nif)),
+                        dim3(TPB), c->nif, c->d_owner.p, c->d_nbr.p, c->f_wC.p, c->f_skew.p, pp, c->gpp.p, c->fscr.p);
+        SEMO_LAUNCH(c, "gg_cell", cell_faces_bytes(c, 24.0 + 8.0) + 8.0 * c->nif + 8.0 * c->nbf + 40.0 * c->nc,
                     k_gg_cell, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p,
                     c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->c_vol.p, pp, c->fscr.p, c->gtmp.p);
         std::swap(c->gpp, c->gtmp);
@@ -1741,12 +1865,7 @@ void lsq1_noalloc(semo_ctx* c, const double* phi, const double* faceR_b, double*
     dim3 gb(nblk(n)), tb(TPB);
     double* xph = c->noalloc_xph;
     SEMO_LAUNCH(c, "lsq1_pack", 64.0 * n, k_pack_xphi, gb, tb, n, c->c_xyz.p, phi, xph, (const int*)nullptr);
-    if (c->d_st_dl.n)
-        SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int16_t>, gb, tb, n, c->d_st_sl.p, c->d_st_dl.p, c->d_bf_ptr.p,
-                    c->d_bf_idx.p, xph, c->f_xyz.p, c->c_lsq.p, faceR_b, c->nif, grad, (const int*)nullptr);
-    else
-        SEMO_LAUNCH(c, "lsq1", lsq1_bytes(c), k_lsq1<int>, gb, tb, n, c->d_st_sl.p, c->d_st_ell.p, c->d_bf_ptr.p,
-                    c->d_bf_idx.p, xph, c->f_xyz.p, c->c_lsq.p, faceR_b, c->nif, grad, (const int*)nullptr);
+    lsq1_launch(c, xph, faceR_b, c->nif, grad, (const int*)nullptr);
     SEMO_HIP(hipGetLastError());
 }
 

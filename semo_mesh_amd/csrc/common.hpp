@@ -118,6 +118,8 @@ struct Profiler {
 // CSR instead (lanes > 0): sl_ptr then holds the n+1 row offsets and `lanes`
 // consecutive lanes of a wavefront share one row.
 constexpr int SL = 64;
+// default lanes per cell of the one-field LSQ pass
+constexpr int LSQ_LANES = 1;
 // Inside a slice the entries go in pairs: pair m (entries 2m, 2m+1 of each
 // row) of the 64 rows is 128 consecutive elements, row r's two at 2r, 2r+1,
 // so one 16-byte (FP64) or 8-byte (FP32) load per lane brings two entries of
@@ -270,6 +272,9 @@ struct semo_ctx {
     semo::DBuf<double> Xvec;                  // [3][n] solutions
     semo::DBuf<double> fscr;                  // per-face scratch [8][nif]
     int last_eq = -1;
+    // lanes per cell of the one-field LSQ pass (1, 2 or 4; the results are
+    // the same bits); SEMO_GPU_LSQ_LANES at context creation overrides it
+    int lsq_lanes = semo::LSQ_LANES;
 
     // ---- case ----
     semo_case cs{};

@@ -45,10 +45,15 @@ struct Coupled {
     DBuf<double> val, diag, rhs;                // [nnzb][16], [n][16], [n][4]
     DBuf<double> mval, mdiag, mrhs, pval, pdiag, prhs;   // scalar momentum / pressure systems
     DBuf<double> x;                             // [n][4] last solution
-    // the couplings the preconditioner reads, FP32: per block (diagonal
-    // blocks first, then the off-diagonal ones) the continuity row's u'
-    // entries (0, 1..3) and the momentum rows' p' entries (1..3, 0)
-    DBuf<float> dg;                             // [n + nnzb][6]
+    // the couplings the preconditioner reads, FP32.  gc [3][nv]: per slot of
+    // the scalar pattern (the sliced ELL of the pressure and momentum
+    // matrices, entry pairs) the off-diagonal block's continuity-row u'
+    // entries (0, 1..3); its momentum rows' p' entries (1..3, 0) are the same
+    // three numbers (k_cpl_row writes both as sg n_k A w_j).  Padding slots
+    // hold 0.  dd [6][n]: the diagonal block's (0, 1..3) and (1..3, 0)
+    // entries.  im [n]: 1 / diag(M).
+    DBuf<float> gc, dd;
+    DBuf<double> im;
     DBuf<double> kv;                            // Krylov vectors
     DBuf<double> tp, zp;                        // Schur-complement right side / solution [n]
     DBuf<double> part, red, sc;                 // host-summed partials; device partials, scalars
@@ -69,12 +74,10 @@ __global__ void __launch_bounds__(TPB) k_cpl_row(
     const int* __restrict__ fso, const int* __restrict__ fsn, const double* __restrict__ mval,
     const double* __restrict__ mdiag, const double* __restrict__ pval, const double* __restrict__ pdiag,
     const double* __restrict__ mrhs, const double* __restrict__ prhs, const double* __restrict__ U,
-    const double* __restrict__ V, const double* __restrict__ W, const int* __restrict__ bptr,
-    double* __restrict__ val, double* __restrict__ diag, double* __restrict__ rhs) {
+    const double* __restrict__ V, const double* __restrict__ W, double* __restrict__ val, double* __restrict__ diag, double* __restrict__ rhs, int64_t nv,
+    float* __restrict__ gc, float* __restrict__ dd, double* __restrict__ im) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
-    for (int e = bptr[i]; e < bptr[i + 1]; ++e)
-        for (int q = 0; q < 16; ++q) val[16 * (size_t)e + q] = 0.0;
     double D[16];
     for (int q = 0; q < 16; ++q) D[q] = 0.0;
     double uc[3] = {U[i], V[i], W[i]};
@@ -83,9 +86,14 @@ __global__ void __launch_bounds__(TPB) k_cpl_row(
         double A = area[f];
         if (f < nif) {
             double wC = wCv[f];
-            double* B = val + 16 * (size_t)(own ? fso[f] : fsn[f]);
-            double a = own ? mval[slot_own[f]] : mval[slot_nbr[f]];
-            double q = own ? pval[slot_own[f]] : pval[slot_nbr[f]];
+            // the block of this face's neighbour in row i: no other face
+            // adds to it (one face per pair of cells), so it is formed in
+            // registers and stored once, 16-byte stores
+            double B[16];
+            for (int q = 0; q < 16; ++q) B[q] = 0.0;
+            int slot = own ? slot_own[f] : slot_nbr[f];
+            double a = mval[slot];
+            double q = pval[slot];
             B[0] += q;
             for (int k = 1; k < 4; ++k) B[5 * k] += a;
             double wi = own ? wC : 1.0 - wC, wj = 1.0 - wi, sg = own ? 1.0 : -1.0;
@@ -93,9 +101,12 @@ __global__ void __launch_bounds__(TPB) k_cpl_row(
                 double g = sg * n[k - 1] * A;
                 D[k] += g * wi;       // continuity row, u'_k of this cell
                 B[k] += g * wj;
+                gc[(k - 1) * nv + slot] = (float)B[k];
                 D[4 * k] += g * wi;   // momentum row k, p' of this cell
                 B[4 * k] += g * wj;
             }
+            double2* Bo = reinterpret_cast<double2*>(val + 16 * (size_t)(own ? fso[f] : fsn[f]));
+            for (int q = 0; q < 8; ++q) Bo[q] = make_double2(B[2 * q], B[2 * q + 1]);
             continue;
         }
         semo_patch_bc bc = bcs[bpatch[f - nif]];
@@ -108,21 +119,17 @@ __global__ void __launch_bounds__(TPB) k_cpl_row(
     }
     D[0] += pdiag[i];
     for (int k = 1; k < 4; ++k) D[5 * k] += mdiag[i];
-    for (int q = 0; q < 16; ++q) diag[16 * (size_t)i + q] = D[q];
+    double2* Do = reinterpret_cast<double2*>(diag + 16 * (size_t)i);
+    for (int q = 0; q < 8; ++q) Do[q] = make_double2(D[2 * q], D[2 * q + 1]);
     size_t N = (size_t)nc;
-    rhs[4 * (size_t)i] = prhs[i];
-    for (int k = 0; k < 3; ++k) rhs[4 * (size_t)i + 1 + k] = mrhs[k * N + i];
-}
-
-// the couplings of every block in FP32: (0, 1..3) then (1..3, 0)
-__global__ void __launch_bounds__(TPB) k_cpl_dg(int64_t nb, int n, const double* __restrict__ diag,
-                                                const double* __restrict__ val, float* __restrict__ dg) {
-    int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-    if (e >= nb) return;
-    const double* B = e < n ? diag + 16 * (size_t)e : val + 16 * (size_t)(e - n);
-    float* o = dg + 6 * (size_t)e;
-    o[0] = (float)B[1]; o[1] = (float)B[2]; o[2] = (float)B[3];
-    o[3] = (float)B[4]; o[4] = (float)B[8]; o[5] = (float)B[12];
+    for (int k = 0; k < 3; ++k) {
+        dd[k * N + i] = (float)D[1 + k];
+        dd[(3 + k) * N + i] = (float)D[4 * (1 + k)];
+    }
+    im[i] = 1.0 / mdiag[i];
+    double2* Ro = reinterpret_cast<double2*>(rhs + 4 * (size_t)i);
+    Ro[0] = make_double2(prhs[i], mrhs[i]);
+    Ro[1] = make_double2(mrhs[N + i], mrhs[2 * N + i]);
 }
 
 // ------------------------------------------------------------- block SpMV --
@@ -158,52 +165,137 @@ __global__ void __launch_bounds__(TPB) k_bsr(int n, const int* __restrict__ ptr,
 // as one FP32 V-cycle of the scalar pressure hierarchy.  (Scaling that
 // approximation by 1.3, 1.6 or 2 -- the part D diag(M)^-1 G adds to smooth
 // modes -- raised the BiCGStab iterations of the 1.05M-cell channel from
-// 11.6 to 12.3, 14.1 and 15.2 per outer iteration.)  D and G come from the
-// compact FP32 couplings dg (6 per block).
-__global__ void __launch_bounds__(TPB) k_sp_u(int n, const double* __restrict__ r, const double* __restrict__ mdiag,
-                                              double* __restrict__ z) {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double im = 1.0 / mdiag[i];
-    for (int k = 1; k < 4; ++k) z[4 * (size_t)i + k] = r[4 * (size_t)i + k] * im;
-}
-// tp = r_p - D u* (u* in z's velocity components)
-__global__ void __launch_bounds__(TPB) k_sp_schur_rhs(int n, const int* __restrict__ ptr, const int* __restrict__ col,
-                                                      const float* __restrict__ dg, const double* __restrict__ r,
-                                                      const double* __restrict__ z, double* __restrict__ tp) {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* d = dg + 6 * (size_t)i;
-    const double* zi = z + 4 * (size_t)i;
-    double t = r[4 * (size_t)i];
-    t -= (double)d[0] * zi[1] + (double)d[1] * zi[2] + (double)d[2] * zi[3];
-    const float* o = dg + 6 * (size_t)n;
-    for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
-        const float* a = o + 6 * (size_t)e;
-        const double* zj = z + 4 * (size_t)col[e];
-        t -= (double)a[0] * zj[1] + (double)a[1] * zj[2] + (double)a[2] * zj[3];
+// 11.6 to 12.3, 14.1 and 15.2 per outer iteration.)
+// The two kernels around the V-cycle walk the scalar pattern one lane per
+// row (sliced ELL, entry pairs: the three coupling arrays and the columns
+// are one 8-byte load per lane per pair, a wavefront's load one contiguous
+// run); u* is formed where it is read (r_u 1/diag(M), the same product the
+// correction forms), so no pass writes it.
+
+// the columns of entries e, e + 1 (e even) of the scalar pattern, from
+// int16 offsets column - row or from int32 columns
+template <bool NARROW>
+__device__ inline int2 pat_cols(const int* __restrict__ col, const int16_t* __restrict__ dcol, int e, int row) {
+    if constexpr (NARROW) {
+        int w = *reinterpret_cast<const int*>(dcol + e);
+        return make_int2(row + (int)(int16_t)(w & 0xffff), row + (w >> 16));
+    } else {
+        return *reinterpret_cast<const int2*>(col + e);
     }
-    tp[i] = t;
+}
+
+constexpr int PC_PAIRS = 2;   // entry pairs per trip: their loads, then their gathers
+
+// tp = r_p - D u*
+template <bool NARROW>
+__global__ void __launch_bounds__(TPB) k_sp_schur_rhs(int n, const int* __restrict__ sl_ptr,
+                                                      const int* __restrict__ col, const int16_t* __restrict__ dcol,
+                                                      const float* __restrict__ gc, int64_t nv,
+                                                      const float* __restrict__ dd, const double* __restrict__ r,
+                                                      const double* __restrict__ im, double* __restrict__ tp) {
+    int row = xcd_block() * TPB + threadIdx.x;
+    int s = row >> 6, lane = row & (SL - 1);
+    if (s >= (n + SL - 1) / SL) return;
+    double t = 0.0;
+    if (row < n) {
+        double2 r01 = *reinterpret_cast<const double2*>(r + 4 * (size_t)row);
+        double2 r23 = *reinterpret_cast<const double2*>(r + 4 * (size_t)row + 2);
+        double m = im[row];
+        t = r01.x;
+        t -= (double)dd[row] * (r01.y * m) + (double)dd[(size_t)n + row] * (r23.x * m) +
+             (double)dd[2 * (size_t)n + row] * (r23.y * m);
+    }
+    int end = sl_ptr[s + 1];
+    for (int e = sl_ptr[s] + 2 * lane; e < end; e += 2 * SL * PC_PAIRS) {
+        int2 j[PC_PAIRS];
+        float2 a0[PC_PAIRS], a1[PC_PAIRS], a2[PC_PAIRS];
+#pragma unroll
+        for (int u = 0; u < PC_PAIRS; ++u) {
+            int ee = e + 2 * SL * u;
+            if (ee < end) {
+                j[u] = pat_cols<NARROW>(col, dcol, ee, row);
+                a0[u] = *reinterpret_cast<const float2*>(gc + ee);
+                a1[u] = *reinterpret_cast<const float2*>(gc + nv + ee);
+                a2[u] = *reinterpret_cast<const float2*>(gc + 2 * nv + ee);
+            } else {
+                j[u] = make_int2(-1, -1);
+            }
+        }
+        double2 x0[PC_PAIRS], x1[PC_PAIRS], y0[PC_PAIRS], y1[PC_PAIRS];
+        double m0[PC_PAIRS], m1[PC_PAIRS];
+#pragma unroll
+        for (int u = 0; u < PC_PAIRS; ++u)
+            if (j[u].x >= 0) {
+                x0[u] = *reinterpret_cast<const double2*>(r + 4 * (size_t)j[u].x);
+                x1[u] = *reinterpret_cast<const double2*>(r + 4 * (size_t)j[u].x + 2);
+                y0[u] = *reinterpret_cast<const double2*>(r + 4 * (size_t)j[u].y);
+                y1[u] = *reinterpret_cast<const double2*>(r + 4 * (size_t)j[u].y + 2);
+                m0[u] = im[j[u].x];
+                m1[u] = im[j[u].y];
+            }
+#pragma unroll
+        for (int u = 0; u < PC_PAIRS; ++u)
+            if (j[u].x >= 0) {
+                t -= (double)a0[u].x * (x0[u].y * m0[u]) + (double)a1[u].x * (x1[u].x * m0[u]) +
+                     (double)a2[u].x * (x1[u].y * m0[u]);
+                t -= (double)a0[u].y * (y0[u].y * m1[u]) + (double)a1[u].y * (y1[u].x * m1[u]) +
+                     (double)a2[u].y * (y1[u].y * m1[u]);
+            }
+    }
+    if (row < n) tp[row] = t;
 }
 // z_p = p', z_u = u* - diag(M)^-1 G p'
-__global__ void __launch_bounds__(TPB) k_sp_correct(int n, const int* __restrict__ ptr, const int* __restrict__ col,
-                                                    const float* __restrict__ dg, const double* __restrict__ mdiag,
-                                                    const double* __restrict__ zp, double* __restrict__ z) {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* d = dg + 6 * (size_t)i;
-    double p = zp[i];
-    double g[3] = {(double)d[3] * p, (double)d[4] * p, (double)d[5] * p};
-    const float* o = dg + 6 * (size_t)n;
-    for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
-        const float* a = o + 6 * (size_t)e;
-        double pj = zp[col[e]];
-        g[0] += (double)a[3] * pj; g[1] += (double)a[4] * pj; g[2] += (double)a[5] * pj;
+template <bool NARROW>
+__global__ void __launch_bounds__(TPB) k_sp_correct(int n, const int* __restrict__ sl_ptr,
+                                                    const int* __restrict__ col, const int16_t* __restrict__ dcol,
+                                                    const float* __restrict__ gc, int64_t nv,
+                                                    const float* __restrict__ dd, const double* __restrict__ r,
+                                                    const double* __restrict__ im, const double* __restrict__ zp,
+                                                    double* __restrict__ z) {
+    int row = xcd_block() * TPB + threadIdx.x;
+    int s = row >> 6, lane = row & (SL - 1);
+    if (s >= (n + SL - 1) / SL) return;
+    double p = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (row < n) {
+        p = zp[row];
+        g0 = (double)dd[3 * (size_t)n + row] * p;
+        g1 = (double)dd[4 * (size_t)n + row] * p;
+        g2 = (double)dd[5 * (size_t)n + row] * p;
     }
-    double im = 1.0 / mdiag[i];
-    double* zi = z + 4 * (size_t)i;
-    zi[0] = p;
-    for (int k = 0; k < 3; ++k) zi[1 + k] -= g[k] * im;
+    int end = sl_ptr[s + 1];
+    for (int e = sl_ptr[s] + 2 * lane; e < end; e += 2 * SL * PC_PAIRS) {
+        int2 j[PC_PAIRS];
+        float2 a0[PC_PAIRS], a1[PC_PAIRS], a2[PC_PAIRS];
+#pragma unroll
+        for (int u = 0; u < PC_PAIRS; ++u) {
+            int ee = e + 2 * SL * u;
+            if (ee < end) {
+                j[u] = pat_cols<NARROW>(col, dcol, ee, row);
+                a0[u] = *reinterpret_cast<const float2*>(gc + ee);
+                a1[u] = *reinterpret_cast<const float2*>(gc + nv + ee);
+                a2[u] = *reinterpret_cast<const float2*>(gc + 2 * nv + ee);
+            } else {
+                j[u] = make_int2(-1, -1);
+            }
+        }
+        double q0[PC_PAIRS], q1[PC_PAIRS];
+#pragma unroll
+        for (int u = 0; u < PC_PAIRS; ++u)
+            if (j[u].x >= 0) { q0[u] = zp[j[u].x]; q1[u] = zp[j[u].y]; }
+#pragma unroll
+        for (int u = 0; u < PC_PAIRS; ++u)
+            if (j[u].x >= 0) {
+                g0 += (double)a0[u].x * q0[u]; g1 += (double)a1[u].x * q0[u]; g2 += (double)a2[u].x * q0[u];
+                g0 += (double)a0[u].y * q1[u]; g1 += (double)a1[u].y * q1[u]; g2 += (double)a2[u].y * q1[u];
+            }
+    }
+    if (row >= n) return;
+    double m = im[row];
+    double2 r01 = *reinterpret_cast<const double2*>(r + 4 * (size_t)row);
+    double2 r23 = *reinterpret_cast<const double2*>(r + 4 * (size_t)row + 2);
+    double* zi = z + 4 * (size_t)row;
+    *reinterpret_cast<double2*>(zi) = make_double2(p, r01.y * m - g0 * m);
+    *reinterpret_cast<double2*>(zi + 2) = make_double2(r23.x * m - g1 * m, r23.y * m - g2 * m);
 }
 
 // ----------------------------------------------------------- Krylov kernels --
@@ -353,12 +445,24 @@ double2 dot2(semo_ctx* c, Coupled& C, size_t n, const double* a, const double* b
 // couplings and pressure V-cycle)
 void precond(semo_ctx* c, Coupled& C, const double* r, double* z) {
     int n = C.n;
-    SEMO_LAUNCH(c, "cpl_precond_u", 56.0 * n, k_sp_u, dim3(nblk(n)), dim3(TPB), n, r, C.mdiag.p, z);
-    SEMO_LAUNCH(c, "cpl_precond_schur_rhs", 28.0 * C.nnzb + 68.0 * n, k_sp_schur_rhs, dim3(nblk(n)), dim3(TPB), n,
-                C.ptr.p, C.col.p, C.dg.p, r, z, C.tp.p);
+    const DevSell<double>& A = c->Amat;
+    int64_t nv = (int64_t)c->pat.col.size();
+    int nb = nblk((long)c->pat.nslices * SL);
+    bool narrow = A.dcol.p != nullptr;
+    double idx = narrow ? 2.0 : 4.0;
+    if (narrow)
+        SEMO_LAUNCH(c, "cpl_precond_schur_rhs", (idx + 12.0) * nv + 60.0 * n, k_sp_schur_rhs<true>, dim3(nb),
+                    dim3(TPB), n, A.sl_ptr.p, A.col.p, A.dcol.p, C.gc.p, nv, C.dd.p, r, C.im.p, C.tp.p);
+    else
+        SEMO_LAUNCH(c, "cpl_precond_schur_rhs", (idx + 12.0) * nv + 60.0 * n, k_sp_schur_rhs<false>, dim3(nb),
+                    dim3(TPB), n, A.sl_ptr.p, A.col.p, A.dcol.p, C.gc.p, nv, C.dd.p, r, C.im.p, C.tp.p);
     pressure_vcycle(c, C.tp.p, C.zp.p, true);
-    SEMO_LAUNCH(c, "cpl_precond_correct", 28.0 * C.nnzb + 100.0 * n, k_sp_correct, dim3(nblk(n)), dim3(TPB), n,
-                C.ptr.p, C.col.p, C.dg.p, C.mdiag.p, C.zp.p, z);
+    if (narrow)
+        SEMO_LAUNCH(c, "cpl_precond_correct", (idx + 12.0) * nv + 84.0 * n, k_sp_correct<true>, dim3(nb),
+                    dim3(TPB), n, A.sl_ptr.p, A.col.p, A.dcol.p, C.gc.p, nv, C.dd.p, r, C.im.p, C.zp.p, z);
+    else
+        SEMO_LAUNCH(c, "cpl_precond_correct", (idx + 12.0) * nv + 84.0 * n, k_sp_correct<false>, dim3(nb),
+                    dim3(TPB), n, A.sl_ptr.p, A.col.p, A.dcol.p, C.gc.p, nv, C.dd.p, r, C.im.p, C.zp.p, z);
 }
 
 void spmv64(semo_ctx* c, Coupled& C, const double* x, double* y) {
@@ -408,7 +512,10 @@ void coupled_setup(semo_ctx* c, Coupled& C) {
     C.sc.alloc(16);
 
     C.nnzb = (int64_t)nnzb;
-    C.dg.alloc(6 * ((size_t)n + nnzb));
+    C.gc.alloc(3 * std::max<size_t>(nv, 1));
+    C.gc.zero(s);                               // padding slots stay 0
+    C.dd.alloc(6 * (size_t)n);
+    C.im.alloc(n);
     SEMO_HIP(hipStreamSynchronize(s));
     C.built = true;
 }
@@ -438,14 +545,13 @@ void assemble_coupled(semo_ctx* c) {
     copy_d(c, C.mdiag.p, c->Amat.diag.p, n);
     copy_d(c, C.mrhs.p, c->Bvec.p, 3 * (size_t)n);
     SEMO_LAUNCH(c, "cpl_assemble",
-                cell_faces_bytes(c, 56.0) + 32.0 * c->nif + 128.0 * ((double)n + C.nnzb) + 112.0 * n, k_cpl_row,
-                dim3(nblk(n)), dim3(TPB), n, c->nif, c->nbf, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_nbr.p,
-                c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->f_wC.p, c->d_slot_own.p, c->d_slot_nbr.p,
-                C.fslot_own.p, C.fslot_nbr.p, C.mval.p, C.mdiag.p, C.pval.p, C.pdiag.p, C.mrhs.p, C.prhs.p, c->U.p,
-                c->V.p, c->W.p, C.ptr.p, C.val.p, C.diag.p, C.rhs.p);
-    int64_t nb = (int64_t)n + C.nnzb;
-    SEMO_LAUNCH(c, "cpl_couplings_fp32", 152.0 * nb, k_cpl_dg, dim3(nblk((long)nb)), dim3(TPB), nb, n, C.diag.p,
-                C.val.p, C.dg.p);
+                cell_faces_bytes(c, 56.0) + 32.0 * c->nif + 128.0 * ((double)n + C.nnzb) + 104.0 * n +
+                    24.0 * c->nif + 32.0 * n,
+                k_cpl_row, dim3(nblk(n)), dim3(TPB), n, c->nif, c->nbf, c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p,
+                c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->f_wC.p, c->d_slot_own.p,
+                c->d_slot_nbr.p, C.fslot_own.p, C.fslot_nbr.p, C.mval.p, C.mdiag.p, C.pval.p, C.pdiag.p, C.mrhs.p,
+                C.prhs.p, c->U.p, c->V.p, c->W.p, C.val.p, C.diag.p, C.rhs.p, (int64_t)nv, C.gc.p, C.dd.p,
+                C.im.p);
     c->last_eq = 3;
 }
 

@@ -13,8 +13,8 @@ import pytest
 import scipy.sparse as sp
 
 from conftest import GOLDEN_CASES, golden_case, load_golden
-from semo_mesh_amd.cases import cavity_case
-from semo_mesh_amd.mesh import cavity_mesh, hex_mesh
+from semo_mesh_amd.cases import cavity_case, dam_break_case
+from semo_mesh_amd.mesh import cavity_mesh, hex_mesh, tet_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -247,6 +247,33 @@ def test_lsq_and_minmax_vs_oracle(oracle_lib):
         mx, mn = g.setCellVarMinMax(phi); mx0, mn0 = o.minmax(phi)
         assert np.array_equal(mx, mx0) and np.array_equal(mn, mn0)
         g.close()
+
+
+@pytest.mark.parametrize("lanes", ["2", "4"])
+def test_lsq_lanes_same_bits(oracle_lib, monkeypatch, lanes):
+    """The one-field LSQ pass with 2 or 4 lanes per cell (the group spreads a
+    cell's stencil gathers, then adds the terms in stencil order on every
+    lane) gives the bits of the one-lane pass, and the oracle's on the
+    un-renumbered mesh: a jittered tet mesh, whose vertex stencils hold up to
+    about 70 cells in counts that are not multiples of the group."""
+    mesh = tet_mesh(5, 4, 4, jitter=0.1, seed=2)
+    case = dam_break_case(dt=1e-4, n_vof=1, urf_vf=1.0)
+    o = oracle_lib.Oracle(mesh, case)
+    rng = np.random.default_rng(7)
+    phi = rng.normal(size=mesh.n_cells); fr = rng.normal(size=mesh.n_faces)
+    ref = o.lsq(phi, fr)
+    for reorder in (False, True):
+        out = {}
+        for lv in ("1", lanes):
+            monkeypatch.setenv("SEMO_GPU_LSQ_LANES", lv)
+            g = _gpu(mesh, case, reorder)
+            try:
+                out[lv] = g.calcLeastSquare(phi, fr)
+            finally:
+                g.close()
+        assert np.array_equal(out[lanes], out["1"]), reorder
+        if not reorder:
+            assert np.array_equal(out[lanes], ref)
 
 
 @pytest.mark.parametrize("name", GOLDEN_CASES)
