@@ -228,8 +228,8 @@ __global__ void __launch_bounds__(TPB) k_gravity(
 // fused with setCellVarMinMax (cellVarMinMax.cpp:15-24) of U, V, W (and P).
 //
 // Layout: each cell's centroid and the five fields sit in one 64-byte record
-// cpk[8i..8i+7] = [x,y,z,P,U,V,W,VF], so a stencil neighbour costs one cache
-// line; the stencil is sliced ELL (lane = cell inside a 64-cell slice) so the
+// cpk[8i..8i+7] = [x,y,z,U,V,W,P,VF], so a stencil neighbour costs one cache
+// line, and the three-field pass (U, V, W) reads only its first 48 bytes; the stencil is sliced ELL (lane = cell inside a 64-cell slice) so the
 // index loads of a wavefront are contiguous.  Summation order is the stencil
 // order, then the cell's boundary faces in ascending order (as the reference).
 __global__ void __launch_bounds__(TPB) k_pack5(int nc, const double* __restrict__ P, const double* __restrict__ U,
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(TPB) k_pack5(int nc, const double* __restrict_
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double* o = cpk + 8 * (size_t)i;
-    o[3] = P[i]; o[4] = U[i]; o[5] = V[i]; o[6] = W[i]; o[7] = VF[i];
+    o[3] = U[i]; o[4] = V[i]; o[5] = W[i]; o[6] = P[i]; o[7] = VF[i];
 }
 
 // U, V, W only (the pressure stage: P and VF are unchanged since the last pack)
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(TPB) k_pack3(int nc, const double* __restrict_
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     double* o = cpk + 8 * (size_t)i;
-    o[4] = U[i]; o[5] = V[i]; o[6] = W[i];
+    o[3] = U[i]; o[4] = V[i]; o[5] = W[i];
 }
 
 __global__ void __launch_bounds__(TPB) k_pack_xyz(int nc, const double* __restrict__ cxyz, double* __restrict__ cpk) {
@@ -281,9 +281,9 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     const double2* me = reinterpret_cast<const double2*>(cpk + 8 * (size_t)i);
-    double2 m0 = me[0], m1 = me[1], m2 = me[2], m3 = me[3];
+    double2 m0 = me[0], m1 = me[1], m2 = me[2], m3 = NF == 5 ? me[3] : make_double2(0.0, 0.0);
     double xi = m0.x, yi = m0.y, zi = m1.x;
-    double ph[5] = {m1.y, m2.x, m2.y, m3.x, m3.y};
+    double ph[5] = {m3.x, m1.y, m2.x, m2.y, m3.y};
     double t[5][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     double mx[4] = {ph[0], ph[1], ph[2], ph[3]}, mn[4] = {ph[0], ph[1], ph[2], ph[3]};
     int s = i >> 6, lane = i & 63;
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
     auto add = [&](const double2& o0, const double2& o1, const double2& o2, const double2& o3) {
         double dx = o0.x - xi, dy = o0.y - yi, dz = o1.x - zi;
         double wk = 1.0 / (dx * dx + dy * dy + dz * dz);
-        double pj[5] = {o1.y, o2.x, o2.y, o3.x, o3.y};
+        double pj[5] = {o3.x, o1.y, o2.x, o2.y, o3.y};
 #pragma unroll
         for (int v = (NF == 5 ? 0 : 1); v < (NF == 5 ? 5 : 4); ++v) {
             double d = pj[v] - ph[v];
@@ -323,7 +323,8 @@ __global__ void __launch_bounds__(TPB) k_grad_minmax(
             j[u] = jn[u];
             if (j[u] >= 0) {
                 const double2* o = reinterpret_cast<const double2*>(cpk + 8 * (size_t)j[u]);
-                rec[u][0] = o[0]; rec[u][1] = o[1]; rec[u][2] = o[2]; rec[u][3] = o[3];
+                rec[u][0] = o[0]; rec[u][1] = o[1]; rec[u][2] = o[2];
+                rec[u][3] = NF == 5 ? o[3] : make_double2(0.0, 0.0);
             }
         }
         e += 64 * UNR;
@@ -1672,11 +1673,11 @@ void assemble_vof(semo_ctx* c) {
 // the stencil pass of the one-field LSQ gradient over packed records xph
 void lsq1_launch(semo_ctx* c, const double* xph, const double* faceR, int foff, double* g, const int* gate) {
     int n = c->nc;
-    int G = c->lsq_lanes;     // 1: k_lsq1; 2, 4: k_lsq1g
-    dim3 tb(TPB), gb(nblk((long)n * G));
+    int G = c->lsq_lanes_eff;     // 1: k_lsq1; 2, 4: k_lsq1g
     const int* sl = c->d_st_sl.p;
     const int* bp = c->d_bf_ptr.p;
     const int* bi = c->d_bf_idx.p;
+    dim3 tb(TPB), gb(nblk((long)n * G));
     if (c->d_st_dl.n) {
         const int16_t* st = c->d_st_dl.p;
         if (G == 4)

@@ -203,7 +203,7 @@ int semo_ctx_create(int device, void* stream, semo_ctx** out) {
         c->graphs = !(gv && gv[0] == '0');
         const char* lv = std::getenv("SEMO_GPU_LSQ_LANES");
         if (lv) c->lsq_lanes = std::atoi(lv);
-        if (c->lsq_lanes != 2 && c->lsq_lanes != 4) c->lsq_lanes = 1;
+        if (c->lsq_lanes != 1 && c->lsq_lanes != 2 && c->lsq_lanes != 4) c->lsq_lanes = 0;
         c->noalloc_xph = noalloc_scratch(device);
         c->sp.precision = SEMO_PREC_FP64; c->sp.tol_p = 1e-8; c->sp.tol_u = 1e-8;
         c->sp.maxit_p = 500; c->sp.maxit_u = 500; c->sp.cheb_degree = 1; c->sp.coarse_size = 1000;

@@ -118,8 +118,9 @@ struct Profiler {
 // CSR instead (lanes > 0): sl_ptr then holds the n+1 row offsets and `lanes`
 // consecutive lanes of a wavefront share one row.
 constexpr int SL = 64;
-// default lanes per cell of the one-field LSQ pass
-constexpr int LSQ_LANES = 1;
+// lanes per cell of the one-field LSQ pass when not set: 4 from this
+// average stencil length on, else 1
+constexpr double LSQ_LANES_LONG_STENCIL = 40.0;
 // Inside a slice the entries go in pairs: pair m (entries 2m, 2m+1 of each
 // row) of the 64 rows is 128 consecutive elements, row r's two at 2r, 2r+1,
 // so one 16-byte (FP64) or 8-byte (FP32) load per lane brings two entries of
@@ -273,8 +274,9 @@ struct semo_ctx {
     semo::DBuf<double> fscr;                  // per-face scratch [8][nif]
     int last_eq = -1;
     // lanes per cell of the one-field LSQ pass (1, 2 or 4; the results are
-    // the same bits); SEMO_GPU_LSQ_LANES at context creation overrides it
-    int lsq_lanes = semo::LSQ_LANES;
+    // the same bits): lsq_lanes from SEMO_GPU_LSQ_LANES at context creation
+    // (0: by stencil length), lsq_lanes_eff the one the mesh set-up chose
+    int lsq_lanes = 0, lsq_lanes_eff = 1;
 
     // ---- case ----
     semo_case cs{};
@@ -356,6 +358,12 @@ inline double cell_faces_bytes(const semo_ctx* c, double per_face) {
     do {                                                                                    \
         ::semo::ProfScope semo_ps_((c), (tag), (c)->prof.on ? (double)(bytes) : 0.0);       \
         hipLaunchKernelGGL(kern, grid, block, 0, (c)->stream, __VA_ARGS__);                 \
+    } while (0)
+// the same with `shmem` bytes of dynamic LDS
+#define SEMO_LAUNCH_LDS(c, tag, bytes, kern, grid, block, shmem, ...)                      \
+    do {                                                                                    \
+        ::semo::ProfScope semo_ps_((c), (tag), (c)->prof.on ? (double)(bytes) : 0.0);       \
+        hipLaunchKernelGGL(kern, grid, block, (shmem), (c)->stream, __VA_ARGS__);           \
     } while (0)
 
 // ------------------------------------------------- Krylov convergence tests --
