@@ -169,6 +169,9 @@ typedef enum {
        updated by the incompressible iteration; read by SEMO_BC_SWITCH.
        Settable after that step.                                             */
     SEMO_F_C = 40,
+    /* subgrid viscosity and kinetic energy of the LES model (semo_set_les),
+       as the last calcCellTransport left them; read-only                    */
+    SEMO_F_MUT = 41, SEMO_F_KSGS = 42,
     SEMO_F_X0 = 48, SEMO_F_X1, SEMO_F_X2,                  /* last solution     */
     SEMO_F_B0 = 56, SEMO_F_B1, SEMO_F_B2,                  /* last right sides  */
     /* geometry (read-only): cell arrays, then face arrays (original face order) */
@@ -317,6 +320,27 @@ int semo_least_square_noalloc(semo_ctx* ctx, const double* d_phi, const double* 
  * values varR[fn] (read at boundary faces), grad[3n] in: the gradient used
  * for the skewness correction and the extrapolation, out: the new gradient. */
 int semo_gauss_green(semo_ctx* ctx, int field, const double* phi, const double* face_r, double* grad);
+/* SEMO_TurbModel_LES (src/turbulenceModels/LES.cpp): calcWALE (:29-131),
+ * calcSmagorinsky (:141-220) or calcDynamicSmagorinsky (:224-467), which
+ * calcCellTransport calls (src/solvers/transport.cpp:39-56) for turbType LES.
+ * The velocity gradients are calcGaussGreen's, two passes from zero, with the
+ * U, V, W patch types of semo_set_case and the right face values face_rU,
+ * face_rV, face_rW [n_faces] (varR of fU, fV, fW, read at boundary faces).
+ * U, V, W, rho [n] cell values; muT, kSGS [n] out: the subgrid viscosity and
+ * kinetic energy.  The dynamic model filters over each cell's vertex stencil
+ * (the calcLeastSquare stencil).                                              */
+enum { SEMO_LES_WALE = 1, SEMO_LES_SMAGORINSKY = 2, SEMO_LES_DYNAMIC_SMAGORINSKY = 3 };
+/* turbType / turbLESModel of constant/turbulenceProperties (src/controls/
+ * build.cpp:100-115): model 0 = laminar (the default), else an SEMO_LES_*
+ * model, which every calcCellTransport of the time step then applies
+ * (src/solvers/transport.cpp:38-97): at the first step and after each VOF
+ * iteration (src/solvers/pressureBased.cpp:17-19, 115-116) mu = the phases'
+ * mixture + muT, with the velocity gradients taken from the boundary faces'
+ * right values of that moment -- zero at the first step, as SEMO holds them
+ * before its first reconstruction.                                          */
+int semo_set_les(semo_ctx* ctx, int model);
+int semo_les(semo_ctx* ctx, int model, const double* U, const double* V, const double* W, const double* rho,
+             const double* face_rU, const double* face_rV, const double* face_rW, double* muT, double* kSGS);
 /* setCellVarMinMax (src/solvers/cellVarMinMax.cpp:8-72)                        */
 int semo_cell_min_max(semo_ctx* ctx, const double* phi, double* vmax, double* vmin);
 

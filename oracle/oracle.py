@@ -56,6 +56,8 @@ def lib():
         L.orc_lsq.argtypes = [C.c_void_p, _D, _D, _D]
         L.orc_minmax.argtypes = [C.c_void_p, _D, _D, _D]
         L.orc_gauss_green.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
+        L.orc_les.argtypes = [C.c_void_p, C.c_int] + [_D] * 9
+        L.orc_set_les.argtypes = [C.c_void_p, C.c_int]
         L.orc_solve.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.orc_momentum_update.restype = C.c_double
         L.orc_momentum_update.argtypes = [C.c_void_p]
@@ -112,6 +114,8 @@ class Oracle:
         if pts is not None:
             a = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 4)
             L.orc_set_vf_points(self._h, a.shape[0], a)
+        les = getattr(case, "les_model", None)
+        L.orc_set_les(self._h, {"WALE": 1, "smagorinsky": 2, "dynamicSmagorinsky": 3}[les] if les else 0)
 
     def set_solver(self, tol=1e-13, maxit=5000):
         lib().orc_set_solver(self._h, tol, maxit)
@@ -151,6 +155,14 @@ class Oracle:
         lib().orc_gauss_green(self._h, sel, np.ascontiguousarray(phi, dtype=np.float64),
                               np.ascontiguousarray(face_r, dtype=np.float64), g)
         return g.reshape(-1, 3)
+
+    def les(self, model, U, V, W, rho, face_rU, face_rV, face_rW):
+        """muT, kSGS of LES model 1 (WALE), 2 (Smagorinsky) or 3 (dynamic)."""
+        n = self.mesh.n_cells
+        mu = np.empty(n); k = np.empty(n)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (U, V, W, rho, face_rU, face_rV, face_rW)]
+        lib().orc_les(self._h, model, *a, mu, k)
+        return mu, k
 
     def minmax(self, phi):
         mx = np.empty(self.mesh.n_cells); mn = np.empty(self.mesh.n_cells)

@@ -30,6 +30,7 @@
 #include "solvers/build.h"
 #include "math/math.h"
 #include "save/save.h"
+#include "turbulenceModels/LES.h"
 
 using std::string;
 using std::vector;
@@ -213,6 +214,8 @@ int main(int argc, char** argv) {
     int adjust_dt = 0;
     double max_co = 1.0, max_dt = -1.0;
     int nsteps = 1, ncorr = 1, nvof = 0;
+    int les = 0;          // "les 1": the three LES models on the final state
+    string turb;          // "turb MODEL": turbType LES with turbLESModel MODEL
     vector<PatchBC> bcs;
     string initfile;
     {
@@ -232,6 +235,8 @@ int main(int argc, char** argv) {
             else if (key == "urfvf") cf >> urf_vf;
             else if (key == "adjustdt") cf >> adjust_dt >> max_co >> max_dt;
             else if (key == "init") cf >> initfile;
+            else if (key == "les") cf >> les;
+            else if (key == "turb") cf >> turb;
             else if (key == "bc") {
                 PatchBC b;
                 cf >> b.name >> b.ptype >> b.pval >> b.utype >> b.u[0] >> b.u[1] >> b.u[2] >> b.vftype >> b.vfval;
@@ -264,7 +269,8 @@ int main(int argc, char** argv) {
     controls.iterReal = 0; controls.iterTotal = 0; controls.time = 0.0;
     controls.iterPBsMax = ncorr; controls.iterPreMax = 1; controls.iterMomMax = 1;
     controls.iterVofMax = nvof;
-    controls.turbType = "laminar";
+    controls.turbType = turb.empty() ? "laminar" : "LES";
+    controls.turbLESModel = turb;
     controls.startClock = 0.0;
 
     if (mode == "loadvtu") {
@@ -460,6 +466,19 @@ int main(int argc, char** argv) {
             dump(string("gg_faceR_") + q.name, fr);
             dump(string("gg_out_") + q.name, gout);
         }
+    }
+    if (les) {   // SEMO_TurbModel_LES (turbulenceModels/LES.cpp): each model sets muT and kSGS
+        SEMO_TurbModel_LES model;
+        dump_cellvar(mesh, controls.Rho, "les_Rho");
+        model.calcWALE(mesh, controls, species);
+        dump_cellvar(mesh, controls.muT, "les_wale_muT");
+        dump_cellvar(mesh, controls.kSGS, "les_wale_kSGS");
+        model.calcSmagorinsky(mesh, controls, species);
+        dump_cellvar(mesh, controls.muT, "les_smagorinsky_muT");
+        dump_cellvar(mesh, controls.kSGS, "les_smagorinsky_kSGS");
+        model.calcDynamicSmagorinsky(mesh, controls, species);
+        dump_cellvar(mesh, controls.muT, "les_dynamic_muT");
+        dump_cellvar(mesh, controls.kSGS, "les_dynamic_kSGS");
     }
     double ts = std::chrono::duration<double>(t1 - t0).count();
     double tr = std::chrono::duration<double>(t3 - t2).count();

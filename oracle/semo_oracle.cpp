@@ -99,6 +99,10 @@ struct Oracle {
     // VF "function" boundary values per boundary face (reconIncom.cpp:929-
     // 1019): the VF of the nearest point of the set given by orc_set_vf_points
     vector<double> vffun;
+    // turbType LES (controls/build.cpp:100-115): model 1 WALE, 2 Smagorinsky,
+    // 3 dynamic Smagorinsky, 0 laminar; the last calcCellTransport's muT, kSGS
+    int les_model = 0;
+    vector<double> muT, kSGS;
     vector<Real> gP, gU, gV, gW, gVF;    // [3n]
     vector<Real> maxP, minP, maxU, minU, maxV, minV, maxW, minW, maxVF, minVF;
     // face left/right values (src/solvers/reconIncom.cpp:501-1035)
@@ -1045,6 +1049,126 @@ void gauss_green(Oracle& o, int sel, const double* phi, const double* faceR, dou
     std::copy(gt.begin(), gt.end(), grad);
 }
 
+// SEMO_TurbModel_LES (turbulenceModels/LES.cpp): the subgrid viscosity muT
+// and kinetic energy kSGS of a cell from the Green-Gauss gradients of U, V, W
+// (two passes from zero, LES.cpp:46-51, with the faces' right values RU, RV,
+// RW [nf]).  model 1: calcWALE (:29-131), 2: calcSmagorinsky (:141-220),
+// 3: calcDynamicSmagorinsky (:224-467).  pow(x, 2.0) is written x * x (the
+// compiler folds it so; it is exact either way).  Kept as written: WALE's
+// SSd sums the squares of S's diagonal with Sd's off-diagonals (:110-112),
+// and the dynamic model's running sums sumV and sumU collect U and V
+// respectively (:330-331, :415-416) while L uses sumU where U is meant.
+void les(Oracle& o, int model, const double* U, const double* V, const double* W, const double* Rho,
+         const double* RU, const double* RV, const double* RW, double* muT, double* kSGS) {
+    int n = o.nc;
+    vector<double> gu(3 * (size_t)n, 0.0), gv(3 * (size_t)n, 0.0), gw(3 * (size_t)n, 0.0);
+    gauss_green(o, 1, U, RU, gu.data()); gauss_green(o, 1, U, RU, gu.data());
+    gauss_green(o, 2, V, RV, gv.data()); gauss_green(o, 2, V, RV, gv.data());
+    gauss_green(o, 3, W, RW, gw.data()); gauss_green(o, 3, W, RW, gw.data());
+    // the strain-rate tensor S of a cell (LES.cpp:80-88) and S_ij S_ij (:106-108)
+    auto strain = [&](int i, double S[3][3]) {
+        const double* a = &gu[3 * (size_t)i]; const double* b = &gv[3 * (size_t)i];
+        const double* c = &gw[3 * (size_t)i];
+        S[0][0] = a[0]; S[0][1] = 0.5 * (a[1] + b[0]); S[0][2] = 0.5 * (a[2] + c[0]);
+        S[1][0] = S[0][1]; S[1][1] = b[1]; S[1][2] = 0.5 * (b[2] + c[1]);
+        S[2][0] = S[0][2]; S[2][1] = S[1][2]; S[2][2] = c[2];
+        return S[0][0] * S[0][0] + S[1][1] * S[1][1] + S[2][2] * S[2][2] +
+               2.0 * (S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][2] * S[1][2]);
+    };
+    for (int i = 0; i < n; ++i) {
+        double S[3][3];
+        double SS = strain(i, S);
+        double Sabs = std::sqrt(2.0 * SS);
+        double vol = o.cvol[i];
+        if (model == 1) {
+            const double Ci = 0.007, Cw = 0.325;
+            const double* a = &gu[3 * (size_t)i]; const double* b = &gv[3 * (size_t)i];
+            const double* c = &gw[3 * (size_t)i];
+            double Sd01 = 0.5 * (a[1] * a[1] + b[0] * b[0]);
+            double Sd02 = 0.5 * (a[2] * a[2] + c[0] * c[0]);
+            double Sd12 = 0.5 * (b[2] * b[2] + c[1] * c[1]);
+            double SSd = S[0][0] * S[0][0] + S[1][1] * S[1][1] + S[2][2] * S[2][2] +
+                         2.0 * (Sd01 * Sd01 + Sd02 * Sd02 + Sd12 * Sd12);
+            double Ls = Cw * std::pow(vol, 0.333333333);
+            double Ls2 = Ls * Ls;
+            double t = std::pow(SS, 2.5) + std::pow(SSd, 1.25);
+            muT[i] = 0.0;
+            if (t != 0.0) muT[i] = Rho[i] * Ls2 * std::pow(SSd, 1.5) / t;
+            kSGS[i] = Ci * std::pow(vol, 0.6666666666) * (0.5 * Sabs * Sabs);
+        } else if (model == 2) {
+            const double C_I = 0.007, C_R = 0.1;
+            double D2 = std::pow(vol, 0.666666666);
+            muT[i] = C_R * Rho[i] * D2 * Sabs;
+            kSGS[i] = C_I * D2 * (0.5 * Sabs * Sabs);
+        } else {
+            const double C_I = 0.007;
+            double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, M[3][3];
+            double r = Rho[i], u = U[i], v = V[i], w = W[i];
+            L[0][0] = r * u * u * vol; L[0][1] = r * u * v * vol; L[0][2] = r * u * w * vol;
+            L[1][1] = r * v * v * vol; L[1][2] = r * v * w * vol; L[2][2] = r * w * w * vol;
+            double sumVol = vol, sumRho = r * vol, sumV = u * vol, sumU = v * vol, sumW = w * vol;
+            double sumSabs = Sabs * vol;
+            double t062 = std::pow(vol, 0.66666666);
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    M[j][k] = -t062 * Sabs * S[j][k] * vol;
+                    S[j][k] *= vol;
+                }
+            for (int q = o.sptr[i]; q < o.sptr[i + 1]; ++q) {
+                int j = o.sidx[q];
+                double T[3][3];
+                double SSs = strain(j, T);
+                double vs = o.cvol[j];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) S[a][b] += T[a][b] * vs;
+                double Sabss = std::sqrt(2.0 * SSs);
+                double t06 = std::pow(vs, 0.66666666);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) M[a][b] -= t06 * Sabss * T[a][b] * vs;
+                sumSabs += Sabss * vs;
+                double rs = Rho[j], us = U[j], vv = V[j], ws = W[j];
+                L[0][0] += rs * us * us * vs; L[0][1] += rs * us * vv * vs; L[0][2] += rs * us * ws * vs;
+                L[1][1] += rs * vv * vv * vs; L[1][2] += rs * vv * ws * vs; L[2][2] += rs * ws * ws * vs;
+                sumRho += rs * vs; sumV += us * vs; sumU += vv * vs; sumW += ws * vs; sumVol += vs;
+            }
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) { M[j][k] /= sumVol; S[j][k] /= sumVol; L[j][k] /= sumVol; }
+            sumRho /= sumVol; sumV /= sumVol; sumU /= sumVol; sumW /= sumVol; sumSabs /= sumVol;
+            L[0][0] -= sumRho * sumU * sumU; L[0][1] -= sumRho * sumU * sumV; L[0][2] -= sumRho * sumU * sumW;
+            L[1][1] -= sumRho * sumV * sumV; L[1][2] -= sumRho * sumV * sumW; L[2][2] -= sumRho * sumW * sumW;
+            L[1][0] = L[0][1]; L[2][0] = L[0][2]; L[2][1] = L[1][2];
+            double sumDelta = std::pow(sumVol, 0.66666666666);
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) M[j][k] = sumDelta * sumSabs * S[j][k] - M[j][k];
+            double s1 = 0.0, s2 = 0.0;
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) { s1 += L[j][k] * M[j][k]; s2 += M[j][k] * M[j][k]; }
+            s1 /= 9.0; s2 /= 9.0;
+            double C_R = 0.0;
+            if (s2 != 0.0) C_R = -0.5 * s1 / s2;
+            double D2 = std::pow(vol, 0.666666666);
+            muT[i] = C_R * r * D2 * Sabs;
+            kSGS[i] = C_I * D2 * (0.5 * Sabs * Sabs);
+        }
+    }
+}
+
+// the turbulence part of calcCellTransport (transport.cpp:38-97) after the
+// mixture viscosity of cell_eos_transport: muT of the LES model, mu += muT.
+// first: the step's first call (pressureBased.cpp:17-19), when SEMO's faces
+// still hold the zero right values they were loaded with; else the right
+// values of the last reconstruction (after a VOF iteration, :115-116).
+void cell_les(Oracle& o, bool first) {
+    if (!o.les_model) return;
+    vector<double> z;
+    const double *RU, *RV, *RW;
+    if (first || o.RU.empty()) { z.assign(o.nf, 0.0); RU = RV = RW = z.data(); }
+    else { RU = o.RU.data(); RV = o.RV.data(); RW = o.RW.data(); }
+    o.muT.assign(o.nc, 0.0); o.kSGS.assign(o.nc, 0.0);
+    les(o, o.les_model, o.U.data(), o.V.data(), o.W.data(), o.Rho.data(), RU, RV, RW, o.muT.data(), o.kSGS.data());
+    for (int i = 0; i < o.nc; ++i) o.Mu[i] += o.muT[i];
+}
+
 // calcSourceSurfaceTension (source/surfaceTension.cpp:7-229): curvature, then
 // sigma kappa grad(alpha) with grad(alpha) = one Green-Gauss pass over the LSQ
 // gradient.  With sigma = 0 every use of kappa vanishes and the pipeline is
@@ -1436,7 +1560,7 @@ double time_step(Oracle& o, int adjust) {
 // momentum + pressure iterations.
 void pressure_based_step(Oracle& o, int ncorr, double* norms) {
     if (o.iterReal == 0) cell_sound_speed(o);
-    if (o.iterReal == 0) cell_eos_transport(o);
+    if (o.iterReal == 0) { cell_eos_transport(o); cell_les(o, true); }
     if (o.iterReal == 0) { o.Qm0 = o.U; o.Qm1 = o.V; o.Qm2 = o.W; o.Qm3 = o.VF; }
     else { o.Qm0 = o.oldU; o.Qm1 = o.oldV; o.Qm2 = o.oldW; o.Qm3 = o.oldVF; }
     o.oldP = o.P; o.oldU = o.U; o.oldV = o.V; o.oldW = o.W; o.oldVF = o.VF; o.oldRho = o.Rho;
@@ -1447,6 +1571,7 @@ void pressure_based_step(Oracle& o, int ncorr, double* norms) {
         solve_last(o, 1);
         o.vof_norm = vof_update(o);
         cell_eos_transport(o);
+        cell_les(o, false);
     }
     recon_faces(o);
     minmax(o, o.VF, o.maxVF, o.minVF);
@@ -1639,6 +1764,7 @@ void orc_set_case(void* h, const double* rho, const double* mu, double sigma, co
 
 // two-phase controls: vofVofURF and nSpeciesEq (VOF iterations per step)
 #ifndef ORC_AD
+void orc_set_les(void* h, int model) { ((Oracle*)h)->les_model = model; }
 void orc_set_vof(void* h, double urf_vf, int n_vof) {
     Oracle& o = *(Oracle*)h;
     o.urf_vf = urf_vf; o.n_vof = n_vof;
@@ -1661,7 +1787,7 @@ void orc_set_solver(void* h, double tol, int maxit) {
 static vector<double>* field(Oracle& o, const char* nm) {
     std::string s(nm);
 #define F(name, var) if (s == name) return &o.var;
-    F("P", P) F("U", U) F("V", V) F("W", W) F("VF", VF) F("Rho", Rho) F("mu", Mu)
+    F("P", P) F("U", U) F("V", V) F("W", W) F("VF", VF) F("Rho", Rho) F("mu", Mu) F("muT", muT) F("kSGS", kSGS)
     F("oldU", oldU) F("oldV", oldV) F("oldW", oldW) F("Qm0", Qm0) F("Qm1", Qm1) F("Qm2", Qm2)
     F("gradP", gP) F("gradU", gU) F("gradV", gV) F("gradW", gW) F("gradVF", gVF)
     F("maxP", maxP) F("minP", minP) F("maxU", maxU) F("minU", minU) F("maxV", maxV) F("minV", minV)
@@ -1733,6 +1859,10 @@ void orc_minmax(void* h, const double* phi, double* mx, double* mn) {
 }
 void orc_gauss_green(void* h, int sel, const double* phi, const double* faceR, double* grad) {
     gauss_green(*(Oracle*)h, sel, phi, faceR, grad);
+}
+void orc_les(void* h, int model, const double* U, const double* V, const double* W, const double* Rho,
+             const double* RU, const double* RV, const double* RW, double* muT, double* kSGS) {
+    les(*(Oracle*)h, model, U, V, W, Rho, RU, RV, RW, muT, kSGS);
 }
 void orc_assemble_momentum(void* h) { assemble_momentum(*(Oracle*)h); }
 void orc_assemble_pressure(void* h) { assemble_pressure(*(Oracle*)h); }

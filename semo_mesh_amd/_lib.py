@@ -23,7 +23,7 @@ STATUS = {0: "SEMO_OK", 1: "SEMO_EINVAL", 2: "SEMO_EINVAL_BC", 3: "SEMO_ESINGULA
 F = dict(P=0, U=1, V=2, W=3, VF=4, Rho=5, mu=6, oldU=7, oldV=8, oldW=9, Qm0=10, Qm1=11, Qm2=12,
          srcG0=13, srcG1=14, srcG2=15, maxP=16, minP=17, maxU=18, minU=19, maxV=20, minV=21,
          maxW=22, minW=23, alphaP=24, Qm3=25, oldVF=26, kappa=27, srcS0=28, srcS1=29, srcS2=30, LS=31, gradP=32, gradU=33, gradV=34, gradW=35, gradVF=36,
-         C=40,
+         C=40, muT=41, kSGS=42,
          x0=48, x1=49, x2=50, b0=56, b1=57, b2=58,
          cell_volume=64, cell_xyz=65, cell_lsq=66,
          face_area=80, face_normal=81, face_xyz=82, face_wC=83, face_magPN=84, face_alphaF=85,
@@ -74,7 +74,7 @@ EXPORTED = [
     "semo_incompressible_pressure_based", "semo_adjoint_alpha_p", "semo_objective_pressure",
     "semo_volume_integral",
     "semo_calc_coupled_eq", "semo_coupled_pressure_based", "semo_coupled_matrix_get", "semo_coupled_spmv", "semo_coupled_solve",
-    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_least_square_noalloc", "semo_cell_min_max", "semo_gauss_green", "semo_solve_coo",
+    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_least_square_noalloc", "semo_cell_min_max", "semo_gauss_green", "semo_les", "semo_set_les", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry", "semo_stream_copy",
 ]
 
@@ -126,6 +126,8 @@ def lib():
         "semo_least_square_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "semo_cell_min_max": (C.c_int, [vp, dp, dp, dp]),
         "semo_gauss_green": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+        "semo_les": (C.c_int, [vp, C.c_int] + [dp] * 9),
+        "semo_set_les": (C.c_int, [vp, C.c_int]),
         "semo_solve_coo": (C.c_int, [vp, C.c_char_p, i32, i64, vp, vp, vp, i32, dp, dp,
                                      C.POINTER(SolveStats)]),
         "semo_profile_enable": (C.c_int, [vp, C.c_int]),

@@ -137,6 +137,17 @@ void set_case(SEMO_Mesh_Builder& mesh, SEMO_Controls_Builder& controls, vector<S
         function = function || p.vf_type == SEMO_BC_FUNCTION;
     }
     check(semo_set_case(g_ctx, &cs, bcs.data()));
+    // turbulence (transport.cpp:38-72): LES models by name, laminar / none
+    int les = 0;
+    if (controls.turbType == "LES") {
+        if (controls.turbLESModel == "WALE") les = SEMO_LES_WALE;
+        else if (controls.turbLESModel == "smagorinsky") les = SEMO_LES_SMAGORINSKY;
+        else if (controls.turbLESModel == "dynamicSmagorinsky") les = SEMO_LES_DYNAMIC_SMAGORINSKY;
+        else fail("not defined LES model");
+    } else if (controls.turbType != "laminar" && controls.turbType != "none") {
+        fail(controls.turbType == "RANS" ? "not yet defined RANS model" : "not defined turb type");
+    }
+    check(semo_set_les(g_ctx, les));
     if (function) {
         vector<double> pts;
         vf_points(pts);

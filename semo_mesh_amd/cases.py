@@ -58,6 +58,10 @@ class Case:
     # points (VF, x, y, z) of the volume-fraction "function" boundary: the
     # rows of SEMO's ./VF.csv, columns 3-6 (reconIncom.cpp:929-1019)
     vf_points: np.ndarray | None = None
+    # turbType LES with this turbLESModel (constant/turbulenceProperties,
+    # controls/build.cpp:100-115): "WALE", "smagorinsky" or
+    # "dynamicSmagorinsky"; None: laminar
+    les_model: str | None = None
 
     def write_vf_csv(self, path: str) -> None:
         """./VF.csv as reconIncomZeroOrder reads it: a header line, then
@@ -80,6 +84,8 @@ class Case:
             fh.write(f"limit {r(self.limit)}\nsteps {int(steps)}\ncorrectors {int(correctors)}\n")
             if self.n_vof:
                 fh.write(f"vof {int(self.n_vof)}\nurfvf {r(self.urf_vf)}\n")
+            if self.les_model:
+                fh.write(f"turb {self.les_model}\n")
             if self.adjust_dt:
                 fh.write(f"adjustdt 1 {r(self.max_co)} {r(self.max_dt)}\n")
             if init:

@@ -1362,6 +1362,134 @@ __global__ void __launch_bounds__(TPB) k_gauss_green(
     for (int a = 0; a < 3; ++a) gout[3 * (size_t)i + a] = g[a];
 }
 
+// y += x
+__global__ void __launch_bounds__(TPB) k_add_to(int n, const double* __restrict__ x, double* __restrict__ y) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+// ----------------------------------------------------------------- LES --
+// SEMO_TurbModel_LES (turbulenceModels/LES.cpp) from the Green-Gauss
+// gradients gU, gV, gW [3n] (two passes from zero, LES.cpp:46-51): the
+// subgrid viscosity muT and kinetic energy kSGS.  MODEL 1: calcWALE
+// (:29-131), 2: calcSmagorinsky (:141-220).  The reference's arithmetic and
+// constants as written, including WALE's SSd, which adds the squares of S's
+// diagonal to those of Sd's off-diagonals (:110-112).
+__device__ inline double les_strain(const double* __restrict__ gU, const double* __restrict__ gV,
+                                    const double* __restrict__ gW, size_t i, double (&S)[3][3]) {
+    const double* a = gU + 3 * i; const double* b = gV + 3 * i; const double* c = gW + 3 * i;
+    S[0][0] = a[0]; S[0][1] = 0.5 * (a[1] + b[0]); S[0][2] = 0.5 * (a[2] + c[0]);
+    S[1][0] = S[0][1]; S[1][1] = b[1]; S[1][2] = 0.5 * (b[2] + c[1]);
+    S[2][0] = S[0][2]; S[2][1] = S[1][2]; S[2][2] = c[2];
+    return S[0][0] * S[0][0] + S[1][1] * S[1][1] + S[2][2] * S[2][2] +
+           2.0 * (S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][2] * S[1][2]);
+}
+template <int MODEL>
+__global__ void __launch_bounds__(TPB) k_les_local(int nc, const double* __restrict__ gU,
+                                                   const double* __restrict__ gV, const double* __restrict__ gW,
+                                                   const double* __restrict__ vol, const double* __restrict__ Rho,
+                                                   double* __restrict__ muT, double* __restrict__ kSGS) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double S[3][3];
+    double SS = les_strain(gU, gV, gW, i, S);
+    double Sabs = sqrt(2.0 * SS);
+    double v = vol[i];
+    if constexpr (MODEL == 1) {
+        const double Ci = 0.007, Cw = 0.325;
+        const double* a = gU + 3 * (size_t)i; const double* b = gV + 3 * (size_t)i;
+        const double* c = gW + 3 * (size_t)i;
+        double Sd01 = 0.5 * (a[1] * a[1] + b[0] * b[0]);
+        double Sd02 = 0.5 * (a[2] * a[2] + c[0] * c[0]);
+        double Sd12 = 0.5 * (b[2] * b[2] + c[1] * c[1]);
+        double SSd = S[0][0] * S[0][0] + S[1][1] * S[1][1] + S[2][2] * S[2][2] +
+                     2.0 * (Sd01 * Sd01 + Sd02 * Sd02 + Sd12 * Sd12);
+        double Ls = Cw * pow(v, 0.333333333);
+        double Ls2 = Ls * Ls;
+        double t = pow(SS, 2.5) + pow(SSd, 1.25);
+        muT[i] = t != 0.0 ? Rho[i] * Ls2 * pow(SSd, 1.5) / t : 0.0;
+        kSGS[i] = Ci * pow(v, 0.6666666666) * (0.5 * Sabs * Sabs);
+    } else {
+        const double C_I = 0.007, C_R = 0.1;
+        double D2 = pow(v, 0.666666666);
+        muT[i] = C_R * Rho[i] * D2 * Sabs;
+        kSGS[i] = C_I * D2 * (0.5 * Sabs * Sabs);
+    }
+}
+// calcDynamicSmagorinsky (LES.cpp:224-467): box-filtered (cell plus its
+// vertex stencil, volume-weighted) S, |S| S, rho u_i u_j, rho and u give the
+// Germano identity's L and M and C_R = -1/2 <L M> / <M M>.  Kept as written:
+// the running sums named sumV and sumU collect U and V (:330-331, :415-416).
+__global__ void __launch_bounds__(TPB) k_les_dynamic(
+    int nc, const int* __restrict__ st_ptr, const int* __restrict__ st_idx, const double* __restrict__ gU,
+    const double* __restrict__ gV, const double* __restrict__ gW, const double* __restrict__ vol,
+    const double* __restrict__ Rho, const double* __restrict__ U, const double* __restrict__ V,
+    const double* __restrict__ W, double* __restrict__ muT, double* __restrict__ kSGS) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const double C_I = 0.007;
+    double S[3][3], M[3][3];
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double SS = les_strain(gU, gV, gW, i, S);
+    double Sabs = sqrt(2.0 * SS);
+    double v = vol[i], r = Rho[i], u = U[i], vv = V[i], w = W[i];
+    L[0][0] = r * u * u * v; L[0][1] = r * u * vv * v; L[0][2] = r * u * w * v;
+    L[1][1] = r * vv * vv * v; L[1][2] = r * vv * w * v; L[2][2] = r * w * w * v;
+    double sumVol = v, sumRho = r * v, sumV = u * v, sumU = vv * v, sumW = w * v, sumSabs = Sabs * v;
+    double t062 = pow(v, 0.66666666);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            M[j][k] = -t062 * Sabs * S[j][k] * v;
+            S[j][k] *= v;
+        }
+    for (int q = st_ptr[i]; q < st_ptr[i + 1]; ++q) {
+        int j = st_idx[q];
+        double T[3][3];
+        double SSs = les_strain(gU, gV, gW, j, T);
+        double vs = vol[j];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) S[a][b] += T[a][b] * vs;
+        double Sabss = sqrt(2.0 * SSs);
+        double t06 = pow(vs, 0.66666666);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) M[a][b] -= t06 * Sabss * T[a][b] * vs;
+        sumSabs += Sabss * vs;
+        double rs = Rho[j], us = U[j], vs2 = V[j], ws = W[j];
+        L[0][0] += rs * us * us * vs; L[0][1] += rs * us * vs2 * vs; L[0][2] += rs * us * ws * vs;
+        L[1][1] += rs * vs2 * vs2 * vs; L[1][2] += rs * vs2 * ws * vs; L[2][2] += rs * ws * ws * vs;
+        sumRho += rs * vs; sumV += us * vs; sumU += vs2 * vs; sumW += ws * vs; sumVol += vs;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { M[j][k] /= sumVol; S[j][k] /= sumVol; L[j][k] /= sumVol; }
+    sumRho /= sumVol; sumV /= sumVol; sumU /= sumVol; sumW /= sumVol; sumSabs /= sumVol;
+    L[0][0] -= sumRho * sumU * sumU; L[0][1] -= sumRho * sumU * sumV; L[0][2] -= sumRho * sumU * sumW;
+    L[1][1] -= sumRho * sumV * sumV; L[1][2] -= sumRho * sumV * sumW; L[2][2] -= sumRho * sumW * sumW;
+    L[1][0] = L[0][1]; L[2][0] = L[0][2]; L[2][1] = L[1][2];
+    double sumDelta = pow(sumVol, 0.66666666666);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) M[j][k] = sumDelta * sumSabs * S[j][k] - M[j][k];
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { s1 += L[j][k] * M[j][k]; s2 += M[j][k] * M[j][k]; }
+    s1 /= 9.0; s2 /= 9.0;
+    double C_R = s2 != 0.0 ? -0.5 * s1 / s2 : 0.0;
+    double D2 = pow(v, 0.666666666);
+    muT[i] = C_R * r * D2 * Sabs;
+    kSGS[i] = C_I * D2 * (0.5 * Sabs * Sabs);
+}
+
 __device__ inline double clipv(double v, double lim) {
     if (v <= -lim) v = -lim;
     if (v >= lim) v = lim;
@@ -1483,9 +1611,24 @@ void fields_alloc(semo_ctx* c) {
     SEMO_HIP(hipStreamSynchronize(c->stream));   // host sources above are temporaries
 }
 
-void eos_transport(semo_ctx* c) {
+// calcIncomCellEOSVF + calcCellTransport (eos.cpp:10-151, transport.cpp:
+// 9-97): rho, the clipped volume fraction and mu of the mixture; with an LES
+// model, then muT from the velocity gradients (the boundary faces' right
+// values bR [3][nbf], null: zero, as before SEMO's first reconstruction)
+// and mu += muT
+void eos_transport(semo_ctx* c, const double* bR) {
     CaseK k = case_k(c);
     SEMO_LAUNCH(c, "eos", 32.0 * c->nc, k_eos, dim3(nblk(c->nc)), dim3(TPB), c->nc, k, c->VF.p, c->Rho.p, c->Mu.p);
+    if (!c->les_model) return;
+    size_t n = c->nc;
+    c->muT.ensure(n); c->kSGS.ensure(n);
+    if (!bR) {
+        size_t nb = 3 * (size_t)std::max(c->nbf, 1);
+        if (c->les_zero.n != nb) { c->les_zero.alloc(nb); zero_d(c, c->les_zero.p, c->les_zero.bytes()); }
+        bR = c->les_zero.p;
+    }
+    les_device(c, c->les_model, c->U.p, c->V.p, c->W.p, c->Rho.p, bR, c->muT.p, c->kSGS.p);
+    SEMO_LAUNCH(c, "les_add", 24.0 * n, k_add_to, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->muT.p, c->Mu.p);
 }
 
 // cell.var[C] as calcCellEOSVF leaves it at load
@@ -1515,6 +1658,36 @@ void gauss_green_standalone(semo_ctx* c, int sel, const double* phi, const doubl
                 k_gauss_green, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, sel, c->d_cf_sl.p, c->d_cf_ell.p,
                 c->d_owner.p, c->d_nbr.p, c->d_bpatch.p, c->d_bcs.p, c->f_nrm.p, c->f_area.p, c->f_wC.p, c->f_skew.p,
                 c->f_xyz.p, c->c_xyz.p, c->c_vol.p, phi, faceR_b, gin, gout);
+}
+
+// SEMO_TurbModel_LES on device arrays (device numbering): the Green-Gauss
+// gradients of U, V, W (two passes from zero, the boundary faces' right
+// values bR [3][nbf] in device face order), then model 1 (WALE),
+// 2 (Smagorinsky) or 3 (dynamic Smagorinsky) into muT, kSGS [n]
+void les_device(semo_ctx* c, int model, const double* U, const double* V, const double* W, const double* Rho,
+                const double* bR, double* muT, double* kSGS) {
+    SEMO_CHECK(model >= 1 && model <= 3, SEMO_EINVAL, "LES model must be 1 (WALE), 2 (Smagorinsky) or 3 (dynamic)");
+    size_t n = c->nc, nb = c->nbf;
+    c->les_g.ensure(12 * n);
+    double* g = c->les_g.p;              // [gU, gV, gW, scratch] x 3n
+    const double* phi[3] = {U, V, W};
+    for (int k = 0; k < 3; ++k) {
+        double* gk = g + 3 * n * k;
+        double* t = g + 9 * n;
+        zero_d(c, t, 3 * n * sizeof(double));
+        gauss_green_standalone(c, 1 + k, phi[k], bR + nb * k, t, gk);
+        gauss_green_standalone(c, 1 + k, phi[k], bR + nb * k, gk, t);
+        copy_d(c, gk, t, 3 * n);
+    }
+    const double *gU = g, *gV = g + 3 * n, *gW = g + 6 * n;
+    dim3 gb(nblk(c->nc)), tb(TPB);
+    if (model == 1)
+        SEMO_LAUNCH(c, "les", 104.0 * n, k_les_local<1>, gb, tb, c->nc, gU, gV, gW, c->c_vol.p, Rho, muT, kSGS);
+    else if (model == 2)
+        SEMO_LAUNCH(c, "les", 104.0 * n, k_les_local<2>, gb, tb, c->nc, gU, gV, gW, c->c_vol.p, Rho, muT, kSGS);
+    else
+        SEMO_LAUNCH(c, "les", 4.0 * (double)c->d_st_idx.n + 4.0 * (n + 1) + 136.0 * n, k_les_dynamic, gb, tb,
+                    c->nc, c->d_st_ptr.p, c->d_st_idx.p, gU, gV, gW, c->c_vol.p, Rho, U, V, W, muT, kSGS);
 }
 
 void vf_function_values(semo_ctx* c, int n, const double* pts4) {

@@ -96,6 +96,7 @@ DBuf<double>* cell_field(semo_ctx* c, int f, int& comps) {
         case SEMO_F_QM3: return &c->Qm3; case SEMO_F_OLDVF: return &c->oldVF;
         case SEMO_F_KAPPA: return &c->kappa; case SEMO_F_LS: return &c->ls;
         case SEMO_F_C: return &c->Csnd;
+        case SEMO_F_MUT: return &c->muT; case SEMO_F_KSGS: return &c->kSGS;
         case SEMO_F_GRADP: comps = 3; return &c->gP; case SEMO_F_GRADU: comps = 3; return &c->gU;
         case SEMO_F_GRADV: comps = 3; return &c->gV; case SEMO_F_GRADW: comps = 3; return &c->gW;
         case SEMO_F_GRADVF: comps = 3; return &c->gVF;
@@ -125,7 +126,7 @@ void step_setup(semo_ctx* c) {
     // incompressible/pressureBased.cpp:79), then pressureBased.cpp:17-69:
     // first-step EOS/transport, time levels
     if (c->iterReal == 0 && !c->csnd_user) sound_speed(c);
-    if (c->iterReal == 0) eos_transport(c);
+    if (c->iterReal == 0) eos_transport(c, nullptr);
     const DBuf<double>& q0 = c->iterReal == 0 ? c->U : c->oldU;
     const DBuf<double>& q1 = c->iterReal == 0 ? c->V : c->oldV;
     const DBuf<double>& q2 = c->iterReal == 0 ? c->W : c->oldW;
@@ -151,7 +152,7 @@ double volfrac_eq(semo_ctx* c, semo_solve_stats* st) {
     assemble_vof(c);
     solve_momentum(c, 1, st);
     double nv = vof_update(c);
-    eos_transport(c);
+    eos_transport(c, c->bUR.p);
     return nv;
 }
 
@@ -382,7 +383,8 @@ int semo_field_set(semo_ctx* c, int f, const double* h, int64_t n) {
             SEMO_HIP(hipStreamSynchronize(c->stream));
             return;
         }
-        SEMO_CHECK(d && f < SEMO_G_CELL_VOLUME, SEMO_EINVAL, "field not settable");
+        SEMO_CHECK(d && f < SEMO_G_CELL_VOLUME && f != SEMO_F_MUT && f != SEMO_F_KSGS, SEMO_EINVAL,
+                   "field not settable");
         SEMO_CHECK(n == (int64_t)c->nc * comps, SEMO_EINVAL, "field size");
         to_dev(c, *d, h, comps);
         SEMO_HIP(hipStreamSynchronize(c->stream));
@@ -764,6 +766,37 @@ int semo_gauss_green(semo_ctx* c, int field, const double* phi, const double* fa
         gout.alloc(3 * (size_t)c->nc);
         gauss_green_standalone(c, sel, dphi.p, dfr.p, gin.p, gout.p);
         from_dev(c, gout.p, grad, 3);
+    });
+}
+
+int semo_set_les(semo_ctx* c, int model) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        SEMO_CHECK(model >= 0 && model <= SEMO_LES_DYNAMIC_SMAGORINSKY, SEMO_EINVAL,
+                   "LES model must be 0 (laminar) or an SEMO_LES_* model");
+        c->les_model = model;
+    });
+}
+
+int semo_les(semo_ctx* c, int model, const double* U, const double* V, const double* W, const double* rho,
+             const double* face_rU, const double* face_rV, const double* face_rW, double* muT, double* kSGS) {
+    if (!c || !U || !V || !W || !rho || !face_rU || !face_rV || !face_rW || !muT || !kSGS) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_case(c);
+        SEMO_CHECK(model >= SEMO_LES_WALE && model <= SEMO_LES_DYNAMIC_SMAGORINSKY, SEMO_EINVAL,
+                   "LES model must be SEMO_LES_WALE, SEMO_LES_SMAGORINSKY or SEMO_LES_DYNAMIC_SMAGORINSKY");
+        DBuf<double> dU, dV, dW, dR, dbr, dm, dk;
+        to_dev(c, dU, U, 1); to_dev(c, dV, V, 1); to_dev(c, dW, W, 1); to_dev(c, dR, rho, 1);
+        size_t nb = (size_t)std::max(c->nbf, 1);
+        std::vector<double> br(3 * nb, 0.0);
+        const double* fr[3] = {face_rU, face_rV, face_rW};
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < c->nbf; ++j) br[k * nb + j] = fr[k][c->fperm[c->nif + j]];
+        dbr.upload(br, c->stream);
+        dm.alloc(c->nc); dk.alloc(c->nc);
+        les_device(c, model, dU.p, dV.p, dW.p, dR.p, dbr.p, dm.p, dk.p);
+        from_dev(c, dm.p, muT, 1);
+        from_dev(c, dk.p, kSGS, 1);
     });
 }
 

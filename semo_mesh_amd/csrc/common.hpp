@@ -248,6 +248,12 @@ struct semo_ctx {
     // packed gather records of the one-field LSQ ([x, y, z, phi]) and of the
     // curvature divergence ([x, y, z, n, -, -])
     semo::DBuf<double> xph, xn;
+    // LES: the velocity gradients and a scratch gradient [4][3n]; the model
+    // calcCellTransport applies (0: laminar), its muT and kSGS [n], zero
+    // boundary right values [3][nbf] (the first step's)
+    semo::DBuf<double> les_g;
+    int les_model = 0;
+    semo::DBuf<double> muT, kSGS, les_zero;
     std::vector<double> h_ufix;     // as set, in the caller's numbering
     // per-face values of the volume-fraction "function" boundary [nbf]
     // (device face order), empty until semo_set_vf_function_points
@@ -429,7 +435,7 @@ SellPattern sell_from_csr(int n, int ncols, const std::vector<int>& ptr, const s
 void geometry_compute(semo_ctx* c);
 // assembly.hip
 void fields_alloc(semo_ctx* c);
-void eos_transport(semo_ctx* c);
+void eos_transport(semo_ctx* c, const double* bR);
 void recon_boundary(semo_ctx* c);
 void sound_speed(semo_ctx* c);
 // nearest-point VF of every boundary face for the "function" boundary
@@ -438,6 +444,8 @@ void vf_function_values(semo_ctx* c, int n, const double* pts4_host);
 // types and values); faceR_b: right face values of the boundary faces [nbf]
 void gauss_green_standalone(semo_ctx* c, int sel, const double* phi, const double* faceR_b, const double* gin,
                             double* gout);
+void les_device(semo_ctx* c, int model, const double* U, const double* V, const double* W, const double* Rho,
+                const double* bR, double* muT, double* kSGS);
 void source_gravity(semo_ctx* c);
 // uvw_only: gradients of U, V, W only; P and VF gradients are kept from the
 // last full pass (valid when P, VF and their boundary values are unchanged)

@@ -78,6 +78,8 @@ class GpuSolver:
         if pts is not None:
             a = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 4)
             self._ok(self._L.semo_set_vf_function_points(self._h, a.shape[0], _ptr(a)))
+        les = getattr(case, "les_model", None)
+        self._ok(self._L.semo_set_les(self._h, self.LES_MODELS[les] if les else 0))
 
     def set_solver(self, precision="fp64", tol_p=1e-8, tol_u=1e-8, maxit_p=500, maxit_u=500,
                    cheb_degree=1, coarse_size=1000):
@@ -292,6 +294,20 @@ class GpuSolver:
         g = np.ascontiguousarray(gradient, dtype=np.float64).reshape(-1).copy()
         self._ok(self._L.semo_gauss_green(self._h, F[field], _ptr(phi), _ptr(fr), _ptr(g)))
         return g.reshape(-1, 3)
+
+    LES_MODELS = {"WALE": 1, "smagorinsky": 2, "dynamicSmagorinsky": 3}
+
+    def calcLES(self, model, U, V, W, rho, face_rU, face_rV, face_rW):
+        """SEMO_TurbModel_LES (turbulenceModels/LES.cpp) by its
+        turbulenceProperties name (transport.cpp:42-50): "WALE",
+        "smagorinsky" or "dynamicSmagorinsky".  U, V, W, rho cell values,
+        face_r* the right face values [n_faces]; returns (muT, kSGS)."""
+        if model not in self.LES_MODELS:
+            raise ValueError(f"LES model {model!r}: not one of {sorted(self.LES_MODELS)}")
+        arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (U, V, W, rho, face_rU, face_rV, face_rW)]
+        mu = np.empty(self.mesh.n_cells); k = np.empty(self.mesh.n_cells)
+        self._ok(self._L.semo_les(self._h, self.LES_MODELS[model], *[_ptr(a) for a in arr], _ptr(mu), _ptr(k)))
+        return mu, k
 
     def setCellVarMinMax(self, phi):
         phi = np.ascontiguousarray(phi, dtype=np.float64)

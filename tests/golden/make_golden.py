@@ -33,7 +33,8 @@ REF = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
 VOF_ONLY = ("_faceLVF", "_faceRVF", "_Qm3", "_oldVF", "_dt")
 
 
-def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
+def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True, extra=()):
+    """extra: more "key value" lines for the driver's case file."""
     with tempfile.TemporaryDirectory() as d:
         mesh.write_raw(os.path.join(d, "mesh.raw"))
         ip = None
@@ -42,6 +43,9 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
             np.ascontiguousarray(init, dtype=np.float64).tofile(ip)
         case.write_ref_case(os.path.join(d, "case.txt"), steps=steps,
                             correctors=correctors, init=ip)
+        if extra:
+            with open(os.path.join(d, "case.txt"), "a") as fh:
+                fh.write("".join(ln + "\n" for ln in extra))
         if case.vf_points is not None:        # read as ./VF.csv (reconIncom.cpp:931)
             case.write_vf_csv(os.path.join(d, "VF.csv"))
         out = os.path.join(d, "out")
@@ -56,7 +60,7 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
                 continue
             if base.startswith("call") and not keep_calls:
                 continue
-            if base.startswith("gg_") and not name.startswith("bc_"):
+            if base.startswith("gg_") and not name.startswith(("bc_", "les_")):
                 continue
             if base.endswith("_gradVF"):
                 continue
@@ -83,6 +87,8 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True):
         case_urf=np.array([case.urf_mom, case.urf_p, case.urf_pu])))
     if case.vf_points is not None:
         arrs["case_vf_points"] = np.asarray(case.vf_points, dtype=np.float64)
+    if case.les_model:
+        arrs["case_les_model"] = np.array(case.les_model)
     if init is not None:
         arrs["init"] = np.asarray(init)
     path = os.path.join(HERE, name + ".npz")
@@ -216,6 +222,48 @@ def _bc_vof():
     run_case("bc_vof_box654_2x1", m, c, 2, 1, init)
 
 
+def _les():
+    """SEMO_TurbModel_LES (turbulenceModels/LES.cpp): WALE, Smagorinsky and
+    dynamic Smagorinsky on the final state of (a) the rotated open box of
+    bc_open (slip, surfaceNormalFixedValue, inletOutlet and zeroGradient U
+    patches, so that the Green-Gauss gradients take every boundary branch)
+    and (b) a two-phase jittered tet box (density varying from cell to
+    cell, vertex stencils of up to about 70 cells for the dynamic model)."""
+    m = rotated(hex_mesh(6, 5, 4, jitter=0.1, seed=41), seed=42, shift=(3.0, 0.0, 0.0))
+    c = cavity_case(); c.rho = (1.0, 1.0); c.mu = (0.05, 0.05); c.dt = 0.02
+    c.bcs["xmin"] = PatchBC(u_type=BC_INLET_OUTLET, u_value=(-0.8, 0.0, 0.0))
+    c.bcs["xmax"] = PatchBC(p_value=0.5, u_type=BC_ZERO_GRADIENT)
+    c.bcs["ymin"] = PatchBC(u_type=BC_SLIP)
+    c.bcs["ymax"] = PatchBC(u_type=BC_SURFACE_NORMAL_FIXED_VALUE, u_value=(0.3, 0.0, 0.0))
+    run_case("les_open_box654", m, c, 2, 1, random_init(m.n_cells, 51), keep_calls=False, extra=("les 1",))
+    m = tet_mesh(4, 3, 3, hi=(1.0, 0.75, 0.75), jitter=0.1, seed=52)
+    c = dam_break_case(dt=1e-3, n_vof=1, urf_vf=1.0)
+    run_case("les_vof_tet433", m, c, 1, 1, vof_init(m, 53), keep_calls=False, extra=("les 1",))
+
+
+def _les_runs():
+    """Whole runs with turbType LES (calcCellTransport applies the model at
+    the first step and after every VOF iteration, transport.cpp:38-97):
+    WALE on a single-phase jittered box (2 steps x 2 correctors: the
+    first-step muT, from zero face values, stays in mu), Smagorinsky on the
+    open box, dynamic Smagorinsky on a two-phase tet box (muT recomputed
+    after each VOF iteration from the reconstructed face values)."""
+    m = hex_mesh(6, 5, 4, jitter=0.1, seed=7)
+    c = cavity_case(); c.mu = (0.05, 0.05); c.dt = 0.02; c.les_model = "WALE"
+    run_case("lesrun_wale_box654_2x2", m, c, 2, 2, random_init(m.n_cells, 61), keep_calls=False)
+    m = rotated(hex_mesh(6, 5, 4, jitter=0.1, seed=41), seed=42, shift=(3.0, 0.0, 0.0))
+    c = cavity_case(); c.rho = (1.0, 1.0); c.mu = (0.05, 0.05); c.dt = 0.02
+    c.bcs["xmin"] = PatchBC(u_type=BC_INLET_OUTLET, u_value=(-0.8, 0.0, 0.0))
+    c.bcs["xmax"] = PatchBC(p_value=0.5, u_type=BC_ZERO_GRADIENT)
+    c.bcs["ymin"] = PatchBC(u_type=BC_SLIP)
+    c.bcs["ymax"] = PatchBC(u_type=BC_SURFACE_NORMAL_FIXED_VALUE, u_value=(0.3, 0.0, 0.0))
+    c.les_model = "smagorinsky"
+    run_case("lesrun_smag_open_box654_2x1", m, c, 2, 1, random_init(m.n_cells, 62), keep_calls=False)
+    m = tet_mesh(4, 3, 3, hi=(1.0, 0.75, 0.75), jitter=0.1, seed=8)
+    c = dam_break_case(dt=1e-3, n_vof=2, urf_vf=1.0); c.les_model = "dynamicSmagorinsky"
+    run_case("lesrun_dyn_vof_tet433_2x1", m, c, 2, 1, vof_init(m, 63), keep_calls=False)
+
+
 def _poly20():
     """Three stacked prisms over a jittered 20-gon: internal faces and caps
     of 20 points (above the 16 the device keeps in registers), lid-driven
@@ -346,7 +394,8 @@ CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "cavity8_plain": _cavity8, "orifice_lid": _orifice, "vof_box654_2x2": _vof_box654,
          "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box,
          "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof,
-         "poly20_stack3": _poly20, "vtu_tet221": _vtu, "initial_cube6": _initial}
+         "poly20_stack3": _poly20, "vtu_tet221": _vtu, "initial_cube6": _initial, "les": _les,
+         "lesrun": _les_runs}
 
 
 def main(names):
