@@ -144,6 +144,16 @@ typedef struct {
     int32_t coarse_size;    /* coarsest AMG level size (dense direct solve)  */
 } semo_solver_params;
 int semo_set_solver(semo_ctx* ctx, const semo_solver_params* p);
+/* Preconditioner of the momentum and volume-fraction BiCGStab solves.  SEMO
+ * solves them with aggregation-AMG-preconditioned Krylov methods (AMGCL,
+ * src/solvers/solveAMGCL.cpp:881-927, 1195-1258).  JACOBI (the default):
+ * diagonal, enough at the time steps of its cases (2 iterations to 1e-8 on
+ * the 128^3 cavity); AMG: one V-cycle of a smoothed-aggregation hierarchy of
+ * the system itself per component (FP32 with SEMO_PREC_MIXED), built on the
+ * host and refreshed on the device like the pressure's; AUTO: Jacobi until a
+ * solve of that system needs more than 30 iterations, AMG from then on.    */
+enum { SEMO_MSOLVER_JACOBI = 0, SEMO_MSOLVER_AMG = 1, SEMO_MSOLVER_AUTO = 2 };
+int semo_set_momentum_solver(semo_ctx* ctx, int mode);
 
 typedef struct {
     int32_t iters[4];       /* per right-hand side                           */

@@ -769,6 +769,16 @@ int semo_gauss_green(semo_ctx* c, int field, const double* phi, const double* fa
     });
 }
 
+int semo_set_momentum_solver(semo_ctx* c, int mode) {
+    if (!c) return SEMO_EINVAL;
+    return guard(c, [&] {
+        SEMO_CHECK(mode >= SEMO_MSOLVER_JACOBI && mode <= SEMO_MSOLVER_AUTO, SEMO_EINVAL,
+                   "momentum solver must be SEMO_MSOLVER_JACOBI, _AMG or _AUTO");
+        c->msolver = mode;
+        if (mode != SEMO_MSOLVER_AUTO) c->amg_mom_on = c->amg_vf_on = false;
+    });
+}
+
 int semo_set_les(semo_ctx* c, int model) {
     if (!c) return SEMO_EINVAL;
     return guard(c, [&] {

@@ -324,6 +324,13 @@ struct semo_ctx {
     semo::Amg amg_p;
     semo::DBuf<double> amg_key_val, amg_key_diag;   // values the pressure hierarchy was built from
     semo::DBuf<double> dinv_mom;
+    // momentum / volume-fraction preconditioner (SEMO_MSOLVER_*): their AMG
+    // hierarchies, the values each was built from, whether AUTO switched to
+    // them, and the staging vectors of their V-cycles
+    int msolver = 0;
+    semo::Amg amg_m, amg_vf;
+    semo::DBuf<double> amg_m_kv, amg_m_kd, amg_vf_kv, amg_vf_kd, mom_stage;
+    bool amg_mom_on = false, amg_vf_on = false;
     semo::DBuf<int> d_flag1;
     semo::DBuf<double> kv64;                   // Krylov work (fp64)
     semo::DBuf<float> kv32;

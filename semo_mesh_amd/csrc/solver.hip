@@ -1029,8 +1029,21 @@ int fpcg(semo_ctx* c, Amg& amg, const DevSell<double>& A, const double* b, doubl
 }
 
 // right-preconditioned (Jacobi) BiCGStab, x0 = 0
+// z = M^-1 r for one component of a momentum / volume-fraction solve: one
+// V-cycle of `amg` in the solve's precision, through fixed staging buffers,
+// so that one captured V-cycle graph serves every component and call
+void stage_vcycle(semo_ctx* c, Amg& amg, const double* r, double* z) {
+    size_t n = c->nc;
+    double* sr = c->mom_stage.p;
+    double* sz = sr + n + 64;
+    copy_d(c, sr, r, n);
+    if (c->sp.precision == SEMO_PREC_MIXED) precondition<float>(c, amg, sr, sz, false);
+    else precondition<double>(c, amg, sr, sz, false);
+    copy_d(c, z, sz, n);
+}
+
 int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const double* b, double* x, double tol,
-             int maxit, double* relres) {
+             int maxit, double* relres, Amg* pre = nullptr) {
     SEMO_CHECK(A.lanes == 0, SEMO_EINVAL, "Krylov operator must be sliced ELL");
     int n = A.n;
     size_t ln = (size_t)n + 64;
@@ -1047,8 +1060,10 @@ int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const do
     ConvCheck cc(c, tol * tol * bb, bb);
     for (it = 1; it <= maxit; ++it) {
         SEMO_LAUNCH(c, "bicg_p", 48.0 * n, k_bicg_p, dim3(nblk(n)), dim3(TPB), n, s, r, v, dinv, p, ph);
+        if (pre) stage_vcycle(c, *pre, p, ph);                            // ph = M^-1 p (AMG)
         spmv_dot(c, A, ph, v, rh, nullptr, W_BICG_ALPHA);                 // v = A ph, alpha
         SEMO_LAUNCH(c, "bicg_s", 40.0 * n, k_bicg_s, dim3(nblk(n)), dim3(TPB), n, s, r, v, dinv, sv, sh);
+        if (pre) stage_vcycle(c, *pre, sv, sh);
         spmv_dot(c, A, sh, t, nullptr, sv, W_BICG_OMEGA);                 // t = A sh, omega
         SEMO_LAUNCH(c, "bicg_x", 64.0 * n, k_bicg_x, dim3(nblk(n)), dim3(TPB), n, s, ph, sh, sv, t, rh, x, r,
                     red_of(c));
@@ -1064,8 +1079,8 @@ int bicgstab(semo_ctx* c, const DevSell<double>& A, const double* dinv, const do
 
 // BiCGStab on the three momentum components in lockstep (Jacobi, x0 = 0);
 // stops when every component has ||r_k|| <= tol ||b_k||
-void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const double* b, double* x, double tol,
-               int maxit, semo_solve_stats* st) {
+int bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const double* b, double* x, double tol,
+              int maxit, semo_solve_stats* st, Amg* pre = nullptr) {
     SEMO_CHECK(A.lanes == 0, SEMO_EINVAL, "Krylov operator must be sliced ELL");
     int n = A.n;
     size_t ln = 3 * (size_t)n + 64;
@@ -1104,9 +1119,13 @@ void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const 
         ConvCheck cc(c, tol * tol, 1.0);
         for (it = 1; it <= maxit; ++it) {
             SEMO_LAUNCH(c, "bicg3_p", 128.0 * n, k_bicg3_p, dim3(nb), dim3(TPB), n, s, r, v, dinv, p, ph);
+            if (pre)                                               // ph_k = M^-1 p_k (AMG)
+                for (int k = 0; k < 3; ++k) stage_vcycle(c, *pre, p + (size_t)k * n, ph + (size_t)k * n);
             spmv3(0, ph, v, rh);                                   // v = A ph, rh.v
             fin(nbs, F3_ALPHA);
             SEMO_LAUNCH(c, "bicg3_s", 104.0 * n, k_bicg3_s, dim3(nb), dim3(TPB), n, s, r, v, dinv, sv, sh);
+            if (pre)
+                for (int k = 0; k < 3; ++k) stage_vcycle(c, *pre, sv + (size_t)k * n, sh + (size_t)k * n);
             spmv3(1, sh, t, sv);                                   // t = A sh, t.s, t.t
             fin(nbs, F3_OMEGA);
             SEMO_LAUNCH(c, "bicg3_x", 192.0 * n, k_bicg3_x, dim3(nb), dim3(TPB), n, s, ph, sh, sv, t, rh, x, r, part);
@@ -1127,6 +1146,7 @@ void bicgstab3(semo_ctx* c, const DevSell<double>& A, const double* dinv, const 
         if (st) { st->iters[k] = bb[k] > 0.0 ? it : 0; st->rel_res[k] = rel; }
         check_converged(std::isfinite(bb[k]) ? rel : bb[k], tol, "momentum");
     }
+    return it;
 }
 
 // level-0 operator of the hierarchy refreshed from the assembled system
@@ -1235,8 +1255,10 @@ void reduction_alloc(semo_ctx* c, long n) {
 // The pressure hierarchy for the operator in c->Amat: built when missing;
 // when the values changed, the finest level is refreshed on the device while
 // keep_coarse says the coarse levels still serve, else rebuilt.
-void pressure_amg_update(semo_ctx* c, bool (*keep_coarse)(const Amg&)) {
-    Amg& amg = c->amg_p;
+// the hierarchy `amg` for the operator in c->Amat, whose values it was built
+// from are kept in key_val / key_diag (device copies)
+void amg_update(semo_ctx* c, Amg& amg, DBuf<double>& key_val, DBuf<double>& key_diag,
+                bool (*keep_coarse)(const Amg&)) {
     bool rebuild = !amg.built;
     if (amg.built) {
         // device-side comparison with the values the hierarchy was built from
@@ -1245,9 +1267,9 @@ void pressure_amg_update(semo_ctx* c, bool (*keep_coarse)(const Amg&)) {
         zero_d(c, flag.p, sizeof(int));
         size_t nv = c->Amat.val.n;
         SEMO_LAUNCH(c, "amg_values_changed", 16.0 * nv, k_neq, dim3((unsigned)((nv + TPB - 1) / TPB)), dim3(TPB), nv,
-                    c->Amat.val.p, c->amg_key_val.p, flag.p);
+                    c->Amat.val.p, key_val.p, flag.p);
         SEMO_LAUNCH(c, "amg_values_changed", 16.0 * c->nc, k_neq, dim3((unsigned)((c->nc + TPB - 1) / TPB)),
-                    dim3(TPB), (size_t)c->nc, c->Amat.diag.p, c->amg_key_diag.p, flag.p);
+                    dim3(TPB), (size_t)c->nc, c->Amat.diag.p, key_diag.p, flag.p);
         int f = 0;
         SEMO_HIP(hipMemcpyAsync(&f, flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         SEMO_HIP(hipStreamSynchronize(c->stream));
@@ -1271,19 +1293,23 @@ void pressure_amg_update(semo_ctx* c, bool (*keep_coarse)(const Amg&)) {
                     dim3(TPB), nv, amg.l0_map.p, c->Amat.val.p, L0.A64.val.p, L0.A32.val.p);
         SEMO_LAUNCH(c, "amg_refresh_L0", 32.0 * c->nc, k_refresh_diag, dim3(nblk(c->nc)), dim3(TPB), c->nc,
                     c->Amat.diag.p, L0.A64.diag.p, L0.A32.diag.p, L0.dinv64.p, L0.dinv32.p);
-        copy_d(c, c->amg_key_val.p, c->Amat.val.p, c->Amat.val.n);
-        copy_d(c, c->amg_key_diag.p, c->Amat.diag.p, c->Amat.diag.n);
+        copy_d(c, key_val.p, c->Amat.val.p, c->Amat.val.n);
+        copy_d(c, key_diag.p, c->Amat.diag.p, c->Amat.diag.n);
         amg.fresh = false;
     }
     if (rebuild) {
         amg.built = false;
         amg_build(c, amg, c->Amat);
-        c->amg_key_val.alloc(c->Amat.val.n);
-        c->amg_key_diag.alloc(c->nc);
-        SEMO_HIP(hipMemcpyAsync(c->amg_key_val.p, c->Amat.val.p, c->Amat.val.bytes(), hipMemcpyDeviceToDevice, c->stream));
-        SEMO_HIP(hipMemcpyAsync(c->amg_key_diag.p, c->Amat.diag.p, c->Amat.diag.bytes(), hipMemcpyDeviceToDevice, c->stream));
+        key_val.alloc(c->Amat.val.n);
+        key_diag.alloc(c->nc);
+        SEMO_HIP(hipMemcpyAsync(key_val.p, c->Amat.val.p, c->Amat.val.bytes(), hipMemcpyDeviceToDevice, c->stream));
+        SEMO_HIP(hipMemcpyAsync(key_diag.p, c->Amat.diag.p, c->Amat.diag.bytes(), hipMemcpyDeviceToDevice, c->stream));
         amg.fresh = true;
     }
+}
+
+void pressure_amg_update(semo_ctx* c, bool (*keep_coarse)(const Amg&)) {
+    amg_update(c, c->amg_p, c->amg_key_val, c->amg_key_diag, keep_coarse);
 }
 
 // z = M^-1 r: one V-cycle of the pressure hierarchy, FP32 (fp32) or FP64,
@@ -1315,22 +1341,53 @@ void solve_pressure(semo_ctx* c, const double* b, double* x, semo_solve_stats* s
     check_converged(rr, c->sp.tol_p, "pressure");
 }
 
+// BiCGStab keeps the coarse levels of its AMG preconditioner while it
+// converges within 1.5x (+3) the iterations right after the last build
+bool bicg_keeps_coarse(const Amg& amg) {
+    return amg.iters_ref >= 0 && amg.last_iters <= std::max((3 * amg.iters_ref) / 2, amg.iters_ref + 3);
+}
+
+// momentum (nrhs = 3, one operator, three right-hand sides in lockstep) and
+// volume fraction (nrhs = 1): BiCGStab right-preconditioned by Jacobi, or by
+// an AMG V-cycle of the system's own hierarchy (SEMO_MSOLVER_AMG, or
+// SEMO_MSOLVER_AUTO once a Jacobi solve of that system has needed more than
+// MSOLVER_AUTO_ITERS iterations), as SEMO's AMGCL solves precondition them
+// with aggregation AMG (solveAMGCL.cpp:881-927, 1195-1258)
+constexpr int MSOLVER_AUTO_ITERS = 30;
 void solve_momentum(semo_ctx* c, int nrhs, semo_solve_stats* st) {
     int n = c->nc;
     reduction_alloc(c, n);
     DBuf<double>& dinv = c->dinv_mom;
     dinv.alloc(n + 64);
     SEMO_LAUNCH(c, "jacobi_dinv", 16.0 * n, k_dinv, dim3(nblk(n)), dim3(TPB), n, c->Amat.diag.p, dinv.p);
-    if (nrhs == 3) {
-        bicgstab3(c, c->Amat, dinv.p, c->Bvec.p, c->Xvec.p, c->sp.tol_u, c->sp.maxit_u, st);
-        return;
+    bool mom = nrhs == 3;
+    bool& auto_on = mom ? c->amg_mom_on : c->amg_vf_on;
+    Amg* pre = nullptr;
+    if (c->msolver == SEMO_MSOLVER_AMG || (c->msolver == SEMO_MSOLVER_AUTO && auto_on)) {
+        Amg& amg = mom ? c->amg_m : c->amg_vf;
+        amg_update(c, amg, mom ? c->amg_m_kv : c->amg_vf_kv, mom ? c->amg_m_kd : c->amg_vf_kd, bicg_keeps_coarse);
+        c->mom_stage.ensure(2 * ((size_t)n + 64));
+        pre = &amg;
     }
-    for (int k = 0; k < nrhs; ++k) {
-        double rr = 0;
-        int it = bicgstab(c, c->Amat, dinv.p, c->Bvec.p + (size_t)k * n, c->Xvec.p + (size_t)k * n, c->sp.tol_u,
-                          c->sp.maxit_u, &rr);
-        if (st) { st->iters[k] = it; st->rel_res[k] = rr; }
-        check_converged(rr, c->sp.tol_u, "momentum/volume-fraction");
+    int it = 0;
+    if (nrhs == 3) {
+        it = bicgstab3(c, c->Amat, dinv.p, c->Bvec.p, c->Xvec.p, c->sp.tol_u, c->sp.maxit_u, st, pre);
+    } else {
+        for (int k = 0; k < nrhs; ++k) {
+            double rr = 0;
+            int ik = bicgstab(c, c->Amat, dinv.p, c->Bvec.p + (size_t)k * n, c->Xvec.p + (size_t)k * n,
+                              c->sp.tol_u, c->sp.maxit_u, &rr, pre);
+            if (st) { st->iters[k] = ik; st->rel_res[k] = rr; }
+            check_converged(rr, c->sp.tol_u, "momentum/volume-fraction");
+            it = std::max(it, ik);
+        }
+    }
+    if (pre) {
+        if (pre->fresh) { pre->iters_ref = it; pre->fresh = false; }
+        pre->last_iters = it;
+        if (st) { st->n_levels = (int)pre->lv.size(); st->setup_ms = pre->setup_ms; }
+    } else if (c->msolver == SEMO_MSOLVER_AUTO && it > MSOLVER_AUTO_ITERS) {
+        auto_on = true;
     }
 }
 

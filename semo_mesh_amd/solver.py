@@ -93,6 +93,14 @@ class GpuSolver:
         n = self.mesh.n_faces if name in FACE_FIELDS else self.mesh.n_cells
         return n * COMPONENTS.get(name, 1)
 
+    MSOLVERS = {"jacobi": 0, "amg": 1, "auto": 2}
+
+    def set_momentum_solver(self, mode: str):
+        """Preconditioner of the momentum / volume-fraction BiCGStab:
+        "jacobi" (default), "amg" (a V-cycle of the system's own hierarchy)
+        or "auto" (Jacobi until a solve needs more than 30 iterations)."""
+        self._ok(self._L.semo_set_momentum_solver(self._h, self.MSOLVERS[mode]))
+
     def set_boundary_velocity(self, values):
         """Per-face values of the fixedValue-U boundaries: values[3][n_b] over
         the boundary faces in the mesh's face order (faces n_internal_faces ..
