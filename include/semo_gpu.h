@@ -340,6 +340,21 @@ int semo_gauss_green(semo_ctx* ctx, int field, const double* phi, const double* 
  * kinetic energy.  The dynamic model filters over each cell's vertex stencil
  * (the calcLeastSquare stencil).                                              */
 enum { SEMO_LES_WALE = 1, SEMO_LES_SMAGORINSKY = 2, SEMO_LES_DYNAMIC_SMAGORINSKY = 3 };
+/* Convective flux schemes of SEMO's density-based solvers
+ * (src/solvers/flux.cpp): calcFluxHAUS (:5-156), calcFluxSLAU2 (:159-238, the
+ * one calcFluxes uses, src/solvers/RHS.cpp:272-287), calcFluxRoe (:243-344,
+ * its central part: the dissipation is commented out), calcFluxAPRoe
+ * (:349-470), calcFluxRoeAMTP (:480-631), calcFluxUpwind (:649-692),
+ * calcFluxAUSMPWP_N (:707-787), calcFluxRoeM_N (:795-922).  Per face:
+ * left[n][8 + nsp] and right[n][8 + nsp] = (P, U, V, W, T, rho, C, Ht,
+ * Y_0 .. Y_{nsp-1}), normal[n][3] (unit), par[n][7] = (w1, w2, w3, fC, Uco,
+ * dt, Lch) for AUSMPWP_N and RoeM_N (may be NULL for the others);
+ * flux[n][4 + nsp] out: (rho, rho u, rho v, rho w, rho Ht, rho Y_0 ..
+ * rho Y_{nsp-2}) fluxes.  1 <= nsp <= 8.  Needs a context, not a mesh.       */
+enum { SEMO_FLUX_HAUS = 0, SEMO_FLUX_SLAU2, SEMO_FLUX_ROE, SEMO_FLUX_APROE, SEMO_FLUX_ROEAMTP,
+       SEMO_FLUX_UPWIND, SEMO_FLUX_AUSMPWP_N, SEMO_FLUX_ROEM_N };
+int semo_flux(semo_ctx* ctx, int scheme, int64_t n, int nsp, const double* left, const double* right,
+              const double* normal, const double* par, double* flux);
 /* turbType / turbLESModel of constant/turbulenceProperties (src/controls/
  * build.cpp:100-115): model 0 = laminar (the default), else an SEMO_LES_*
  * model, which every calcCellTransport of the time step then applies

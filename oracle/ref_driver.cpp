@@ -203,6 +203,54 @@ int main(int argc, char** argv) {
     mkdir(g_out.c_str(), 0755);
     if (argc > 4 && !strcmp(argv[4], "nodump")) g_dump = false;
     string mode = argc > 4 ? argv[4] : "";
+    if (mode == "flux") {   // the convective flux schemes (solvers/flux.cpp) on face states from argv[5]
+        if (argc < 6) { fprintf(stderr, "flux needs FILE\n"); return 2; }
+        FILE* f = fopen(argv[5], "rb");
+        if (!f) { perror(argv[5]); return 2; }
+        int hdr[2];
+        rd(f, hdr, 2);
+        int n = hdr[0], nsp = hdr[1];
+        int rec = 2 * (8 + nsp) + 3 + 7;          // L, R, normal, w1 w2 w3 fC Uco dt Lch
+        vector<double> in((size_t)n * rec);
+        rd(f, in.data(), in.size());
+        fclose(f);
+        SEMO_Solvers_Builder sb;
+        const char* names[8] = {"haus", "slau2", "roe", "aproe", "roeamtp", "upwind", "ausmpwp_n", "roem_n"};
+        for (int k = 0; k < 8; ++k) {
+            vector<double> out;
+            for (int i = 0; i < n; ++i) {
+                const double* q = &in[(size_t)i * rec];
+                double PL = q[0], UL = q[1], VL = q[2], WL = q[3], TL = q[4], rhoL = q[5], CL = q[6], HtL = q[7];
+                vector<double> YL(q + 8, q + 8 + nsp);
+                const double* r = q + 8 + nsp;
+                double PR = r[0], UR = r[1], VR = r[2], WR = r[3], TR = r[4], rhoR = r[5], CR = r[6], HtR = r[7];
+                vector<double> YR(r + 8, r + 8 + nsp);
+                const double* t = r + 8 + nsp;
+                vector<double> nv(t, t + 3);
+                double w1 = t[3], w2 = t[4], w3 = t[5], fC = t[6], Uco = t[7], dts = t[8], Lch = t[9];
+                vector<double> fl;
+                switch (k) {
+                case 0: sb.calcFluxHAUS(PL, UL, VL, WL, TL, YL, rhoL, CL, HtL, PR, UR, VR, WR, TR, YR, rhoR, CR, HtR, nv, fl); break;
+                case 1: sb.calcFluxSLAU2(PL, UL, VL, WL, TL, YL, rhoL, CL, HtL, PR, UR, VR, WR, TR, YR, rhoR, CR, HtR, nv, fl); break;
+                case 2: sb.calcFluxRoe(PL, UL, VL, WL, TL, YL, rhoL, CL, HtL, PR, UR, VR, WR, TR, YR, rhoR, CR, HtR, nv, fl); break;
+                case 3: sb.calcFluxAPRoe(PL, UL, VL, WL, TL, YL, rhoL, CL, HtL, PR, UR, VR, WR, TR, YR, rhoR, CR, HtR, nv, fl); break;
+                case 4: sb.calcFluxRoeAMTP(PL, UL, VL, WL, TL, YL, rhoL, CL, HtL, PR, UR, VR, WR, TR, YR, rhoR, CR, HtR, nv, fl); break;
+                case 5: sb.calcFluxUpwind(PL, UL, VL, WL, TL, YL, rhoL, CL, HtL, PR, UR, VR, WR, TR, YR, rhoR, CR, HtR, nv, fl); break;
+                case 6: sb.calcFluxAUSMPWP_N(PL, UL, VL, WL, TL, YL, rhoL, CL, HtL, PR, UR, VR, WR, TR, YR, rhoR, CR, HtR, nv,
+                                             w1, w2, w3, fC, Uco, dts, Lch, fl); break;
+                default: sb.calcFluxRoeM_N(PL, UL, VL, WL, TL, YL, rhoL, CL, HtL, PR, UR, VR, WR, TR, YR, rhoR, CR, HtR, nv,
+                                           w1, w2, w3, fC, Uco, dts, Lch, fl); break;
+                }
+                if ((int)fl.size() != 4 + nsp) { fprintf(stderr, "flux length %zu\n", fl.size()); return 2; }
+                out.insert(out.end(), fl.begin(), fl.end());
+            }
+            dump(string("flux_") + names[k], out);
+        }
+        printf("SEMO_REF_DONE flux\n");
+        fflush(stdout);
+        MPI_Finalize();
+        return 0;
+    }
     if ((mode == "savevtu" && argc < 7) || (mode == "loadvtu" && argc < 6)) {
         fprintf(stderr, "savevtu needs FORMAT COMPRESSION, loadvtu needs FOLDER\n");
         return 2;

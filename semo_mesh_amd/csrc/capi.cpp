@@ -769,6 +769,29 @@ int semo_gauss_green(semo_ctx* c, int field, const double* phi, const double* fa
     });
 }
 
+int semo_flux(semo_ctx* c, int scheme, int64_t n, int nsp, const double* left, const double* right,
+              const double* normal, const double* par, double* flux) {
+    if (!c || n < 0 || !left || !right || !normal || !flux) return SEMO_EINVAL;
+    return guard(c, [&] {
+        SEMO_CHECK(scheme >= SEMO_FLUX_HAUS && scheme <= SEMO_FLUX_ROEM_N, SEMO_EINVAL, "unknown flux scheme");
+        SEMO_CHECK(nsp >= 1 && nsp <= 8, SEMO_EINVAL, "flux: 1 <= nsp <= 8");
+        SEMO_CHECK(n <= INT32_MAX / 64, SEMO_EINVAL, "flux: too many faces");
+        bool needs_par = scheme == SEMO_FLUX_AUSMPWP_N || scheme == SEMO_FLUX_ROEM_N;
+        SEMO_CHECK(par || !needs_par, SEMO_EINVAL, "flux: AUSMPWP_N and RoeM_N need par");
+        if (n == 0) return;
+        size_t w = 8 + (size_t)nsp;
+        DBuf<double> dL, dR, dN, dP, dF;
+        dL.upload(left, n * w, c->stream);
+        dR.upload(right, n * w, c->stream);
+        dN.upload(normal, 3 * (size_t)n, c->stream);
+        if (needs_par) dP.upload(par, 7 * (size_t)n, c->stream);
+        dF.alloc((size_t)n * (4 + nsp));
+        flux_device(c, scheme, (int)n, nsp, dL.p, dR.p, dN.p, dP.p, dF.p);
+        dF.download(flux, dF.n, c->stream);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 int semo_set_momentum_solver(semo_ctx* c, int mode) {
     if (!c) return SEMO_EINVAL;
     return guard(c, [&] {

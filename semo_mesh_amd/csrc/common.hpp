@@ -451,6 +451,8 @@ void vf_function_values(semo_ctx* c, int n, const double* pts4_host);
 // types and values); faceR_b: right face values of the boundary faces [nbf]
 void gauss_green_standalone(semo_ctx* c, int sel, const double* phi, const double* faceR_b, const double* gin,
                             double* gout);
+void flux_device(semo_ctx* c, int scheme, int n, int nsp, const double* L, const double* R, const double* nrm,
+                 const double* par, double* flux);
 void les_device(semo_ctx* c, int model, const double* U, const double* V, const double* W, const double* Rho,
                 const double* bR, double* muT, double* kSGS);
 void source_gravity(semo_ctx* c);

@@ -1,0 +1,431 @@
+// Convective flux schemes of SEMO's density-based solvers
+// (src/solvers/flux.cpp): per face, from the left and right states
+// (P, U, V, W, T, rho, C, Ht and the species fractions Y) and the unit
+// normal, the flux of (rho, rho u, rho v, rho w, rho Ht, rho Y_0 ..
+// rho Y_{nsp-2}).  One thread per face; the arithmetic is the reference's,
+// in its order (pow(x, 2.0) written x * x, which is what it evaluates to),
+// including what it does differently from the textbook forms:
+//   - calcFluxRoe returns the central part only (its dissipation is
+//     commented out, flux.cpp:330-340);
+//   - calcFluxAPRoe and calcFluxRoeAMTP put the pressure dissipation's
+//     z-momentum term into the energy slot (DP[4] = DP[0] n_z, flux.cpp:
+//     456, 616), leaving z-momentum without it;
+//   - calcFluxAUSMPWP_N and calcFluxRoeM_N take the sensors w1, w2, w3, the
+//     face sound speed fC, a cut-off velocity, the time step and a length
+//     (flux.cpp:707-923).
+#include "common.hpp"
+#include "kern.hpp"
+
+namespace semo {
+namespace {
+
+// std::min / std::max as the reference calls them: on ties (including
+// +0 / -0) the first argument
+__device__ inline double mn(double a, double b) { return b < a ? b : a; }
+__device__ inline double mx(double a, double b) { return a < b ? b : a; }
+
+struct FSide {
+    double P, U, V, W, T, rho, C, Ht;
+    const double* Y;
+};
+
+// flux.cpp:924-953
+__device__ inline double M_func(double M, double op, double alp) {
+    if (fabs(M) > 1.0) return 0.5 * (M + op * fabs(M));
+    return op * 0.25 * ((M + op) * (M + op)) + op * alp * ((M * M - 1.0) * (M * M - 1.0));
+}
+__device__ inline double pre_func(double M, double op, double alp) {
+    if (fabs(M) > 1.0) return 0.5 * (1.0 + op * (M > 0.0 ? 1.0 : -1.0));
+    return 0.25 * ((M + op) * (M + op)) * (2.0 - op * M) + op * alp * M * ((M * M - 1.0) * (M * M - 1.0));
+}
+// flux.cpp:635-644
+__device__ inline double f_func(double M) {
+    return mn(M * sqrt(4.0 + (1.0 - M * M) * (1.0 - M * M)) / (1.0 + M * M), 1.0);
+}
+
+// the flux of an upwind mass split (f1L, f1R) with the interface pressure PLR
+__device__ inline void split_flux(const FSide& L, const FSide& R, const double* n, double f1L, double f1R,
+                                  double PLR, int nsp, double* fl) {
+    fl[0] = f1L + f1R;
+    fl[1] = f1L * L.U + f1R * R.U + PLR * n[0];
+    fl[2] = f1L * L.V + f1R * R.V + PLR * n[1];
+    fl[3] = f1L * L.W + f1R * R.W + PLR * n[2];
+    fl[4] = f1L * L.Ht + f1R * R.Ht;
+    for (int i = 0; i < nsp - 1; ++i) fl[5 + i] = f1L * L.Y[i] + f1R * R.Y[i];
+}
+// the advection flux of one side, component k
+__device__ inline double adv(const FSide& s, double Un, const double* n, int k) {
+    switch (k) {
+    case 0: return s.rho * Un;
+    case 1: return s.rho * Un * s.U + s.P * n[0];
+    case 2: return s.rho * Un * s.V + s.P * n[1];
+    case 3: return s.rho * Un * s.W + s.P * n[2];
+    case 4: return s.rho * Un * s.Ht;
+    default: return s.rho * Un * s.Y[k - 5];
+    }
+}
+
+// calcFluxHAUS (flux.cpp:5-156)
+__device__ void flux_haus(const FSide& L, const FSide& R, const double* n, int nsp, double* fl) {
+    double UnL = L.U * n[0] + L.V * n[1] + L.W * n[2];
+    double UnR = R.U * n[0] + R.V * n[1] + R.W * n[2];
+    double chat = 0.5 * (L.C + R.C);
+    double ML = UnL / chat, MR = UnR / chat;
+    double KLR = sqrt(0.5 * (L.U * L.U + L.V * L.V + L.W * L.W + R.U * R.U + R.V * R.V + R.W * R.W));
+    double MLP = M_func(ML, 1.0, 0.0), MRM = M_func(MR, -1.0, 0.0);
+    double preP = pre_func(ML, 1.0, 0.0), preM = pre_func(MR, -1.0, 0.0);
+    double Mbar = (L.rho * fabs(ML) + R.rho * fabs(MR)) / (L.rho + R.rho);
+    double g_c = 1.0 + mx(mn(ML, 0.0), -1.0) * mn(mx(MR, 0.0), 1.0);
+    double D_L = ML + (1.0 - g_c) * fabs(ML);
+    double D_R = MR - (1.0 - g_c) * fabs(MR);
+    double D_rho = Mbar * g_c;
+    double preLs = fabs(L.P) + 0.1 * L.rho * L.C * L.C;
+    double preRs = fabs(R.P) + 0.1 * R.rho * R.C * R.C;
+    double mr = mn(preLs / preRs, preRs / preLs);
+    double w = 1.0 - mr * mr;
+    double ps = preP * L.P + preM * R.P;
+    double pr = mn(L.P / R.P, R.P / L.P);
+    double pll = (3.0 / 4.0 <= pr && 1.0 > pr) ? 4.0 * pr - 3.0 : 0.0;
+    double fL = fabs(ML) <= 1.0 ? (L.P / ps - 1.0) * pll * fabs(MLP) * mn(1.0, pow((fabs(UnL) / chat), 0.25)) : 0.0;
+    double fR = fabs(MR) <= 1.0 ? (L.P / ps - 1.0) * pll * fabs(MRM) * mn(1.0, pow((fabs(UnR) / chat), 0.25)) : 0.0;
+    double MPP = MLP + MRM;
+    double MLP_AUSM = 0.5 * (MPP + fabs(MPP)), MRM_AUSM = 0.5 * (MPP - fabs(MPP));
+    double MLP_SLAU = 0.5 * (D_L + D_rho), MRM_SLAU = 0.5 * (D_R - D_rho);
+    double fa1 = w;
+    double MLPL = fa1 * MLP_AUSM + (1.0 - fa1) * MLP_SLAU;
+    double MRMR = fa1 * MRM_AUSM + (1.0 - fa1) * MRM_SLAU;
+    double mdot = L.rho * chat * MLPL + R.rho * chat * MRMR -
+                  0.5 * (1.0 - 0.5 * (1.0 - cos(3.141592 * mn(1.0, mx(fabs(ML), fabs(MR)))))) *
+                      (1.0 - 0.5 * (1.0 + cos(3.141592 * mn(fabs(L.P / R.P), fabs(R.P / L.P))))) *
+                      (R.P - L.P) / chat;
+    MLP = 0.5 * (ML + fabs(ML));
+    MRM = 0.5 * (MR - fabs(MR));
+    double f1L, f1R;
+    if (mdot >= 0.0) {
+        f1L = mdot - (L.rho * chat * MRM) * (w * (1.0 + fR) - fR + fL);
+        f1R = (R.rho * chat * MRM) * (w * (1.0 + fR));
+    } else {
+        f1L = (L.rho * chat * MLP) * (w * (1.0 + fL));
+        f1R = mdot - (R.rho * chat * MLP) * (w * (1.0 + fL) - fL + fR);
+    }
+    double fa2 = w, gam = 0.6;
+    double PLR = 0.5 * (L.P + R.P) - fa2 * (KLR / chat) * 0.5 * preP * preM * 0.5 * (L.P + R.P) / chat * (UnR - UnL) +
+                 mx(0.2, gam) * (KLR / chat) * 0.5 * (L.P + R.P) * (preP + preM - 1.0) -
+                 0.5 * (preP - preM) * (R.P - L.P);
+    split_flux(L, R, n, f1L, f1R, PLR, nsp, fl);
+}
+
+// calcFluxSLAU2 (flux.cpp:159-238)
+__device__ void flux_slau2(const FSide& L, const FSide& R, const double* n, int nsp, double* fl) {
+    double UnL = L.U * n[0] + L.V * n[1] + L.W * n[2];
+    double UnR = R.U * n[0] + R.V * n[1] + R.W * n[2];
+    double chat = 0.5 * (L.C + R.C);
+    double ML = UnL / chat, MR = UnR / chat;
+    double KLR = sqrt(0.5 * (L.U * L.U + L.V * L.V + L.W * L.W + R.U * R.U + R.V * R.V + R.W * R.W));
+    double preP = pre_func(ML, 1.0, 0.0), preM = pre_func(MR, -1.0, 0.0);
+    double Mbar = (L.rho * fabs(ML) + R.rho * fabs(MR)) / (L.rho + R.rho);
+    double Mcy = mn(1.0, KLR / chat);
+    double phi_c = (1.0 - Mcy) * (1.0 - Mcy);
+    double g_c = 1.0 + mx(mn(ML, 0.0), -1.0) * mn(mx(MR, 0.0), 1.0);
+    double D_L = ML + (1.0 - g_c) * fabs(ML);
+    double D_R = MR - (1.0 - g_c) * fabs(MR);
+    double D_rho = Mbar * g_c;
+    double MLPL = 0.5 * (D_L + D_rho), MRMR = 0.5 * (D_R - D_rho);
+    double mdot = L.rho * chat * MLPL + R.rho * chat * MRMR - 0.5 * phi_c / chat * (R.P - L.P);
+    double f1L = mdot >= 0.0 ? mdot : 0.0, f1R = mdot >= 0.0 ? 0.0 : mdot;
+    double PLR = 0.5 * (L.P + R.P) - 0.5 * Mcy * preP * preM * 0.5 * (L.P + R.P) / chat * (UnR - UnL) +
+                 Mcy * 0.5 * (L.P + R.P) * (preP + preM - 1.0) - 0.5 * (preP - preM) * (R.P - L.P);
+    split_flux(L, R, n, f1L, f1R, PLR, nsp, fl);
+}
+
+// calcFluxRoe (flux.cpp:243-344): the central part
+__device__ void flux_roe(const FSide& L, const FSide& R, const double* n, int nsp, double* fl) {
+    double UnL = L.U * n[0] + L.V * n[1] + L.W * n[2];
+    double UnR = R.U * n[0] + R.V * n[1] + R.W * n[2];
+    for (int k = 0; k < 4 + nsp; ++k) fl[k] = 0.5 * (adv(L, UnL, n, k) + adv(R, UnR, n, k));
+}
+
+// the Roe averages of calcFluxAPRoe / calcFluxRoeAMTP
+struct RoeAvg {
+    double RT, uhat, vhat, what, Hthat, rhohat, chat, unhat;
+};
+__device__ inline RoeAvg roe_avg(const FSide& L, const FSide& R, double UnL, double UnR) {
+    RoeAvg a;
+    a.RT = sqrt(R.rho / L.rho);
+    a.uhat = (L.U + a.RT * R.U) / (1.0 + a.RT);
+    a.vhat = (L.V + a.RT * R.V) / (1.0 + a.RT);
+    a.what = (L.W + a.RT * R.W) / (1.0 + a.RT);
+    a.Hthat = (L.Ht + a.RT * R.Ht) / (1.0 + a.RT);
+    a.rhohat = a.RT * L.rho;
+    a.chat = (L.C + a.RT * R.C) / (1.0 + a.RT);
+    a.unhat = (UnL + a.RT * UnR) / (1.0 + a.RT);
+    return a;
+}
+// the jump of the conserved variables, component k
+__device__ inline double dW(const FSide& L, const FSide& R, int k) {
+    switch (k) {
+    case 0: return R.rho - L.rho;
+    case 1: return R.rho * R.U - L.rho * L.U;
+    case 2: return R.rho * R.V - L.rho * L.V;
+    case 3: return R.rho * R.W - L.rho * L.W;
+    case 4: return R.rho * (R.Ht - R.P / R.rho) - L.rho * (L.Ht - L.P / L.rho);
+    default: return R.rho * R.Y[k - 5] - L.rho * L.Y[k - 5];
+    }
+}
+// U' times (1, uhat, vhat, what, Hthat, Yhat...), component k
+__device__ inline double dUk(const RoeAvg& a, const FSide& L, const FSide& R, double d0, int k) {
+    switch (k) {
+    case 0: return d0;
+    case 1: return d0 * a.uhat;
+    case 2: return d0 * a.vhat;
+    case 3: return d0 * a.what;
+    case 4: return d0 * a.Hthat;
+    default: return d0 * ((L.Y[k - 5] + a.RT * R.Y[k - 5]) / (1.0 + a.RT));
+    }
+}
+
+// calcFluxAPRoe (flux.cpp:349-470)
+__device__ void flux_aproe(const FSide& L, const FSide& R, const double* n, int nsp, double* fl) {
+    double UnL = L.U * n[0] + L.V * n[1] + L.W * n[2];
+    double UnR = R.U * n[0] + R.V * n[1] + R.W * n[2];
+    RoeAvg a = roe_avg(L, R, UnL, UnR);
+    double chat = a.chat, unhat = a.unhat, rhohat = a.rhohat;
+    double xi = fabs(unhat);
+    double Mcy = sqrt(0.5 * (L.U * L.U + L.V * L.V + L.W * L.W + R.U * R.U + R.V * R.V + R.W * R.W)) / chat;
+    double theta = mn(mx(Mcy * Mcy, 0.01 * 0.01), 1.0);
+    double Um = unhat * mn(mx(Mcy, 0.01), 1.0);
+    double Cm = chat * mn(mx(Mcy, 0.01), 1.0);
+    double thetad = mn(Mcy * Mcy, 1.0);
+    double Ud = 0.5 * (1.0 + thetad);
+    double Cd = 0.5 * sqrt(4.0 * chat * chat * thetad + ((1.0 - thetad) * (1.0 - thetad)) * unhat * unhat);
+    double DU0 = (Cm - 0.5 * (1.0 - theta) * unhat * Um / Cm - theta * fabs(unhat)) * (R.P - L.P) / rhohat / theta /
+                     chat / chat +
+                 Ud / Cm * (UnR - UnL);
+    DU0 = DU0 * rhohat;
+    double DP0 = (Cd - fabs(unhat) + 0.5 * (1.0 - thetad) * unhat * Ud / Cm) * rhohat * (UnR - UnL) +
+                 Ud * (R.P - L.P) / Cm;
+    for (int k = 0; k < 4 + nsp; ++k) {
+        double DP = k == 1 ? DP0 * n[0] : k == 2 ? DP0 * n[1] : k == 4 ? DP0 * n[2] : 0.0;
+        fl[k] = 0.5 * (adv(L, UnL, n, k) + adv(R, UnR, n, k)) - 0.5 * (xi * dW(L, R, k) + DP + dUk(a, L, R, DU0, k));
+    }
+}
+
+// calcFluxRoeAMTP (flux.cpp:480-631)
+__device__ void flux_roeamtp(const FSide& L, const FSide& R, const double* n, int nsp, double* fl) {
+    double UnL = L.U * n[0] + L.V * n[1] + L.W * n[2];
+    double UnR = R.U * n[0] + R.V * n[1] + R.W * n[2];
+    RoeAvg a = roe_avg(L, R, UnL, UnR);
+    double chat = a.chat, unhat = a.unhat, rhohat = a.rhohat;
+    double Morg = 0.5 * (sqrt(L.U * L.U + L.V * L.V + L.W * L.W) / L.C + sqrt(R.U * R.U + R.V * R.V + R.W * R.W) / R.C);
+    double Mbar = fabs(unhat) / chat;
+    double fef = 0.05 * chat;
+    double ksqrt = sqrt((R.U - L.U) * (R.U - L.U) + (R.V - L.V) * (R.V - L.V) + (R.W - L.W) * (R.W - L.W));
+    double n1x, n1y, n1z;
+    if (ksqrt < 1.e-5) { n1x = n[0]; n1y = n[1]; n1z = n[2]; }
+    else { n1x = (R.U - L.U) / ksqrt; n1y = (R.V - L.V) / ksqrt; n1z = (R.W - L.W) / ksqrt; }
+    double n2x = -n1y * (n1x * n[1] - n1y * n[0]) - n1z * (n1x * n[2] - n1z * n[0]);
+    double n2y = n1x * (n1x * n[1] - n1y * n[0]) - n1z * (n1y * n[2] - n1z * n[1]);
+    double n2z = n1x * (n1x * n[2] - n1z * n[0]) + n1y * (n1y * n[2] - n1z * n[1]);
+    double a1 = n1x * n[0] + n1y * n[1] + n1z * n[2];
+    double a2 = n2x * n[0] + n2y * n[1] + n2z * n[2];
+    double U1 = n1x * a.uhat + n1y * a.vhat + n1z * a.what;
+    double U2 = n2x * a.uhat + n2y * a.vhat + n2z * a.what;
+    double frr = fabs(a1 * U1) + fabs(a2 * U2);
+    double fMbar8 = pow(f_func(Mbar), 8.0), fMorg8 = pow(f_func(Morg), 8.0);
+    double xi = mx(fabs(0.5 * (UnR + UnL)) + 0.5 * (UnR - UnL), (1.0 - fMbar8) * fMorg8 * mn(fef, frr));
+    double Und = fabs(unhat) - fMbar8 * ((unhat + chat > 0.0 ? 1.0 : -1.0) * mx(0.0, UnR - UnL) -
+                                         (unhat - chat > 0.0 ? 1.0 : -1.0) * mx(0.0, UnR - UnL)) / 4.0;
+    double theta = 1.0;
+    double Umul = 0.5 * (1.0 + theta) * unhat;
+    double Cmul = 0.5 * sqrt(4.0 * chat * chat * theta + ((1.0 - theta) * (1.0 - theta)) * unhat * unhat);
+    double cpi3 = mn(fabs(L.P / R.P), fabs(R.P / L.P));
+    double s1 = pow(f_func(cpi3), 8.0);
+    double dUp0 = s1 * (1.0 - fMorg8) *
+                  (mx(0.0, Cmul - Und) +
+                   (1.0 - theta) * (Und - unhat * 0.5 / Cmul * (Umul > 0.0 ? 1.0 : -1.0) * mn(fabs(Umul), Cmul))) *
+                  (R.P - L.P) / (theta * chat * chat);
+    double xi_LR = 0.5 * (unhat > 0.0 ? 1.0 : -1.0) * mn(Und, chat) * (UnR - UnL) / chat;
+    double Mcy = sqrt(0.5 * (L.U * L.U + L.V * L.V + L.W * L.W + R.U * R.U + R.V * R.V + R.W * R.W)) / chat;
+    double phi_rer = 1.0 - s1 * (1.0 - f_func(Mcy));
+    double dP0 = phi_rer * mx(0.0, chat - Und) * rhohat * (UnR - UnL) +
+                 (unhat > 0.0 ? 1.0 : -1.0) * mn(Und, chat) * (R.P - L.P) / chat;
+    for (int k = 0; k < 4 + nsp; ++k) {
+        double dLR;
+        switch (k) {
+        case 0: dLR = xi_LR * (L.rho + R.rho); break;
+        case 1: dLR = xi_LR * (L.rho * L.U + R.rho * R.U); break;
+        case 2: dLR = xi_LR * (L.rho * L.V + R.rho * R.V); break;
+        case 3: dLR = xi_LR * (L.rho * L.W + R.rho * R.W); break;
+        case 4: dLR = xi_LR * (L.rho * L.Ht + R.rho * R.Ht); break;
+        default: dLR = xi_LR * (L.rho * L.Y[k - 5] + R.rho * R.Y[k - 5]); break;
+        }
+        double dP = k == 1 ? dP0 * n[0] : k == 2 ? dP0 * n[1] : k == 4 ? dP0 * n[2] : 0.0;
+        fl[k] = 0.5 * (adv(L, UnL, n, k) + adv(R, UnR, n, k)) -
+                0.5 * (xi * dW(L, R, k) + dP + dUk(a, L, R, dUp0, k) + dLR);
+    }
+}
+
+// calcFluxUpwind (flux.cpp:649-692)
+__device__ void flux_upwind(const FSide& L, const FSide& R, const double* n, int nsp, double* fl) {
+    double UnL = L.U * n[0] + L.V * n[1] + L.W * n[2];
+    double UnR = R.U * n[0] + R.V * n[1] + R.W * n[2];
+    double f1L, f1R;
+    if (UnL + UnR >= 0.0) { f1L = L.rho * 0.5 * (UnL + UnR); f1R = 0.0; }
+    else { f1L = 0.0; f1R = R.rho * 0.5 * (UnL + UnR); }
+    split_flux(L, R, n, f1L, f1R, 0.5 * (L.P + R.P), nsp, fl);
+}
+
+// the low-Mach scaling of calcFluxAUSMPWP_N / calcFluxRoeM_N
+struct Scale {
+    double phiP, phiU, aSDST;
+};
+__device__ inline Scale low_mach_scale(double uhat, double vhat, double what, double chat, double Uco, double dtstep,
+                                       double Lch) {
+    double Mco = Uco / chat;
+    double Mkhat = sqrt(uhat * uhat + vhat * vhat + what * what) / chat;
+    double Mun = Lch / 3.141592 / dtstep / chat;
+    double thetaTemp = mx(Mkhat, Mco);
+    double thetaP = mn(1.0, mx(thetaTemp, Mun));
+    double thetaU = mn(1.0, thetaTemp);
+    Scale s;
+    s.phiP = thetaP * (2.0 - thetaP);
+    s.phiU = thetaU * (2.0 - thetaU);
+    s.aSDST = 0.1875 * (-4.0 + 5.0 * s.phiU * s.phiU);
+    return s;
+}
+
+// calcFluxAUSMPWP_N (flux.cpp:707-787)
+__device__ void flux_ausmpwp_n(const FSide& L, const FSide& R, const double* n, int nsp, const double* par,
+                               double* fl) {
+    double w1 = par[0], w2 = par[1], fC = par[3], Uco = par[4], dtstep = par[5], Lch = par[6];
+    double UnL = L.U * n[0] + L.V * n[1] + L.W * n[2];
+    double UnR = R.U * n[0] + R.V * n[1] + R.W * n[2];
+    double RT = sqrt(R.rho / L.rho);
+    double rhohat = (L.rho + RT * R.rho) / (1.0 + RT);
+    double uhat = (L.U + RT * R.U) / (1.0 + RT);
+    double vhat = (L.V + RT * R.V) / (1.0 + RT);
+    double what = (L.W + RT * R.W) / (1.0 + RT);
+    double chat = fC;
+    Scale sc = low_mach_scale(uhat, vhat, what, chat, Uco, dtstep, Lch);
+    double ML = UnL / chat, MR = UnR / chat;
+    double MLP = M_func(ML, 1.0, 0.125), preP = pre_func(ML, 1.0, sc.aSDST);
+    double MRM = M_func(MR, -1.0, 0.125), preM = pre_func(MR, -1.0, sc.aSDST);
+    double rhoLR = L.rho;
+    double MLR = MLP + MRM;
+    if (MLR < 0.0) rhoLR = R.rho;
+    w1 = 1.0 - w1 * w1 * w1;
+    w2 = 1.0 - w2 * w2;
+    double w = mx(w1, w2);
+    double fL = L.P / (rhohat * chat * chat) * (1.0 - w) * rhohat / rhoLR / sc.phiP;
+    double fR = R.P / (rhohat * chat * chat) * (1.0 - w) * rhohat / rhoLR / sc.phiP;
+    double MBLP, MBLM;
+    if (MLR >= 0.0) { MBLP = MLP + MRM * ((1.0 - w) * (1.0 + fR) - fL); MBLM = MRM * w * (1.0 + fR); }
+    else { MBLP = MLP * w * (1.0 + fL); MBLM = MRM + MLP * ((1.0 - w) * (1.0 + fL) - fR); }
+    double f1L = chat * L.rho * MBLP, f1R = chat * R.rho * MBLM;
+    double Ku = 0.5;
+    double pu = -2.0 * Ku * preP * preM * rhohat * chat * (UnR - UnL);
+    double PLR = preP * L.P + preM * R.P + pu * sc.phiU;
+    split_flux(L, R, n, f1L, f1R, PLR, nsp, fl);
+}
+
+// calcFluxRoeM_N (flux.cpp:795-922)
+__device__ void flux_roem_n(const FSide& L, const FSide& R, const double* n, int nsp, const double* par,
+                            double* fl) {
+    double w1 = par[0], w3 = par[2], fC = par[3], Uco = par[4], dtstep = par[5], Lch = par[6];
+    double UnL = L.U * n[0] + L.V * n[1] + L.W * n[2];
+    double UnR = R.U * n[0] + R.V * n[1] + R.W * n[2];
+    double RT = sqrt(R.rho / L.rho);
+    double rhohat = (L.rho + RT * R.rho) / (1.0 + RT);
+    double uhat = (L.U + RT * R.U) / (1.0 + RT);
+    double vhat = (L.V + RT * R.V) / (1.0 + RT);
+    double what = (L.W + RT * R.W) / (1.0 + RT);
+    double Hthat = (L.Ht + RT * R.Ht) / (1.0 + RT);
+    double Unhat = uhat * n[0] + vhat * n[1] + what * n[2];
+    double chat = fC;
+    Scale sc = low_mach_scale(uhat, vhat, what, chat, Uco, dtstep, Lch);
+    double Mhat = Unhat / chat;
+    double h = 1.0 - w3;
+    double f = 1.0;
+    if (uhat * uhat + vhat * vhat + what * what != 0.0) f = pow(fabs(Mhat), h);
+    double g = 1.0;
+    if (Mhat != 0.0) g = pow(fabs(Mhat), 1.0 - w1);
+    double b1 = mx(Unhat + chat, UnR + chat); b1 = mx(b1, 0.0);
+    double b2 = mn(Unhat - chat, UnL - chat); b2 = mn(b2, 0.0);
+    double Mtilde = (Mhat > 0.0 ? 1.0 : -1.0) * mn(1.0, fabs(Mhat));
+    double b1v = mx(Unhat + chat * sc.phiU, UnR + chat * sc.phiU); b1v = mx(b1v, 0.0);
+    double b2v = mn(Unhat - chat * sc.phiU, UnL - chat * sc.phiU); b2v = mn(b2v, 0.0);
+    double Mvtilde = (Unhat / chat / sc.phiU > 0.0 ? 1.0 : -1.0) * mn(1.0, fabs(Unhat / chat / sc.phiU));
+    double B0 = R.rho - L.rho - f * (R.P - L.P) / chat / chat / sc.phiP;
+    for (int k = 0; k < 4 + nsp; ++k) {
+        double dWd, B1, B2;
+        switch (k) {
+        case 0: dWd = R.rho - L.rho; B1 = B0; B2 = 0.0; break;
+        case 1: dWd = R.rho * R.U - L.rho * L.U; B1 = B0 * uhat + rhohat * (R.U - L.U);
+                B2 = rhohat * (UnR - UnL) * (-n[0]); break;
+        case 2: dWd = R.rho * R.V - L.rho * L.V; B1 = B0 * vhat + rhohat * (R.V - L.V);
+                B2 = rhohat * (UnR - UnL) * (-n[1]); break;
+        case 3: dWd = R.rho * R.W - L.rho * L.W; B1 = B0 * what + rhohat * (R.W - L.W);
+                B2 = rhohat * (UnR - UnL) * (-n[2]); break;
+        case 4: dWd = R.rho * R.Ht - L.rho * L.Ht; B1 = B0 * Hthat + rhohat * (R.Ht - L.Ht); B2 = 0.0; break;
+        default: {
+            int i = k - 5;
+            dWd = R.rho * R.Y[i] - L.rho * L.Y[i];
+            B1 = B0 * ((L.Y[i] + RT * R.Y[i]) / (1.0 + RT)) + rhohat * (R.Y[i] - L.Y[i]);
+            B2 = 0.0;
+        }
+        }
+        fl[k] = (b1 * adv(L, UnL, n, k) - b2 * adv(R, UnR, n, k)) / (b1 - b2) +
+                b1 * b2 / (b1 - b2) * (dWd - g / (1.0 + fabs(Mtilde)) * B1) -
+                b1v * b2v / (b1v - b2v) * g / (1.0 + fabs(Mvtilde)) * B2;
+    }
+}
+
+// one thread per face: states [n][8 + nsp] (P, U, V, W, T, rho, C, Ht, Y),
+// normals [n][3], parameters [n][7] (w1, w2, w3, fC, Uco, dt, Lch; the two
+// schemes that take them), fluxes [n][4 + nsp]
+template <int SCHEME>
+__global__ void __launch_bounds__(TPB) k_flux(int n, int nsp, const double* __restrict__ Ls,
+                                              const double* __restrict__ Rs, const double* __restrict__ nrm,
+                                              const double* __restrict__ par, double* __restrict__ flux) {
+    int f = xcd_block() * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    int w = 8 + nsp;
+    const double* l = Ls + (size_t)f * w;
+    const double* r = Rs + (size_t)f * w;
+    FSide L{l[0], l[1], l[2], l[3], l[4], l[5], l[6], l[7], l + 8};
+    FSide R{r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r + 8};
+    const double* nv = nrm + 3 * (size_t)f;
+    double* fl = flux + (size_t)f * (4 + nsp);
+    switch (SCHEME) {
+    case SEMO_FLUX_HAUS: flux_haus(L, R, nv, nsp, fl); break;
+    case SEMO_FLUX_SLAU2: flux_slau2(L, R, nv, nsp, fl); break;
+    case SEMO_FLUX_ROE: flux_roe(L, R, nv, nsp, fl); break;
+    case SEMO_FLUX_APROE: flux_aproe(L, R, nv, nsp, fl); break;
+    case SEMO_FLUX_ROEAMTP: flux_roeamtp(L, R, nv, nsp, fl); break;
+    case SEMO_FLUX_UPWIND: flux_upwind(L, R, nv, nsp, fl); break;
+    case SEMO_FLUX_AUSMPWP_N: flux_ausmpwp_n(L, R, nv, nsp, par + 7 * (size_t)f, fl); break;
+    default: flux_roem_n(L, R, nv, nsp, par + 7 * (size_t)f, fl); break;
+    }
+}
+
+}  // namespace
+
+void flux_device(semo_ctx* c, int scheme, int n, int nsp, const double* L, const double* R, const double* nrm,
+                 const double* par, double* flux) {
+    dim3 g(nblk(n)), b(TPB);
+    double bytes = (double)n * (2.0 * (8 + nsp) + 3 + 7 + 4 + nsp) * 8.0;
+    switch (scheme) {
+    case SEMO_FLUX_HAUS: SEMO_LAUNCH(c, "flux", bytes, k_flux<SEMO_FLUX_HAUS>, g, b, n, nsp, L, R, nrm, par, flux); break;
+    case SEMO_FLUX_SLAU2: SEMO_LAUNCH(c, "flux", bytes, k_flux<SEMO_FLUX_SLAU2>, g, b, n, nsp, L, R, nrm, par, flux); break;
+    case SEMO_FLUX_ROE: SEMO_LAUNCH(c, "flux", bytes, k_flux<SEMO_FLUX_ROE>, g, b, n, nsp, L, R, nrm, par, flux); break;
+    case SEMO_FLUX_APROE: SEMO_LAUNCH(c, "flux", bytes, k_flux<SEMO_FLUX_APROE>, g, b, n, nsp, L, R, nrm, par, flux); break;
+    case SEMO_FLUX_ROEAMTP:
+        SEMO_LAUNCH(c, "flux", bytes, k_flux<SEMO_FLUX_ROEAMTP>, g, b, n, nsp, L, R, nrm, par, flux); break;
+    case SEMO_FLUX_UPWIND: SEMO_LAUNCH(c, "flux", bytes, k_flux<SEMO_FLUX_UPWIND>, g, b, n, nsp, L, R, nrm, par, flux); break;
+    case SEMO_FLUX_AUSMPWP_N:
+        SEMO_LAUNCH(c, "flux", bytes, k_flux<SEMO_FLUX_AUSMPWP_N>, g, b, n, nsp, L, R, nrm, par, flux); break;
+    default: SEMO_LAUNCH(c, "flux", bytes, k_flux<SEMO_FLUX_ROEM_N>, g, b, n, nsp, L, R, nrm, par, flux); break;
+    }
+}
+
+}  // namespace semo

@@ -317,6 +317,25 @@ class GpuSolver:
         self._ok(self._L.semo_les(self._h, self.LES_MODELS[model], *[_ptr(a) for a in arr], _ptr(mu), _ptr(k)))
         return mu, k
 
+    FLUX_SCHEMES = {"HAUS": 0, "SLAU2": 1, "Roe": 2, "APRoe": 3, "RoeAMTP": 4, "Upwind": 5, "AUSMPWP_N": 6,
+                    "RoeM_N": 7}
+
+    def calcFlux(self, scheme, left, right, normal, par=None):
+        """The convective flux of SEMO's scheme calcFlux<scheme> (solvers/
+        flux.cpp) per face: left / right [n, 8 + nsp] = (P, U, V, W, T, rho,
+        C, Ht, Y...), normal [n, 3], par [n, 7] = (w1, w2, w3, fC, Uco, dt,
+        Lch) for AUSMPWP_N and RoeM_N; returns [n, 4 + nsp]."""
+        L = np.ascontiguousarray(left, dtype=np.float64)
+        R = np.ascontiguousarray(right, dtype=np.float64)
+        nv = np.ascontiguousarray(normal, dtype=np.float64)
+        n, w = L.shape
+        nsp = w - 8
+        pa = None if par is None else np.ascontiguousarray(par, dtype=np.float64)
+        out = np.empty((n, 4 + nsp))
+        self._ok(self._L.semo_flux(self._h, self.FLUX_SCHEMES[scheme], n, nsp, _ptr(L), _ptr(R), _ptr(nv),
+                                   None if pa is None else _ptr(pa), _ptr(out)))
+        return out
+
     def setCellVarMinMax(self, phi):
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         mx = np.empty(self.mesh.n_cells); mn = np.empty(self.mesh.n_cells)

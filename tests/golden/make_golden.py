@@ -264,6 +264,78 @@ def _les_runs():
     run_case("lesrun_dyn_vof_tet433_2x1", m, c, 2, 1, vof_init(m, 63), keep_calls=False)
 
 
+def flux_states(n=400, nsp=2, seed=71):
+    """Face states for the convective flux schemes: [n][rec] with rec =
+    2 (8 + nsp) + 10 -- left (P, U, V, W, T, rho, C, Ht, Y...), right, the
+    unit normal, then w1, w2, w3, fC, Uco, dt, Lch.  Gas- and liquid-like
+    sides, Mach numbers from 0 to about 3, pressure ratios on both sides of
+    3/4; the last rows are special cases: identical sides, both at rest,
+    nearly equal velocities (|dU| < 1e-5) and supersonic sides."""
+    rng = np.random.default_rng(seed)
+    rec = 2 * (8 + nsp) + 10
+    out = np.zeros((n, rec))
+
+    def side(k):
+        liquid = rng.random(k) < 0.4
+        rho = np.where(liquid, 900 + 200 * rng.random(k), 0.8 + 0.8 * rng.random(k))
+        C = np.where(liquid, 1200 + 600 * rng.random(k), 300 + 80 * rng.random(k))
+        mach = 3.0 * rng.random(k) ** 2
+        d = rng.normal(size=(k, 3)); d /= np.linalg.norm(d, axis=1)[:, None]
+        U = d * (mach * C)[:, None]
+        P = 1e5 * (0.5 + 1.5 * rng.random(k))
+        T = 280 + 40 * rng.random(k)
+        Ht = C * C / 0.4 + 0.5 * (U * U).sum(axis=1)
+        Y = rng.random((k, nsp)); Y /= Y.sum(axis=1)[:, None]
+        return np.column_stack([P, U, T, rho, C, Ht, Y])
+
+    L, R = side(n), side(n)
+    out[:, :8 + nsp] = L
+    out[:, 8 + nsp:2 * (8 + nsp)] = R
+    nv = rng.normal(size=(n, 3)); nv /= np.linalg.norm(nv, axis=1)[:, None]
+    o = 2 * (8 + nsp)
+    out[:, o:o + 3] = nv
+    out[:, o + 3:o + 6] = 0.05 + 0.95 * rng.random((n, 3))          # w1, w2, w3
+    out[:, o + 6] = 0.5 * (L[:, 6] + R[:, 6])                        # fC
+    out[:, o + 7] = 1.0 + rng.random(n)                              # Uco
+    out[:, o + 8] = 1e-4 * (1.0 + rng.random(n))                     # dt
+    out[:, o + 9] = 0.05 + 0.1 * rng.random(n)                       # Lch
+    m = n - 40
+    out[m:m + 10, 8 + nsp:2 * (8 + nsp)] = out[m:m + 10, :8 + nsp]  # identical sides
+    out[m + 10:m + 20, 1:4] = 0.0                                    # both at rest
+    out[m + 10:m + 20, 8 + nsp + 1:8 + nsp + 4] = 0.0
+    out[m + 20:m + 30, 8 + nsp + 1:8 + nsp + 4] = (out[m + 20:m + 30, 1:4]
+                                                  + 1e-7 * rng.normal(size=(10, 3)))
+    for s0 in (0, 8 + nsp):                                          # supersonic sides
+        u = out[m + 30:m + 40, s0 + 1:s0 + 4]
+        out[m + 30:m + 40, s0 + 1:s0 + 4] = u / np.linalg.norm(u, axis=1)[:, None] * (
+            2.0 * out[m + 30:m + 40, s0 + 6])[:, None]
+    return out
+
+
+def _flux():
+    """SEMO_Solvers_Builder::calcFluxHAUS, SLAU2, Roe, APRoe, RoeAMTP, Upwind,
+    AUSMPWP_N and RoeM_N (solvers/flux.cpp) on the face states above."""
+    nsp = 2
+    st = flux_states(nsp=nsp)
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "faces.bin")
+        with open(fn, "wb") as fh:
+            np.array([st.shape[0], nsp], dtype=np.int32).tofile(fh)
+            st.tofile(fh)
+        out = os.path.join(d, "out")
+        r = subprocess.run([REF, "-", "-", out, "flux", fn], capture_output=True, text=True, timeout=120, cwd=d)
+        if "SEMO_REF_DONE" not in r.stdout:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        arrs = {"flux_states": st, "flux_nsp": np.array(nsp)}
+        for fnm in sorted(os.listdir(out)):
+            base, ext = os.path.splitext(fnm)
+            if ext == ".f64":
+                arrs[base] = np.fromfile(os.path.join(out, fnm), dtype=np.float64).reshape(st.shape[0], -1)
+    path = os.path.join(HERE, "flux_faces.npz")
+    np.savez_compressed(path, **arrs)
+    print(f"flux_faces: {len(arrs)} arrays, {os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def _poly20():
     """Three stacked prisms over a jittered 20-gon: internal faces and caps
     of 20 points (above the 16 the device keeps in registers), lid-driven
@@ -395,7 +467,7 @@ CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box,
          "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof,
          "poly20_stack3": _poly20, "vtu_tet221": _vtu, "initial_cube6": _initial, "les": _les,
-         "lesrun": _les_runs}
+         "lesrun": _les_runs, "flux": _flux}
 
 
 def main(names):
