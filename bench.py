@@ -329,6 +329,130 @@ def coupled_leg(nx=256, ny=64, nz=64, n_iter=10, reps=2):
             "mean_u_over_inlet": round(u_mean / u_in, 4)}
 
 
+def linsol_inputs(n_cells, nsp, h, seed=20261203):
+    """Seeded density-based cell states [n][12 + 3 nsp] and residuals
+    [n][4 + nsp] for SEMO's implicit updates: ideal-gas air cells (R 287,
+    gamma 1.4) and a third liquid-like cells (rho ~ 1000, C ~ 1400), Mach up
+    to 0.8, Ur = |U| clipped to [0.01 C, C], dtPseudo at a pseudo CFL of 1 to
+    5 on the cell size h; residuals scaled per equation."""
+    rng = np.random.default_rng(seed)
+    n = n_cells
+    liquid = rng.random(n) < 0.35
+    P = 1e5 * (0.8 + 0.4 * rng.random(n))
+    T = 280.0 + 40.0 * rng.random(n)
+    Rg, gam = 287.0, 1.4
+    cp = gam * Rg / (gam - 1.0)
+    rho = np.where(liquid, 1000.0 * (1.0 + 0.02 * rng.random(n)), P / (Rg * T))
+    C = np.where(liquid, 1300.0 + 200.0 * rng.random(n), np.sqrt(gam * Rg * T))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1)[:, None]
+    u = d * (0.8 * rng.random(n) * C)[:, None]
+    um = np.linalg.norm(u, axis=1)
+    st = np.column_stack([P, u, T, rho, C, np.where(liquid, 4180.0, cp) * T + 0.5 * um * um,
+                          np.where(liquid, 1.0 / (C * C), 1.0 / (Rg * T)),
+                          np.where(liquid, 1e-3 * (0.5 + rng.random(n)), 0.0),
+                          np.where(liquid, -0.2 - 0.2 * rng.random(n), -P / (Rg * T * T)),
+                          np.where(liquid, 4180.0, cp), np.clip(um, 0.01 * C, C),
+                          (1.0 + 4.0 * rng.random(n)) * h / (um + C)])
+    Y = rng.random((n, nsp)); Y /= Y.sum(axis=1)[:, None]
+    st = np.column_stack([st, Y, rho[:, None] * (rng.random((n, nsp - 1)) * 4.0 - 2.0),
+                          1e4 * (rng.random((n, nsp - 1)) * 2.0 - 1.0)])
+    scale = np.array([1.0, 100.0, 100.0, 100.0, 1e5] + [0.1] * (nsp - 1))
+    return st, rng.normal(size=(n, 4 + nsp)) * scale
+
+
+def lusgs_leg(n=128, nsp=2, reps=5, cpu_n=48):
+    """SEMO's density-based implicit updates (solvers/linearSolver.cpp) on
+    the jittered n^3 cavity mesh with seeded gas / liquid-like states and two
+    species (6 equations): LU-SGS (calcLinearSolverLUSGS, forward and
+    backward sweeps in lib_RCM's order, level-scheduled) and the
+    point-implicit update (calcLinearSolver), state and residuals resident
+    on the device (semo_linear_solver_device); the compiled reference's
+    calcLinearSolverLUSGS on a jittered cpu_n^3 sample, one core."""
+    import ctypes as C
+    from semo_mesh_amd.mesh import cavity_mesh
+    from semo_mesh_amd.solver import GpuSolver
+    m = cavity_mesh(n, jitter=0.1)
+    nc = m.n_cells
+    st, r0 = linsol_inputs(nc, nsp, 1.0 / n)
+    g = GpuSolver(0)
+    g.upload_mesh(m, reorder=True)
+    perm, _ = g.permutation()
+    hip = C.CDLL("libamdhip64.so.7")
+    bufs = []
+
+    def put(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(a.nbytes)) == 0
+        assert hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) == 0
+        bufs.append(p)
+        return p
+
+    ds, dr0, dr = put(st[perm]), put(r0[perm]), put(r0[perm])
+    nbytes = C.c_size_t(r0.nbytes)
+    out = {"workload": f"density-based implicit updates, jittered {n}^3 hex ({nc} cells), 2 species "
+                       f"(6 equations), seeded gas / liquid-like states, state and residuals resident"}
+    try:
+        for method, key in ((2, "lusgs"), (0, "point")):
+            def call():
+                assert hip.hipMemcpy(dr, dr0, nbytes, 3) == 0
+                assert hip.hipDeviceSynchronize() == 0
+                t0 = time.perf_counter()
+                g._ok(g._L.semo_linear_solver_device(g._h, method, nsp, 1e-3, ds, dr))
+                return time.perf_counter() - t0
+            first = call()
+            times = [call() for _ in range(reps)]
+            g.profile(True); g.profile_reset()
+            call()
+            prof = g.profile_stats()
+            g.profile(False)
+            t = min(times)
+            leg = {"ms": round(1e3 * t, 3), "Mcell_per_s": round(nc / t / 1e6, 2),
+                   "first_call_ms": round(1e3 * first, 1),
+                   "kernels": {k: {"launches": v["count"], "ms": round(v["ms"], 3),
+                                   "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                               for k, v in prof.items()}}
+            if key == "lusgs":
+                leg["sweep_levels"] = prof.get("lusgs_sweep", {}).get("count")
+                leg["note"] = ("first call levels the sweeps on the host (once per mesh); the level "
+                               "kernels are latency-bound: one launch per level")
+            out[key] = leg
+        hout = np.empty_like(r0)
+        assert hip.hipMemcpy(hout.ctypes.data_as(C.c_void_p), dr, nbytes, 2) == 0
+        out["point_finite"] = bool(np.isfinite(hout).all())
+    finally:
+        for p in bufs:
+            hip.hipFree(p)
+        g.close()
+    ref = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
+    if os.path.exists(ref):
+        mc = cavity_mesh(cpu_n, jitter=0.1)
+        sc, rc = linsol_inputs(mc.n_cells, nsp, 1.0 / cpu_n)
+        from semo_mesh_amd.cases import cavity_case
+        with tempfile.TemporaryDirectory() as d:
+            mc.write_raw(os.path.join(d, "mesh.raw"))
+            cavity_case().write_ref_case(os.path.join(d, "case.txt"), steps=0, correctors=1)
+            fn = os.path.join(d, "state.f64")
+            with open(fn, "wb") as fh:
+                sc.tofile(fh)
+                rc.tofile(fh)
+            with open(os.path.join(d, "case.txt"), "a") as fh:
+                fh.write(f"linsol {fn} 0.001\n")
+            r = subprocess.run([ref, os.path.join(d, "mesh.raw"), os.path.join(d, "case.txt"),
+                                os.path.join(d, "out"), "nodump"], capture_output=True, text=True, timeout=600)
+            line = [x for x in r.stdout.splitlines() if x.startswith("SEMO_REF_DONE linsol")]
+            if line:
+                kv = dict(tok.split("=") for tok in line[0].split()[2:])
+                out["cpu_reference"] = {
+                    "lusgs_Mcell_per_s": round(mc.n_cells / float(kv["lusgs_s"]) / 1e6, 4),
+                    "point_Mcell_per_s": round(mc.n_cells / float(kv["point_s"]) / 1e6, 4),
+                    "cores": 1, "kind": "reference",
+                    "sample": f"compiled SEMO calcLinearSolverLUSGS / calcLinearSolver on a jittered {cpu_n}^3 "
+                              f"cavity ({mc.n_cells} cells), one call each: {float(kv['lusgs_s']):.2f} s / "
+                              f"{float(kv['point_s']):.2f} s"}
+    return out
+
+
 def cpu_baseline(n=48):
     """The compiled reference (SEMO + AMGCL, 1 MPI rank, no OpenMP) on a
     bounded sample: one SIMPLE iteration of the same cavity at n^3."""
@@ -377,6 +501,7 @@ def main():
     ap.add_argument("--vof-steps", type=int, default=50)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint leg")
     ap.add_argument("--no-coupled", action="store_true", help="skip the block-coupled (BSR4) leg")
+    ap.add_argument("--no-lusgs", action="store_true", help="skip the density-based LU-SGS leg")
     ap.add_argument("--cheb-degree", type=int, default=1, help="Chebyshev smoother degree of the pressure AMG")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the library's streaming copy of MIB MiB (read MIB, write MIB; "
@@ -503,6 +628,9 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
         print(f"[bench] adjoint: {out['adjoint']['ms_per_gradient']} ms per gradient", file=sys.stderr, flush=True)
+    if not args.no_lusgs:
+        out["lusgs"] = lusgs_leg()
+        print(f"[bench] lusgs: {out['lusgs']['lusgs']['ms']} ms per update", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -355,6 +355,34 @@ enum { SEMO_FLUX_HAUS = 0, SEMO_FLUX_SLAU2, SEMO_FLUX_ROE, SEMO_FLUX_APROE, SEMO
        SEMO_FLUX_UPWIND, SEMO_FLUX_AUSMPWP_N, SEMO_FLUX_ROEM_N };
 int semo_flux(semo_ctx* ctx, int scheme, int64_t n, int nsp, const double* left, const double* right,
               const double* normal, const double* par, double* flux);
+/* Implicit updates of SEMO's density-based dual-time solvers
+ * (src/solvers/linearSolver.cpp), on the uploaded mesh:
+ *   SEMO_LINSOL_POINT  calcLinearSolver (:666-725): per cell,
+ *                      res <- (P + 1.5 dtPseudo / time_step T)^-1 res;
+ *   SEMO_LINSOL_SINGLE calcSingleLinearSolver (:729-787): res <- T^-1 res;
+ *   SEMO_LINSOL_LUSGS  calcLinearSolverLUSGS (:144-389): forward then
+ *                      backward sweep in lib_RCM's cell order (RCM.cpp:34-
+ *                      210) with the faces' upwind flux Jacobians
+ *                      (getNumericalFluxJacobianUpwind, :394-640).
+ * P and T are constructMatrix's (:791-890); blocks are inverted as
+ * GaussSeidelSOR does (src/math/math.cpp:141-200, no pivoting).
+ * state[n][12 + 3 nsp] per cell: P, U, V, W, T, rho, C, Ht, dRhoDP, dHtDP,
+ * dRhoDT, dHtDT, Ur, dtPseudo, Y_0..Y_{nsp-1}, dRhoDY_0..dRhoDY_{nsp-2},
+ * dHtDY_0..dHtDY_{nsp-2}; residuals[n][4 + nsp] in/out (the calcRHS
+ * residuals in, the update out).  1 <= nsp <= 4; time_step is controls.
+ * timeStep (not read by SEMO_LINSOL_SINGLE).  The LU-SGS order is the RCM
+ * of the mesh as the caller numbers it, whether or not the upload reordered
+ * the cells; the first LU-SGS call on a mesh levels the sweeps on the host. */
+enum { SEMO_LINSOL_POINT = 0, SEMO_LINSOL_SINGLE = 1, SEMO_LINSOL_LUSGS = 2 };
+int semo_linear_solver(semo_ctx* ctx, int method, int nsp, double time_step, const double* state,
+                       double* residuals);
+/* The same on device arrays in the context's cell numbering (the device
+ * order semo_mesh_get_perm reports), on the context's stream, synchronised;
+ * d_state and d_residuals as above.  No counterpart in the reference (it
+ * keeps the state in the device's memory between the calls of a solver
+ * loop).                                                                     */
+int semo_linear_solver_device(semo_ctx* ctx, int method, int nsp, double time_step, const double* d_state,
+                              double* d_residuals);
 /* turbType / turbLESModel of constant/turbulenceProperties (src/controls/
  * build.cpp:100-115): model 0 = laminar (the default), else an SEMO_LES_*
  * model, which every calcCellTransport of the time step then applies

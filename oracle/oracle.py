@@ -58,6 +58,7 @@ def lib():
         L.orc_gauss_green.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
         L.orc_les.argtypes = [C.c_void_p, C.c_int] + [_D] * 9
         L.orc_set_les.argtypes = [C.c_void_p, C.c_int]
+        L.orc_linear_solver.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _D, _D]
         L.orc_solve.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.orc_momentum_update.restype = C.c_double
         L.orc_momentum_update.argtypes = [C.c_void_p]
@@ -163,6 +164,18 @@ class Oracle:
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (U, V, W, rho, face_rU, face_rV, face_rW)]
         lib().orc_les(self._h, model, *a, mu, k)
         return mu, k
+
+    def linear_solver(self, method, nsp, time_step, state, res):
+        """SEMO's density-based implicit updates: method "point"
+        (calcLinearSolver), "single" (calcSingleLinearSolver) or "lusgs"
+        (calcLinearSolverLUSGS, in lib_RCM's order).  state [nc][12 + 3 nsp],
+        res [nc][4 + nsp]; returns the updated residuals."""
+        m = {"point": 0, "single": 1, "lusgs": 2}[method]
+        r = np.ascontiguousarray(res, dtype=np.float64).copy()
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        if lib().orc_linear_solver(self._h, m, nsp, time_step, st, r) != 0:
+            raise ValueError("linear_solver: bad method or nsp")
+        return r
 
     def minmax(self, phi):
         mx = np.empty(self.mesh.n_cells); mn = np.empty(self.mesh.n_cells)

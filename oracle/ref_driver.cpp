@@ -264,6 +264,9 @@ int main(int argc, char** argv) {
     int nsteps = 1, ncorr = 1, nvof = 0;
     int les = 0;          // "les 1": the three LES models on the final state
     string turb;          // "turb MODEL": turbType LES with turbLESModel MODEL
+    string linsol;        // "linsol FILE DT": the density-based implicit updates on FILE's state
+    double linsol_dt = 0.0;
+    int nspecies = 2;     // "nspecies K": K species (the third and later ones take the second's properties)
     vector<PatchBC> bcs;
     string initfile;
     {
@@ -285,6 +288,8 @@ int main(int argc, char** argv) {
             else if (key == "init") cf >> initfile;
             else if (key == "les") cf >> les;
             else if (key == "turb") cf >> turb;
+            else if (key == "linsol") cf >> linsol >> linsol_dt;
+            else if (key == "nspecies") cf >> nspecies;
             else if (key == "bc") {
                 PatchBC b;
                 cf >> b.name >> b.ptype >> b.pval >> b.utype >> b.u[0] >> b.u[1] >> b.u[2] >> b.vftype >> b.vfval;
@@ -293,10 +298,12 @@ int main(int argc, char** argv) {
         }
     }
 
-    vector<SEMO_Species> species(2);
-    species[0].name = "water"; species[1].name = "air";
-    for (int s = 0; s < 2; ++s) {
-        species[s].rhoType = "const"; species[s].rho = rho[s]; species[s].mu = mu[s];
+    if (nspecies < 1 || nspecies > 8) { fprintf(stderr, "nspecies 1..8\n"); return 2; }
+    vector<SEMO_Species> species(nspecies);
+    for (int s = 0; s < nspecies; ++s) species[s].name = s == 0 ? "water" : s == 1 ? "air" : "gas" + std::to_string(s);
+    for (int s = 0; s < nspecies; ++s) {
+        int q = std::min(s, 1);
+        species[s].rhoType = "const"; species[s].rho = rho[q]; species[s].mu = mu[q];
         species[s].sigma = sigma; species[s].phase = "liquid";
     }
     SEMO_Controls_Builder controls;
@@ -416,8 +423,12 @@ int main(int argc, char** argv) {
         cl.var[controls.P] = init[i]; cl.var[controls.U] = init[nc + i];
         cl.var[controls.V] = init[2 * nc + i]; cl.var[controls.W] = init[3 * nc + i];
         cl.var[controls.T] = 300.0;
-        cl.var[controls.VF[0]] = init[4 * nc + i]; cl.var[controls.VF[1]] = 1.0 - init[4 * nc + i];
-        cl.var[controls.MF[0]] = init[4 * nc + i]; cl.var[controls.MF[1]] = 1.0 - init[4 * nc + i];
+        cl.var[controls.VF[0]] = init[4 * nc + i];
+        cl.var[controls.MF[0]] = init[4 * nc + i];
+        if (nspecies > 1) {
+            cl.var[controls.VF[1]] = 1.0 - init[4 * nc + i];
+            cl.var[controls.MF[1]] = 1.0 - init[4 * nc + i];
+        }
     }
 
     if (mode == "savevtu") {
@@ -483,6 +494,56 @@ int main(int argc, char** argv) {
         dump("cell_volume", vol); dump("cell_xyz", cxyz); dump("cell_lsq", lsq);
         dump("stencil_ptr", sptr); dump("stencil_idx", sidx);
         dump("rcm_perm", perm); dump("rcm_inv", inv);
+    }
+
+    if (!linsol.empty()) {
+        // calcLinearSolver, calcSingleLinearSolver and calcLinearSolverLUSGS
+        // (solvers/linearSolver.cpp) on the cell state and residuals of FILE:
+        // [nc][12 + 3 nSp] state records (P, U, V, W, T, rho, C, Ht, dRhoDP,
+        // dHtDP, dRhoDT, dHtDT, Ur, dtPseudo, Y.., dRhoDY.., dHtDY..), then
+        // [nc][nEq] residuals, in the mesh's cell order
+        const int nsp = controls.nSp, N = controls.nEq, K = 12 + 3 * nsp;
+        vector<double> st(nc * K), r0(nc * N);
+        FILE* f = fopen(linsol.c_str(), "rb");
+        if (!f) { perror(linsol.c_str()); return 2; }
+        rd(f, st.data(), st.size());
+        rd(f, r0.data(), r0.size());
+        fclose(f);
+        controls.timeStep = linsol_dt;
+        for (size_t i = 0; i < nc; ++i) {
+            auto& v = mesh.cells[i].var;
+            const double* s = &st[i * K];
+            v[controls.P] = s[0]; v[controls.U] = s[1]; v[controls.V] = s[2]; v[controls.W] = s[3];
+            v[controls.T] = s[4]; v[controls.Rho] = s[5]; v[controls.C] = s[6]; v[controls.Ht] = s[7];
+            v[controls.dRhoDP] = s[8]; v[controls.dHtDP] = s[9]; v[controls.dRhoDT] = s[10];
+            v[controls.dHtDT] = s[11]; v[controls.Ur] = s[12]; v[controls.dtPseudo] = s[13];
+            for (int a = 0; a < nsp; ++a) v[controls.MF[a]] = s[14 + a];
+            for (int a = 0; a < nsp - 1; ++a) {
+                v[controls.dRhoDMF[a]] = s[14 + nsp + a];
+                v[controls.dHtDMF[a]] = s[13 + 2 * nsp + a];
+            }
+        }
+        const char* names[3] = {"linsol_point", "linsol_single", "linsol_lusgs"};
+        double secs[3];
+        for (int m = 0; m < 3; ++m) {
+            vector<vector<double>> res(nc, vector<double>(N));
+            for (size_t i = 0; i < nc; ++i)
+                for (int k = 0; k < N; ++k) res[i][k] = r0[i * N + k];
+            auto ta = std::chrono::steady_clock::now();
+            if (m == 0) solvers.calcLinearSolver(mesh, controls, res);
+            else if (m == 1) solvers.calcSingleLinearSolver(mesh, controls, res);
+            else solvers.calcLinearSolverLUSGS(mesh, controls, res);
+            secs[m] = std::chrono::duration<double>(std::chrono::steady_clock::now() - ta).count();
+            vector<double> out(nc * N);
+            for (size_t i = 0; i < nc; ++i)
+                for (int k = 0; k < N; ++k) out[i * N + k] = res[i][k];
+            dump(names[m], out);
+        }
+        printf("SEMO_REF_DONE linsol point_s=%.6f single_s=%.6f lusgs_s=%.6f cells=%zu\n", secs[0], secs[1],
+               secs[2], nc);
+        fflush(stdout);
+        MPI_Finalize();
+        return 0;
     }
 
     solvers.calcCellTransport(mesh, controls, species);

@@ -2048,3 +2048,254 @@ void orc_cell_graph(void* h, int* ptr, int* adj) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------
+// Density-based implicit updates (src/solvers/linearSolver.cpp).  A cell's
+// state record holds, in order: P, U, V, W, T, rho, C, Ht, dRhoDP, dHtDP,
+// dRhoDT, dHtDT, Ur, dtPseudo, Y_0..Y_{nsp-1}, dRhoDY_0..dRhoDY_{nsp-2},
+// dHtDY_0..dHtDY_{nsp-2}: 12 + 3 nsp values.  nEq = 4 + nsp equations
+// (controls/build.cpp:640).  Matrices are N x N row-major.
+namespace db {
+enum { P = 0, U, V, W, T, RHO, C, HT, DRDP, DHDP, DRDT, DHDT, UR, DTP, Y0 };
+inline int rec(int nsp) { return 12 + 3 * nsp; }
+inline double Y(const double* s, int k) { return s[Y0 + k]; }
+inline double dRdY(const double* s, int nsp, int k) { return s[Y0 + nsp + k]; }
+inline double dHdY(const double* s, int nsp, int k) { return s[Y0 + 2 * nsp - 1 + k]; }
+
+// constructMatrix (linearSolver.cpp:791-890): the conservative-to-primitive
+// Jacobian T and its preconditioned form P (column 0 with beta).
+void construct(const double* s, int nsp, double* Pm, double* Tm) {
+    const int N = 4 + nsp;
+    std::fill(Tm, Tm + N * N, 0.0);
+    auto t = [&](int i, int j) -> double& { return Tm[i * N + j]; };
+    const double DRDP_ = s[DRDP], DHDP_ = s[DHDP], DRDT_ = s[DRDT], DHDT_ = s[DHDT];
+    const double Rho = s[RHO], U_ = s[U], V_ = s[V], W_ = s[W], Ht = s[HT];
+    t(0, 0) = DRDP_; t(1, 0) = DRDP_ * U_; t(2, 0) = DRDP_ * V_; t(3, 0) = DRDP_ * W_;
+    t(4, 0) = DRDP_ * Ht + Rho * DHDP_ - 1.0;
+    t(1, 1) = Rho; t(4, 1) = Rho * U_;
+    t(2, 2) = Rho; t(4, 2) = Rho * V_;
+    t(3, 3) = Rho; t(4, 3) = Rho * W_;
+    t(0, 4) = DRDT_; t(1, 4) = DRDT_ * U_; t(2, 4) = DRDT_ * V_; t(3, 4) = DRDT_ * W_;
+    t(4, 4) = DRDT_ * Ht + Rho * DHDT_;
+    for (int a = 0; a < nsp - 1; ++a) {
+        int q = 5 + a;
+        t(q, 0) = DRDP_ * Y(s, a);
+        t(q, 4) = DRDT_ * Y(s, a);
+        for (int b = 0; b < nsp - 1; ++b) t(q, 5 + b) = Y(s, a) * dRdY(s, nsp, b);
+        t(q, q) += Rho;
+        t(0, q) = dRdY(s, nsp, a); t(1, q) = dRdY(s, nsp, a) * U_; t(2, q) = dRdY(s, nsp, a) * V_;
+        t(3, q) = dRdY(s, nsp, a) * W_;
+        t(4, q) = dRdY(s, nsp, a) * Ht + Rho * dHdY(s, nsp, a);
+    }
+    std::copy(Tm, Tm + N * N, Pm);
+    double beta = 1.0 / std::pow(s[UR], 2.0) + DRDP_ - 1.0 / std::pow(s[C], 2.0);
+    Pm[0] = beta; Pm[N] = beta * U_; Pm[2 * N] = beta * V_; Pm[3 * N] = beta * W_;
+    Pm[4 * N] = beta * Ht + Rho * DHDP_ - 1.0;
+    for (int a = 0; a < nsp - 1; ++a) Pm[(5 + a) * N] = beta * Y(s, a);
+}
+
+// getNumericalFluxJacobianUpwind (linearSolver.cpp:394-640): 0.5 dF/dQ of the
+// chosen side (right = the "R" side) +/- 0.5 of a Roe-type dissipation matrix
+// built from the face's averages; the species rows carry no dissipation.
+void jacobian(const double* L, const double* R, int nsp, const double* n, bool right, double* J) {
+    const int N = 4 + nsp;
+    double UnL = L[U] * n[0] + L[V] * n[1] + L[W] * n[2];
+    double UnR = R[U] * n[0] + R[V] * n[1] + R[W] * n[2];
+    double RT = std::sqrt(L[RHO] / R[RHO]);
+    double Rhohat = 0.5 * (L[RHO] + R[RHO]);
+    double Chat = 0.5 * (L[C] + R[C]);
+    double Unhat = 0.5 * (UnL + UnR);
+    double Uhat = (RT * L[U] + R[U]) / (RT + 1.0);
+    double Vhat = (RT * L[V] + R[V]) / (RT + 1.0);
+    double What = (RT * L[W] + R[W]) / (RT + 1.0);
+    double Hthat = (RT * L[HT] + R[HT]) / (RT + 1.0);
+    double fgL = right ? 0.0 : 1.0, fgR = right ? 1.0 : 0.0;
+    auto lr = [&](int k) { return L[k] * fgL + R[k] * fgR; };
+    double ULR = lr(U), VLR = lr(V), WLR = lr(W), HtLR = lr(HT), RhoLR = lr(RHO);
+    double DRDPLR = lr(DRDP), DHDPLR = lr(DHDP), DRDTLR = lr(DRDT), DHDTLR = lr(DHDT);
+    double UnLR = UnL * fgL + UnR * fgR;
+    vector<double> F(N * N, 0.0), D(N * N, 0.0), YLR(nsp), DRDYLR(nsp), DHDYLR(nsp);
+    for (int a = 0; a < nsp - 1; ++a) {
+        YLR[a] = Y(L, a) * fgL + Y(R, a) * fgR;
+        DRDYLR[a] = dRdY(L, nsp, a) * fgL + dRdY(R, nsp, a) * fgR;
+        DHDYLR[a] = dHdY(L, nsp, a) * fgL + dHdY(R, nsp, a) * fgR;
+    }
+    auto f = [&](int i, int j) -> double& { return F[i * N + j]; };
+    f(0, 0) = DRDPLR * UnLR;
+    f(1, 0) = DRDPLR * UnLR * ULR + n[0];
+    f(2, 0) = DRDPLR * UnLR * VLR + n[1];
+    f(3, 0) = DRDPLR * UnLR * WLR + n[2];
+    f(4, 0) = DRDPLR * UnLR * HtLR + RhoLR * UnLR * DHDPLR;
+    for (int d = 0; d < 3; ++d) {
+        double vel[3] = {ULR, VLR, WLR};
+        f(0, 1 + d) = RhoLR * n[d];
+        for (int r = 0; r < 3; ++r) f(1 + r, 1 + d) = RhoLR * n[d] * vel[r];
+        f(1 + d, 1 + d) = f(1 + d, 1 + d) + RhoLR * UnLR;
+        f(4, 1 + d) = RhoLR * n[d] * HtLR + RhoLR * UnLR * vel[d];
+    }
+    f(0, 4) = DRDTLR * UnLR;
+    f(1, 4) = DRDTLR * UnLR * ULR;
+    f(2, 4) = DRDTLR * UnLR * VLR;
+    f(3, 4) = DRDTLR * UnLR * WLR;
+    f(4, 4) = DRDTLR * UnLR * HtLR + RhoLR * UnLR * DHDTLR;
+    for (int a = 0; a < nsp - 1; ++a) {
+        int q = 5 + a;
+        f(q, 0) = DRDPLR * YLR[a] * UnLR;
+        f(q, 1) = RhoLR * YLR[a] * n[0];
+        f(q, 2) = RhoLR * YLR[a] * n[1];
+        f(q, 3) = RhoLR * YLR[a] * n[2];
+        f(q, 4) = DRDTLR * YLR[a] * UnLR;
+        for (int b = 0; b < nsp - 1; ++b) f(q, 5 + b) = DRDYLR[b] * YLR[a] * UnLR;
+        f(q, q) += RhoLR * UnLR;
+        f(0, q) = DRDYLR[a] * UnLR;
+        f(1, q) = DRDYLR[a] * UnLR * ULR;
+        f(2, q) = DRDYLR[a] * UnLR * VLR;
+        f(3, q) = DRDYLR[a] * UnLR * WLR;
+        f(4, q) = DRDYLR[a] * UnLR * HtLR + RhoLR * UnLR * DHDYLR[a];
+    }
+    double dissUp = (Chat - std::fabs(Unhat)) / Rhohat / Chat / Chat;
+    double dissUu = Unhat / Chat;
+    double dissPp = Unhat / Chat;
+    double dissPu = (Chat - std::fabs(Unhat)) * Rhohat;
+    double AD = std::fabs(Unhat);
+    double hat[3] = {Uhat, Vhat, What}, vel[3] = {ULR, VLR, WLR};
+    auto d = [&](int i, int j) -> double& { return D[i * N + j]; };
+    d(0, 0) = AD * DRDPLR + dissUp * Rhohat;
+    for (int r = 0; r < 3; ++r) d(1 + r, 0) = AD * DRDPLR * vel[r] + dissUp * Rhohat * hat[r] + dissPp * n[r];
+    d(4, 0) = AD * (DRDPLR * HtLR + RhoLR * DHDPLR - 1.0) + dissUp * Rhohat * Hthat + dissPp * Unhat;
+    for (int c = 0; c < 3; ++c) {
+        d(0, 1 + c) = dissUu * n[c] * Rhohat;
+        for (int r = 0; r < 3; ++r) {
+            d(1 + r, 1 + c) = r == c ? AD * RhoLR + dissUu * n[c] * Rhohat * hat[r] + dissPu * n[c] * n[r]
+                                     : dissUu * n[c] * Rhohat * hat[r] + dissPu * n[c] * n[r];
+        }
+        d(4, 1 + c) = AD * RhoLR * vel[c] + dissUu * n[c] * Rhohat * Hthat + dissPu * n[c] * Unhat;
+    }
+    d(0, 4) = AD * DRDTLR;
+    for (int r = 0; r < 3; ++r) d(1 + r, 4) = AD * DRDTLR * vel[r];
+    d(4, 4) = AD * (DRDTLR * HtLR + RhoLR * DHDTLR);
+    for (int k = 0; k < N * N; ++k) J[k] = right ? 0.5 * F[k] - 0.5 * D[k] : 0.5 * F[k] + 0.5 * D[k];
+}
+
+// GaussSeidelSOR (src/math/math.cpp:141-200): A <- inverse(A) by Doolittle
+// LU without pivoting, then one forward and one back substitution per unit
+// column.
+void invert(double* A, int N) {
+    vector<double> Lm(N * N, 0.0), Um(N * N, 0.0), B(N, 0.0), D(N, 0.0), X(N, 0.0);
+    for (int k = 0; k < N - 1; ++k)
+        for (int i = k + 1; i < N; ++i) {
+            double coeff = A[i * N + k] / A[k * N + k];
+            Lm[i * N + k] = coeff;
+            for (int j = k + 1; j < N; ++j) A[i * N + j] = A[i * N + j] - coeff * A[k * N + j];
+        }
+    for (int i = 0; i < N; ++i) Lm[i * N + i] = 1.0;
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i <= j; ++i) Um[i * N + j] = A[i * N + j];
+    for (int k = 0; k < N; ++k) {
+        B[k] = 1.0;
+        D[0] = B[0];
+        for (int i = 1; i < N; ++i) {
+            D[i] = B[i];
+            for (int j = 0; j < i; ++j) D[i] = D[i] - Lm[i * N + j] * D[j];
+        }
+        X[N - 1] = D[N - 1] / Um[(N - 1) * N + N - 1];
+        for (int i = N - 2; i >= 0; --i) {
+            X[i] = D[i];
+            for (int j = N - 1; j >= i + 1; --j) X[i] = X[i] - Um[i * N + j] * X[j];
+            X[i] = X[i] / Um[i * N + i];
+        }
+        for (int i = 0; i < N; ++i) A[i * N + k] = X[i];
+        B[k] = 0.0;
+    }
+}
+
+// res <- A res, the product order of linearSolver.cpp:261-268
+void apply(const double* A, int N, const double* b, double* out) {
+    for (int j = 0; j < N; ++j) {
+        double u = 0.0;
+        for (int k = 0; k < N; ++k) u += A[j * N + k] * b[k];
+        out[j] = u;
+    }
+}
+}  // namespace db
+
+// method 0: calcLinearSolver (linearSolver.cpp:666-725), point-implicit
+// (P + 1.5 dtau/dt T) per cell; 1: calcSingleLinearSolver (:729-787), T per
+// cell; 2: calcLinearSolverLUSGS (:144-389), forward then backward sweep in
+// lib_RCM's order (cell.RCM / cell.invRCM) with the upwind flux Jacobians of
+// the faces.  state[nc][12 + 3 nsp], res[nc][4 + nsp] in/out, both in the
+// mesh's cell order.
+void linear_solver(Oracle& o, int method, int nsp, double time_step, const double* state, double* res) {
+    using namespace db;
+    const int N = 4 + nsp, K = rec(nsp);
+    vector<double> Pm(N * N), Tm(N * N), A(N * N), b(N), out(N);
+    if (method == 0 || method == 1) {
+        for (int i = 0; i < o.nc; ++i) {
+            const double* s = state + (size_t)i * K;
+            construct(s, nsp, Pm.data(), Tm.data());
+            for (int a = 0; a < N; ++a)
+                for (int c = 0; c < N; ++c)
+                    A[a * N + c] = method == 1 ? Tm[a * N + c]
+                                               : Pm[a * N + c] + 1.5 * s[DTP] / time_step * Tm[a * N + c];
+            std::copy(res + (size_t)i * N, res + (size_t)(i + 1) * N, b.begin());
+            invert(A.data(), N);
+            apply(A.data(), N, b.data(), res + (size_t)i * N);
+        }
+        return;
+    }
+    // lib_RCM's order (RCM.cpp:34-210)
+    vector<int> gptr(o.nc + 1), gadj(2 * (size_t)o.nif + 1), perm(o.nc), inv(o.nc);
+    orc_cell_graph(&o, gptr.data(), gadj.data());
+    orc_genrcm(o.nc, gptr.data(), gadj.data(), perm.data(), inv.data());
+    vector<double> J(N * N), DS(N * N), LU(N);
+    auto cell_pass = [&](int pos, bool forward) {
+        int own = perm[pos];
+        const double* s = state + (size_t)own * K;
+        std::fill(DS.begin(), DS.end(), 0.0);
+        std::fill(LU.begin(), LU.end(), 0.0);
+        double rDtauPerVol = s[DTP] / o.cvol[own];
+        for (int k = o.cfptr[own]; k < o.cfptr[own + 1]; ++k) {
+            int f = o.cfidx[k];
+            double nv[3];
+            double sg = o.owner[f] == own ? 1.0 : -1.0;
+            for (int a = 0; a < 3; ++a) nv[a] = sg > 0 ? o.fnrm[3 * f + a] : -o.fnrm[3 * f + a];
+            if (f < o.nif) {
+                int ngb = o.nbr[f] == own ? o.owner[f] : o.nbr[f];
+                const double* sn = state + (size_t)ngb * K;
+                if (forward ? inv[ngb] < pos : inv[ngb] > pos) {
+                    jacobian(s, sn, nsp, nv, true, J.data());
+                    for (int a = 0; a < N; ++a) {
+                        double sig = 0.0;
+                        for (int c = 0; c < N; ++c) sig += res[(size_t)ngb * N + c] * J[a * N + c];
+                        LU[a] += rDtauPerVol * sig * o.farea[f];
+                    }
+                }
+                jacobian(s, sn, nsp, nv, false, J.data());
+            } else {
+                jacobian(s, s, nsp, nv, false, J.data());
+            }
+            for (int a = 0; a < N * N; ++a) DS[a] += rDtauPerVol * J[a] * o.farea[f];
+        }
+        construct(s, nsp, Pm.data(), Tm.data());
+        for (int a = 0; a < N; ++a)
+            for (int c = 0; c < N; ++c) {
+                double timeFrac = s[DTP] / time_step;
+                A[a * N + c] = Pm[a * N + c] + 1.5 * timeFrac * Tm[a * N + c];
+                A[a * N + c] += DS[a * N + c];
+            }
+        for (int a = 0; a < N; ++a) b[a] = forward ? res[(size_t)own * N + a] - LU[a] : LU[a];
+        invert(A.data(), N);
+        apply(A.data(), N, b.data(), out.data());
+        for (int a = 0; a < N; ++a)
+            res[(size_t)own * N + a] = forward ? out[a] : res[(size_t)own * N + a] - out[a];
+    };
+    for (int i = 0; i < o.nc; ++i) cell_pass(i, true);
+    for (int i = o.nc - 1; i >= 0; --i) cell_pass(i, false);
+}
+
+extern "C" int orc_linear_solver(void* h, int method, int nsp, double time_step, const double* state,
+                                 double* res) {
+    if (method < 0 || method > 2 || nsp < 1 || nsp > 8) return -1;
+    linear_solver(*(Oracle*)h, method, nsp, time_step, state, res);
+    return 0;
+}

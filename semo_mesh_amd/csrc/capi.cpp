@@ -792,6 +792,37 @@ int semo_flux(semo_ctx* c, int scheme, int64_t n, int nsp, const double* left, c
     });
 }
 
+static void linsol_check(semo_ctx* c, int method, int nsp, double time_step) {
+    need_mesh(c);
+    SEMO_CHECK(method >= SEMO_LINSOL_POINT && method <= SEMO_LINSOL_LUSGS, SEMO_EINVAL,
+               "method must be SEMO_LINSOL_POINT, _SINGLE or _LUSGS");
+    SEMO_CHECK(nsp >= 1 && nsp <= 4, SEMO_EINVAL, "linear solver: 1 <= nsp <= 4");
+    SEMO_CHECK(method == SEMO_LINSOL_SINGLE || (std::isfinite(time_step) && time_step != 0.0), SEMO_EINVAL,
+               "linear solver: time_step must be finite and non-zero");
+}
+
+int semo_linear_solver(semo_ctx* c, int method, int nsp, double time_step, const double* state, double* res) {
+    if (!c || !state || !res) return SEMO_EINVAL;
+    return guard(c, [&] {
+        linsol_check(c, method, nsp, time_step);
+        DBuf<double> ds, dr;
+        to_dev(c, ds, state, 12 + 3 * nsp);
+        to_dev(c, dr, res, 4 + nsp);
+        linsol_device(c, method, nsp, time_step, ds.p, dr.p);
+        from_dev(c, dr.p, res, 4 + nsp);
+    });
+}
+
+int semo_linear_solver_device(semo_ctx* c, int method, int nsp, double time_step, const double* d_state,
+                              double* d_res) {
+    if (!c || !d_state || !d_res) return SEMO_EINVAL;
+    return guard(c, [&] {
+        linsol_check(c, method, nsp, time_step);
+        linsol_device(c, method, nsp, time_step, d_state, d_res);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 int semo_set_momentum_solver(semo_ctx* c, int mode) {
     if (!c) return SEMO_EINVAL;
     return guard(c, [&] {

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <map>
 #include <memory>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -254,6 +255,15 @@ struct semo_ctx {
     semo::DBuf<double> les_g;
     int les_model = 0;
     semo::DBuf<double> muT, kSGS, les_zero;
+    // density-based LU-SGS (linearSolver.cpp): per sweep (forward, backward)
+    // the level offsets, the cells in level order (slots), per slot the
+    // (neighbour, face) pairs it reads; the inverted diagonal blocks [n][N*N]
+    bool reordered = false;
+    bool ls_ready = false;
+    std::vector<int> ls_lev_ptr[2];
+    semo::DBuf<int> ls_cells[2], ls_pptr[2];
+    semo::DBuf<int2> ls_pairs[2];
+    semo::DBuf<double> ls_dinv;
     std::vector<double> h_ufix;     // as set, in the caller's numbering
     // per-face values of the volume-fraction "function" boundary [nbf]
     // (device face order), empty until semo_set_vf_function_points
@@ -451,6 +461,8 @@ void vf_function_values(semo_ctx* c, int n, const double* pts4_host);
 // types and values); faceR_b: right face values of the boundary faces [nbf]
 void gauss_green_standalone(semo_ctx* c, int sel, const double* phi, const double* faceR_b, const double* gin,
                             double* gout);
+void linsol_device(semo_ctx* c, int method, int nsp, double time_step, const double* st, double* res);
+int lusgs_levels(semo_ctx* c, int dir);
 void flux_device(semo_ctx* c, int scheme, int n, int nsp, const double* L, const double* R, const double* nrm,
                  const double* par, double* flux);
 void les_device(semo_ctx* c, int model, const double* U, const double* V, const double* W, const double* Rho,

@@ -192,6 +192,8 @@ void mesh_setup(semo_ctx* c, const semo_mesh_desc* m, int reorder) {
         SEMO_CHECK(expect == nf, SEMO_EINVAL, "patches must cover every boundary face");
     }
     c->np = np; c->nc = nc; c->nf = nf; c->nif = nif; c->nbf = nf - nif; c->npatch = m->n_patches;
+    c->reordered = reorder != 0;
+    c->ls_ready = false;
 
     // ---- cell order: lib_RCM adjacency (RCM.cpp:79-126) -> GENRCM
     c->perm.resize(nc); c->inv.resize(nc);

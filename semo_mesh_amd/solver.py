@@ -336,6 +336,36 @@ class GpuSolver:
                                    None if pa is None else _ptr(pa), _ptr(out)))
         return out
 
+    LINSOL_METHODS = {"point": 0, "single": 1, "lusgs": 2}
+
+    def _linsol(self, method, time_step, state, residuals):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        r = np.ascontiguousarray(residuals, dtype=np.float64).copy()
+        n, w = r.shape
+        nsp = w - 4
+        if st.shape != (n, 12 + 3 * nsp):
+            raise ValueError(f"state must be [n, {12 + 3 * nsp}] for {nsp} species")
+        self._ok(self._L.semo_linear_solver(self._h, self.LINSOL_METHODS[method], nsp, float(time_step),
+                                            _ptr(st), _ptr(r)))
+        return r
+
+    def calcLinearSolver(self, time_step, state, residuals):
+        """SEMO's point-implicit update (solvers/linearSolver.cpp:666-725):
+        per cell (P + 1.5 dtPseudo / time_step T)^-1 residuals.  state [n,
+        12 + 3 nsp] = (P, U, V, W, T, rho, C, Ht, dRhoDP, dHtDP, dRhoDT,
+        dHtDT, Ur, dtPseudo, Y.., dRhoDY.., dHtDY..), residuals [n, 4 + nsp];
+        returns the update."""
+        return self._linsol("point", time_step, state, residuals)
+
+    def calcSingleLinearSolver(self, state, residuals):
+        """calcSingleLinearSolver (linearSolver.cpp:729-787): T^-1 residuals."""
+        return self._linsol("single", 1.0, state, residuals)
+
+    def calcLinearSolverLUSGS(self, time_step, state, residuals):
+        """calcLinearSolverLUSGS (linearSolver.cpp:144-389): forward and
+        backward sweeps in lib_RCM's order of the mesh as uploaded."""
+        return self._linsol("lusgs", time_step, state, residuals)
+
     def setCellVarMinMax(self, phi):
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         mx = np.empty(self.mesh.n_cells); mn = np.empty(self.mesh.n_cells)

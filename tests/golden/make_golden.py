@@ -33,8 +33,9 @@ REF = os.path.join(ROOT, "oracle", "_ref", "semo_ref")
 VOF_ONLY = ("_faceLVF", "_faceRVF", "_Qm3", "_oldVF", "_dt")
 
 
-def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True, extra=()):
-    """extra: more "key value" lines for the driver's case file."""
+def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True, extra=(), arrays=None):
+    """extra: more "key value" lines for the driver's case file; arrays: more
+    arrays for the fixture."""
     with tempfile.TemporaryDirectory() as d:
         mesh.write_raw(os.path.join(d, "mesh.raw"))
         ip = None
@@ -91,6 +92,7 @@ def run_case(name, mesh, case, steps, correctors, init=None, keep_calls=True, ex
         arrs["case_les_model"] = np.array(case.les_model)
     if init is not None:
         arrs["init"] = np.asarray(init)
+    arrs.update(arrays or {})
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, **arrs)
     print(f"{name}: {len(arrs)} arrays, {os.path.getsize(path) / 1024:.0f} KiB")
@@ -336,6 +338,78 @@ def _flux():
     print(f"flux_faces: {len(arrs)} arrays, {os.path.getsize(path) / 1024:.0f} KiB")
 
 
+def linsol_states(mesh, nsp=2, seed=81, vol=None):
+    """Cell states of SEMO's density-based solvers for the implicit updates:
+    [nc][12 + 3 nsp] = (P, U, V, W, T, rho, C, Ht, dRhoDP, dHtDP, dRhoDT,
+    dHtDT, Ur, dtPseudo, Y.., dRhoDY.., dHtDY..).  Ideal-gas air cells
+    (R = 287, gamma = 1.4) and about a third liquid-like cells (rho ~ 1000,
+    C ~ 1400), Mach up to 0.8, Ur = |U| clipped to [0.01 C, C], dtPseudo from
+    a pseudo CFL of 1 to 5 on the cell size (vol: the cell volumes; the
+    oracle's geometry when not given)."""
+    rng = np.random.default_rng(seed)
+    n = mesh.n_cells
+    liquid = rng.random(n) < 0.35
+    P = 1e5 * (0.8 + 0.4 * rng.random(n))
+    T = 280.0 + 40.0 * rng.random(n)
+    Rg, gam = 287.0, 1.4
+    cp = gam * Rg / (gam - 1.0)
+    rho = np.where(liquid, 1000.0 * (1.0 + 0.02 * rng.random(n)), P / (Rg * T))
+    C = np.where(liquid, 1300.0 + 200.0 * rng.random(n), np.sqrt(gam * Rg * T))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1)[:, None]
+    u = d * (0.8 * rng.random(n) * C)[:, None]
+    Ht = np.where(liquid, 4180.0, cp) * T + 0.5 * (u * u).sum(axis=1)
+    drdp = np.where(liquid, 1.0 / (C * C), 1.0 / (Rg * T))
+    dhdp = np.where(liquid, 1e-3 * (0.5 + rng.random(n)), 0.0)
+    drdt = np.where(liquid, -0.2 - 0.2 * rng.random(n), -P / (Rg * T * T))
+    dhdt = np.where(liquid, 4180.0, cp)
+    Ur = np.clip(np.linalg.norm(u, axis=1), 0.01 * C, C)
+    h = np.abs(np.asarray(geometry_volumes(mesh) if vol is None else vol)) ** (1.0 / 3.0)
+    dtp = (1.0 + 4.0 * rng.random(n)) * h / (np.linalg.norm(u, axis=1) + C)
+    Y = rng.random((n, nsp)); Y /= Y.sum(axis=1)[:, None]
+    drdy = rho[:, None] * (rng.random((n, nsp - 1)) * 4.0 - 2.0)
+    dhdy = 1e4 * (rng.random((n, nsp - 1)) * 2.0 - 1.0)
+    return np.column_stack([P, u, T, rho, C, Ht, drdp, dhdp, drdt, dhdt, Ur, dtp, Y, drdy, dhdy])
+
+
+def geometry_volumes(mesh):
+    """Cell volumes from the oracle's geometry (only to size dtPseudo)."""
+    from oracle.oracle import Oracle
+    from semo_mesh_amd.cases import cavity_case
+    return Oracle(mesh, cavity_case()).get("cell_volume")
+
+
+def linsol_residuals(n, nsp=2, seed=82):
+    """Residuals [nc][4 + nsp] scaled per equation (mass, momentum, energy,
+    species)."""
+    rng = np.random.default_rng(seed)
+    scale = np.array([1.0, 100.0, 100.0, 100.0, 1e5] + [0.1] * (nsp - 1))
+    return rng.normal(size=(n, 4 + nsp)) * scale
+
+
+def _linsol():
+    """calcLinearSolver (point-implicit), calcSingleLinearSolver and
+    calcLinearSolverLUSGS (solvers/linearSolver.cpp, LU-SGS in lib_RCM's
+    order) on seeded density-based states: jittered hex and tet boxes with
+    two species, a hex box with one and a tet box with three."""
+    dt = 1e-3
+    for name, m, seed, nsp in (
+            ("linsol_box654", hex_mesh(6, 5, 4, jitter=0.1, seed=91), 83, 2),
+            ("linsol_tet433", tet_mesh(4, 3, 3, hi=(1.0, 0.75, 0.75), jitter=0.1, seed=92), 84, 2),
+            ("linsol_box543_nsp1", hex_mesh(5, 4, 3, jitter=0.1, seed=93), 85, 1),
+            ("linsol_tet322_nsp3", tet_mesh(3, 2, 2, jitter=0.1, seed=94), 86, 3)):
+        st = linsol_states(m, nsp, seed)
+        r0 = linsol_residuals(m.n_cells, nsp, seed + 100)
+        with tempfile.TemporaryDirectory() as d:
+            fn = os.path.join(d, "state.f64")
+            with open(fn, "wb") as fh:
+                np.ascontiguousarray(st).tofile(fh)
+                np.ascontiguousarray(r0).tofile(fh)
+            run_case(name, m, cavity_case(), 0, 1, keep_calls=False,
+                     extra=(f"linsol {fn} {dt!r}", f"nspecies {nsp}"),
+                     arrays={"linsol_state": st, "linsol_res0": r0, "linsol_dt": np.array(dt),
+                             "linsol_nsp": np.array(nsp)})
+
+
 def _poly20():
     """Three stacked prisms over a jittered 20-gon: internal faces and caps
     of 20 points (above the 16 the device keeps in registers), lid-driven
@@ -467,7 +541,7 @@ CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box,
          "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof,
          "poly20_stack3": _poly20, "vtu_tet221": _vtu, "initial_cube6": _initial, "les": _les,
-         "lesrun": _les_runs, "flux": _flux}
+         "lesrun": _les_runs, "flux": _flux, "linsol": _linsol}
 
 
 def main(names):
