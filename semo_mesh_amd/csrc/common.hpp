@@ -256,13 +256,15 @@ struct semo_ctx {
     int les_model = 0;
     semo::DBuf<double> muT, kSGS, les_zero;
     // density-based LU-SGS (linearSolver.cpp): per sweep (forward, backward)
-    // the level offsets, the cells in level order (slots), per slot the
-    // (neighbour, face) pairs it reads; the inverted diagonal blocks [n][N*N]
+    // the level offsets and, in level order, one slot record per cell (the
+    // cell and the first four (neighbour, face) pairs it reads, 3 x int4),
+    // the further pairs in an overflow list; the inverted diagonal blocks
+    // [n][N*N]
     bool reordered = false;
     bool ls_ready = false;
     std::vector<int> ls_lev_ptr[2];
-    semo::DBuf<int> ls_cells[2], ls_pptr[2];
-    semo::DBuf<int2> ls_pairs[2];
+    semo::DBuf<int4> ls_slots[2];
+    semo::DBuf<int2> ls_ovf[2];
     semo::DBuf<double> ls_dinv;
     std::vector<double> h_ufix;     // as set, in the caller's numbering
     // per-face values of the volume-fraction "function" boundary [nbf]

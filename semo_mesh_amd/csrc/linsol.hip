@@ -333,18 +333,19 @@ __device__ inline double quad_bcast(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// One level of a sweep, four lanes (a quad) per cell.  Slot t of the level is
-// cell cells[t]; its faces toward earlier (FWD) / later cells, in ascending
-// face order, are the pairs pr[pr_ptr[t] .. pr_ptr[t + 1]) = (neighbour,
-// face), the face stored as ~f when the cell is the face's neighbour (normal
-// reversed).  Lane l of the quad takes pairs l, l + 4, ...: its term
+// One level of a sweep, four lanes (a quad) per cell.  Slot t of the level
+// holds its cell, and the cell's faces toward earlier (FWD) / later cells in
+// ascending face order as (neighbour, face) pairs, the face stored as ~f when
+// the cell is the face's neighbour (normal reversed): the first four in the
+// slot's 48-byte record (one load round gives a lane its cell and its pair),
+// the rest in an overflow list.  Lane l of the quad takes pairs l, l + 4, ...: its term
 // dtau / V * (J_R r_ngb) * area; the quad adds the terms in pair order (the
 // reference's sum), then forward: r <- Dinv (r - sum), backward:
 // r <- r - Dinv sum, rows l and l + 4 on lane l.  Control flow is uniform
 // within a quad, so every DPP source lane is active.
 template <int NSP, bool FWD>
-__global__ __launch_bounds__(TPB) void k_lusgs_level(int cnt, const int* __restrict__ cells,
-                                                     const int* __restrict__ pr_ptr, const int2* __restrict__ pr,
+__global__ __launch_bounds__(TPB) void k_lusgs_level(int cnt, const int4* __restrict__ slots,
+                                                     const int2* __restrict__ ovf,
                                                      const double* __restrict__ nrm, const double* __restrict__ area,
                                                      const double* __restrict__ vol, const double* __restrict__ st,
                                                      const double* __restrict__ dinv, double* __restrict__ res) {
@@ -353,8 +354,12 @@ __global__ __launch_bounds__(TPB) void k_lusgs_level(int cnt, const int* __restr
     const int gt = (int)blockIdx.x * TPB + (int)threadIdx.x;
     const int t = gt >> 2, l = gt & 3;
     if (t >= cnt) return;
-    const int i = cells[t];
-    const int p0 = pr_ptr[t], p1 = pr_ptr[t + 1];
+    // slot record: {cell, pair count, overflow offset, -}, then pairs 0..3
+    const int4 h = slots[3 * (size_t)t];
+    const int4 qq = slots[3 * (size_t)t + 1 + (l >> 1)];
+    const int i = h.x, p1 = h.y;
+    const int2 q0 = (l & 1) ? make_int2(qq.z, qq.w) : make_int2(qq.x, qq.y);
+    const int p0 = 0;
     double s[LS<NSP>::K];
     load_rec<NSP>(st, i, s);
     const double rDtauPerVol = s[sDTP] / vol[i];
@@ -377,7 +382,7 @@ __global__ __launch_bounds__(TPB) void k_lusgs_level(int cnt, const int* __restr
 #pragma unroll
         for (int a = 0; a < N; ++a) c[a] = 0.0;
         if (p < p1) {
-            const int2 q = pr[p];
+            const int2 q = p < 4 ? q0 : ovf[h.z + p - 4];
             const int g = q.x;
             const bool own_is_owner = q.y >= 0;
             const int f = own_is_owner ? q.y : ~q.y;
@@ -468,24 +473,33 @@ void lusgs_schedule(semo_ctx* c) {
             const int i = order[dir == 0 ? p : nc - 1 - p];
             cells[fill[lev[i]]++] = i;
         }
-        // per slot, the faces toward earlier (later) cells, ascending
-        std::vector<int> pptr(nc + 1, 0);
-        std::vector<int2> pr;
-        pr.reserve(nif);
+        // per slot: {cell, pair count, overflow offset, 0} and its first four
+        // (neighbour, face) pairs toward earlier (later) cells, ascending
+        std::vector<int4> rec(3 * (size_t)nc);
+        std::vector<int2> ovf;
         for (int t = 0; t < nc; ++t) {
             const int i = cells[t];
+            int np = 0;
+            int2 q[4] = {};
+            const int off = (int)ovf.size();
             for (int k = c->h_cf_ptr[i]; k < c->h_cf_ptr[i + 1]; ++k) {
                 const int f = c->h_cf_idx[k];
                 if (f >= nif) continue;
                 const bool own = c->h_owner[f] == i;
                 const int g = own ? c->h_nbr[f] : c->h_owner[f];
-                if (dir == 0 ? rank[g] < rank[i] : rank[g] > rank[i]) pr.push_back(make_int2(g, own ? f : ~f));
+                if (!(dir == 0 ? rank[g] < rank[i] : rank[g] > rank[i])) continue;
+                const int2 pq = make_int2(g, own ? f : ~f);
+                if (np < 4) q[np] = pq;
+                else ovf.push_back(pq);
+                ++np;
             }
-            pptr[t + 1] = (int)pr.size();
+            rec[3 * (size_t)t] = make_int4(i, np, off, 0);
+            rec[3 * (size_t)t + 1] = make_int4(q[0].x, q[0].y, q[1].x, q[1].y);
+            rec[3 * (size_t)t + 2] = make_int4(q[2].x, q[2].y, q[3].x, q[3].y);
         }
-        c->ls_cells[dir].upload(cells, c->stream);
-        c->ls_pptr[dir].upload(pptr, c->stream);
-        c->ls_pairs[dir].upload(pr, c->stream);
+        if (ovf.empty()) ovf.push_back(make_int2(0, 0));
+        c->ls_slots[dir].upload(rec, c->stream);
+        c->ls_ovf[dir].upload(ovf, c->stream);
     }
     SEMO_HIP(hipStreamSynchronize(c->stream));
     c->ls_ready = true;
@@ -518,19 +532,18 @@ void linsol_run(semo_ctx* c, int method, double time_step, const double* st, dou
         for (int l = 0; l < nlev; ++l) {
             const int cnt = ptr[l + 1] - ptr[l];
             if (cnt == 0) continue;
-            // per cell: record, Dinv, own residual read + write, slot data;
+            // per cell: record, Dinv, own residual read + write, slot record;
             // per earlier (later) neighbour: its record and residual, the
             // face's normal and area, the pair (half the sides)
-            const double bytes = (double)cnt * ((K + N * N + 2 * N + 1) * 8.0 + 12.0) +
+            const double bytes = (double)cnt * ((K + N * N + 2 * N + 1) * 8.0 + 48.0) +
                                  0.5 * (double)cnt * (sides / n) * ((K + N) * 8.0 + 40.0);
-            const int* cells = c->ls_cells[dir].p + ptr[l];
-            const int* pp = c->ls_pptr[dir].p + ptr[l];
+            const int4* slots = c->ls_slots[dir].p + 3 * (size_t)ptr[l];
             if (dir == 0)
-                SEMO_LAUNCH(c, "lusgs_sweep", bytes, (k_lusgs_level<NSP, true>), dim3(nblk(4L * cnt)), b, cnt, cells, pp,
-                            c->ls_pairs[0].p, c->f_nrm.p, c->f_area.p, c->c_vol.p, st, c->ls_dinv.p, res);
+                SEMO_LAUNCH(c, "lusgs_sweep", bytes, (k_lusgs_level<NSP, true>), dim3(nblk(4L * cnt)), b, cnt, slots,
+                            c->ls_ovf[0].p, c->f_nrm.p, c->f_area.p, c->c_vol.p, st, c->ls_dinv.p, res);
             else
-                SEMO_LAUNCH(c, "lusgs_sweep", bytes, (k_lusgs_level<NSP, false>), dim3(nblk(4L * cnt)), b, cnt, cells,
-                            pp, c->ls_pairs[1].p, c->f_nrm.p, c->f_area.p, c->c_vol.p, st, c->ls_dinv.p, res);
+                SEMO_LAUNCH(c, "lusgs_sweep", bytes, (k_lusgs_level<NSP, false>), dim3(nblk(4L * cnt)), b, cnt,
+                            slots, c->ls_ovf[1].p, c->f_nrm.p, c->f_area.p, c->c_vol.p, st, c->ls_dinv.p, res);
         }
     }
 }
