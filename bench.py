@@ -361,9 +361,10 @@ def linsol_inputs(n_cells, nsp, h, seed=20261203):
 
 
 def lusgs_leg(n=128, nsp=2, reps=5, cpu_n=48):
-    """SEMO's density-based implicit updates (solvers/linearSolver.cpp) on
-    the jittered n^3 cavity mesh with seeded gas / liquid-like states and two
-    species (6 equations): LU-SGS (calcLinearSolverLUSGS, forward and
+    """SEMO's density-based pseudo step on the jittered n^3 cavity mesh with
+    seeded gas / liquid-like states and two species (6 equations): the
+    residual (calcRHS, kernel times) and the implicit updates
+    (solvers/linearSolver.cpp): LU-SGS (calcLinearSolverLUSGS, forward and
     backward sweeps in lib_RCM's order, level-scheduled) and the
     point-implicit update (calcLinearSolver), state and residuals resident
     on the device (semo_linear_solver_device); the compiled reference's
@@ -420,6 +421,38 @@ def lusgs_leg(n=128, nsp=2, reps=5, cpu_n=48):
         hout = np.empty_like(r0)
         assert hip.hipMemcpy(hout.ctypes.data_as(C.c_void_p), dr, nbytes, 2) == 0
         out["point_finite"] = bool(np.isfinite(hout).all())
+        # the residual the updates consume: calcRHS (SLAU2 + viscous flux per
+        # face, summed per cell, dual-time source, gravity, scaling), kernel
+        # times of one call with host arrays (the transfers are not timed)
+        rng = np.random.default_rng(7)
+        nf = m.n_faces
+        fs = np.empty((2, nf, 8 + nsp))
+        for k in range(2):
+            fs[k, :, 0] = 1e5 * (0.8 + 0.4 * rng.random(nf))
+            fs[k, :, 1:4] = rng.normal(size=(nf, 3)) * 50.0
+            fs[k, :, 4] = 280.0 + 40.0 * rng.random(nf)
+            fs[k, :, 5] = fs[k, :, 0] / (287.0 * fs[k, :, 4])
+            fs[k, :, 6] = np.sqrt(1.4 * 287.0 * fs[k, :, 4])
+            fs[k, :, 7] = 1004.5 * fs[k, :, 4] + 0.5 * (fs[k, :, 1:4] ** 2).sum(axis=1)
+            fs[k, :, 8] = 1.8e-5
+            fs[k, :, 9:] = rng.random((nf, nsp - 1))
+        N = 4 + nsp
+        Q = np.column_stack([st[:, 5], st[:, 5:6] * st[:, 1:4], st[:, 5] * st[:, 7] - st[:, 0],
+                             st[:, 5:6] * st[:, 14:13 + nsp]])
+        cells = np.column_stack([st[:, 5], st[:, 1:4], st[:, 7], st[:, 0], st[:, 13], np.ones(nc),
+                                 st[:, 14:13 + nsp], Q * 1.01, Q * 0.99])
+        g.profile(True); g.profile_reset()
+        t0 = time.perf_counter()
+        rr = g.calcRHS(1e-3, np.array([0.0, 0.0, -9.81]), fs[0], fs[1], cells)
+        t_rhs = time.perf_counter() - t0
+        prof = g.profile_stats()
+        g.profile(False)
+        out["rhs"] = {"kernels": {k: {"ms": round(v["ms"], 3),
+                                      "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                                  for k, v in prof.items() if k.startswith("rhs")},
+                      "kernel_ms": round(sum(v["ms"] for k, v in prof.items() if k.startswith("rhs")), 3),
+                      "call_ms_with_transfers": round(1e3 * t_rhs, 1), "finite": bool(np.isfinite(rr).all()),
+                      "faces": nf}
     finally:
         for p in bufs:
             hip.hipFree(p)

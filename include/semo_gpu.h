@@ -373,6 +373,33 @@ int semo_flux(semo_ctx* ctx, int scheme, int64_t n, int nsp, const double* left,
  * timeStep (not read by SEMO_LINSOL_SINGLE).  The LU-SGS order is the RCM
  * of the mesh as the caller numbers it, whether or not the upload reordered
  * the cells; the first LU-SGS call on a mesh levels the sweeps on the host. */
+/* calcFluxes (src/solvers/RHS.cpp:98-406) of the density-based solvers on
+ * the uploaded mesh: per face SLAU2 (calcFluxSLAU2, flux.cpp:159-238) of the
+ * left and right records and the viscous flux (calcViscousFlux,
+ * viscousFlux.cpp:5-67) of the face's velocity gradients, times the area,
+ * summed into the cells in face order (+ owner, - neighbour).
+ * face_left / face_right[n_faces][8 + nsp] = (P, U, V, W, T, rho, C, Ht, mu,
+ * Y_0..Y_{nsp-2}) in SEMO's face order and orientation; face_grad[n_faces][9]
+ * = (dU/dx, dU/dy, dU/dz, dV/dx, .., dW/dz) (face.var dUdx..dWdz), or NULL
+ * for zero -- SEMO fills them nowhere (calcGradientFace is commented out),
+ * so its runs see zero; residuals[n][4 + nsp] out.  2 <= nsp <= 8 (with one
+ * species SEMO's calcFluxes reads a face volume fraction that does not
+ * exist).                                                                    */
+int semo_calc_fluxes(semo_ctx* ctx, int nsp, const double* face_left, const double* face_right,
+                     const double* face_grad, double* residuals);
+/* calcRHS (RHS.cpp:5-28): calcFluxes as above, then sourceTerms
+ * (source.cpp:5-35) -- per cell the dual-time source (1.5 Q - 2 Qn + 0.5 Qm)
+ * / time_step * V of Q = (rho, rho U, rho V, rho W, rho Ht - P, rho Y..),
+ * minus rho g V in the momentum rows, minus V * st_force (the surface-tension
+ * force -sigma kappa grad(alpha) per cell, [n][3], or NULL for none) -- and
+ * the scaling by -dtPseudo / V.  cells[n][8 + (nsp-1) + 2(4 + nsp)] = (rho,
+ * U, V, W, Ht, P, dtPseudo, VF, Y_0..Y_{nsp-2}, Qn_0.., Qm_0..); g[3].
+ * SEMO's own sourceTerms computes the surface-tension force itself and, as
+ * written, hands calcGaussGreen an empty gradient vector that it indexes;
+ * here the caller supplies the force.                                        */
+int semo_calc_rhs(semo_ctx* ctx, int nsp, double time_step, const double* g, const double* face_left,
+                  const double* face_right, const double* face_grad, const double* cells, const double* st_force,
+                  double* residuals);
 enum { SEMO_LINSOL_POINT = 0, SEMO_LINSOL_SINGLE = 1, SEMO_LINSOL_LUSGS = 2 };
 int semo_linear_solver(semo_ctx* ctx, int method, int nsp, double time_step, const double* state,
                        double* residuals);

@@ -37,7 +37,7 @@ SRCS := mesh/build mesh/geometric math/math math/gradient math/RCM mpi/build \
         solvers/solveAMGCL solvers/solvePETSc solvers/timestep solvers/eos solvers/transport \
         solvers/NVD solvers/cellVarMinMax solvers/surfaceNormalVelocity solvers/pressureBased \
         solvers/source/gravity solvers/source/surfaceTension solvers/source/curvature \
-        turbulenceModels/LES solvers/flux solvers/linearSolver \
+        turbulenceModels/LES solvers/flux solvers/linearSolver solvers/RHS solvers/source solvers/viscousFlux \
         save/save save/saveAscii save/saveBinary save/saveCompress save/saveBase64 save/saveMkdirs
 OBJS := $(foreach s,$(SRCS),$(OBJ)/$(subst /,_,$(s)).o)
 

@@ -267,6 +267,8 @@ int main(int argc, char** argv) {
     string linsol;        // "linsol FILE DT": the density-based implicit updates on FILE's state
     double linsol_dt = 0.0;
     int nspecies = 2;     // "nspecies K": K species (the third and later ones take the second's properties)
+    string rhsfile;       // "rhs FILE DT": calcRHS of the density-based solvers on FILE's face and cell state
+    double rhs_dt = 0.0;
     vector<PatchBC> bcs;
     string initfile;
     {
@@ -290,6 +292,7 @@ int main(int argc, char** argv) {
             else if (key == "turb") cf >> turb;
             else if (key == "linsol") cf >> linsol >> linsol_dt;
             else if (key == "nspecies") cf >> nspecies;
+            else if (key == "rhs") cf >> rhsfile >> rhs_dt;
             else if (key == "bc") {
                 PatchBC b;
                 cf >> b.name >> b.ptype >> b.pval >> b.utype >> b.u[0] >> b.u[1] >> b.u[2] >> b.vftype >> b.vfval;
@@ -541,6 +544,68 @@ int main(int argc, char** argv) {
         }
         printf("SEMO_REF_DONE linsol point_s=%.6f single_s=%.6f lusgs_s=%.6f cells=%zu\n", secs[0], secs[1],
                secs[2], nc);
+        fflush(stdout);
+        MPI_Finalize();
+        return 0;
+    }
+
+    if (!rhsfile.empty()) {
+        // calcFluxes (solvers/RHS.cpp:98-406: SLAU2 and the viscous flux per
+        // face, summed into the cells in face order) on FILE's state -- not
+        // the whole calcRHS, whose sourceTerms hands calcGaussGreen an empty
+        // gradient vector that it indexes (gradient.cpp:1787-1788 leave the
+        // resize commented out): per face the left and right
+        // records (P, U, V, W, T, rho, C, Ht, mu, Y_0..Y_{nSp-2}), then per
+        // face the nine velocity gradients (face.var dUdx..dWdz), then per
+        // cell (rho, U, V, W, Ht, P, dtPseudo, VF, Y_0..Y_{nSp-2},
+        // Qn_0..Qn_{nEq-1}, Qm_0..Qm_{nEq-1}); the mesh's face and cell order
+        const int nsp = controls.nSp, N = controls.nEq, KF = 8 + nsp, KC = 8 + (nsp - 1) + 2 * N;
+        const size_t nf = mesh.faces.size();
+        vector<double> fl(nf * KF), fr(nf * KF), fg(nf * 9), cs(nc * KC);
+        FILE* f = fopen(rhsfile.c_str(), "rb");
+        if (!f) { perror(rhsfile.c_str()); return 2; }
+        rd(f, fl.data(), fl.size()); rd(f, fr.data(), fr.size()); rd(f, fg.data(), fg.size());
+        rd(f, cs.data(), cs.size());
+        fclose(f);
+        controls.timeStep = rhs_dt;
+        for (size_t i = 0; i < nc; ++i) {
+            auto& v = mesh.cells[i].var;
+            const double* c = &cs[i * KC];
+            v[controls.Rho] = c[0]; v[controls.U] = c[1]; v[controls.V] = c[2]; v[controls.W] = c[3];
+            v[controls.Ht] = c[4]; v[controls.P] = c[5]; v[controls.dtPseudo] = c[6];
+            v[controls.VF[0]] = c[7];
+            if (nsp > 1) v[controls.VF[1]] = 1.0 - c[7];
+            for (int a = 0; a < nsp - 1; ++a) v[controls.MF[a]] = c[8 + a];
+            for (int j = 0; j < N; ++j) {
+                v[controls.Qn[j]] = c[7 + nsp + j];
+                v[controls.Qm[j]] = c[7 + nsp + N + j];
+            }
+        }
+        const int dv[9] = {controls.dUdx, controls.dUdy, controls.dUdz, controls.dVdx, controls.dVdy,
+                           controls.dVdz, controls.dWdx, controls.dWdy, controls.dWdz};
+        for (size_t q = 0; q < nf; ++q) {
+            auto& fc = mesh.faces[q];
+            for (int side = 0; side < 2; ++side) {
+                auto& lr = side == 0 ? fc.varL : fc.varR;
+                const double* r = side == 0 ? &fl[q * KF] : &fr[q * KF];
+                lr[controls.fP] = r[0]; lr[controls.fU] = r[1]; lr[controls.fV] = r[2]; lr[controls.fW] = r[3];
+                lr[controls.fT] = r[4]; lr[controls.fRho] = r[5]; lr[controls.fC] = r[6]; lr[controls.fHt] = r[7];
+                lr[controls.fmu] = r[8];
+                for (int a = 0; a < nsp - 1; ++a) lr[controls.fMF[a]] = r[9 + a];
+                int cell = side == 0 || fc.neighbour < 0 ? fc.owner : fc.neighbour;
+                if (nsp > 1) lr[controls.fVF[0]] = mesh.cells[cell].var[controls.VF[0]];
+            }
+            for (int k = 0; k < 9; ++k) fc.var[dv[k]] = fg[q * 9 + k];
+        }
+        vector<vector<double>> res(nc, vector<double>(N, 0.0));
+        auto ta = std::chrono::steady_clock::now();
+        solvers.calcFluxes(mesh, controls, species, res);
+        double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - ta).count();
+        vector<double> out(nc * N);
+        for (size_t i = 0; i < nc; ++i)
+            for (int k = 0; k < N; ++k) out[i * N + k] = res[i][k];
+        dump("rhs_flux_res", out);
+        printf("SEMO_REF_DONE rhs rhs_s=%.6f cells=%zu\n", secs, nc);
         fflush(stdout);
         MPI_Finalize();
         return 0;

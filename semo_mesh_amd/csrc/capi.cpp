@@ -792,6 +792,48 @@ int semo_flux(semo_ctx* c, int scheme, int64_t n, int nsp, const double* left, c
     });
 }
 
+// face arrays: caller numbering -> device face order
+static void face_to_dev(semo_ctx* c, DBuf<double>& d, const double* h, int comps) {
+    std::vector<double> t((size_t)c->nf * comps);
+    for (int k = 0; k < c->nf; ++k)
+        for (int a = 0; a < comps; ++a) t[(size_t)k * comps + a] = h[(size_t)c->fperm[k] * comps + a];
+    d.upload(t, c->stream);
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+}
+
+static int rhs_common(semo_ctx* c, int nsp, bool src, double time_step, const double* g, const double* L,
+                      const double* R, const double* grad, const double* cells, const double* st, double* res) {
+    return guard(c, [&] {
+        need_mesh(c);
+        SEMO_CHECK(nsp >= 2 && nsp <= 8, SEMO_EINVAL, "calcFluxes / calcRHS: 2 <= nsp <= 8");
+        SEMO_CHECK(!src || (std::isfinite(time_step) && time_step != 0.0), SEMO_EINVAL,
+                   "calcRHS: time_step must be finite and non-zero");
+        const int N = 4 + nsp;
+        DBuf<double> dL, dR, dG, dC, dS, dF, dr;
+        face_to_dev(c, dL, L, 8 + nsp);
+        face_to_dev(c, dR, R, 8 + nsp);
+        if (grad) face_to_dev(c, dG, grad, 9);
+        if (src) to_dev(c, dC, cells, 8 + (nsp - 1) + 2 * N);
+        if (src && st) to_dev(c, dS, st, 3);
+        dF.alloc((size_t)c->nf * N);
+        dr.alloc((size_t)c->nc * N);
+        rhs_device(c, nsp, src, time_step, g, dL.p, dR.p, grad ? dG.p : nullptr, src ? dC.p : nullptr,
+                   (src && st) ? dS.p : nullptr, dF.p, dr.p);
+        from_dev(c, dr.p, res, N);
+    });
+}
+
+int semo_calc_fluxes(semo_ctx* c, int nsp, const double* L, const double* R, const double* grad, double* res) {
+    if (!c || !L || !R || !res) return SEMO_EINVAL;
+    return rhs_common(c, nsp, false, 1.0, nullptr, L, R, grad, nullptr, nullptr, res);
+}
+
+int semo_calc_rhs(semo_ctx* c, int nsp, double time_step, const double* g, const double* L, const double* R,
+                  const double* grad, const double* cells, const double* st, double* res) {
+    if (!c || !g || !L || !R || !cells || !res) return SEMO_EINVAL;
+    return rhs_common(c, nsp, true, time_step, g, L, R, grad, cells, st, res);
+}
+
 static void linsol_check(semo_ctx* c, int method, int nsp, double time_step) {
     need_mesh(c);
     SEMO_CHECK(method >= SEMO_LINSOL_POINT && method <= SEMO_LINSOL_LUSGS, SEMO_EINVAL,

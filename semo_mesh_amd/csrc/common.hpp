@@ -463,6 +463,8 @@ void vf_function_values(semo_ctx* c, int n, const double* pts4_host);
 // types and values); faceR_b: right face values of the boundary faces [nbf]
 void gauss_green_standalone(semo_ctx* c, int sel, const double* phi, const double* faceR_b, const double* gin,
                             double* gout);
+void rhs_device(semo_ctx* c, int nsp, bool src, double time_step, const double* g, const double* L, const double* R,
+                const double* grad, const double* cells, const double* st, double* fterm, double* res);
 void linsol_device(semo_ctx* c, int method, int nsp, double time_step, const double* st, double* res);
 int lusgs_levels(semo_ctx* c, int dir);
 void flux_device(semo_ctx* c, int scheme, int n, int nsp, const double* L, const double* R, const double* nrm,

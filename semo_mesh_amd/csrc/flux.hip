@@ -408,7 +408,125 @@ __global__ void __launch_bounds__(TPB) k_flux(int n, int nsp, const double* __re
     }
 }
 
+// calcFluxes (RHS.cpp:98-406), per face: SLAU2 of the face's left and right
+// records (P, U, V, W, T, rho, C, Ht, mu, Y_0..Y_{nsp-2}; the last mass
+// fraction is 1 - the sum of the others, as calcFluxes forms it) and
+// calcViscousFlux (viscousFlux.cpp:5-67) of the face's velocity gradients
+// (zero when grad is null); out: (cFlux - vFlux) * area, [nf][4 + nsp].  The
+// records and the normal are in the caller's orientation: a face the
+// renumbering flipped takes the negated normal.
+__global__ void __launch_bounds__(TPB) k_rhs_face(int nf, int nsp, const double* __restrict__ Ls,
+                                                  const double* __restrict__ Rs, const double* __restrict__ grad,
+                                                  const double* __restrict__ nrm, const double* __restrict__ area,
+                                                  const int8_t* __restrict__ flip, double* __restrict__ out) {
+    int f = xcd_block() * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int w = 8 + nsp, N = 4 + nsp;
+    const double* l = Ls + (size_t)f * w;
+    const double* r = Rs + (size_t)f * w;
+    double YL[8], YR[8];
+    double sl = 0.0, sr = 0.0;
+    for (int k = 0; k < nsp - 1; ++k) { YL[k] = l[9 + k]; sl += YL[k]; YR[k] = r[9 + k]; sr += YR[k]; }
+    YL[nsp - 1] = 1.0 - sl;
+    YR[nsp - 1] = 1.0 - sr;
+    FSide L{l[0], l[1], l[2], l[3], l[4], l[5], l[6], l[7], YL};
+    FSide R{r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], YR};
+    double nv[3];
+    const bool fl_ = flip[f] != 0;
+    for (int a = 0; a < 3; ++a) nv[a] = fl_ ? -nrm[3 * (size_t)f + a] : nrm[3 * (size_t)f + a];
+    double cf[12];
+    flux_slau2(L, R, nv, nsp, cf);
+    // calcViscousFlux: face values at gc = 0.5, stresses from the gradients
+    double g[9];
+    for (int k = 0; k < 9; ++k) g[k] = grad ? grad[(size_t)f * 9 + k] : 0.0;
+    const double gc = 0.5;
+    const double muF = gc * l[8] + (1.0 - gc) * r[8];
+    const double UF = gc * l[1] + (1.0 - gc) * r[1];
+    const double VF = gc * l[2] + (1.0 - gc) * r[2];
+    const double WF = gc * l[3] + (1.0 - gc) * r[3];
+    const double dUdx = g[0], dUdy = g[1], dUdz = g[2], dVdx = g[3], dVdy = g[4], dVdz = g[5];
+    const double dWdx = g[6], dWdy = g[7], dWdz = g[8];
+    const double txx = dUdx + dUdx - 2.0 / 3.0 * (dUdx + dVdy + dWdz);
+    const double txy = dVdx + dUdy, txz = dWdx + dUdz;
+    const double tyx = txy, tyy = dVdy + dVdy - 2.0 / 3.0 * (dUdx + dVdy + dWdz), tyz = dVdz + dWdy;
+    const double tzx = txz, tzy = tyz, tzz = dWdz + dWdz - 2.0 / 3.0 * (dUdx + dVdy + dWdz);
+    const double nx = nv[0], ny = nv[1], nz = nv[2];
+    double vf[12];
+    for (int k = 0; k < N; ++k) vf[k] = 0.0;
+    vf[1] = muF * (txx * nx + txy * ny + txz * nz);
+    vf[2] = muF * (tyx * nx + tyy * ny + tyz * nz);
+    vf[3] = muF * (tzx * nx + tzy * ny + tzz * nz);
+    vf[4] = muF * (txx * UF + txy * VF + txz * WF) * nx + muF * (tyx * UF + tyy * VF + tyz * WF) * ny +
+            muF * (tzx * UF + tzy * VF + tzz * WF) * nz;
+    const double A = area[f];
+    for (int k = 0; k < N; ++k) out[(size_t)f * N + k] = (cf[k] - vf[k]) * A;
+}
+
+// per cell: the faces' terms in ascending face order, + at faces the cell
+// owns (in the caller's orientation), - at the others (calcFluxes); with
+// SRC, then sourceTerms (source.cpp:5-35: the dual-time source from Q, Qn,
+// Qm, gravity, the caller's surface-tension force) and calcRHS's scaling by
+// -dtPseudo / V (RHS.cpp:17-25).  cells[n][8 + (nsp-1) + 2N] = (rho, U, V,
+// W, Ht, P, dtPseudo, VF, Y_0..Y_{nsp-2}, Qn.., Qm..).
+template <bool SRC>
+__global__ void __launch_bounds__(TPB) k_rhs_cell(int nc, int nif, int nsp, const int* __restrict__ cf_ptr,
+                                                  const int* __restrict__ cf_idx, const int* __restrict__ owner,
+                                                  const int8_t* __restrict__ flip, const double* __restrict__ fterm,
+                                                  const double* __restrict__ vol, const double* __restrict__ cells,
+                                                  const double* __restrict__ st, double time_step, double g0,
+                                                  double g1, double g2, double* __restrict__ res) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const int N = 4 + nsp;
+    double r[12];
+    for (int k = 0; k < N; ++k) r[k] = 0.0;
+    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
+        const int f = cf_idx[q];
+        const double* c = fterm + (size_t)f * N;
+        const bool plus = f >= nif || ((owner[f] == i) != (flip[f] != 0));
+        if (plus) for (int k = 0; k < N; ++k) r[k] += c[k];
+        else for (int k = 0; k < N; ++k) r[k] -= c[k];
+    }
+    if (SRC) {
+        const int KC = 8 + (nsp - 1) + 2 * N;
+        const double* cs = cells + (size_t)i * KC;
+        const double Rho = cs[0], V = vol[i];
+        double Q[12];
+        Q[0] = Rho; Q[1] = Rho * cs[1]; Q[2] = Rho * cs[2]; Q[3] = Rho * cs[3];
+        Q[4] = Rho * cs[4] - cs[5];
+        for (int a = 0; a < nsp - 1; ++a) Q[5 + a] = Rho * cs[8 + a];
+        const double* Qn = cs + 7 + nsp;
+        const double* Qm = Qn + N;
+        for (int j = 0; j < N; ++j) r[j] += (1.5 * Q[j] - 2.0 * Qn[j] + 0.5 * Qm[j]) / time_step * V;
+        r[1] -= Rho * g0 * V;
+        r[2] -= Rho * g1 * V;
+        r[3] -= Rho * g2 * V;
+        if (st)
+            for (int a = 0; a < 3; ++a) r[1 + a] -= V * st[3 * (size_t)i + a];
+        for (int k = 0; k < N; ++k) r[k] *= -cs[6] / V;
+    }
+    for (int k = 0; k < N; ++k) res[(size_t)i * N + k] = r[k];
+}
+
 }  // namespace
+
+void rhs_device(semo_ctx* c, int nsp, bool src, double time_step, const double* g, const double* L, const double* R,
+                const double* grad, const double* cells, const double* st, double* fterm, double* res) {
+    const int N = 4 + nsp, KC = 8 + (nsp - 1) + 2 * N;
+    SEMO_LAUNCH(c, "rhs_face", (double)c->nf * ((2.0 * (8 + nsp) + (grad ? 9 : 0) + 4 + N) * 8.0 + 1.0),
+                k_rhs_face, dim3(nblk(c->nf)), dim3(TPB), c->nf, nsp, L, R, grad, c->f_nrm.p, c->f_area.p,
+                c->d_fflip.p, fterm);
+    const double sides = (double)c->h_cf_idx.size();
+    const double cbytes = sides * (4.0 + 4.0 + 1.0 + N * 8.0) + (double)c->nc * (N + 2 + (src ? KC + 1 : 0)) * 8.0;
+    if (src)
+        SEMO_LAUNCH(c, "rhs_cell", cbytes, k_rhs_cell<true>, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, nsp,
+                    c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_fflip.p, fterm, c->c_vol.p, cells, st,
+                    time_step, g[0], g[1], g[2], res);
+    else
+        SEMO_LAUNCH(c, "rhs_cell", cbytes, k_rhs_cell<false>, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, nsp,
+                    c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_fflip.p, fterm, c->c_vol.p, cells, st,
+                    time_step, 0.0, 0.0, 0.0, res);
+}
 
 void flux_device(semo_ctx* c, int scheme, int n, int nsp, const double* L, const double* R, const double* nrm,
                  const double* par, double* flux) {

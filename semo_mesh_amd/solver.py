@@ -336,6 +336,39 @@ class GpuSolver:
                                    None if pa is None else _ptr(pa), _ptr(out)))
         return out
 
+    def calcFluxes(self, face_left, face_right, face_grad=None):
+        """calcFluxes of SEMO's density-based solvers (solvers/RHS.cpp:98-406):
+        SLAU2 + viscous flux per face, summed into the cells.  face_left /
+        face_right [n_faces, 8 + nsp] = (P, U, V, W, T, rho, C, Ht, mu,
+        Y_0..Y_{nsp-2}), face_grad [n_faces, 9] or None (zero, as SEMO's runs
+        have them); returns [n, 4 + nsp]."""
+        L = np.ascontiguousarray(face_left, dtype=np.float64)
+        R = np.ascontiguousarray(face_right, dtype=np.float64)
+        G = None if face_grad is None else np.ascontiguousarray(face_grad, dtype=np.float64)
+        nsp = L.shape[1] - 8
+        out = np.empty((self.mesh.n_cells, 4 + nsp))
+        self._ok(self._L.semo_calc_fluxes(self._h, nsp, _ptr(L), _ptr(R), None if G is None else _ptr(G),
+                                          _ptr(out)))
+        return out
+
+    def calcRHS(self, time_step, g, face_left, face_right, cells, face_grad=None, st_force=None):
+        """calcRHS (RHS.cpp:5-28): calcFluxes, the dual-time source, gravity,
+        the caller's surface-tension force [n, 3] (or None), the scaling by
+        -dtPseudo / V.  cells [n, 8 + (nsp-1) + 2 (4 + nsp)] = (rho, U, V, W,
+        Ht, P, dtPseudo, VF, Y.., Qn.., Qm..)."""
+        L = np.ascontiguousarray(face_left, dtype=np.float64)
+        R = np.ascontiguousarray(face_right, dtype=np.float64)
+        G = None if face_grad is None else np.ascontiguousarray(face_grad, dtype=np.float64)
+        cs = np.ascontiguousarray(cells, dtype=np.float64)
+        S = None if st_force is None else np.ascontiguousarray(st_force, dtype=np.float64)
+        gv = np.ascontiguousarray(g, dtype=np.float64)
+        nsp = L.shape[1] - 8
+        out = np.empty((self.mesh.n_cells, 4 + nsp))
+        self._ok(self._L.semo_calc_rhs(self._h, nsp, float(time_step), _ptr(gv), _ptr(L), _ptr(R),
+                                       None if G is None else _ptr(G), _ptr(cs), None if S is None else _ptr(S),
+                                       _ptr(out)))
+        return out
+
     LINSOL_METHODS = {"point": 0, "single": 1, "lusgs": 2}
 
     def _linsol(self, method, time_step, state, residuals):

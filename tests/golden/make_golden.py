@@ -410,6 +410,67 @@ def _linsol():
                              "linsol_nsp": np.array(nsp)})
 
 
+def rhs_inputs(mesh, nsp=2, seed=101, vol=None):
+    """calcRHS inputs: per face left / right records (P, U, V, W, T, rho, C,
+    Ht, mu, Y_0..Y_{nsp-2}), per face nine velocity gradients, per cell (rho,
+    U, V, W, Ht, P, dtPseudo, VF, Y_0..Y_{nsp-2}, Qn.., Qm..).  Gas- and
+    liquid-like face sides (Mach 0 to 1.5), a smooth two-phase VF, Qn / Qm
+    the conserved variables perturbed by a few per cent."""
+    rng = np.random.default_rng(seed)
+    nf, nc, N = mesh.n_faces, mesh.n_cells, 4 + nsp
+
+    def side(k):
+        liquid = rng.random(k) < 0.3
+        rho = np.where(liquid, 900 + 200 * rng.random(k), 0.8 + 0.8 * rng.random(k))
+        C = np.where(liquid, 1200 + 600 * rng.random(k), 300 + 80 * rng.random(k))
+        d = rng.normal(size=(k, 3)); d /= np.linalg.norm(d, axis=1)[:, None]
+        U = d * (1.5 * rng.random(k) ** 2 * C)[:, None]
+        P = 1e5 * (0.5 + 1.5 * rng.random(k))
+        T = 280 + 40 * rng.random(k)
+        Ht = C * C / 0.4 + 0.5 * (U * U).sum(axis=1)
+        mu = np.where(liquid, 1e-3, 1.8e-5) * (0.5 + rng.random(k))
+        Y = rng.random((k, max(nsp - 1, 0)))
+        return np.column_stack([P, U, T, rho, C, Ht, mu, Y])
+
+    fl, fr = side(nf), side(nf)
+    fg = rng.normal(size=(nf, 9)) * 50.0
+    h = 1.0 if vol is None else np.cbrt(np.abs(vol))
+    x = np.asarray(mesh.points)[np.asarray(mesh.cell_pts)[np.asarray(mesh.cell_ptr)[:-1]]]
+    VF = 0.5 * (1.0 + np.tanh((0.45 - x[:, 2]) / 0.15))
+    cs = side(nc)
+    rho, U, Ht, P = cs[:, 5], cs[:, 1:4], cs[:, 7], cs[:, 0]
+    Y = rng.random((nc, max(nsp - 1, 0)))
+    Q = np.column_stack([rho, rho[:, None] * U, rho * Ht - P] + [rho[:, None] * Y])
+    Qn = Q * (1.0 + 0.03 * rng.normal(size=Q.shape))
+    Qm = Q * (1.0 + 0.03 * rng.normal(size=Q.shape))
+    dtp = (1.0 + 4.0 * rng.random(nc)) * h * 0.1 / (np.linalg.norm(U, axis=1) + cs[:, 6])
+    cells = np.column_stack([rho, U, Ht, P, dtp, VF, Y, Qn, Qm])
+    assert cells.shape[1] == 8 + (nsp - 1) + 2 * N
+    return fl, fr, fg, cells
+
+
+def _rhs():
+    """calcFluxes of SEMO's density-based solvers (solvers/RHS.cpp:98-406:
+    SLAU2 and the viscous flux per face, summed into the cells in face order;
+    calcRHS itself crashes in its sourceTerms) on seeded face and cell
+    states: jittered hex and tet boxes with two species, a tet box with three
+    (with one species calcFluxes reads fVF[0], which does not exist)."""
+    dt = 1e-3
+    for name, m, seed, nsp in (("rhs_box654", hex_mesh(6, 5, 4, jitter=0.1, seed=111), 121, 2),
+                               ("rhs_tet433", tet_mesh(4, 3, 3, hi=(1.0, 0.75, 0.75), jitter=0.1, seed=112), 122, 2),
+                               ("rhs_tet322_nsp3", tet_mesh(3, 2, 2, jitter=0.1, seed=113), 123, 3)):
+        fl, fr, fg, cells = rhs_inputs(m, nsp, seed, vol=geometry_volumes(m))
+        c = cavity_case(); c.g = (0.5, -1.0, -9.81); c.sigma = 0.0
+        with tempfile.TemporaryDirectory() as d:
+            fn = os.path.join(d, "rhs.f64")
+            with open(fn, "wb") as fh:
+                for a in (fl, fr, fg, cells):
+                    np.ascontiguousarray(a).tofile(fh)
+            run_case(name, m, c, 0, 1, keep_calls=False, extra=(f"rhs {fn} {dt!r}", f"nspecies {nsp}"),
+                     arrays={"rhs_face_left": fl, "rhs_face_right": fr, "rhs_face_grad": fg, "rhs_cells": cells,
+                             "rhs_dt": np.array(dt), "rhs_nsp": np.array(nsp)})
+
+
 def _poly20():
     """Three stacked prisms over a jittered 20-gon: internal faces and caps
     of 20 points (above the 16 the device keeps in registers), lid-driven
@@ -541,7 +602,7 @@ CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box,
          "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof,
          "poly20_stack3": _poly20, "vtu_tet221": _vtu, "initial_cube6": _initial, "les": _les,
-         "lesrun": _les_runs, "flux": _flux, "linsol": _linsol}
+         "lesrun": _les_runs, "flux": _flux, "linsol": _linsol, "rhs": _rhs}
 
 
 def main(names):
