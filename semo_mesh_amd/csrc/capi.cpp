@@ -792,11 +792,21 @@ int semo_flux(semo_ctx* c, int scheme, int64_t n, int nsp, const double* left, c
     });
 }
 
-// face arrays: caller numbering -> device face order
-static void face_to_dev(semo_ctx* c, DBuf<double>& d, const double* h, int comps) {
-    std::vector<double> t((size_t)c->nf * comps);
-    for (int k = 0; k < c->nf; ++k)
-        for (int a = 0; a < comps; ++a) t[(size_t)k * comps + a] = h[(size_t)c->fperm[k] * comps + a];
+// [n][comps] records, caller numbering -> device order, stored field-major
+// ([comps][n]) for coalesced loads
+static void face_to_dev_soa(semo_ctx* c, DBuf<double>& d, const double* h, int comps) {
+    const size_t n = (size_t)c->nf;
+    std::vector<double> t(n * comps);
+    for (size_t k = 0; k < n; ++k)
+        for (int a = 0; a < comps; ++a) t[(size_t)a * n + k] = h[(size_t)c->fperm[k] * comps + a];
+    d.upload(t, c->stream);
+    SEMO_HIP(hipStreamSynchronize(c->stream));
+}
+static void cell_to_dev_soa(semo_ctx* c, DBuf<double>& d, const double* h, int comps) {
+    const size_t n = (size_t)c->nc;
+    std::vector<double> t(n * comps);
+    for (size_t i = 0; i < n; ++i)
+        for (int a = 0; a < comps; ++a) t[(size_t)a * n + i] = h[(size_t)c->perm[i] * comps + a];
     d.upload(t, c->stream);
     SEMO_HIP(hipStreamSynchronize(c->stream));
 }
@@ -810,10 +820,10 @@ static int rhs_common(semo_ctx* c, int nsp, bool src, double time_step, const do
                    "calcRHS: time_step must be finite and non-zero");
         const int N = 4 + nsp;
         DBuf<double> dL, dR, dG, dC, dS, dF, dr;
-        face_to_dev(c, dL, L, 8 + nsp);
-        face_to_dev(c, dR, R, 8 + nsp);
-        if (grad) face_to_dev(c, dG, grad, 9);
-        if (src) to_dev(c, dC, cells, 8 + (nsp - 1) + 2 * N);
+        face_to_dev_soa(c, dL, L, 8 + nsp);
+        face_to_dev_soa(c, dR, R, 8 + nsp);
+        if (grad) face_to_dev_soa(c, dG, grad, 9);
+        if (src) cell_to_dev_soa(c, dC, cells, 8 + (nsp - 1) + 2 * N);
         if (src && st) to_dev(c, dS, st, 3);
         dF.alloc((size_t)c->nf * N);
         dr.alloc((size_t)c->nc * N);

@@ -422,8 +422,9 @@ __global__ void __launch_bounds__(TPB) k_rhs_face(int nf, int nsp, const double*
     int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     const int w = 8 + nsp, N = 4 + nsp;
-    const double* l = Ls + (size_t)f * w;
-    const double* r = Rs + (size_t)f * w;
+    // records are stored field-major ([8 + nsp][nf]): coalesced loads
+    double l[16], r[16];
+    for (int k = 0; k < w; ++k) { l[k] = Ls[(size_t)k * nf + f]; r[k] = Rs[(size_t)k * nf + f]; }
     double YL[8], YR[8];
     double sl = 0.0, sr = 0.0;
     for (int k = 0; k < nsp - 1; ++k) { YL[k] = l[9 + k]; sl += YL[k]; YR[k] = r[9 + k]; sr += YR[k]; }
@@ -438,7 +439,7 @@ __global__ void __launch_bounds__(TPB) k_rhs_face(int nf, int nsp, const double*
     flux_slau2(L, R, nv, nsp, cf);
     // calcViscousFlux: face values at gc = 0.5, stresses from the gradients
     double g[9];
-    for (int k = 0; k < 9; ++k) g[k] = grad ? grad[(size_t)f * 9 + k] : 0.0;
+    for (int k = 0; k < 9; ++k) g[k] = grad ? grad[(size_t)k * nf + f] : 0.0;
     const double gc = 0.5;
     const double muF = gc * l[8] + (1.0 - gc) * r[8];
     const double UF = gc * l[1] + (1.0 - gc) * r[1];
@@ -488,22 +489,23 @@ __global__ void __launch_bounds__(TPB) k_rhs_cell(int nc, int nif, int nsp, cons
         else for (int k = 0; k < N; ++k) r[k] -= c[k];
     }
     if (SRC) {
-        const int KC = 8 + (nsp - 1) + 2 * N;
-        const double* cs = cells + (size_t)i * KC;
-        const double Rho = cs[0], V = vol[i];
+        // cell records field-major ([8 + (nsp-1) + 2N][nc]): field k of
+        // cell i at cells[k * nc + i]
+        const size_t n = (size_t)nc;
+        auto cs = [&](int k) { return cells[(size_t)k * n + i]; };
+        const double Rho = cs(0), V = vol[i];
         double Q[12];
-        Q[0] = Rho; Q[1] = Rho * cs[1]; Q[2] = Rho * cs[2]; Q[3] = Rho * cs[3];
-        Q[4] = Rho * cs[4] - cs[5];
-        for (int a = 0; a < nsp - 1; ++a) Q[5 + a] = Rho * cs[8 + a];
-        const double* Qn = cs + 7 + nsp;
-        const double* Qm = Qn + N;
-        for (int j = 0; j < N; ++j) r[j] += (1.5 * Q[j] - 2.0 * Qn[j] + 0.5 * Qm[j]) / time_step * V;
+        Q[0] = Rho; Q[1] = Rho * cs(1); Q[2] = Rho * cs(2); Q[3] = Rho * cs(3);
+        Q[4] = Rho * cs(4) - cs(5);
+        for (int a = 0; a < nsp - 1; ++a) Q[5 + a] = Rho * cs(8 + a);
+        for (int j = 0; j < N; ++j) r[j] += (1.5 * Q[j] - 2.0 * cs(7 + nsp + j) + 0.5 * cs(7 + nsp + N + j)) / time_step * V;
         r[1] -= Rho * g0 * V;
         r[2] -= Rho * g1 * V;
         r[3] -= Rho * g2 * V;
         if (st)
             for (int a = 0; a < 3; ++a) r[1 + a] -= V * st[3 * (size_t)i + a];
-        for (int k = 0; k < N; ++k) r[k] *= -cs[6] / V;
+        const double dtp = cs(6);
+        for (int k = 0; k < N; ++k) r[k] *= -dtp / V;
     }
     for (int k = 0; k < N; ++k) res[(size_t)i * N + k] = r[k];
 }
