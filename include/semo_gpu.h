@@ -400,6 +400,16 @@ int semo_calc_fluxes(semo_ctx* ctx, int nsp, const double* face_left, const doub
 int semo_calc_rhs(semo_ctx* ctx, int nsp, double time_step, const double* g, const double* face_left,
                   const double* face_right, const double* face_grad, const double* cells, const double* st_force,
                   double* residuals);
+/* updateValues (src/solvers/densityBased.cpp:366-512): cells[n][5 + nsp]
+ * in/out = (P, U, V, W, T, MF_0..MF_{nsp-1}) += urf (dualTimeURF_P, _U, _T,
+ * _MF: urf[4]) times the residuals[n][4 + nsp] (the implicit update), each
+ * of P, U, V, W, T then held within limits[10] = (minP, maxP, minU, maxU,
+ * minV, maxV, minW, maxW, minT, maxT), the first nsp - 1 mass fractions
+ * clipped to [0, 1] and the last set to 1 - their sum; face_un[n_faces]
+ * in/out (face.var Un) += dualTimeURF_U * the face-weighted (wC) momentum
+ * increments . n at internal faces (boundary faces unchanged).  1 <= nsp <= 8.*/
+int semo_update_values(semo_ctx* ctx, int nsp, const double* urf, const double* limits, const double* residuals,
+                       double* cells, double* face_un);
 enum { SEMO_LINSOL_POINT = 0, SEMO_LINSOL_SINGLE = 1, SEMO_LINSOL_LUSGS = 2 };
 int semo_linear_solver(semo_ctx* ctx, int method, int nsp, double time_step, const double* state,
                        double* residuals);

@@ -37,8 +37,8 @@ SRCS := mesh/build mesh/geometric math/math math/gradient math/RCM mpi/build \
         solvers/solveAMGCL solvers/solvePETSc solvers/timestep solvers/eos solvers/transport \
         solvers/NVD solvers/cellVarMinMax solvers/surfaceNormalVelocity solvers/pressureBased \
         solvers/source/gravity solvers/source/surfaceTension solvers/source/curvature \
-        turbulenceModels/LES solvers/flux solvers/linearSolver solvers/RHS solvers/source solvers/viscousFlux \
-        save/save save/saveAscii save/saveBinary save/saveCompress save/saveBase64 save/saveMkdirs
+        turbulenceModels/LES solvers/flux solvers/linearSolver solvers/RHS solvers/source solvers/viscousFlux solvers/densityBased solvers/norm solvers/reconComp \
+        save/save save/saveAscii save/saveBinary save/saveCompress save/saveBase64 save/saveMkdirs save/saveGnuplot
 OBJS := $(foreach s,$(SRCS),$(OBJ)/$(subst /,_,$(s)).o)
 
 SOLVE1 := _ZN20SEMO_Solvers_Builder10solveAMGCLENSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEER17SEMO_Mesh_BuilderiRSt6vectorIiSaIiEESB_RS8_IdSaIdEESE_SE_

@@ -268,6 +268,7 @@ int main(int argc, char** argv) {
     double linsol_dt = 0.0;
     int nspecies = 2;     // "nspecies K": K species (the third and later ones take the second's properties)
     string rhsfile;       // "rhs FILE DT": calcRHS of the density-based solvers on FILE's face and cell state
+    string updfile;       // "update FILE": updateValues of the density-based solvers on FILE's state
     double rhs_dt = 0.0;
     vector<PatchBC> bcs;
     string initfile;
@@ -293,6 +294,7 @@ int main(int argc, char** argv) {
             else if (key == "linsol") cf >> linsol >> linsol_dt;
             else if (key == "nspecies") cf >> nspecies;
             else if (key == "rhs") cf >> rhsfile >> rhs_dt;
+            else if (key == "update") cf >> updfile;
             else if (key == "bc") {
                 PatchBC b;
                 cf >> b.name >> b.ptype >> b.pval >> b.utype >> b.u[0] >> b.u[1] >> b.u[2] >> b.vftype >> b.vfval;
@@ -544,6 +546,48 @@ int main(int argc, char** argv) {
         }
         printf("SEMO_REF_DONE linsol point_s=%.6f single_s=%.6f lusgs_s=%.6f cells=%zu\n", secs[0], secs[1],
                secs[2], nc);
+        fflush(stdout);
+        MPI_Finalize();
+        return 0;
+    }
+
+    if (!updfile.empty()) {
+        // updateValues (solvers/densityBased.cpp:366-512) on FILE's state:
+        // 4 under-relaxation factors (P, U, T, MF), 10 limits (minP, maxP,
+        // minU, maxU, minV, maxV, minW, maxW, minT, maxT), then per cell (P,
+        // U, V, W, T, MF_0..MF_{nSp-1}), per cell the nEq residuals, per face Un
+        const int nsp = controls.nSp, N = controls.nEq, KV = 5 + nsp;
+        const size_t nf = mesh.faces.size();
+        vector<double> hdr(14), cv(nc * KV), rs(nc * N), un(nf);
+        FILE* f = fopen(updfile.c_str(), "rb");
+        if (!f) { perror(updfile.c_str()); return 2; }
+        rd(f, hdr.data(), 14); rd(f, cv.data(), cv.size()); rd(f, rs.data(), rs.size()); rd(f, un.data(), nf);
+        fclose(f);
+        controls.dualTimeURF_P = hdr[0]; controls.dualTimeURF_U = hdr[1];
+        controls.dualTimeURF_T = hdr[2]; controls.dualTimeURF_MF = hdr[3];
+        controls.minP = hdr[4]; controls.maxP = hdr[5]; controls.minU = hdr[6]; controls.maxU = hdr[7];
+        controls.minV = hdr[8]; controls.maxV = hdr[9]; controls.minW = hdr[10]; controls.maxW = hdr[11];
+        controls.minT = hdr[12]; controls.maxT = hdr[13];
+        const int vi[5] = {controls.P, controls.U, controls.V, controls.W, controls.T};
+        for (size_t i = 0; i < nc; ++i) {
+            auto& v = mesh.cells[i].var;
+            for (int k = 0; k < 5; ++k) v[vi[k]] = cv[i * KV + k];
+            for (int a = 0; a < nsp; ++a) v[controls.MF[a]] = cv[i * KV + 5 + a];
+        }
+        for (size_t q = 0; q < nf; ++q) mesh.faces[q].var[controls.Un] = un[q];
+        vector<vector<double>> res(nc, vector<double>(N));
+        for (size_t i = 0; i < nc; ++i)
+            for (int k = 0; k < N; ++k) res[i][k] = rs[i * N + k];
+        solvers.updateValues(mesh, controls, res);
+        for (size_t i = 0; i < nc; ++i) {
+            auto& v = mesh.cells[i].var;
+            for (int k = 0; k < 5; ++k) cv[i * KV + k] = v[vi[k]];
+            for (int a = 0; a < nsp; ++a) cv[i * KV + 5 + a] = v[controls.MF[a]];
+        }
+        for (size_t q = 0; q < nf; ++q) un[q] = mesh.faces[q].var[controls.Un];
+        dump("upd_cells", cv);
+        dump("upd_un", un);
+        printf("SEMO_REF_DONE update\n");
         fflush(stdout);
         MPI_Finalize();
         return 0;

@@ -844,6 +844,27 @@ int semo_calc_rhs(semo_ctx* c, int nsp, double time_step, const double* g, const
     return rhs_common(c, nsp, true, time_step, g, L, R, grad, cells, st, res);
 }
 
+int semo_update_values(semo_ctx* c, int nsp, const double* urf, const double* limits, const double* res,
+                       double* cells, double* face_un) {
+    if (!c || !urf || !limits || !res || !cells || !face_un) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        SEMO_CHECK(nsp >= 1 && nsp <= 8, SEMO_EINVAL, "updateValues: 1 <= nsp <= 8");
+        DBuf<double> dr, dc, du, dl;
+        to_dev(c, dr, res, 4 + nsp);
+        to_dev(c, dc, cells, 5 + nsp);
+        std::vector<double> u(c->nf);
+        for (int k = 0; k < c->nf; ++k) u[k] = face_un[c->fperm[k]];
+        du.upload(u, c->stream);
+        dl.upload(limits, 10, c->stream);
+        update_values_device(c, nsp, urf, dl.p, dr.p, dc.p, du.p);
+        from_dev(c, dc.p, cells, 5 + nsp);
+        du.download(u.data(), u.size(), c->stream);
+        SEMO_HIP(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < c->nf; ++k) face_un[c->fperm[k]] = u[k];
+    });
+}
+
 static void linsol_check(semo_ctx* c, int method, int nsp, double time_step) {
     need_mesh(c);
     SEMO_CHECK(method >= SEMO_LINSOL_POINT && method <= SEMO_LINSOL_LUSGS, SEMO_EINVAL,

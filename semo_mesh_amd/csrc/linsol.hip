@@ -548,7 +548,71 @@ void linsol_run(semo_ctx* c, int method, double time_step, const double* st, dou
     }
 }
 
+// updateValues (src/solvers/densityBased.cpp:366-512), per cell: the
+// under-relaxed increments of P, U, V, W, T and the first nsp - 1 mass
+// fractions, the limits (P, U, V, W, T: a value at or beyond a bound takes
+// it), the mass fractions clipped to [0, 1] and the last one closing the sum.
+// cells[n][5 + nsp] = (P, U, V, W, T, MF_0..MF_{nsp-1}); lim = (minP, maxP,
+// minU, maxU, minV, maxV, minW, maxW, minT, maxT).
+__global__ __launch_bounds__(TPB) void k_update_cells(int n, int nsp, double uP, double uU, double uT, double uMF,
+                                                      const double* __restrict__ lim, const double* __restrict__ res,
+                                                      double* __restrict__ cells) {
+    const int i = xcd_block() * TPB + (int)threadIdx.x;
+    if (i >= n) return;
+    const int N = 4 + nsp, K = 5 + nsp;
+    double* c = cells + (size_t)i * K;
+    const double* r = res + (size_t)i * N;
+    double v[5] = {c[0] + uP * r[0], c[1] + uU * r[1], c[2] + uU * r[2], c[3] + uU * r[3], c[4] + uT * r[4]};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (v[k] <= lim[2 * k]) v[k] = lim[2 * k];
+        if (v[k] >= lim[2 * k + 1]) v[k] = lim[2 * k + 1];
+        c[k] = v[k];
+    }
+    double tmp = 0.0;
+    for (int a = 0; a < nsp - 1; ++a) {
+        double y = c[5 + a] + uMF * r[5 + a];
+        y = y < 1.0 ? y : 1.0;      // std::min(1.0, y): (y < 1) ? y : 1
+        y = 0.0 < y ? y : 0.0;      // std::max(0.0, y): (0 < y) ? y : 0
+        c[5 + a] = y;
+        tmp += y;
+    }
+    c[5 + nsp - 1] = 1.0 - tmp;
+}
+
+// per internal face: Un += URF_U * (wC r_own + (1 - wC) r_ngb) . n over the
+// momentum increments, in the caller's orientation (a face the renumbering
+// flipped takes the other side's weight and the negated normal)
+__global__ __launch_bounds__(TPB) void k_update_un(int nif, int nsp, double uU, const int* __restrict__ owner,
+                                                   const int* __restrict__ nbr, const int8_t* __restrict__ flip,
+                                                   const double* __restrict__ wC, const double* __restrict__ nrm,
+                                                   const double* __restrict__ res, double* __restrict__ un) {
+    const int f = xcd_block() * TPB + (int)threadIdx.x;
+    if (f >= nif) return;
+    const int N = 4 + nsp;
+    const bool fl = flip[f] != 0;
+    const int o = fl ? nbr[f] : owner[f], g = fl ? owner[f] : nbr[f];
+    const double wCL = fl ? 1.0 - wC[f] : wC[f], wCR = 1.0 - wCL;
+    double n[3];
+    for (int a = 0; a < 3; ++a) n[a] = fl ? -nrm[3 * (size_t)f + a] : nrm[3 * (size_t)f + a];
+    const double* ro = res + (size_t)o * N;
+    const double* rg = res + (size_t)g * N;
+    un[f] += uU * ((wCL * ro[1] + wCR * rg[1]) * n[0] + (wCL * ro[2] + wCR * rg[2]) * n[1] +
+                   (wCL * ro[3] + wCR * rg[3]) * n[2]);
+}
+
 }  // namespace
+
+void update_values_device(semo_ctx* c, int nsp, const double* urf, const double* d_lim, const double* res,
+                          double* cells, double* un) {
+    const int N = 4 + nsp;
+    SEMO_LAUNCH(c, "update_cells", (double)c->nc * (2.0 * (5 + nsp) + N) * 8.0, k_update_cells,
+                dim3(nblk(c->nc)), dim3(TPB), c->nc, nsp, urf[0], urf[1], urf[2], urf[3], d_lim, res, cells);
+    if (c->nif)
+        SEMO_LAUNCH(c, "update_un", (double)c->nif * (8.0 + 1.0 + 8.0 + 24.0 + 16.0) + 6.0 * 8.0 * c->nif,
+                    k_update_un, dim3(nblk(c->nif)), dim3(TPB), c->nif, nsp, urf[1], c->d_owner.p, c->d_nbr.p,
+                    c->d_fflip.p, c->f_wC.p, c->f_nrm.p, res, un);
+}
 
 void linsol_device(semo_ctx* c, int method, int nsp, double time_step, const double* st, double* res) {
     switch (nsp) {

@@ -369,6 +369,19 @@ class GpuSolver:
                                        _ptr(out)))
         return out
 
+    def updateValues(self, urf, limits, residuals, cells, face_un):
+        """updateValues (solvers/densityBased.cpp:366-512): returns the new
+        cells [n, 5 + nsp] = (P, U, V, W, T, MF..) and face Un [n_faces].
+        urf = (dualTimeURF_P, _U, _T, _MF); limits = (minP, maxP, .., maxT)."""
+        r = np.ascontiguousarray(residuals, dtype=np.float64)
+        cv = np.ascontiguousarray(cells, dtype=np.float64).copy()
+        un = np.ascontiguousarray(face_un, dtype=np.float64).copy()
+        u = np.ascontiguousarray(urf, dtype=np.float64)
+        lim = np.ascontiguousarray(limits, dtype=np.float64)
+        self._ok(self._L.semo_update_values(self._h, r.shape[1] - 4, _ptr(u), _ptr(lim), _ptr(r), _ptr(cv),
+                                            _ptr(un)))
+        return cv, un
+
     LINSOL_METHODS = {"point": 0, "single": 1, "lusgs": 2}
 
     def _linsol(self, method, time_step, state, residuals):

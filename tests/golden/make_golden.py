@@ -471,6 +471,32 @@ def _rhs():
                              "rhs_dt": np.array(dt), "rhs_nsp": np.array(nsp)})
 
 
+def _update():
+    """updateValues of SEMO's density-based solvers (solvers/densityBased.cpp:
+    366-512): under-relaxed increments with the limits, the mass fractions
+    clipped to [0, 1] and closed, the face normal velocity from the weighted
+    momentum increments; hex and tet boxes, two and three species, limits
+    that clip a share of the cells."""
+    for name, m, seed, nsp in (("update_box654", hex_mesh(6, 5, 4, jitter=0.1, seed=131), 141, 2),
+                               ("update_tet322_nsp3", tet_mesh(3, 2, 2, jitter=0.1, seed=132), 142, 3)):
+        rng = np.random.default_rng(seed)
+        nc, nf, N = m.n_cells, m.n_faces, 4 + nsp
+        hdr = np.array([0.7, 0.5, 0.6, 0.8, 9e4, 1.1e5, -30.0, 30.0, -30.0, 30.0, -30.0, 30.0, 250.0, 350.0])
+        Y = rng.random((nc, nsp)); Y /= Y.sum(axis=1)[:, None]
+        cv = np.column_stack([1e5 * (1.0 + 0.05 * rng.normal(size=nc)), rng.normal(size=(nc, 3)) * 20.0,
+                              300.0 + 20.0 * rng.normal(size=nc), Y])
+        rs = rng.normal(size=(nc, N)) * np.array([5e3, 8.0, 8.0, 8.0, 20.0] + [0.3] * (nsp - 1))
+        un = rng.normal(size=nf) * 10.0
+        with tempfile.TemporaryDirectory() as d:
+            fn = os.path.join(d, "upd.f64")
+            with open(fn, "wb") as fh:
+                for a in (hdr, cv, rs, un):
+                    np.ascontiguousarray(a).tofile(fh)
+            run_case(name, m, cavity_case(), 0, 1, keep_calls=False, extra=(f"update {fn}", f"nspecies {nsp}"),
+                     arrays={"upd_hdr": hdr, "upd_cells_in": cv, "upd_res": rs, "upd_un_in": un,
+                             "upd_nsp": np.array(nsp)})
+
+
 def _poly20():
     """Three stacked prisms over a jittered 20-gon: internal faces and caps
     of 20 points (above the 16 the device keeps in registers), lid-driven
@@ -602,7 +628,7 @@ CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box,
          "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof,
          "poly20_stack3": _poly20, "vtu_tet221": _vtu, "initial_cube6": _initial, "les": _les,
-         "lesrun": _les_runs, "flux": _flux, "linsol": _linsol, "rhs": _rhs}
+         "lesrun": _les_runs, "flux": _flux, "linsol": _linsol, "rhs": _rhs, "update": _update}
 
 
 def main(names):
