@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(TPB) k_rhs_face(int nf, int nsp, const double*
 // W, Ht, P, dtPseudo, VF, Y_0..Y_{nsp-2}, Qn.., Qm..).
 template <bool SRC>
 __global__ void __launch_bounds__(TPB) k_rhs_cell(int nc, int nif, int nsp, const int* __restrict__ cf_ptr,
-                                                  const int* __restrict__ cf_idx, const int* __restrict__ owner,
+                                                  const int* __restrict__ cf_idx, const int* __restrict__,
                                                   const int8_t* __restrict__ flip, const double* __restrict__ fterm,
                                                   const double* __restrict__ vol, const double* __restrict__ cells,
                                                   const double* __restrict__ st, double time_step, double g0,
@@ -481,10 +481,11 @@ __global__ void __launch_bounds__(TPB) k_rhs_cell(int nc, int nif, int nsp, cons
     const int N = 4 + nsp;
     double r[12];
     for (int k = 0; k < N; ++k) r[k] = 0.0;
-    for (int q = cf_ptr[i]; q < cf_ptr[i + 1]; ++q) {
-        const int f = cf_idx[q];
+    // the cell's faces in ascending order from the sliced-ELL lists
+    // (coalesced index loads, ownership in the sign of the code)
+    CELL_FACES_BEGIN(i, cf_ptr, cf_idx, f, own)
         const double* c = fterm + (size_t)f * N;
-        const bool plus = f >= nif || ((owner[f] == i) != (flip[f] != 0));
+        const bool plus = f >= nif || (own != (flip[f] != 0));
         if (plus) for (int k = 0; k < N; ++k) r[k] += c[k];
         else for (int k = 0; k < N; ++k) r[k] -= c[k];
     }
@@ -522,11 +523,11 @@ void rhs_device(semo_ctx* c, int nsp, bool src, double time_step, const double* 
     const double cbytes = sides * (4.0 + 4.0 + 1.0 + N * 8.0) + (double)c->nc * (N + 2 + (src ? KC + 1 : 0)) * 8.0;
     if (src)
         SEMO_LAUNCH(c, "rhs_cell", cbytes, k_rhs_cell<true>, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, nsp,
-                    c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_fflip.p, fterm, c->c_vol.p, cells, st,
+                    c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_fflip.p, fterm, c->c_vol.p, cells, st,
                     time_step, g[0], g[1], g[2], res);
     else
         SEMO_LAUNCH(c, "rhs_cell", cbytes, k_rhs_cell<false>, dim3(nblk(c->nc)), dim3(TPB), c->nc, c->nif, nsp,
-                    c->d_cf_ptr.p, c->d_cf_idx.p, c->d_owner.p, c->d_fflip.p, fterm, c->c_vol.p, cells, st,
+                    c->d_cf_sl.p, c->d_cf_ell.p, c->d_owner.p, c->d_fflip.p, fterm, c->c_vol.p, cells, st,
                     time_step, 0.0, 0.0, 0.0, res);
 }
 
