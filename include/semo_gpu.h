@@ -410,6 +410,15 @@ int semo_calc_rhs(semo_ctx* ctx, int nsp, double time_step, const double* g, con
  * increments . n at internal faces (boundary faces unchanged).  1 <= nsp <= 8.*/
 int semo_update_values(semo_ctx* ctx, int nsp, const double* urf, const double* limits, const double* residuals,
                        double* cells, double* face_un);
+/* SEMO's interface-compression utility (src/utility/compressVolumeFraction.
+ * cpp:121-228) on the uploaded mesh: iters pseudo steps of CFL cfl that
+ * move the volume fraction along its own unit normal, flux alpha(1 - alpha)
+ * upwinded, alpha clipped to [0, 1].  vf[n] in/out; face_r[n_faces] the
+ * boundary faces' values the LSQ gradient reads (NULL: zero, which is what
+ * the utility holds after loading a case); residuals[iters] out (may be
+ * NULL): each step's sqrt(sum of squared updates).                          */
+int semo_compress_volume_fraction(semo_ctx* ctx, double cfl, int iters, double* vf, const double* face_r,
+                                  double* residuals);
 enum { SEMO_LINSOL_POINT = 0, SEMO_LINSOL_SINGLE = 1, SEMO_LINSOL_LUSGS = 2 };
 int semo_linear_solver(semo_ctx* ctx, int method, int nsp, double time_step, const double* state,
                        double* residuals);

@@ -310,3 +310,73 @@ class AdjointOracle:
         g = np.empty(self.mesh.n_cells)
         J = lib_ad().orc_ad_adjoint(self._h, n_iter, self._w(weights), g)
         return J, g
+
+
+def compress_volume_fraction(o, vf, cfl, iters, face_r=None):
+    """SEMO's interface-compression utility (src/utility/compressVolumeFraction
+    .cpp:121-228, the flux's zero tests at :284-285) restated over the oracle's mesh and LSQ: tau = min over the
+    cells of min(V^0.3, V / max face area); per iteration the LSQ gradient of
+    alpha (the cell-vertex stencil, boundary faces at face_r -- zero, as the
+    utility holds them after loading), the unit normal n = grad / |grad|
+    (zero where |grad| = 0), at each face UnF = wC n_o . n_f + (1 - wC) n_n .
+    n_f (0 at boundary faces), the upwind alpha(1 - alpha) of the side UnF
+    leaves (0 when that quantity q of either side has q (1 - q) < 1e-15),
+    flux = alphaF UnF area taken from the owner and given to the neighbour
+    in face order; alpha += cfl tau resi / V, clipped to [0, 1].  Returns
+    the fractions and each iteration's sqrt(sum update^2)."""
+    vf = np.array(vf, dtype=np.float64)
+    m = o.mesh
+    nc, nif, nf = m.n_cells, m.n_internal_faces, m.n_faces
+    V = o.get("cell_volume")
+    area = o.get("face_area")
+    nrm = o.get("face_normal").reshape(-1, 3)
+    wC = o.get("face_wC")
+    owner = np.asarray(m.owner)
+    nbr = np.asarray(m.neighbour)
+    fr = np.zeros(nf) if face_r is None else np.asarray(face_r, dtype=np.float64)
+    maxA = np.zeros(nc)
+    for f in range(nf):
+        maxA[owner[f]] = max(maxA[owner[f]], area[f])
+        if f < nif:
+            maxA[nbr[f]] = max(maxA[nbr[f]], area[f])
+    tau = 1e15
+    for i in range(nc):
+        tau = min(tau, min(V[i] ** 0.3, V[i] / maxA[i]))
+    res_hist = []
+    for _ in range(iters):
+        g = o.lsq(vf, fr)
+        mag = np.sqrt(g[:, 0] ** 2 + g[:, 1] ** 2 + g[:, 2] ** 2)
+        nv = np.zeros_like(g)
+        nz = mag != 0.0
+        nv[nz] = g[nz] / mag[nz, None]
+        aN = vf * (1.0 - vf)
+        resi = np.zeros(nc)
+        for f in range(nf):
+            o_ = owner[f]
+            n = nrm[f]
+            unf = wC[f] * (nv[o_, 0] * n[0] + nv[o_, 1] * n[1] + nv[o_, 2] * n[2])
+            aL, aR = aN[o_], 0.0
+            if f < nif:
+                q = nbr[f]
+                unf += (1.0 - wC[f]) * (nv[q, 0] * n[0] + nv[q, 1] * n[1] + nv[q, 2] * n[2])
+                aR = aN[q]
+            else:
+                unf = 0.0
+            wl = 1.0 if unf > 0.0 else 0.0
+            af = wl * aL + (1.0 - wl) * aR
+            if aL * (1.0 - aL) < 1e-15:
+                af = 0.0
+            if aR * (1.0 - aR) < 1e-15:
+                af = 0.0
+            flux = af * unf * area[f]
+            resi[o_] -= flux
+            if f < nif:
+                resi[nbr[f]] += flux
+        r2 = 0.0
+        for i in range(nc):
+            up = cfl * tau * resi[i] / V[i]
+            vf[i] += up
+            r2 += up * up
+            vf[i] = max(0.0, min(1.0, vf[i]))
+        res_hist.append(np.sqrt(r2))
+    return vf, res_hist

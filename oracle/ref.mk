@@ -15,7 +15,8 @@
 #    default argument; g++ accepts both with -D__int64="long long" -fpermissive.
 #  * Every solveAMGCL call is intercepted with -Wl,--wrap (see ref_driver.cpp).
 #  * semo_initial: SEMO's initial-field utility (src/utility/initial.cpp) built
-#    as it is, for the region-initialiser fixtures of tests/golden.
+#    as it is, for the region-initialiser fixtures of tests/golden;
+#    semo_compressvf likewise from src/utility/compressVolumeFraction.cpp.
 #  * semo_gpu_eq / semo_gpu_step: the same driver and reference objects with
 #    the C++ bridge of semo_mesh_amd/bridge linked in place of the equation
 #    files (eq: SEMO's own incompressiblePressureBased, its equation calls on
@@ -38,7 +39,7 @@ SRCS := mesh/build mesh/geometric math/math math/gradient math/RCM mpi/build \
         solvers/NVD solvers/cellVarMinMax solvers/surfaceNormalVelocity solvers/pressureBased \
         solvers/source/gravity solvers/source/surfaceTension solvers/source/curvature \
         turbulenceModels/LES solvers/flux solvers/linearSolver solvers/RHS solvers/source solvers/viscousFlux solvers/densityBased solvers/norm solvers/reconComp \
-        save/save save/saveAscii save/saveBinary save/saveCompress save/saveBase64 save/saveMkdirs save/saveGnuplot
+        save/save save/saveAscii save/saveBinary save/saveCompress save/saveBase64 save/saveMkdirs save/saveGnuplot save/saveCellData
 OBJS := $(foreach s,$(SRCS),$(OBJ)/$(subst /,_,$(s)).o)
 
 SOLVE1 := _ZN20SEMO_Solvers_Builder10solveAMGCLENSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEER17SEMO_Mesh_BuilderiRSt6vectorIiSaIiEESB_RS8_IdSaIdEESE_SE_
@@ -51,7 +52,8 @@ BOBJS_EQ := $(filter-out $(EQ_OBJS),$(OBJS))
 BOBJS_STEP := $(filter-out $(OBJ)/solvers_pressureBased.o,$(BOBJS_EQ))
 MPILIBS := $(MPIDIR)/lib/libmpicxx.so.12 $(MPIDIR)/lib/libmpi.so.12
 
-all: $(OUT)/semo_ref $(OUT)/rcm_ref $(OUT)/lib/.stamp $(OUT)/semo_gpu_eq $(OUT)/semo_gpu_step $(OUT)/semo_initial
+all: $(OUT)/semo_ref $(OUT)/rcm_ref $(OUT)/lib/.stamp $(OUT)/semo_gpu_eq $(OUT)/semo_gpu_step $(OUT)/semo_initial \
+     $(OUT)/semo_compressvf
 
 $(OBJ)/.dir:
 	mkdir -p $(OBJ) && touch $@
@@ -91,6 +93,14 @@ $(OBJ)/utility_initial.o: $(REF)/src/utility/initial.cpp $(OBJ)/.dir
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
 $(OUT)/semo_initial: $(OBJS) $(OBJ)/utility_initial.o
+	$(CXX) -o $@ $^ $(MPILIBS) -lz -ldl -Wl,-rpath,'$$ORIGIN/lib' -Wl,--allow-shlib-undefined
+
+# SEMO's interface-compression utility (src/utility/compressVolumeFraction.cpp,
+# its own main) as it is, for the compression fixtures
+$(OBJ)/utility_compressVolumeFraction.o: $(REF)/src/utility/compressVolumeFraction.cpp $(OBJ)/.dir
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/semo_compressvf: $(OBJS) $(OBJ)/utility_compressVolumeFraction.o
 	$(CXX) -o $@ $^ $(MPILIBS) -lz -ldl -Wl,-rpath,'$$ORIGIN/lib' -Wl,--allow-shlib-undefined
 
 $(OBJ)/rcm_ref_main.o: $(HERE)rcm_ref_main.cpp $(OBJ)/.dir

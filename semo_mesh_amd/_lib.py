@@ -74,7 +74,7 @@ EXPORTED = [
     "semo_incompressible_pressure_based", "semo_adjoint_alpha_p", "semo_objective_pressure",
     "semo_volume_integral",
     "semo_calc_coupled_eq", "semo_coupled_pressure_based", "semo_coupled_matrix_get", "semo_coupled_spmv", "semo_coupled_solve",
-    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_least_square_noalloc", "semo_cell_min_max", "semo_gauss_green", "semo_les", "semo_set_les", "semo_set_momentum_solver", "semo_flux", "semo_linear_solver", "semo_linear_solver_device", "semo_calc_fluxes", "semo_calc_rhs", "semo_update_values", "semo_solve_coo",
+    "semo_get_solve_stats", "semo_assemble", "semo_matrix_get", "semo_least_square", "semo_least_square_noalloc", "semo_cell_min_max", "semo_gauss_green", "semo_les", "semo_set_les", "semo_set_momentum_solver", "semo_flux", "semo_linear_solver", "semo_linear_solver_device", "semo_calc_fluxes", "semo_calc_rhs", "semo_update_values", "semo_compress_volume_fraction", "semo_solve_coo",
     "semo_profile_enable", "semo_profile_reset", "semo_profile_entry", "semo_stream_copy",
 ]
 
@@ -133,6 +133,7 @@ def lib():
         "semo_linear_solver": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, dp, dp]),
         "semo_calc_fluxes": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
         "semo_update_values": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp]),
+        "semo_compress_volume_fraction": (C.c_int, [vp, C.c_double, C.c_int, dp, dp, dp]),
         "semo_calc_rhs": (C.c_int, [vp, C.c_int, C.c_double, dp, dp, dp, dp, dp, dp, dp]),
         "semo_linear_solver_device": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, vp, vp]),
         "semo_solve_coo": (C.c_int, [vp, C.c_char_p, i32, i64, vp, vp, vp, i32, dp, dp,

@@ -865,6 +865,23 @@ int semo_update_values(semo_ctx* c, int nsp, const double* urf, const double* li
     });
 }
 
+int semo_compress_volume_fraction(semo_ctx* c, double cfl, int iters, double* vf, const double* face_r,
+                                  double* residuals) {
+    if (!c || !vf) return SEMO_EINVAL;
+    return guard(c, [&] {
+        need_mesh(c);
+        SEMO_CHECK(iters >= 0 && std::isfinite(cfl), SEMO_EINVAL, "compressVolumeFraction: iters >= 0, finite cfl");
+        DBuf<double> dv, db;
+        to_dev(c, dv, vf, 1);
+        std::vector<double> br(std::max(c->nbf, 1), 0.0);
+        if (face_r)
+            for (int j = 0; j < c->nbf; ++j) br[j] = face_r[c->fperm[c->nif + j]];
+        db.upload(br, c->stream);
+        compress_vf_device(c, cfl, iters, dv.p, db.p, residuals);
+        from_dev(c, dv.p, vf, 1);
+    });
+}
+
 static void linsol_check(semo_ctx* c, int method, int nsp, double time_step) {
     need_mesh(c);
     SEMO_CHECK(method >= SEMO_LINSOL_POINT && method <= SEMO_LINSOL_LUSGS, SEMO_EINVAL,

@@ -467,6 +467,7 @@ void rhs_device(semo_ctx* c, int nsp, bool src, double time_step, const double* 
                 const double* grad, const double* cells, const double* st, double* fterm, double* res);
 void update_values_device(semo_ctx* c, int nsp, const double* urf, const double* d_lim, const double* res,
                           double* cells, double* un);
+void compress_vf_device(semo_ctx* c, double cfl, int iters, double* vf, const double* faceR_b, double* residuals);
 void linsol_device(semo_ctx* c, int method, int nsp, double time_step, const double* st, double* res);
 int lusgs_levels(semo_ctx* c, int dir);
 void flux_device(semo_ctx* c, int scheme, int n, int nsp, const double* L, const double* R, const double* nrm,

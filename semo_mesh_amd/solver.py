@@ -382,6 +382,17 @@ class GpuSolver:
                                             _ptr(un)))
         return cv, un
 
+    def compressVolumeFraction(self, vf, cfl, iters, face_r=None):
+        """SEMO's interface-compression utility (src/utility/
+        compressVolumeFraction.cpp:121-228): returns the compressed fractions
+        and each iteration's residual."""
+        v = np.ascontiguousarray(vf, dtype=np.float64).copy()
+        fr = None if face_r is None else np.ascontiguousarray(face_r, dtype=np.float64)
+        res = np.zeros(max(int(iters), 1))
+        self._ok(self._L.semo_compress_volume_fraction(self._h, float(cfl), int(iters), _ptr(v),
+                                                       None if fr is None else _ptr(fr), _ptr(res)))
+        return v, res[:int(iters)]
+
     LINSOL_METHODS = {"point": 0, "single": 1, "lusgs": 2}
 
     def _linsol(self, method, time_step, state, residuals):

@@ -623,12 +623,61 @@ def _initial():
     print("initial_cube6:", len(INITIAL_SPECS), "specs")
 
 
+SEMO_COMPRESSVF = os.path.join(ROOT, "oracle", "_ref", "semo_compressvf")
+
+
+def _compressvf():
+    """SEMO's interface-compression utility (src/utility/compressVolumeFraction
+    .cpp, built as it is into oracle/_ref/semo_compressvf) on a jittered
+    8x6x5 box whose water fraction is a smeared tanh layer: CFL 0.4, 6
+    iterations, in a case holding the reference's sample system/ and
+    constant/ files (binary saves, start from 0), the state as ./save/0/
+    plot.0.vtu.  Stores the mesh, the input and the compressed fractions."""
+    import shutil
+    from semo_mesh_amd.vtu import read_vtu, semo_fields, write_vtu
+    m = hex_mesh(8, 6, 5, jitter=0.1, seed=151)
+    x = np.asarray(m.points)[np.asarray(m.cell_pts)[np.asarray(m.cell_ptr)[:-1]]]
+    vf = 0.5 * (1.0 + np.tanh((0.5 - x[:, 0] - 0.3 * x[:, 2]) / 0.2))
+    nc = m.n_cells
+    with tempfile.TemporaryDirectory() as d:
+        shutil.copytree(os.path.join(REF_TREE, "system"), os.path.join(d, "system"))
+        shutil.copytree(os.path.join(REF_TREE, "constant"), os.path.join(d, "constant"))
+        cd = os.path.join(d, "system", "controlDict")
+        with open(cd) as fh:
+            lines = fh.read().splitlines()
+        lines = ["saveFormat = binary" if ln.startswith("saveFormat") else
+                 "saveCompression = 0" if ln.startswith("saveCompression") else
+                 "startFrom = 0" if ln.startswith("startFrom") else ln for ln in lines]
+        with open(cd, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        for fname, body in (("pressure", ["type fixedValue;", "value 101325;"]),
+                            ("velocities", ["type fixedValue;", "value (0 0 0);"]),
+                            ("temperature", ["type fixedValue;", "value 300;"]),
+                            ("water", ["type fixedValue;", "value 0;"]), ("air", ["type fixedValue;", "value 0;"])):
+            write_initial_file(os.path.join(d, "constant", fname), body, m.patch_names)
+        os.makedirs(os.path.join(d, "save", "0"))
+        write_vtu(os.path.join(d, "save", "0", "plot.0.vtu"), m,
+                  semo_fields(np.full(nc, 101325.0), np.zeros(nc), np.zeros(nc), np.zeros(nc), VF=vf,
+                              T=np.full(nc, 300.0)), fmt="binary")
+        r = subprocess.run([SEMO_COMPRESSVF, "-c", "0.4", "-i", "6"], capture_output=True, text=True, timeout=120,
+                           cwd=d)
+        outs = [os.path.join(d, "save", k, "plot.0.vtu") for k in os.listdir(os.path.join(d, "save"))
+                if k.endswith("_CompressVF")]
+        if not outs:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        _, f, _ = read_vtu(outs[0])
+        res = [float(ln.split("residual = ")[1]) for ln in r.stdout.splitlines() if "residual = " in ln]
+    np.savez_compressed(os.path.join(HERE, "compressvf_box865.npz"), cvf_in=vf, cvf_out=f["volume-fraction-water"],
+                        cvf_cfl=np.array(0.4), cvf_iters=np.array(6), cvf_residuals=np.array(res), **mesh_arrays(m))
+    print("compressvf_box865: residuals", res)
+
+
 CASES = {"cavity4_jitter_random": _cavity4, "box654_jitter_2x2": _box654,
          "cavity8_plain": _cavity8, "orifice_lid": _orifice, "vof_box654_2x2": _vof_box654,
          "vof_tet433_2x1": _vof_tet, "sigma_box766_2x1": _sigma_box,
          "bc_open_box654_2x2": _bc_open, "bc_vof_box654_2x1": _bc_vof,
          "poly20_stack3": _poly20, "vtu_tet221": _vtu, "initial_cube6": _initial, "les": _les,
-         "lesrun": _les_runs, "flux": _flux, "linsol": _linsol, "rhs": _rhs, "update": _update}
+         "lesrun": _les_runs, "flux": _flux, "linsol": _linsol, "rhs": _rhs, "update": _update, "compressvf": _compressvf}
 
 
 def main(names):
